@@ -1,0 +1,851 @@
+"""goosefem_amd: MI355X-native element-quadrature and assembly path of GooseFEM.
+
+Python mirror of the reference's Python API (python/*.hpp bindings of include/GooseFEM) over the
+C ABI declared in include/goosefem_amd.h.  Device arrays are torch tensors on the GPU (torch is
+used for device memory and streams only); every computation runs in libgoosefem_amd.so.  There is
+no CPU fallback: if the library is missing, importing the compute classes raises.
+
+    import goosefem_amd as gf
+    mesh = gf.Mesh.Quad4.Regular(64, 64)
+    vec = gf.Vector(mesh.conn, mesh.dofs)
+    quad = gf.Element.Quad4.QuadraturePlanar(vec.AsElement(gf.to_device(mesh.coor)))
+    eps = quad.SymGradN_vector(vec.AsElement(u))
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from types import SimpleNamespace
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgoosefem_amd.so")
+
+GF_OK, GF_EINVAL, GF_EHIP, GF_ENOMEM, GF_ESTATE, GF_ENOTCONV = 0, -1, -2, -3, -4, -5
+QUAD4, QUAD4_PLANAR, HEX8 = 0, 1, 2
+GAUSS, NODAL = 0, 1
+F64, F32, BF16X3 = 0, 1, 2
+
+_lib = None
+
+
+class GooseFEMError(RuntimeError):
+    """Raised for a nonzero status of the C ABI (the reference raises std::runtime_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _hip_runtimes_mapped():
+    try:
+        maps = open("/proc/self/maps").read().splitlines()
+    except OSError:
+        return set()
+    return {ln.split()[-1] for ln in maps if "libamdhip64" in ln}
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded C-ABI library.  Raises if it has not been built.
+
+    torch is imported first so that the library's libamdhip64.so.7 dependency binds (by SONAME)
+    to the HIP runtime torch already loaded: one runtime per process, so device pointers and
+    streams are shared.  Two mapped runtimes raise instead of failing later.
+    """
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} not found: build it with `python -m goosefem_amd.build` "
+                "(there is no CPU fallback)"
+            )
+        import torch  # noqa: F401  (binds the HIP runtime before the library loads)
+
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.gf_last_error.restype = ctypes.c_char_p
+        rts = _hip_runtimes_mapped()
+        if len(rts) > 1:
+            raise ImportError(f"two HIP runtimes mapped in one process: {sorted(rts)}")
+    return _lib
+
+
+def _check(rc: int):
+    if rc != GF_OK:
+        raise GooseFEMError(rc, lib().gf_last_error().decode())
+
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_f64 = ctypes.c_double
+
+
+def _call(name, *args):
+    f = getattr(lib(), name)
+    f.restype = ctypes.c_int
+    _check(f(*args))
+
+
+def _hp(a: np.ndarray):
+    return a.ctypes.data_as(_vp)
+
+
+# -------------------------------------------------------------------------------------------------
+# device memory helpers (torch)
+# -------------------------------------------------------------------------------------------------
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def device():
+    return _torch().device("cuda", _torch().cuda.current_device())
+
+
+def to_device(a, dtype=None):
+    """numpy / torch array -> contiguous device tensor (float64 unless dtype given)."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        t = a
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is None:
+        dtype = torch.float64 if t.is_floating_point() else t.dtype
+    return t.to(device=device(), dtype=dtype).contiguous()
+
+
+def _dp(t):
+    """Device pointer of a contiguous torch tensor on the GPU."""
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous(), "expected a contiguous GPU tensor"
+    return _vp(t.data_ptr())
+
+
+def _stream():
+    return _vp(_torch().cuda.current_stream().cuda_stream)
+
+
+def _empty(shape, dtype=None):
+    torch = _torch()
+    return torch.empty(shape, dtype=dtype or torch.float64, device=device())
+
+
+def _dtype_code(t):
+    torch = _torch()
+    if t.dtype == torch.float64:
+        return F64
+    if t.dtype == torch.float32:
+        return F32
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+# -------------------------------------------------------------------------------------------------
+# Mesh (host, set-up): include/GooseFEM/Mesh*.h
+# -------------------------------------------------------------------------------------------------
+
+
+def _i64a(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _nodeset(nx, ny, nz, name):
+    n = _i64(0)
+    _call("gf_mesh_nodeset", _i64(nx), _i64(ny), _i64(nz), name.encode(), None, ctypes.byref(n))
+    out = np.empty(n.value, dtype=np.int64)
+    _call("gf_mesh_nodeset", _i64(nx), _i64(ny), _i64(nz), name.encode(), _hp(out), ctypes.byref(n))
+    return out
+
+
+class _RegularBase:
+    _nz = 0
+
+    def __getattr__(self, name):
+        if name.startswith("nodes"):
+            r = _nodeset(self.nelx, self.nely, self._nz, name)
+            return int(r[0]) if name.endswith("Corner") or name == "nodesOrigin" else r
+        raise AttributeError(name)
+
+    @property
+    def dofs(self):
+        """Mesh::dofs (Mesh.h:93-96)."""
+        out = np.empty((self.nnode, self.ndim), dtype=np.int64)
+        _call("gf_mesh_dofs", _i64(self.nnode), _i32(self.ndim), _hp(out))
+        return out
+
+    @property
+    def nodesPeriodic(self):
+        """RegularBase::nodesPeriodic (Mesh.h:511-540, 1577-1692)."""
+        n = _i64(0)
+        _call("gf_mesh_nodes_periodic", _i64(self.nelx), _i64(self.nely), _i64(self._nz), None, ctypes.byref(n))
+        out = np.empty((n.value, 2), dtype=np.int64)
+        _call("gf_mesh_nodes_periodic", _i64(self.nelx), _i64(self.nely), _i64(self._nz), _hp(out), ctypes.byref(n))
+        return out
+
+    @property
+    def dofsPeriodic(self):
+        """RegularBase::dofsPeriodic (Mesh.h:291-304)."""
+        ties = self.nodesPeriodic
+        out = np.empty((self.nnode, self.ndim), dtype=np.int64)
+        _call("gf_mesh_dofs_periodic", _i64(self.nnode), _i32(self.ndim), _hp(ties), _i64(ties.shape[0]), _hp(out))
+        return out
+
+
+class _Quad4Regular(_RegularBase):
+    """Mesh::Quad4::Regular (include/GooseFEM/MeshQuad4.h:32-195)."""
+
+    def __init__(self, nelx, nely, h=1.0):
+        self.nelx, self.nely, self.h = int(nelx), int(nely), float(h)
+        self.nne, self.ndim = 4, 2
+        self.nelem = self.nelx * self.nely
+        self.nnode = (self.nelx + 1) * (self.nely + 1)
+
+    @property
+    def coor(self):
+        out = np.empty((self.nnode, 2))
+        _call("gf_mesh_quad4_regular", _i64(self.nelx), _i64(self.nely), _f64(self.h), _hp(out), None)
+        return out
+
+    @property
+    def conn(self):
+        out = np.empty((self.nelem, 4), dtype=np.int64)
+        _call("gf_mesh_quad4_regular", _i64(self.nelx), _i64(self.nely), _f64(self.h), None, _hp(out))
+        return out
+
+    @property
+    def elementgrid(self):
+        return np.arange(self.nelem).reshape(self.nely, self.nelx)
+
+
+class _Hex8Regular(_RegularBase):
+    """Mesh::Hex8::Regular (include/GooseFEM/MeshHex8.h:26-609)."""
+
+    def __init__(self, nelx, nely, nelz, h=1.0):
+        self.nelx, self.nely, self._nz, self.h = int(nelx), int(nely), int(nelz), float(h)
+        self.nelz = self._nz
+        self.nne, self.ndim = 8, 3
+        self.nelem = self.nelx * self.nely * self.nelz
+        self.nnode = (self.nelx + 1) * (self.nely + 1) * (self.nelz + 1)
+
+    @property
+    def coor(self):
+        out = np.empty((self.nnode, 3))
+        _call("gf_mesh_hex8_regular", _i64(self.nelx), _i64(self.nely), _i64(self.nelz), _f64(self.h), _hp(out), None)
+        return out
+
+    @property
+    def conn(self):
+        out = np.empty((self.nelem, 8), dtype=np.int64)
+        _call("gf_mesh_hex8_regular", _i64(self.nelx), _i64(self.nely), _i64(self.nelz), _f64(self.h), None, _hp(out))
+        return out
+
+
+def _csr_lists(ptr, idx):
+    return [list(map(int, idx[ptr[i] : ptr[i + 1]])) for i in range(len(ptr) - 1)]
+
+
+def _renumber(a):
+    a = _i64a(a)
+    out = np.empty_like(a)
+    _call("gf_mesh_renumber", _hp(a), _i64(a.size), _hp(out))
+    return out
+
+
+def _coordination(conn):
+    conn = _i64a(conn)
+    nv = int(conn.max()) + 1
+    out = np.empty(nv, dtype=np.int64)
+    _call("gf_mesh_coordination", _hp(conn), _i64(conn.size), _i64(nv), _hp(out))
+    return out
+
+
+def _elem2node(conn, dofs=None):
+    conn = _i64a(conn)
+    nelem, nne = conn.shape
+    if dofs is None:
+        nnode, ndim, dp = int(conn.max()) + 1, 1, None
+    else:
+        dofs = _i64a(dofs)
+        (nnode, ndim), dp = dofs.shape, _hp(dofs)
+    nnz = _i64(0)
+    ptr = np.empty(nnode + 1, dtype=np.int64)
+    _call("gf_mesh_elem2node", _hp(conn), _i64(nelem), _i32(nne), dp, _i64(nnode), _i32(ndim), _hp(ptr), None,
+          ctypes.byref(nnz))
+    idx = np.empty(nnz.value, dtype=np.int64)
+    _call("gf_mesh_elem2node", _hp(conn), _i64(nelem), _i32(nne), dp, _i64(nnode), _i32(ndim), _hp(ptr), _hp(idx),
+          ctypes.byref(nnz))
+    return _csr_lists(ptr, idx)
+
+
+def _nodaltyings(dofs):
+    dofs = _i64a(dofs)
+    nnode, ndim = dofs.shape
+    nnz = _i64(0)
+    ptr = np.empty(nnode + 1, dtype=np.int64)
+    _call("gf_mesh_nodaltyings", _hp(dofs), _i64(nnode), _i32(ndim), _hp(ptr), None, ctypes.byref(nnz))
+    idx = np.empty(nnz.value, dtype=np.int64)
+    _call("gf_mesh_nodaltyings", _hp(dofs), _i64(nnode), _i32(ndim), _hp(ptr), _hp(idx), ctypes.byref(nnz))
+    return _csr_lists(ptr, idx)
+
+
+Mesh = SimpleNamespace(
+    Quad4=SimpleNamespace(Regular=_Quad4Regular),
+    Hex8=SimpleNamespace(Regular=_Hex8Regular),
+    dofs=lambda nnode, ndim: np.arange(nnode * ndim, dtype=np.int64).reshape(nnode, ndim),
+    renumber=_renumber,
+    coordination=_coordination,
+    elem2node=_elem2node,
+    nodaltyings=_nodaltyings,
+    node2dof=lambda dofs: _elem2node(dofs),
+)
+
+
+# -------------------------------------------------------------------------------------------------
+# Tyings (host, set-up): include/GooseFEM/TyingsPeriodic.h
+# -------------------------------------------------------------------------------------------------
+
+
+class _Control:
+    """Tyings::Control (TyingsPeriodic.h:361-439)."""
+
+    def __init__(self, coor, dofs):
+        coor = np.ascontiguousarray(coor, dtype=np.float64)
+        dofs = _i64a(dofs)
+        nnode, ndim = coor.shape
+        self.coor = np.empty((nnode + ndim, ndim))
+        self.dofs = np.empty((nnode + ndim, ndim), dtype=np.int64)
+        self.controlDofs = np.empty((ndim, ndim), dtype=np.int64)
+        self.controlNodes = np.empty(ndim, dtype=np.int64)
+        _call("gf_tyings_control", _hp(coor), _hp(dofs), _i64(nnode), _i32(ndim), _hp(self.coor), _hp(self.dofs),
+              _hp(self.controlDofs), _hp(self.controlNodes))
+
+
+class _Periodic:
+    """Tyings::Periodic (TyingsPeriodic.h:82-343).  Tying matrices are returned as scipy-free
+    CSR triples (rowptr, col, val) via Cdi_csr / Cdu_csr / Cdp_csr, and dense via Cdi / Cdu / Cdp."""
+
+    def __init__(self, coor, dofs, control_dofs, nodal_tyings, iip=()):
+        self._coor = np.ascontiguousarray(coor, dtype=np.float64)
+        dofs = _i64a(dofs)
+        control_dofs = _i64a(control_dofs)
+        self._ties = _i64a(nodal_tyings)
+        iip = _i64a(iip)
+        nnode, ndim = dofs.shape
+        self.ndim = ndim
+        self.dofs = np.empty_like(dofs)
+        self.control = np.empty_like(control_dofs)
+        sizes = np.empty(4, dtype=np.int64)
+        _call("gf_tyings_periodic", _hp(dofs), _i64(nnode), _i32(ndim), _hp(control_dofs), _hp(self._ties),
+              _i64(self._ties.shape[0]), _hp(iip), _i64(iip.size), _hp(self.dofs), _hp(self.control), _hp(sizes))
+        self.nnu, self.nnp, self.nni, self.nnd = (int(s) for s in sizes)
+
+    def _csr(self, which):
+        nrows = self._ties.shape[0] * self.ndim
+        rowptr = np.empty(nrows + 1, dtype=np.int64)
+        nnz = _i64(0)
+        args = (_i32(which), _hp(self._coor), _hp(self.dofs), _i32(self.ndim), _hp(self.control), _hp(self._ties),
+                _i64(self._ties.shape[0]), _i64(self.nnu))
+        _call("gf_tyings_matrix", *args, _hp(rowptr), None, None, ctypes.byref(nnz))
+        col = np.empty(nnz.value, dtype=np.int64)
+        val = np.empty(nnz.value)
+        _call("gf_tyings_matrix", *args, _hp(rowptr), _hp(col), _hp(val), ctypes.byref(nnz))
+        return rowptr, col, val
+
+    def _dense(self, which, ncol):
+        rowptr, col, val = self._csr(which)
+        out = np.zeros((rowptr.size - 1, ncol))
+        for r in range(rowptr.size - 1):
+            out[r, col[rowptr[r]:rowptr[r + 1]]] += val[rowptr[r]:rowptr[r + 1]]
+        return out
+
+    Cdi_csr = property(lambda self: self._csr(0))
+    Cdu_csr = property(lambda self: self._csr(1))
+    Cdp_csr = property(lambda self: self._csr(2))
+    Cdi = property(lambda self: self._dense(0, self.nni))
+    Cdu = property(lambda self: self._dense(1, self.nnu))
+    Cdp = property(lambda self: self._dense(2, self.nnp))
+
+
+Tyings = SimpleNamespace(Control=_Control, Periodic=_Periodic)
+
+
+def randn(seed: int, n: int, mean: float = 0.0, std: float = 1.0) -> np.ndarray:
+    """Portable SplitMix64 + Box-Muller normal samples (host)."""
+    out = np.empty(n)
+    f = lib().gf_randn
+    f.argtypes = [ctypes.c_uint64, _i64, _f64, _f64, _vp]
+    f.restype = ctypes.c_int
+    _check(f(seed, n, mean, std, _hp(out)))
+    return out
+
+
+# -------------------------------------------------------------------------------------------------
+# Vector (include/GooseFEM/Vector.h)
+# -------------------------------------------------------------------------------------------------
+
+
+class _Handle:
+    _destroy = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and self._destroy and _lib is not None:
+            try:
+                f = getattr(_lib, self._destroy)
+                f(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class Vector(_Handle):
+    """GooseFEM::Vector (Vector.h:33-47): conversions elemvec <-> nodevec <-> dofval on the GPU."""
+
+    _destroy = "gf_vector_destroy"
+
+    def __init__(self, conn, dofs):
+        conn = _i64a(conn)
+        dofs = _i64a(dofs)
+        self.nelem, self.nne = conn.shape
+        self.nnode, self.ndim = dofs.shape
+        self.conn, self.dofs_h = conn, dofs
+        h = _vp()
+        _call("gf_vector_create", ctypes.byref(h), _hp(conn), _i64(self.nelem), _i32(self.nne), _hp(dofs),
+              _i64(self.nnode), _i32(self.ndim))
+        self._h = h
+        nd = _i64(0)
+        _call("gf_vector_sizes", h, None, None, None, None, ctypes.byref(nd))
+        self.ndof = nd.value
+
+    @property
+    def dofs(self):
+        return self.dofs_h
+
+    def shape_elemvec(self):
+        return (self.nelem, self.nne, self.ndim)
+
+    def shape_nodevec(self):
+        return (self.nnode, self.ndim)
+
+    def shape_dofval(self):
+        return (self.ndof,)
+
+    def AsElement(self, arg):
+        """AsElement(nodevec | dofval) (Vector.h:193-211)."""
+        arg = to_device(arg)
+        out = _empty(self.shape_elemvec())
+        if arg.dim() == 2:
+            _call("gf_vector_as_element_nodevec", self._h, _dp(arg), _dp(out), _stream())
+        else:
+            assert arg.numel() == self.ndof
+            _call("gf_vector_as_element_dofval", self._h, _dp(arg), _dp(out), _stream())
+        return out
+
+    def AsDofs(self, arg):
+        """AsDofs(nodevec | elemvec) (Vector.h:141-159): overwrite, last in loop order wins."""
+        arg = to_device(arg)
+        out = _empty(self.shape_dofval())
+        if arg.dim() == 2:
+            _call("gf_vector_as_dofs_nodevec", self._h, _dp(arg), _dp(out), _stream())
+        else:
+            _call("gf_vector_as_dofs_elemvec", self._h, _dp(arg), _dp(out), _stream())
+        return out
+
+    def AsNode(self, arg):
+        """AsNode(dofval | elemvec) (Vector.h:167-185)."""
+        arg = to_device(arg)
+        out = _empty(self.shape_nodevec())
+        if arg.dim() == 1:
+            _call("gf_vector_as_node_dofval", self._h, _dp(arg), _dp(out), _stream())
+        else:
+            _call("gf_vector_as_node_elemvec", self._h, _dp(arg), _dp(out), _stream())
+        return out
+
+    def AssembleDofs(self, arg):
+        """AssembleDofs(nodevec | elemvec) (Vector.h:219-237); elemvec may be float32."""
+        torch = _torch()
+        arg = arg if isinstance(arg, torch.Tensor) and arg.is_cuda else to_device(arg)
+        out = _empty(self.shape_dofval())
+        if arg.dim() == 2:
+            _call("gf_vector_assemble_dofs_nodevec", self._h, _dp(arg.double().contiguous()), _dp(out), _stream())
+        else:
+            arg = arg.contiguous()
+            _call("gf_vector_assemble_dofs_elemvec", self._h, _dp(arg), _i32(_dtype_code(arg)), _dp(out), _stream())
+        return out
+
+    def AssembleNode(self, arg):
+        """AssembleNode(elemvec) (Vector.h:245-263) = AsNode(AssembleDofs(elemvec))."""
+        torch = _torch()
+        arg = arg.contiguous() if isinstance(arg, torch.Tensor) and arg.is_cuda else to_device(arg)
+        out = _empty(self.shape_nodevec())
+        _call("gf_vector_assemble_node_elemvec", self._h, _dp(arg), _i32(_dtype_code(arg)), _dp(out), _stream())
+        return out
+
+
+# -------------------------------------------------------------------------------------------------
+# Quadrature (include/GooseFEM/Element*.h)
+# -------------------------------------------------------------------------------------------------
+
+
+class _Quadrature(_Handle):
+    _destroy = "gf_quad_destroy"
+    _etype = None
+
+    def __init__(self, x, xi=None, w=None, thick=1.0, rule=GAUSS):
+        x = to_device(x)
+        self.nelem = x.shape[0]
+        xi_p = w_p = None
+        nip = 0
+        if xi is not None:
+            self._xi = np.ascontiguousarray(xi, dtype=np.float64)
+            self._w = np.ascontiguousarray(w, dtype=np.float64)
+            xi_p, w_p, nip = _hp(self._xi), _hp(self._w), self._w.size
+        h = _vp()
+        _call("gf_quad_create", ctypes.byref(h), _i32(self._etype), _dp(x), _i64(self.nelem), _i32(rule), xi_p, w_p,
+              _i32(nip), _f64(thick), _stream())
+        self._h = h
+        ne, nne, nd, td, ni = _i64(), _i32(), _i32(), _i32(), _i32()
+        _call("gf_quad_sizes", h, ctypes.byref(ne), ctypes.byref(nne), ctypes.byref(nd), ctypes.byref(td),
+              ctypes.byref(ni))
+        self.nne, self.ndim, self.tdim, self.nip = nne.value, nd.value, td.value, ni.value
+
+    def _wrap(self, ptr, shape):
+        # copy out of the internal buffer (a view would outlive the handle)
+        out = _empty(shape)
+        _hip_memcpy_d2d(out, ptr, int(np.prod(shape)) * 8)
+        return out
+
+    @property
+    def GradN(self):
+        """GradN() (Element.h:575-578) [nelem, nip, nne, ndim] (copy of the device array)."""
+        p = _vp()
+        _call("gf_quad_gradN", self._h, ctypes.byref(p))
+        return self._wrap(p, (self.nelem, self.nip, self.nne, self.ndim))
+
+    @property
+    def dV(self):
+        """dV() (Element.h:584-587) [nelem, nip]."""
+        p = _vp()
+        _call("gf_quad_dV", self._h, ctypes.byref(p))
+        return self._wrap(p, (self.nelem, self.nip))
+
+    def update_x(self, x):
+        _call("gf_quad_update_x", self._h, _dp(to_device(x)), _stream())
+
+    def shape_qtensor(self, rank=2):
+        return (self.nelem, self.nip) + (self.tdim,) * rank
+
+    def _grad(self, fn, elemvec):
+        torch = _torch()
+        ev = elemvec.contiguous() if isinstance(elemvec, torch.Tensor) and elemvec.is_cuda else to_device(elemvec)
+        out = _empty(self.shape_qtensor(2), dtype=ev.dtype)
+        _call(fn, self._h, _dp(ev), _dp(out), _i32(_dtype_code(ev)), _stream())
+        return out
+
+    def GradN_vector(self, elemvec):
+        return self._grad("gf_quad_gradN_vector", elemvec)
+
+    def GradN_vector_T(self, elemvec):
+        return self._grad("gf_quad_gradN_vector_T", elemvec)
+
+    def SymGradN_vector(self, elemvec):
+        return self._grad("gf_quad_symgradN_vector", elemvec)
+
+    def InterpQuad_vector(self, elemvec):
+        ev = to_device(elemvec)
+        n = ev.shape[2]
+        out = _empty((self.nelem, self.nip, n))
+        _call("gf_quad_interp_quad_vector", self._h, _dp(ev), _i32(n), _dp(out), _stream())
+        return out
+
+    def Int_N_vector_dV(self, qvector):
+        qv = to_device(qvector)
+        n = qv.shape[2]
+        out = _empty((self.nelem, self.nne, n))
+        _call("gf_quad_int_N_vector_dV", self._h, _dp(qv), _i32(n), _dp(out), _stream())
+        return out
+
+    def Int_N_scalar_NT_dV(self, qscalar):
+        qs = to_device(qscalar)
+        K = self.nne * self.ndim
+        out = _empty((self.nelem, K, K))
+        _call("gf_quad_int_N_scalar_NT_dV", self._h, _dp(qs), _dp(out), _stream())
+        return out
+
+    def Int_gradN_dot_tensor2_dV(self, qtensor):
+        torch = _torch()
+        qt = qtensor.contiguous() if isinstance(qtensor, torch.Tensor) and qtensor.is_cuda else to_device(qtensor)
+        out = _empty((self.nelem, self.nne, self.ndim), dtype=qt.dtype)
+        _call("gf_quad_int_gradN_dot_tensor2_dV", self._h, _dp(qt), _dp(out), _i32(_dtype_code(qt)), _stream())
+        return out
+
+    def Int_gradN_dot_tensor4_dot_gradNT_dV(self, qtensor, precision=F64):
+        qt = to_device(qtensor)
+        K = self.nne * self.ndim
+        out = _empty((self.nelem, K, K))
+        _call("gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV", self._h, _dp(qt), _dp(out), _i32(precision), _stream())
+        return out
+
+
+def _hip_memcpy_d2d(dst, src_ptr, nbytes):
+    """Device-to-device copy from a library-owned buffer into a torch tensor (current stream)."""
+    hip = _hip_runtime()
+    rc = hip.hipMemcpyAsync(_vp(dst.data_ptr()), src_ptr, ctypes.c_size_t(nbytes), 3, _stream())
+    if rc != 0:
+        raise GooseFEMError(GF_EHIP, f"hipMemcpyAsync failed ({rc})")
+
+
+_hip = None
+
+
+def _hip_runtime():
+    global _hip
+    if _hip is None:
+        lib()
+        _hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime already mapped (same SONAME)
+        _hip.hipMemcpyAsync.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp]
+        _hip.hipMemcpyAsync.restype = ctypes.c_int
+    return _hip
+
+
+class _Quad4Quadrature(_Quadrature):
+    """Element::Quad4::Quadrature (ElementQuad4.h:229-547), tdim = 2."""
+
+    _etype = QUAD4
+
+    def __init__(self, x, xi=None, w=None):
+        super().__init__(x, xi, w)
+
+
+class _Quad4QuadraturePlanar(_Quadrature):
+    """Element::Quad4::QuadraturePlanar (ElementQuad4Planar.h:37-335), tdim = 3, plane strain."""
+
+    _etype = QUAD4_PLANAR
+
+    def __init__(self, x, xi=None, w=None, thick=1.0):
+        super().__init__(x, xi, w, thick)
+
+
+class _Hex8Quadrature(_Quadrature):
+    """Element::Hex8::Quadrature (ElementHex8.h:206-400)."""
+
+    _etype = HEX8
+
+    def __init__(self, x, xi=None, w=None):
+        super().__init__(x, xi, w)
+
+
+def _rule(nd, nodal):
+    a = 1.0 if nodal else 1.0 / np.sqrt(3.0)
+    if nd == 2:
+        s = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], dtype=np.float64)
+    else:
+        s = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                      [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=np.float64)
+    return s * a, np.ones(s.shape[0])
+
+
+Element = SimpleNamespace(
+    Quad4=SimpleNamespace(
+        Quadrature=_Quad4Quadrature,
+        QuadraturePlanar=_Quad4QuadraturePlanar,
+        Gauss=SimpleNamespace(xi=lambda: _rule(2, False)[0], w=lambda: _rule(2, False)[1], nip=lambda: 4),
+        Nodal=SimpleNamespace(xi=lambda: _rule(2, True)[0], w=lambda: _rule(2, True)[1], nip=lambda: 4),
+    ),
+    Hex8=SimpleNamespace(
+        Quadrature=_Hex8Quadrature,
+        Gauss=SimpleNamespace(xi=lambda: _rule(3, False)[0], w=lambda: _rule(3, False)[1], nip=lambda: 8),
+        Nodal=SimpleNamespace(xi=lambda: _rule(3, True)[0], w=lambda: _rule(3, True)[1], nip=lambda: 8),
+    ),
+)
+
+
+def elastic_stress(eps, K, G):
+    """sigma = K tr(eps) I + 2 G dev(eps) per quadrature point (callers' GMat law)."""
+    torch = _torch()
+    eps = eps.contiguous() if isinstance(eps, torch.Tensor) and eps.is_cuda else to_device(eps)
+    td = eps.shape[-1]
+    nq = eps.numel() // (td * td)
+    Kq = to_device(torch.as_tensor(K).expand(eps.shape[:-2]) if not isinstance(K, torch.Tensor)
+                   else K.expand(eps.shape[:-2]), dtype=eps.dtype)
+    Gq = to_device(torch.as_tensor(G).expand(eps.shape[:-2]) if not isinstance(G, torch.Tensor)
+                   else G.expand(eps.shape[:-2]), dtype=eps.dtype)
+    out = torch.empty_like(eps)
+    _call("gf_material_elastic_stress", _i64(nq), _i32(td), _dp(eps), _dp(Kq), _dp(Gq), _dp(out),
+          _i32(_dtype_code(eps)), _stream())
+    return out
+
+
+def elastic_tangent(K, G, td=3):
+    """C_ijkl = K I(x)I + 2 G (I_sym - I(x)I / 3) per point (FP64)."""
+    torch = _torch()
+    K = to_device(K)
+    G = to_device(torch.as_tensor(G).expand(K.shape) if not isinstance(G, torch.Tensor) else G.expand(K.shape))
+    out = _empty(tuple(K.shape) + (td,) * 4)
+    _call("gf_material_elastic_tangent", _i64(K.numel()), _i32(td), _dp(K), _dp(G), _dp(out), _stream())
+    return out
+
+
+# -------------------------------------------------------------------------------------------------
+# MatrixDiagonal / Matrix (include/GooseFEM/MatrixDiagonal.h, Matrix.h)
+# -------------------------------------------------------------------------------------------------
+
+
+class MatrixDiagonal(_Handle):
+    """GooseFEM::MatrixDiagonal (MatrixDiagonal.h:132-280)."""
+
+    _destroy = "gf_diag_destroy"
+
+    def __init__(self, conn, dofs):
+        self._vec = conn if isinstance(conn, Vector) else Vector(conn, dofs)
+        h = _vp()
+        _call("gf_diag_create", ctypes.byref(h), self._vec._h)
+        self._h = h
+
+    def assemble(self, elemmat):
+        _call("gf_diag_assemble", self._h, _dp(to_device(elemmat)), _stream())
+
+    def set(self, A):
+        _call("gf_diag_set", self._h, _dp(to_device(A)), _stream())
+
+    @property
+    def data(self):
+        p = _vp()
+        _call("gf_diag_data", self._h, ctypes.byref(p))
+        out = _empty((self._vec.ndof,))
+        _hip_memcpy_d2d(out, p, self._vec.ndof * 8)
+        return out
+
+    def Dot(self, x):
+        x = to_device(x)
+        out = _empty(tuple(x.shape))
+        fn = "gf_diag_dot_nodevec" if x.dim() == 2 else "gf_diag_dot_dofval"
+        _call(fn, self._h, _dp(x), _dp(out), _stream())
+        return out
+
+    def Solve(self, b):
+        b = to_device(b)
+        out = _empty(tuple(b.shape))
+        fn = "gf_diag_solve_nodevec" if b.dim() == 2 else "gf_diag_solve_dofval"
+        _call(fn, self._h, _dp(b), _dp(out), _stream())
+        return out
+
+    def inverse_ptr(self):
+        p = _vp()
+        _call("gf_diag_inverse", self._h, ctypes.byref(p), _stream())
+        return p
+
+
+class Matrix(_Handle):
+    """GooseFEM::Matrix (Matrix.h:653-860) as a device CSR."""
+
+    _destroy = "gf_matrix_destroy"
+
+    def __init__(self, conn, dofs):
+        self._vec = conn if isinstance(conn, Vector) else Vector(conn, dofs)
+        h = _vp()
+        _call("gf_matrix_create", ctypes.byref(h), self._vec._h)
+        self._h = h
+        nr, nz = _i64(), _i64()
+        _call("gf_matrix_sizes", h, ctypes.byref(nr), ctypes.byref(nz))
+        self.nrows, self.nnz = nr.value, nz.value
+
+    def assemble(self, elemmat):
+        _call("gf_matrix_assemble", self._h, _dp(to_device(elemmat)), _stream())
+
+    def Dot(self, x):
+        x = to_device(x)
+        out = _empty(tuple(x.shape))
+        fn = "gf_matrix_dot_nodevec" if x.dim() == 2 else "gf_matrix_dot_dofval"
+        _call(fn, self._h, _dp(x), _dp(out), _stream())
+        return out
+
+    def Todense(self):
+        out = _empty((self.nrows, self.nrows))
+        _call("gf_matrix_todense", self._h, _dp(out), _stream())
+        return out
+
+    def csr(self):
+        """(rowptr, col, val) as GPU tensors (copies)."""
+        torch = _torch()
+        rp, cp, vp = _vp(), _vp(), _vp()
+        _call("gf_matrix_csr", self._h, ctypes.byref(rp), ctypes.byref(cp), ctypes.byref(vp))
+        rowptr = torch.empty(self.nrows + 1, dtype=torch.int32, device=device())
+        col = torch.empty(self.nnz, dtype=torch.int32, device=device())
+        val = _empty((self.nnz,))
+        _hip_memcpy_d2d(rowptr, rp, 4 * (self.nrows + 1))
+        _hip_memcpy_d2d(col, cp, 4 * self.nnz)
+        _hip_memcpy_d2d(val, vp, 8 * self.nnz)
+        return rowptr, col, val
+
+
+# -------------------------------------------------------------------------------------------------
+# Explicit dynamics (fused velocity-Verlet step)
+# -------------------------------------------------------------------------------------------------
+
+
+class ExplicitQuad4(_Handle):
+    """Fused velocity-Verlet integrator of docs/examples/dynamics/Elastic-VelocityVerlet/main.cpp
+    on a Quad4 plane-strain mesh with per-element moduli (K_e, G_e as float32)."""
+
+    _destroy = "gf_explicit_destroy"
+
+    def __init__(self, vector: Vector, coor, K_e, G_e, inv_mass, dt, fext=None):
+        torch = _torch()
+        self._vec = vector
+        self._coor = to_device(coor)
+        self._K = to_device(K_e, dtype=torch.float32)
+        self._G = to_device(G_e, dtype=torch.float32)
+        self._inv = to_device(inv_mass)
+        self._fext = None if fext is None else to_device(fext)
+        h = _vp()
+        _call("gf_explicit_create", ctypes.byref(h), vector._h, _dp(self._coor), _dp(self._K), _dp(self._G),
+              _dp(self._inv), _dp(self._fext), _f64(dt))
+        self._h = h
+
+    @property
+    def structured(self):
+        s = _i32()
+        _call("gf_explicit_is_structured", self._h, ctypes.byref(s))
+        return bool(s.value)
+
+    def _state_ptrs(self):
+        u, v, a = _vp(), _vp(), _vp()
+        _call("gf_explicit_state", self._h, ctypes.byref(u), ctypes.byref(v), ctypes.byref(a))
+        return u, v, a
+
+    def set_state(self, u, v, a):
+        n = self._vec.ndof
+        for ptr, src in zip(self._state_ptrs(), (u, v, a)):
+            src = to_device(src)
+            assert src.numel() == n
+            hip = _hip_runtime()
+            rc = hip.hipMemcpyAsync(ptr, _vp(src.data_ptr()), ctypes.c_size_t(8 * n), 3, _stream())
+            if rc:
+                raise GooseFEMError(GF_EHIP, "hipMemcpyAsync")
+        _torch().cuda.current_stream().synchronize()
+
+    def state(self):
+        n = self._vec.ndof
+        outs = []
+        for ptr in self._state_ptrs():
+            o = _empty((n,))
+            _hip_memcpy_d2d(o, ptr, 8 * n)
+            outs.append(o)
+        return tuple(outs)
+
+    def step(self, nsteps=1, precision=F64, store_fint=False, stream=None):
+        s = _stream() if stream is None else stream
+        _call("gf_explicit_step", self._h, _i64(nsteps), _i32(precision), _i32(int(store_fint)), s)
+
+    def fint(self):
+        p = _vp()
+        _call("gf_explicit_fint", self._h, ctypes.byref(p))
+        o = _empty((self._vec.ndof,))
+        _hip_memcpy_d2d(o, p, 8 * self._vec.ndof)
+        return o

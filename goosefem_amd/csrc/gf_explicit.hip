@@ -1,0 +1,610 @@
+// Fused velocity-Verlet step for Quad4 plane-strain explicit dynamics: the caller loop of
+// docs/examples/dynamics/Elastic-VelocityVerlet/main.cpp:110-157, i.e.
+//   u <- (u + dt v) + c a                       (c = dt^2 / 2)
+//   fint = AssembleDofs(Int_gradN_dot_tensor2_dV(sigma(SymGradN_vector(AsElement(u)))))
+//   a_new = inv_mass (fext - fint);  v <- v + (dt/2)(a + a_new);  a <- a_new
+// with QuadraturePlanar geometry (ElementQuad4Planar.h:280-319, Gauss 2x2, thick 1) and the linear
+// elastic law sigma = K tr(eps) I + 2 G dev(eps).
+//
+// Structured path (conn is a Mesh::Quad4::Regular grid, dofs = Mesh::dofs): one kernel per step.
+// A workgroup owns a TX x TY tile of nodes.  It stages the updated displacement and coordinates
+// of the tile plus a one-node halo in LDS, computes the forces of the (TX+1) x (TY+1) elements
+// touching the tile (geometry recomputed from the coordinates), keeps them in LDS, and each
+// owned node sums its <= 4 element contributions in ascending element order -- the reference's
+// serial assembly order -- then applies the velocity-Verlet update.  HBM traffic per step is the
+// state (u, v, a read + write), inv_mass, coordinates and per-element moduli; no elemvec,
+// qtensor or connectivity stream.  State is double-buffered (read old, write new).
+// Tiles are ordered so each XCD sweeps a contiguous band of the mesh (its L2 sees the halo rows
+// its own previous tiles loaded).
+//
+// General path (any Quad4 mesh / dof numbering): three kernels per step -- displacement update,
+// element kernel (gather + strain + stress + force, FP64 or FP32), DOF-centric ordered gather +
+// update.
+#include <cmath>
+#include <type_traits>
+
+#include "gf_internal.hpp"
+
+using namespace gf;
+
+namespace gf {
+const double* quad_dNx(gf_quad q);
+const double* quad_vol(gf_quad q);
+} // namespace gf
+
+struct gf_explicit_s {
+    gf_vector v = nullptr;
+    int64_t ndof = 0, nelem = 0;
+    double dt = 0.0;
+    bool structured = false;
+    int64_t nx = 0, ny = 0; // structured grid size (elements)
+    DBuf<double> coor;      // nodevec [nnode, 2]
+    DBuf<float> Ke, Ge;     // per element
+    DBuf<double> inv_mass, fext;
+    bool has_fext = false;
+    DBuf<double> u[2], vv[2], a[2];
+    int cur = 0;
+    DBuf<double> fint;
+    bool fint_valid = false;
+    // general path
+    gf_quad quad = nullptr;
+    DBuf<double> fe64;
+    DBuf<float> fe32, dNx32, vol32;
+};
+
+namespace {
+
+struct ShapeTable {
+    double g[4][4][2]; // dN/dxi at the 4 Gauss points [q][m][i]
+    double w[4];
+};
+
+constexpr int BS = 256;
+
+// ---- per-element force, FP64 in the reference's expression order ------------------------------
+__device__ inline void element_force_f64(const ShapeTable& T, const double x[4][2], const double u[4][2],
+                                         double k, double gmod, double f[4][2])
+{
+#pragma unroll
+    for (int m = 0; m < 4; ++m) f[m][0] = f[m][1] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double(*g)[2] = T.g[q];
+        double J00 = g[0][0] * x[0][0] + g[1][0] * x[1][0] + g[2][0] * x[2][0] + g[3][0] * x[3][0];
+        double J01 = g[0][0] * x[0][1] + g[1][0] * x[1][1] + g[2][0] * x[2][1] + g[3][0] * x[3][1];
+        double J10 = g[0][1] * x[0][0] + g[1][1] * x[1][0] + g[2][1] * x[2][0] + g[3][1] * x[3][0];
+        double J11 = g[0][1] * x[0][1] + g[1][1] * x[1][1] + g[2][1] * x[2][1] + g[3][1] * x[3][1];
+        double det = J00 * J11 - J01 * J10;
+        double I00 = J11 / det, I01 = -1.0 * J01 / det, I10 = -1.0 * J10 / det, I11 = J00 / det;
+        double d[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            d[m][0] = I00 * g[m][0] + I01 * g[m][1];
+            d[m][1] = I10 * g[m][0] + I11 * g[m][1];
+        }
+        double vol = T.w[q] * det * 1.0;
+        // SymGradN_vector, ElementQuad4Planar.h:191-221
+        double e00 = d[0][0] * u[0][0] + d[1][0] * u[1][0] + d[2][0] * u[2][0] + d[3][0] * u[3][0];
+        double e11 = d[0][1] * u[0][1] + d[1][1] * u[1][1] + d[2][1] * u[2][1] + d[3][1] * u[3][1];
+        double e01 = 0.5 * (d[0][0] * u[0][1] + d[1][0] * u[1][1] + d[2][0] * u[2][1] + d[3][0] * u[3][1] +
+                            d[0][1] * u[0][0] + d[1][1] * u[1][0] + d[2][1] * u[2][0] + d[3][1] * u[3][0]);
+        double e22 = 0.0;
+        // linear elastic law (tdim 3, plane strain)
+        double tr = e00 + e11 + e22;
+        double s00 = k * tr + 2.0 * gmod * (e00 - tr / 3.0);
+        double s11 = k * tr + 2.0 * gmod * (e11 - tr / 3.0);
+        double s01 = 2.0 * gmod * e01;
+        double s10 = 2.0 * gmod * e01;
+        // Int_gradN_dot_tensor2_dV, ElementQuad4Planar.h:253-278
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            f[m][0] += (d[m][0] * s00 + d[m][1] * s10) * vol;
+            f[m][1] += (d[m][0] * s01 + d[m][1] * s11) * vol;
+        }
+    }
+}
+
+// ---- per-element force, FP32 (differences taken in FP64 first, so no cancellation) --------------
+__device__ inline void element_force_f32(const ShapeTable& T, const double x[4][2], const double u[4][2], float k,
+                                         float gmod, float f[4][2])
+{
+    float xr[4][2], ur[4][2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            xr[m][i] = static_cast<float>(x[m][i] - x[0][i]);
+            ur[m][i] = static_cast<float>(u[m][i] - u[0][i]);
+        }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) f[m][0] = f[m][1] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float g[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            g[m][0] = static_cast<float>(T.g[q][m][0]);
+            g[m][1] = static_cast<float>(T.g[q][m][1]);
+        }
+        float J00 = 0.f, J01 = 0.f, J10 = 0.f, J11 = 0.f;
+#pragma unroll
+        for (int m = 1; m < 4; ++m) {
+            J00 = fmaf(g[m][0], xr[m][0], J00);
+            J01 = fmaf(g[m][0], xr[m][1], J01);
+            J10 = fmaf(g[m][1], xr[m][0], J10);
+            J11 = fmaf(g[m][1], xr[m][1], J11);
+        }
+        float det = J00 * J11 - J01 * J10;
+        float rdet = 1.0f / det;
+        float I00 = J11 * rdet, I01 = -J01 * rdet, I10 = -J10 * rdet, I11 = J00 * rdet;
+        float vol = static_cast<float>(T.w[q]) * det;
+        float du00 = 0.f, du01 = 0.f, du10 = 0.f, du11 = 0.f; // du(i,j) = dNx(m,i) u(m,j)
+        float d[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            d[m][0] = fmaf(I00, g[m][0], I01 * g[m][1]);
+            d[m][1] = fmaf(I10, g[m][0], I11 * g[m][1]);
+        }
+#pragma unroll
+        for (int m = 1; m < 4; ++m) {
+            du00 = fmaf(d[m][0], ur[m][0], du00);
+            du01 = fmaf(d[m][0], ur[m][1], du01);
+            du10 = fmaf(d[m][1], ur[m][0], du10);
+            du11 = fmaf(d[m][1], ur[m][1], du11);
+        }
+        float e01 = 0.5f * (du01 + du10);
+        float tr = du00 + du11;
+        float kt = k * tr, t3 = tr * (1.0f / 3.0f), g2 = 2.0f * gmod;
+        float s00 = fmaf(g2, du00 - t3, kt);
+        float s11 = fmaf(g2, du11 - t3, kt);
+        float s01 = g2 * e01;
+        float sv00 = s00 * vol, sv11 = s11 * vol, sv01 = s01 * vol;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            f[m][0] += fmaf(d[m][0], sv00, d[m][1] * sv01);
+            f[m][1] += fmaf(d[m][0], sv01, d[m][1] * sv11);
+        }
+    }
+}
+
+// ---- structured fused step ----------------------------------------------------------------------
+template <int TX, int TY, bool F32>
+__global__ void __launch_bounds__(TX * TY) k_vv_structured(
+    int nx, int ny, int ntx, int nty, int per_xcd, ShapeTable T, const double* __restrict__ coor,
+    const float* __restrict__ Ke, const float* __restrict__ Ge, const double* __restrict__ inv_mass,
+    const double* __restrict__ fext, double dt, double c, double hdt, const double* __restrict__ u0,
+    const double* __restrict__ v0, const double* __restrict__ a0, double* __restrict__ u1, double* __restrict__ v1,
+    double* __restrict__ a1, double* __restrict__ fint_out)
+{
+    constexpr int HX = TX + 2, HY = TY + 2;   // halo node tile
+    constexpr int EX = TX + 1, EY = TY + 1;   // element tile
+    constexpr int NT = TX * TY;
+    using FT = typename std::conditional<F32, float, double>::type;
+    __shared__ double2 s_u[HY * HX];
+    __shared__ double2 s_x[HY * HX];
+    __shared__ FT s_f[EY * EX][8];
+
+    // XCD-aware tile order: hardware places block b on XCD b % 8; give each XCD a contiguous band
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);
+    if (tile >= ntx * nty) return; // uniform for the whole block
+    const int tx = tile % ntx, ty = tile / ntx;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int NX = nx + 1, NY = ny + 1;
+
+    // phase A: updated displacement and coordinates of the tile + halo
+    for (int t = threadIdx.x; t < HX * HY; t += NT) {
+        const int ly = t / HX, lx = t - ly * HX;
+        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+        if (gx >= 0 && gx < NX && gy >= 0 && gy < NY) {
+            const int64_t n = static_cast<int64_t>(gy) * NX + gx;
+            const double2 uu = reinterpret_cast<const double2*>(u0)[n];
+            const double2 vv = reinterpret_cast<const double2*>(v0)[n];
+            const double2 aa = reinterpret_cast<const double2*>(a0)[n];
+            double2 un;
+            un.x = uu.x + dt * vv.x + c * aa.x;
+            un.y = uu.y + dt * vv.y + c * aa.y;
+            s_u[t] = un;
+            s_x[t] = reinterpret_cast<const double2*>(coor)[n];
+        }
+    }
+    __syncthreads();
+
+    // phase B: element forces of the element tile
+    for (int t = threadIdx.x; t < EX * EY; t += NT) {
+        const int ly = t / EX, lx = t - ly * EX;
+        const int ex = x0 - 1 + lx, ey = y0 - 1 + ly;
+        if (ex >= 0 && ex < nx && ey >= 0 && ey < ny) {
+            const int64_t e = static_cast<int64_t>(ey) * nx + ex;
+            // local node (lx, ly) is global (ex, ey); conn order 0:(0,0) 1:(1,0) 2:(1,1) 3:(0,1)
+            const int n0 = ly * HX + lx;
+            const int nid[4] = {n0, n0 + 1, n0 + HX + 1, n0 + HX};
+            double x[4][2], u[4][2];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const double2 xx = s_x[nid[m]], uu = s_u[nid[m]];
+                x[m][0] = xx.x;
+                x[m][1] = xx.y;
+                u[m][0] = uu.x;
+                u[m][1] = uu.y;
+            }
+            if constexpr (F32) {
+                float f[4][2];
+                element_force_f32(T, x, u, Ke[e], Ge[e], f);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    s_f[t][2 * m] = f[m][0];
+                    s_f[t][2 * m + 1] = f[m][1];
+                }
+            }
+            else {
+                double f[4][2];
+                element_force_f64(T, x, u, static_cast<double>(Ke[e]), static_cast<double>(Ge[e]), f);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    s_f[t][2 * m] = f[m][0];
+                    s_f[t][2 * m + 1] = f[m][1];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // phase C: owned nodes -- ordered assembly + velocity-Verlet update
+    {
+        const int ly = threadIdx.x / TX, lx = threadIdx.x - (threadIdx.x / TX) * TX;
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx < NX && gy < NY) {
+            const int64_t n = static_cast<int64_t>(gy) * NX + gx;
+            // elements in ascending number: (gx-1,gy-1) slot 2, (gx,gy-1) slot 3, (gx-1,gy) slot 1,
+            // (gx,gy) slot 0; local element (lx', ly') = (ex - x0 + 1, ey - y0 + 1)
+            double f0 = 0.0, f1 = 0.0;
+            const bool hasL = gx > 0, hasR = gx < nx, hasB = gy > 0, hasT = gy < ny;
+            if (hasB && hasL) {
+                const int le = ly * EX + lx;
+                f0 += static_cast<double>(s_f[le][4]);
+                f1 += static_cast<double>(s_f[le][5]);
+            }
+            if (hasB && hasR) {
+                const int le = ly * EX + lx + 1;
+                f0 += static_cast<double>(s_f[le][6]);
+                f1 += static_cast<double>(s_f[le][7]);
+            }
+            if (hasT && hasL) {
+                const int le = (ly + 1) * EX + lx;
+                f0 += static_cast<double>(s_f[le][2]);
+                f1 += static_cast<double>(s_f[le][3]);
+            }
+            if (hasT && hasR) {
+                const int le = (ly + 1) * EX + lx + 1;
+                f0 += static_cast<double>(s_f[le][0]);
+                f1 += static_cast<double>(s_f[le][1]);
+            }
+            const double2 un = s_u[(ly + 1) * HX + lx + 1];
+            const double2 vv = reinterpret_cast<const double2*>(v0)[n];
+            const double2 aa = reinterpret_cast<const double2*>(a0)[n];
+            const double2 im = reinterpret_cast<const double2*>(inv_mass)[n];
+            double2 fe = make_double2(0.0, 0.0);
+            if (fext) fe = reinterpret_cast<const double2*>(fext)[n];
+            double2 an, vn;
+            an.x = im.x * (fe.x - f0);
+            an.y = im.y * (fe.y - f1);
+            vn.x = vv.x + hdt * (aa.x + an.x);
+            vn.y = vv.y + hdt * (aa.y + an.y);
+            reinterpret_cast<double2*>(u1)[n] = un;
+            reinterpret_cast<double2*>(v1)[n] = vn;
+            reinterpret_cast<double2*>(a1)[n] = an;
+            if (fint_out) reinterpret_cast<double2*>(fint_out)[n] = make_double2(f0, f1);
+        }
+    }
+}
+
+// ---- general path --------------------------------------------------------------------------------
+__global__ void k_update_u(int64_t n, double dt, double c, const double* __restrict__ u0, const double* __restrict__ v0,
+                           const double* __restrict__ a0, double* __restrict__ u1)
+{
+    int64_t d = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (d < n) u1[d] = u0[d] + dt * v0[d] + c * a0[d];
+}
+
+template <bool F32>
+__global__ void k_element_general(int64_t nelem, const int32_t* __restrict__ conn, const int32_t* __restrict__ dofs,
+                                  const double* __restrict__ dNx, const double* __restrict__ vol,
+                                  const float* __restrict__ Ke, const float* __restrict__ Ge,
+                                  const double* __restrict__ u, double* __restrict__ fe64, float* __restrict__ fe32)
+{
+    int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= nelem) return;
+    double ue[4][2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) ue[m][i] = u[dofs[static_cast<int64_t>(conn[e * 4 + m]) * 2 + i]];
+    if constexpr (!F32) {
+        // reference order: SymGradN_vector -> stress -> Int_gradN_dot_tensor2_dV (stored geometry)
+        double f[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) f[m][0] = f[m][1] = 0.0;
+        const double k = Ke[e], gmod = Ge[e];
+        for (int q = 0; q < 4; ++q) {
+            const double* d = dNx + (e * 4 + q) * 8;
+            double e00 = d[0] * ue[0][0] + d[2] * ue[1][0] + d[4] * ue[2][0] + d[6] * ue[3][0];
+            double e11 = d[1] * ue[0][1] + d[3] * ue[1][1] + d[5] * ue[2][1] + d[7] * ue[3][1];
+            double e01 = 0.5 * (d[0] * ue[0][1] + d[2] * ue[1][1] + d[4] * ue[2][1] + d[6] * ue[3][1] +
+                                d[1] * ue[0][0] + d[3] * ue[1][0] + d[5] * ue[2][0] + d[7] * ue[3][0]);
+            double e22 = 0.0;
+            double tr = e00 + e11 + e22;
+            double s00 = k * tr + 2.0 * gmod * (e00 - tr / 3.0);
+            double s11 = k * tr + 2.0 * gmod * (e11 - tr / 3.0);
+            double s01 = 2.0 * gmod * e01;
+            double s10 = 2.0 * gmod * e01;
+            double v = vol[e * 4 + q];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                f[m][0] += (d[2 * m] * s00 + d[2 * m + 1] * s10) * v;
+                f[m][1] += (d[2 * m] * s01 + d[2 * m + 1] * s11) * v;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            fe64[e * 8 + 2 * m] = f[m][0];
+            fe64[e * 8 + 2 * m + 1] = f[m][1];
+        }
+    }
+    else {
+        float ur[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) ur[m][i] = static_cast<float>(ue[m][i] - ue[0][i]);
+        float f[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) f[m][0] = f[m][1] = 0.0f;
+        const float k = Ke[e], gmod = Ge[e];
+        for (int q = 0; q < 4; ++q) {
+            float d[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) d[j] = static_cast<float>(dNx[(e * 4 + q) * 8 + j]);
+            float du00 = 0.f, du01 = 0.f, du10 = 0.f, du11 = 0.f;
+#pragma unroll
+            for (int m = 1; m < 4; ++m) {
+                du00 = fmaf(d[2 * m], ur[m][0], du00);
+                du01 = fmaf(d[2 * m], ur[m][1], du01);
+                du10 = fmaf(d[2 * m + 1], ur[m][0], du10);
+                du11 = fmaf(d[2 * m + 1], ur[m][1], du11);
+            }
+            float tr = du00 + du11, t3 = tr * (1.0f / 3.0f), g2 = 2.0f * gmod, kt = k * tr;
+            float vq = static_cast<float>(vol[e * 4 + q]);
+            float sv00 = fmaf(g2, du00 - t3, kt) * vq, sv11 = fmaf(g2, du11 - t3, kt) * vq;
+            float sv01 = g2 * 0.5f * (du01 + du10) * vq;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                f[m][0] += fmaf(d[2 * m], sv00, d[2 * m + 1] * sv01);
+                f[m][1] += fmaf(d[2 * m], sv01, d[2 * m + 1] * sv11);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            fe32[e * 8 + 2 * m] = f[m][0];
+            fe32[e * 8 + 2 * m + 1] = f[m][1];
+        }
+    }
+}
+
+template <class T>
+__global__ void k_node_update(int64_t ndof, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                              const T* __restrict__ fe, const double* __restrict__ inv_mass,
+                              const double* __restrict__ fext, double hdt, const double* __restrict__ v0,
+                              const double* __restrict__ a0, double* __restrict__ v1, double* __restrict__ a1,
+                              double* __restrict__ fint_out)
+{
+    int64_t d = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (d >= ndof) return;
+    double s = 0.0;
+    for (int32_t j = ptr[d]; j < ptr[d + 1]; ++j) s += static_cast<double>(fe[idx[j]]);
+    double an = inv_mass[d] * ((fext ? fext[d] : 0.0) - s);
+    v1[d] = v0[d] + hdt * (a0[d] + an);
+    a1[d] = an;
+    if (fint_out) fint_out[d] = s;
+}
+
+ShapeTable gauss_table()
+{
+    ShapeTable T;
+    const double a = 1.0 / std::sqrt(3.0);
+    const double xi[4][2] = {{-a, -a}, {+a, -a}, {+a, +a}, {-a, +a}};
+    for (int q = 0; q < 4; ++q) {
+        const double x0 = xi[q][0], x1 = xi[q][1];
+        T.g[q][0][0] = -0.25 * (1.0 - x1);
+        T.g[q][1][0] = +0.25 * (1.0 - x1);
+        T.g[q][2][0] = +0.25 * (1.0 + x1);
+        T.g[q][3][0] = -0.25 * (1.0 + x1);
+        T.g[q][0][1] = -0.25 * (1.0 - x0);
+        T.g[q][1][1] = -0.25 * (1.0 + x0);
+        T.g[q][2][1] = +0.25 * (1.0 + x0);
+        T.g[q][3][1] = +0.25 * (1.0 - x0);
+        T.w[q] = 1.0;
+    }
+    return T;
+}
+
+// Is conn the Mesh::Quad4::Regular(nx, ny) connectivity and dofs = Mesh::dofs?
+bool detect_structured(const gf_vector_s* v, int64_t& nx, int64_t& ny)
+{
+    if (v->nne != 4 || v->ndim != 2 || v->nelem == 0) return false;
+    const auto& c = v->conn_h;
+    nx = c[3] - 1;
+    if (nx < 1 || v->nelem % nx) return false;
+    ny = v->nelem / nx;
+    if ((nx + 1) * (ny + 1) != v->nnode) return false;
+    for (int64_t ey = 0, e = 0; ey < ny; ++ey)
+        for (int64_t ex = 0; ex < nx; ++ex, ++e) {
+            int64_t n0 = ey * (nx + 1) + ex;
+            if (c[4 * e] != n0 || c[4 * e + 1] != n0 + 1 || c[4 * e + 2] != n0 + nx + 2 || c[4 * e + 3] != n0 + nx + 1)
+                return false;
+        }
+    for (int64_t i = 0; i < v->nnode * 2; ++i)
+        if (v->dofs_h[i] != i) return false;
+    return nx < (1 << 30) && ny < (1 << 30);
+}
+
+constexpr int TX = 64, TY = 8;
+
+} // namespace
+
+extern "C" {
+
+int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const float* K_e, const float* G_e,
+                       const double* inv_mass, const double* fext, double dt)
+{
+    return guard([&] {
+        GF_REQUIRE(out && v && coor && K_e && G_e && inv_mass, "null argument");
+        GF_REQUIRE(v->nne == 4 && v->ndim == 2, "explicit integrator needs a Quad4 mesh (nne 4, ndim 2)");
+        auto x = new gf_explicit_s();
+        try {
+            x->v = v;
+            x->ndof = v->ndof;
+            x->nelem = v->nelem;
+            x->dt = dt;
+            x->coor.alloc(v->nnode * 2);
+            GF_HIP(hipMemcpy(x->coor.p, coor, x->coor.n * sizeof(double), hipMemcpyDeviceToDevice));
+            x->Ke.alloc(v->nelem);
+            x->Ge.alloc(v->nelem);
+            GF_HIP(hipMemcpy(x->Ke.p, K_e, v->nelem * sizeof(float), hipMemcpyDeviceToDevice));
+            GF_HIP(hipMemcpy(x->Ge.p, G_e, v->nelem * sizeof(float), hipMemcpyDeviceToDevice));
+            x->inv_mass.alloc(v->ndof);
+            GF_HIP(hipMemcpy(x->inv_mass.p, inv_mass, v->ndof * sizeof(double), hipMemcpyDeviceToDevice));
+            if (fext) {
+                x->fext.alloc(v->ndof);
+                GF_HIP(hipMemcpy(x->fext.p, fext, v->ndof * sizeof(double), hipMemcpyDeviceToDevice));
+                x->has_fext = true;
+            }
+            for (int k = 0; k < 2; ++k) {
+                x->u[k].alloc(v->ndof);
+                x->vv[k].alloc(v->ndof);
+                x->a[k].alloc(v->ndof);
+            }
+            for (auto* b : {&x->u[0], &x->vv[0], &x->a[0]})
+                GF_HIP(hipMemset(b->p, 0, v->ndof * sizeof(double)));
+            x->fint.alloc(v->ndof);
+            x->structured = detect_structured(v, x->nx, x->ny);
+            if (!x->structured) {
+                // stored Gauss geometry (QuadraturePlanar(AsElement(coor)))
+                DBuf<double> xe(v->nelem * 8);
+                int rc = gf_vector_as_element_nodevec(v, x->coor.p, xe.p, nullptr);
+                if (rc) throw Error(rc, gf_last_error());
+                rc = gf_quad_create(&x->quad, GF_QUAD4_PLANAR, xe.p, v->nelem, GF_GAUSS, nullptr, nullptr, 0, 1.0,
+                                    nullptr);
+                if (rc) throw Error(rc, gf_last_error());
+                GF_HIP(hipDeviceSynchronize());
+                x->fe64.alloc(v->nelem * 8);
+                x->fe32.alloc(v->nelem * 8);
+            }
+            GF_HIP(hipDeviceSynchronize());
+        }
+        catch (...) {
+            if (x->quad) gf_quad_destroy(x->quad);
+            delete x;
+            throw;
+        }
+        *out = x;
+    });
+}
+
+int gf_explicit_destroy(gf_explicit x)
+{
+    if (x && x->quad) gf_quad_destroy(x->quad);
+    delete x;
+    return GF_OK;
+}
+
+int gf_explicit_state(gf_explicit x, double** u, double** v, double** a)
+{
+    return guard([&] {
+        GF_REQUIRE(x, "null integrator");
+        if (u) *u = x->u[x->cur].p;
+        if (v) *v = x->vv[x->cur].p;
+        if (a) *a = x->a[x->cur].p;
+    });
+}
+
+int gf_explicit_is_structured(gf_explicit x, int* structured)
+{
+    return guard([&] {
+        GF_REQUIRE(x && structured, "null argument");
+        *structured = x->structured ? 1 : 0;
+    });
+}
+
+int gf_explicit_fint(gf_explicit x, const double** fint)
+{
+    return guard([&] {
+        GF_REQUIRE(x && fint, "null argument");
+        if (!x->fint_valid) throw Error(GF_ESTATE, "no step run with store_fint");
+        *fint = x->fint.p;
+    });
+}
+
+int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fint, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(x, "null integrator");
+        GF_REQUIRE(precision == GF_F64 || precision == GF_F32, "precision must be GF_F64 or GF_F32");
+        GF_REQUIRE(nsteps >= 0, "nsteps >= 0");
+        hipStream_t s = S(stream);
+        const double dt = x->dt, c = 0.5 * dt * dt, hdt = 0.5 * dt;
+        const double* fext = x->has_fext ? x->fext.p : nullptr;
+        const bool f32 = precision == GF_F32;
+        if (x->structured) {
+            static const ShapeTable T = gauss_table();
+            const int nx = static_cast<int>(x->nx), ny = static_cast<int>(x->ny);
+            const int ntx = (nx + 1 + TX - 1) / TX, nty = (ny + 1 + TY - 1) / TY;
+            const int ntiles = ntx * nty, per_xcd = (ntiles + 7) / 8;
+            const unsigned grid = static_cast<unsigned>(per_xcd * 8);
+            for (int64_t k = 0; k < nsteps; ++k) {
+                const int i0 = x->cur, i1 = 1 - x->cur;
+                double* fo = (store_fint && k == nsteps - 1) ? x->fint.p : nullptr;
+                if (f32)
+                    k_vv_structured<TX, TY, true><<<grid, TX * TY, 0, s>>>(
+                        nx, ny, ntx, nty, per_xcd, T, x->coor.p, x->Ke.p, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt,
+                        x->u[i0].p, x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p, x->a[i1].p, fo);
+                else
+                    k_vv_structured<TX, TY, false><<<grid, TX * TY, 0, s>>>(
+                        nx, ny, ntx, nty, per_xcd, T, x->coor.p, x->Ke.p, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt,
+                        x->u[i0].p, x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p, x->a[i1].p, fo);
+                check_launch();
+                x->cur = i1;
+            }
+        }
+        else {
+            auto v = x->v;
+            const double* dNx = gf::quad_dNx(x->quad);
+            const double* vol = gf::quad_vol(x->quad);
+            for (int64_t k = 0; k < nsteps; ++k) {
+                const int i0 = x->cur, i1 = 1 - x->cur;
+                double* fo = (store_fint && k == nsteps - 1) ? x->fint.p : nullptr;
+                k_update_u<<<nblocks(x->ndof, BS), BS, 0, s>>>(x->ndof, dt, c, x->u[i0].p, x->vv[i0].p, x->a[i0].p,
+                                                             x->u[i1].p);
+                if (f32) {
+                    k_element_general<true><<<nblocks(x->nelem, BS), BS, 0, s>>>(
+                        x->nelem, v->conn.p, v->dofs.p, dNx, vol, x->Ke.p, x->Ge.p, x->u[i1].p, nullptr, x->fe32.p);
+                    k_node_update<float><<<nblocks(x->ndof, BS), BS, 0, s>>>(
+                        x->ndof, v->dof_eptr.p, v->dof_eidx.p, x->fe32.p, x->inv_mass.p, fext, hdt, x->vv[i0].p,
+                        x->a[i0].p, x->vv[i1].p, x->a[i1].p, fo);
+                }
+                else {
+                    k_element_general<false><<<nblocks(x->nelem, BS), BS, 0, s>>>(
+                        x->nelem, v->conn.p, v->dofs.p, dNx, vol, x->Ke.p, x->Ge.p, x->u[i1].p, x->fe64.p, nullptr);
+                    k_node_update<double><<<nblocks(x->ndof, BS), BS, 0, s>>>(
+                        x->ndof, v->dof_eptr.p, v->dof_eidx.p, x->fe64.p, x->inv_mass.p, fext, hdt, x->vv[i0].p,
+                        x->a[i0].p, x->vv[i1].p, x->a[i1].p, fo);
+                }
+                check_launch();
+                x->cur = i1;
+            }
+        }
+        if (store_fint && nsteps > 0) x->fint_valid = true;
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,156 @@
+// Internal helpers shared by the goosefem_amd translation units: error reporting, HIP checks,
+// device buffers, launch geometry.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/goosefem_amd.h"
+
+namespace gf {
+
+void set_error(const std::string& msg);
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define GF_REQUIRE(cond, msg)                                                                       \
+    do {                                                                                            \
+        if (!(cond)) throw ::gf::Error(GF_EINVAL, std::string(__func__) + ": " + (msg));            \
+    } while (0)
+
+#define GF_HIP(call)                                                                                \
+    do {                                                                                            \
+        hipError_t _e = (call);                                                                     \
+        if (_e != hipSuccess)                                                                       \
+            throw ::gf::Error(GF_EHIP, std::string(#call) + ": " + hipGetErrorString(_e));          \
+    } while (0)
+
+// Runs `body`, converting exceptions into a status code + thread-local message.
+template <class F>
+inline int guard(F&& body)
+{
+    try {
+        body();
+        return GF_OK;
+    }
+    catch (const Error& e) {
+        set_error(e.what());
+        return e.code;
+    }
+    catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return GF_ENOMEM;
+    }
+    catch (const std::exception& e) {
+        set_error(e.what());
+        return GF_EINVAL;
+    }
+}
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Owning device buffer.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        release();
+        p = o.p;
+        n = o.n;
+        o.p = nullptr;
+        o.n = 0;
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count == 0) return;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            n = 0;
+            throw Error(GF_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count)
+    {
+        if (count != n) alloc(count);
+        if (count) GF_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void upload(const std::vector<T>& h) { upload(h.data(), h.size()); }
+};
+
+inline unsigned nblocks(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
+
+inline void check_launch() { GF_HIP(hipGetLastError()); }
+
+// Host-side parallel loop over [0, n) in contiguous chunks (set-up code only).
+template <class F>
+inline void parallel_for(int64_t n, F&& f)
+{
+    int nt = static_cast<int>(std::min<int64_t>(std::max(1u, std::thread::hardware_concurrency()), 16));
+    if (n < 4096 || nt == 1) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    int64_t chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        int64_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([b, e, &f] {
+            for (int64_t i = b; i < e; ++i) f(i);
+        });
+    }
+    for (auto& t : th) t.join();
+}
+
+// Hex8 stiffness on MFMA (BF16 hi/lo split, FP32 accumulation), gf_stiffness.hip
+int stiffness_bf16x3(gf_quad q, const double* qtensor4, double* elemmat, hipStream_t s);
+
+// Narrow an int64 host index array to int32 (device form), checking the range.
+std::vector<int32_t> narrow(const int64_t* a, size_t n, const char* what);
+
+} // namespace gf
+
+// Opaque handle definitions shared across translation units.
+struct gf_vector_s {
+    int64_t nelem = 0, nnode = 0, ndof = 0;
+    int nne = 0, ndim = 0;
+    gf::DBuf<int32_t> conn, dofs;
+    // DOF-centric gathers, entries ascending = the reference's serial loop order:
+    //   elemvec entries k = (e*nne + m)*ndim + i with dofs(conn(e,m), i) == d
+    gf::DBuf<int32_t> dof_eptr, dof_eidx;
+    //   nodevec entries j = n*ndim + i with dofs(n, i) == d
+    gf::DBuf<int32_t> dof_nptr, dof_nidx;
+    //   node-centric: elemvec entries (e*nne + m) with conn(e,m) == n (ascending)
+    gf::DBuf<int32_t> node_eptr, node_eidx;
+    // host copies (int64) for set-up code that builds derived structures
+    std::vector<int64_t> conn_h, dofs_h;
+};

@@ -1,0 +1,278 @@
+/*
+ * goosefem_amd.h -- C ABI of the MI355X-native element-quadrature and assembly path.
+ *
+ * Plain pointers and sizes only.  Conventions:
+ *   - every function returns 0 on success or a negative GF_E* code; gf_last_error() returns a
+ *     thread-local message for the last failure (the reference throws std::runtime_error from
+ *     GOOSEFEM_ASSERT / GOOSEFEM_CHECK, include/GooseFEM/config.h:65-98);
+ *   - pointers named *_h are host memory, every other array pointer is device memory (HBM);
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream); device work is
+ *     enqueued on it and the call returns without synchronising, unless stated otherwise;
+ *   - index arrays handed in at set-up are int64 on the host (the reference uses size_t); the
+ *     library keeps int32 copies on the device;
+ *   - array layouts are the reference's row-major ones: elemvec [nelem, nne, ndim],
+ *     qtensor [nelem, nip, tdim, tdim], qscalar [nelem, nip], elemmat [nelem, nne*ndim, nne*ndim],
+ *     nodevec [nnode, ndim], dofval [ndof].
+ * Each declaration names the reference interface it replaces (paths relative to the GooseFEM
+ * repository root).
+ */
+#ifndef GOOSEFEM_AMD_H
+#define GOOSEFEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------------- */
+#define GF_OK 0
+#define GF_EINVAL -1   /* invalid argument / shape mismatch (GOOSEFEM_ASSERT) */
+#define GF_EHIP -2     /* HIP runtime error */
+#define GF_ENOMEM -3   /* device allocation failed */
+#define GF_ESTATE -4   /* object not in the required state (e.g. matrix not assembled) */
+#define GF_ENOTCONV -5 /* iterative solve reached its iteration cap */
+
+/* element types */
+#define GF_QUAD4 0        /* Element::Quad4::Quadrature, tdim = 2 */
+#define GF_QUAD4_PLANAR 1 /* Element::Quad4::QuadraturePlanar, tdim = 3 (plane strain) */
+#define GF_HEX8 2         /* Element::Hex8::Quadrature, tdim = 3 */
+
+/* integration rules */
+#define GF_GAUSS 0 /* Element::*::Gauss (ElementQuad4.h:36-93, ElementHex8.h:26-110) */
+#define GF_NODAL 1 /* Element::*::Nodal (ElementQuad4.h:103-163, ElementHex8.h:116-200) */
+
+/* arithmetic of element kernels */
+#define GF_F64 0    /* FP64 throughout, reference expression order, no FMA contraction */
+#define GF_F32 1    /* FP32 per-quadrature-point work (inputs/outputs FP32 where typed void*) */
+#define GF_BF16X3 2 /* stiffness only: BF16 hi/lo split operands on MFMA, FP32 accumulation */
+
+typedef struct gf_vector_s* gf_vector;
+typedef struct gf_quad_s* gf_quad;
+typedef struct gf_diag_s* gf_diag;
+typedef struct gf_matrix_s* gf_matrix;
+typedef struct gf_explicit_s* gf_explicit;
+typedef struct gf_pcg_s* gf_pcg;
+
+const char* gf_last_error(void);
+int gf_version(int* major, int* minor, int* patch);
+/* Synchronise a stream (host-side wait).  Convenience for FFI callers. */
+int gf_stream_sync(void* stream);
+
+/* ================================================================================================
+ * Mesh bookkeeping (host, set-up only).  Integer outputs are bit-exact with the reference.
+ * Two-call pattern for variable-size outputs: pass NULL to get the size.
+ * ============================================================================================== */
+
+/* Mesh::Quad4::Regular(nelx, nely, h).coor() / .conn()   (include/GooseFEM/MeshQuad4.h:43-122)
+ * coor_h [(nelx+1)(nely+1), 2], conn_h [nelx*nely, 4] */
+int gf_mesh_quad4_regular(int64_t nelx, int64_t nely, double h, double* coor_h, int64_t* conn_h);
+/* Mesh::Hex8::Regular(nelx, nely, nelz, h).coor() / .conn()   (include/GooseFEM/MeshHex8.h:41-131) */
+int gf_mesh_hex8_regular(int64_t nelx, int64_t nely, int64_t nelz, double h, double* coor_h,
+                         int64_t* conn_h);
+/* Named node sets of the Regular meshes, e.g. "nodesBottomEdge", "nodesTopOpenEdge",
+ * "nodesFrontFace", "nodesBackTopRightCorner", "nodesOrigin" (MeshQuad4.h:128-187,
+ * MeshHex8.h:133-597, Mesh.h:264-330).  nelz <= 0 selects the 2-D mesh.  *n receives the count. */
+int gf_mesh_nodeset(int64_t nelx, int64_t nely, int64_t nelz, const char* name, int64_t* out_h,
+                    int64_t* n);
+/* RegularBase::nodesPeriodic()  (Mesh.h:511-540 for 2-D, Mesh.h:1577-1692 for 3-D).
+ * out_h [nties, 2] (independent, dependent); *nties receives the count. */
+int gf_mesh_nodes_periodic(int64_t nelx, int64_t nely, int64_t nelz, int64_t* out_h, int64_t* nties);
+/* Mesh::dofs(nnode, ndim)  (Mesh.h:93-96) */
+int gf_mesh_dofs(int64_t nnode, int ndim, int64_t* dofs_h);
+/* RegularBase::dofsPeriodic()  (Mesh.h:291-304): dependent nodes take their independent node's
+ * dofs, then Mesh::renumber (Mesh.h:114-175). */
+int gf_mesh_dofs_periodic(int64_t nnode, int ndim, const int64_t* ties_h, int64_t nties,
+                          int64_t* dofs_h);
+/* Mesh::renumber  (Mesh.h:114-175) */
+int gf_mesh_renumber(const int64_t* in_h, int64_t n, int64_t* out_h);
+/* Mesh::coordination  (Mesh.h:2462-2477): out_h [max(conn)+1] */
+int gf_mesh_coordination(const int64_t* conn_h, int64_t n, int64_t nvalues, int64_t* out_h);
+/* Mesh::elem2node(conn[, dofs])  (Mesh.h:2485-2551) as CSR: ptr_h [nnode+1], idx_h [nnz].
+ * dofs_h may be NULL (non-periodic).  *nnz receives the entry count. */
+int gf_mesh_elem2node(const int64_t* conn_h, int64_t nelem, int nne, const int64_t* dofs_h,
+                      int64_t nnode, int ndim, int64_t* ptr_h, int64_t* idx_h, int64_t* nnz);
+/* Mesh::nodaltyings(dofs)  (Mesh.h:2839-2861) as CSR. */
+int gf_mesh_nodaltyings(const int64_t* dofs_h, int64_t nnode, int ndim, int64_t* ptr_h,
+                        int64_t* idx_h, int64_t* nnz);
+
+/* Tyings::Control(coor, dofs)  (include/GooseFEM/TyingsPeriodic.h:361-392).
+ * coor_out_h [nnode+ndim, ndim], dofs_out_h [nnode+ndim, ndim], control_dofs_h [ndim, ndim],
+ * control_nodes_h [ndim]. */
+int gf_tyings_control(const double* coor_h, const int64_t* dofs_h, int64_t nnode, int ndim,
+                      double* coor_out_h, int64_t* dofs_out_h, int64_t* control_dofs_h,
+                      int64_t* control_nodes_h);
+/* Tyings::Periodic(coor, dofs, control, nodal_tyings, iip)  (TyingsPeriodic.h:82-121):
+ * reordered dofs_out_h [nnode, ndim] ([u, p, d] numbering, Mesh::Reorder Mesh.h:2384-2454),
+ * control_out_h [ndim, ndim], sizes_h = {nnu, nnp, nni, nnd}. */
+int gf_tyings_periodic(const int64_t* dofs_h, int64_t nnode, int ndim, const int64_t* control_h,
+                       const int64_t* ties_h, int64_t nties, const int64_t* iip_h, int64_t niip,
+                       int64_t* dofs_out_h, int64_t* control_out_h, int64_t* sizes_h);
+/* Tyings::Periodic::Cdi() / Cdu() / Cdp()  (TyingsPeriodic.h:238-343) as CSR with sorted columns,
+ * duplicates summed.  which: 0 = Cdi, 1 = Cdu, 2 = Cdp.  dofs_h / control_h are the reordered
+ * ones.  rowptr_h [nties*ndim + 1]; col_h/val_h may be NULL to query *nnz. */
+int gf_tyings_matrix(int which, const double* coor_h, const int64_t* dofs_h, int ndim,
+                     const int64_t* control_h, const int64_t* ties_h, int64_t nties, int64_t nnu,
+                     int64_t* rowptr_h, int64_t* col_h, double* val_h, int64_t* nnz);
+
+/* Portable synthetic input generator (SplitMix64 -> Box-Muller, host FP64).  No reference
+ * counterpart; used to build identical inputs on every machine. */
+int gf_randn(uint64_t seed, int64_t n, double mean, double stddev, double* out_h);
+
+/* ================================================================================================
+ * Vector: storage conversions  (include/GooseFEM/Vector.h)
+ * Scatter-adds are deterministic node/DOF-centric gathers over precomputed CSR lists sorted in the
+ * reference's serial loop order, so FP64 sums are bit-identical to the serial reference loop.
+ * ============================================================================================== */
+
+/* Vector(conn, dofs)  (Vector.h:33-47); ndof = max(dofs) + 1 */
+int gf_vector_create(gf_vector* out, const int64_t* conn_h, int64_t nelem, int nne,
+                     const int64_t* dofs_h, int64_t nnode, int ndim);
+int gf_vector_destroy(gf_vector v);
+int gf_vector_sizes(gf_vector v, int64_t* nelem, int* nne, int64_t* nnode, int* ndim, int64_t* ndof);
+/* device copies of conn / dofs (int32), valid for the lifetime of v */
+int gf_vector_device_indices(gf_vector v, const int32_t** conn, const int32_t** dofs);
+/* AsElement(nodevec) / asElement(nodevec, elemvec)  (Vector.h:193-211, :680-697) */
+int gf_vector_as_element_nodevec(gf_vector v, const double* nodevec, double* elemvec, void* stream);
+/* AsElement(dofval)  (Vector.h:193-211, :658-675) */
+int gf_vector_as_element_dofval(gf_vector v, const double* dofval, double* elemvec, void* stream);
+/* AsDofs(nodevec)  (Vector.h:141-159, :568-585): overwrite, last node in loop order wins */
+int gf_vector_as_dofs_nodevec(gf_vector v, const double* nodevec, double* dofval, void* stream);
+/* AsDofs(elemvec)  (Vector.h:141-159, :590-609): overwrite, last (e, m) in loop order wins */
+int gf_vector_as_dofs_elemvec(gf_vector v, const double* elemvec, double* dofval, void* stream);
+/* AsNode(dofval)  (Vector.h:167-185, :614-629) */
+int gf_vector_as_node_dofval(gf_vector v, const double* dofval, double* nodevec, void* stream);
+/* AsNode(elemvec)  (Vector.h:167-185, :634-653) */
+int gf_vector_as_node_elemvec(gf_vector v, const double* elemvec, double* nodevec, void* stream);
+/* AssembleDofs(nodevec)  (Vector.h:219-237, :702-718) */
+int gf_vector_assemble_dofs_nodevec(gf_vector v, const double* nodevec, double* dofval, void* stream);
+/* AssembleDofs(elemvec)  (Vector.h:219-237, :723-741).  elemvec is FP64 (dtype GF_F64) or FP32
+ * (GF_F32, summed in FP64). */
+int gf_vector_assemble_dofs_elemvec(gf_vector v, const void* elemvec, int dtype, double* dofval,
+                                    void* stream);
+/* AssembleNode(elemvec)  (Vector.h:245-263, :746-757) = AsNode(AssembleDofs(elemvec)) */
+int gf_vector_assemble_node_elemvec(gf_vector v, const void* elemvec, int dtype, double* nodevec,
+                                    void* stream);
+
+/* ================================================================================================
+ * Quadrature  (include/GooseFEM/Element.h:545-868, ElementQuad4.h, ElementQuad4Planar.h,
+ * ElementHex8.h).  Geometry (GradN, dV) is computed on the device at construction.
+ * ============================================================================================== */
+
+/* Quadrature(x) / Quadrature(x, xi, w) / QuadraturePlanar(x, thick)  (ElementQuad4.h:255-318,
+ * ElementQuad4Planar.h:58-123, ElementHex8.h:233-317).  x: device elemvec of coordinates.
+ * xi_h [nip, ndim] and w_h [nip] may be NULL to select the rule `rule`.  The Hex8 geometry uses
+ * the chain rule dNdx = J^-1 dN/dxi (the generic reference loop, Element.h:1220, indexes
+ * dN/dxi(m,i) for dN/dxi(m,j); both agree to round-off on every Regular mesh). */
+int gf_quad_create(gf_quad* out, int etype, const double* x, int64_t nelem, int rule,
+                   const double* xi_h, const double* w_h, int nip, double thick, void* stream);
+int gf_quad_destroy(gf_quad q);
+int gf_quad_sizes(gf_quad q, int64_t* nelem, int* nne, int* ndim, int* tdim, int* nip);
+/* update_x(x)  (Element.h:563-569) */
+int gf_quad_update_x(gf_quad q, const double* x, void* stream);
+/* GradN() [nelem, nip, nne, ndim] and dV() [nelem, nip]  (Element.h:575-587): pointers to the
+ * internal device arrays (aliases, like the reference's const references). */
+int gf_quad_gradN(gf_quad q, const double** dNx);
+int gf_quad_dV(gf_quad q, const double** dV);
+/* InterpQuad_vector(elemvec) -> qvector [nelem, nip, n]  (Element.h:597-616) */
+int gf_quad_interp_quad_vector(gf_quad q, const double* elemvec, int n, double* qvector, void* stream);
+/* GradN_vector  (Element.h:634-652): qtensor(i,j) = dNx(m,i) u(m,j).  dtype GF_F64 or GF_F32 for
+ * both elemvec and qtensor. */
+int gf_quad_gradN_vector(gf_quad q, const void* elemvec, void* qtensor, int dtype, void* stream);
+/* GradN_vector_T  (Element.h:666-684) */
+int gf_quad_gradN_vector_T(gf_quad q, const void* elemvec, void* qtensor, int dtype, void* stream);
+/* SymGradN_vector  (Element.h:699-717) */
+int gf_quad_symgradN_vector(gf_quad q, const void* elemvec, void* qtensor, int dtype, void* stream);
+/* Int_N_vector_dV(qvector [nelem, nip, n]) -> elemvec [nelem, nne, n]  (Element.h:731-750) */
+int gf_quad_int_N_vector_dV(gf_quad q, const double* qvector, int n, double* elemvec, void* stream);
+/* Int_N_scalar_NT_dV(qscalar) -> elemmat  (Element.h:770-788) */
+int gf_quad_int_N_scalar_NT_dV(gf_quad q, const double* qscalar, double* elemmat, void* stream);
+/* Int_gradN_dot_tensor2_dV(qtensor) -> elemvec  (Element.h:807-825); dtype as above */
+int gf_quad_int_gradN_dot_tensor2_dV(gf_quad q, const void* qtensor, void* elemvec, int dtype,
+                                     void* stream);
+/* Int_gradN_dot_tensor4_dot_gradNT_dV(qtensor4 [nelem, nip, tdim^4]) -> elemmat [FP64]
+ * (Element.h:849-868).  precision GF_F64 (reference order) or GF_BF16X3 (MFMA, Hex8). */
+int gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV(gf_quad q, const double* qtensor4, double* elemmat,
+                                                int precision, void* stream);
+
+/* Material: linear elastic law of the callers (GMat* in docs/examples; not in the reference):
+ * sigma = K tr(eps) I + 2 G dev(eps), per quadrature point; K_q/G_q [nelem*nip] FP64, or FP32
+ * for dtype GF_F32 (then eps/sig are FP32 too). */
+int gf_material_elastic_stress(int64_t nq, int tdim, const void* eps, const void* K_q,
+                               const void* G_q, void* sig, int dtype, void* stream);
+/* Tangent C_ijkl = K I(x)I + 2G (I_sym - I(x)I/3), FP64 [nq, tdim^4] */
+int gf_material_elastic_tangent(int64_t nq, int tdim, const double* K_q, const double* G_q, double* C,
+                                void* stream);
+
+/* ================================================================================================
+ * MatrixDiagonal  (include/GooseFEM/MatrixDiagonal.h)
+ * ============================================================================================== */
+
+/* MatrixDiagonal(conn, dofs)  (MatrixDiagonal.h:132-147): uses v's conn/dofs */
+int gf_diag_create(gf_diag* out, gf_vector v);
+int gf_diag_destroy(gf_diag d);
+/* assemble(elemmat): sums diagonal entries only, serial order  (MatrixDiagonal.h:150-167) */
+int gf_diag_assemble(gf_diag d, const double* elemmat, void* stream);
+/* set(A)  (MatrixDiagonal.h:185-190) */
+int gf_diag_set(gf_diag d, const double* A, void* stream);
+/* data()  (MatrixDiagonal.h:206-209): device pointer to A [ndof] */
+int gf_diag_data(gf_diag d, const double** A);
+/* inverse (factorize, MatrixDiagonal.h:268-280), computed lazily; device pointer [ndof] */
+int gf_diag_inverse(gf_diag d, const double** inv, void* stream);
+/* Dot(x)  (MatrixDiagonal.h:212-233) */
+int gf_diag_dot_dofval(gf_diag d, const double* x, double* b, void* stream);
+int gf_diag_dot_nodevec(gf_diag d, const double* x, double* b, void* stream);
+/* Solve(b)  (MatrixDiagonal.h:236-259) */
+int gf_diag_solve_dofval(gf_diag d, const double* b, double* x, void* stream);
+int gf_diag_solve_nodevec(gf_diag d, const double* b, double* x, void* stream);
+
+/* ================================================================================================
+ * Matrix: sparse (CSR, sorted columns)  (include/GooseFEM/Matrix.h:653-860)
+ * Assembly is a deterministic gather: every CSR entry sums its element contributions in the
+ * reference's triplet order (ascending e), so FP64 values match the reference's duplicate sum.
+ * ============================================================================================== */
+
+/* Matrix(conn, dofs)  (Matrix.h:679-693): builds the sparsity pattern and gather lists */
+int gf_matrix_create(gf_matrix* out, gf_vector v);
+int gf_matrix_destroy(gf_matrix A);
+int gf_matrix_sizes(gf_matrix A, int64_t* nrows, int64_t* nnz);
+/* assemble(elemmat)  (Matrix.h:704-727) */
+int gf_matrix_assemble(gf_matrix A, const double* elemmat, void* stream);
+/* device CSR arrays (rowptr [nrows+1], col [nnz] int32; val [nnz] FP64) */
+int gf_matrix_csr(gf_matrix A, const int32_t** rowptr, const int32_t** col, const double** val);
+/* Dot(x)  (Matrix.h:804-815): b = A x on dofval; nodevec variant via AsDofs / AsNode */
+int gf_matrix_dot_dofval(gf_matrix A, const double* x, double* b, void* stream);
+int gf_matrix_dot_nodevec(gf_matrix A, const double* x, double* b, void* stream);
+/* Todense()  (Matrix.h:338-355, :790-802): out [nrows, nrows] */
+int gf_matrix_todense(gf_matrix A, double* out, void* stream);
+
+/* ================================================================================================
+ * Explicit dynamics: one fused velocity-Verlet step of the caller loop
+ * docs/examples/dynamics/Elastic-VelocityVerlet/main.cpp:110-157 (asElement -> symGradN_vector ->
+ * stress -> int_gradN_dot_tensor2_dV -> assembleNode -> MatrixDiagonal::solve -> updates), on a
+ * Quad4 plane-strain mesh with per-element (K, G).  State is dofval FP64 (u, v, a).  Prescribed
+ * DOFs carry inv_mass = 0: their acceleration stays 0 and velocity stays as set
+ * (MatrixDiagonalPartitioned::solve, MatrixDiagonalPartitioned.h:305-335).
+ * ============================================================================================== */
+
+/* coor: device nodevec [nnode, 2] FP64; K_e, G_e: device per-element FP32 moduli (the FP64 path
+ * widens them exactly); inv_mass: device dofval FP64; fext: device dofval or NULL (zero). */
+int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const float* K_e,
+                       const float* G_e, const double* inv_mass, const double* fext, double dt);
+int gf_explicit_destroy(gf_explicit x);
+/* state arrays (device, [ndof] each); writable between steps */
+int gf_explicit_state(gf_explicit x, double** u, double** v, double** a);
+/* run nsteps steps; precision GF_F64 (bit-exact with the serial reference order) or GF_F32
+ * (FP32 strain/stress/force, FP64 assembly and update).  store_fint != 0 keeps the internal force
+ * of the last step (one extra [ndof] write). */
+int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fint, void* stream);
+/* internal force (dofval) of the last stored step's displacement, FP64 [ndof] */
+int gf_explicit_fint(gf_explicit x, const double** fint);
+/* 1 if the structured-grid fused kernel is used (conn is a Regular Quad4 grid), else 0 */
+int gf_explicit_is_structured(gf_explicit x, int* structured);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOOSEFEM_AMD_H */

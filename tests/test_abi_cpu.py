@@ -1,0 +1,124 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol declared in include/*.h,
+and its host-side set-up logic (mesh bookkeeping, periodic tyings) is bit-exact with the
+reference's golden arrays and with the oracle."""
+
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        syms |= set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(gf_\w+)\s*\(", src, flags=re.M))
+    return sorted(syms)
+
+
+@pytest.fixture(scope="module")
+def gf():
+    import goosefem_amd
+
+    goosefem_amd.lib()
+    return goosefem_amd
+
+
+def test_library_exports_every_declared_symbol(gf):
+    syms = declared_symbols()
+    assert len(syms) > 60
+    lib = gf.lib()
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_single_hip_runtime(gf):
+    assert len(gf._hip_runtimes_mapped()) == 1
+
+
+def test_quad4_mesh_golden(gf, fixtures):
+    g = fixtures["MeshQuad4_Regular_5_3"]
+    m = gf.Mesh.Quad4.Regular(5, 3)
+    assert np.array_equal(m.coor, np.array(g["coor_"]))
+    assert np.array_equal(m.conn, np.array(g["conn_"]))
+    assert np.array_equal(m.dofs, np.array(g["dofs_"]))
+    assert np.array_equal(m.nodesPeriodic, np.array(g["nodesPeriodic_"]))
+    assert np.array_equal(m.dofsPeriodic, np.array(g["dofsPeriodic_"]))
+    for k, v in g.items():
+        if k.startswith("nodes") and k != "nodesPeriodic_":
+            got = getattr(m, k[:-1])
+            assert np.array_equal(np.atleast_1d(got), np.atleast_1d(v)), k
+
+
+def test_hex8_mesh_golden(gf, fixtures):
+    g = fixtures["MeshHex8_Regular_3_3_3"]
+    m = gf.Mesh.Hex8.Regular(3, 3, 3)
+    assert np.array_equal(m.coor, np.array(g["coor_"]))
+    assert np.array_equal(m.conn, np.array(g["conn_"]))
+    assert np.array_equal(m.dofs, np.array(g["dofs_"]))
+    assert np.array_equal(m.nodesPeriodic, np.array(g["nodesPeriodic_"]))
+    assert np.array_equal(m.dofsPeriodic, np.array(g["dofsPeriodic_"]))
+    for k, v in g.items():
+        if k.startswith("nodes") and k != "nodesPeriodic_":
+            got = getattr(m, k[:-1])
+            assert np.array_equal(np.atleast_1d(got), np.atleast_1d(v)), k
+
+
+@pytest.mark.parametrize("n", [(4, 3, 5), (7, 2, 3), (12, 12, 12)])
+def test_hex8_periodic_vs_oracle(gf, n):
+    m = gf.Mesh.Hex8.Regular(*n)
+    assert np.array_equal(m.nodesPeriodic, oracle.hex8_nodes_periodic(*n))
+    c, conn = oracle.hex8_regular(*n)
+    assert np.array_equal(m.conn, conn) and np.array_equal(m.coor, c)
+
+
+def test_elem2node_coordination_golden(gf, fixtures):
+    m = gf.Mesh.Quad4.Regular(3, 3)
+    assert np.array_equal(gf.Mesh.coordination(m.conn), fixtures["Mesh_coordination_Regular_3_3"])
+    assert gf.Mesh.elem2node(m.conn) == fixtures["Mesh_elem2node_Regular_3_3"]
+    m = gf.Mesh.Quad4.Regular(4, 4)
+    d = m.dofsPeriodic
+    assert np.array_equal(gf.Mesh.coordination(d), np.array(fixtures["test_Mesh_coordination_dofs_4_4"]).ravel())
+    assert gf.Mesh.node2dof(d) == fixtures["test_Mesh_node2dof_4_4"]
+    assert gf.Mesh.nodaltyings(d) == fixtures["test_Mesh_nodaltyings_4_4"]
+    real = np.sort(np.array(fixtures["test_Mesh_elem2node_periodic_4_4"]), axis=1)
+    assert np.array_equal(np.array(gf.Mesh.elem2node(m.conn, d)), real)
+
+
+@pytest.mark.parametrize("nx,ny", [(5, 4), (16, 9)])
+def test_tyings_vs_oracle(gf, nx, ny):
+    m = gf.Mesh.Quad4.Regular(nx, ny)
+    ctrl = gf.Tyings.Control(m.coor, m.dofs)
+    co, do, cd, cn = oracle.tyings_control(m.coor, m.dofs)
+    assert np.array_equal(ctrl.coor, co) and np.array_equal(ctrl.dofs, do)
+    assert np.array_equal(ctrl.controlDofs, cd) and np.array_equal(ctrl.controlNodes, cn)
+    iip = np.concatenate((cd.ravel(), do[m.nodesOrigin]))
+    ty = gf.Tyings.Periodic(ctrl.coor, ctrl.dofs, ctrl.controlDofs, m.nodesPeriodic, iip)
+    ref = oracle.tyings_periodic(co, do, cd, oracle.quad4_nodes_periodic(nx, ny), iip)
+    assert np.array_equal(ty.dofs, ref["dofs"]) and np.array_equal(ty.control, ref["control"])
+    assert (ty.nnu, ty.nnp, ty.nni, ty.nnd) == (ref["nnu"], ref["nnp"], ref["nni"], ref["nnd"])
+    for name in ("Cdi", "Cdu", "Cdp"):
+        a, b = getattr(ty, name + "_csr"), ref[name + "_csr"]
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), name
+
+
+def test_randn_matches_specification(gf):
+    assert np.array_equal(gf.randn(3, 1001, 0.5, 2.0), oracle.randn(3, 1001, 0.5, 2.0))
+
+
+def test_error_path_reports_message(gf):
+    n = ctypes.c_int64(0)
+    f = gf.lib().gf_mesh_nodeset
+    f.restype = ctypes.c_int
+    rc = f(ctypes.c_int64(3), ctypes.c_int64(3), ctypes.c_int64(0), b"nodesNowhere", None, ctypes.byref(n))
+    assert rc == gf.GF_EINVAL
+    assert b"unknown node set" in gf.lib().gf_last_error()

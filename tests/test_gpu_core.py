@@ -1,0 +1,280 @@
+"""GPU parity of the HIP path against the FP64 oracle (bit-exact for FP64 and integer work,
+stated tolerances for FP32 kernels).  Runs on an MI355X; every call goes through the C ABI."""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import HEX8, QUAD4, QUAD4_PLANAR
+
+pytestmark = pytest.mark.gpu
+
+# FP32 element kernels vs FP64: norm-wise relative error bound (stated tolerance)
+TOL_F32 = 2e-6
+
+
+@pytest.fixture(scope="module")
+def gf():
+    import torch
+
+    import goosefem_amd
+
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    goosefem_amd.lib()
+    return goosefem_amd
+
+
+def host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def relerr(a, b):
+    return np.linalg.norm(np.asarray(a, dtype=np.float64) - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def mesh(et, n, distort=0.0, seed=3):
+    if et == HEX8:
+        c, conn = oracle.hex8_regular(n, n, n)
+    else:
+        c, conn = oracle.quad4_regular(n + 1, n)
+    if distort:
+        c = c + distort * np.random.default_rng(seed).uniform(-1, 1, c.shape)
+    return c, conn
+
+
+QUAD_CLS = {QUAD4: "Quadrature", QUAD4_PLANAR: "QuadraturePlanar", HEX8: "Quadrature"}
+
+
+def gquad(gf, et, x, nodal=False):
+    ns = gf.Element.Hex8 if et == HEX8 else gf.Element.Quad4
+    cls = getattr(ns, QUAD_CLS[et])
+    if nodal:
+        return cls(x, ns.Nodal.xi(), ns.Nodal.w())
+    return cls(x)
+
+
+@pytest.mark.parametrize("et", [QUAD4, QUAD4_PLANAR, HEX8])
+@pytest.mark.parametrize("distort", [0.0, 0.2])
+def test_quadrature_fp64_bit_exact(gf, et, distort):
+    c, conn = mesh(et, 5, distort)
+    nd = c.shape[1]
+    vec = oracle.Vector(conn, oracle.dofs(c.shape[0], nd))
+    x = vec.AsElement(c)
+    q = oracle.Quadrature(et, x)
+    gq = gquad(gf, et, x)
+    assert np.array_equal(host(gq.GradN), q.GradN())
+    assert np.array_equal(host(gq.dV), q.dV())
+    rng = np.random.default_rng(1)
+    ue = rng.standard_normal((q.nelem, q.nne, nd))
+    for name in ("GradN_vector", "GradN_vector_T", "SymGradN_vector", "InterpQuad_vector"):
+        assert np.array_equal(host(getattr(gq, name)(ue)), getattr(q, name)(ue)), name
+    td = q.tdim
+    sig = rng.standard_normal((q.nelem, q.nip, td, td))
+    assert np.array_equal(host(gq.Int_gradN_dot_tensor2_dV(sig)), q.Int_gradN_dot_tensor2_dV(sig))
+    qv = rng.standard_normal((q.nelem, q.nip, nd))
+    assert np.array_equal(host(gq.Int_N_vector_dV(qv)), q.Int_N_vector_dV(qv))
+    rho = rng.uniform(0.5, 2.0, (q.nelem, q.nip))
+    assert np.array_equal(host(gq.Int_N_scalar_NT_dV(rho)), q.Int_N_scalar_NT_dV(rho))
+    C = rng.standard_normal((q.nelem, q.nip) + (td,) * 4)
+    assert np.array_equal(host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C)), q.Int_gradN_dot_tensor4_dot_gradNT_dV(C))
+
+
+@pytest.mark.parametrize("et", [QUAD4_PLANAR, HEX8])
+def test_quadrature_fp32_tolerance(gf, et):
+    import torch
+
+    c, conn = mesh(et, 6, 0.1)
+    nd = c.shape[1]
+    vec = oracle.Vector(conn, oracle.dofs(c.shape[0], nd))
+    q = oracle.Quadrature(et, vec.AsElement(c))
+    gq = gquad(gf, et, vec.AsElement(c))
+    rng = np.random.default_rng(2)
+    ue = rng.standard_normal((q.nelem, q.nne, nd))
+    ue32 = torch.from_numpy(ue.astype(np.float32)).cuda()
+    eps32 = gq.SymGradN_vector(ue32)
+    assert eps32.dtype == torch.float32
+    assert relerr(host(eps32), q.SymGradN_vector(ue)) < TOL_F32
+    sig = q.SymGradN_vector(ue) * 3.0
+    f32 = gq.Int_gradN_dot_tensor2_dV(torch.from_numpy(sig.astype(np.float32)).cuda())
+    assert relerr(host(f32), q.Int_gradN_dot_tensor2_dV(sig)) < TOL_F32
+
+
+def test_material_bit_exact(gf):
+    rng = np.random.default_rng(4)
+    for td in (2, 3):
+        e = rng.standard_normal((40, 4, td, td))
+        K = np.full((40, 4), 10.0)
+        G = rng.uniform(0.5, 1.5, (40, 4))
+        assert np.array_equal(host(gf.elastic_stress(e, gf.to_device(K), gf.to_device(G))),
+                              oracle.elastic_stress(e, K, G))
+    K = np.full((7, 8), 10.0)
+    G = rng.uniform(0.5, 1.5, (7, 8))
+    assert np.array_equal(host(gf.elastic_tangent(K, G)), oracle.elastic_tangent(K, G))
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("et", [QUAD4, HEX8])
+def test_vector_bit_exact(gf, et, periodic):
+    c, conn = mesh(et, 4)
+    nd = c.shape[1]
+    if periodic:
+        ties = oracle.hex8_nodes_periodic(4, 4, 4) if et == HEX8 else oracle.quad4_nodes_periodic(5, 4)
+        dofs = oracle.dofs_periodic(c.shape[0], nd, ties)
+    else:
+        dofs = oracle.dofs(c.shape[0], nd)
+    ov = oracle.Vector(conn, dofs)
+    gv = gf.Vector(conn, dofs)
+    assert gv.ndof == ov.ndof
+    rng = np.random.default_rng(7)
+    ev = rng.standard_normal(ov.conn.shape + (nd,))
+    nv = rng.standard_normal((c.shape[0], nd))
+    dv = rng.standard_normal(ov.ndof)
+    assert np.array_equal(host(gv.AsElement(nv)), ov.AsElement(nv))
+    assert np.array_equal(host(gv.AsElement(dv)), ov.AsElement(dv))
+    assert np.array_equal(host(gv.AsDofs(nv)), ov.AsDofs(nv))
+    assert np.array_equal(host(gv.AsDofs(ev)), ov.AsDofs(ev))
+    assert np.array_equal(host(gv.AsNode(dv)), ov.AsNode(dv))
+    assert np.array_equal(host(gv.AsNode(ev)), ov.AsNode(ev))
+    assert np.array_equal(host(gv.AssembleDofs(nv)), ov.AssembleDofs(nv))
+    assert np.array_equal(host(gv.AssembleDofs(ev)), ov.AssembleDofs(ev))
+    assert np.array_equal(host(gv.AssembleNode(ev)), ov.AssembleNode(ev))
+
+
+def test_vector_reference_golden(gf, fixtures):
+    # tests/basic/Vector.cpp:11-167 on the GPU path
+    g = fixtures["Vector_periodic_Regular_2_2"]
+    m = gf.Mesh.Quad4.Regular(2, 2)
+    vec = gf.Vector(m.conn, m.dofsPeriodic)
+    v = np.array(g["asDofs - nodevec"]["input"])
+    for V in (host(vec.AsDofs(v)), host(vec.AsDofs(vec.AsElement(vec.AsDofs(v))))):
+        for k, (n, i) in g["asDofs - nodevec"]["expected"]:
+            assert V[k] == v[n, i]
+    f = np.array(g["asDofs - assembleDofs"]["input"])
+    for F in (host(vec.AssembleDofs(f)), host(vec.AssembleDofs(vec.AsElement(f)))):
+        assert np.allclose(F, 0.0, atol=1e-12)
+
+
+def test_lumped_mass_golden_and_diag(gf, fixtures):
+    m = gf.Mesh.Quad4.Regular(2, 2)
+    vec = gf.Vector(m.conn, m.dofs)
+    x = vec.AsElement(m.coor)
+    q = gquad(gf, QUAD4, x, nodal=True)
+    M = gf.MatrixDiagonal(vec, None)
+    M.assemble(q.Int_N_scalar_NT_dV(np.ones((m.nelem, 4))))
+    assert np.allclose(host(M.data), fixtures["MatrixDiagonal_lumped_mass_Regular_2_2"], atol=1e-12)
+    # bit-exact vs oracle on a larger periodic Hex8 mesh; Dot / Solve round trip
+    c, conn = oracle.hex8_regular(3, 3, 3)
+    dp = oracle.dofs_periodic(c.shape[0], 3, oracle.hex8_nodes_periodic(3, 3, 3))
+    ov = oracle.Vector(conn, oracle.dofs(c.shape[0], 3))
+    xi, w = oracle.quad_rule(HEX8, True)
+    oq = oracle.Quadrature(HEX8, ov.AsElement(c), xi, w)
+    mm = oq.Int_N_scalar_NT_dV(np.ones((oq.nelem, 8)))
+    A = oracle.diag_assemble(conn, dp, mm)
+    gM = gf.MatrixDiagonal(conn, dp)
+    gM.assemble(mm)
+    assert np.array_equal(host(gM.data), A)
+    assert np.allclose(A, 1.0)
+    rng = np.random.default_rng(0)
+    xv = rng.random(A.size)
+    assert np.array_equal(host(gM.Dot(xv)), A * xv)
+    assert np.array_equal(host(gM.Solve(A * xv)), oracle.diag_inverse(A) * (A * xv))
+
+
+@pytest.mark.parametrize("et,periodic", [(QUAD4, False), (QUAD4, True), (HEX8, False), (HEX8, True)])
+def test_matrix_csr_bit_exact(gf, et, periodic):
+    c, conn = mesh(et, 3)
+    nd = c.shape[1]
+    if periodic:
+        ties = oracle.hex8_nodes_periodic(3, 3, 3) if et == HEX8 else oracle.quad4_nodes_periodic(4, 3)
+        dofs = oracle.dofs_periodic(c.shape[0], nd, ties)
+    else:
+        dofs = oracle.dofs(c.shape[0], nd)
+    rng = np.random.default_rng(11)
+    K = conn.shape[1] * nd
+    a = rng.standard_normal((conn.shape[0], K, K))
+    rowptr, col, val = oracle.matrix_assemble(conn, dofs, a)
+    A = gf.Matrix(conn, dofs)
+    A.assemble(a)
+    grp, gcol, gval = (host(t) for t in A.csr())
+    assert np.array_equal(grp, rowptr) and np.array_equal(gcol, col)
+    assert np.array_equal(gval, val)
+    x = rng.standard_normal(rowptr.size - 1)
+    assert np.array_equal(host(A.Dot(x)), oracle.csr_dot(rowptr, col, val, x))
+    dense = np.zeros((rowptr.size - 1,) * 2)
+    for r in range(rowptr.size - 1):
+        dense[r, col[rowptr[r]:rowptr[r + 1]]] = val[rowptr[r]:rowptr[r + 1]]
+    assert np.array_equal(host(A.Todense()), dense)
+
+
+def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True):
+    coor, conn = oracle.quad4_regular(nx, ny)
+    if shuffle:  # same mesh, elements renumbered: defeats the structured-grid detection
+        perm = np.random.default_rng(5).permutation(conn.shape[0])
+        conn = conn[perm]
+    dofs = oracle.dofs(coor.shape[0], 2)
+    vec = oracle.Vector(conn, dofs)
+    x = vec.AsElement(coor)
+    q = oracle.Quadrature(QUAD4_PLANAR, x)
+    xi, w = oracle.quad_rule(QUAD4_PLANAR, True)
+    qn = oracle.Quadrature(QUAD4_PLANAR, x, xi, w)
+    M = oracle.diag_assemble(conn, dofs, qn.Int_N_scalar_NT_dV(np.ones((q.nelem, 4))))
+    inv = oracle.diag_inverse(M)
+    if fixed:
+        bot = oracle.quad4_nodeset(nx, ny, "nodesBottomEdge")
+        top = oracle.quad4_nodeset(nx, ny, "nodesTopEdge")
+        inv[dofs[bot].ravel()] = 0.0
+        inv[dofs[top].ravel()] = 0.0
+    Ke = np.full(q.nelem, 10.0)
+    Ge = np.exp(0.3 * oracle.randn(seed + 100, q.nelem)).astype(np.float32).astype(np.float64)
+    u = oracle.randn(seed, dofs.size, 0.0, 1e-3)
+    v = np.zeros_like(u)
+    if fixed:
+        v[dofs[top, 0]] = 1e-4
+        u[dofs[bot].ravel()] = 0.0
+        u[dofs[top].ravel()] = 0.0
+    a = np.zeros_like(u)
+    return coor, conn, dofs, q, inv, Ke, Ge, u, v, a
+
+
+@pytest.mark.parametrize("nx,ny,shuffle", [(64, 64, False), (130, 17, False), (3, 5, False), (24, 20, True)])
+def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle):
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle)
+    vec = gf.Vector(conn, dofs)
+    X = gf.ExplicitQuad4(vec, coor, Ke, Ge, inv, 0.01)
+    assert X.structured == (not shuffle)
+    X.set_state(u, v, a)
+    for nsteps in (1, 9):
+        fint = oracle.vv_steps(nsteps, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a)
+        X.step(nsteps, gf.F64, store_fint=True)
+        gu, gv_, ga = (host(t) for t in X.state())
+        assert np.array_equal(gu, u) and np.array_equal(gv_, v) and np.array_equal(ga, a)
+        assert np.array_equal(host(X.fint()), fint)
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_explicit_fp32_tolerance(gf, shuffle):
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(64, 48, shuffle=shuffle)
+    vec = gf.Vector(conn, dofs)
+    X = gf.ExplicitQuad4(vec, coor, Ke, Ge, inv, 0.01)
+    X.set_state(u, v, a)
+    fint = oracle.vv_steps(1, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a)
+    X.step(1, gf.F32, store_fint=True)
+    assert relerr(host(X.fint()), fint) < TOL_F32
+    gu, gv_, ga = (host(t) for t in X.state())
+    assert np.array_equal(gu, u)  # displacement update is FP64 in both
+    assert relerr(ga, a) < TOL_F32
+
+
+def test_explicit_rejects_bad_input(gf):
+    import ctypes
+
+    m = gf.Mesh.Hex8.Regular(2, 2, 2)
+    vec = gf.Vector(m.conn, m.dofs)
+    with pytest.raises(gf.GooseFEMError):
+        gf.ExplicitQuad4(vec, np.zeros((27, 3)), np.ones(8), np.ones(8), np.ones(81), 0.01)
+    f = gf.lib().gf_quad_int_gradN_dot_tensor2_dV
+    f.restype = ctypes.c_int
+    assert f(None, None, None, ctypes.c_int(0), None) == gf.GF_EINVAL
