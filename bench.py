@@ -116,12 +116,26 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (PMC counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the path shards by mesh slabs "
                                    "(halo exchange of one node row) but no multi-GPU run is provided"}))
         sys.exit(2)
     import torch
+
+    if args.copy_calibration:
+        # plain device-to-device copy of a known size: its FETCH_SIZE / WRITE_SIZE counters give the
+        # scale between counter units and bytes
+        n = (1 << 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        return
 
     nx = ny = args.nx
     S = setup(nx, ny)

@@ -21,6 +21,8 @@
 // element kernel (gather + strain + stress + force, FP64 or FP32), DOF-centric ordered gather +
 // update.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <type_traits>
 
 #include "gf_internal.hpp"
@@ -40,6 +42,8 @@ struct gf_explicit_s {
     int64_t nx = 0, ny = 0; // structured grid size (elements)
     DBuf<double> coor;      // nodevec [nnode, 2]
     DBuf<float> Ke, Ge;     // per element
+    bool k_uniform = false; // all K_e equal: pass the scalar, skip the per-element K stream
+    float k_value = 0.0f;
     DBuf<double> inv_mass, fext;
     bool has_fext = false;
     DBuf<double> u[2], vv[2], a[2];
@@ -53,6 +57,21 @@ struct gf_explicit_s {
 };
 
 namespace {
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+// streaming (non-temporal) 16-byte load / store of a node's two DOFs
+__device__ inline double2 nt_load2(const double* p, int64_t n)
+{
+    dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p) + n);
+    return make_double2(v.x, v.y);
+}
+
+__device__ inline void nt_store2(double* p, int64_t n, double2 v)
+{
+    dvec2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<dvec2*>(p) + n);
+}
 
 struct ShapeTable {
     double g[4][4][2]; // dN/dxi at the 4 Gauss points [q][m][i]
@@ -104,66 +123,84 @@ __device__ inline void element_force_f64(const ShapeTable& T, const double x[4][
     }
 }
 
-// ---- per-element force, FP32 (differences taken in FP64 first, so no cancellation) --------------
+// ---- per-element force, FP32 --------------------------------------------------------------------
+// Same 2x2 Gauss quadrature of the bilinear element, evaluated through the bilinear structure of
+// the shape functions instead of per-node loops.  With node signs (s_m, t_m) = (-1,-1), (1,-1),
+// (1,1), (-1,1):  dN_m/dxi = s_m (1 + t_m eta) / 4,  dN_m/deta = t_m (1 + s_m xi) / 4, so for any
+// nodal field w:  sum_m dN_m/dxi w_m = (A_w + eta H_w) / 4,  sum_m dN_m/deta w_m = (B_w + xi H_w) / 4
+// with A_w = -w0 + w1 + w2 - w3, B_w = -w0 - w1 + w2 + w3, H_w = w0 - w1 + w2 - w3.  The nodal force
+// f(m,j) = sum_q dV_q sum_k dN_m/dxi_k (J^-T sigma)_kj collapses to three per-element sums.
+// Differences to node 0 are taken in FP64 before rounding to FP32 (no cancellation of large
+// coordinates or rigid displacements).  The Gauss weights are 1 (w[] kept for generality).
 __device__ inline void element_force_f32(const ShapeTable& T, const double x[4][2], const double u[4][2], float k,
                                          float gmod, float f[4][2])
 {
     float xr[4][2], ur[4][2];
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int i = 0; i < 2; ++i) {
+        xr[0][i] = 0.0f;
+        ur[0][i] = 0.0f;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int m = 1; m < 4; ++m) {
             xr[m][i] = static_cast<float>(x[m][i] - x[0][i]);
             ur[m][i] = static_cast<float>(u[m][i] - u[0][i]);
         }
+    }
+    float Ax[2], Bx[2], Hx[2], Au[2], Bu[2], Hu[2];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) f[m][0] = f[m][1] = 0.0f;
+    for (int i = 0; i < 2; ++i) {
+        Ax[i] = 0.25f * (xr[1][i] + xr[2][i] - xr[3][i]);
+        Bx[i] = 0.25f * (xr[2][i] + xr[3][i] - xr[1][i]);
+        Hx[i] = 0.25f * (xr[2][i] - xr[1][i] - xr[3][i]);
+        Au[i] = 0.25f * (ur[1][i] + ur[2][i] - ur[3][i]);
+        Bu[i] = 0.25f * (ur[2][i] + ur[3][i] - ur[1][i]);
+        Hu[i] = 0.25f * (ur[2][i] - ur[1][i] - ur[3][i]);
+    }
+    const float g2 = 2.0f * gmod;
+    // accumulated P_q = dV_q J^-T sigma: S0 = sum P(0,.), S1 = sum P(1,.), SH = sum (eta P(0,.) + xi P(1,.))
+    float S0[2] = {0.f, 0.f}, S1[2] = {0.f, 0.f}, SH[2] = {0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        float g[4][2];
+        // Gauss points (-a,-a), (a,-a), (a,a), (-a,a) with a = 1/sqrt(3)
+        const float a = 0.57735026918962576f;
+        const float xi = (q == 1 || q == 2) ? a : -a;
+        const float eta = (q >= 2) ? a : -a;
+        // J(i,j) = dx_j / dxi_i
+        const float J00 = fmaf(eta, Hx[0], Ax[0]), J01 = fmaf(eta, Hx[1], Ax[1]);
+        const float J10 = fmaf(xi, Hx[0], Bx[0]), J11 = fmaf(xi, Hx[1], Bx[1]);
+        const float det = fmaf(J00, J11, -J01 * J10);
+        const float rdet = __builtin_amdgcn_rcpf(det);
+        const float I00 = J11 * rdet, I01 = -J01 * rdet, I10 = -J10 * rdet, I11 = J00 * rdet;
+        // du_j / dxi_k
+        const float d0x = fmaf(eta, Hu[0], Au[0]), d0y = fmaf(eta, Hu[1], Au[1]);
+        const float d1x = fmaf(xi, Hu[0], Bu[0]), d1y = fmaf(xi, Hu[1], Bu[1]);
+        // grad(i,j) = sum_k Jinv(i,k) du_j/dxi_k
+        const float gxx = fmaf(I00, d0x, I01 * d1x), gxy = fmaf(I00, d0y, I01 * d1y);
+        const float gyx = fmaf(I10, d0x, I11 * d1x), gyy = fmaf(I10, d0y, I11 * d1y);
+        const float tr = gxx + gyy;
+        const float kt = k * tr, t3 = tr * (1.0f / 3.0f);
+        const float vol = static_cast<float>(T.w[q]) * det;
+        const float sxx = fmaf(g2, gxx - t3, kt) * vol;
+        const float syy = fmaf(g2, gyy - t3, kt) * vol;
+        const float sxy = gmod * (gxy + gyx) * vol;
+        // P(k,j) = sum_i Jinv(i,k) sigma(i,j) dV
+        const float P0x = fmaf(I00, sxx, I10 * sxy), P0y = fmaf(I00, sxy, I10 * syy);
+        const float P1x = fmaf(I01, sxx, I11 * sxy), P1y = fmaf(I01, sxy, I11 * syy);
+        S0[0] += P0x;
+        S0[1] += P0y;
+        S1[0] += P1x;
+        S1[1] += P1y;
+        SH[0] = fmaf(eta, P0x, fmaf(xi, P1x, SH[0]));
+        SH[1] = fmaf(eta, P0y, fmaf(xi, P1y, SH[1]));
+    }
+    // f(m,j) = (s_m S0_j + t_m S1_j + s_m t_m SH_j) / 4
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            g[m][0] = static_cast<float>(T.g[q][m][0]);
-            g[m][1] = static_cast<float>(T.g[q][m][1]);
-        }
-        float J00 = 0.f, J01 = 0.f, J10 = 0.f, J11 = 0.f;
-#pragma unroll
-        for (int m = 1; m < 4; ++m) {
-            J00 = fmaf(g[m][0], xr[m][0], J00);
-            J01 = fmaf(g[m][0], xr[m][1], J01);
-            J10 = fmaf(g[m][1], xr[m][0], J10);
-            J11 = fmaf(g[m][1], xr[m][1], J11);
-        }
-        float det = J00 * J11 - J01 * J10;
-        float rdet = 1.0f / det;
-        float I00 = J11 * rdet, I01 = -J01 * rdet, I10 = -J10 * rdet, I11 = J00 * rdet;
-        float vol = static_cast<float>(T.w[q]) * det;
-        float du00 = 0.f, du01 = 0.f, du10 = 0.f, du11 = 0.f; // du(i,j) = dNx(m,i) u(m,j)
-        float d[4][2];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            d[m][0] = fmaf(I00, g[m][0], I01 * g[m][1]);
-            d[m][1] = fmaf(I10, g[m][0], I11 * g[m][1]);
-        }
-#pragma unroll
-        for (int m = 1; m < 4; ++m) {
-            du00 = fmaf(d[m][0], ur[m][0], du00);
-            du01 = fmaf(d[m][0], ur[m][1], du01);
-            du10 = fmaf(d[m][1], ur[m][0], du10);
-            du11 = fmaf(d[m][1], ur[m][1], du11);
-        }
-        float e01 = 0.5f * (du01 + du10);
-        float tr = du00 + du11;
-        float kt = k * tr, t3 = tr * (1.0f / 3.0f), g2 = 2.0f * gmod;
-        float s00 = fmaf(g2, du00 - t3, kt);
-        float s11 = fmaf(g2, du11 - t3, kt);
-        float s01 = g2 * e01;
-        float sv00 = s00 * vol, sv11 = s11 * vol, sv01 = s01 * vol;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            f[m][0] += fmaf(d[m][0], sv00, d[m][1] * sv01);
-            f[m][1] += fmaf(d[m][0], sv01, d[m][1] * sv11);
-        }
+    for (int j = 0; j < 2; ++j) {
+        const float p = 0.25f * S0[j], r = 0.25f * S1[j], h = 0.25f * SH[j];
+        f[0][j] = -p - r + h;
+        f[1][j] = p - r - h;
+        f[2][j] = p + r + h;
+        f[3][j] = -p + r - h;
     }
 }
 
@@ -171,7 +208,7 @@ __device__ inline void element_force_f32(const ShapeTable& T, const double x[4][
 template <int TX, int TY, bool F32>
 __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     int nx, int ny, int ntx, int nty, int per_xcd, ShapeTable T, const double* __restrict__ coor,
-    const float* __restrict__ Ke, const float* __restrict__ Ge, const double* __restrict__ inv_mass,
+    const float* __restrict__ Ke, float Kconst, const float* __restrict__ Ge, const double* __restrict__ inv_mass,
     const double* __restrict__ fext, double dt, double c, double hdt, const double* __restrict__ u0,
     const double* __restrict__ v0, const double* __restrict__ a0, double* __restrict__ u1, double* __restrict__ v1,
     double* __restrict__ a1, double* __restrict__ fint_out)
@@ -179,6 +216,8 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     constexpr int HX = TX + 2, HY = TY + 2;   // halo node tile
     constexpr int EX = TX + 1, EY = TY + 1;   // element tile
     constexpr int NT = TX * TY;
+    constexpr int NA = (HX * HY + NT - 1) / NT; // halo nodes per thread
+    constexpr int NB = (EX * EY + NT - 1) / NT; // elements per thread
     using FT = typename std::conditional<F32, float, double>::type;
     __shared__ double2 s_u[HY * HX];
     __shared__ double2 s_x[HY * HX];
@@ -192,26 +231,57 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     const int x0 = tx * TX, y0 = ty * TY;
     const int NX = nx + 1, NY = ny + 1;
 
-    // phase A: updated displacement and coordinates of the tile + halo
-    for (int t = threadIdx.x; t < HX * HY; t += NT) {
-        const int ly = t / HX, lx = t - ly * HX;
-        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-        if (gx >= 0 && gx < NX && gy >= 0 && gy < NY) {
-            const int64_t n = static_cast<int64_t>(gy) * NX + gx;
-            const double2 uu = reinterpret_cast<const double2*>(u0)[n];
-            const double2 vv = reinterpret_cast<const double2*>(v0)[n];
-            const double2 aa = reinterpret_cast<const double2*>(a0)[n];
-            double2 un;
-            un.x = uu.x + dt * vv.x + c * aa.x;
-            un.y = uu.y + dt * vv.y + c * aa.y;
-            s_u[t] = un;
-            s_x[t] = reinterpret_cast<const double2*>(coor)[n];
+    // owned node of this thread; its update operands are fetched first so they are in flight
+    // while the tile is staged and the element forces are computed
+    const int oly = threadIdx.x / TX, olx = threadIdx.x - (threadIdx.x / TX) * TX;
+    const int ogx = x0 + olx, ogy = y0 + oly;
+    const bool owner = ogx < NX && ogy < NY;
+    const int64_t on = static_cast<int64_t>(ogy) * NX + ogx;
+    double2 o_v = make_double2(0.0, 0.0), o_a = o_v, o_im = o_v, o_fe = o_v;
+    if (owner) {
+        o_v = reinterpret_cast<const double2*>(v0)[on];
+        o_a = reinterpret_cast<const double2*>(a0)[on];
+        o_im = nt_load2(inv_mass, on);
+        if (fext) o_fe = nt_load2(fext, on);
+    }
+
+    // phase A: updated displacement and coordinates of the tile + halo (all loads issued first)
+    {
+        double2 ru[NA], rv[NA], ra[NA], rx[NA];
+        bool ok[NA];
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            const int t = threadIdx.x + k * NT;
+            const int ly = t / HX, lx = t - ly * HX;
+            const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+            ok[k] = t < HX * HY && gx >= 0 && gx < NX && gy >= 0 && gy < NY;
+            if (ok[k]) {
+                const int64_t n = static_cast<int64_t>(gy) * NX + gx;
+                ru[k] = reinterpret_cast<const double2*>(u0)[n];
+                rv[k] = reinterpret_cast<const double2*>(v0)[n];
+                ra[k] = reinterpret_cast<const double2*>(a0)[n];
+                rx[k] = reinterpret_cast<const double2*>(coor)[n];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            if (ok[k]) {
+                const int t = threadIdx.x + k * NT;
+                double2 un;
+                un.x = ru[k].x + dt * rv[k].x + c * ra[k].x;
+                un.y = ru[k].y + dt * rv[k].y + c * ra[k].y;
+                s_u[t] = un;
+                s_x[t] = rx[k];
+            }
         }
     }
     __syncthreads();
 
     // phase B: element forces of the element tile
-    for (int t = threadIdx.x; t < EX * EY; t += NT) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int t = threadIdx.x + k * NT;
+        if (t >= EX * EY) break;
         const int ly = t / EX, lx = t - ly * EX;
         const int ex = x0 - 1 + lx, ey = y0 - 1 + ly;
         if (ex >= 0 && ex < nx && ey >= 0 && ey < ny) {
@@ -230,7 +300,7 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
             }
             if constexpr (F32) {
                 float f[4][2];
-                element_force_f32(T, x, u, Ke[e], Ge[e], f);
+                element_force_f32(T, x, u, Ke ? Ke[e] : Kconst, Ge[e], f);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     s_f[t][2 * m] = f[m][0];
@@ -239,7 +309,7 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
             }
             else {
                 double f[4][2];
-                element_force_f64(T, x, u, static_cast<double>(Ke[e]), static_cast<double>(Ge[e]), f);
+                element_force_f64(T, x, u, static_cast<double>(Ke ? Ke[e] : Kconst), static_cast<double>(Ge[e]), f);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     s_f[t][2 * m] = f[m][0];
@@ -250,52 +320,43 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     }
     __syncthreads();
 
-    // phase C: owned nodes -- ordered assembly + velocity-Verlet update
-    {
-        const int ly = threadIdx.x / TX, lx = threadIdx.x - (threadIdx.x / TX) * TX;
-        const int gx = x0 + lx, gy = y0 + ly;
-        if (gx < NX && gy < NY) {
-            const int64_t n = static_cast<int64_t>(gy) * NX + gx;
-            // elements in ascending number: (gx-1,gy-1) slot 2, (gx,gy-1) slot 3, (gx-1,gy) slot 1,
-            // (gx,gy) slot 0; local element (lx', ly') = (ex - x0 + 1, ey - y0 + 1)
-            double f0 = 0.0, f1 = 0.0;
-            const bool hasL = gx > 0, hasR = gx < nx, hasB = gy > 0, hasT = gy < ny;
-            if (hasB && hasL) {
-                const int le = ly * EX + lx;
-                f0 += static_cast<double>(s_f[le][4]);
-                f1 += static_cast<double>(s_f[le][5]);
-            }
-            if (hasB && hasR) {
-                const int le = ly * EX + lx + 1;
-                f0 += static_cast<double>(s_f[le][6]);
-                f1 += static_cast<double>(s_f[le][7]);
-            }
-            if (hasT && hasL) {
-                const int le = (ly + 1) * EX + lx;
-                f0 += static_cast<double>(s_f[le][2]);
-                f1 += static_cast<double>(s_f[le][3]);
-            }
-            if (hasT && hasR) {
-                const int le = (ly + 1) * EX + lx + 1;
-                f0 += static_cast<double>(s_f[le][0]);
-                f1 += static_cast<double>(s_f[le][1]);
-            }
-            const double2 un = s_u[(ly + 1) * HX + lx + 1];
-            const double2 vv = reinterpret_cast<const double2*>(v0)[n];
-            const double2 aa = reinterpret_cast<const double2*>(a0)[n];
-            const double2 im = reinterpret_cast<const double2*>(inv_mass)[n];
-            double2 fe = make_double2(0.0, 0.0);
-            if (fext) fe = reinterpret_cast<const double2*>(fext)[n];
-            double2 an, vn;
-            an.x = im.x * (fe.x - f0);
-            an.y = im.y * (fe.y - f1);
-            vn.x = vv.x + hdt * (aa.x + an.x);
-            vn.y = vv.y + hdt * (aa.y + an.y);
-            reinterpret_cast<double2*>(u1)[n] = un;
-            reinterpret_cast<double2*>(v1)[n] = vn;
-            reinterpret_cast<double2*>(a1)[n] = an;
-            if (fint_out) reinterpret_cast<double2*>(fint_out)[n] = make_double2(f0, f1);
+    // phase C: owned node -- ordered assembly + velocity-Verlet update
+    if (owner) {
+        const int lx = olx, ly = oly, gx = ogx, gy = ogy;
+        // elements in ascending number: (gx-1,gy-1) slot 2, (gx,gy-1) slot 3, (gx-1,gy) slot 1,
+        // (gx,gy) slot 0; local element (lx', ly') = (ex - x0 + 1, ey - y0 + 1)
+        double f0 = 0.0, f1 = 0.0;
+        const bool hasL = gx > 0, hasR = gx < nx, hasB = gy > 0, hasT = gy < ny;
+        if (hasB && hasL) {
+            const int le = ly * EX + lx;
+            f0 += static_cast<double>(s_f[le][4]);
+            f1 += static_cast<double>(s_f[le][5]);
         }
+        if (hasB && hasR) {
+            const int le = ly * EX + lx + 1;
+            f0 += static_cast<double>(s_f[le][6]);
+            f1 += static_cast<double>(s_f[le][7]);
+        }
+        if (hasT && hasL) {
+            const int le = (ly + 1) * EX + lx;
+            f0 += static_cast<double>(s_f[le][2]);
+            f1 += static_cast<double>(s_f[le][3]);
+        }
+        if (hasT && hasR) {
+            const int le = (ly + 1) * EX + lx + 1;
+            f0 += static_cast<double>(s_f[le][0]);
+            f1 += static_cast<double>(s_f[le][1]);
+        }
+        const double2 un = s_u[(ly + 1) * HX + lx + 1];
+        double2 an, vn;
+        an.x = o_im.x * (o_fe.x - f0);
+        an.y = o_im.y * (o_fe.y - f1);
+        vn.x = o_v.x + hdt * (o_a.x + an.x);
+        vn.y = o_v.y + hdt * (o_a.y + an.y);
+        nt_store2(u1, on, un);
+        nt_store2(v1, on, vn);
+        nt_store2(a1, on, an);
+        if (fint_out) reinterpret_cast<double2*>(fint_out)[on] = make_double2(f0, f1);
     }
 }
 
@@ -448,7 +509,41 @@ bool detect_structured(const gf_vector_s* v, int64_t& nx, int64_t& ny)
     return nx < (1 << 30) && ny < (1 << 30);
 }
 
-constexpr int TX = 64, TY = 8;
+// Tile shape (nodes per workgroup).  Default from a sweep on MI355X; GF_EXPLICIT_TILE=WxH
+// ("64x8", "32x8", "32x16", "64x4", "16x16") selects another instantiation.
+struct TileCfg {
+    int tx, ty;
+};
+
+TileCfg tile_config()
+{
+    static TileCfg cfg = [] {
+        TileCfg c{32, 16};
+        if (const char* e = std::getenv("GF_EXPLICIT_TILE")) {
+            int a = 0, b = 0;
+            if (std::sscanf(e, "%dx%d", &a, &b) == 2) c = TileCfg{a, b};
+        }
+        return c;
+    }();
+    return cfg;
+}
+
+template <int TX, int TY>
+void launch_structured(bool f32, unsigned grid, hipStream_t s, int nx, int ny, int ntx, int nty, int per_xcd,
+                       const ShapeTable& T, gf_explicit_s* x, const double* fext, double dt, double c, double hdt,
+                       int i0, int i1, double* fo)
+{
+    if (f32)
+        k_vv_structured<TX, TY, true><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T, x->coor.p,
+                                                               x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
+                                                               x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
+                                                               x->a[i1].p, fo);
+    else
+        k_vv_structured<TX, TY, false><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T, x->coor.p,
+                                                                x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
+                                                                x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
+                                                                x->a[i1].p, fo);
+}
 
 } // namespace
 
@@ -471,6 +566,12 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
             x->Ke.alloc(v->nelem);
             x->Ge.alloc(v->nelem);
             GF_HIP(hipMemcpy(x->Ke.p, K_e, v->nelem * sizeof(float), hipMemcpyDeviceToDevice));
+            {
+                std::vector<float> kh(v->nelem);
+                if (v->nelem) GF_HIP(hipMemcpy(kh.data(), K_e, v->nelem * sizeof(float), hipMemcpyDeviceToHost));
+                x->k_uniform = v->nelem > 0 && std::all_of(kh.begin(), kh.end(), [&](float z) { return z == kh[0]; });
+                x->k_value = v->nelem ? kh[0] : 0.0f;
+            }
             GF_HIP(hipMemcpy(x->Ge.p, G_e, v->nelem * sizeof(float), hipMemcpyDeviceToDevice));
             x->inv_mass.alloc(v->ndof);
             GF_HIP(hipMemcpy(x->inv_mass.p, inv_mass, v->ndof * sizeof(double), hipMemcpyDeviceToDevice));
@@ -557,21 +658,27 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
         const bool f32 = precision == GF_F32;
         if (x->structured) {
             static const ShapeTable T = gauss_table();
+            const TileCfg tc = tile_config();
+            const int TXr = tc.tx, TYr = tc.ty;
             const int nx = static_cast<int>(x->nx), ny = static_cast<int>(x->ny);
-            const int ntx = (nx + 1 + TX - 1) / TX, nty = (ny + 1 + TY - 1) / TY;
+            const int ntx = (nx + 1 + TXr - 1) / TXr, nty = (ny + 1 + TYr - 1) / TYr;
             const int ntiles = ntx * nty, per_xcd = (ntiles + 7) / 8;
             const unsigned grid = static_cast<unsigned>(per_xcd * 8);
             for (int64_t k = 0; k < nsteps; ++k) {
                 const int i0 = x->cur, i1 = 1 - x->cur;
                 double* fo = (store_fint && k == nsteps - 1) ? x->fint.p : nullptr;
-                if (f32)
-                    k_vv_structured<TX, TY, true><<<grid, TX * TY, 0, s>>>(
-                        nx, ny, ntx, nty, per_xcd, T, x->coor.p, x->Ke.p, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt,
-                        x->u[i0].p, x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p, x->a[i1].p, fo);
+                if (TXr == 32 && TYr == 8)
+                    launch_structured<32, 8>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                else if (TXr == 64 && TYr == 8)
+                    launch_structured<64, 8>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                else if (TXr == 32 && TYr == 16)
+                    launch_structured<32, 16>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                else if (TXr == 64 && TYr == 4)
+                    launch_structured<64, 4>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                else if (TXr == 16 && TYr == 16)
+                    launch_structured<16, 16>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
                 else
-                    k_vv_structured<TX, TY, false><<<grid, TX * TY, 0, s>>>(
-                        nx, ny, ntx, nty, per_xcd, T, x->coor.p, x->Ke.p, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt,
-                        x->u[i0].p, x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p, x->a[i1].p, fo);
+                    throw Error(GF_EINVAL, "GF_EXPLICIT_TILE must be one of 32x8, 64x8, 32x16, 64x4, 16x16");
                 check_launch();
                 x->cur = i1;
             }

@@ -209,7 +209,7 @@ def test_matrix_csr_bit_exact(gf, et, periodic):
     assert np.array_equal(host(A.Todense()), dense)
 
 
-def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True):
+def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False):
     coor, conn = oracle.quad4_regular(nx, ny)
     if shuffle:  # same mesh, elements renumbered: defeats the structured-grid detection
         perm = np.random.default_rng(5).permutation(conn.shape[0])
@@ -228,6 +228,8 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True):
         inv[dofs[bot].ravel()] = 0.0
         inv[dofs[top].ravel()] = 0.0
     Ke = np.full(q.nelem, 10.0)
+    if kvar:
+        Ke = (10.0 + oracle.randn(seed + 200, q.nelem)).astype(np.float32).astype(np.float64)
     Ge = np.exp(0.3 * oracle.randn(seed + 100, q.nelem)).astype(np.float32).astype(np.float64)
     u = oracle.randn(seed, dofs.size, 0.0, 1e-3)
     v = np.zeros_like(u)
@@ -239,9 +241,10 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True):
     return coor, conn, dofs, q, inv, Ke, Ge, u, v, a
 
 
-@pytest.mark.parametrize("nx,ny,shuffle", [(64, 64, False), (130, 17, False), (3, 5, False), (24, 20, True)])
-def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle):
-    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle)
+@pytest.mark.parametrize("nx,ny,shuffle,kvar", [(64, 64, False, False), (130, 17, False, True), (3, 5, False, False),
+                                               (24, 20, True, True)])
+def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar):
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle, kvar=kvar)
     vec = gf.Vector(conn, dofs)
     X = gf.ExplicitQuad4(vec, coor, Ke, Ge, inv, 0.01)
     assert X.structured == (not shuffle)
@@ -254,9 +257,9 @@ def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle):
         assert np.array_equal(host(X.fint()), fint)
 
 
-@pytest.mark.parametrize("shuffle", [False, True])
-def test_explicit_fp32_tolerance(gf, shuffle):
-    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(64, 48, shuffle=shuffle)
+@pytest.mark.parametrize("shuffle,kvar", [(False, False), (False, True), (True, False)])
+def test_explicit_fp32_tolerance(gf, shuffle, kvar):
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(64, 48, shuffle=shuffle, kvar=kvar)
     vec = gf.Vector(conn, dofs)
     X = gf.ExplicitQuad4(vec, coor, Ke, Ge, inv, 0.01)
     X.set_state(u, v, a)
