@@ -665,6 +665,16 @@ Element = SimpleNamespace(
 )
 
 
+def elemmat_dot(elemmat, elemvec):
+    """out[e] = elemmat[e] . elemvec[e] for every element (the element part of sum_e K_e u_e)."""
+    Ke = to_device(elemmat)
+    ue = to_device(elemvec)
+    nelem, K = Ke.shape[0], Ke.shape[1]
+    out = _empty(tuple(ue.shape))
+    _call("gf_elemmat_dot", _i64(nelem), _i32(K), _dp(Ke), _dp(ue), _dp(out), _stream())
+    return out
+
+
 def elastic_stress(eps, K, G):
     """sigma = K tr(eps) I + 2 G dev(eps) per quadrature point (callers' GMat law)."""
     torch = _torch()

@@ -659,6 +659,11 @@ int gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV(gf_quad q, const double* C, doub
             return;
         }
         GF_REQUIRE(precision == GF_F64, "precision must be GF_F64 or GF_BF16X3");
+        if (q->et == GF_HEX8 && q->nip == 8) {
+            int rc = gf::stiffness_f64_hex8(q, C, elemmat, s);
+            if (rc != GF_OK) throw Error(rc, gf_last_error());
+            return;
+        }
         if (q->et == GF_HEX8)
             k_int_tensor4_f64<8, 3, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->dNx.p, q->vol.p, C, elemmat);
         else if (q->et == GF_QUAD4_PLANAR)

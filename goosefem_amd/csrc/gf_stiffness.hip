@@ -1,10 +1,249 @@
-// Hex8 element stiffness on MFMA (placeholder until the BF16 hi/lo kernel lands).
+// Hex8 element stiffness Int_gradN_dot_tensor4_dot_gradNT_dV (include/GooseFEM/Element.h:1131-1176):
+//   K(m*3+j, n*3+k) = sum_q sum_{i,l} dNx(q,m,i) C_q(i,j,k,l) dNx(q,n,l) dV_q
+//
+// GF_BF16X3 (matrix cores).  Write kk = (q, j2, i) (72 values) and
+//   T[kk][(n,k)] = dV_q sum_l C_q(i, j2, k, l) dNx(q,n,l)                      (72 x 24, VALU, FP32)
+//   A[(m,j)][kk] = delta(j, j2) dNx(q,m,i)                                    (24 x 72)
+// so that K = A T: one 24 x 72 by 72 x 24 product per element, evaluated on
+// v_mfma_f32_32x32x16_bf16 (tile padded to 32 x 32, contraction to 80).  Both operands are split
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi); the product accumulates hi*hi + hi*lo + lo*hi
+// in FP32 (the dropped lo*lo term and the bf16 rounding of lo are ~2^-16 relative).
+// One wavefront owns one element at a time: C_q (8 x 81), dNx and dV are staged in LDS with
+// coalesced loads, T is built in LDS as bf16 hi/lo pairs, the 15 MFMAs accumulate the 32 x 32 tile,
+// and the 24 x 24 result is written back through LDS as contiguous FP64 rows.
+//
+// GF_F64: the same per-entry accumulation order as the reference loop (q, i, l for each (j, k)),
+// with one wavefront per element reading C and dNx from LDS (each thread owns one (m, n) block).
 #include "gf_internal.hpp"
 
 using namespace gf;
 
-int gf::stiffness_bf16x3(gf_quad, const double*, double*, hipStream_t)
+namespace gf {
+const double* quad_dNx(gf_quad q);
+const double* quad_vol(gf_quad q);
+int64_t quad_nelem(gf_quad q);
+int quad_nip(gf_quad q);
+int quad_etype(gf_quad q);
+} // namespace gf
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int WAVES = 4;           // wavefronts per workgroup
+constexpr int NQ = 8, NN = 8;      // quadrature points, nodes
+constexpr int NC = NQ * 81;        // tensor4 doubles per element
+constexpr int NG = NQ * NN * 3;    // dNx doubles per element
+constexpr int KK = 72, KKP = 80;   // contraction length, padded to 5 x 16
+
+struct WaveLDS {
+    float C[NC];
+    float g[NG];
+    float vol[NQ];
+    __bf16 Thi[KKP][24];
+    __bf16 Tlo[KKP][24];
+    float K[24 * 24];
+};
+
+__device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo)
 {
-    set_error("GF_BF16X3 stiffness not available in this build");
-    return GF_EINVAL;
+    hi = static_cast<__bf16>(x);
+    lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+__global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, const double* __restrict__ C,
+                                                                  const double* __restrict__ dNx,
+                                                                  const double* __restrict__ vol,
+                                                                  double* __restrict__ elemmat)
+{
+    __shared__ WaveLDS lds[WAVES];
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    WaveLDS& L = lds[w];
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * WAVES;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * WAVES + w; e < nelem; e += stride) {
+        // stage C, dNx, dV (coalesced, converted to FP32)
+        const double* Ce = C + e * NC;
+        for (int t = lane; t < NC; t += 64) L.C[t] = static_cast<float>(Ce[t]);
+        const double* ge = dNx + e * NG;
+        for (int t = lane; t < NG; t += 64) L.g[t] = static_cast<float>(ge[t]);
+        if (lane < NQ) L.vol[lane] = static_cast<float>(vol[e * NQ + lane]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // T[kk][(n,k)], kk = q*9 + j2*3 + i
+        for (int t = lane; t < KKP * 24; t += 64) {
+            const int kk = t / 24, col = t - (t / 24) * 24;
+            float v = 0.0f;
+            if (kk < KK) {
+                const int q = kk / 9, r = kk - q * 9, j2 = r / 3, i = r - j2 * 3;
+                const int n = col / 3, k = col - n * 3;
+                const float* Cq = L.C + q * 81 + ((i * 3 + j2) * 3 + k) * 3;
+                const float* gq = L.g + (q * NN + n) * 3;
+                v = (Cq[0] * gq[0] + Cq[1] * gq[1] + Cq[2] * gq[2]) * L.vol[q];
+            }
+            __bf16 hi, lo;
+            split_bf16(v, hi, lo);
+            L.Thi[kk][col] = hi;
+            L.Tlo[kk][col] = lo;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // MFMA: lane supplies A row (l % 32) and B column (l % 32), contraction slots 8*(l/32)..+7
+        const int row = lane % 32, half = lane / 32;
+        const int m = row / 3, j = row - (row / 3) * 3;
+        floatx16 acc = {};
+#pragma unroll
+        for (int ch = 0; ch < KKP / 16; ++ch) {
+            bf16x8 ahi, alo, bhi, blo;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int kk = ch * 16 + half * 8 + s;
+                float av = 0.0f;
+                if (row < 24 && kk < KK) {
+                    const int q = kk / 9, r = kk - q * 9, j2 = r / 3, i = r - j2 * 3;
+                    av = (j2 == j) ? L.g[(q * NN + m) * 3 + i] : 0.0f;
+                }
+                __bf16 h, l;
+                split_bf16(av, h, l);
+                ahi[s] = h;
+                alo[s] = l;
+                if (row < 24) {
+                    bhi[s] = L.Thi[kk][row];
+                    blo[s] = L.Tlo[kk][row];
+                }
+                else {
+                    bhi[s] = static_cast<__bf16>(0.0f);
+                    blo[s] = static_cast<__bf16>(0.0f);
+                }
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
+        }
+        // accumulator layout: value r of lane l is C[8*(r/4) + 4*(l/32) + r%4][l % 32]
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = 8 * (r / 4) + 4 * half + (r % 4);
+            if (i < 24 && row < 24) L.K[i * 24 + row] = acc[r];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double* Ke = elemmat + e * 576;
+        for (int t = lane; t < 576; t += 64) Ke[t] = static_cast<double>(L.K[t]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// FP64 in reference order, one wavefront per element, C and dNx from LDS.
+__global__ void __launch_bounds__(64 * WAVES) k_stiffness_f64(int64_t nelem, const double* __restrict__ C,
+                                                               const double* __restrict__ dNx,
+                                                               const double* __restrict__ vol,
+                                                               double* __restrict__ elemmat)
+{
+    __shared__ double sC[WAVES][NC];
+    __shared__ double sg[WAVES][NG];
+    __shared__ double sv[WAVES][NQ];
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int m = lane / 8, n = lane % 8;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * WAVES;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * WAVES + w; e < nelem; e += stride) {
+        for (int t = lane; t < NC; t += 64) sC[w][t] = C[e * NC + t];
+        for (int t = lane; t < NG; t += 64) sg[w][t] = dNx[e * NG + t];
+        if (lane < NQ) sv[w][lane] = vol[e * NQ + lane];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double acc[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[j][k] = 0.0;
+        for (int q = 0; q < NQ; ++q) {
+            const double* g = sg[w] + q * 24;
+            const double* Cq = sC[w] + q * 81;
+            const double v = sv[w][q];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+#pragma unroll
+                        for (int l = 0; l < 3; ++l)
+                            acc[j][k] += g[m * 3 + i] * Cq[((i * 3 + j) * 3 + k) * 3 + l] * g[n * 3 + l] * v;
+        }
+        double* Ke = elemmat + e * 576;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Ke[(m * 3 + j) * 24 + n * 3 + k] = acc[j][k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// out[e] = Ke[e] ue[e]  (batched element GEMV), one thread per element row
+__global__ void k_elemmat_dot(int64_t nrows, int K, const double* __restrict__ Ke, const double* __restrict__ ue,
+                              double* __restrict__ out)
+{
+    int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= nrows) return;
+    const int64_t e = t / K;
+    const double* row = Ke + t * K;
+    const double* x = ue + e * K;
+    double s = 0.0;
+    for (int c = 0; c < K; ++c) s += row[c] * x[c];
+    out[t] = s;
+}
+
+unsigned grid_for(int64_t nelem)
+{
+    int64_t g = (nelem + WAVES - 1) / WAVES;
+    return static_cast<unsigned>(std::min<int64_t>(g, 256 * 64));
+}
+
+} // namespace
+
+namespace gf {
+
+int stiffness_bf16x3(gf_quad q, const double* C, double* elemmat, hipStream_t s)
+{
+    return guard([&] {
+        GF_REQUIRE(quad_etype(q) == GF_HEX8 && quad_nip(q) == NQ, "GF_BF16X3 needs Hex8 with 8 points");
+        const int64_t ne = quad_nelem(q);
+        if (ne == 0) return;
+        k_stiffness_bf16x3<<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);
+        check_launch();
+    });
+}
+
+int stiffness_f64_hex8(gf_quad q, const double* C, double* elemmat, hipStream_t s)
+{
+    return guard([&] {
+        GF_REQUIRE(quad_etype(q) == GF_HEX8 && quad_nip(q) == NQ, "Hex8 with 8 points");
+        const int64_t ne = quad_nelem(q);
+        if (ne == 0) return;
+        k_stiffness_f64<<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);
+        check_launch();
+    });
+}
+
+} // namespace gf
+
+extern "C" int gf_elemmat_dot(int64_t nelem, int K, const double* elemmat, const double* elemvec, double* out,
+                              void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(elemmat && elemvec && out && K > 0, "invalid argument");
+        const int64_t n = nelem * K;
+        if (n == 0) return;
+        k_elemmat_dot<<<nblocks(n, 256), 256, 0, S(stream)>>>(n, K, elemmat, elemvec, out);
+        check_launch();
+    });
 }

@@ -195,6 +195,13 @@ int gf_quad_int_gradN_dot_tensor2_dV(gf_quad q, const void* qtensor, void* elemv
 int gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV(gf_quad q, const double* qtensor4, double* elemmat,
                                                 int precision, void* stream);
 
+/* Batched element matrix-vector product out[e] = elemmat[e] . elemvec[e] (K = nne*ndim), the
+ * element part of the matrix-free product sum_e K_e u_e; assemble with
+ * gf_vector_assemble_dofs_elemvec.  No single reference function: it is Matrix::Dot
+ * (Matrix.h:804-815) without forming the sparse matrix. */
+int gf_elemmat_dot(int64_t nelem, int K, const double* elemmat, const double* elemvec, double* out,
+                   void* stream);
+
 /* Material: linear elastic law of the callers (GMat* in docs/examples; not in the reference):
  * sigma = K tr(eps) I + 2 G dev(eps), per quadrature point; K_q/G_q [nelem*nip] FP64, or FP32
  * for dtype GF_F32 (then eps/sig are FP32 too). */

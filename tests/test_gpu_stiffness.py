@@ -1,0 +1,94 @@
+"""Hex8 element stiffness on the GPU: FP64 path bit-exact with the oracle's reference-order loop,
+BF16 hi/lo MFMA path within a stated tolerance, and the assembled CSR product equal to the
+matrix-free sum_e K_e u_e."""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import HEX8
+
+pytestmark = pytest.mark.gpu
+
+# BF16x3 (hi*hi + hi*lo + lo*hi, FP32 accumulation) vs FP64: norm-wise relative error bound
+TOL_BF16X3 = 2e-5
+
+
+@pytest.fixture(scope="module")
+def gf():
+    import goosefem_amd
+
+    goosefem_amd.lib()
+    return goosefem_amd
+
+
+def host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def hex8_problem(n=4, distort=0.0, seed=0):
+    c, conn = oracle.hex8_regular(n, n, n)
+    if distort:
+        c = c + distort * np.random.default_rng(seed).uniform(-1, 1, c.shape)
+    vec = oracle.Vector(conn, oracle.dofs(c.shape[0], 3))
+    x = vec.AsElement(c)
+    q = oracle.Quadrature(HEX8, x)
+    rng = np.random.default_rng(seed + 1)
+    K = np.full((q.nelem, 8), 10.0)
+    G = rng.uniform(0.5, 1.5, (q.nelem, 8))
+    C = oracle.elastic_tangent(K, G)
+    return c, conn, x, q, C
+
+
+@pytest.mark.parametrize("distort", [0.0, 0.15])
+def test_stiffness_fp64_and_bf16x3(gf, distort):
+    c, conn, x, q, C = hex8_problem(5, distort)
+    ref = q.Int_gradN_dot_tensor4_dot_gradNT_dV(C)
+    gq = gf.Element.Hex8.Quadrature(x)
+    k64 = host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C, gf.F64))
+    assert np.array_equal(k64, ref)
+    kbf = host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C, gf.BF16X3))
+    err = relerr(kbf, ref)
+    assert err < TOL_BF16X3, err
+    # symmetric and rigid-body free to the same tolerance
+    assert relerr(np.swapaxes(kbf, 1, 2), ref) < TOL_BF16X3
+    rigid = np.tile(np.array([1.0, 0.0, 0.0]), (q.nelem, 8, 1)).reshape(q.nelem, 24)
+    r = np.einsum("eab,eb->ea", kbf, rigid)
+    assert np.abs(r).max() < TOL_BF16X3 * np.abs(ref).max() * 24
+
+
+def test_stiffness_bf16x3_general_tensor(gf):
+    # no symmetry assumed: random C_ijkl
+    c, conn, x, q, _ = hex8_problem(3, 0.1, seed=5)
+    C = np.random.default_rng(9).standard_normal((q.nelem, 8, 3, 3, 3, 3))
+    ref = q.Int_gradN_dot_tensor4_dot_gradNT_dV(C)
+    gq = gf.Element.Hex8.Quadrature(x)
+    kbf = host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C, gf.BF16X3))
+    assert relerr(kbf, ref) < TOL_BF16X3
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_assembled_dot_equals_matrix_free(gf, periodic):
+    n = 4
+    c, conn, x, q, C = hex8_problem(n)
+    dofs = (oracle.dofs_periodic(c.shape[0], 3, oracle.hex8_nodes_periodic(n, n, n)) if periodic
+            else oracle.dofs(c.shape[0], 3))
+    gq = gf.Element.Hex8.Quadrature(x)
+    Ke = gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C, gf.F64)
+    vec = gf.Vector(conn, dofs)
+    A = gf.Matrix(vec, None)
+    A.assemble(Ke)
+    u = np.random.default_rng(3).standard_normal(vec.ndof)
+    Ku = host(A.Dot(u))
+    mf = host(vec.AssembleDofs(gf.elemmat_dot(Ke, vec.AsElement(u))))
+    assert relerr(Ku, mf) < 1e-14
+    # against the oracle's CSR product (bit-exact: same ordered sums)
+    rowptr, col, val = oracle.matrix_assemble(conn, dofs, host(Ke))
+    assert np.array_equal(Ku, oracle.csr_dot(rowptr, col, val, u))
