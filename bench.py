@@ -107,6 +107,101 @@ def load_traffic(nx):
     return None
 
 
+def _timed(fn, reps):
+    """Average seconds per call of fn() between HIP events on the launch stream (after 1 warm-up)."""
+    import torch
+
+    fn()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e-3 / reps
+
+
+def bench_hex8(args):
+    """Hex8 Regular n^3, per-point isotropic tangent (K = 10, G ~ U[0.5, 1.5]): element stiffness in
+    FP64 vs BF16x3 MFMA, CSR assembly, assembled Dot vs matrix-free sum_e K_e u_e."""
+    import torch
+
+    import goosefem_amd as gf
+
+    n = args.hex_n
+    t0 = time.perf_counter()
+    mesh = gf.Mesh.Hex8.Regular(n, n, n)
+    coor, conn, dofs = mesh.coor, mesh.conn, mesh.dofs
+    vec = gf.Vector(conn, dofs)
+    quad = gf.Element.Hex8.Quadrature(vec.AsElement(coor))
+    E = mesh.nelem
+    z = torch.from_numpy(gf.randn(3, E * 8)).cuda()
+    Gq = (0.5 + 0.5 * (1.0 + torch.erf(z / np.sqrt(2.0)))).reshape(E, 8)  # U[0.5, 1.5]
+    Kq = torch.full((E, 8), 10.0, dtype=torch.float64, device="cuda")
+    C = gf.elastic_tangent(Kq, Gq)
+    del z, Gq, Kq
+    k64 = torch.empty((E, 24, 24), dtype=torch.float64, device="cuda")
+    kbf = torch.empty_like(k64)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    t64 = _timed(lambda: quad.Int_gradN_dot_tensor4_dot_gradNT_dV(C, gf.F64, out=k64), 3)
+    tbf = _timed(lambda: quad.Int_gradN_dot_tensor4_dot_gradNT_dV(C, gf.BF16X3, out=kbf), 5)
+    err_ke = float(torch.linalg.vector_norm(kbf - k64) / torch.linalg.vector_norm(k64))
+    del C
+    t0 = time.perf_counter()
+    A = gf.Matrix(vec, None)
+    t_pattern = time.perf_counter() - t0
+    t_asm = _timed(lambda: A.assemble(k64), 3)
+    u = torch.from_numpy(gf.randn(4, vec.ndof)).cuda()
+    Ku = torch.empty_like(u)
+    t_dot = _timed(lambda: A.Dot(u, out=Ku), 10)
+
+    def matrix_free(K):
+        return vec.AssembleDofs(gf.elemmat_dot(K, vec.AsElement(u)))
+
+    t_mf = _timed(lambda: matrix_free(k64), 3)
+    err_mf = float(torch.linalg.vector_norm(matrix_free(k64) - Ku) / torch.linalg.vector_norm(Ku))
+    err_ku_bf = float(torch.linalg.vector_norm(matrix_free(kbf) - Ku) / torch.linalg.vector_norm(Ku))
+    bytes_el = 11392  # FP64 API bytes per element: C 5184 + GradN 1536 + dV 64 + K_e 4608
+    flops_el = 69120  # Voigt B^T (D dV) B, 8 points
+    ach = bytes_el * E / tbf / 1e9
+    res = {
+        "metric": "Hex8 element stiffness (BF16x3 MFMA) throughput",
+        "value": E / tbf,
+        "unit": "elements/s",
+        "config": {"workload": f"Hex8 Regular {n}^3 element stiffness + CSR assembly + K.u", "nelem": E,
+                   "ndof": vec.ndof, "nnz": A.nnz},
+        "dtype": "bf16x3 (hi/lo split) mfma, fp32 accumulate; fp64 inputs/outputs",
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_stiffness_bf16x3",
+                     "algorithmic_bytes_per_launch": bytes_el * E, "avg_launch_us": tbf * 1e6,
+                     "tflops_voigt": flops_el * E / tbf / 1e12},
+        "fp64_stiffness": {"avg_launch_us": t64 * 1e6, "achieved_GBs": bytes_el * E / t64 / 1e9},
+        "error_vs_fp64": {"Ke_rel": err_ke, "Ku_rel": err_ku_bf},
+        "csr": {"pattern_build_s": t_pattern, "assemble_ms": t_asm * 1e3,
+                "assemble_GBs": (E * 4608 + A.nnz * 8) / t_asm / 1e9,
+                "dot_ms": t_dot * 1e3, "dot_GBs": (A.nnz * 12 + vec.ndof * 16) / t_dot / 1e9,
+                "matrix_free_ms": t_mf * 1e3, "matrix_free_vs_dot_rel": err_mf},
+        "setup_s": t_setup,
+    }
+    if not args.no_cpu:
+        import oracle
+
+        m = 2000  # bounded CPU sample: elements of the same mesh
+        ov = oracle.Vector(conn[:m], oracle.dofs(coor.shape[0], 3))
+        oq = oracle.Quadrature(oracle.HEX8, ov.AsElement(coor))
+        Gs = 0.5 + np.random.default_rng(0).random((m, 8))
+        Cs = oracle.elastic_tangent(np.full((m, 8), 10.0), Gs)
+        t0 = time.perf_counter()
+        oq.Int_gradN_dot_tensor4_dot_gradNT_dV(Cs)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / dt, "unit": "elements/s", "cores": 1, "kind": "port",
+                               "sample": f"{m} elements, FP64 oracle reference loop, one thread, {dt:.2f} s"}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +211,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness"], default="quad4-explicit")
+    ap.add_argument("--hex-n", type=int, default=128)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC counter calibration) and exit")
     args = ap.parse_args()
@@ -135,6 +232,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        return
+
+    if args.workload == "hex8-stiffness":
+        bench_hex8(args)
         return
 
     nx = ny = args.nx

@@ -584,10 +584,11 @@ class _Quadrature(_Handle):
         _call("gf_quad_int_gradN_dot_tensor2_dV", self._h, _dp(qt), _dp(out), _i32(_dtype_code(qt)), _stream())
         return out
 
-    def Int_gradN_dot_tensor4_dot_gradNT_dV(self, qtensor, precision=F64):
+    def Int_gradN_dot_tensor4_dot_gradNT_dV(self, qtensor, precision=F64, out=None):
         qt = to_device(qtensor)
         K = self.nne * self.ndim
-        out = _empty((self.nelem, K, K))
+        if out is None:
+            out = _empty((self.nelem, K, K))
         _call("gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV", self._h, _dp(qt), _dp(out), _i32(precision), _stream())
         return out
 
@@ -768,9 +769,10 @@ class Matrix(_Handle):
     def assemble(self, elemmat):
         _call("gf_matrix_assemble", self._h, _dp(to_device(elemmat)), _stream())
 
-    def Dot(self, x):
+    def Dot(self, x, out=None):
         x = to_device(x)
-        out = _empty(tuple(x.shape))
+        if out is None:
+            out = _empty(tuple(x.shape))
         fn = "gf_matrix_dot_nodevec" if x.dim() == 2 else "gf_matrix_dot_dofval"
         _call(fn, self._h, _dp(x), _dp(out), _stream())
         return out
