@@ -202,6 +202,89 @@ def bench_hex8(args):
     print(json.dumps(res))
 
 
+def periodic_setup(n, hex8=False, seed=4, sigma=0.4):
+    """Periodic Regular mesh with Tyings::Control + Tyings::Periodic as in
+    docs/examples/statics/periodic_elastoplastic.py:28-47; per-element G log-normal, K = 10."""
+    import goosefem_amd as gf
+
+    mesh = gf.Mesh.Hex8.Regular(n, n, n) if hex8 else gf.Mesh.Quad4.Regular(n, n)
+    coor, conn = mesh.coor, mesh.conn
+    ctrl = gf.Tyings.Control(coor, mesh.dofs)
+    iip = np.concatenate((ctrl.controlDofs.ravel(), ctrl.dofs[mesh.nodesOrigin]))
+    ty = gf.Tyings.Periodic(ctrl.coor, ctrl.dofs, ctrl.controlDofs, mesh.nodesPeriodic, iip)
+    vec = gf.Vector(conn, ty.dofs)
+    x = gf.Vector(conn, mesh.dofs).AsElement(coor)
+    quad = gf.Element.Hex8.Quadrature(x) if hex8 else gf.Element.Quad4.QuadraturePlanar(x)
+    G = np.exp(sigma * gf.randn(seed, mesh.nelem))
+    K = np.full(mesh.nelem, 10.0)
+    P = gf.TyingsPCG(vec, quad, K, G, ty)
+    return dict(gf=gf, mesh=mesh, ty=ty, vec=vec, quad=quad, G=G, K=K, P=P, conn=conn, coor=coor)
+
+
+def shear_rhs(S, dgamma):
+    """b_u = -K'_up x_p for a simple-shear increment of the control dof ctrl(0, 1)."""
+    ty, P = S["ty"], S["P"]
+    xp = np.zeros(ty.nnp)
+    xp[ty.control[0, 1] - ty.nnu] = dgamma
+    return -P.apply(None, xp), xp
+
+
+def bench_pcg(args):
+    """Plain FP64 Jacobi-PCG on the periodic tied operator, zero initial guess, to 1e-10."""
+    import torch
+
+    n = args.pcg_n
+    S = periodic_setup(n, args.pcg_hex)
+    P, ty = S["P"], S["ty"]
+    b, xp = shear_rhs(S, 1e-3)
+    torch.cuda.synchronize()
+    P.solve(b, tol=1e-10)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, it, res = P.solve(b, tol=1e-10)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    res_d = {"metric": "PCG time to relative residual 1e-10 (plain, zero guess)", "value": t, "unit": "s",
+             "higher_is_better": False,
+             "config": {"workload": f"periodic {'Hex8' if args.pcg_hex else 'Quad4'} Regular {n}"
+                                    f"{'^3' if args.pcg_hex else '^2'}, log-normal G (sigma 0.4), K 10, "
+                                    f"simple shear increment 1e-3", "nnu": ty.nnu},
+             "iterations": it, "true_relres": res, "ms_per_iteration": t / max(it, 1) * 1e3}
+    if not args.no_cpu:
+        res_d["cpu_baseline"] = pcg_cpu_baseline(S, b, 50)
+    print(json.dumps(res_d))
+
+
+def pcg_cpu_baseline(S, b, iters):
+    """The oracle's FP64 CG on the assembled K' (scipy sparse products), a bounded number of
+    iterations on one thread; reported per iteration."""
+    import scipy.sparse as sp
+
+    import oracle
+
+    ty = S["ty"]
+    conn = S["conn"]
+    q = oracle.Quadrature(oracle.HEX8 if S["mesh"].ndim == 3 else oracle.QUAD4_PLANAR,
+                          oracle.Vector(conn, oracle.dofs(S["coor"].shape[0], S["coor"].shape[1])).AsElement(S["coor"]))
+    nip = q.nip
+    C = oracle.elastic_tangent(np.repeat(S["K"][:, None], nip, 1), np.repeat(S["G"][:, None], nip, 1))
+    Ke = q.Int_gradN_dot_tensor4_dot_gradNT_dV(C)
+    rp, cl, vl = oracle.matrix_assemble(conn, ty.dofs[: S["coor"].shape[0]], Ke)
+    ndof = ty.nni + ty.nnd
+    A = sp.csr_matrix((vl, cl, rp), shape=(ndof, ndof))
+    u, d = slice(0, ty.nnu), slice(ty.nni, ndof)
+    Cdu = sp.csr_matrix(ty.Cdu)
+    Kp = (A[u, u] + A[u, d] @ Cdu + Cdu.T @ A[d, u] + Cdu.T @ A[d, d] @ Cdu).tocsr()
+    Kp.sort_indices()
+    bh = b.cpu().numpy()
+    t0 = time.perf_counter()
+    _, it, _ = oracle.pcg_csr(Kp.indptr.astype(np.int64), Kp.indices.astype(np.int64), Kp.data,
+                              Kp.diagonal().copy(), bh, tol=1e-30, maxiter=iters)
+    dt = time.perf_counter() - t0
+    return {"value": dt / it * 1e3, "unit": "ms/iteration", "cores": 1, "kind": "port",
+            "sample": f"{it} CG iterations on the assembled K' ({Kp.nnz} nnz), FP64 oracle, one thread"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,7 +294,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness"], default="quad4-explicit")
+    ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic"],
+                    default="quad4-explicit")
+    ap.add_argument("--pcg-n", type=int, default=512)
+    ap.add_argument("--pcg-hex", action="store_true", help="periodic Hex8 n^3 instead of Quad4 n^2")
     ap.add_argument("--hex-n", type=int, default=128)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC counter calibration) and exit")
@@ -236,6 +322,9 @@ def main():
 
     if args.workload == "hex8-stiffness":
         bench_hex8(args)
+        return
+    if args.workload == "pcg-periodic":
+        bench_pcg(args)
         return
 
     nx = ny = args.nx

@@ -861,3 +861,72 @@ class ExplicitQuad4(_Handle):
         o = _empty((self._vec.ndof,))
         _hip_memcpy_d2d(o, p, 8 * self._vec.ndof)
         return o
+
+
+# -------------------------------------------------------------------------------------------------
+# Tied-system PCG (MatrixPartitionedTyings operator, FP64 Jacobi-PCG)
+# -------------------------------------------------------------------------------------------------
+
+
+class TyingsPCG(_Handle):
+    """FP64 Jacobi-PCG on K' = A_uu + A_ud C_du + C_du^T A_du + C_du^T A_dd C_du
+    (MatrixPartitionedTyings.h:655-674), matrix-free for a linear elastic material.
+
+    vector: Vector(conn, tyings.dofs); quad: QuadraturePlanar / Hex8 Quadrature (Gauss) of the mesh;
+    K_e, G_e: per-element moduli; tyings: Tyings.Periodic.
+    """
+
+    _destroy = "gf_pcg_destroy"
+
+    def __init__(self, vector, quad, K_e, G_e, tyings):
+        self._vec, self._quad = vector, quad
+        self._K = to_device(K_e)
+        self._G = to_device(G_e)
+        self.nnu, self.nnp = tyings.nnu, tyings.nnp
+        self.nnd = tyings.nnd
+        self._cdu = tuple(np.ascontiguousarray(a) for a in tyings.Cdu_csr)
+        self._cdp = tuple(np.ascontiguousarray(a) for a in tyings.Cdp_csr)
+        h = _vp()
+        _call("gf_pcg_create", ctypes.byref(h), vector._h, quad._h, _dp(self._K), _dp(self._G), _i64(self.nnu),
+              _i64(self.nnp), _hp(self._cdu[0]), _hp(self._cdu[1]), _hp(self._cdu[2]), _hp(self._cdp[0]),
+              _hp(self._cdp[1]), _hp(self._cdp[2]))
+        self._h = h
+
+    def apply(self, x_u=None, x_p=None):
+        """K' x_u + K'_up x_p."""
+        xu = None if x_u is None else to_device(x_u)
+        xp = None if x_p is None else to_device(x_p)
+        out = _empty((self.nnu,))
+        _call("gf_pcg_apply", self._h, _dp(xu), _dp(xp), _dp(out), _stream())
+        return out
+
+    def expand(self, x_u=None, x_p=None):
+        """Full dofval [x_u, x_p, C_du x_u + C_dp x_p]."""
+        xu = None if x_u is None else to_device(x_u)
+        xp = None if x_p is None else to_device(x_p)
+        out = _empty((self.nnu + self.nnp + self.nnd,))
+        _call("gf_pcg_expand", self._h, _dp(xu), _dp(xp), _dp(out), _stream())
+        return out
+
+    @property
+    def inverse_diagonal(self):
+        p = _vp()
+        _call("gf_pcg_inverse_diagonal", self._h, ctypes.byref(p))
+        out = _empty((self.nnu,))
+        _hip_memcpy_d2d(out, p, 8 * self.nnu)
+        return out
+
+    def solve(self, b_u, x0=None, tol=1e-10, maxiter=100000, check_every=32, raise_on_fail=True):
+        """Returns (x_u, iterations, true relative residual)."""
+        b = to_device(b_u)
+        x = _empty((self.nnu,)) if x0 is None else to_device(x0).clone()
+        if x0 is None:
+            x.zero_()
+        it, res = _i64(), _f64()
+        f = lib().gf_pcg_solve
+        f.restype = ctypes.c_int
+        rc = f(self._h, _dp(b), _dp(x), _f64(tol), _i64(maxiter), _i64(check_every), ctypes.byref(it),
+               ctypes.byref(res), _stream())
+        if rc != GF_OK and (raise_on_fail or rc != GF_ENOTCONV):
+            _check(rc)
+        return x, it.value, res.value

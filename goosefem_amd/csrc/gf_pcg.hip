@@ -1,0 +1,581 @@
+// FP64 Jacobi-preconditioned conjugate gradients on the tied operator of
+// MatrixPartitionedTyings (include/GooseFEM/MatrixPartitionedTyings.h:655-674, Q9):
+//   K' = A_uu + A_ud C_du + C_du^T A_du + C_du^T A_dd C_du        (ACuu)
+//   K'_up x_p = (A_up + A_ud C_dp + C_du^T A_dp + C_du^T A_dd C_dp) x_p   (ACup)
+// The reference factorises ACuu with Eigen's SimplicialLDLT (MatrixPartitionedTyings.h:527-675);
+// here the operator is applied matrix-free, for a linear elastic material:
+//   expand  x = [x_u, x_p, C_du x_u + C_dp x_p]           ([u, p, d] numbering of Tyings::Periodic)
+//   element f_e = Int_gradN_dot_tensor2_dV(sigma(SymGradN_vector(AsElement(x))))   (FP64, reference order)
+//   gather  y = AssembleDofs(f)                          (ordered DOF-centric gather, Vector.h:723-741)
+//   fold    out_u = y_u + C_du^T y_d
+// Reductions are two-stage and deterministic: every block sums its grid-stride range in a fixed
+// order and a fixed LDS tree; consumers reduce the block partials in block order.  The iteration
+// runs with no host synchronisation: scalars live on the device and a device flag stops the
+// iteration (kernels launched after convergence return immediately).
+#include <algorithm>
+#include <cmath>
+
+#include "gf_internal.hpp"
+
+using namespace gf;
+
+namespace gf {
+const double* quad_dNx(gf_quad q);
+const double* quad_vol(gf_quad q);
+int64_t quad_nelem(gf_quad q);
+int quad_nip(gf_quad q);
+int quad_etype(gf_quad q);
+} // namespace gf
+
+namespace {
+
+constexpr int BS = 256;
+constexpr int NBR = 512; // blocks of the reduction kernels (fixed: partial count)
+
+struct Csr {
+    DBuf<int32_t> ptr, idx;
+    DBuf<double> val;
+};
+
+} // namespace
+
+struct gf_pcg_s {
+    gf_vector v = nullptr;
+    gf_quad q = nullptr;
+    int et = 0, nne = 0, nd = 0, K = 0;
+    int64_t nelem = 0, nnu = 0, nnp = 0, nnd = 0, nni = 0, ndof = 0;
+    DBuf<double> Ke, Ge;
+    Csr cdu, cdp, cduT;
+    DBuf<double> xfull, fe, invdiag, x, r, z, qv, tmp;
+    DBuf<double> partial;  // [4][NBR]
+    DBuf<double> scal;     // [8]: rz (2 slots), bb, rr, tol2, ...
+    DBuf<int32_t> state;   // [0] done, [1] iterations, [2] maxiter
+};
+
+namespace {
+
+// ---- element force from a full dofval, FP64, reference expression order -------------------------
+template <int ET>
+__global__ void k_elem_force(int64_t nelem, const int32_t* __restrict__ conn, const int32_t* __restrict__ dofs,
+                             const double* __restrict__ dNx, const double* __restrict__ vol,
+                             const double* __restrict__ Ke, const double* __restrict__ Ge,
+                             const double* __restrict__ x, double* __restrict__ fe, const int32_t* __restrict__ state)
+{
+    if (state && state[0]) return;
+    constexpr int NNE = ET == GF_HEX8 ? 8 : 4, ND = ET == GF_HEX8 ? 3 : 2;
+    int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= nelem) return;
+    double u[NNE][ND];
+#pragma unroll
+    for (int m = 0; m < NNE; ++m)
+#pragma unroll
+        for (int i = 0; i < ND; ++i) u[m][i] = x[dofs[static_cast<int64_t>(conn[e * NNE + m]) * ND + i]];
+    const double k = Ke[e], gm = Ge[e];
+    double f[NNE][ND];
+#pragma unroll
+    for (int m = 0; m < NNE; ++m)
+#pragma unroll
+        for (int i = 0; i < ND; ++i) f[m][i] = 0.0;
+    for (int q = 0; q < NNE; ++q) {
+        const double* g = dNx + (e * NNE + q) * NNE * ND;
+        const double v = vol[e * NNE + q];
+        double s[3][3];
+        if constexpr (ET == GF_HEX8) {
+            // SymGradN_vector, generic loop Element.h:990-1022
+            double t[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) t[i][j] = 0.0;
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        double h = 0.5 * g[m * 3 + i] * u[m][j];
+                        t[i][j] += h;
+                        t[j][i] += h;
+                    }
+            double tr = t[0][0] + t[1][1] + t[2][2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    s[i][j] = (i == j) ? k * tr + 2.0 * gm * (t[i][j] - tr / 3.0) : 2.0 * gm * t[i][j];
+            // ElementHex8.h:354-383
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    f[m][j] += (g[m * 3 + 0] * s[0][j] + g[m * 3 + 1] * s[1][j] + g[m * 3 + 2] * s[2][j]) * v;
+        }
+        else {
+            // ElementQuad4Planar.h:191-221, :253-278
+            double e00 = g[0] * u[0][0] + g[2] * u[1][0] + g[4] * u[2][0] + g[6] * u[3][0];
+            double e11 = g[1] * u[0][1] + g[3] * u[1][1] + g[5] * u[2][1] + g[7] * u[3][1];
+            double e01 = 0.5 * (g[0] * u[0][1] + g[2] * u[1][1] + g[4] * u[2][1] + g[6] * u[3][1] +
+                                g[1] * u[0][0] + g[3] * u[1][0] + g[5] * u[2][0] + g[7] * u[3][0]);
+            double e22 = 0.0;
+            double tr = e00 + e11 + e22;
+            s[0][0] = k * tr + 2.0 * gm * (e00 - tr / 3.0);
+            s[1][1] = k * tr + 2.0 * gm * (e11 - tr / 3.0);
+            s[0][1] = 2.0 * gm * e01;
+            s[1][0] = 2.0 * gm * e01;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                f[m][0] += (g[2 * m] * s[0][0] + g[2 * m + 1] * s[1][0]) * v;
+                f[m][1] += (g[2 * m] * s[0][1] + g[2 * m + 1] * s[1][1]) * v;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NNE; ++m)
+#pragma unroll
+        for (int i = 0; i < ND; ++i) fe[(e * NNE + m) * ND + i] = f[m][i];
+}
+
+// element stiffness diagonal K_e(mi, mi) = sum_q dV [(K + G/3) g_i^2 + G |g|^2] (isotropic law)
+template <int ET>
+__global__ void k_elem_diag(int64_t nelem, const double* __restrict__ dNx, const double* __restrict__ vol,
+                            const double* __restrict__ Ke, const double* __restrict__ Ge, double* __restrict__ de)
+{
+    constexpr int NNE = ET == GF_HEX8 ? 8 : 4, ND = ET == GF_HEX8 ? 3 : 2;
+    int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= nelem * NNE) return;
+    const int64_t e = t / NNE;
+    const int m = static_cast<int>(t - e * NNE);
+    const double a = Ke[e] + Ge[e] / 3.0, b = Ge[e];
+    double d[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) d[i] = 0.0;
+    for (int q = 0; q < NNE; ++q) {
+        const double* g = dNx + ((e * NNE + q) * NNE + m) * ND;
+        double n2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) n2 += g[i] * g[i];
+#pragma unroll
+        for (int i = 0; i < ND; ++i) d[i] += vol[e * NNE + q] * (a * g[i] * g[i] + b * n2);
+    }
+#pragma unroll
+    for (int i = 0; i < ND; ++i) de[t * ND + i] = d[i];
+}
+
+__device__ inline double row_sum(const int32_t* ptr, const int32_t* idx, const double* fe, int64_t r)
+{
+    double s = 0.0;
+    for (int32_t j = ptr[r]; j < ptr[r + 1]; ++j) s += fe[idx[j]];
+    return s;
+}
+
+// block reduction of one value per thread into partial[blockIdx]; fixed tree
+__device__ inline void block_reduce_store(double v, double* partial)
+{
+    __shared__ double sh[BS];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+    __syncthreads();
+}
+
+// sum of NBR partials in fixed order (every block computes the same value)
+__device__ inline double reduce_partials(const double* partial)
+{
+    __shared__ double sh[BS];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < NBR; i += BS) v += partial[i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// out_u = y_u + C_du^T y_d with y = ordered gather of fe; optional partial of dot(out, w)
+__global__ void __launch_bounds__(BS) k_gather_fold(int64_t nnu, int64_t nni, const int32_t* __restrict__ eptr,
+                                                    const int32_t* __restrict__ eidx, const double* __restrict__ fe,
+                                                    const int32_t* __restrict__ tptr, const int32_t* __restrict__ tidx,
+                                                    const double* __restrict__ tval, double* __restrict__ out,
+                                                    const double* __restrict__ w, double* __restrict__ partial,
+                                                    const int32_t* __restrict__ state)
+{
+    if (state && state[0]) return;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < nnu;
+         i += static_cast<int64_t>(gridDim.x) * BS) {
+        double s = row_sum(eptr, eidx, fe, i);
+        for (int32_t k = tptr[i]; k < tptr[i + 1]; ++k) s += tval[k] * row_sum(eptr, eidx, fe, nni + tidx[k]);
+        out[i] = s;
+        if (w) acc += w[i] * s;
+    }
+    if (partial) block_reduce_store(acc, partial);
+}
+
+// x_full = [x_u, x_p, C_du x_u + C_dp x_p] (x_u / x_p may be NULL = 0); when x_u aliases the
+// u-part of x_full it is not copied
+__global__ void k_expand(int64_t nnu, int64_t nnp, int64_t nnd, const double* __restrict__ xu,
+                         const double* __restrict__ xp, const int32_t* __restrict__ uptr,
+                         const int32_t* __restrict__ uidx, const double* __restrict__ uval,
+                         const int32_t* __restrict__ pptr, const int32_t* __restrict__ pidx,
+                         const double* __restrict__ pval, double* __restrict__ xfull, int copy_u,
+                         const int32_t* __restrict__ state)
+{
+    if (state && state[0]) return;
+    const int64_t n = (copy_u ? nnu : 0) + nnp + nnd;
+    for (int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; t < n;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int64_t i = copy_u ? t : t + nnu;
+        if (i < nnu) xfull[i] = xu ? xu[i] : 0.0;
+        else if (i < nnu + nnp) xfull[i] = xp ? xp[i - nnu] : 0.0;
+        else {
+            const int64_t d = i - nnu - nnp;
+            double s = 0.0;
+            if (xu)
+                for (int32_t k = uptr[d]; k < uptr[d + 1]; ++k) s += uval[k] * xu[uidx[k]];
+            if (xp)
+                for (int32_t k = pptr[d]; k < pptr[d + 1]; ++k) s += pval[k] * xp[pidx[k]];
+            xfull[i] = s;
+        }
+    }
+}
+
+__global__ void k_diag_fold(int64_t nnu, int64_t nni, const int32_t* __restrict__ eptr, const int32_t* __restrict__ eidx,
+                            const double* __restrict__ de, const int32_t* __restrict__ tptr,
+                            const int32_t* __restrict__ tidx, const double* __restrict__ tval,
+                            double* __restrict__ invdiag)
+{
+    int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= nnu) return;
+    double s = row_sum(eptr, eidx, de, i);
+    for (int32_t k = tptr[i]; k < tptr[i + 1]; ++k) s += tval[k] * tval[k] * row_sum(eptr, eidx, de, nni + tidx[k]);
+    invdiag[i] = 1.0 / s;
+}
+
+// r = b - q, z = r ./ d, x = x0 (taken from p), p = z; partials rz, rr, bb
+__global__ void __launch_bounds__(BS) k_pcg_init(int64_t n, const double* __restrict__ b, const double* __restrict__ q,
+                                                 const double* __restrict__ invd, double* __restrict__ r,
+                                                 double* __restrict__ z, double* __restrict__ x, double* __restrict__ p,
+                                                 double* __restrict__ partial)
+{
+    double rz = 0.0, rr = 0.0, bb = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * BS) {
+        double ri = b[i] - q[i];
+        double zi = ri * invd[i];
+        r[i] = ri;
+        z[i] = zi;
+        x[i] = p[i];
+        p[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+        bb += b[i] * b[i];
+    }
+    block_reduce_store(rz, partial);
+    block_reduce_store(rr, partial + NBR);
+    block_reduce_store(bb, partial + 2 * NBR);
+}
+
+// finalize the initial scalars: rz -> scal[0], bb -> scal[2]; done if rr <= tol^2 bb
+__global__ void k_pcg_init_scalars(const double* __restrict__ partial, double* __restrict__ scal, int32_t* __restrict__ state,
+                                   double tol)
+{
+    double rz = reduce_partials(partial), rr = reduce_partials(partial + NBR), bb = reduce_partials(partial + 2 * NBR);
+    if (threadIdx.x == 0) {
+        scal[0] = rz;
+        scal[2] = bb;
+        scal[3] = rr;
+        scal[4] = tol * tol * (bb > 0.0 ? bb : 1.0);
+        state[0] = (rr <= scal[4]) ? 1 : 0;
+        state[1] = 0;
+    }
+}
+
+// alpha = rz / (p.q); x += alpha p; r -= alpha q; z = r ./ d; partials of rz_new, rr
+__global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, const double* __restrict__ pq_partial,
+                                                   const double* __restrict__ scal, const double* __restrict__ p,
+                                                   const double* __restrict__ q, const double* __restrict__ invd,
+                                                   double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                   double* __restrict__ partial, const int32_t* __restrict__ state)
+{
+    if (state[0]) return;
+    const double pq = reduce_partials(pq_partial);
+    const int slot = state[1] & 1;
+    const double alpha = scal[slot] / pq;
+    double rz = 0.0, rr = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * BS) {
+        x[i] += alpha * p[i];
+        double ri = r[i] - alpha * q[i];
+        double zi = ri * invd[i];
+        r[i] = ri;
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    block_reduce_store(rz, partial);
+    block_reduce_store(rr, partial + NBR);
+}
+
+// beta = rz_new / rz_old; p = z + beta p; block 0 records rz_new, counts the iteration and
+// raises the done flag (state[0]) at convergence or at the iteration cap
+__global__ void __launch_bounds__(BS) k_pcg_direction(int64_t n, const double* __restrict__ partial,
+                                                      double* __restrict__ scal, const double* __restrict__ z,
+                                                      double* __restrict__ p, int32_t* __restrict__ state)
+{
+    if (state[0]) return;
+    const double rz_new = reduce_partials(partial), rr = reduce_partials(partial + NBR);
+    const int it = state[1];
+    const double beta = rz_new / scal[it & 1];
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * BS)
+        p[i] = z[i] + beta * p[i];
+    // every block read scal[it & 1] and state before this point; block 0 publishes the next
+    // iteration's values (other blocks of this launch never read the slot written here)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[(it + 1) & 1] = rz_new;
+        scal[3] = rr;
+    }
+}
+
+// separate tiny kernel so that no block of k_pcg_direction observes the update of state[1]
+__global__ void k_pcg_count(const double* __restrict__ scal, int32_t* __restrict__ state)
+{
+    if (state[0]) return;
+    if (threadIdx.x == 0) {
+        state[1] += 1;
+        if (scal[3] <= scal[4] || state[1] >= state[2]) state[0] = 1;
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_residual(int64_t n, const double* __restrict__ b, const double* __restrict__ q,
+                                                 double* __restrict__ partial)
+{
+    double rr = 0.0, bb = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * BS) {
+        double d = b[i] - q[i];
+        rr += d * d;
+        bb += b[i] * b[i];
+    }
+    block_reduce_store(rr, partial);
+    block_reduce_store(bb, partial + NBR);
+}
+
+__global__ void k_residual_final(const double* __restrict__ partial, double* __restrict__ out)
+{
+    double rr = reduce_partials(partial), bb = reduce_partials(partial + NBR);
+    if (threadIdx.x == 0) out[0] = std::sqrt(rr) / (bb > 0.0 ? std::sqrt(bb) : 1.0);
+}
+
+void upload_csr(Csr& c, const int64_t* ptr, const int64_t* idx, const double* val, int64_t nrows)
+{
+    int64_t nnz = ptr[nrows];
+    c.ptr.upload(narrow(ptr, static_cast<size_t>(nrows + 1), "csr ptr"));
+    c.idx.upload(narrow(idx, static_cast<size_t>(nnz), "csr idx"));
+    c.val.upload(val, static_cast<size_t>(nnz));
+    if (nnz == 0) {
+        // keep valid (non-null) pointers for empty matrices
+        c.idx.alloc(1);
+        c.val.alloc(1);
+    }
+}
+
+// y_u = K' x_u + K'_up x_p (x_u or x_p may be NULL); state == NULL: always runs
+void apply(gf_pcg_s* P, const double* xu, const double* xp, double* yu, bool xu_in_full, hipStream_t s,
+           const double* w, double* partial, const int32_t* state)
+{
+    auto v = P->v;
+    const int64_t nexp = (xu_in_full ? 0 : P->nnu) + P->nnp + P->nnd;
+    if (nexp > 0)
+        k_expand<<<std::min<unsigned>(nblocks(nexp, BS), 4096), BS, 0, s>>>(
+            P->nnu, P->nnp, P->nnd, xu_in_full ? P->xfull.p : xu, xp, P->cdu.ptr.p, P->cdu.idx.p, P->cdu.val.p,
+            P->cdp.ptr.p, P->cdp.idx.p, P->cdp.val.p, P->xfull.p, xu_in_full ? 0 : 1, state);
+    const unsigned ge = nblocks(P->nelem, 128);
+    if (P->et == GF_HEX8)
+        k_elem_force<GF_HEX8><<<ge, 128, 0, s>>>(P->nelem, v->conn.p, v->dofs.p, quad_dNx(P->q), quad_vol(P->q),
+                                                 P->Ke.p, P->Ge.p, P->xfull.p, P->fe.p, state);
+    else
+        k_elem_force<GF_QUAD4_PLANAR><<<ge, 128, 0, s>>>(P->nelem, v->conn.p, v->dofs.p, quad_dNx(P->q),
+                                                         quad_vol(P->q), P->Ke.p, P->Ge.p, P->xfull.p, P->fe.p, state);
+    k_gather_fold<<<NBR, BS, 0, s>>>(P->nnu, P->nni, v->dof_eptr.p, v->dof_eidx.p, P->fe.p, P->cduT.ptr.p,
+                                     P->cduT.idx.p, P->cduT.val.p, yu, w, partial, state);
+    check_launch();
+}
+
+} // namespace
+
+extern "C" {
+
+int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const double* G_e, int64_t nnu,
+                  int64_t nnp, const int64_t* cdu_ptr_h, const int64_t* cdu_col_h, const double* cdu_val_h,
+                  const int64_t* cdp_ptr_h, const int64_t* cdp_col_h, const double* cdp_val_h)
+{
+    return guard([&] {
+        GF_REQUIRE(out && v && q && K_e && G_e && cdu_ptr_h && cdp_ptr_h, "null argument");
+        const int et = quad_etype(q);
+        GF_REQUIRE(et == GF_QUAD4_PLANAR || et == GF_HEX8, "PCG operator needs QuadraturePlanar or Hex8");
+        GF_REQUIRE(quad_nelem(q) == v->nelem && quad_nip(q) == v->nne, "quadrature / vector mismatch (Gauss rule)");
+        auto P = new gf_pcg_s();
+        try {
+            P->v = v;
+            P->q = q;
+            P->et = et;
+            P->nne = v->nne;
+            P->nd = v->ndim;
+            P->K = v->nne * v->ndim;
+            P->nelem = v->nelem;
+            P->nnu = nnu;
+            P->nnp = nnp;
+            P->ndof = v->ndof;
+            P->nni = nnu + nnp;
+            P->nnd = P->ndof - P->nni;
+            GF_REQUIRE(P->nnd >= 0 && nnu > 0, "sizes: ndof = nnu + nnp + nnd");
+            P->Ke.alloc(P->nelem);
+            P->Ge.alloc(P->nelem);
+            GF_HIP(hipMemcpy(P->Ke.p, K_e, P->nelem * sizeof(double), hipMemcpyDeviceToDevice));
+            GF_HIP(hipMemcpy(P->Ge.p, G_e, P->nelem * sizeof(double), hipMemcpyDeviceToDevice));
+            upload_csr(P->cdu, cdu_ptr_h, cdu_col_h, cdu_val_h, P->nnd);
+            upload_csr(P->cdp, cdp_ptr_h, cdp_col_h, cdp_val_h, P->nnd);
+            for (int64_t k = 0; k < cdu_ptr_h[P->nnd]; ++k)
+                GF_REQUIRE(cdu_col_h[k] >= 0 && cdu_col_h[k] < nnu, "Cdu column out of range");
+            for (int64_t k = 0; k < cdp_ptr_h[P->nnd]; ++k)
+                GF_REQUIRE(cdp_col_h[k] >= 0 && cdp_col_h[k] < nnp, "Cdp column out of range");
+            // transpose of C_du: for each u row the (d, coefficient) pairs
+            std::vector<int64_t> tptr(nnu + 1, 0);
+            for (int64_t k = 0; k < cdu_ptr_h[P->nnd]; ++k) tptr[cdu_col_h[k] + 1]++;
+            for (int64_t i = 0; i < nnu; ++i) tptr[i + 1] += tptr[i];
+            std::vector<int64_t> tidx(tptr[nnu]), fill(tptr.begin(), tptr.end() - 1);
+            std::vector<double> tval(tptr[nnu]);
+            for (int64_t d = 0; d < P->nnd; ++d)
+                for (int64_t k = cdu_ptr_h[d]; k < cdu_ptr_h[d + 1]; ++k) {
+                    int64_t j = fill[cdu_col_h[k]]++;
+                    tidx[j] = d;
+                    tval[j] = cdu_val_h[k];
+                }
+            upload_csr(P->cduT, tptr.data(), tidx.data(), tval.data(), nnu);
+            P->xfull.alloc(P->ndof);
+            GF_HIP(hipMemset(P->xfull.p, 0, P->ndof * sizeof(double)));
+            P->fe.alloc(P->nelem * P->K);
+            for (auto* b : {&P->invdiag, &P->x, &P->r, &P->z, &P->qv, &P->tmp}) b->alloc(nnu);
+            P->partial.alloc(4 * NBR);
+            P->scal.alloc(8);
+            P->state.alloc(4);
+            // Jacobi diagonal of K' (element diagonals, ordered gather, folded through C_du)
+            hipStream_t s = nullptr;
+            const int64_t n = P->nelem * P->nne;
+            if (et == GF_HEX8)
+                k_elem_diag<GF_HEX8><<<nblocks(n, BS), BS, 0, s>>>(P->nelem, quad_dNx(q), quad_vol(q), P->Ke.p, P->Ge.p,
+                                                                 P->fe.p);
+            else
+                k_elem_diag<GF_QUAD4_PLANAR><<<nblocks(n, BS), BS, 0, s>>>(P->nelem, quad_dNx(q), quad_vol(q), P->Ke.p,
+                                                                         P->Ge.p, P->fe.p);
+            k_diag_fold<<<nblocks(nnu, BS), BS, 0, s>>>(nnu, P->nni, v->dof_eptr.p, v->dof_eidx.p, P->fe.p,
+                                                       P->cduT.ptr.p, P->cduT.idx.p, P->cduT.val.p, P->invdiag.p);
+            check_launch();
+            GF_HIP(hipDeviceSynchronize());
+        }
+        catch (...) {
+            delete P;
+            throw;
+        }
+        *out = P;
+    });
+}
+
+int gf_pcg_destroy(gf_pcg P)
+{
+    delete P;
+    return GF_OK;
+}
+
+int gf_pcg_sizes(gf_pcg P, int64_t* nnu, int64_t* nnp, int64_t* nnd)
+{
+    return guard([&] {
+        GF_REQUIRE(P, "null solver");
+        if (nnu) *nnu = P->nnu;
+        if (nnp) *nnp = P->nnp;
+        if (nnd) *nnd = P->nnd;
+    });
+}
+
+int gf_pcg_apply(gf_pcg P, const double* x_u, const double* x_p, double* y_u, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(P && y_u, "null argument");
+        apply(P, x_u, x_p, y_u, false, S(stream), nullptr, nullptr, nullptr);
+    });
+}
+
+int gf_pcg_expand(gf_pcg P, const double* x_u, const double* x_p, double* dofval, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(P && dofval, "null argument");
+        const int64_t n = P->ndof;
+        k_expand<<<std::min<unsigned>(nblocks(n, BS), 4096), BS, 0, S(stream)>>>(
+            P->nnu, P->nnp, P->nnd, x_u, x_p, P->cdu.ptr.p, P->cdu.idx.p, P->cdu.val.p, P->cdp.ptr.p, P->cdp.idx.p,
+            P->cdp.val.p, dofval, 1, nullptr);
+        check_launch();
+    });
+}
+
+int gf_pcg_inverse_diagonal(gf_pcg P, const double** invdiag)
+{
+    return guard([&] {
+        GF_REQUIRE(P && invdiag, "null argument");
+        *invdiag = P->invdiag.p;
+    });
+}
+
+int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t maxiter, int64_t check_every,
+                 int64_t* iterations, double* relres, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(P && b_u && x_u, "null argument");
+        GF_REQUIRE(tol > 0.0 && maxiter > 0 && maxiter < INT32_MAX, "tol > 0, 0 < maxiter < 2^31");
+        if (check_every <= 0) check_every = 32;
+        hipStream_t s = S(stream);
+        const int64_t n = P->nnu;
+        // x0 -> u-part of x_full (the search direction lives there during the iteration)
+        GF_HIP(hipMemcpyAsync(P->xfull.p, x_u, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        apply(P, nullptr, nullptr, P->qv.p, true, s, nullptr, nullptr, nullptr);
+        k_pcg_init<<<NBR, BS, 0, s>>>(n, b_u, P->qv.p, P->invdiag.p, P->r.p, P->z.p, P->x.p, P->xfull.p, P->partial.p);
+        k_pcg_init_scalars<<<1, BS, 0, s>>>(P->partial.p, P->scal.p, P->state.p, tol);
+        int32_t cap = static_cast<int32_t>(maxiter);
+        GF_HIP(hipMemcpyAsync(P->state.p + 2, &cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        check_launch();
+        int32_t st[2] = {0, 0};
+        for (int64_t it = 0; it < maxiter;) {
+            for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
+                apply(P, nullptr, nullptr, P->qv.p, true, s, P->xfull.p, P->partial.p + 3 * NBR, P->state.p);
+                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->scal.p, P->xfull.p, P->qv.p,
+                                                P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p);
+                k_pcg_direction<<<NBR, BS, 0, s>>>(n, P->partial.p, P->scal.p, P->z.p, P->xfull.p, P->state.p);
+                k_pcg_count<<<1, 64, 0, s>>>(P->scal.p, P->state.p);
+            }
+            check_launch();
+            GF_HIP(hipMemcpyAsync(st, P->state.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            GF_HIP(hipStreamSynchronize(s));
+            if (st[0]) break;
+        }
+        // true residual ||b - K' x|| / ||b||
+        GF_HIP(hipMemcpyAsync(x_u, P->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        GF_HIP(hipMemcpyAsync(P->xfull.p, P->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        apply(P, nullptr, nullptr, P->qv.p, true, s, nullptr, nullptr, nullptr);
+        k_residual<<<NBR, BS, 0, s>>>(n, b_u, P->qv.p, P->partial.p);
+        k_residual_final<<<1, BS, 0, s>>>(P->partial.p, P->tmp.p);
+        double res = 0.0;
+        GF_HIP(hipMemcpyAsync(&res, P->tmp.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipMemcpyAsync(st, P->state.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipStreamSynchronize(s));
+        if (iterations) *iterations = st[1];
+        if (relres) *relres = res;
+        if (!(res <= tol * 10.0)) throw Error(GF_ENOTCONV, "PCG did not reach the tolerance (true residual " +
+                                                               std::to_string(res) + ")");
+    });
+}
+
+} // extern "C"

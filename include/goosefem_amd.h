@@ -278,6 +278,36 @@ int gf_explicit_fint(gf_explicit x, const double** fint);
 /* 1 if the structured-grid fused kernel is used (conn is a Regular Quad4 grid), else 0 */
 int gf_explicit_is_structured(gf_explicit x, int* structured);
 
+/* ================================================================================================
+ * Tied-system solver: FP64 Jacobi-preconditioned CG on the operator of MatrixPartitionedTyings
+ * (include/GooseFEM/MatrixPartitionedTyings.h:655-674):
+ *   K' = A_uu + A_ud C_du + C_du^T A_du + C_du^T A_dd C_du,   K'_up = A_up + A_ud C_dp + ...
+ * replacing the direct MatrixPartitionedTyingsSolver (Eigen SimplicialLDLT, :527-675).  A is the
+ * stiffness of a linear elastic material (per-element K, G) on the quadrature q, applied
+ * matrix-free; v carries conn and the [u, p, d] dofs of Tyings::Periodic (TyingsPeriodic.h:82-121).
+ * ============================================================================================== */
+
+/* K_e, G_e: device per-element moduli (FP64).  C_du [nnd x nnu] and C_dp [nnd x nnp] as host CSR
+ * (Tyings::Periodic::Cdu / Cdp, TyingsPeriodic.h:272-343; see gf_tyings_matrix). */
+int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const double* G_e, int64_t nnu,
+                  int64_t nnp, const int64_t* cdu_ptr_h, const int64_t* cdu_col_h, const double* cdu_val_h,
+                  const int64_t* cdp_ptr_h, const int64_t* cdp_col_h, const double* cdp_val_h);
+int gf_pcg_destroy(gf_pcg p);
+int gf_pcg_sizes(gf_pcg p, int64_t* nnu, int64_t* nnp, int64_t* nnd);
+/* y_u = K' x_u + K'_up x_p; x_u or x_p may be NULL (zero). */
+int gf_pcg_apply(gf_pcg p, const double* x_u, const double* x_p, double* y_u, void* stream);
+/* full dofval [x_u, x_p, C_du x_u + C_dp x_p] (the x_d = Cdu X_u + Cdp X_p step of
+ * MatrixPartitionedTyingsSolver::solve, MatrixPartitionedTyings.h:547-551) */
+int gf_pcg_expand(gf_pcg p, const double* x_u, const double* x_p, double* dofval, void* stream);
+/* device pointer to 1 / diag(K') [nnu] (Jacobi preconditioner) */
+int gf_pcg_inverse_diagonal(gf_pcg p, const double** invdiag);
+/* Solve K' x_u = b_u.  x_u: initial guess in, solution out.  Stops when the recursive residual
+ * ||r|| <= tol ||b||; *relres receives the true residual ||b - K' x|| / ||b|| recomputed in FP64,
+ * *iterations the CG iterations.  The host checks the device convergence flag every check_every
+ * iterations (<= 0: 32).  Returns GF_ENOTCONV if the true residual exceeds 10 tol. */
+int gf_pcg_solve(gf_pcg p, const double* b_u, double* x_u, double tol, int64_t maxiter, int64_t check_every,
+                 int64_t* iterations, double* relres, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

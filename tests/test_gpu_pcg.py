@@ -1,0 +1,108 @@
+"""Tied-system PCG on the GPU: the matrix-free K' and K'_up of MatrixPartitionedTyings against a
+dense restatement built from the oracle's element stiffness, the reordered [u, p, d] dofs and the
+tying matrices; the solve against a direct dense solve with the true residual as judge."""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import HEX8, QUAD4_PLANAR
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gf():
+    import goosefem_amd
+
+    goosefem_amd.lib()
+    return goosefem_amd
+
+
+def host(t):
+    import torch
+
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def periodic_system(gf, et, n, seed=1):
+    if et == HEX8:
+        mesh = gf.Mesh.Hex8.Regular(n, n, n)
+    else:
+        mesh = gf.Mesh.Quad4.Regular(n + 2, n)
+    coor, conn = mesh.coor, mesh.conn
+    ctrl = gf.Tyings.Control(coor, mesh.dofs)
+    iip = np.concatenate((ctrl.controlDofs.ravel(), ctrl.dofs[mesh.nodesOrigin]))
+    ty = gf.Tyings.Periodic(ctrl.coor, ctrl.dofs, ctrl.controlDofs, mesh.nodesPeriodic, iip)
+    vec = gf.Vector(conn, ty.dofs)
+    nd = coor.shape[1]
+    ov = oracle.Vector(conn, oracle.dofs(coor.shape[0], nd))
+    x = ov.AsElement(coor)
+    oq = oracle.Quadrature(et, x)
+    quad = (gf.Element.Hex8.Quadrature(x) if et == HEX8 else gf.Element.Quad4.QuadraturePlanar(x))
+    rng = np.random.default_rng(seed)
+    G = np.exp(0.4 * rng.standard_normal(oq.nelem))
+    K = np.full(oq.nelem, 10.0)
+    P = gf.TyingsPCG(vec, quad, K, G, ty)
+    # dense reference: A in the [u, p, d] numbering, then the reference's condensation
+    C = oracle.elastic_tangent(np.repeat(K[:, None], oq.nip, 1), np.repeat(G[:, None], oq.nip, 1))
+    Ke = oq.Int_gradN_dot_tensor4_dot_gradNT_dV(C)
+    ndof = ty.nni + ty.nnd
+    A = np.zeros((ndof, ndof))
+    for e in range(oq.nelem):
+        d = ty.dofs[conn[e]].ravel()
+        A[np.ix_(d, d)] += Ke[e]
+    u, p, dd = slice(0, ty.nnu), slice(ty.nnu, ty.nni), slice(ty.nni, ndof)
+    Cdu, Cdp = ty.Cdu, ty.Cdp
+    Kp = A[u, u] + A[u, dd] @ Cdu + Cdu.T @ A[dd, u] + Cdu.T @ A[dd, dd] @ Cdu
+    Kup = A[u, p] + A[u, dd] @ Cdp + Cdu.T @ A[dd, p] + Cdu.T @ A[dd, dd] @ Cdp
+    return mesh, ty, P, Kp, Kup
+
+
+@pytest.mark.parametrize("et,n", [(QUAD4_PLANAR, 6), (HEX8, 3)])
+def test_operator_matches_dense_condensation(gf, et, n):
+    mesh, ty, P, Kp, Kup = periodic_system(gf, et, n)
+    rng = np.random.default_rng(2)
+    xu = rng.standard_normal(ty.nnu)
+    xp = rng.standard_normal(ty.nnp)
+    y = host(P.apply(xu))
+    assert np.linalg.norm(y - Kp @ xu) / np.linalg.norm(Kp @ xu) < 1e-13
+    y = host(P.apply(None, xp))
+    assert np.linalg.norm(y - Kup @ xp) / np.linalg.norm(Kup @ xp) < 1e-13
+    assert np.allclose(host(P.inverse_diagonal), 1.0 / np.diag(Kp), rtol=1e-13)
+    full = host(P.expand(xu, xp))
+    assert np.array_equal(full[: ty.nnu], xu) and np.array_equal(full[ty.nnu:ty.nni], xp)
+    assert np.allclose(full[ty.nni:], ty.Cdu @ xu + ty.Cdp @ xp, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("et,n", [(QUAD4_PLANAR, 10), (HEX8, 4)])
+def test_solve_matches_direct_and_true_residual(gf, et, n):
+    mesh, ty, P, Kp, Kup = periodic_system(gf, et, n)
+    xp = np.zeros(ty.nnp)
+    xp[ty.control[0, 1] - ty.nnu] = 1e-3  # simple shear increment through ctrl(0, 1)
+    b = -host(P.apply(None, xp))
+    assert np.allclose(b, -Kup @ xp, rtol=1e-12, atol=1e-18)
+    x, it, res = P.solve(b, tol=1e-10)
+    x = host(x)
+    xd = np.linalg.solve(Kp, b)
+    assert res <= 1e-10 * 10
+    assert np.linalg.norm(b - Kp @ x) / np.linalg.norm(b) < 1e-9
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7
+    assert 0 < it < 10 * ty.nnu
+    # warm start from the solution: converged at once
+    x2, it2, res2 = P.solve(b, x0=xd, tol=1e-10)
+    assert it2 == 0 and res2 < 1e-10
+    # deterministic: a second solve gives identical bits
+    x3, it3, _ = P.solve(b, tol=1e-10)
+    assert it3 == it and np.array_equal(host(x3), x)
+
+
+def test_iteration_cap_reports_nonconvergence(gf):
+    mesh, ty, P, Kp, Kup = periodic_system(gf, QUAD4_PLANAR, 10)
+    b = np.random.default_rng(0).standard_normal(ty.nnu)
+    with pytest.raises(gf.GooseFEMError) as e:
+        P.solve(b, tol=1e-12, maxiter=3)
+    assert e.value.code == gf.GF_ENOTCONV
+    x, it, res = P.solve(b, tol=1e-12, maxiter=3, raise_on_fail=False)
+    assert it == 3 and res > 1e-12
