@@ -278,6 +278,43 @@ int gf_stream_sync(void* stream)
     return guard([&] { GF_HIP(hipStreamSynchronize(S(stream))); });
 }
 
+int gf_device_malloc(void** ptr, int64_t bytes)
+{
+    return guard([&] {
+        GF_REQUIRE(ptr && bytes >= 0, "invalid argument");
+        *ptr = nullptr;
+        if (bytes) GF_HIP(hipMalloc(ptr, static_cast<size_t>(bytes)));
+    });
+}
+
+int gf_device_free(void* ptr)
+{
+    return guard([&] {
+        if (ptr) GF_HIP(hipFree(ptr));
+    });
+}
+
+int gf_memcpy_h2d(void* dst, const void* src, int64_t bytes)
+{
+    return guard([&] {
+        if (bytes) GF_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice));
+    });
+}
+
+int gf_memcpy_d2h(void* dst, const void* src, int64_t bytes)
+{
+    return guard([&] {
+        if (bytes) GF_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost));
+    });
+}
+
+int gf_memcpy_d2d(void* dst, const void* src, int64_t bytes)
+{
+    return guard([&] {
+        if (bytes) GF_HIP(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToDevice));
+    });
+}
+
 int gf_mesh_quad4_regular(int64_t nx, int64_t ny, double h, double* coor, int64_t* conn)
 {
     return guard([&] {

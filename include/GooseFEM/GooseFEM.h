@@ -1,0 +1,1033 @@
+/*
+ * GooseFEM/GooseFEM.h -- header-only C++ face of goosefem_amd: the class names and call forms of
+ * the reference's C++ API (the include/GooseFEM headers of tdegeus/GooseFEM) for the element-quadrature and
+ * assembly path, forwarded to the C ABI in goosefem_amd.h (link with -lgoosefem_amd).
+ *
+ * Containers: any array with data(), size(), shape() / shape(i) and a row-major layout is accepted
+ * (xt::xtensor, xt::pytensor, GooseFEM::array_type::tensor below).  Results are returned as
+ * GooseFEM::array_type::tensor.  Host containers are copied to the GPU for the call and back, as the
+ * reference's value semantics require; GooseFEM::amd::device_array arguments stay in HBM and
+ * results can be written into them without host traffic (the lower-case write-into forms).
+ * Errors: every nonzero status of the C ABI throws std::runtime_error with the library's message
+ * (the reference's GOOSEFEM_CHECK behaviour, include/GooseFEM/config.h:65-98).
+ */
+#ifndef GOOSEFEM_AMD_GOOSEFEM_H
+#define GOOSEFEM_AMD_GOOSEFEM_H
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <initializer_list>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../goosefem_amd.h"
+
+namespace GooseFEM {
+
+namespace amd {
+
+inline void check(int rc)
+{
+    if (rc != GF_OK) throw std::runtime_error(std::string("goosefem_amd: ") + gf_last_error());
+}
+
+/* Owning row-major host array with xtensor-like accessors. */
+template <class T>
+class array {
+public:
+    array() = default;
+    explicit array(std::vector<size_t> shape, T fill = T(0)) : m_shape(std::move(shape)), m_data(count(), fill) {}
+    array(std::initializer_list<size_t> shape) : m_shape(shape), m_data(count(), T(0)) {}
+    static array from_shape(std::vector<size_t> shape) { return array(std::move(shape)); }
+    T* data() { return m_data.data(); }
+    const T* data() const { return m_data.data(); }
+    size_t size() const { return m_data.size(); }
+    size_t dimension() const { return m_shape.size(); }
+    const std::vector<size_t>& shape() const { return m_shape; }
+    size_t shape(size_t i) const { return m_shape.at(i); }
+    template <class... I>
+    T& operator()(I... idx) { return m_data[offset({static_cast<size_t>(idx)...})]; }
+    template <class... I>
+    const T& operator()(I... idx) const { return m_data[offset({static_cast<size_t>(idx)...})]; }
+    T& operator[](size_t i) { return m_data[i]; }
+    const T& operator[](size_t i) const { return m_data[i]; }
+    typename std::vector<T>::iterator begin() { return m_data.begin(); }
+    typename std::vector<T>::iterator end() { return m_data.end(); }
+    typename std::vector<T>::const_iterator begin() const { return m_data.begin(); }
+    typename std::vector<T>::const_iterator end() const { return m_data.end(); }
+    void fill(T v) { std::fill(m_data.begin(), m_data.end(), v); }
+
+private:
+    size_t count() const
+    {
+        return std::accumulate(m_shape.begin(), m_shape.end(), size_t(1), std::multiplies<size_t>());
+    }
+    size_t offset(std::initializer_list<size_t> idx) const
+    {
+        size_t o = 0, d = 0;
+        for (size_t i : idx) o = o * m_shape[d++] + i;
+        return o;
+    }
+    std::vector<size_t> m_shape;
+    std::vector<T> m_data;
+};
+
+/* Owning device (HBM) array. */
+template <class T>
+class device_array {
+public:
+    device_array() = default;
+    explicit device_array(std::vector<size_t> shape) : m_shape(std::move(shape)) { allocate(); }
+    template <class H>
+    static device_array from_host(const H& h)
+    {
+        std::vector<size_t> s;
+        for (size_t i = 0; i < h.dimension(); ++i) s.push_back(h.shape()[i]);
+        device_array d(s);
+        check(gf_memcpy_h2d(d.m_ptr, h.data(), static_cast<int64_t>(d.bytes())));
+        return d;
+    }
+    device_array(const device_array&) = delete;
+    device_array& operator=(const device_array&) = delete;
+    device_array(device_array&& o) noexcept : m_shape(std::move(o.m_shape)), m_ptr(o.m_ptr) { o.m_ptr = nullptr; }
+    device_array& operator=(device_array&& o) noexcept
+    {
+        release();
+        m_shape = std::move(o.m_shape);
+        m_ptr = o.m_ptr;
+        o.m_ptr = nullptr;
+        return *this;
+    }
+    ~device_array() { release(); }
+    T* data() { return m_ptr; }
+    const T* data() const { return m_ptr; }
+    size_t size() const
+    {
+        return std::accumulate(m_shape.begin(), m_shape.end(), size_t(1), std::multiplies<size_t>());
+    }
+    size_t bytes() const { return size() * sizeof(T); }
+    size_t dimension() const { return m_shape.size(); }
+    const std::vector<size_t>& shape() const { return m_shape; }
+    size_t shape(size_t i) const { return m_shape.at(i); }
+    array<T> to_host() const
+    {
+        array<T> h(m_shape);
+        check(gf_memcpy_d2h(h.data(), m_ptr, static_cast<int64_t>(bytes())));
+        return h;
+    }
+
+private:
+    void allocate()
+    {
+        void* p = nullptr;
+        check(gf_device_malloc(&p, static_cast<int64_t>(bytes())));
+        m_ptr = static_cast<T*>(p);
+    }
+    void release()
+    {
+        if (m_ptr) gf_device_free(m_ptr);
+        m_ptr = nullptr;
+    }
+    std::vector<size_t> m_shape;
+    T* m_ptr = nullptr;
+};
+
+template <class C>
+struct is_device : std::false_type {};
+template <class T>
+struct is_device<device_array<T>> : std::true_type {};
+
+template <class C>
+std::vector<size_t> shape_of(const C& c)
+{
+    std::vector<size_t> s;
+    for (size_t i = 0; i < c.dimension(); ++i) s.push_back(static_cast<size_t>(c.shape()[i]));
+    return s;
+}
+
+/* Device view of an argument: no copy for device arrays, an upload for host containers. */
+template <class C>
+struct dev_in {
+    device_array<double> tmp;
+    const double* ptr = nullptr;
+    explicit dev_in(const C& c)
+    {
+        if constexpr (is_device<C>::value) {
+            ptr = c.data();
+        }
+        else {
+            static_assert(std::is_same<typename std::decay<decltype(*c.data())>::type, double>::value,
+                          "host containers must hold double");
+            tmp = device_array<double>(shape_of(c));
+            check(gf_memcpy_h2d(tmp.data(), c.data(), static_cast<int64_t>(tmp.bytes())));
+            ptr = tmp.data();
+        }
+    }
+};
+
+/* Output for a write-into call: device arrays are written directly, host containers via a
+ * device temporary copied back in finish(). */
+template <class R>
+struct dev_out {
+    device_array<double> tmp;
+    double* ptr = nullptr;
+    R& ret;
+    dev_out(R& r, const std::vector<size_t>& shape) : ret(r)
+    {
+        if constexpr (is_device<R>::value) {
+            ptr = r.data();
+        }
+        else {
+            tmp = device_array<double>(shape);
+            ptr = tmp.data();
+        }
+    }
+    void finish()
+    {
+        if constexpr (!is_device<R>::value) check(gf_memcpy_d2h(ret.data(), tmp.data(), static_cast<int64_t>(tmp.bytes())));
+        else check(gf_stream_sync(nullptr));
+    }
+};
+
+template <class C>
+std::vector<int64_t> to_i64(const C& c)
+{
+    std::vector<int64_t> r(c.size());
+    const auto* p = c.data();
+    for (size_t i = 0; i < r.size(); ++i) r[i] = static_cast<int64_t>(p[i]);
+    return r;
+}
+
+inline array<size_t> from_i64(const std::vector<int64_t>& v, std::vector<size_t> shape)
+{
+    array<size_t> r(std::move(shape));
+    for (size_t i = 0; i < v.size(); ++i) r[i] = static_cast<size_t>(v[i]);
+    return r;
+}
+
+} // namespace amd
+
+namespace array_type {
+template <class T, size_t N>
+using tensor = amd::array<T>;
+}
+
+// =================================================================================================
+// Mesh (include/GooseFEM/Mesh.h, MeshQuad4.h, MeshHex8.h)
+// =================================================================================================
+namespace Mesh {
+
+inline amd::array<size_t> dofs(size_t nnode, size_t ndim)
+{
+    amd::array<size_t> r({nnode, ndim});
+    for (size_t i = 0; i < r.size(); ++i) r[i] = i;
+    return r;
+}
+
+template <class T>
+inline amd::array<size_t> renumber(const T& dofs)
+{
+    auto in = amd::to_i64(dofs);
+    std::vector<int64_t> out(in.size());
+    amd::check(gf_mesh_renumber(in.data(), static_cast<int64_t>(in.size()), out.data()));
+    return amd::from_i64(out, amd::shape_of(dofs));
+}
+
+template <class T>
+inline amd::array<size_t> coordination(const T& conn)
+{
+    auto c = amd::to_i64(conn);
+    int64_t nv = c.empty() ? 0 : *std::max_element(c.begin(), c.end()) + 1;
+    std::vector<int64_t> out(static_cast<size_t>(nv));
+    amd::check(gf_mesh_coordination(c.data(), static_cast<int64_t>(c.size()), nv, out.data()));
+    return amd::from_i64(out, {static_cast<size_t>(nv)});
+}
+
+namespace detail {
+inline std::vector<std::vector<size_t>> csr_lists(const std::vector<int64_t>& ptr, const std::vector<int64_t>& idx)
+{
+    std::vector<std::vector<size_t>> r(ptr.size() - 1);
+    for (size_t i = 0; i + 1 < ptr.size(); ++i)
+        for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) r[i].push_back(static_cast<size_t>(idx[k]));
+    return r;
+}
+} // namespace detail
+
+/* Mesh::elem2node(conn[, dofs])  (Mesh.h:2485-2551) */
+template <class E>
+inline std::vector<std::vector<size_t>> elem2node(const E& conn)
+{
+    auto c = amd::to_i64(conn);
+    int64_t nnode = c.empty() ? 0 : *std::max_element(c.begin(), c.end()) + 1, nnz = 0;
+    std::vector<int64_t> ptr(nnode + 1);
+    const int nne = static_cast<int>(conn.shape()[1]);
+    const int64_t ne = static_cast<int64_t>(conn.shape()[0]);
+    amd::check(gf_mesh_elem2node(c.data(), ne, nne, nullptr, nnode, 1, ptr.data(), nullptr, &nnz));
+    std::vector<int64_t> idx(nnz);
+    amd::check(gf_mesh_elem2node(c.data(), ne, nne, nullptr, nnode, 1, ptr.data(), idx.data(), &nnz));
+    return detail::csr_lists(ptr, idx);
+}
+
+template <class E, class D>
+inline std::vector<std::vector<size_t>> elem2node(const E& conn, const D& dofs)
+{
+    auto c = amd::to_i64(conn);
+    auto d = amd::to_i64(dofs);
+    const int64_t nnode = static_cast<int64_t>(dofs.shape()[0]);
+    const int ndim = static_cast<int>(dofs.shape()[1]);
+    const int nne = static_cast<int>(conn.shape()[1]);
+    const int64_t ne = static_cast<int64_t>(conn.shape()[0]);
+    int64_t nnz = 0;
+    std::vector<int64_t> ptr(nnode + 1);
+    amd::check(gf_mesh_elem2node(c.data(), ne, nne, d.data(), nnode, ndim, ptr.data(), nullptr, &nnz));
+    std::vector<int64_t> idx(nnz);
+    amd::check(gf_mesh_elem2node(c.data(), ne, nne, d.data(), nnode, ndim, ptr.data(), idx.data(), &nnz));
+    return detail::csr_lists(ptr, idx);
+}
+
+/* Mesh::nodaltyings(dofs)  (Mesh.h:2839-2861) */
+template <class D>
+inline std::vector<std::vector<size_t>> nodaltyings(const D& dofs)
+{
+    auto d = amd::to_i64(dofs);
+    const int64_t nnode = static_cast<int64_t>(dofs.shape()[0]);
+    const int ndim = static_cast<int>(dofs.shape()[1]);
+    int64_t nnz = 0;
+    std::vector<int64_t> ptr(nnode + 1);
+    amd::check(gf_mesh_nodaltyings(d.data(), nnode, ndim, ptr.data(), nullptr, &nnz));
+    std::vector<int64_t> idx(nnz);
+    amd::check(gf_mesh_nodaltyings(d.data(), nnode, ndim, ptr.data(), idx.data(), &nnz));
+    return detail::csr_lists(ptr, idx);
+}
+
+namespace detail {
+
+class RegularBase {
+public:
+    size_t nelem() const { return m_nelem; }
+    size_t nnode() const { return m_nnode; }
+    size_t nne() const { return m_nne; }
+    size_t ndim() const { return m_ndim; }
+    size_t nelx() const { return static_cast<size_t>(m_nx); }
+    size_t nely() const { return static_cast<size_t>(m_ny); }
+    double h() const { return m_h; }
+    amd::array<size_t> dofs() const { return Mesh::dofs(m_nnode, m_ndim); }
+    amd::array<size_t> nodesPeriodic() const
+    {
+        int64_t n = 0;
+        amd::check(gf_mesh_nodes_periodic(m_nx, m_ny, m_nz, nullptr, &n));
+        std::vector<int64_t> r(static_cast<size_t>(2 * n));
+        amd::check(gf_mesh_nodes_periodic(m_nx, m_ny, m_nz, r.data(), &n));
+        return amd::from_i64(r, {static_cast<size_t>(n), 2});
+    }
+    amd::array<size_t> dofsPeriodic() const
+    {
+        auto t = amd::to_i64(nodesPeriodic());
+        std::vector<int64_t> r(m_nnode * m_ndim);
+        amd::check(gf_mesh_dofs_periodic(static_cast<int64_t>(m_nnode), static_cast<int>(m_ndim), t.data(),
+                                         static_cast<int64_t>(t.size() / 2), r.data()));
+        return amd::from_i64(r, {m_nnode, m_ndim});
+    }
+    /* any named node set of the reference (e.g. "nodesBottomEdge", "nodesFrontFace") */
+    amd::array<size_t> nodeset(const std::string& name) const
+    {
+        int64_t n = 0;
+        amd::check(gf_mesh_nodeset(m_nx, m_ny, m_nz, name.c_str(), nullptr, &n));
+        std::vector<int64_t> r(static_cast<size_t>(n));
+        amd::check(gf_mesh_nodeset(m_nx, m_ny, m_nz, name.c_str(), r.data(), &n));
+        return amd::from_i64(r, {static_cast<size_t>(n)});
+    }
+    size_t node(const std::string& name) const { return nodeset(name)[0]; }
+    size_t nodesOrigin() const { return node("nodesOrigin"); }
+
+protected:
+    int64_t m_nx = 0, m_ny = 0, m_nz = 0;
+    double m_h = 1.0;
+    size_t m_nelem = 0, m_nnode = 0, m_nne = 0, m_ndim = 0;
+};
+
+} // namespace detail
+
+namespace Quad4 {
+
+/* Mesh::Quad4::Regular  (include/GooseFEM/MeshQuad4.h:32-195) */
+class Regular : public detail::RegularBase {
+public:
+    Regular(size_t nelx, size_t nely, double h = 1.0)
+    {
+        m_nx = static_cast<int64_t>(nelx);
+        m_ny = static_cast<int64_t>(nely);
+        m_nz = 0;
+        m_h = h;
+        m_nne = 4;
+        m_ndim = 2;
+        m_nelem = nelx * nely;
+        m_nnode = (nelx + 1) * (nely + 1);
+    }
+    amd::array<double> coor() const
+    {
+        amd::array<double> c({m_nnode, 2});
+        amd::check(gf_mesh_quad4_regular(m_nx, m_ny, m_h, c.data(), nullptr));
+        return c;
+    }
+    amd::array<size_t> conn() const
+    {
+        std::vector<int64_t> c(m_nelem * 4);
+        amd::check(gf_mesh_quad4_regular(m_nx, m_ny, m_h, nullptr, c.data()));
+        return amd::from_i64(c, {m_nelem, 4});
+    }
+    amd::array<size_t> nodesBottomEdge() const { return nodeset("nodesBottomEdge"); }
+    amd::array<size_t> nodesTopEdge() const { return nodeset("nodesTopEdge"); }
+    amd::array<size_t> nodesLeftEdge() const { return nodeset("nodesLeftEdge"); }
+    amd::array<size_t> nodesRightEdge() const { return nodeset("nodesRightEdge"); }
+    amd::array<size_t> nodesBottomOpenEdge() const { return nodeset("nodesBottomOpenEdge"); }
+    amd::array<size_t> nodesTopOpenEdge() const { return nodeset("nodesTopOpenEdge"); }
+    amd::array<size_t> nodesLeftOpenEdge() const { return nodeset("nodesLeftOpenEdge"); }
+    amd::array<size_t> nodesRightOpenEdge() const { return nodeset("nodesRightOpenEdge"); }
+    size_t nodesBottomLeftCorner() const { return node("nodesBottomLeftCorner"); }
+    size_t nodesBottomRightCorner() const { return node("nodesBottomRightCorner"); }
+    size_t nodesTopLeftCorner() const { return node("nodesTopLeftCorner"); }
+    size_t nodesTopRightCorner() const { return node("nodesTopRightCorner"); }
+};
+
+} // namespace Quad4
+
+namespace Hex8 {
+
+/* Mesh::Hex8::Regular  (include/GooseFEM/MeshHex8.h:26-609); node sets via nodeset(name) */
+class Regular : public detail::RegularBase {
+public:
+    Regular(size_t nelx, size_t nely, size_t nelz, double h = 1.0)
+    {
+        m_nx = static_cast<int64_t>(nelx);
+        m_ny = static_cast<int64_t>(nely);
+        m_nz = static_cast<int64_t>(nelz);
+        m_h = h;
+        m_nne = 8;
+        m_ndim = 3;
+        m_nelem = nelx * nely * nelz;
+        m_nnode = (nelx + 1) * (nely + 1) * (nelz + 1);
+    }
+    size_t nelz() const { return static_cast<size_t>(m_nz); }
+    amd::array<double> coor() const
+    {
+        amd::array<double> c({m_nnode, 3});
+        amd::check(gf_mesh_hex8_regular(m_nx, m_ny, m_nz, m_h, c.data(), nullptr));
+        return c;
+    }
+    amd::array<size_t> conn() const
+    {
+        std::vector<int64_t> c(m_nelem * 8);
+        amd::check(gf_mesh_hex8_regular(m_nx, m_ny, m_nz, m_h, nullptr, c.data()));
+        return amd::from_i64(c, {m_nelem, 8});
+    }
+};
+
+} // namespace Hex8
+} // namespace Mesh
+
+// =================================================================================================
+// Vector (include/GooseFEM/Vector.h)
+// =================================================================================================
+
+class Vector {
+public:
+    template <class S, class T>
+    Vector(const S& conn, const T& dofs)
+    {
+        auto c = amd::to_i64(conn);
+        auto d = amd::to_i64(dofs);
+        m_nelem = conn.shape()[0];
+        m_nne = conn.shape()[1];
+        m_nnode = dofs.shape()[0];
+        m_ndim = dofs.shape()[1];
+        amd::check(gf_vector_create(&m_h, c.data(), static_cast<int64_t>(m_nelem), static_cast<int>(m_nne), d.data(),
+                                    static_cast<int64_t>(m_nnode), static_cast<int>(m_ndim)));
+        int64_t nd = 0;
+        amd::check(gf_vector_sizes(m_h, nullptr, nullptr, nullptr, nullptr, &nd));
+        m_ndof = static_cast<size_t>(nd);
+    }
+    Vector(const Vector&) = delete;
+    Vector& operator=(const Vector&) = delete;
+    ~Vector() { gf_vector_destroy(m_h); }
+
+    size_t nelem() const { return m_nelem; }
+    size_t nne() const { return m_nne; }
+    size_t nnode() const { return m_nnode; }
+    size_t ndim() const { return m_ndim; }
+    size_t ndof() const { return m_ndof; }
+    std::vector<size_t> shape_elemvec() const { return {m_nelem, m_nne, m_ndim}; }
+    std::vector<size_t> shape_nodevec() const { return {m_nnode, m_ndim}; }
+    std::vector<size_t> shape_dofval() const { return {m_ndof}; }
+    amd::array<double> allocate_elemvec(double v = 0.0) const { return amd::array<double>(shape_elemvec(), v); }
+    amd::array<double> allocate_nodevec(double v = 0.0) const { return amd::array<double>(shape_nodevec(), v); }
+    amd::array<double> allocate_dofval(double v = 0.0) const { return amd::array<double>(shape_dofval(), v); }
+    gf_vector handle() const { return m_h; }
+
+    /* AsElement(dofval | nodevec)  (Vector.h:193-211) */
+    template <class T>
+    amd::array<double> AsElement(const T& arg) const
+    {
+        amd::array<double> r(shape_elemvec());
+        asElement(arg, r);
+        return r;
+    }
+    template <class T, class R>
+    void asElement(const T& arg, R& ret) const
+    {
+        amd::dev_in<T> in(arg);
+        amd::dev_out<R> out(ret, shape_elemvec());
+        if (arg.dimension() == 1) amd::check(gf_vector_as_element_dofval(m_h, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_vector_as_element_nodevec(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+    }
+    /* AsDofs(nodevec | elemvec)  (Vector.h:141-159) */
+    template <class T>
+    amd::array<double> AsDofs(const T& arg) const
+    {
+        amd::array<double> r(shape_dofval());
+        asDofs(arg, r);
+        return r;
+    }
+    template <class T, class R>
+    void asDofs(const T& arg, R& ret) const
+    {
+        amd::dev_in<T> in(arg);
+        amd::dev_out<R> out(ret, shape_dofval());
+        if (arg.dimension() == 2) amd::check(gf_vector_as_dofs_nodevec(m_h, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_vector_as_dofs_elemvec(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+    }
+    /* AsNode(dofval | elemvec)  (Vector.h:167-185) */
+    template <class T>
+    amd::array<double> AsNode(const T& arg) const
+    {
+        amd::array<double> r(shape_nodevec());
+        asNode(arg, r);
+        return r;
+    }
+    template <class T, class R>
+    void asNode(const T& arg, R& ret) const
+    {
+        amd::dev_in<T> in(arg);
+        amd::dev_out<R> out(ret, shape_nodevec());
+        if (arg.dimension() == 1) amd::check(gf_vector_as_node_dofval(m_h, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_vector_as_node_elemvec(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+    }
+    /* AssembleDofs(nodevec | elemvec)  (Vector.h:219-237) */
+    template <class T>
+    amd::array<double> AssembleDofs(const T& arg) const
+    {
+        amd::array<double> r(shape_dofval());
+        assembleDofs(arg, r);
+        return r;
+    }
+    template <class T, class R>
+    void assembleDofs(const T& arg, R& ret) const
+    {
+        amd::dev_in<T> in(arg);
+        amd::dev_out<R> out(ret, shape_dofval());
+        if (arg.dimension() == 2) amd::check(gf_vector_assemble_dofs_nodevec(m_h, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_vector_assemble_dofs_elemvec(m_h, in.ptr, GF_F64, out.ptr, nullptr));
+        out.finish();
+    }
+    /* AssembleNode(elemvec)  (Vector.h:245-263) */
+    template <class T>
+    amd::array<double> AssembleNode(const T& arg) const
+    {
+        amd::array<double> r(shape_nodevec());
+        assembleNode(arg, r);
+        return r;
+    }
+    template <class T, class R>
+    void assembleNode(const T& arg, R& ret) const
+    {
+        amd::dev_in<T> in(arg);
+        amd::dev_out<R> out(ret, shape_nodevec());
+        amd::check(gf_vector_assemble_node_elemvec(m_h, in.ptr, GF_F64, out.ptr, nullptr));
+        out.finish();
+    }
+
+private:
+    gf_vector m_h = nullptr;
+    size_t m_nelem = 0, m_nne = 0, m_nnode = 0, m_ndim = 0, m_ndof = 0;
+};
+
+// =================================================================================================
+// Element quadrature (include/GooseFEM/Element.h, ElementQuad4.h, ElementQuad4Planar.h, ElementHex8.h)
+// =================================================================================================
+namespace Element {
+namespace detail {
+
+inline amd::array<double> rule(size_t ndim, bool nodal, bool weights)
+{
+    const double a = nodal ? 1.0 : 1.0 / std::sqrt(3.0);
+    const size_t nip = ndim == 2 ? 4 : 8;
+    static const int s2[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+    static const int s3[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                 {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+    if (weights) return amd::array<double>({nip}, 1.0);
+    amd::array<double> xi({nip, ndim});
+    for (size_t q = 0; q < nip; ++q)
+        for (size_t d = 0; d < ndim; ++d) xi(q, d) = (ndim == 2 ? s2[q][d] : s3[q][d]) < 0 ? -a : a;
+    return xi;
+}
+
+class QuadratureBase {
+public:
+    QuadratureBase(const QuadratureBase&) = delete;
+    QuadratureBase& operator=(const QuadratureBase&) = delete;
+    ~QuadratureBase() { gf_quad_destroy(m_h); }
+    size_t nelem() const { return m_nelem; }
+    size_t nne() const { return m_nne; }
+    size_t ndim() const { return m_ndim; }
+    size_t tdim() const { return m_tdim; }
+    size_t nip() const { return m_nip; }
+    gf_quad handle() const { return m_h; }
+    std::vector<size_t> shape_elemvec() const { return {m_nelem, m_nne, m_ndim}; }
+    std::vector<size_t> shape_elemmat() const { return {m_nelem, m_nne * m_ndim, m_nne * m_ndim}; }
+    std::vector<size_t> shape_qscalar() const { return {m_nelem, m_nip}; }
+    std::vector<size_t> shape_qtensor2() const { return {m_nelem, m_nip, m_tdim, m_tdim}; }
+    std::vector<size_t> shape_qtensor4() const { return {m_nelem, m_nip, m_tdim, m_tdim, m_tdim, m_tdim}; }
+
+    /* update_x(x)  (Element.h:563-569) */
+    template <class T>
+    void update_x(const T& x)
+    {
+        amd::dev_in<T> in(x);
+        amd::check(gf_quad_update_x(m_h, in.ptr, nullptr));
+        m_synced = false;
+    }
+    /* GradN() / dV()  (Element.h:575-587): const references, refreshed from the device on demand */
+    const amd::array<double>& GradN() const
+    {
+        sync();
+        return m_dNx;
+    }
+    const amd::array<double>& dV() const
+    {
+        sync();
+        return m_vol;
+    }
+    /* InterpQuad_vector  (Element.h:597-616) */
+    template <class T>
+    amd::array<double> InterpQuad_vector(const T& elemvec) const
+    {
+        const size_t n = elemvec.shape()[2];
+        amd::array<double> r({m_nelem, m_nip, n});
+        amd::dev_in<T> in(elemvec);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_quad_interp_quad_vector(m_h, in.ptr, static_cast<int>(n), out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+#define GOOSEFEM_AMD_QT2(Name, name, fn)                                                                    \
+    template <class T>                                                                                     \
+    amd::array<double> Name(const T& elemvec) const                                                        \
+    {                                                                                                      \
+        amd::array<double> r(shape_qtensor2());                                                            \
+        name(elemvec, r);                                                                                  \
+        return r;                                                                                          \
+    }                                                                                                      \
+    template <class T, class R>                                                                            \
+    void name(const T& elemvec, R& qtensor) const                                                          \
+    {                                                                                                      \
+        amd::dev_in<T> in(elemvec);                                                                        \
+        amd::dev_out<R> out(qtensor, shape_qtensor2());                                                    \
+        amd::check(fn(m_h, in.ptr, out.ptr, GF_F64, nullptr));                                             \
+        out.finish();                                                                                      \
+    }
+    /* GradN_vector / GradN_vector_T / SymGradN_vector  (Element.h:634-717) */
+    GOOSEFEM_AMD_QT2(GradN_vector, gradN_vector, gf_quad_gradN_vector)
+    GOOSEFEM_AMD_QT2(GradN_vector_T, gradN_vector_T, gf_quad_gradN_vector_T)
+    GOOSEFEM_AMD_QT2(SymGradN_vector, symGradN_vector, gf_quad_symgradN_vector)
+#undef GOOSEFEM_AMD_QT2
+    /* Int_N_vector_dV  (Element.h:731-750) */
+    template <class T>
+    amd::array<double> Int_N_vector_dV(const T& qvector) const
+    {
+        const size_t n = qvector.shape()[2];
+        amd::array<double> r({m_nelem, m_nne, n});
+        amd::dev_in<T> in(qvector);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_quad_int_N_vector_dV(m_h, in.ptr, static_cast<int>(n), out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    /* Int_N_scalar_NT_dV  (Element.h:770-788) */
+    template <class T>
+    amd::array<double> Int_N_scalar_NT_dV(const T& qscalar) const
+    {
+        amd::array<double> r(shape_elemmat());
+        int_N_scalar_NT_dV(qscalar, r);
+        return r;
+    }
+    template <class T, class R>
+    void int_N_scalar_NT_dV(const T& qscalar, R& elemmat) const
+    {
+        amd::dev_in<T> in(qscalar);
+        amd::dev_out<R> out(elemmat, shape_elemmat());
+        amd::check(gf_quad_int_N_scalar_NT_dV(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+    }
+    /* Int_gradN_dot_tensor2_dV  (Element.h:807-825) */
+    template <class T>
+    amd::array<double> Int_gradN_dot_tensor2_dV(const T& qtensor) const
+    {
+        amd::array<double> r(shape_elemvec());
+        int_gradN_dot_tensor2_dV(qtensor, r);
+        return r;
+    }
+    template <class T, class R>
+    void int_gradN_dot_tensor2_dV(const T& qtensor, R& elemvec) const
+    {
+        amd::dev_in<T> in(qtensor);
+        amd::dev_out<R> out(elemvec, shape_elemvec());
+        amd::check(gf_quad_int_gradN_dot_tensor2_dV(m_h, in.ptr, out.ptr, GF_F64, nullptr));
+        out.finish();
+    }
+    /* Int_gradN_dot_tensor4_dot_gradNT_dV  (Element.h:849-868); precision GF_F64 or GF_BF16X3 */
+    template <class T>
+    amd::array<double> Int_gradN_dot_tensor4_dot_gradNT_dV(const T& qtensor, int precision = GF_F64) const
+    {
+        amd::array<double> r(shape_elemmat());
+        int_gradN_dot_tensor4_dot_gradNT_dV(qtensor, r, precision);
+        return r;
+    }
+    template <class T, class R>
+    void int_gradN_dot_tensor4_dot_gradNT_dV(const T& qtensor, R& elemmat, int precision = GF_F64) const
+    {
+        amd::dev_in<T> in(qtensor);
+        amd::dev_out<R> out(elemmat, shape_elemmat());
+        amd::check(gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV(m_h, in.ptr, out.ptr, precision, nullptr));
+        out.finish();
+    }
+
+protected:
+    QuadratureBase() = default;
+    template <class T, class X, class W>
+    void init(int etype, const T& x, const X& xi, const W& w, double thick)
+    {
+        amd::dev_in<T> in(x);
+        m_nelem = x.shape()[0];
+        amd::check(gf_quad_create(&m_h, etype, in.ptr, static_cast<int64_t>(m_nelem), GF_GAUSS, xi.data(), w.data(),
+                                  static_cast<int>(w.size()), thick, nullptr));
+        int64_t ne = 0;
+        int nne = 0, nd = 0, td = 0, ni = 0;
+        amd::check(gf_quad_sizes(m_h, &ne, &nne, &nd, &td, &ni));
+        m_nne = nne;
+        m_ndim = nd;
+        m_tdim = td;
+        m_nip = ni;
+        amd::check(gf_stream_sync(nullptr));
+    }
+    void sync() const
+    {
+        if (m_synced) return;
+        const double *g = nullptr, *v = nullptr;
+        amd::check(gf_quad_gradN(m_h, &g));
+        amd::check(gf_quad_dV(m_h, &v));
+        m_dNx = amd::array<double>({m_nelem, m_nip, m_nne, m_ndim});
+        m_vol = amd::array<double>({m_nelem, m_nip});
+        amd::check(gf_memcpy_d2h(m_dNx.data(), g, static_cast<int64_t>(m_dNx.size() * sizeof(double))));
+        amd::check(gf_memcpy_d2h(m_vol.data(), v, static_cast<int64_t>(m_vol.size() * sizeof(double))));
+        m_synced = true;
+    }
+    gf_quad m_h = nullptr;
+    size_t m_nelem = 0, m_nne = 0, m_ndim = 0, m_tdim = 0, m_nip = 0;
+    mutable amd::array<double> m_dNx, m_vol;
+    mutable bool m_synced = false;
+};
+
+} // namespace detail
+
+namespace Quad4 {
+namespace Gauss {
+inline size_t nip() { return 4; }
+inline amd::array<double> xi() { return detail::rule(2, false, false); }
+inline amd::array<double> w() { return detail::rule(2, false, true); }
+} // namespace Gauss
+namespace Nodal {
+inline size_t nip() { return 4; }
+inline amd::array<double> xi() { return detail::rule(2, true, false); }
+inline amd::array<double> w() { return detail::rule(2, true, true); }
+} // namespace Nodal
+
+/* Element::Quad4::Quadrature  (ElementQuad4.h:229-547) */
+class Quadrature : public detail::QuadratureBase {
+public:
+    template <class T>
+    explicit Quadrature(const T& x) { init(GF_QUAD4, x, Gauss::xi(), Gauss::w(), 1.0); }
+    template <class T, class X, class W>
+    Quadrature(const T& x, const X& xi, const W& w) { init(GF_QUAD4, x, xi, w, 1.0); }
+};
+
+/* Element::Quad4::QuadraturePlanar  (ElementQuad4Planar.h:37-335) */
+class QuadraturePlanar : public detail::QuadratureBase {
+public:
+    template <class T>
+    explicit QuadraturePlanar(const T& x, double thick = 1.0) { init(GF_QUAD4_PLANAR, x, Gauss::xi(), Gauss::w(), thick); }
+    template <class T, class X, class W>
+    QuadraturePlanar(const T& x, const X& xi, const W& w, double thick = 1.0)
+    {
+        init(GF_QUAD4_PLANAR, x, xi, w, thick);
+    }
+};
+} // namespace Quad4
+
+namespace Hex8 {
+namespace Gauss {
+inline size_t nip() { return 8; }
+inline amd::array<double> xi() { return detail::rule(3, false, false); }
+inline amd::array<double> w() { return detail::rule(3, false, true); }
+} // namespace Gauss
+namespace Nodal {
+inline size_t nip() { return 8; }
+inline amd::array<double> xi() { return detail::rule(3, true, false); }
+inline amd::array<double> w() { return detail::rule(3, true, true); }
+} // namespace Nodal
+
+/* Element::Hex8::Quadrature  (ElementHex8.h:206-400) */
+class Quadrature : public detail::QuadratureBase {
+public:
+    template <class T>
+    explicit Quadrature(const T& x) { init(GF_HEX8, x, Gauss::xi(), Gauss::w(), 1.0); }
+    template <class T, class X, class W>
+    Quadrature(const T& x, const X& xi, const W& w) { init(GF_HEX8, x, xi, w, 1.0); }
+};
+} // namespace Hex8
+} // namespace Element
+
+// =================================================================================================
+// MatrixDiagonal (include/GooseFEM/MatrixDiagonal.h) and Matrix (include/GooseFEM/Matrix.h)
+// =================================================================================================
+
+class MatrixDiagonal {
+public:
+    template <class C, class D>
+    MatrixDiagonal(const C& conn, const D& dofs) : m_vec(conn, dofs)
+    {
+        amd::check(gf_diag_create(&m_h, m_vec.handle()));
+    }
+    ~MatrixDiagonal() { gf_diag_destroy(m_h); }
+    size_t ndof() const { return m_vec.ndof(); }
+    /* assemble(elemmat): diagonal entries only (MatrixDiagonal.h:150-167) */
+    template <class T>
+    void assemble(const T& elemmat)
+    {
+        amd::dev_in<T> in(elemmat);
+        amd::check(gf_diag_assemble(m_h, in.ptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    template <class T>
+    void set(const T& A)
+    {
+        amd::dev_in<T> in(A);
+        amd::check(gf_diag_set(m_h, in.ptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    /* data()  (MatrixDiagonal.h:206-209) */
+    amd::array<double> data() const
+    {
+        const double* p = nullptr;
+        amd::check(gf_diag_data(m_h, &p));
+        amd::array<double> r({ndof()});
+        amd::check(gf_memcpy_d2h(r.data(), p, static_cast<int64_t>(r.size() * sizeof(double))));
+        return r;
+    }
+    /* Dot(x) / Solve(b) on dofval or nodevec  (MatrixDiagonal.h:212-259) */
+    template <class T>
+    amd::array<double> Dot(const T& x) const
+    {
+        amd::array<double> r(amd::shape_of(x));
+        amd::dev_in<T> in(x);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        if (x.dimension() == 1) amd::check(gf_diag_dot_dofval(m_h, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_diag_dot_nodevec(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    template <class T>
+    amd::array<double> Solve(const T& b)
+    {
+        amd::array<double> r(amd::shape_of(b));
+        solve(b, r);
+        return r;
+    }
+    template <class T, class R>
+    void solve(const T& b, R& x)
+    {
+        amd::dev_in<T> in(b);
+        amd::dev_out<R> out(x, amd::shape_of(b));
+        if (b.dimension() == 1) amd::check(gf_diag_solve_dofval(m_h, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_diag_solve_nodevec(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+    }
+
+private:
+    Vector m_vec;
+    gf_diag m_h = nullptr;
+};
+
+class Matrix {
+public:
+    template <class C, class D>
+    Matrix(const C& conn, const D& dofs) : m_vec(conn, dofs)
+    {
+        amd::check(gf_matrix_create(&m_h, m_vec.handle()));
+        int64_t nr = 0, nz = 0;
+        amd::check(gf_matrix_sizes(m_h, &nr, &nz));
+        m_nrows = static_cast<size_t>(nr);
+        m_nnz = static_cast<size_t>(nz);
+    }
+    ~Matrix() { gf_matrix_destroy(m_h); }
+    size_t ndof() const { return m_nrows; }
+    size_t nnz() const { return m_nnz; }
+    /* assemble(elemmat)  (Matrix.h:704-727) */
+    template <class T>
+    void assemble(const T& elemmat)
+    {
+        amd::dev_in<T> in(elemmat);
+        amd::check(gf_matrix_assemble(m_h, in.ptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    /* Dot(x)  (Matrix.h:804-815) */
+    template <class T>
+    amd::array<double> Dot(const T& x) const
+    {
+        amd::array<double> r(amd::shape_of(x));
+        amd::dev_in<T> in(x);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        if (x.dimension() == 1) amd::check(gf_matrix_dot_dofval(m_h, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_matrix_dot_nodevec(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    /* Todense()  (Matrix.h:338-355) */
+    amd::array<double> Todense() const
+    {
+        amd::array<double> r({m_nrows, m_nrows});
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_matrix_todense(m_h, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+
+private:
+    Vector m_vec;
+    gf_matrix m_h = nullptr;
+    size_t m_nrows = 0, m_nnz = 0;
+};
+
+// =================================================================================================
+// Tyings (include/GooseFEM/TyingsPeriodic.h)
+// =================================================================================================
+namespace Tyings {
+
+/* Compressed sparse rows of a tying matrix (in place of the reference's Eigen::SparseMatrix). */
+struct SparseMatrix {
+    size_t rows = 0, cols = 0;
+    std::vector<int64_t> rowptr, col;
+    std::vector<double> val;
+};
+
+/* Tyings::Control  (TyingsPeriodic.h:361-439) */
+class Control {
+public:
+    template <class C, class N>
+    Control(const C& coor, const N& dofs)
+    {
+        const size_t nnode = coor.shape()[0], ndim = coor.shape()[1];
+        std::vector<double> c(coor.data(), coor.data() + coor.size());
+        auto d = amd::to_i64(dofs);
+        std::vector<double> co((nnode + ndim) * ndim);
+        std::vector<int64_t> dout((nnode + ndim) * ndim), cd(ndim * ndim), cn(ndim);
+        amd::check(gf_tyings_control(c.data(), d.data(), static_cast<int64_t>(nnode), static_cast<int>(ndim), co.data(),
+                                     dout.data(), cd.data(), cn.data()));
+        m_coor = amd::array<double>({nnode + ndim, ndim});
+        std::copy(co.begin(), co.end(), m_coor.data());
+        m_dofs = amd::from_i64(dout, {nnode + ndim, ndim});
+        m_control_dofs = amd::from_i64(cd, {ndim, ndim});
+        m_control_nodes = amd::from_i64(cn, {ndim});
+    }
+    const amd::array<double>& coor() const { return m_coor; }
+    const amd::array<size_t>& dofs() const { return m_dofs; }
+    const amd::array<size_t>& controlDofs() const { return m_control_dofs; }
+    const amd::array<size_t>& controlNodes() const { return m_control_nodes; }
+
+private:
+    amd::array<double> m_coor;
+    amd::array<size_t> m_dofs, m_control_dofs, m_control_nodes;
+};
+
+/* Tyings::Periodic  (TyingsPeriodic.h:82-343) */
+class Periodic {
+public:
+    template <class C, class D, class S, class T, class U>
+    Periodic(const C& coor, const D& dofs, const S& control_dofs, const T& nodal_tyings, const U& iip)
+    {
+        m_ndim = coor.shape()[1];
+        m_coor.assign(coor.data(), coor.data() + coor.size());
+        auto d = amd::to_i64(dofs);
+        auto c = amd::to_i64(control_dofs);
+        m_ties = amd::to_i64(nodal_tyings);
+        auto p = amd::to_i64(iip);
+        std::vector<int64_t> dout(d.size()), cout(c.size()), sizes(4);
+        amd::check(gf_tyings_periodic(d.data(), static_cast<int64_t>(dofs.shape()[0]), static_cast<int>(m_ndim), c.data(),
+                                      m_ties.data(), static_cast<int64_t>(m_ties.size() / 2), p.data(),
+                                      static_cast<int64_t>(p.size()), dout.data(), cout.data(), sizes.data()));
+        m_dofs = amd::from_i64(dout, amd::shape_of(dofs));
+        m_control = amd::from_i64(cout, {m_ndim, m_ndim});
+        m_nnu = static_cast<size_t>(sizes[0]);
+        m_nnp = static_cast<size_t>(sizes[1]);
+        m_nni = static_cast<size_t>(sizes[2]);
+        m_nnd = static_cast<size_t>(sizes[3]);
+    }
+    size_t nnu() const { return m_nnu; }
+    size_t nnp() const { return m_nnp; }
+    size_t nni() const { return m_nni; }
+    size_t nnd() const { return m_nnd; }
+    const amd::array<size_t>& dofs() const { return m_dofs; }
+    const amd::array<size_t>& control() const { return m_control; }
+    SparseMatrix Cdi() const { return matrix(0, m_nni); }
+    SparseMatrix Cdu() const { return matrix(1, m_nnu); }
+    SparseMatrix Cdp() const { return matrix(2, m_nnp); }
+
+private:
+    SparseMatrix matrix(int which, size_t ncols) const
+    {
+        auto d = amd::to_i64(m_dofs);
+        auto c = amd::to_i64(m_control);
+        const int64_t nties = static_cast<int64_t>(m_ties.size() / 2);
+        SparseMatrix M;
+        M.rows = m_nnd;
+        M.cols = ncols;
+        M.rowptr.resize(m_nnd + 1);
+        int64_t nnz = 0;
+        amd::check(gf_tyings_matrix(which, m_coor.data(), d.data(), static_cast<int>(m_ndim), c.data(), m_ties.data(),
+                                    nties, static_cast<int64_t>(m_nnu), M.rowptr.data(), nullptr, nullptr, &nnz));
+        M.col.resize(static_cast<size_t>(nnz));
+        M.val.resize(static_cast<size_t>(nnz));
+        amd::check(gf_tyings_matrix(which, m_coor.data(), d.data(), static_cast<int>(m_ndim), c.data(), m_ties.data(),
+                                    nties, static_cast<int64_t>(m_nnu), M.rowptr.data(), M.col.data(), M.val.data(),
+                                    &nnz));
+        return M;
+    }
+    size_t m_ndim = 0, m_nnu = 0, m_nnp = 0, m_nni = 0, m_nnd = 0;
+    std::vector<double> m_coor;
+    std::vector<int64_t> m_ties;
+    amd::array<size_t> m_dofs, m_control;
+};
+
+} // namespace Tyings
+
+} // namespace GooseFEM
+
+#endif

@@ -59,6 +59,13 @@ int gf_version(int* major, int* minor, int* patch);
 /* Synchronise a stream (host-side wait).  Convenience for FFI callers. */
 int gf_stream_sync(void* stream);
 
+/* Device memory for FFI callers without HIP headers (the C++ shim uses these). */
+int gf_device_malloc(void** ptr, int64_t bytes);
+int gf_device_free(void* ptr);
+int gf_memcpy_h2d(void* dst, const void* src_h, int64_t bytes);
+int gf_memcpy_d2h(void* dst_h, const void* src, int64_t bytes);
+int gf_memcpy_d2d(void* dst, const void* src, int64_t bytes);
+
 /* ================================================================================================
  * Mesh bookkeeping (host, set-up only).  Integer outputs are bit-exact with the reference.
  * Two-call pattern for variable-size outputs: pass NULL to get the size.

@@ -1,0 +1,192 @@
+// Drop-in check of the C++ face (include/GooseFEM/GooseFEM.h): the same calls and checks as the
+// reference's C++ unit tests (tests/basic/ElementQuad4Planar.cpp, ElementHex8.cpp,
+// MatrixDiagonal.cpp, Matrix.cpp, Vector.cpp, MeshQuad4.cpp), against the GPU path.
+#include <GooseFEM/GooseFEM.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+
+static int g_fail = 0;
+#define REQUIRE(cond)                                                                               \
+    do {                                                                                            \
+        if (!(cond)) {                                                                              \
+            std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);                          \
+            ++g_fail;                                                                               \
+        }                                                                                           \
+    } while (0)
+
+template <class A>
+static double maxabs(const A& a)
+{
+    double m = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) m = std::max(m, std::abs(a.data()[i]));
+    return m;
+}
+
+int main()
+{
+    namespace GF = GooseFEM;
+    // MeshQuad4.cpp: Regular(5, 3)
+    {
+        GF::Mesh::Quad4::Regular mesh(5, 3);
+        auto conn = mesh.conn();
+        REQUIRE(mesh.nelem() == 15 && mesh.nnode() == 24);
+        REQUIRE(conn(0, 0) == 0 && conn(0, 1) == 1 && conn(0, 2) == 7 && conn(0, 3) == 6);
+        REQUIRE(conn(14, 0) == 16 && conn(14, 2) == 23);
+        auto per = mesh.nodesPeriodic();
+        REQUIRE(per.shape(0) == 9 && per(1, 1) == 23 && per(8, 0) == 12 && per(8, 1) == 17);
+        auto dp = mesh.dofsPeriodic();
+        REQUIRE(dp(5, 0) == 0 && dp(11, 0) == 10 && dp(23, 1) == 1);
+        REQUIRE(mesh.nodesTopRightCorner() == 23 && mesh.nodesBottomEdge().size() == 6);
+    }
+    // ElementQuad4Planar.cpp: dV, patch tests, zero periodic force
+    {
+        GF::Mesh::Quad4::Regular mesh(3, 3);
+        GF::Vector vec(mesh.conn(), mesh.dofs());
+        GF::Element::Quad4::QuadraturePlanar quad(vec.AsElement(mesh.coor()));
+        REQUIRE(std::abs(maxabs(quad.dV()) - 0.25) < 1e-14);
+        auto coor = mesh.coor();
+        GF::array_type::tensor<double, 2> disp(coor.shape());
+        for (size_t n = 0; n < mesh.nnode(); ++n) disp(n, 0) = 0.1 * coor(n, 1);
+        auto ue = vec.AsElement(disp);
+        auto gu = quad.GradN_vector(ue);
+        auto eps = quad.SymGradN_vector(ue);
+        bool ok = true;
+        for (size_t e = 0; e < mesh.nelem(); ++e)
+            for (size_t q = 0; q < quad.nip(); ++q) {
+                ok &= std::abs(gu(e, q, 1, 0) - 0.1) < 1e-12 && std::abs(gu(e, q, 0, 1)) < 1e-12;
+                ok &= std::abs(eps(e, q, 0, 1) - 0.05) < 1e-12 && std::abs(eps(e, q, 2, 2)) < 1e-14;
+            }
+        REQUIRE(ok);
+        GF::Vector vp(mesh.conn(), mesh.dofsPeriodic());
+        auto Fi = vp.AssembleDofs(quad.Int_gradN_dot_tensor2_dV(eps));
+        REQUIRE(Fi.size() == vp.ndof() && maxabs(Fi) < 1e-12);
+    }
+    // MatrixDiagonal.cpp: lumped mass known answer and Dot / Solve round trip
+    {
+        GF::Mesh::Quad4::Regular mesh(2, 2);
+        GF::Vector vector(mesh.conn(), mesh.dofs());
+        GF::MatrixDiagonal A(mesh.conn(), mesh.dofs());
+        GF::Element::Quad4::Quadrature quad(vector.AsElement(mesh.coor()), GF::Element::Quad4::Nodal::xi(),
+                                            GF::Element::Quad4::Nodal::w());
+        GF::array_type::tensor<double, 2> rho({mesh.nelem(), quad.nip()}, 1.0);
+        A.assemble(quad.Int_N_scalar_NT_dV(rho));
+        const double a[18] = {0.25, 0.25, 0.5, 0.5, 0.25, 0.25, 0.5, 0.5, 1.0,
+                              1.0,  0.5,  0.5, 0.25, 0.25, 0.5, 0.5, 0.25, 0.25};
+        auto d = A.data();
+        bool ok = true;
+        for (int i = 0; i < 18; ++i) ok &= std::abs(d[i] - a[i]) < 1e-12;
+        REQUIRE(ok);
+        std::mt19937 rng(1);
+        std::uniform_real_distribution<double> U(0.5, 1.5);
+        GF::array_type::tensor<double, 1> x({A.ndof()});
+        for (auto& v : x) v = U(rng);
+        auto b = A.Dot(x);
+        auto X = A.Solve(b);
+        ok = true;
+        for (size_t i = 0; i < x.size(); ++i) ok &= std::abs(X[i] - x[i]) < 1e-12;
+        REQUIRE(ok);
+    }
+    // Matrix.cpp: assemble random symmetric element matrices, Dot against the dense matrix
+    {
+        GF::Mesh::Quad4::Regular mesh(2, 2);
+        GF::Matrix A(mesh.conn(), mesh.dofs());
+        std::mt19937 rng(2);
+        std::uniform_real_distribution<double> U(0.0, 1.0);
+        GF::array_type::tensor<double, 3> a({mesh.nelem(), 8, 8});
+        for (size_t e = 0; e < mesh.nelem(); ++e)
+            for (size_t i = 0; i < 8; ++i)
+                for (size_t j = 0; j <= i; ++j) a(e, i, j) = a(e, j, i) = U(rng);
+        A.assemble(a);
+        auto K = A.Todense();
+        GF::array_type::tensor<double, 1> x({A.ndof()});
+        for (auto& v : x) v = U(rng);
+        auto b = A.Dot(x);
+        bool ok = true;
+        for (size_t i = 0; i < A.ndof(); ++i) {
+            double s = 0.0;
+            for (size_t j = 0; j < A.ndof(); ++j) s += K(i, j) * x[j];
+            ok &= std::abs(s - b[i]) < 1e-12 && std::abs(K(i, (i + 3) % A.ndof()) - K((i + 3) % A.ndof(), i)) < 1e-14;
+        }
+        REQUIRE(ok);
+    }
+    // ElementHex8.cpp: dV and periodic lumped mass of ones; BF16x3 stiffness close to FP64
+    {
+        GF::Mesh::Hex8::Regular mesh(3, 3, 3);
+        GF::Vector vec(mesh.conn(), mesh.dofsPeriodic());
+        GF::MatrixDiagonal M(mesh.conn(), mesh.dofsPeriodic());
+        GF::Element::Hex8::Quadrature nodal(vec.AsElement(mesh.coor()), GF::Element::Hex8::Nodal::xi(),
+                                            GF::Element::Hex8::Nodal::w());
+        GF::array_type::tensor<double, 2> rho({mesh.nelem(), nodal.nip()}, 1.0);
+        M.assemble(nodal.Int_N_scalar_NT_dV(rho));
+        auto m = M.data();
+        bool ok = m.size() == vec.ndof();
+        for (size_t i = 0; i < m.size(); ++i) ok &= std::abs(m[i] - 1.0) < 1e-12;
+        REQUIRE(ok);
+        GF::Element::Hex8::Quadrature quad(vec.AsElement(mesh.coor()));
+        REQUIRE(std::abs(maxabs(quad.dV()) - 0.125) < 1e-14);
+        GF::array_type::tensor<double, 6> C(quad.shape_qtensor4());
+        for (size_t e = 0; e < mesh.nelem(); ++e)
+            for (size_t q = 0; q < 8; ++q)
+                for (size_t i = 0; i < 3; ++i)
+                    for (size_t j = 0; j < 3; ++j)
+                        for (size_t k = 0; k < 3; ++k)
+                            for (size_t l = 0; l < 3; ++l)
+                                C(e, q, i, j, k, l) = 10.0 * (i == j) * (k == l) +
+                                                      2.0 * ((0.5 * ((i == k) * (j == l) + (i == l) * (j == k))) -
+                                                             (i == j) * (k == l) / 3.0);
+        auto K64 = quad.Int_gradN_dot_tensor4_dot_gradNT_dV(C, GF_F64);
+        auto Kbf = quad.Int_gradN_dot_tensor4_dot_gradNT_dV(C, GF_BF16X3);
+        double num = 0.0, den = 0.0;
+        for (size_t i = 0; i < K64.size(); ++i) {
+            num += (Kbf[i] - K64[i]) * (Kbf[i] - K64[i]);
+            den += K64[i] * K64[i];
+        }
+        REQUIRE(std::sqrt(num / den) < 2e-5);
+    }
+    // device-resident path: no host round trips between calls
+    {
+        GF::Mesh::Quad4::Regular mesh(16, 16);
+        GF::Vector vec(mesh.conn(), mesh.dofs());
+        GF::Element::Quad4::QuadraturePlanar quad(vec.AsElement(mesh.coor()));
+        GF::array_type::tensor<double, 2> u(vec.shape_nodevec());
+        std::mt19937 rng(3);
+        std::normal_distribution<double> N(0.0, 1e-3);
+        for (auto& v : u) v = N(rng);
+        auto du = GF::amd::device_array<double>::from_host(u);
+        GF::amd::device_array<double> ue(vec.shape_elemvec()), eps(quad.shape_qtensor2()), fe(vec.shape_elemvec()),
+            f(vec.shape_nodevec());
+        vec.asElement(du, ue);
+        quad.symGradN_vector(ue, eps);
+        quad.int_gradN_dot_tensor2_dV(eps, fe);
+        vec.assembleNode(fe, f);
+        auto fh = f.to_host();
+        auto ref = vec.AssembleNode(quad.Int_gradN_dot_tensor2_dV(quad.SymGradN_vector(vec.AsElement(u))));
+        bool ok = true;
+        for (size_t i = 0; i < ref.size(); ++i) ok &= fh[i] == ref[i];
+        REQUIRE(ok);
+    }
+    // Tyings::Control + Tyings::Periodic as in docs/examples/statics/periodic_elastoplastic.py
+    {
+        GF::Mesh::Quad4::Regular mesh(5, 5);
+        GF::Tyings::Control control(mesh.coor(), mesh.dofs());
+        auto cd = control.controlDofs();
+        GF::array_type::tensor<size_t, 1> iip({6});
+        for (size_t i = 0; i < 4; ++i) iip[i] = cd[i];
+        iip[4] = control.dofs()(mesh.nodesOrigin(), 0);
+        iip[5] = control.dofs()(mesh.nodesOrigin(), 1);
+        GF::Tyings::Periodic tyings(control.coor(), control.dofs(), cd, mesh.nodesPeriodic(), iip);
+        REQUIRE(tyings.nnp() == 6 && tyings.nnd() == 2 * mesh.nodesPeriodic().shape(0));
+        REQUIRE(tyings.nnu() + tyings.nnp() == tyings.nni());
+        auto Cdu = tyings.Cdu();
+        REQUIRE(Cdu.rows == tyings.nnd() && Cdu.cols == tyings.nnu());
+    }
+    if (g_fail) {
+        std::printf("%d check(s) failed\n", g_fail);
+        return 1;
+    }
+    std::printf("ALL OK\n");
+    return 0;
+}

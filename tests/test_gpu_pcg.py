@@ -82,7 +82,7 @@ def test_solve_matches_direct_and_true_residual(gf, et, n):
     xp = np.zeros(ty.nnp)
     xp[ty.control[0, 1] - ty.nnu] = 1e-3  # simple shear increment through ctrl(0, 1)
     b = -host(P.apply(None, xp))
-    assert np.allclose(b, -Kup @ xp, rtol=1e-12, atol=1e-18)
+    assert np.linalg.norm(b + Kup @ xp) / np.linalg.norm(Kup @ xp) < 1e-13
     x, it, res = P.solve(b, tol=1e-10)
     x = host(x)
     xd = np.linalg.solve(Kp, b)
