@@ -60,17 +60,11 @@ namespace {
 
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 
-// streaming (non-temporal) 16-byte load / store of a node's two DOFs
+// streaming (non-temporal) 16-byte load of a node's two DOFs (read once per step)
 __device__ inline double2 nt_load2(const double* p, int64_t n)
 {
     dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2*>(p) + n);
     return make_double2(v.x, v.y);
-}
-
-__device__ inline void nt_store2(double* p, int64_t n, double2 v)
-{
-    dvec2 w = {v.x, v.y};
-    __builtin_nontemporal_store(w, reinterpret_cast<dvec2*>(p) + n);
 }
 
 struct ShapeTable {
@@ -247,8 +241,15 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
 
     // phase A: updated displacement and coordinates of the tile + halo (all loads issued first)
     {
+        // every element initialised: keeps the prefetch arrays in registers (a conditionally
+        // initialised array was demoted to scratch memory)
         double2 ru[NA], rv[NA], ra[NA], rx[NA];
         bool ok[NA];
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            ru[k] = rv[k] = ra[k] = rx[k] = make_double2(0.0, 0.0);
+            ok[k] = false;
+        }
 #pragma unroll
         for (int k = 0; k < NA; ++k) {
             const int t = threadIdx.x + k * NT;
@@ -353,9 +354,11 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
         an.y = o_im.y * (o_fe.y - f1);
         vn.x = o_v.x + hdt * (o_a.x + an.x);
         vn.y = o_v.y + hdt * (o_a.y + an.y);
-        nt_store2(u1, on, un);
-        nt_store2(v1, on, vn);
-        nt_store2(a1, on, an);
+        // plain stores: the write-back L2 merges the halves of each 128-byte line written by
+        // neighbouring lanes (non-temporal stores measured ~1.5x the compulsory write bytes)
+        reinterpret_cast<double2*>(u1)[on] = un;
+        reinterpret_cast<double2*>(v1)[on] = vn;
+        reinterpret_cast<double2*>(a1)[on] = an;
         if (fint_out) reinterpret_cast<double2*>(fint_out)[on] = make_double2(f0, f1);
     }
 }
@@ -509,23 +512,26 @@ bool detect_structured(const gf_vector_s* v, int64_t& nx, int64_t& ny)
     return nx < (1 << 30) && ny < (1 << 30);
 }
 
-// Tile shape (nodes per workgroup).  Default from a sweep on MI355X; GF_EXPLICIT_TILE=WxH
-// ("64x8", "32x8", "32x16", "64x4", "16x16") selects another instantiation.
+// Tile shape (nodes per workgroup).  Defaults from a sweep on MI355X at 2048x2048 elements:
+// 32x16 for FP32 element bodies (112 us/step), 32x8 for FP64 (172 us/step; the FP64 body
+// needs more registers, so the smaller tile keeps more waves resident).  GF_EXPLICIT_TILE=WxH
+// ("64x8", "32x8", "32x16", "64x4", "16x16") overrides both.
 struct TileCfg {
     int tx, ty;
 };
 
-TileCfg tile_config()
+TileCfg tile_config(bool f32)
 {
-    static TileCfg cfg = [] {
-        TileCfg c{32, 16};
+    static const TileCfg over = [] {
+        TileCfg c{0, 0};
         if (const char* e = std::getenv("GF_EXPLICIT_TILE")) {
             int a = 0, b = 0;
             if (std::sscanf(e, "%dx%d", &a, &b) == 2) c = TileCfg{a, b};
         }
         return c;
     }();
-    return cfg;
+    if (over.tx > 0) return over;
+    return f32 ? TileCfg{32, 16} : TileCfg{32, 8};
 }
 
 template <int TX, int TY>
@@ -658,7 +664,7 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
         const bool f32 = precision == GF_F32;
         if (x->structured) {
             static const ShapeTable T = gauss_table();
-            const TileCfg tc = tile_config();
+            const TileCfg tc = tile_config(f32);
             const int TXr = tc.tx, TYr = tc.ty;
             const int nx = static_cast<int>(x->nx), ny = static_cast<int>(x->ny);
             const int ntx = (nx + 1 + TXr - 1) / TXr, nty = (ny + 1 + TYr - 1) / TYr;
