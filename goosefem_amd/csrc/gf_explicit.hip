@@ -210,7 +210,6 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     constexpr int HX = TX + 2, HY = TY + 2;   // halo node tile
     constexpr int EX = TX + 1, EY = TY + 1;   // element tile
     constexpr int NT = TX * TY;
-    constexpr int NA = (HX * HY + NT - 1) / NT; // halo nodes per thread
     constexpr int NB = (EX * EY + NT - 1) / NT; // elements per thread
     using FT = typename std::conditional<F32, float, double>::type;
     __shared__ double2 s_u[HY * HX];
@@ -225,56 +224,71 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     const int x0 = tx * TX, y0 = ty * TY;
     const int NX = nx + 1, NY = ny + 1;
 
-    // owned node of this thread; its update operands are fetched first so they are in flight
-    // while the tile is staged and the element forces are computed
+    // phase A: updated displacement and coordinates of the node tile.  Each thread loads its own
+    // node (whose v, a, M^-1 and f_ext stay in registers for phase C) and threads 0..NR-1 load
+    // one node each of the one-node halo ring, so no node is fetched twice by one workgroup.
+    constexpr int NR = 2 * HX + 2 * TY;
+    static_assert(NR <= NT, "halo ring must fit one pass of the workgroup");
+    const double2 zero = make_double2(0.0, 0.0);
     const int oly = threadIdx.x / TX, olx = threadIdx.x - (threadIdx.x / TX) * TX;
     const int ogx = x0 + olx, ogy = y0 + oly;
     const bool owner = ogx < NX && ogy < NY;
     const int64_t on = static_cast<int64_t>(ogy) * NX + ogx;
-    double2 o_v = make_double2(0.0, 0.0), o_a = o_v, o_im = o_v, o_fe = o_v;
+    double2 o_u = zero, o_v = zero, o_a = zero, o_x = zero, o_im = zero, o_fe = zero;
     if (owner) {
+        o_u = reinterpret_cast<const double2*>(u0)[on];
         o_v = reinterpret_cast<const double2*>(v0)[on];
         o_a = reinterpret_cast<const double2*>(a0)[on];
+        o_x = reinterpret_cast<const double2*>(coor)[on];
         o_im = nt_load2(inv_mass, on);
         if (fext) o_fe = nt_load2(fext, on);
     }
-
-    // phase A: updated displacement and coordinates of the tile + halo (all loads issued first)
-    {
-        // every element initialised: keeps the prefetch arrays in registers (a conditionally
-        // initialised array was demoted to scratch memory)
-        double2 ru[NA], rv[NA], ra[NA], rx[NA];
-        bool ok[NA];
-#pragma unroll
-        for (int k = 0; k < NA; ++k) {
-            ru[k] = rv[k] = ra[k] = rx[k] = make_double2(0.0, 0.0);
-            ok[k] = false;
+    int rt = -1;
+    double2 h_u = zero, h_v = zero, h_a = zero, h_x = zero;
+    if (threadIdx.x < NR) {
+        // ring order: bottom row, top row, left column, right column
+        const int r = threadIdx.x;
+        int lx, ly;
+        if (r < HX) {
+            lx = r;
+            ly = 0;
         }
-#pragma unroll
-        for (int k = 0; k < NA; ++k) {
-            const int t = threadIdx.x + k * NT;
-            const int ly = t / HX, lx = t - ly * HX;
-            const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-            ok[k] = t < HX * HY && gx >= 0 && gx < NX && gy >= 0 && gy < NY;
-            if (ok[k]) {
-                const int64_t n = static_cast<int64_t>(gy) * NX + gx;
-                ru[k] = reinterpret_cast<const double2*>(u0)[n];
-                rv[k] = reinterpret_cast<const double2*>(v0)[n];
-                ra[k] = reinterpret_cast<const double2*>(a0)[n];
-                rx[k] = reinterpret_cast<const double2*>(coor)[n];
-            }
+        else if (r < 2 * HX) {
+            lx = r - HX;
+            ly = HY - 1;
         }
-#pragma unroll
-        for (int k = 0; k < NA; ++k) {
-            if (ok[k]) {
-                const int t = threadIdx.x + k * NT;
-                double2 un;
-                un.x = ru[k].x + dt * rv[k].x + c * ra[k].x;
-                un.y = ru[k].y + dt * rv[k].y + c * ra[k].y;
-                s_u[t] = un;
-                s_x[t] = rx[k];
-            }
+        else if (r < 2 * HX + TY) {
+            lx = 0;
+            ly = 1 + r - 2 * HX;
         }
+        else {
+            lx = HX - 1;
+            ly = 1 + r - 2 * HX - TY;
+        }
+        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+        if (gx >= 0 && gx < NX && gy >= 0 && gy < NY) {
+            const int64_t n = static_cast<int64_t>(gy) * NX + gx;
+            rt = ly * HX + lx;
+            h_u = reinterpret_cast<const double2*>(u0)[n];
+            h_v = reinterpret_cast<const double2*>(v0)[n];
+            h_a = reinterpret_cast<const double2*>(a0)[n];
+            h_x = reinterpret_cast<const double2*>(coor)[n];
+        }
+    }
+    double2 o_un;
+    o_un.x = o_u.x + dt * o_v.x + c * o_a.x;
+    o_un.y = o_u.y + dt * o_v.y + c * o_a.y;
+    if (owner) {
+        const int t = (oly + 1) * HX + olx + 1;
+        s_u[t] = o_un;
+        s_x[t] = o_x;
+    }
+    if (rt >= 0) {
+        double2 hn;
+        hn.x = h_u.x + dt * h_v.x + c * h_a.x;
+        hn.y = h_u.y + dt * h_v.y + c * h_a.y;
+        s_u[rt] = hn;
+        s_x[rt] = h_x;
     }
     __syncthreads();
 
@@ -348,7 +362,7 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
             f0 += static_cast<double>(s_f[le][0]);
             f1 += static_cast<double>(s_f[le][1]);
         }
-        const double2 un = s_u[(ly + 1) * HX + lx + 1];
+        const double2 un = o_un;
         double2 an, vn;
         an.x = o_im.x * (o_fe.x - f0);
         an.y = o_im.y * (o_fe.y - f1);
