@@ -535,7 +535,8 @@ class _Quadrature(_Handle):
         return self._wrap(p, (self.nelem, self.nip))
 
     def update_x(self, x):
-        _call("gf_quad_update_x", self._h, _dp(to_device(x)), _stream())
+        xd = to_device(x)  # held until the call has enqueued its kernels
+        _call("gf_quad_update_x", self._h, _dp(xd), _stream())
 
     def shape_qtensor(self, rank=2):
         return (self.nelem, self.nip) + (self.tdim,) * rank
@@ -719,10 +720,12 @@ class MatrixDiagonal(_Handle):
         self._h = h
 
     def assemble(self, elemmat):
-        _call("gf_diag_assemble", self._h, _dp(to_device(elemmat)), _stream())
+        em = to_device(elemmat)
+        _call("gf_diag_assemble", self._h, _dp(em), _stream())
 
     def set(self, A):
-        _call("gf_diag_set", self._h, _dp(to_device(A)), _stream())
+        Ad = to_device(A)
+        _call("gf_diag_set", self._h, _dp(Ad), _stream())
 
     @property
     def data(self):
@@ -767,7 +770,8 @@ class Matrix(_Handle):
         self.nrows, self.nnz = nr.value, nz.value
 
     def assemble(self, elemmat):
-        _call("gf_matrix_assemble", self._h, _dp(to_device(elemmat)), _stream())
+        em = to_device(elemmat)
+        _call("gf_matrix_assemble", self._h, _dp(em), _stream())
 
     def Dot(self, x, out=None):
         x = to_device(x)
@@ -891,6 +895,12 @@ class TyingsPCG(_Handle):
               _i64(self.nnp), _hp(self._cdu[0]), _hp(self._cdu[1]), _hp(self._cdu[2]), _hp(self._cdp[0]),
               _hp(self._cdp[1]), _hp(self._cdp[2]))
         self._h = h
+
+    def set_moduli(self, K_e, G_e):
+        """Replace the per-element moduli (same mesh and tyings) and refresh the Jacobi diagonal."""
+        self._K = to_device(K_e)
+        self._G = to_device(G_e)
+        _call("gf_pcg_set_moduli", self._h, _dp(self._K), _dp(self._G), _stream())
 
     def apply(self, x_u=None, x_p=None):
         """K' x_u + K'_up x_p."""

@@ -410,6 +410,25 @@ void apply(gf_pcg_s* P, const double* xu, const double* xp, double* yu, bool xu_
 
 extern "C" {
 
+namespace {
+
+// Jacobi diagonal of K' (element diagonals, ordered gather, folded through C_du), on stream s
+void compute_invdiag(gf_pcg P, hipStream_t s)
+{
+    const int64_t n = P->nelem * P->nne;
+    if (P->et == GF_HEX8)
+        k_elem_diag<GF_HEX8><<<nblocks(n, BS), BS, 0, s>>>(P->nelem, quad_dNx(P->q), quad_vol(P->q), P->Ke.p,
+                                                         P->Ge.p, P->fe.p);
+    else
+        k_elem_diag<GF_QUAD4_PLANAR><<<nblocks(n, BS), BS, 0, s>>>(P->nelem, quad_dNx(P->q), quad_vol(P->q),
+                                                                 P->Ke.p, P->Ge.p, P->fe.p);
+    k_diag_fold<<<nblocks(P->nnu, BS), BS, 0, s>>>(P->nnu, P->nni, P->v->dof_eptr.p, P->v->dof_eidx.p, P->fe.p,
+                                                  P->cduT.ptr.p, P->cduT.idx.p, P->cduT.val.p, P->invdiag.p);
+    check_launch();
+}
+
+} // namespace
+
 int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const double* G_e, int64_t nnu,
                   int64_t nnp, const int64_t* cdu_ptr_h, const int64_t* cdu_col_h, const double* cdu_val_h,
                   const int64_t* cdp_ptr_h, const int64_t* cdp_col_h, const double* cdp_val_h)
@@ -464,18 +483,7 @@ int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const 
             P->partial.alloc(4 * NBR);
             P->scal.alloc(8);
             P->state.alloc(4);
-            // Jacobi diagonal of K' (element diagonals, ordered gather, folded through C_du)
-            hipStream_t s = nullptr;
-            const int64_t n = P->nelem * P->nne;
-            if (et == GF_HEX8)
-                k_elem_diag<GF_HEX8><<<nblocks(n, BS), BS, 0, s>>>(P->nelem, quad_dNx(q), quad_vol(q), P->Ke.p, P->Ge.p,
-                                                                 P->fe.p);
-            else
-                k_elem_diag<GF_QUAD4_PLANAR><<<nblocks(n, BS), BS, 0, s>>>(P->nelem, quad_dNx(q), quad_vol(q), P->Ke.p,
-                                                                         P->Ge.p, P->fe.p);
-            k_diag_fold<<<nblocks(nnu, BS), BS, 0, s>>>(nnu, P->nni, v->dof_eptr.p, v->dof_eidx.p, P->fe.p,
-                                                       P->cduT.ptr.p, P->cduT.idx.p, P->cduT.val.p, P->invdiag.p);
-            check_launch();
+            compute_invdiag(P, nullptr);
             GF_HIP(hipDeviceSynchronize());
         }
         catch (...) {
@@ -490,6 +498,17 @@ int gf_pcg_destroy(gf_pcg P)
 {
     delete P;
     return GF_OK;
+}
+
+int gf_pcg_set_moduli(gf_pcg P, const double* K_e, const double* G_e, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(P && K_e && G_e, "null argument");
+        hipStream_t s = S(stream);
+        GF_HIP(hipMemcpyAsync(P->Ke.p, K_e, P->nelem * sizeof(double), hipMemcpyDeviceToDevice, s));
+        GF_HIP(hipMemcpyAsync(P->Ge.p, G_e, P->nelem * sizeof(double), hipMemcpyDeviceToDevice, s));
+        compute_invdiag(P, s);
+    });
 }
 
 int gf_pcg_sizes(gf_pcg P, int64_t* nnu, int64_t* nnp, int64_t* nnd)

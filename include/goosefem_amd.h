@@ -301,6 +301,10 @@ int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const 
                   const int64_t* cdp_ptr_h, const int64_t* cdp_col_h, const double* cdp_val_h);
 int gf_pcg_destroy(gf_pcg p);
 int gf_pcg_sizes(gf_pcg p, int64_t* nnu, int64_t* nnp, int64_t* nnd);
+/* replace the per-element moduli (device, FP64 [nelem] each) and recompute the Jacobi diagonal,
+ * enqueued on stream: the tied operator of a new material field without rebuilding the tyings
+ * (MatrixPartitionedTyings::assemble + factorize, MatrixPartitionedTyings.h:210-305,655-674) */
+int gf_pcg_set_moduli(gf_pcg p, const double* K_e, const double* G_e, void* stream);
 /* y_u = K' x_u + K'_up x_p; x_u or x_p may be NULL (zero). */
 int gf_pcg_apply(gf_pcg p, const double* x_u, const double* x_p, double* y_u, void* stream);
 /* full dofval [x_u, x_p, C_du x_u + C_dp x_p] (the x_d = Cdu X_u + Cdp X_p step of
@@ -314,6 +318,61 @@ int gf_pcg_inverse_diagonal(gf_pcg p, const double** invdiag);
  * iterations (<= 0: 32).  Returns GF_ENOTCONV if the true residual exceeds 10 tol. */
 int gf_pcg_solve(gf_pcg p, const double* b_u, double* x_u, double tol, int64_t maxiter, int64_t check_every,
                  int64_t* iterations, double* relres, void* stream);
+
+/* ================================================================================================
+ * Learned warm start of the tied PCG (no counterpart in the reference): kernels of a periodic
+ * convolutional encoder-decoder, trained and evaluated on the device.  Activations NHWC bfloat16
+ * (channel counts multiples of 16); master weights FP32 [9][cout][cin] (tap t = 3 (dy + 1) + dx + 1);
+ * packed weights bfloat16.  All reductions are ordered (deterministic).
+ * ============================================================================================== */
+
+/* 3x3 circular convolution on the [P = N*H*W] pixel grid.  Input channels [0, c1) from x1 (read
+ * at half resolution when up1 != 0: nearest upsampling), [c1, c1 + c2) from x2 (may be NULL if
+ * c2 = 0).  wpk: [9][coutp][c1 + c2] (coutp in {32, 64, 128, 256}).  bias: [cout] or NULL.
+ * flags: 1 = ReLU, 2 = accumulate into y.  Output channels [0, ysplit) -> y (row stride ysplit),
+ * [ysplit, cout) -> y2; y32 (optional) FP32 [P][cout]; mask (optional, [P][cout]) zeroes outputs
+ * whose mask value is not > 0.  The data gradient of a convolution is this call with the weights
+ * packed tap-flipped and transposed (gf_conv_pack). */
+int gf_conv3x3(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int H, int W, const void* wpk,
+               int coutp, const float* bias, int cout, int flags, void* y, int ysplit, void* y2, float* y32,
+               const void* mask, void* stream);
+/* workspace (floats) and split count of gf_conv3x3_wgrad for P pixels, cg gradient channels and
+ * cin input channels */
+int gf_conv3x3_wgrad_work(int64_t P, int cg, int cin, int* nsplit, int64_t* work_floats);
+/* weight gradient dw [9][cout][cin] = sum_p g[p][co] x[p + tap][ci] (same input addressing as
+ * gf_conv3x3), bias gradient db [cout] (optional); g has cg channels */
+int gf_conv3x3_wgrad(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int H, int W, const void* g,
+                     int cg, int cout, int cin, float* work, int64_t work_floats, float* dw, float* db, void* stream);
+/* 2x2 average pooling [N, H, W, C] -> [N, H/2, W/2, C] */
+int gf_avgpool2(const void* in, int64_t N, int H, int W, int C, void* out, void* stream);
+/* fine-grid gradient din[N, H, W, C] (+)= w * dout[N, H/2, W/2, C] (accum != 0 adds), then zeroed
+ * where act (optional, fine grid) is not > 0 */
+int gf_unpool2(const void* dout, int64_t N, int H, int W, int C, float w, int accum, const void* act, void* din,
+               void* stream);
+/* coarse gradient of nearest upsampling: out[N, H/2, W/2, C] = sum of each 2x2 block of in[N, H, W, C],
+ * zeroed where act (optional, coarse grid) is not > 0 */
+int gf_sumpool2(const void* in, int64_t N, int H, int W, int C, const void* act, void* out, void* stream);
+/* master weights w [9][cout][cin] -> wf [9][coutp][cin_s] and (optional) wb [9][cinp][cout_s]
+ * with wb[t][ci][co] = w[8 - t][co][ci]; zero padded */
+int gf_conv_pack(int cout, int cin, const float* w, int coutp, int cin_s, void* wf, int cinp, int cout_s, void* wb,
+                 void* stream);
+/* Adam (bias-corrected) on n FP32 parameters; step counts from 1 */
+int gf_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float beta1, float beta2, float eps,
+            int64_t step, void* stream);
+/* mean squared error of pred vs target ([P][nc] FP32): gradient g [P][cg] bfloat16 (channels >= nc
+ * zero, scaled by `scale`), *loss_sum (device) = sum of squared differences; part: >= ceil(P*cg/256)
+ * floats of workspace */
+int gf_mse(int64_t P, int nc, const float* pred, const float* target, float scale, int cg, void* g, float* part,
+           float* loss_sum, void* stream);
+/* x_u[idx[k]] = gamma * (scale * (grid[k] - grid[k % nc]) + affine[k]) where idx[k] >= 0 */
+int gf_grid_to_dofs(int64_t n, int nc, const float* grid, const int32_t* idx, const double* affine, double scale,
+                    double gamma, double* xu, void* stream);
+/* grid[k] = (x_u[idx[k]] / gamma - affine[k]) / scale (0 where idx[k] < 0); center != 0 subtracts
+ * each channel's mean */
+int gf_dofs_to_grid(int64_t n, int nc, const double* xu, const int32_t* idx, const double* affine, double scale,
+                    double gamma, int center, float* grid, void* stream);
+/* network input x [P][cs] bfloat16: channel 0 = scale * (log G - shift), others 0 */
+int gf_input_logG(int64_t P, const double* G, double shift, double scale, int cs, void* x, void* stream);
 
 #ifdef __cplusplus
 }

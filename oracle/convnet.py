@@ -1,0 +1,65 @@
+"""FP64 CPU restatement of the warm-start network (test infrastructure only: tests/ load it as the
+checker of goosefem_amd/csrc/gf_conv.hip; nothing in goosefem_amd/ imports it).
+
+Same layout and architecture as the device implementation: NHWC activations, weights
+[9][cout][cin] with tap t = 3 (dy + 1) + (dx + 1), 3x3 circular convolution, 2x2 average pooling,
+nearest upsampling with the upsampled tensor first in the skip concatenation, ReLU after every
+convolution but the last.  Gradients come from torch autograd in float64 and are pinned by a
+central finite-difference check in tests/test_oracle_convnet.py.  No counterpart in the reference.
+"""
+
+import torch
+
+
+def conv3x3(x, w, b=None):
+    """x [N, H, W, Cin], w [9, Cout, Cin] -> [N, H, W, Cout]; y[p] = sum_t w[t] x[p + (dy, dx)]."""
+    out = None
+    for t in range(9):
+        dy, dx = t // 3 - 1, t % 3 - 1
+        xs = torch.roll(x, shifts=(-dy, -dx), dims=(1, 2))
+        term = torch.einsum("nhwc,oc->nhwo", xs, w[t])
+        out = term if out is None else out + term
+    if b is not None:
+        out = out + b
+    return out
+
+
+def avgpool2(x):
+    N, H, W, C = x.shape
+    return x.reshape(N, H // 2, 2, W // 2, 2, C).mean(dim=(2, 4))
+
+
+def upsample2(x):
+    return x.repeat_interleave(2, dim=1).repeat_interleave(2, dim=2)
+
+
+def bf16(x):
+    """Round to bfloat16 (round to nearest even) and back."""
+    return x.to(torch.bfloat16).to(x.dtype)
+
+
+def unet_forward(z, params, channels, round_bf16=False):
+    """z [N, H, W, 1] (normalised log G); params: list of (w, b) in the device network's layer
+    order (encoder pairs per level, decoder pairs per level 0..L-2, output layer).  With
+    round_bf16 the activations are rounded to bfloat16 where the device stores them."""
+    L = len(channels)
+    r = bf16 if round_bf16 else (lambda t: t)
+    enc = params[: 2 * L]
+    dec = params[2 * L: 2 * L + 2 * (L - 1)]
+    out = params[-1]
+    x = r(z)
+    E = []
+    for l in range(L):
+        if l > 0:
+            x = r(avgpool2(E[-1]))
+        (w1, b1), (w2, b2) = enc[2 * l], enc[2 * l + 1]
+        a = r(torch.relu(conv3x3(x, w1, b1)))
+        e = r(torch.relu(conv3x3(a, w2, b2)))
+        E.append(e)
+    d = E[-1]
+    for l in range(L - 2, -1, -1):
+        (w1, b1), (w2, b2) = dec[2 * l], dec[2 * l + 1]
+        cat = torch.cat([upsample2(d), E[l]], dim=3)
+        bb = r(torch.relu(conv3x3(cat, w1, b1)))
+        d = r(torch.relu(conv3x3(bb, w2, b2)))
+    return conv3x3(d, out[0], out[1])

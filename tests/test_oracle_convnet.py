@@ -1,0 +1,88 @@
+"""Pins the FP64 CPU restatement of the warm-start network (oracle/convnet.py): the convolution
+against torch's own circular conv2d (an independent library routine), pooling/upsampling against
+closed forms, and the autograd gradients against central finite differences."""
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from oracle import convnet
+
+
+def test_conv3x3_matches_library_circular_conv():
+    g = torch.Generator().manual_seed(0)
+    x = torch.randn(2, 6, 8, 5, dtype=torch.float64, generator=g)
+    w = torch.randn(9, 7, 5, dtype=torch.float64, generator=g)
+    b = torch.randn(7, dtype=torch.float64, generator=g)
+    mine = convnet.conv3x3(x, w, b)
+    # conv2d is a cross-correlation: y[p] = sum_{ky,kx} k[ky, kx] x[p + (ky - 1, kx - 1)], tap t = 3 ky + kx
+    k = w.reshape(3, 3, 7, 5).permute(2, 3, 0, 1)
+    xp = F.pad(x.permute(0, 3, 1, 2), (1, 1, 1, 1), mode="circular")
+    ref = F.conv2d(xp, k, b).permute(0, 2, 3, 1)
+    assert torch.allclose(mine, ref, rtol=1e-13, atol=1e-13)
+
+
+def test_pool_and_upsample_closed_forms():
+    x = torch.arange(2 * 4 * 6 * 3, dtype=torch.float64).reshape(2, 4, 6, 3)
+    p = convnet.avgpool2(x)
+    assert p.shape == (2, 2, 3, 3)
+    assert p[1, 1, 2, 0] == (x[1, 2, 4, 0] + x[1, 2, 5, 0] + x[1, 3, 4, 0] + x[1, 3, 5, 0]) / 4
+    u = convnet.upsample2(p)
+    assert u.shape == x.shape and torch.equal(u[:, 1::2, 1::2], p) and torch.equal(u[:, ::2, 1::2], p)
+    # the adjoint of nearest upsampling is the 2x2 block sum: <up(a), b> = <a, 4 avgpool(b)>
+    a, bb = torch.randn(2, 2, 3, 3, dtype=torch.float64), torch.randn(2, 4, 6, 3, dtype=torch.float64)
+    assert torch.allclose((convnet.upsample2(a) * bb).sum(), (a * 4 * convnet.avgpool2(bb)).sum())
+
+
+def make_params(channels, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    L = len(channels)
+    shapes = []
+    for l in range(L):
+        cin = 1 if l == 0 else channels[l - 1]
+        shapes += [(channels[l], cin), (channels[l], channels[l])]
+    for l in range(L - 1):
+        shapes += [(channels[l], channels[l + 1] + channels[l]), (channels[l], channels[l])]
+    shapes.append((2, channels[0]))
+    params = []
+    for co, ci in shapes:
+        w = torch.randn(9, co, ci, dtype=torch.float64, generator=g) * np.sqrt(2.0 / (9 * ci))
+        b = 0.1 * torch.randn(co, dtype=torch.float64, generator=g)
+        params.append((w.requires_grad_(), b.requires_grad_()))
+    return params
+
+
+def test_unet_gradient_finite_differences():
+    ch = (4, 6)
+    params = make_params(ch)
+    g = torch.Generator().manual_seed(1)
+    z = torch.randn(1, 8, 8, 1, dtype=torch.float64, generator=g)
+    tgt = torch.randn(1, 8, 8, 2, dtype=torch.float64, generator=g)
+
+    def loss():
+        return ((convnet.unet_forward(z, params, ch) - tgt) ** 2).mean()
+
+    L = loss()
+    L.backward()
+    rng = np.random.default_rng(2)
+    h = 1e-6
+    for li in (0, 3, 4, len(params) - 1):  # first encoder, second level, decoder concat conv, output
+        w, b = params[li]
+        for _ in range(3):
+            idx = tuple(int(rng.integers(0, s)) for s in w.shape)
+            with torch.no_grad():
+                w[idx] += h
+                lp = loss().item()
+                w[idx] -= 2 * h
+                lm = loss().item()
+                w[idx] += h
+            fd = (lp - lm) / (2 * h)
+            assert fd == pytest.approx(w.grad[idx].item(), rel=1e-5, abs=1e-9)
+        with torch.no_grad():
+            b[0] += h
+            lp = loss().item()
+            b[0] -= 2 * h
+            lm = loss().item()
+            b[0] += h
+        assert (lp - lm) / (2 * h) == pytest.approx(b.grad[0].item(), rel=1e-5, abs=1e-9)
