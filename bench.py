@@ -285,6 +285,82 @@ def pcg_cpu_baseline(S, b, iters):
             "sample": f"{it} CG iterations on the assembled K' ({Kp.nnz} nnz), FP64 oracle, one thread"}
 
 
+def bench_pcg_learned(args):
+    """Wall time of the FP64 Jacobi-PCG to a true relative residual of 1e-10 on the periodic Quad4
+    cell under a simple-shear increment, from the learned warm start (network inference included)
+    and from a zero guess, on modulus fields the network was not trained on."""
+    import torch
+
+    import goosefem_amd as gf
+    from goosefem_amd import warmstart as ws
+
+    if not os.path.exists(args.ws_weights):
+        print(json.dumps({"error": f"no trained weights at {args.ws_weights}; run "
+                                   "python -m goosefem_amd.warmstart --out <file> first"}))
+        sys.exit(2)
+    net = ws.WarmStartNet.load(args.ws_weights)
+    n = int(net.meta["n"])
+    pb = ws.periodic_problem(n, sigma=float(net.meta["sigma"]))
+    W = ws.PeriodicShearWarmStart(pb.P, pb.ty, n, net, sigma=pb.sigma)
+    P = pb.P
+    gamma = 1e-3
+    Kd = gf.to_device(pb.K)
+    zero_grid = torch.zeros(n * n * 2, dtype=torch.float32, device="cuda")
+
+    def affine_guess():
+        x = torch.zeros(pb.ty.nnu, dtype=torch.float64, device="cuda")
+        gf._call("gf_grid_to_dofs", gf._i64(n * n * 2), gf._i32(2), gf._dp(zero_grid), gf._dp(W.idx),
+                 gf._dp(W.affine), gf._f64(1.0), gf._f64(gamma), gf._dp(x), gf._stream())
+        return x
+
+    rows = []
+    for k in range(args.ws_tests + 1):
+        seed = 900000 + k  # training fields use seeds 1000 + i
+        G = gf.to_device(np.exp(pb.sigma * gf.randn(seed, n * n)))
+        P.set_moduli(Kd, G)
+        b = W.rhs(gamma)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, it0, res0 = P.solve(b, tol=1e-10)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        xg = W.predict(G, gamma)
+        torch.cuda.synchronize()
+        t_inf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        xg = W.predict(G, gamma)
+        _, it1, res1 = P.solve(b, x0=xg, tol=1e-10)
+        torch.cuda.synchronize()
+        t_learned = time.perf_counter() - t0
+        _, it2, _ = P.solve(b, x0=affine_guess(), tol=1e-10)
+        if k == 0:
+            continue  # first field: warm-up of kernels and buffers
+        rows.append(dict(seed=seed, plain_s=t_plain, learned_s=t_learned, inference_s=t_inf, plain_iterations=it0,
+                         learned_iterations=it1, affine_only_iterations=it2, plain_relres=res0, learned_relres=res1))
+    mean = lambda key: float(np.mean([r[key] for r in rows]))  # noqa: E731
+    m = net.meta
+    res = {
+        "metric": "PCG wall time to true relative residual 1e-10, learned warm start (inference included)",
+        "value": mean("learned_s"), "unit": "s", "higher_is_better": False, "n_gpus": 1,
+        "plain_s": mean("plain_s"), "speedup_vs_plain": mean("plain_s") / mean("learned_s"),
+        "iterations": {"plain": mean("plain_iterations"), "learned": mean("learned_iterations"),
+                       "affine_only": mean("affine_only_iterations")},
+        "inference_ms": mean("inference_s") * 1e3,
+        "max_true_relres": max(max(r["plain_relres"], r["learned_relres"]) for r in rows),
+        "dtype": "bf16 mfma network (fp32 accumulate), fp64 pcg",
+        "data": "synthetic log-normal moduli (held-out seeds)",
+        "config": {"workload": f"periodic Quad4 Regular {n}^2 plane strain, G = exp({pb.sigma} z), K 10, "
+                               f"simple shear increment {gamma}", "nnu": pb.ty.nnu, "test_fields": len(rows),
+                   "channels": net.ch, "parameters": net.nparams()},
+        "training": {"samples": int(m["samples"]), "epochs": int(m["epochs"]), "batch": int(m["batch"]),
+                     "data_generation_s": float(m["data_seconds"]), "training_s": float(m["train_seconds"]),
+                     "final_mse": float(m["final_mse"])},
+        "per_field": rows,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,11 +370,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic"],
+    ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned"],
                     default="quad4-explicit")
     ap.add_argument("--pcg-n", type=int, default=512)
     ap.add_argument("--pcg-hex", action="store_true", help="periodic Hex8 n^3 instead of Quad4 n^2")
     ap.add_argument("--hex-n", type=int, default=128)
+    ap.add_argument("--ws-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "goosefem_amd",
+                                                         "data", "warmstart_quad4_512.npz"))
+    ap.add_argument("--ws-tests", type=int, default=4, help="held-out modulus fields timed")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC counter calibration) and exit")
     args = ap.parse_args()
@@ -325,6 +404,9 @@ def main():
         return
     if args.workload == "pcg-periodic":
         bench_pcg(args)
+        return
+    if args.workload == "pcg-learned":
+        bench_pcg_learned(args)
         return
 
     nx = ny = args.nx

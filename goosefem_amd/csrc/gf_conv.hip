@@ -247,6 +247,21 @@ __global__ void k_reduce_rows(int nsplit, int n, const float* __restrict__ part,
     out[i] = acc;
 }
 
+// out[0] = sum of part[0..n) by one workgroup: strided per-thread sums, then an LDS tree (fixed order)
+__global__ void __launch_bounds__(256) k_sum_block(int64_t n, const float* __restrict__ part, float* __restrict__ out)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) acc += part[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = static_cast<float>(sh[0]);
+}
+
 // ---- 2x2 pooling ------------------------------------------------------------------------------------
 // avg: out[n][y][x][c] = mean of the 2x2 block of in (H x W -> H/2 x W/2)
 __global__ void k_avgpool2(int64_t Pout, int Ho, int Wo, int C, const __bf16* __restrict__ in, __bf16* __restrict__ out)
@@ -588,7 +603,7 @@ int gf_mse(int64_t P, int nc, const float* pred, const float* target, float scal
         const int64_t nb = nblocks(P * cg, 256);
         hipStream_t s = S(stream);
         k_mse<<<nb, 256, 0, s>>>(P, nc, pred, target, scale, cg, static_cast<__bf16*>(g), part);
-        k_reduce_rows<<<1, 1, 0, s>>>(static_cast<int>(nb), 1, part, loss_sum);
+        k_sum_block<<<1, 256, 0, s>>>(nb, part, loss_sum);
         check_launch();
     });
 }

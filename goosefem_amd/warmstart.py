@@ -361,6 +361,18 @@ class PeriodicShearWarmStart:
         torch.cuda.synchronize()
         return Gs, T, its, time.perf_counter() - t0
 
+    def holdout_error(self, Gs, T):
+        """Mean relative L2 error of the prediction (up to a translation) against centred targets."""
+        n = self.n
+        errs = []
+        for k in range(Gs.shape[0]):
+            self.net.set_input(1, n, n, [Gs[k]], 0.0, 1.0 / self.sigma)
+            p = self.net.forward(1, n, n).reshape(n * n, 2)
+            p = p - p.mean(dim=0, keepdim=True)
+            t = T[k].reshape(n * n, 2)
+            errs.append(float(((p - t).norm() / t.norm()).item()))
+        return float(np.mean(errs))
+
     def train(self, Gs, T, epochs, batch=4, lr=2e-3, lr_min=1e-5, log_every=1, time_limit=None):
         """Adam with cosine learning-rate decay on (Gs, T); returns per-epoch mean losses."""
         torch = _torch()
@@ -397,3 +409,79 @@ class PeriodicShearWarmStart:
                 break
         del bf
         return hist
+
+
+def periodic_problem(n, sigma=0.4, seed=4, K=10.0):
+    """Periodic Quad4 Regular(n, n) cell with Tyings::Control + Tyings::Periodic as in
+    docs/examples/statics/periodic_elastoplastic.py:28-47 (iip = control DOFs + origin DOFs),
+    plane strain, per-element G = exp(sigma z), K constant; returns a namespace with the tied PCG."""
+    from types import SimpleNamespace
+
+    import goosefem_amd as gf
+
+    mesh = gf.Mesh.Quad4.Regular(n, n)
+    ctrl = gf.Tyings.Control(mesh.coor, mesh.dofs)
+    iip = np.concatenate((ctrl.controlDofs.ravel(), ctrl.dofs[mesh.nodesOrigin]))
+    ty = gf.Tyings.Periodic(ctrl.coor, ctrl.dofs, ctrl.controlDofs, mesh.nodesPeriodic, iip)
+    vec = gf.Vector(mesh.conn, ty.dofs)
+    quad = gf.Element.Quad4.QuadraturePlanar(gf.Vector(mesh.conn, mesh.dofs).AsElement(mesh.coor))
+    G = np.exp(sigma * randn(seed, mesh.nelem))
+    Kv = np.full(mesh.nelem, K)
+    P = gf.TyingsPCG(vec, quad, Kv, G, ty)
+    return SimpleNamespace(mesh=mesh, ty=ty, vec=vec, quad=quad, G=G, K=Kv, P=P, n=n, sigma=sigma)
+
+
+def train_main(argv=None):
+    """Generate PCG solutions for random modulus fields, train one or more networks on the device,
+    report their error on held-out fields, save the weights (npz) with the measured data-generation
+    and training times."""
+    import argparse
+    import os
+
+    torch = _torch()
+    ap = argparse.ArgumentParser(description="train the learned PCG warm start (periodic Quad4 n x n, simple shear)")
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--samples", type=int, default=400)
+    ap.add_argument("--holdout", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", default="2e-3", help="learning rate (';'-separated, one per configuration)")
+    ap.add_argument("--channels", default="16,32,64,128", help="per-level widths; ';' separates configurations")
+    ap.add_argument("--data-tol", type=float, default=1e-7)
+    ap.add_argument("--seed0", type=int, default=1000)
+    ap.add_argument("--time-limit", type=float, default=None, help="seconds of training per configuration at most")
+    ap.add_argument("--out", required=True, help="weights file (.npz); with several configurations a suffix is added")
+    a = ap.parse_args(argv)
+    pb = periodic_problem(a.n)
+    configs = [tuple(int(c) for c in cfg.split(",")) for cfg in a.channels.split(";")]
+    lrs = [float(x) for x in a.lr.split(";")]
+    lrs = lrs + [lrs[-1]] * (len(configs) - len(lrs))
+    probe = PeriodicShearWarmStart(pb.P, pb.ty, a.n, WarmStartNet(configs[0]), sigma=pb.sigma, target_scale=1.0)
+    seeds = list(range(a.seed0, a.seed0 + a.samples + a.holdout))
+    Gs, T, its, t_data = probe.make_dataset(pb.K, seeds, tol=a.data_tol, progress=50)
+    scale = float(T[: a.samples].std().item())
+    T.div_(scale)
+    print(f"data: {len(seeds)} solves, {its} CG iterations, {t_data:.1f} s; target scale {scale:.4e}", flush=True)
+    Gtr, Ttr, Gho, Tho = Gs[: a.samples], T[: a.samples], Gs[a.samples:], T[a.samples:]
+    for cfg, lr in zip(configs, lrs):
+        net = WarmStartNet(cfg)
+        W = PeriodicShearWarmStart(pb.P, pb.ty, a.n, net, sigma=pb.sigma, target_scale=scale)
+        t0 = time.perf_counter()
+        hist = W.train(Gtr, Ttr, a.epochs, batch=a.batch, lr=lr, time_limit=a.time_limit)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        err = W.holdout_error(Gho, Tho) if a.holdout else float("nan")
+        net.meta = {"target_scale": scale, "sigma": pb.sigma, "n": a.n, "samples": a.samples, "epochs": len(hist),
+                    "batch": a.batch, "lr": lr, "data_seconds": t_data, "train_seconds": t_train, "final_mse": hist[-1],
+                    "holdout_rel_error": err, "data_tol": a.data_tol, "seed0": a.seed0, "params": net.nparams()}
+        out = a.out
+        if len(configs) > 1:
+            stem, ext = os.path.splitext(a.out)
+            out = f"{stem}_{'-'.join(map(str, cfg))}{ext}"
+        net.save(out)
+        print(f"channels {cfg} lr {lr}: {net.nparams()} parameters, {len(hist)} epochs in {t_train:.1f} s; final mse "
+              f"{hist[-1]:.4e}; held-out relative error {err:.4f}; saved {out}", flush=True)
+
+
+if __name__ == "__main__":
+    train_main()

@@ -122,3 +122,17 @@ def test_error_path_reports_message(gf):
     rc = f(ctypes.c_int64(3), ctypes.c_int64(3), ctypes.c_int64(0), b"nodesNowhere", None, ctypes.byref(n))
     assert rc == gf.GF_EINVAL
     assert b"unknown node set" in gf.lib().gf_last_error()
+
+
+def test_warmstart_weights_file_is_plain_arrays():
+    """The committed network weights load without unpickling and carry their training record."""
+    import os
+
+    path = os.path.join(ROOT, "goosefem_amd", "data", "warmstart_quad4_512.npz")
+    with np.load(path, allow_pickle=False) as f:
+        ch = [int(c) for c in f["channels"]]
+        L = len(ch)
+        nlayers = 2 * L + 2 * (L - 1) + 1
+        assert all(f"w{i}" in f.files and f"b{i}" in f.files for i in range(nlayers))
+        assert f["w0"].shape == (9 * ch[0] * 1,) and f[f"w{nlayers - 1}"].shape == (9 * 2 * ch[0],)
+        assert int(f["meta_n"]) == 512 and float(f["meta_holdout_rel_error"]) < 0.1

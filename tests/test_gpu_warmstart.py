@@ -321,3 +321,23 @@ def test_warm_start_pcg_same_tolerance(gf, ws):
     xg = W.predict(torch.from_numpy(G).cuda(), gamma)
     _, it2, res2 = P.solve(b, x0=xg, tol=1e-10)
     assert res2 <= 1e-9 and it2 > 0
+
+
+def test_committed_weights_cut_pcg_iterations_at_512(gf, ws):
+    """The committed 512^2 network on a modulus field outside its training seeds: the warm-started
+    PCG reaches the same true residual with at least 25% fewer iterations than from a zero guess."""
+    import os
+
+    path = os.path.join(os.path.dirname(gf.__file__), "data", "warmstart_quad4_512.npz")
+    net = ws.WarmStartNet.load(path)
+    n = int(net.meta["n"])
+    pb = ws.periodic_problem(n, sigma=float(net.meta["sigma"]))
+    W = ws.PeriodicShearWarmStart(pb.P, pb.ty, n, net, sigma=pb.sigma)
+    G = gf.to_device(np.exp(pb.sigma * gf.randn(123456, n * n)))
+    pb.P.set_moduli(gf.to_device(pb.K), G)
+    b = W.rhs(1e-3)
+    _, it0, r0 = pb.P.solve(b, tol=1e-10)
+    x0 = W.predict(G, 1e-3)
+    _, it1, r1 = pb.P.solve(b, x0=x0, tol=1e-10)
+    assert r0 <= 1e-9 and r1 <= 1e-9
+    assert it1 < 0.75 * it0, (it0, it1)
