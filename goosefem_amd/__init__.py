@@ -896,6 +896,13 @@ class TyingsPCG(_Handle):
               _hp(self._cdp[1]), _hp(self._cdp[2]))
         self._h = h
 
+    @property
+    def structured(self):
+        """True when K' x_u runs in the fused structured periodic kernel."""
+        f = _i32(0)
+        _call("gf_pcg_is_structured", self._h, ctypes.byref(f))
+        return bool(f.value)
+
     def set_moduli(self, K_e, G_e):
         """Replace the per-element moduli (same mesh and tyings) and refresh the Jacobi diagonal."""
         self._K = to_device(K_e)

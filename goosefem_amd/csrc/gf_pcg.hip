@@ -50,9 +50,146 @@ struct gf_pcg_s {
     DBuf<double> partial;  // [4][NBR]
     DBuf<double> scal;     // [8]: rz (2 slots), bb, rr, tol2, ...
     DBuf<int32_t> state;   // [0] done, [1] iterations, [2] maxiter
+    // structured periodic Quad4 operator (Regular n x n cell, periodic tyings, uniform geometry)
+    bool structured = false;
+    int sn = 0, ntiles = 0, ntx = 0, per_xcd = 0;
+    double sh_g[32] = {}, sh_v[4] = {};
+    DBuf<int32_t> sidx; // [n*n*2] u-DOF of grid node (i, j) component c, or -1
 };
 
 namespace {
+
+// block reduction of one value per thread (block of NTH threads) into partial[blockIdx]; fixed tree
+template <int NTH>
+__device__ inline void block_reduce_store_n(double v, double* partial)
+{
+    __shared__ double shr[NTH];
+    shr[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = NTH / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) shr[threadIdx.x] += shr[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = shr[0];
+    __syncthreads();
+}
+
+// ---- one Gauss point of the plane-strain Quad4 element force, FP64, reference expression order ----
+// ElementQuad4Planar.h:191-221 (strain), the linear elastic stress, :253-278 (force); g = dN/dx of
+// the point [4][2], v = its dV; f accumulates
+__device__ inline void quad4_planar_qp_force(const double* g, double v, const double u[4][2], double k, double gm,
+                                             double f[4][2])
+{
+    double s[2][2];
+    double e00 = g[0] * u[0][0] + g[2] * u[1][0] + g[4] * u[2][0] + g[6] * u[3][0];
+    double e11 = g[1] * u[0][1] + g[3] * u[1][1] + g[5] * u[2][1] + g[7] * u[3][1];
+    double e01 = 0.5 * (g[0] * u[0][1] + g[2] * u[1][1] + g[4] * u[2][1] + g[6] * u[3][1] +
+                        g[1] * u[0][0] + g[3] * u[1][0] + g[5] * u[2][0] + g[7] * u[3][0]);
+    double e22 = 0.0;
+    double tr = e00 + e11 + e22;
+    s[0][0] = k * tr + 2.0 * gm * (e00 - tr / 3.0);
+    s[1][1] = k * tr + 2.0 * gm * (e11 - tr / 3.0);
+    s[0][1] = 2.0 * gm * e01;
+    s[1][0] = 2.0 * gm * e01;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f[m][0] += (g[2 * m] * s[0][0] + g[2 * m + 1] * s[1][0]) * v;
+        f[m][1] += (g[2 * m] * s[0][1] + g[2 * m + 1] * s[1][1]) * v;
+    }
+}
+
+// ---- structured periodic operator: Quad4 Regular n x n cell with periodic tyings ---------------------
+// Every element has the same dN/dx and dV (Regular mesh), so they are kernel arguments.  The tied
+// operator K' x_u on a periodic field is the element-force assembly on the torus: node (i, j)
+// collects the elements (i-1, j-1), (i, j-1), (i-1, j), (i, j) taken modulo n (MatrixPartitionedTyings
+// condensation with C_du = periodic copy, MatrixPartitionedTyings.h:655-674).  sidx maps grid node
+// (i, j), component c to its u-DOF, or -1 for a prescribed DOF (the origin), whose value is 0.
+struct ShapeQ4 {
+    double g[4][8];
+    double v[4];
+};
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, int ntiles, int per_xcd, ShapeQ4 sh,
+                                                               const int32_t* __restrict__ sidx,
+                                                               const double* __restrict__ Ke,
+                                                               const double* __restrict__ Ge,
+                                                               const double* __restrict__ x, double* __restrict__ y,
+                                                               double* __restrict__ partial,
+                                                               const int32_t* __restrict__ state)
+{
+    if (state && state[0]) return; // uniform for the grid
+    constexpr int HX = TX + 2, HY = TY + 2, EX = TX + 1, EY = TY + 1, NT = TX * TY;
+    __shared__ double2 s_x[HY * HX];
+    __shared__ double s_f[EY * EX][8];
+    const int b = blockIdx.x;
+    const int tile = per_xcd > 0 ? (b & 7) * per_xcd + (b >> 3) : b;
+    const int tx = tile % ntx, ty = tile / ntx;
+    const int x0 = tx * TX, y0 = ty * TY;
+    // halo tile of x (periodic wrap, prescribed DOFs read as 0)
+    for (int t = threadIdx.x; t < HX * HY; t += NT) {
+        const int ly = t / HX, lx = t - (t / HX) * HX;
+        int gi = x0 - 1 + lx, gj = y0 - 1 + ly;
+        gi += gi < 0 ? n : 0;
+        gi -= gi >= n ? n : 0;
+        gj += gj < 0 ? n : 0;
+        gj -= gj >= n ? n : 0;
+        const int64_t k = (static_cast<int64_t>(gj) * n + gi) * 2;
+        const int32_t d0 = sidx[k], d1 = sidx[k + 1];
+        s_x[t] = make_double2(d0 >= 0 ? x[d0] : 0.0, d1 >= 0 ? x[d1] : 0.0);
+    }
+    __syncthreads();
+    // element forces of the (TX+1) x (TY+1) elements touching the tile
+    for (int t = threadIdx.x; t < EX * EY; t += NT) {
+        const int ly = t / EX, lx = t - (t / EX) * EX;
+        int ei = x0 - 1 + lx, ej = y0 - 1 + ly;
+        ei += ei < 0 ? n : 0;
+        ej += ej < 0 ? n : 0;
+        const int64_t e = static_cast<int64_t>(ej) * n + ei;
+        const int n0 = ly * HX + lx;
+        const int nid[4] = {n0, n0 + 1, n0 + HX + 1, n0 + HX};
+        double u[4][2], f[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double2 xx = s_x[nid[m]];
+            u[m][0] = xx.x;
+            u[m][1] = xx.y;
+            f[m][0] = f[m][1] = 0.0;
+        }
+        const double k = Ke[e], gm = Ge[e];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) quad4_planar_qp_force(sh.g[q], sh.v[q], u, k, gm, f);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            s_f[t][2 * m] = f[m][0];
+            s_f[t][2 * m + 1] = f[m][1];
+        }
+    }
+    __syncthreads();
+    // owned node: elements (i-1, j-1) as its node 2, (i, j-1) node 3, (i-1, j) node 1, (i, j) node 0
+    const int oly = threadIdx.x / TX, olx = threadIdx.x - (threadIdx.x / TX) * TX;
+    const int l0 = oly * EX + olx;
+    double f0 = s_f[l0][4], f1 = s_f[l0][5];
+    f0 += s_f[l0 + 1][6];
+    f1 += s_f[l0 + 1][7];
+    f0 += s_f[l0 + EX][2];
+    f1 += s_f[l0 + EX][3];
+    f0 += s_f[l0 + EX + 1][0];
+    f1 += s_f[l0 + EX + 1][1];
+    const int64_t k = (static_cast<int64_t>(y0 + oly) * n + x0 + olx) * 2;
+    const int32_t d0 = sidx[k], d1 = sidx[k + 1];
+    double acc = 0.0;
+    if (d0 >= 0) {
+        y[d0] = f0;
+        acc += x[d0] * f0;
+    }
+    if (d1 >= 0) {
+        y[d1] = f1;
+        acc += x[d1] * f1;
+    }
+    (void)ntiles;
+    if (partial) block_reduce_store_n<NT>(acc, partial);
+}
 
 // ---- element force from a full dofval, FP64, reference expression order -------------------------
 template <int ET>
@@ -111,22 +248,8 @@ __global__ void k_elem_force(int64_t nelem, const int32_t* __restrict__ conn, co
                     f[m][j] += (g[m * 3 + 0] * s[0][j] + g[m * 3 + 1] * s[1][j] + g[m * 3 + 2] * s[2][j]) * v;
         }
         else {
-            // ElementQuad4Planar.h:191-221, :253-278
-            double e00 = g[0] * u[0][0] + g[2] * u[1][0] + g[4] * u[2][0] + g[6] * u[3][0];
-            double e11 = g[1] * u[0][1] + g[3] * u[1][1] + g[5] * u[2][1] + g[7] * u[3][1];
-            double e01 = 0.5 * (g[0] * u[0][1] + g[2] * u[1][1] + g[4] * u[2][1] + g[6] * u[3][1] +
-                                g[1] * u[0][0] + g[3] * u[1][0] + g[5] * u[2][0] + g[7] * u[3][0]);
-            double e22 = 0.0;
-            double tr = e00 + e11 + e22;
-            s[0][0] = k * tr + 2.0 * gm * (e00 - tr / 3.0);
-            s[1][1] = k * tr + 2.0 * gm * (e11 - tr / 3.0);
-            s[0][1] = 2.0 * gm * e01;
-            s[1][0] = 2.0 * gm * e01;
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                f[m][0] += (g[2 * m] * s[0][0] + g[2 * m + 1] * s[1][0]) * v;
-                f[m][1] += (g[2 * m] * s[0][1] + g[2 * m + 1] * s[1][1]) * v;
-            }
+            (void)s;
+            quad4_planar_qp_force(g, v, u, k, gm, f);
         }
     }
 #pragma unroll
@@ -182,12 +305,12 @@ __device__ inline void block_reduce_store(double v, double* partial)
     __syncthreads();
 }
 
-// sum of NBR partials in fixed order (every block computes the same value)
-__device__ inline double reduce_partials(const double* partial)
+// sum of `count` partials in fixed order (every block computes the same value)
+__device__ inline double reduce_partials(const double* partial, int count = NBR)
 {
     __shared__ double sh[BS];
     double v = 0.0;
-    for (int i = threadIdx.x; i < NBR; i += BS) v += partial[i];
+    for (int i = threadIdx.x; i < count; i += BS) v += partial[i];
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int s = BS / 2; s > 0; s >>= 1) {
@@ -298,14 +421,14 @@ __global__ void k_pcg_init_scalars(const double* __restrict__ partial, double* _
 }
 
 // alpha = rz / (p.q); x += alpha p; r -= alpha q; z = r ./ d; partials of rz_new, rr
-__global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, const double* __restrict__ pq_partial,
+__global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, const double* __restrict__ pq_partial, int npq,
                                                    const double* __restrict__ scal, const double* __restrict__ p,
                                                    const double* __restrict__ q, const double* __restrict__ invd,
                                                    double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
                                                    double* __restrict__ partial, const int32_t* __restrict__ state)
 {
     if (state[0]) return;
-    const double pq = reduce_partials(pq_partial);
+    const double pq = reduce_partials(pq_partial, npq);
     const int slot = state[1] & 1;
     const double alpha = scal[slot] / pq;
     double rz = 0.0, rr = 0.0;
@@ -389,6 +512,20 @@ void apply(gf_pcg_s* P, const double* xu, const double* xp, double* yu, bool xu_
            const double* w, double* partial, const int32_t* state)
 {
     auto v = P->v;
+    if (P->structured && !xp) {
+        const double* xs = xu_in_full ? P->xfull.p : xu;
+        if (!w || w == xs) {
+            ShapeQ4 sh;
+            for (int q = 0; q < 4; ++q) {
+                for (int k = 0; k < 8; ++k) sh.g[q][k] = P->sh_g[q * 8 + k];
+                sh.v[q] = P->sh_v[q];
+            }
+            k_apply_periodic_q4<32, 8><<<P->ntiles, 256, 0, s>>>(P->sn, P->ntx, P->ntiles, P->per_xcd, sh, P->sidx.p,
+                                                                 P->Ke.p, P->Ge.p, xs, yu, w ? partial : nullptr, state);
+            check_launch();
+            return;
+        }
+    }
     const int64_t nexp = (xu_in_full ? 0 : P->nnu) + P->nnp + P->nnd;
     if (nexp > 0)
         k_expand<<<std::min<unsigned>(nblocks(nexp, BS), 4096), BS, 0, s>>>(
@@ -425,6 +562,77 @@ void compute_invdiag(gf_pcg P, hipStream_t s)
     k_diag_fold<<<nblocks(P->nnu, BS), BS, 0, s>>>(P->nnu, P->nni, P->v->dof_eptr.p, P->v->dof_eidx.p, P->fe.p,
                                                   P->cduT.ptr.p, P->cduT.idx.p, P->cduT.val.p, P->invdiag.p);
     check_launch();
+}
+
+// Regular n x n Quad4 cell whose dependent DOFs are periodic copies (C_du rows = one unit entry of
+// the partner's u-DOF, or empty when the partner is prescribed) and whose elements share one
+// geometry: then K' x_u is applied by k_apply_periodic_q4.
+void detect_structured(gf_pcg P, const int64_t* cdu_ptr_h, const int64_t* cdu_col_h, const double* cdu_val_h)
+{
+    constexpr int TX = 32, TY = 8;
+    auto v = P->v;
+    if (P->et != GF_QUAD4_PLANAR) return;
+    const int64_t n = static_cast<int64_t>(std::llround(std::sqrt(static_cast<double>(P->nelem))));
+    if (n * n != P->nelem || n % TX != 0 || n % TY != 0 || n >= (1 << 15)) return;
+    const int64_t N1 = n + 1;
+    if (v->nnode < N1 * N1 || v->ndim != 2) return;
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t e = j * n + i, a = j * N1 + i;
+            const int64_t want[4] = {a, a + 1, a + N1 + 1, a + N1};
+            for (int m = 0; m < 4; ++m)
+                if (v->conn_h[e * 4 + m] != want[m]) return;
+        }
+    auto dof = [&](int64_t node, int c) { return v->dofs_h[node * 2 + c]; };
+    // independent grid nodes: u or p; dependent edge nodes: periodic copies of their partner
+    for (int64_t j = 0; j <= n; ++j)
+        for (int64_t i = 0; i <= n; ++i)
+            for (int c = 0; c < 2; ++c) {
+                const int64_t d = dof(j * N1 + i, c);
+                if (i < n && j < n) {
+                    if (d < 0 || d >= P->nni) return;
+                    continue;
+                }
+                if (d < P->nni || d >= P->ndof) return;
+                const int64_t pd = dof((j % n) * N1 + (i % n), c), r = d - P->nni;
+                const int64_t cnt = cdu_ptr_h[r + 1] - cdu_ptr_h[r];
+                if (pd < P->nnu) {
+                    if (cnt != 1 || cdu_col_h[cdu_ptr_h[r]] != pd || cdu_val_h[cdu_ptr_h[r]] != 1.0) return;
+                }
+                else if (cnt != 0)
+                    return;
+            }
+    // uniform geometry: every element's dN/dx and dV equal element 0's to 1e-12 relative (a Regular
+    // mesh gives the same Jacobian everywhere up to the rounding of the coordinate sums, ~1e-16)
+    std::vector<double> g(static_cast<size_t>(P->nelem) * 32), vol(static_cast<size_t>(P->nelem) * 4);
+    GF_HIP(hipMemcpy(g.data(), quad_dNx(P->q), g.size() * sizeof(double), hipMemcpyDeviceToHost));
+    GF_HIP(hipMemcpy(vol.data(), quad_vol(P->q), vol.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double gmax = 0.0, vmax = 0.0;
+    for (int k = 0; k < 32; ++k) gmax = std::max(gmax, std::abs(g[k]));
+    for (int k = 0; k < 4; ++k) vmax = std::max(vmax, std::abs(vol[k]));
+    if (!(gmax > 0.0 && vmax > 0.0)) return;
+    for (int64_t e = 1; e < P->nelem; ++e) {
+        for (int k = 0; k < 32; ++k)
+            if (std::abs(g[e * 32 + k] - g[k]) > 1e-12 * gmax) return;
+        for (int k = 0; k < 4; ++k)
+            if (std::abs(vol[e * 4 + k] - vol[k]) > 1e-12 * vmax) return;
+    }
+    std::vector<int32_t> sidx(static_cast<size_t>(n * n * 2));
+    for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < n; ++i)
+            for (int c = 0; c < 2; ++c) {
+                const int64_t d = dof(j * N1 + i, c);
+                sidx[(j * n + i) * 2 + c] = d < P->nnu ? static_cast<int32_t>(d) : -1;
+            }
+    P->sidx.alloc(sidx.size());
+    GF_HIP(hipMemcpy(P->sidx.p, sidx.data(), sidx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    for (int k = 0; k < 32; ++k) P->sh_g[k] = g[k];
+    for (int k = 0; k < 4; ++k) P->sh_v[k] = vol[k];
+    P->sn = static_cast<int>(n);
+    P->ntx = static_cast<int>(n / TX);
+    P->ntiles = static_cast<int>((n / TX) * (n / TY));
+    P->per_xcd = P->ntiles % 8 == 0 ? P->ntiles / 8 : 0;
+    P->structured = true;
 }
 
 } // namespace
@@ -480,9 +688,10 @@ int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const 
             GF_HIP(hipMemset(P->xfull.p, 0, P->ndof * sizeof(double)));
             P->fe.alloc(P->nelem * P->K);
             for (auto* b : {&P->invdiag, &P->x, &P->r, &P->z, &P->qv, &P->tmp}) b->alloc(nnu);
-            P->partial.alloc(4 * NBR);
             P->scal.alloc(8);
             P->state.alloc(4);
+            detect_structured(P, cdu_ptr_h, cdu_col_h, cdu_val_h);
+            P->partial.alloc(3 * NBR + std::max(NBR, P->ntiles));
             compute_invdiag(P, nullptr);
             GF_HIP(hipDeviceSynchronize());
         }
@@ -508,6 +717,14 @@ int gf_pcg_set_moduli(gf_pcg P, const double* K_e, const double* G_e, void* stre
         GF_HIP(hipMemcpyAsync(P->Ke.p, K_e, P->nelem * sizeof(double), hipMemcpyDeviceToDevice, s));
         GF_HIP(hipMemcpyAsync(P->Ge.p, G_e, P->nelem * sizeof(double), hipMemcpyDeviceToDevice, s));
         compute_invdiag(P, s);
+    });
+}
+
+int gf_pcg_is_structured(gf_pcg P, int* structured)
+{
+    return guard([&] {
+        GF_REQUIRE(P && structured, "null argument");
+        *structured = P->structured ? 1 : 0;
     });
 }
 
@@ -570,7 +787,7 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         for (int64_t it = 0; it < maxiter;) {
             for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
                 apply(P, nullptr, nullptr, P->qv.p, true, s, P->xfull.p, P->partial.p + 3 * NBR, P->state.p);
-                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->scal.p, P->xfull.p, P->qv.p,
+                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->structured ? P->ntiles : NBR, P->scal.p, P->xfull.p, P->qv.p,
                                                 P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p);
                 k_pcg_direction<<<NBR, BS, 0, s>>>(n, P->partial.p, P->scal.p, P->z.p, P->xfull.p, P->state.p);
                 k_pcg_count<<<1, 64, 0, s>>>(P->scal.p, P->state.p);

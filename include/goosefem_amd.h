@@ -301,6 +301,9 @@ int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const 
                   const int64_t* cdp_ptr_h, const int64_t* cdp_col_h, const double* cdp_val_h);
 int gf_pcg_destroy(gf_pcg p);
 int gf_pcg_sizes(gf_pcg p, int64_t* nnu, int64_t* nnp, int64_t* nnd);
+/* 1 if K' x_u is applied by the fused structured kernel (Quad4 Regular n x n cell, n a multiple of
+ * 32, periodic tyings, identical element geometry), else 0 (general element + gather path) */
+int gf_pcg_is_structured(gf_pcg p, int* structured);
 /* replace the per-element moduli (device, FP64 [nelem] each) and recompute the Jacobi diagonal,
  * enqueued on stream: the tied operator of a new material field without rebuilding the tyings
  * (MatrixPartitionedTyings::assemble + factorize, MatrixPartitionedTyings.h:210-305,655-674) */
