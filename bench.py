@@ -202,9 +202,51 @@ def bench_hex8(args):
     print(json.dumps(res))
 
 
-def periodic_setup(n, hex8=False, seed=4, sigma=0.4):
+def splitmix_uniform(seed, n):
+    """n uniform [0, 1) numbers from SplitMix64 (53-bit mantissa), identical on every platform."""
+    out = np.empty(n)
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    M = 0xFFFFFFFFFFFFFFFF
+    for i in range(n):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        out[i] = (z >> 11) * (1.0 / (1 << 53))
+    return out
+
+
+def stiff_spheres(n, radius=4.0, fraction=0.2, seed=105):
+    """Element mask of periodic spheres in an n^3 cell: centres uniform (seeded), added one at a
+    time until the fraction of element centres inside a sphere reaches `fraction`."""
+    inside = np.zeros((n, n, n), dtype=bool)  # [i, j, k], element (i, j, k) = (k*n + j)*n + i
+    u = splitmix_uniform(seed, 3 * 100000)
+    r = int(np.ceil(radius)) + 1
+    off = np.arange(-r, r + 1)
+    count, total, k = 0, n ** 3, 0
+    while count < fraction * total:
+        cen = n * u[3 * k: 3 * k + 3]
+        k += 1
+        idx = [np.floor(cen[a]).astype(int) + off for a in range(3)]
+        d2 = np.zeros((len(off),) * 3)
+        for a in range(3):
+            sh = [1, 1, 1]
+            sh[a] = -1
+            d2 = d2 + ((idx[a] + 0.5 - cen[a]) ** 2).reshape(sh)
+        sel = d2 <= radius * radius
+        I, J, K = np.meshgrid(idx[0] % n, idx[1] % n, idx[2] % n, indexing="ij")
+        new = sel & ~inside[I, J, K]
+        inside[I[new], J[new], K[new]] = True
+        count += int(new.sum())
+    return inside.transpose(2, 1, 0).ravel(), k
+
+
+def periodic_setup(n, hex8=False, seed=4, sigma=0.4, spheres=False):
     """Periodic Regular mesh with Tyings::Control + Tyings::Periodic as in
-    docs/examples/statics/periodic_elastoplastic.py:28-47; per-element G log-normal, K = 10."""
+    docs/examples/statics/periodic_elastoplastic.py:28-47; per-element G log-normal, K = 10;
+    with spheres (Hex8) G is multiplied by 10 inside periodic spheres of radius 4 elements filling
+    20% of the element centres."""
     import goosefem_amd as gf
 
     mesh = gf.Mesh.Hex8.Regular(n, n, n) if hex8 else gf.Mesh.Quad4.Regular(n, n)
@@ -216,16 +258,23 @@ def periodic_setup(n, hex8=False, seed=4, sigma=0.4):
     x = gf.Vector(conn, mesh.dofs).AsElement(coor)
     quad = gf.Element.Hex8.Quadrature(x) if hex8 else gf.Element.Quad4.QuadraturePlanar(x)
     G = np.exp(sigma * gf.randn(seed, mesh.nelem))
+    nsph = 0
+    if spheres:
+        mask, nsph = stiff_spheres(n)
+        G[mask] *= 10.0
     K = np.full(mesh.nelem, 10.0)
     P = gf.TyingsPCG(vec, quad, K, G, ty)
-    return dict(gf=gf, mesh=mesh, ty=ty, vec=vec, quad=quad, G=G, K=K, P=P, conn=conn, coor=coor)
+    return dict(gf=gf, mesh=mesh, ty=ty, vec=vec, quad=quad, G=G, K=K, P=P, conn=conn, coor=coor, nspheres=nsph)
 
 
-def shear_rhs(S, dgamma):
-    """b_u = -K'_up x_p for a simple-shear increment of the control dof ctrl(0, 1)."""
+def shear_rhs(S, dgamma, pure=False):
+    """b_u = -K'_up x_p for a shear increment: simple shear ctrl(0, 1) = dgamma, or pure shear
+    ctrl(0, 1) = ctrl(1, 0) = dgamma."""
     ty, P = S["ty"], S["P"]
     xp = np.zeros(ty.nnp)
     xp[ty.control[0, 1] - ty.nnu] = dgamma
+    if pure:
+        xp[ty.control[1, 0] - ty.nnu] = dgamma
     return -P.apply(None, xp), xp
 
 
@@ -234,9 +283,10 @@ def bench_pcg(args):
     import torch
 
     n = args.pcg_n
-    S = periodic_setup(n, args.pcg_hex)
+    S = periodic_setup(n, args.pcg_hex, spheres=args.spheres)
     P, ty = S["P"], S["ty"]
-    b, xp = shear_rhs(S, 1e-3)
+    gamma = 5e-4 if args.spheres else 1e-3
+    b, xp = shear_rhs(S, gamma, pure=args.spheres)
     torch.cuda.synchronize()
     P.solve(b, tol=1e-10)  # warm-up
     torch.cuda.synchronize()
@@ -247,8 +297,10 @@ def bench_pcg(args):
     res_d = {"metric": "PCG time to relative residual 1e-10 (plain, zero guess)", "value": t, "unit": "s",
              "higher_is_better": False,
              "config": {"workload": f"periodic {'Hex8' if args.pcg_hex else 'Quad4'} Regular {n}"
-                                    f"{'^3' if args.pcg_hex else '^2'}, log-normal G (sigma 0.4), K 10, "
-                                    f"simple shear increment 1e-3", "nnu": ty.nnu},
+                                    f"{'^3' if args.pcg_hex else '^2'}, log-normal G (sigma 0.4)"
+                                    f"{', G x10 in stiff spheres (r 4, 20%)' if args.spheres else ''}, K 10, "
+                                    f"{'pure' if args.spheres else 'simple'} shear increment {gamma}",
+                        "nnu": ty.nnu, "spheres": S["nspheres"], "structured_operator": P.structured},
              "iterations": it, "true_relres": res, "ms_per_iteration": t / max(it, 1) * 1e3}
     if not args.no_cpu:
         res_d["cpu_baseline"] = pcg_cpu_baseline(S, b, 50)
@@ -374,6 +426,8 @@ def main():
                     default="quad4-explicit")
     ap.add_argument("--pcg-n", type=int, default=512)
     ap.add_argument("--pcg-hex", action="store_true", help="periodic Hex8 n^3 instead of Quad4 n^2")
+    ap.add_argument("--spheres", action="store_true",
+                    help="stiff periodic spheres (G x10, radius 4, 20%% of elements) and pure shear 5e-4")
     ap.add_argument("--hex-n", type=int, default=128)
     ap.add_argument("--ws-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "goosefem_amd",
                                                          "data", "warmstart_quad4_512.npz"))

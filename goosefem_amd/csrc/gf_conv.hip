@@ -34,8 +34,10 @@ __device__ inline int wrap(int i, int n)
 // input channels [0, c1) come from x1 (at half resolution when up1: nearest upsampling is read in
 // place), [c1, c1 + c2) from x2.  Output channels [0, ysplit) go to y (row stride ysplit),
 // [ysplit, cout) to y2 (row stride cout - ysplit); y32 (optional) receives FP32 [P][cout].
-template <int NC>
-__global__ void __launch_bounds__(256) k_conv3x3(int64_t P, int H, int W, const __bf16* __restrict__ x1, int c1,
+// D = 2: 3x3 taps t = 3 (dy + 1) + dx + 1 on [N, H, W] (Z = 1); D = 3: 3x3x3 taps
+// t = 9 (dz + 1) + 3 (dy + 1) + dx + 1 on [N, Z, H, W].
+template <int NC, int D>
+__global__ void __launch_bounds__(256) k_conv3x3(int64_t P, int Z, int H, int W, const __bf16* __restrict__ x1, int c1,
                                                  int up1, const __bf16* __restrict__ x2, int c2,
                                                  const __bf16* __restrict__ wpk, const float* __restrict__ bias,
                                                  int cout, int flags, __bf16* __restrict__ y, int ysplit,
@@ -48,13 +50,16 @@ __global__ void __launch_bounds__(256) k_conv3x3(int64_t P, int H, int W, const 
     const int row = lane & 31, kh = lane >> 5;
     const int cin = c1 + c2;
     constexpr int COUTP = 32 * NC;
-    const int64_t HW = static_cast<int64_t>(H) * W;
+    constexpr int NTAP = D == 3 ? 27 : 9;
+    const int64_t HW = static_cast<int64_t>(H) * W, ZHW = HW * Z;
     const int64_t p = p0 + row;
     const bool pv = p < P;
-    const int64_t pn = pv ? p / HW : 0;
-    const int rem = pv ? static_cast<int>(p - pn * HW) : 0;
+    const int64_t pn = pv ? p / ZHW : 0;
+    const int64_t rz = pv ? p - pn * ZHW : 0;
+    const int pz = static_cast<int>(rz / HW);
+    const int rem = static_cast<int>(rz - static_cast<int64_t>(pz) * HW);
     const int py = rem / W, px = rem - (rem / W) * W;
-    const int H1 = up1 ? H >> 1 : H, W1 = up1 ? W >> 1 : W;
+    const int Z1 = (up1 && D == 3) ? Z >> 1 : Z, H1 = up1 ? H >> 1 : H, W1 = up1 ? W >> 1 : W;
     floatx16 acc[NC];
 #pragma unroll
     for (int j = 0; j < NC; ++j)
@@ -64,11 +69,13 @@ __global__ void __launch_bounds__(256) k_conv3x3(int64_t P, int H, int W, const 
 #pragma unroll
     for (int i = 0; i < 8; ++i) zero[i] = static_cast<__bf16>(0.0f);
 
-    for (int t = 0; t < 9; ++t) {
-        const int yy = wrap(py + t / 3 - 1, H), xx = wrap(px + t % 3 - 1, W);
-        const int y1 = up1 ? yy >> 1 : yy, xq = up1 ? xx >> 1 : xx;
-        const __bf16* a1 = x1 + ((pn * H1 + y1) * W1 + xq) * c1 + 8 * kh;
-        const __bf16* a2 = x2 ? x2 + ((pn * H + yy) * W + xx) * c2 + 8 * kh : nullptr;
+    for (int t = 0; t < NTAP; ++t) {
+        const int t2 = D == 3 ? t % 9 : t;
+        const int zz = D == 3 ? wrap(pz + t / 9 - 1, Z) : 0;
+        const int yy = wrap(py + t2 / 3 - 1, H), xx = wrap(px + t2 % 3 - 1, W);
+        const int z1 = (up1 && D == 3) ? zz >> 1 : zz, y1 = up1 ? yy >> 1 : yy, xq = up1 ? xx >> 1 : xx;
+        const __bf16* a1 = x1 + (((pn * Z1 + z1) * H1 + y1) * W1 + xq) * c1 + 8 * kh;
+        const __bf16* a2 = x2 ? x2 + (((pn * Z + zz) * H + yy) * W + xx) * c2 + 8 * kh : nullptr;
         const __bf16* wt = wpk + (static_cast<int64_t>(t) * COUTP + row) * cin + 8 * kh;
         for (int c0 = 0; c0 < cin; c0 += 16) {
             bf16x8 a = zero;
@@ -109,7 +116,8 @@ constexpr int WG_PIX = 64;  // pixels per LDS tile
 constexpr int WG_LD = 72;   // LDS row pitch (bf16), 144 B: 16-byte aligned rows
 constexpr int WG_MAXC = 256;
 
-__global__ void __launch_bounds__(256) k_conv3x3_wgrad(int64_t P, int H, int W, const __bf16* __restrict__ x1, int c1,
+template <int D>
+__global__ void __launch_bounds__(256) k_conv3x3_wgrad(int64_t P, int Z, int H, int W, const __bf16* __restrict__ x1, int c1,
                                                        int up1, const __bf16* __restrict__ x2, int c2,
                                                        const __bf16* __restrict__ g, int cg, int nsplit,
                                                        float* __restrict__ part, int coutp, int cinp)
@@ -124,9 +132,10 @@ __global__ void __launch_bounds__(256) k_conv3x3_wgrad(int64_t P, int H, int W, 
     const int row = lane & 31, kh = lane >> 5;
     const int cin = c1 + c2;
     const int nci = (cin + 31) / 32;
-    const int64_t HW = static_cast<int64_t>(H) * W;
-    const int H1 = up1 ? H >> 1 : H, W1 = up1 ? W >> 1 : W;
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const int64_t HW = static_cast<int64_t>(H) * W, ZHW = HW * Z;
+    const int Z1 = (up1 && D == 3) ? Z >> 1 : Z, H1 = up1 ? H >> 1 : H, W1 = up1 ? W >> 1 : W;
+    const int t2 = D == 3 ? t % 9 : t;
+    const int dz = D == 3 ? t / 9 - 1 : 0, dy = t2 / 3 - 1, dx = t2 % 3 - 1;
     floatx16 acc[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k)
@@ -154,16 +163,19 @@ __global__ void __launch_bounds__(256) k_conv3x3_wgrad(int64_t P, int H, int W, 
             const int64_t p = pbase + pp;
             bf16x8 v;
             if (p < P && c8 < cin) {
-                const int64_t pn = p / HW;
-                const int rem = static_cast<int>(p - pn * HW);
+                const int64_t pn = p / ZHW;
+                const int64_t rz = p - pn * ZHW;
+                const int pz = static_cast<int>(rz / HW);
+                const int rem = static_cast<int>(rz - static_cast<int64_t>(pz) * HW);
                 const int py = rem / W, px = rem - (rem / W) * W;
+                const int zz = D == 3 ? wrap(pz + dz, Z) : 0;
                 const int yy = wrap(py + dy, H), xx = wrap(px + dx, W);
                 if (c8 < c1) {
-                    const int y1 = up1 ? yy >> 1 : yy, xq = up1 ? xx >> 1 : xx;
-                    v = *reinterpret_cast<const bf16x8*>(x1 + ((pn * H1 + y1) * W1 + xq) * c1 + c8);
+                    const int z1 = (up1 && D == 3) ? zz >> 1 : zz, y1 = up1 ? yy >> 1 : yy, xq = up1 ? xx >> 1 : xx;
+                    v = *reinterpret_cast<const bf16x8*>(x1 + (((pn * Z1 + z1) * H1 + y1) * W1 + xq) * c1 + c8);
                 }
                 else
-                    v = *reinterpret_cast<const bf16x8*>(x2 + ((pn * H + yy) * W + xx) * c2 + (c8 - c1));
+                    v = *reinterpret_cast<const bf16x8*>(x2 + (((pn * Z + zz) * H + yy) * W + xx) * c2 + (c8 - c1));
             }
             else
                 for (int i = 0; i < 8; ++i) v[i] = static_cast<__bf16>(0.0f);
@@ -186,7 +198,8 @@ __global__ void __launch_bounds__(256) k_conv3x3_wgrad(int64_t P, int H, int W, 
         __syncthreads();
     }
     // acc[k][r]: co = ct*32 + 8 (r / 4) + 4 kh + r % 4, ci = (wave + 4k)*32 + row
-    float* dst = part + (static_cast<int64_t>(split) * 9 + t) * coutp * cinp;
+    constexpr int NTAP = D == 3 ? 27 : 9;
+    float* dst = part + (static_cast<int64_t>(split) * NTAP + t) * coutp * cinp;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int ci_t = wave + 4 * k;
@@ -201,16 +214,16 @@ __global__ void __launch_bounds__(256) k_conv3x3_wgrad(int64_t P, int H, int W, 
 }
 
 // dw[t][co][ci] (master shape cout x cin) = sum_s part[s][t][co][ci], splits in ascending order
-__global__ void k_wgrad_reduce(int nsplit, int coutp, int cinp, int cout, int cin, const float* __restrict__ part,
+__global__ void k_wgrad_reduce(int ntap, int nsplit, int coutp, int cinp, int cout, int cin, const float* __restrict__ part,
                                float* __restrict__ dw)
 {
-    const int64_t n = 9LL * cout * cin;
+    const int64_t n = static_cast<int64_t>(ntap) * cout * cin;
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) return;
     const int t = static_cast<int>(i / (static_cast<int64_t>(cout) * cin));
     const int rem = static_cast<int>(i - static_cast<int64_t>(t) * cout * cin);
     const int co = rem / cin, ci = rem - (rem / cin) * cin;
-    const int64_t stride = 9LL * coutp * cinp;
+    const int64_t stride = static_cast<int64_t>(ntap) * coutp * cinp;
     const float* s = part + (static_cast<int64_t>(t) * coutp + co) * cinp + ci;
     float acc = 0.0f;
     for (int k = 0; k < nsplit; ++k) acc += s[k * stride];
@@ -262,73 +275,90 @@ __global__ void __launch_bounds__(256) k_sum_block(int64_t n, const float* __res
     if (threadIdx.x == 0) out[0] = static_cast<float>(sh[0]);
 }
 
-// ---- 2x2 pooling ------------------------------------------------------------------------------------
-// avg: out[n][y][x][c] = mean of the 2x2 block of in (H x W -> H/2 x W/2)
-__global__ void k_avgpool2(int64_t Pout, int Ho, int Wo, int C, const __bf16* __restrict__ in, __bf16* __restrict__ out)
+// ---- 2x2 (D = 2) / 2x2x2 (D = 3) pooling --------------------------------------------------------------
+// grids [N, Z, H, W, C] with Z = 1 for D = 2; the coarse grid halves every pooled dimension
+template <int D>
+__device__ inline float block_sum(const __bf16* b, int64_t sy, int64_t sz, int C)
+{
+    float s = static_cast<float>(b[0]) + static_cast<float>(b[C]) + static_cast<float>(b[sy]) +
+              static_cast<float>(b[sy + C]);
+    if constexpr (D == 3)
+        s += static_cast<float>(b[sz]) + static_cast<float>(b[sz + C]) + static_cast<float>(b[sz + sy]) +
+             static_cast<float>(b[sz + sy + C]);
+    return s;
+}
+
+// avg: out = mean of the 2^D block of in
+template <int D>
+__global__ void k_avgpool2(int64_t Pout, int Zo, int Ho, int Wo, int C, const __bf16* __restrict__ in,
+                           __bf16* __restrict__ out)
 {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= Pout * C) return;
     const int c = static_cast<int>(i % C);
     const int64_t p = i / C;
-    const int64_t HWo = static_cast<int64_t>(Ho) * Wo;
-    const int64_t n = p / HWo;
-    const int rem = static_cast<int>(p - n * HWo);
+    const int64_t HWo = static_cast<int64_t>(Ho) * Wo, ZHWo = HWo * Zo;
+    const int64_t n = p / ZHWo, rz = p - n * ZHWo;
+    const int z = static_cast<int>(rz / HWo), rem = static_cast<int>(rz - z * HWo);
     const int y = rem / Wo, x = rem - (rem / Wo) * Wo;
-    const int Wi = 2 * Wo;
-    const __bf16* b = in + ((n * 2 * Ho + 2 * y) * Wi + 2 * x) * static_cast<int64_t>(C) + c;
-    const float s = static_cast<float>(b[0]) + static_cast<float>(b[C]) + static_cast<float>(b[static_cast<int64_t>(Wi) * C]) +
-                    static_cast<float>(b[static_cast<int64_t>(Wi) * C + C]);
-    out[i] = static_cast<__bf16>(0.25f * s);
+    const int Zi = D == 3 ? 2 * Zo : Zo, Hi = 2 * Ho, Wi = 2 * Wo;
+    const int zi = D == 3 ? 2 * z : z;
+    const __bf16* b = in + (((n * Zi + zi) * Hi + 2 * y) * Wi + 2 * x) * static_cast<int64_t>(C) + c;
+    const int64_t sy = static_cast<int64_t>(Wi) * C, sz = sy * Hi;
+    out[i] = static_cast<__bf16>((D == 3 ? 0.125f : 0.25f) * block_sum<D>(b, sy, sz, C));
 }
 
-// fine-grid gradient from a coarse one: d_in[fine] (+)= w * d_out[coarse]; then masked by act > 0.
-// w = 0.25 for average pooling, 1 for nearest upsampling read backwards (sum over the 2x2 block is
-// the other direction, k_sumpool2).
-__global__ void k_unpool2(int64_t Pin, int Hi, int Wi, int C, float w, const __bf16* __restrict__ dout, int accum,
-                          const __bf16* __restrict__ act, __bf16* __restrict__ din)
+// fine-grid gradient from a coarse one: din[fine] (+)= w * dout[coarse]; then masked by act > 0
+// (w = 1 / 2^D for average pooling)
+template <int D>
+__global__ void k_unpool2(int64_t Pin, int Zi, int Hi, int Wi, int C, float w, const __bf16* __restrict__ dout,
+                          int accum, const __bf16* __restrict__ act, __bf16* __restrict__ din)
 {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= Pin * C) return;
     const int c = static_cast<int>(i % C);
     const int64_t p = i / C;
-    const int64_t HWi = static_cast<int64_t>(Hi) * Wi;
-    const int64_t n = p / HWi;
-    const int rem = static_cast<int>(p - n * HWi);
+    const int64_t HWi = static_cast<int64_t>(Hi) * Wi, ZHWi = HWi * Zi;
+    const int64_t n = p / ZHWi, rz = p - n * ZHWi;
+    const int z = static_cast<int>(rz / HWi), rem = static_cast<int>(rz - z * HWi);
     const int y = rem / Wi, x = rem - (rem / Wi) * Wi;
-    const int Ho = Hi / 2, Wo = Wi / 2;
-    float v = w * static_cast<float>(dout[((n * Ho + y / 2) * Wo + x / 2) * static_cast<int64_t>(C) + c]);
+    const int Zo = D == 3 ? Zi / 2 : Zi, Ho = Hi / 2, Wo = Wi / 2;
+    const int zo = D == 3 ? z / 2 : z;
+    float v = w * static_cast<float>(dout[(((n * Zo + zo) * Ho + y / 2) * Wo + x / 2) * static_cast<int64_t>(C) + c]);
     if (accum) v += static_cast<float>(din[i]);
     if (act && !(static_cast<float>(act[i]) > 0.0f)) v = 0.0f;
     din[i] = static_cast<__bf16>(v);
 }
 
-// coarse gradient of nearest upsampling: out[coarse] = sum of the 2x2 fine block; masked by act > 0
-__global__ void k_sumpool2(int64_t Pout, int Ho, int Wo, int C, const __bf16* __restrict__ in,
+// coarse gradient of nearest upsampling: out[coarse] = sum of the 2^D fine block; masked by act > 0
+template <int D>
+__global__ void k_sumpool2(int64_t Pout, int Zo, int Ho, int Wo, int C, const __bf16* __restrict__ in,
                            const __bf16* __restrict__ act, __bf16* __restrict__ out)
 {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= Pout * C) return;
     const int c = static_cast<int>(i % C);
     const int64_t p = i / C;
-    const int64_t HWo = static_cast<int64_t>(Ho) * Wo;
-    const int64_t n = p / HWo;
-    const int rem = static_cast<int>(p - n * HWo);
+    const int64_t HWo = static_cast<int64_t>(Ho) * Wo, ZHWo = HWo * Zo;
+    const int64_t n = p / ZHWo, rz = p - n * ZHWo;
+    const int z = static_cast<int>(rz / HWo), rem = static_cast<int>(rz - z * HWo);
     const int y = rem / Wo, x = rem - (rem / Wo) * Wo;
-    const int Wi = 2 * Wo;
-    const __bf16* b = in + ((n * 2 * Ho + 2 * y) * Wi + 2 * x) * static_cast<int64_t>(C) + c;
-    float s = static_cast<float>(b[0]) + static_cast<float>(b[C]) + static_cast<float>(b[static_cast<int64_t>(Wi) * C]) +
-              static_cast<float>(b[static_cast<int64_t>(Wi) * C + C]);
-    if (act && !(static_cast<float>(act[i]) > 0.0f)) s = 0.0f;
-    out[i] = static_cast<__bf16>(s);
+    const int Zi = D == 3 ? 2 * Zo : Zo, Hi = 2 * Ho, Wi = 2 * Wo;
+    const int zi = D == 3 ? 2 * z : z;
+    const __bf16* b = in + (((n * Zi + zi) * Hi + 2 * y) * Wi + 2 * x) * static_cast<int64_t>(C) + c;
+    const int64_t sy = static_cast<int64_t>(Wi) * C, sz = sy * Hi;
+    float sum = block_sum<D>(b, sy, sz, C);
+    if (act && !(static_cast<float>(act[i]) > 0.0f)) sum = 0.0f;
+    out[i] = static_cast<__bf16>(sum);
 }
 
 // ---- weights ------------------------------------------------------------------------------------------
 // master w [9][cout][cin] FP32 -> forward pack [9][coutp][cin_s] and data-gradient pack
 // [9][cinp][cout_s] (tap-flipped, transposed), zero padded
-__global__ void k_pack(int cout, int cin, const float* __restrict__ w, int coutp, int cin_s, __bf16* __restrict__ wf,
-                       int cinp, int cout_s, __bf16* __restrict__ wb)
+__global__ void k_pack(int ntap, int cout, int cin, const float* __restrict__ w, int coutp, int cin_s,
+                       __bf16* __restrict__ wf, int cinp, int cout_s, __bf16* __restrict__ wb)
 {
-    const int64_t nf = 9LL * coutp * cin_s, nb = 9LL * cinp * cout_s;
+    const int64_t nf = static_cast<int64_t>(ntap) * coutp * cin_s, nb = static_cast<int64_t>(ntap) * cinp * cout_s;
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i < nf) {
         const int t = static_cast<int>(i / (static_cast<int64_t>(coutp) * cin_s));
@@ -341,7 +371,7 @@ __global__ void k_pack(int cout, int cin, const float* __restrict__ w, int coutp
         const int t = static_cast<int>(i / (static_cast<int64_t>(cinp) * cout_s));
         const int rem = static_cast<int>(i - static_cast<int64_t>(t) * cinp * cout_s);
         const int ci = rem / cout_s, co = rem - (rem / cout_s) * cout_s;
-        const float v = (co < cout && ci < cin) ? w[(static_cast<int64_t>(8 - t) * cout + co) * cin + ci] : 0.0f;
+        const float v = (co < cout && ci < cin) ? w[(static_cast<int64_t>(ntap - 1 - t) * cout + co) * cin + ci] : 0.0f;
         wb[i] = static_cast<__bf16>(v);
     }
 }
@@ -445,12 +475,133 @@ __global__ void k_input_logG(int64_t P, const double* __restrict__ G, double shi
     x[i] = static_cast<__bf16>(c == 0 ? static_cast<float>(scale * (log(G[p]) - shift)) : 0.0f);
 }
 
-void check_conv_shapes(int64_t P, int H, int W, int c1, int up1, int c2)
+bool aligned16(const void* p)
 {
-    GF_REQUIRE(P > 0 && H > 0 && W > 0 && P % (static_cast<int64_t>(H) * W) == 0, "P must be a multiple of H*W");
+    return reinterpret_cast<uintptr_t>(p) % 16 == 0;
+}
+
+void check_conv_shapes(int64_t P, int Z, int H, int W, int c1, int up1, int c2, int D)
+{
+    GF_REQUIRE(P > 0 && Z > 0 && H > 0 && W > 0 && P % (static_cast<int64_t>(Z) * H * W) == 0,
+               "P must be a multiple of Z*H*W");
+    GF_REQUIRE(D == 3 || Z == 1, "2-D grids have Z = 1");
+    GF_REQUIRE(!up1 || D == 2 || Z % 2 == 0, "upsampled input needs even Z");
     GF_REQUIRE(c1 > 0 && c1 % 16 == 0 && c2 >= 0 && c2 % 16 == 0, "channel counts must be multiples of 16");
     GF_REQUIRE(c1 + c2 <= WG_MAXC, "at most 256 input channels");
     GF_REQUIRE(!up1 || (H % 2 == 0 && W % 2 == 0), "upsampled input needs even H, W");
+}
+
+
+template <int D>
+void conv_impl(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int Z, int H, int W,
+               const void* wpk, int coutp, const float* bias, int cout, int flags, void* y, int ysplit, void* y2,
+               float* y32, const void* mask, void* stream)
+{
+    check_conv_shapes(P, Z, H, W, c1, up1, c2, D);
+    GF_REQUIRE(x1 && wpk && (c2 == 0 || x2), "null input");
+    GF_REQUIRE(aligned16(x1) && aligned16(x2) && aligned16(wpk), "inputs and weights must be 16-byte aligned");
+    GF_REQUIRE(coutp == 32 || coutp == 64 || coutp == 128 || coutp == 256, "coutp must be 32, 64, 128 or 256");
+    GF_REQUIRE(cout > 0 && cout <= coutp, "cout must be in (0, coutp]");
+    GF_REQUIRE(y || y32, "no output");
+    GF_REQUIRE(!y || (ysplit > 0 && ysplit <= cout && (ysplit == cout || y2)), "bad output split");
+    hipStream_t s = S(stream);
+    const unsigned grid = static_cast<unsigned>((P + 127) / 128);
+    auto X1 = static_cast<const __bf16*>(x1);
+    auto X2 = static_cast<const __bf16*>(x2);
+    auto Wp = static_cast<const __bf16*>(wpk);
+    auto Y = static_cast<__bf16*>(y);
+    auto Y2 = static_cast<__bf16*>(y2);
+    auto M = static_cast<const __bf16*>(mask);
+    const int ys = y ? ysplit : cout;
+    switch (coutp) {
+    case 32:
+        k_conv3x3<1, D><<<grid, 256, 0, s>>>(P, Z, H, W, X1, c1, up1, X2, c2, Wp, bias, cout, flags, Y, ys, Y2, y32, M);
+        break;
+    case 64:
+        k_conv3x3<2, D><<<grid, 256, 0, s>>>(P, Z, H, W, X1, c1, up1, X2, c2, Wp, bias, cout, flags, Y, ys, Y2, y32, M);
+        break;
+    case 128:
+        k_conv3x3<4, D><<<grid, 256, 0, s>>>(P, Z, H, W, X1, c1, up1, X2, c2, Wp, bias, cout, flags, Y, ys, Y2, y32, M);
+        break;
+    default:
+        k_conv3x3<8, D><<<grid, 256, 0, s>>>(P, Z, H, W, X1, c1, up1, X2, c2, Wp, bias, cout, flags, Y, ys, Y2, y32, M);
+    }
+    check_launch();
+}
+
+void wgrad_work(int ntap, int64_t P, int cg, int cin, int* nsplit, int64_t* work_floats)
+{
+    GF_REQUIRE(P > 0 && cg > 0 && cin > 0 && nsplit && work_floats, "bad argument");
+    const int64_t nchunk = (P + WG_PIX - 1) / WG_PIX;
+    const int coutp = (cg + 31) / 32 * 32, cinp = (cin + 31) / 32 * 32;
+    // about 2048 blocks over (tap, co tile, split), at least 4 chunks per split
+    const int ns = static_cast<int>(
+        std::min<int64_t>(std::max<int64_t>(nchunk / 4, 1), std::max(1, 2048 / (ntap * coutp / 32))));
+    *nsplit = ns;
+    *work_floats = static_cast<int64_t>(ns) * ntap * coutp * cinp + static_cast<int64_t>(ns) * cg;
+}
+
+template <int D>
+void wgrad_impl(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int Z, int H, int W, const void* g,
+                int cg, int cout, int cin, float* work, int64_t work_floats, float* dw, float* db, void* stream)
+{
+    constexpr int NTAP = D == 3 ? 27 : 9;
+    check_conv_shapes(P, Z, H, W, c1, up1, c2, D);
+    GF_REQUIRE(x1 && g && work && dw && (c2 == 0 || x2), "null argument");
+    GF_REQUIRE(aligned16(x1) && aligned16(x2) && aligned16(g), "inputs must be 16-byte aligned");
+    GF_REQUIRE(cg > 0 && cg % 16 == 0 && cg <= 256, "g channels must be a multiple of 16, at most 256");
+    GF_REQUIRE(cout > 0 && cout <= cg && cin > 0 && cin <= c1 + c2, "master shape exceeds the tensors");
+    int ns = 0;
+    int64_t need = 0;
+    wgrad_work(NTAP, P, cg, c1 + c2, &ns, &need);
+    GF_REQUIRE(work_floats >= need, "workspace too small (gf_conv*_wgrad_work)");
+    const int coutp = (cg + 31) / 32 * 32, cinp = (c1 + c2 + 31) / 32 * 32;
+    hipStream_t s = S(stream);
+    const unsigned grid = static_cast<unsigned>(NTAP * (coutp / 32) * ns);
+    k_conv3x3_wgrad<D><<<grid, 256, 0, s>>>(P, Z, H, W, static_cast<const __bf16*>(x1), c1, up1,
+                                            static_cast<const __bf16*>(x2), c2, static_cast<const __bf16*>(g), cg, ns,
+                                            work, coutp, cinp);
+    const int64_t n = static_cast<int64_t>(NTAP) * cout * cin;
+    k_wgrad_reduce<<<nblocks(n, 256), 256, 0, s>>>(NTAP, ns, coutp, cinp, cout, cin, work, dw);
+    if (db) {
+        float* bpart = work + static_cast<int64_t>(ns) * NTAP * coutp * cinp;
+        k_bias_grad<<<ns, 256, 0, s>>>(P, static_cast<const __bf16*>(g), cg, cout, ns, bpart);
+        k_reduce_rows<<<nblocks(cout, 256), 256, 0, s>>>(ns, cout, bpart, db);
+    }
+    check_launch();
+}
+
+template <int D>
+void pool_impl(int kind, const void* in, int64_t N, int Z, int H, int W, int C, float w, int accum, const void* act,
+               void* out, void* stream)
+{
+    GF_REQUIRE(in && out && N > 0 && Z > 0 && H % 2 == 0 && W % 2 == 0 && C > 0 && (D == 2 ? Z == 1 : Z % 2 == 0),
+               "bad argument (even pooled dimensions)");
+    hipStream_t s = S(stream);
+    const int Zo = D == 3 ? Z / 2 : Z;
+    const int64_t Pfine = N * Z * H * W, Pcoarse = N * Zo * (H / 2) * (W / 2);
+    auto I = static_cast<const __bf16*>(in);
+    auto A = static_cast<const __bf16*>(act);
+    auto O = static_cast<__bf16*>(out);
+    if (kind == 0)
+        k_avgpool2<D><<<nblocks(Pcoarse * C, 256), 256, 0, s>>>(Pcoarse, Zo, H / 2, W / 2, C, I, O);
+    else if (kind == 1)
+        k_unpool2<D><<<nblocks(Pfine * C, 256), 256, 0, s>>>(Pfine, Z, H, W, C, w, I, accum, A, O);
+    else
+        k_sumpool2<D><<<nblocks(Pcoarse * C, 256), 256, 0, s>>>(Pcoarse, Zo, H / 2, W / 2, C, I, A, O);
+    check_launch();
+}
+
+void pack_impl(int ntap, int cout, int cin, const float* w, int coutp, int cin_s, void* wf, int cinp, int cout_s,
+               void* wb, void* stream)
+{
+    GF_REQUIRE(w && wf && cout <= coutp && cin <= cin_s, "bad argument");
+    GF_REQUIRE(!wb || (cin <= cinp && cout <= cout_s), "bad transposed shape");
+    const int64_t n = std::max(static_cast<int64_t>(ntap) * coutp * cin_s,
+                               wb ? static_cast<int64_t>(ntap) * cinp * cout_s : int64_t(0));
+    k_pack<<<nblocks(n, 256), 256, 0, S(stream)>>>(ntap, cout, cin, w, coutp, cin_s, static_cast<__bf16*>(wf), cinp,
+                                                  cout_s, static_cast<__bf16*>(wb));
+    check_launch();
 }
 
 } // namespace
@@ -462,125 +613,88 @@ int gf_conv3x3(const void* x1, int c1, int up1, const void* x2, int c2, int64_t 
                const void* mask, void* stream)
 {
     return guard([&] {
-        check_conv_shapes(P, H, W, c1, up1, c2);
-        GF_REQUIRE(x1 && wpk && (c2 == 0 || x2), "null input");
-        GF_REQUIRE(coutp == 32 || coutp == 64 || coutp == 128 || coutp == 256, "coutp must be 32, 64, 128 or 256");
-        GF_REQUIRE(cout > 0 && cout <= coutp, "cout must be in (0, coutp]");
-        GF_REQUIRE(y || y32, "no output");
-        GF_REQUIRE(!y || (ysplit > 0 && ysplit <= cout && (ysplit == cout || y2)), "bad output split");
-        hipStream_t s = S(stream);
-        const unsigned grid = static_cast<unsigned>((P + 127) / 128);
-        auto X1 = static_cast<const __bf16*>(x1);
-        auto X2 = static_cast<const __bf16*>(x2);
-        auto Wp = static_cast<const __bf16*>(wpk);
-        auto Y = static_cast<__bf16*>(y);
-        auto Y2 = static_cast<__bf16*>(y2);
-        auto M = static_cast<const __bf16*>(mask);
-        const int ys = y ? ysplit : cout;
-        switch (coutp) {
-        case 32:
-            k_conv3x3<1><<<grid, 256, 0, s>>>(P, H, W, X1, c1, up1, X2, c2, Wp, bias, cout, flags, Y, ys, Y2, y32, M);
-            break;
-        case 64:
-            k_conv3x3<2><<<grid, 256, 0, s>>>(P, H, W, X1, c1, up1, X2, c2, Wp, bias, cout, flags, Y, ys, Y2, y32, M);
-            break;
-        case 128:
-            k_conv3x3<4><<<grid, 256, 0, s>>>(P, H, W, X1, c1, up1, X2, c2, Wp, bias, cout, flags, Y, ys, Y2, y32, M);
-            break;
-        default:
-            k_conv3x3<8><<<grid, 256, 0, s>>>(P, H, W, X1, c1, up1, X2, c2, Wp, bias, cout, flags, Y, ys, Y2, y32, M);
-        }
-        check_launch();
+        conv_impl<2>(x1, c1, up1, x2, c2, P, 1, H, W, wpk, coutp, bias, cout, flags, y, ysplit, y2, y32, mask, stream);
+    });
+}
+
+int gf_conv3x3x3(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int Z, int H, int W,
+                 const void* wpk, int coutp, const float* bias, int cout, int flags, void* y, int ysplit, void* y2,
+                 float* y32, const void* mask, void* stream)
+{
+    return guard([&] {
+        conv_impl<3>(x1, c1, up1, x2, c2, P, Z, H, W, wpk, coutp, bias, cout, flags, y, ysplit, y2, y32, mask, stream);
     });
 }
 
 int gf_conv3x3_wgrad_work(int64_t P, int cg, int cin, int* nsplit, int64_t* work_floats)
 {
-    return guard([&] {
-        GF_REQUIRE(P > 0 && cg > 0 && cin > 0 && nsplit && work_floats, "bad argument");
-        const int64_t nchunk = (P + WG_PIX - 1) / WG_PIX;
-        const int coutp = (cg + 31) / 32 * 32, cinp = (cin + 31) / 32 * 32;
-        // about 2048 blocks over (tap, co tile, split), at least 4 chunks per split
-        int ns = static_cast<int>(std::min<int64_t>(std::max<int64_t>(nchunk / 4, 1), std::max(1, 2048 / (9 * coutp / 32))));
-        *nsplit = ns;
-        *work_floats = static_cast<int64_t>(ns) * 9 * coutp * cinp + static_cast<int64_t>(ns) * cg;
-    });
+    return guard([&] { wgrad_work(9, P, cg, cin, nsplit, work_floats); });
+}
+
+int gf_conv3x3x3_wgrad_work(int64_t P, int cg, int cin, int* nsplit, int64_t* work_floats)
+{
+    return guard([&] { wgrad_work(27, P, cg, cin, nsplit, work_floats); });
 }
 
 int gf_conv3x3_wgrad(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int H, int W, const void* g,
                      int cg, int cout, int cin, float* work, int64_t work_floats, float* dw, float* db, void* stream)
 {
     return guard([&] {
-        check_conv_shapes(P, H, W, c1, up1, c2);
-        GF_REQUIRE(x1 && g && work && dw && (c2 == 0 || x2), "null argument");
-        GF_REQUIRE(cg > 0 && cg % 16 == 0 && cg <= 256, "g channels must be a multiple of 16, at most 256");
-        GF_REQUIRE(cout > 0 && cout <= cg && cin > 0 && cin <= c1 + c2, "master shape exceeds the tensors");
-        int ns = 0;
-        int64_t need = 0;
-        gf_conv3x3_wgrad_work(P, cg, c1 + c2, &ns, &need);
-        GF_REQUIRE(work_floats >= need, "workspace too small (gf_conv3x3_wgrad_work)");
-        const int coutp = (cg + 31) / 32 * 32, cinp = (c1 + c2 + 31) / 32 * 32;
-        hipStream_t s = S(stream);
-        const unsigned grid = static_cast<unsigned>(9 * (coutp / 32) * ns);
-        k_conv3x3_wgrad<<<grid, 256, 0, s>>>(P, H, W, static_cast<const __bf16*>(x1), c1, up1,
-                                             static_cast<const __bf16*>(x2), c2, static_cast<const __bf16*>(g), cg, ns,
-                                             work, coutp, cinp);
-        const int64_t n = 9LL * cout * cin;
-        k_wgrad_reduce<<<nblocks(n, 256), 256, 0, s>>>(ns, coutp, cinp, cout, cin, work, dw);
-        if (db) {
-            float* bpart = work + static_cast<int64_t>(ns) * 9 * coutp * cinp;
-            k_bias_grad<<<ns, 256, 0, s>>>(P, static_cast<const __bf16*>(g), cg, cout, ns, bpart);
-            k_reduce_rows<<<nblocks(cout, 256), 256, 0, s>>>(ns, cout, bpart, db);
-        }
-        check_launch();
+        wgrad_impl<2>(x1, c1, up1, x2, c2, P, 1, H, W, g, cg, cout, cin, work, work_floats, dw, db, stream);
+    });
+}
+
+int gf_conv3x3x3_wgrad(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int Z, int H, int W,
+                       const void* g, int cg, int cout, int cin, float* work, int64_t work_floats, float* dw, float* db,
+                       void* stream)
+{
+    return guard([&] {
+        wgrad_impl<3>(x1, c1, up1, x2, c2, P, Z, H, W, g, cg, cout, cin, work, work_floats, dw, db, stream);
     });
 }
 
 int gf_avgpool2(const void* in, int64_t N, int H, int W, int C, void* out, void* stream)
 {
-    return guard([&] {
-        GF_REQUIRE(in && out && N > 0 && H % 2 == 0 && W % 2 == 0 && C > 0, "bad argument");
-        const int64_t Po = N * (H / 2) * (W / 2);
-        k_avgpool2<<<nblocks(Po * C, 256), 256, 0, S(stream)>>>(Po, H / 2, W / 2, C, static_cast<const __bf16*>(in),
-                                                              static_cast<__bf16*>(out));
-        check_launch();
-    });
+    return guard([&] { pool_impl<2>(0, in, N, 1, H, W, C, 0.0f, 0, nullptr, out, stream); });
 }
 
 int gf_unpool2(const void* dout, int64_t N, int H, int W, int C, float w, int accum, const void* act, void* din,
                void* stream)
 {
-    return guard([&] {
-        GF_REQUIRE(dout && din && N > 0 && H % 2 == 0 && W % 2 == 0 && C > 0, "bad argument");
-        const int64_t Pi = N * H * W;
-        k_unpool2<<<nblocks(Pi * C, 256), 256, 0, S(stream)>>>(Pi, H, W, C, w, static_cast<const __bf16*>(dout), accum,
-                                                             static_cast<const __bf16*>(act), static_cast<__bf16*>(din));
-        check_launch();
-    });
+    return guard([&] { pool_impl<2>(1, dout, N, 1, H, W, C, w, accum, act, din, stream); });
 }
 
 int gf_sumpool2(const void* in, int64_t N, int H, int W, int C, const void* act, void* out, void* stream)
 {
-    return guard([&] {
-        GF_REQUIRE(in && out && N > 0 && H % 2 == 0 && W % 2 == 0 && C > 0, "bad argument");
-        const int64_t Po = N * (H / 2) * (W / 2);
-        k_sumpool2<<<nblocks(Po * C, 256), 256, 0, S(stream)>>>(Po, H / 2, W / 2, C, static_cast<const __bf16*>(in),
-                                                              static_cast<const __bf16*>(act), static_cast<__bf16*>(out));
-        check_launch();
-    });
+    return guard([&] { pool_impl<2>(2, in, N, 1, H, W, C, 0.0f, 0, act, out, stream); });
+}
+
+int gf_avgpool2x2x2(const void* in, int64_t N, int Z, int H, int W, int C, void* out, void* stream)
+{
+    return guard([&] { pool_impl<3>(0, in, N, Z, H, W, C, 0.0f, 0, nullptr, out, stream); });
+}
+
+int gf_unpool2x2x2(const void* dout, int64_t N, int Z, int H, int W, int C, float w, int accum, const void* act,
+                   void* din, void* stream)
+{
+    return guard([&] { pool_impl<3>(1, dout, N, Z, H, W, C, w, accum, act, din, stream); });
+}
+
+int gf_sumpool2x2x2(const void* in, int64_t N, int Z, int H, int W, int C, const void* act, void* out, void* stream)
+{
+    return guard([&] { pool_impl<3>(2, in, N, Z, H, W, C, 0.0f, 0, act, out, stream); });
 }
 
 int gf_conv_pack(int cout, int cin, const float* w, int coutp, int cin_s, void* wf, int cinp, int cout_s, void* wb,
                  void* stream)
 {
-    return guard([&] {
-        GF_REQUIRE(w && wf && cout <= coutp && cin <= cin_s, "bad argument");
-        GF_REQUIRE(!wb || (cin <= cinp && cout <= cout_s), "bad transposed shape");
-        const int64_t n = std::max(9LL * coutp * cin_s, wb ? 9LL * cinp * cout_s : 0LL);
-        k_pack<<<nblocks(n, 256), 256, 0, S(stream)>>>(cout, cin, w, coutp, cin_s, static_cast<__bf16*>(wf), cinp,
-                                                      cout_s, static_cast<__bf16*>(wb));
-        check_launch();
-    });
+    return guard([&] { pack_impl(9, cout, cin, w, coutp, cin_s, wf, cinp, cout_s, wb, stream); });
+}
+
+int gf_conv_pack3(int cout, int cin, const float* w, int coutp, int cin_s, void* wf, int cinp, int cout_s, void* wb,
+                  void* stream)
+{
+    return guard([&] { pack_impl(27, cout, cin, w, coutp, cin_s, wf, cinp, cout_s, wb, stream); });
 }
 
 int gf_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float beta1, float beta2, float eps,

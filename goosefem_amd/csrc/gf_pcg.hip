@@ -54,7 +54,10 @@ struct gf_pcg_s {
     bool structured = false;
     int sn = 0, ntiles = 0, ntx = 0, per_xcd = 0;
     double sh_g[32] = {}, sh_v[4] = {};
-    DBuf<int32_t> sidx; // [n*n*2] u-DOF of grid node (i, j) component c, or -1
+    DBuf<int32_t> sidx; // [n^d * nd] u-DOF of grid node component c, or -1
+    int nty = 0;
+    DBuf<double> ke8;   // Hex8: [2][24][24] element matrices per unit K and per unit G
+    DBuf<double> p2;    // Quad4: second search-direction buffer of the fused iteration
 };
 
 namespace {
@@ -72,6 +75,23 @@ __device__ inline void block_reduce_store_n(double v, double* partial)
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = shr[0];
     __syncthreads();
+}
+
+// sum of `count` partials in fixed order (every block computes the same value)
+__device__ inline double reduce_partials(const double* partial, int count = NBR)
+{
+    __shared__ double sh[BS];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < count; i += BS) v += partial[i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
 }
 
 // ---- one Gauss point of the plane-strain Quad4 element force, FP64, reference expression order ----
@@ -109,24 +129,47 @@ struct ShapeQ4 {
     double v[4];
 };
 
-template <int TX, int TY>
+// FUSED (CG iteration it): the operand is the new search direction p = z + beta p_prev, built on the
+// fly (beta = rz_it / rz_it-1 from the ordered partials of the previous update, identical in every
+// block); the owned nodes' p is written to `pcur` (x is then z, `pprev` the previous direction).
+// Every block takes the same convergence decision from the same partials; block 0 records rz_it
+// and raises the done flag.
+template <int TX, int TY, bool FUSED>
 __global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, int ntiles, int per_xcd, ShapeQ4 sh,
                                                                const int32_t* __restrict__ sidx,
                                                                const double* __restrict__ Ke,
                                                                const double* __restrict__ Ge,
                                                                const double* __restrict__ x, double* __restrict__ y,
                                                                double* __restrict__ partial,
-                                                               const int32_t* __restrict__ state)
+                                                               const int32_t* __restrict__ state_c,
+                                                               const double* __restrict__ pprev, double* __restrict__ pcur,
+                                                               const double* __restrict__ rz_rr, double* __restrict__ scal,
+                                                               int32_t* __restrict__ state, int64_t it)
 {
-    if (state && state[0]) return; // uniform for the grid
+    if (state_c && state_c[0]) return; // uniform for the grid
     constexpr int HX = TX + 2, HY = TY + 2, EX = TX + 1, EY = TY + 1, NT = TX * TY;
     __shared__ double2 s_x[HY * HX];
     __shared__ double s_f[EY * EX][8];
+    double beta = 0.0;
+    if constexpr (FUSED) {
+        if (it > 0) {
+            const double rz_new = reduce_partials(rz_rr), rr = reduce_partials(rz_rr + NBR);
+            if (rr <= scal[4]) { // converged after `it` iterations
+                if (blockIdx.x == 0 && threadIdx.x == 0) state[0] = 1;
+                return;
+            }
+            beta = rz_new / scal[(it - 1) & 1];
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                scal[it & 1] = rz_new;
+                scal[3] = rr;
+            }
+        }
+    }
     const int b = blockIdx.x;
     const int tile = per_xcd > 0 ? (b & 7) * per_xcd + (b >> 3) : b;
     const int tx = tile % ntx, ty = tile / ntx;
     const int x0 = tx * TX, y0 = ty * TY;
-    // halo tile of x (periodic wrap, prescribed DOFs read as 0)
+    // halo tile of the operand (periodic wrap, prescribed DOFs read as 0)
     for (int t = threadIdx.x; t < HX * HY; t += NT) {
         const int ly = t / HX, lx = t - (t / HX) * HX;
         int gi = x0 - 1 + lx, gj = y0 - 1 + ly;
@@ -136,7 +179,16 @@ __global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, i
         gj -= gj >= n ? n : 0;
         const int64_t k = (static_cast<int64_t>(gj) * n + gi) * 2;
         const int32_t d0 = sidx[k], d1 = sidx[k + 1];
-        s_x[t] = make_double2(d0 >= 0 ? x[d0] : 0.0, d1 >= 0 ? x[d1] : 0.0);
+        double v0 = 0.0, v1 = 0.0;
+        if constexpr (FUSED) {
+            if (d0 >= 0) v0 = it > 0 ? x[d0] + beta * pprev[d0] : x[d0];
+            if (d1 >= 0) v1 = it > 0 ? x[d1] + beta * pprev[d1] : x[d1];
+        }
+        else {
+            v0 = d0 >= 0 ? x[d0] : 0.0;
+            v1 = d1 >= 0 ? x[d1] : 0.0;
+        }
+        s_x[t] = make_double2(v0, v1);
     }
     __syncthreads();
     // element forces of the (TX+1) x (TY+1) elements touching the tile
@@ -178,17 +230,109 @@ __global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, i
     f1 += s_f[l0 + EX + 1][1];
     const int64_t k = (static_cast<int64_t>(y0 + oly) * n + x0 + olx) * 2;
     const int32_t d0 = sidx[k], d1 = sidx[k + 1];
+    const double2 po = s_x[(oly + 1) * HX + olx + 1]; // the operand at the owned node
     double acc = 0.0;
     if (d0 >= 0) {
         y[d0] = f0;
-        acc += x[d0] * f0;
+        acc += po.x * f0;
+        if constexpr (FUSED) pcur[d0] = po.x;
     }
     if (d1 >= 0) {
         y[d1] = f1;
-        acc += x[d1] * f1;
+        acc += po.y * f1;
+        if constexpr (FUSED) pcur[d1] = po.y;
     }
     (void)ntiles;
     if (partial) block_reduce_store_n<NT>(acc, partial);
+}
+
+// Hex8 Regular n^3 cell with periodic tyings: identical element geometry makes the element
+// stiffness K_e = K_e,bulk KK + G_e KG with two fixed 24 x 24 matrices (isotropic linear elasticity,
+// C = K I(x)I + 2G (I_sym - I(x)I/3)).  Each thread owns one node and sums, over its 8 elements,
+// the node's 3 rows of K_e u_e; the x halo of the 8 x 8 x 4 node tile sits in LDS (per component).
+template <int TX, int TY, int TZ>
+__global__ void __launch_bounds__(TX * TY * TZ) k_apply_periodic_h8(int n, int ntx, int nty, const double* __restrict__ kkg,
+                                                                    const int32_t* __restrict__ sidx,
+                                                                    const double* __restrict__ Ke,
+                                                                    const double* __restrict__ Ge,
+                                                                    const double* __restrict__ x, double* __restrict__ y,
+                                                                    double* __restrict__ partial,
+                                                                    const int32_t* __restrict__ state)
+{
+    if (state && state[0]) return;
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, NH = HX * HY * HZ, NT = TX * TY * TZ;
+    __shared__ double s_x[3][NH];
+    const int tile = blockIdx.x;
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+    for (int t = threadIdx.x; t < NH; t += NT) {
+        const int lz = t / (HX * HY), r = t - lz * HX * HY, ly = r / HX, lx = r - (r / HX) * HX;
+        int gi = x0 - 1 + lx, gj = y0 - 1 + ly, gk = z0 - 1 + lz;
+        gi += gi < 0 ? n : 0;
+        gi -= gi >= n ? n : 0;
+        gj += gj < 0 ? n : 0;
+        gj -= gj >= n ? n : 0;
+        gk += gk < 0 ? n : 0;
+        gk -= gk >= n ? n : 0;
+        const int64_t k = ((static_cast<int64_t>(gk) * n + gj) * n + gi) * 3;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int32_t d = sidx[k + c];
+            s_x[c][t] = d >= 0 ? x[d] : 0.0;
+        }
+    }
+    __syncthreads();
+    const int olz = threadIdx.x / (TX * TY), orr = threadIdx.x - olz * TX * TY, oly = orr / TX,
+              olx = orr - (orr / TX) * TX;
+    // local node offsets of the Hex8 connectivity: 0 (0,0,0) 1 (1,0,0) 2 (1,1,0) 3 (0,1,0) 4..7 at z + 1
+    // (OX, OY, OZ)[m] = (((m + 1) >> 1) & 1, (m >> 1) & 1, m >> 2)
+    constexpr int OX[8] = {0, 1, 1, 0, 0, 1, 1, 0}, OY[8] = {0, 0, 1, 1, 0, 0, 1, 1}, OZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int el = 0; el < 8; ++el) {
+        // element whose lower corner is the owned node minus (1 - e): the owned node is its local node la
+        const int ex = ((el + 1) >> 1) & 1, ey = (el >> 1) & 1, ez = el >> 2;
+        const int ax = 1 - ex, ay = 1 - ey, az = 1 - ez;
+        const int la = az * 4 + (ay ? (ax ? 2 : 3) : (ax ? 1 : 0));
+        int ei = x0 + olx - 1 + ex, ej = y0 + oly - 1 + ey, ek = z0 + olz - 1 + ez;
+        ei += ei < 0 ? n : 0;
+        ej += ej < 0 ? n : 0;
+        ek += ek < 0 ? n : 0;
+        const int64_t e = (static_cast<int64_t>(ek) * n + ej) * n + ei;
+        // element nodes in the halo tile: lower corner (olx + ex, oly + ey, olz + ez)
+        const int c0 = ((olz + ez) * HY + (oly + ey)) * HX + (olx + ex);
+        double ue[24];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int h = c0 + (OZ[m] * HY + OY[m]) * HX + OX[m];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ue[3 * m + c] = s_x[c][h];
+        }
+        const double k = Ke[e], gm = Ge[e];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double* rk = kkg + (la * 3 + j) * 24;
+            const double* rg = kkg + 576 + (la * 3 + j) * 24;
+            double sk = 0.0, sg = 0.0;
+#pragma unroll
+            for (int c = 0; c < 24; ++c) {
+                sk += rk[c] * ue[c];
+                sg += rg[c] * ue[c];
+            }
+            acc[j] += k * sk + gm * sg;
+        }
+    }
+    const int64_t kn = ((static_cast<int64_t>(z0 + olz) * n + y0 + oly) * n + x0 + olx) * 3;
+    double dot = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int32_t d = sidx[kn + c];
+        if (d >= 0) {
+            y[d] = acc[c];
+            dot += x[d] * acc[c];
+        }
+    }
+    if (partial) block_reduce_store_n<NT>(dot, partial);
 }
 
 // ---- element force from a full dofval, FP64, reference expression order -------------------------
@@ -305,22 +449,6 @@ __device__ inline void block_reduce_store(double v, double* partial)
     __syncthreads();
 }
 
-// sum of `count` partials in fixed order (every block computes the same value)
-__device__ inline double reduce_partials(const double* partial, int count = NBR)
-{
-    __shared__ double sh[BS];
-    double v = 0.0;
-    for (int i = threadIdx.x; i < count; i += BS) v += partial[i];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = BS / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    double r = sh[0];
-    __syncthreads();
-    return r;
-}
 
 // out_u = y_u + C_du^T y_d with y = ordered gather of fe; optional partial of dot(out, w)
 __global__ void __launch_bounds__(BS) k_gather_fold(int64_t nnu, int64_t nni, const int32_t* __restrict__ eptr,
@@ -425,11 +553,12 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, const double* __re
                                                    const double* __restrict__ scal, const double* __restrict__ p,
                                                    const double* __restrict__ q, const double* __restrict__ invd,
                                                    double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
-                                                   double* __restrict__ partial, const int32_t* __restrict__ state)
+                                                   double* __restrict__ partial, const int32_t* __restrict__ state,
+                                                   int64_t it)
 {
     if (state[0]) return;
     const double pq = reduce_partials(pq_partial, npq);
-    const int slot = state[1] & 1;
+    const int slot = static_cast<int>(it & 1);
     const double alpha = scal[slot] / pq;
     double rz = 0.0, rr = 0.0;
     for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * BS) {
@@ -443,35 +572,28 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n, const double* __re
     }
     block_reduce_store(rz, partial);
     block_reduce_store(rr, partial + NBR);
+    if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<int32_t*>(state)[1] = static_cast<int32_t>(it + 1);
 }
 
-// beta = rz_new / rz_old; p = z + beta p; block 0 records rz_new, counts the iteration and
-// raises the done flag (state[0]) at convergence or at the iteration cap
+// beta = rz_new / rz_old; p = z + beta p.  Block 0 records rz_new in the other slot, stores the
+// iteration count and raises the done flag (state[0]) at convergence or at the iteration cap.  The
+// iteration number comes from the host (it equals the device count until the flag is raised, after
+// which every kernel returns at once), so no block reads a value that block 0 writes in this launch
+// except the flag itself, and a block that sees the flag raised skips a p update nobody uses.
 __global__ void __launch_bounds__(BS) k_pcg_direction(int64_t n, const double* __restrict__ partial,
                                                       double* __restrict__ scal, const double* __restrict__ z,
-                                                      double* __restrict__ p, int32_t* __restrict__ state)
+                                                      double* __restrict__ p, int32_t* __restrict__ state, int64_t it)
 {
     if (state[0]) return;
     const double rz_new = reduce_partials(partial), rr = reduce_partials(partial + NBR);
-    const int it = state[1];
     const double beta = rz_new / scal[it & 1];
     for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * BS)
         p[i] = z[i] + beta * p[i];
-    // every block read scal[it & 1] and state before this point; block 0 publishes the next
-    // iteration's values (other blocks of this launch never read the slot written here)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[(it + 1) & 1] = rz_new;
         scal[3] = rr;
-    }
-}
-
-// separate tiny kernel so that no block of k_pcg_direction observes the update of state[1]
-__global__ void k_pcg_count(const double* __restrict__ scal, int32_t* __restrict__ state)
-{
-    if (state[0]) return;
-    if (threadIdx.x == 0) {
-        state[1] += 1;
-        if (scal[3] <= scal[4] || state[1] >= state[2]) state[0] = 1;
+        state[1] = static_cast<int32_t>(it + 1);
+        if (rr <= scal[4] || it + 1 >= state[2]) state[0] = 1;
     }
 }
 
@@ -514,14 +636,22 @@ void apply(gf_pcg_s* P, const double* xu, const double* xp, double* yu, bool xu_
     auto v = P->v;
     if (P->structured && !xp) {
         const double* xs = xu_in_full ? P->xfull.p : xu;
+        if ((!w || w == xs) && P->et == GF_HEX8) {
+            k_apply_periodic_h8<8, 8, 4><<<P->ntiles, 256, 0, s>>>(P->sn, P->ntx, P->nty, P->ke8.p, P->sidx.p, P->Ke.p,
+                                                                   P->Ge.p, xs, yu, w ? partial : nullptr, state);
+            check_launch();
+            return;
+        }
         if (!w || w == xs) {
             ShapeQ4 sh;
             for (int q = 0; q < 4; ++q) {
                 for (int k = 0; k < 8; ++k) sh.g[q][k] = P->sh_g[q * 8 + k];
                 sh.v[q] = P->sh_v[q];
             }
-            k_apply_periodic_q4<32, 8><<<P->ntiles, 256, 0, s>>>(P->sn, P->ntx, P->ntiles, P->per_xcd, sh, P->sidx.p,
-                                                                 P->Ke.p, P->Ge.p, xs, yu, w ? partial : nullptr, state);
+            k_apply_periodic_q4<32, 8, false><<<P->ntiles, 256, 0, s>>>(P->sn, P->ntx, P->ntiles, P->per_xcd, sh,
+                                                                        P->sidx.p, P->Ke.p, P->Ge.p, xs, yu,
+                                                                        w ? partial : nullptr, state, nullptr,
+                                                                        nullptr, nullptr, nullptr, nullptr, 0);
             check_launch();
             return;
         }
@@ -632,6 +762,98 @@ void detect_structured(gf_pcg P, const int64_t* cdu_ptr_h, const int64_t* cdu_co
     P->ntx = static_cast<int>(n / TX);
     P->ntiles = static_cast<int>((n / TX) * (n / TY));
     P->per_xcd = P->ntiles % 8 == 0 ? P->ntiles / 8 : 0;
+    P->p2.alloc(P->nnu);
+    P->structured = true;
+}
+
+// Hex8 Regular n x n x n cell with periodic copies as dependent DOFs and identical element
+// geometry (to 1e-12): K' x_u is applied by k_apply_periodic_h8 with K_e = K KK + G KG.
+void detect_structured_hex8(gf_pcg P, const int64_t* cdu_ptr_h, const int64_t* cdu_col_h, const double* cdu_val_h)
+{
+    constexpr int TX = 8, TY = 8, TZ = 4;
+    auto v = P->v;
+    if (P->et != GF_HEX8 || v->ndim != 3) return;
+    const int64_t n = static_cast<int64_t>(std::llround(std::cbrt(static_cast<double>(P->nelem))));
+    if (n * n * n != P->nelem || n % TX != 0 || n % TY != 0 || n % TZ != 0 || n >= 1024) return;
+    const int64_t N1 = n + 1;
+    if (v->nnode < N1 * N1 * N1) return;
+    auto node = [&](int64_t i, int64_t j, int64_t k) { return (k * N1 + j) * N1 + i; };
+    static const int OX[8] = {0, 1, 1, 0, 0, 1, 1, 0}, OY[8] = {0, 0, 1, 1, 0, 0, 1, 1}, OZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    for (int64_t k = 0; k < n; ++k)
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t e = (k * n + j) * n + i;
+                for (int m = 0; m < 8; ++m)
+                    if (v->conn_h[e * 8 + m] != node(i + OX[m], j + OY[m], k + OZ[m])) return;
+            }
+    auto dof = [&](int64_t nd, int c) { return v->dofs_h[nd * 3 + c]; };
+    for (int64_t k = 0; k <= n; ++k)
+        for (int64_t j = 0; j <= n; ++j)
+            for (int64_t i = 0; i <= n; ++i)
+                for (int c = 0; c < 3; ++c) {
+                    const int64_t d = dof(node(i, j, k), c);
+                    if (i < n && j < n && k < n) {
+                        if (d < 0 || d >= P->nni) return;
+                        continue;
+                    }
+                    if (d < P->nni || d >= P->ndof) return;
+                    const int64_t pd = dof(node(i % n, j % n, k % n), c), r = d - P->nni;
+                    const int64_t cnt = cdu_ptr_h[r + 1] - cdu_ptr_h[r];
+                    if (pd < P->nnu) {
+                        if (cnt != 1 || cdu_col_h[cdu_ptr_h[r]] != pd || cdu_val_h[cdu_ptr_h[r]] != 1.0) return;
+                    }
+                    else if (cnt != 0)
+                        return;
+                }
+    constexpr int NGE = 8 * 8 * 3;
+    std::vector<double> g(static_cast<size_t>(P->nelem) * NGE), vol(static_cast<size_t>(P->nelem) * 8);
+    GF_HIP(hipMemcpy(g.data(), quad_dNx(P->q), g.size() * sizeof(double), hipMemcpyDeviceToHost));
+    GF_HIP(hipMemcpy(vol.data(), quad_vol(P->q), vol.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double gmax = 0.0, vmax = 0.0;
+    for (int t = 0; t < NGE; ++t) gmax = std::max(gmax, std::abs(g[t]));
+    for (int t = 0; t < 8; ++t) vmax = std::max(vmax, std::abs(vol[t]));
+    if (!(gmax > 0.0 && vmax > 0.0)) return;
+    for (int64_t e = 1; e < P->nelem; ++e) {
+        for (int t = 0; t < NGE; ++t)
+            if (std::abs(g[e * NGE + t] - g[t]) > 1e-12 * gmax) return;
+        for (int t = 0; t < 8; ++t)
+            if (std::abs(vol[e * 8 + t] - vol[t]) > 1e-12 * vmax) return;
+    }
+    // KK(m*3+j, n*3+k) = sum_q dV dNx(m,j) dNx(n,k)
+    // KG(m*3+j, n*3+k) = sum_q dV [dNx(m,k) dNx(n,j) + delta_jk dNx(m).dNx(n) - 2/3 dNx(m,j) dNx(n,k)]
+    std::vector<double> kkg(2 * 576, 0.0);
+    for (int q = 0; q < 8; ++q) {
+        const double* gq = g.data() + q * 24;
+        const double dv = vol[q];
+        for (int m = 0; m < 8; ++m)
+            for (int nn = 0; nn < 8; ++nn) {
+                const double dot = gq[m * 3] * gq[nn * 3] + gq[m * 3 + 1] * gq[nn * 3 + 1] + gq[m * 3 + 2] * gq[nn * 3 + 2];
+                for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 3; ++k) {
+                        const int r = m * 3 + j, c = nn * 3 + k;
+                        kkg[r * 24 + c] += dv * gq[m * 3 + j] * gq[nn * 3 + k];
+                        kkg[576 + r * 24 + c] += dv * (gq[m * 3 + k] * gq[nn * 3 + j] + (j == k ? dot : 0.0) -
+                                                       2.0 / 3.0 * gq[m * 3 + j] * gq[nn * 3 + k]);
+                    }
+            }
+    }
+    std::vector<int32_t> sidx(static_cast<size_t>(n * n * n * 3));
+    for (int64_t k = 0; k < n; ++k)
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t i = 0; i < n; ++i)
+                for (int c = 0; c < 3; ++c) {
+                    const int64_t d = dof(node(i, j, k), c);
+                    sidx[((k * n + j) * n + i) * 3 + c] = d < P->nnu ? static_cast<int32_t>(d) : -1;
+                }
+    P->sidx.alloc(sidx.size());
+    GF_HIP(hipMemcpy(P->sidx.p, sidx.data(), sidx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    P->ke8.alloc(kkg.size());
+    GF_HIP(hipMemcpy(P->ke8.p, kkg.data(), kkg.size() * sizeof(double), hipMemcpyHostToDevice));
+    P->sn = static_cast<int>(n);
+    P->ntx = static_cast<int>(n / TX);
+    P->nty = static_cast<int>(n / TY);
+    P->ntiles = static_cast<int>((n / TX) * (n / TY) * (n / TZ));
+    P->per_xcd = 0;
     P->structured = true;
 }
 
@@ -691,6 +913,7 @@ int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const 
             P->scal.alloc(8);
             P->state.alloc(4);
             detect_structured(P, cdu_ptr_h, cdu_col_h, cdu_val_h);
+            detect_structured_hex8(P, cdu_ptr_h, cdu_col_h, cdu_val_h);
             P->partial.alloc(3 * NBR + std::max(NBR, P->ntiles));
             compute_invdiag(P, nullptr);
             GF_HIP(hipDeviceSynchronize());
@@ -784,13 +1007,30 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         GF_HIP(hipMemcpyAsync(P->state.p + 2, &cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
         check_launch();
         int32_t st[2] = {0, 0};
+        const bool fused = P->structured && P->et == GF_QUAD4_PLANAR;
+        ShapeQ4 sh;
+        for (int q = 0; q < 4; ++q) {
+            for (int k = 0; k < 8; ++k) sh.g[q][k] = P->sh_g[q * 8 + k];
+            sh.v[q] = P->sh_v[q];
+        }
+        double* pb[2] = {P->xfull.p, P->p2.p};
         for (int64_t it = 0; it < maxiter;) {
-            for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
+            for (int64_t k = 0; fused && k < check_every && it < maxiter; ++k, ++it) {
+                // two launches per iteration: q = K' p with p = z + beta p_prev built in the operator,
+                // then the x / r / z update
+                k_apply_periodic_q4<32, 8, true><<<P->ntiles, 256, 0, s>>>(
+                    P->sn, P->ntx, P->ntiles, P->per_xcd, sh, P->sidx.p, P->Ke.p, P->Ge.p, P->z.p, P->qv.p,
+                    P->partial.p + 3 * NBR, P->state.p, pb[(it + 1) & 1], pb[it & 1], P->partial.p, P->scal.p,
+                    P->state.p, it);
+                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->ntiles, P->scal.p, pb[it & 1], P->qv.p,
+                                                P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p, it);
+            }
+            for (int64_t k = 0; !fused && k < check_every && it < maxiter; ++k, ++it) {
                 apply(P, nullptr, nullptr, P->qv.p, true, s, P->xfull.p, P->partial.p + 3 * NBR, P->state.p);
-                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->structured ? P->ntiles : NBR, P->scal.p, P->xfull.p, P->qv.p,
-                                                P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p);
-                k_pcg_direction<<<NBR, BS, 0, s>>>(n, P->partial.p, P->scal.p, P->z.p, P->xfull.p, P->state.p);
-                k_pcg_count<<<1, 64, 0, s>>>(P->scal.p, P->state.p);
+                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->structured ? P->ntiles : NBR, P->scal.p,
+                                                P->xfull.p, P->qv.p, P->invdiag.p, P->x.p, P->r.p, P->z.p,
+                                                P->partial.p, P->state.p, it);
+                k_pcg_direction<<<NBR, BS, 0, s>>>(n, P->partial.p, P->scal.p, P->z.p, P->xfull.p, P->state.p, it);
             }
             check_launch();
             GF_HIP(hipMemcpyAsync(st, P->state.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -52,6 +52,55 @@ __device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo)
     lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
+// one element's C, dNx, dV in registers (16-byte loads), so that the next element's loads are in
+// flight while the current one is computed
+constexpr int NC2 = NC / 2, NG2 = NG / 2;
+constexpr int RC = (NC2 + 63) / 64, RG = (NG2 + 63) / 64;
+struct Stage {
+    double2 c[RC];
+    double2 g[RG];
+    double v;
+};
+
+__device__ inline void load_stage(Stage& st, const double* __restrict__ C, const double* __restrict__ dNx,
+                                  const double* __restrict__ vol, int64_t e, int lane)
+{
+    const double2* Ce = reinterpret_cast<const double2*>(C + e * NC);
+    const double2* ge = reinterpret_cast<const double2*>(dNx + e * NG);
+#pragma unroll
+    for (int k = 0; k < RC; ++k) {
+        const int t = lane + 64 * k;
+        st.c[k] = t < NC2 ? Ce[t] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < RG; ++k) {
+        const int t = lane + 64 * k;
+        st.g[k] = t < NG2 ? ge[t] : make_double2(0.0, 0.0);
+    }
+    st.v = lane < NQ ? vol[e * NQ + lane] : 0.0;
+}
+
+__device__ inline void store_stage(const Stage& st, WaveLDS& L, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < RC; ++k) {
+        const int t = lane + 64 * k;
+        if (t < NC2) {
+            L.C[2 * t] = static_cast<float>(st.c[k].x);
+            L.C[2 * t + 1] = static_cast<float>(st.c[k].y);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RG; ++k) {
+        const int t = lane + 64 * k;
+        if (t < NG2) {
+            L.g[2 * t] = static_cast<float>(st.g[k].x);
+            L.g[2 * t + 1] = static_cast<float>(st.g[k].y);
+        }
+    }
+    if (lane < NQ) L.vol[lane] = static_cast<float>(st.v);
+}
+
 __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, const double* __restrict__ C,
                                                                   const double* __restrict__ dNx,
                                                                   const double* __restrict__ vol,
@@ -61,13 +110,13 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, 
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     WaveLDS& L = lds[w];
     const int64_t stride = static_cast<int64_t>(gridDim.x) * WAVES;
-    for (int64_t e = static_cast<int64_t>(blockIdx.x) * WAVES + w; e < nelem; e += stride) {
-        // stage C, dNx, dV (coalesced, converted to FP32)
-        const double* Ce = C + e * NC;
-        for (int t = lane; t < NC; t += 64) L.C[t] = static_cast<float>(Ce[t]);
-        const double* ge = dNx + e * NG;
-        for (int t = lane; t < NG; t += 64) L.g[t] = static_cast<float>(ge[t]);
-        if (lane < NQ) L.vol[lane] = static_cast<float>(vol[e * NQ + lane]);
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * WAVES + w;
+    Stage st;
+    if (first < nelem) load_stage(st, C, dNx, vol, first, lane);
+    for (int64_t e = first; e < nelem; e += stride) {
+        // stage C, dNx, dV of this element in LDS (FP32), then start the next element's loads
+        store_stage(st, L, lane);
+        if (e + stride < nelem) load_stage(st, C, dNx, vol, e + stride, lane);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -131,8 +180,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double* Ke = elemmat + e * 576;
-        for (int t = lane; t < 576; t += 64) Ke[t] = static_cast<double>(L.K[t]);
+        double2* Ke = reinterpret_cast<double2*>(elemmat + e * 576);
+        for (int t = lane; t < 288; t += 64)
+            Ke[t] = make_double2(static_cast<double>(L.K[2 * t]), static_cast<double>(L.K[2 * t + 1]));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -216,6 +266,8 @@ int stiffness_bf16x3(gf_quad q, const double* C, double* elemmat, hipStream_t s)
 {
     return guard([&] {
         GF_REQUIRE(quad_etype(q) == GF_HEX8 && quad_nip(q) == NQ, "GF_BF16X3 needs Hex8 with 8 points");
+        GF_REQUIRE(reinterpret_cast<uintptr_t>(C) % 16 == 0 && reinterpret_cast<uintptr_t>(elemmat) % 16 == 0,
+                   "GF_BF16X3 needs 16-byte aligned qtensor4 and elemmat arrays");
         const int64_t ne = quad_nelem(q);
         if (ne == 0) return;
         k_stiffness_bf16x3<<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);

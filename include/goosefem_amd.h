@@ -301,8 +301,9 @@ int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const 
                   const int64_t* cdp_ptr_h, const int64_t* cdp_col_h, const double* cdp_val_h);
 int gf_pcg_destroy(gf_pcg p);
 int gf_pcg_sizes(gf_pcg p, int64_t* nnu, int64_t* nnp, int64_t* nnd);
-/* 1 if K' x_u is applied by the fused structured kernel (Quad4 Regular n x n cell, n a multiple of
- * 32, periodic tyings, identical element geometry), else 0 (general element + gather path) */
+/* 1 if K' x_u is applied by a fused structured kernel (Quad4 Regular n x n cell, n a multiple of 32,
+ * or Hex8 Regular n^3 cell, n a multiple of 8; periodic tyings; identical element geometry), else 0
+ * (general element + gather path) */
 int gf_pcg_is_structured(gf_pcg p, int* structured);
 /* replace the per-element moduli (device, FP64 [nelem] each) and recompute the Jacobi diagonal,
  * enqueued on stream: the tied operator of a new material field without rebuilding the tyings
@@ -374,6 +375,21 @@ int gf_grid_to_dofs(int64_t n, int nc, const float* grid, const int32_t* idx, co
  * each channel's mean */
 int gf_dofs_to_grid(int64_t n, int nc, const double* xu, const int32_t* idx, const double* affine, double scale,
                     double gamma, int center, float* grid, void* stream);
+/* 3-D variants on [N, Z, H, W] grids (taps t = 9 (dz + 1) + 3 (dy + 1) + dx + 1, weights [27][..][..],
+ * 2x2x2 pooling); arguments as the 2-D calls above */
+int gf_conv3x3x3(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int Z, int H, int W,
+                 const void* wpk, int coutp, const float* bias, int cout, int flags, void* y, int ysplit, void* y2,
+                 float* y32, const void* mask, void* stream);
+int gf_conv3x3x3_wgrad_work(int64_t P, int cg, int cin, int* nsplit, int64_t* work_floats);
+int gf_conv3x3x3_wgrad(const void* x1, int c1, int up1, const void* x2, int c2, int64_t P, int Z, int H, int W,
+                       const void* g, int cg, int cout, int cin, float* work, int64_t work_floats, float* dw, float* db,
+                       void* stream);
+int gf_avgpool2x2x2(const void* in, int64_t N, int Z, int H, int W, int C, void* out, void* stream);
+int gf_unpool2x2x2(const void* dout, int64_t N, int Z, int H, int W, int C, float w, int accum, const void* act,
+                   void* din, void* stream);
+int gf_sumpool2x2x2(const void* in, int64_t N, int Z, int H, int W, int C, const void* act, void* out, void* stream);
+int gf_conv_pack3(int cout, int cin, const float* w, int coutp, int cin_s, void* wf, int cinp, int cout_s, void* wb,
+                  void* stream);
 /* network input x [P][cs] bfloat16: channel 0 = scale * (log G - shift), others 0 */
 int gf_input_logG(int64_t P, const double* G, double shift, double scale, int cs, void* x, void* stream);
 

@@ -129,6 +129,34 @@ def test_structured_periodic_operator_and_solve(gf):
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-6
     x2, it2, _ = P.solve(b, tol=1e-10)
     assert it2 == it and np.array_equal(host(x2), x)
+    # the fused two-launch iteration honours the cap and warm starts
+    _, it3, res3 = P.solve(b, tol=1e-12, maxiter=3, raise_on_fail=False)
+    assert it3 == 3 and res3 > 1e-12
+    _, it4, res4 = P.solve(b, x0=xd, tol=1e-10)
+    assert it4 == 0 and res4 < 1e-10
     # the non-square cell of the other tests stays on the general path
     _, _, P2, _, _ = periodic_system(gf, QUAD4_PLANAR, 6)
     assert not P2.structured
+
+
+def test_structured_periodic_hex8_operator_and_solve(gf):
+    """Hex8 cube (n = 8): the structured operator (K_e = K KK + G KG from one shared geometry,
+    node-centric sums over the 8 elements of each node) against the dense condensation, and its solve
+    against the direct solution."""
+    mesh, ty, P, Kp, Kup = periodic_system(gf, HEX8, 8)
+    assert P.structured
+    rng = np.random.default_rng(6)
+    xu = rng.standard_normal(ty.nnu)
+    y = host(P.apply(xu))
+    assert np.linalg.norm(y - Kp @ xu) / np.linalg.norm(Kp @ xu) < 1e-13
+    xp = np.zeros(ty.nnp)
+    xp[ty.control[0, 1] - ty.nnu] = 5e-4
+    xp[ty.control[1, 0] - ty.nnu] = 5e-4
+    b = -Kup @ xp
+    x, it, res = P.solve(b, tol=1e-10)
+    x = host(x)
+    xd = np.linalg.solve(Kp, b)
+    assert res <= 1e-9 and np.linalg.norm(b - Kp @ x) / np.linalg.norm(b) < 1e-9
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-6
+    x2, it2, _ = P.solve(b, tol=1e-10)
+    assert it2 == it and np.array_equal(host(x2), x)
