@@ -202,46 +202,6 @@ def bench_hex8(args):
     print(json.dumps(res))
 
 
-def splitmix_uniform(seed, n):
-    """n uniform [0, 1) numbers from SplitMix64 (53-bit mantissa), identical on every platform."""
-    out = np.empty(n)
-    x = seed & 0xFFFFFFFFFFFFFFFF
-    M = 0xFFFFFFFFFFFFFFFF
-    for i in range(n):
-        x = (x + 0x9E3779B97F4A7C15) & M
-        z = x
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
-        z ^= z >> 31
-        out[i] = (z >> 11) * (1.0 / (1 << 53))
-    return out
-
-
-def stiff_spheres(n, radius=4.0, fraction=0.2, seed=105):
-    """Element mask of periodic spheres in an n^3 cell: centres uniform (seeded), added one at a
-    time until the fraction of element centres inside a sphere reaches `fraction`."""
-    inside = np.zeros((n, n, n), dtype=bool)  # [i, j, k], element (i, j, k) = (k*n + j)*n + i
-    u = splitmix_uniform(seed, 3 * 100000)
-    r = int(np.ceil(radius)) + 1
-    off = np.arange(-r, r + 1)
-    count, total, k = 0, n ** 3, 0
-    while count < fraction * total:
-        cen = n * u[3 * k: 3 * k + 3]
-        k += 1
-        idx = [np.floor(cen[a]).astype(int) + off for a in range(3)]
-        d2 = np.zeros((len(off),) * 3)
-        for a in range(3):
-            sh = [1, 1, 1]
-            sh[a] = -1
-            d2 = d2 + ((idx[a] + 0.5 - cen[a]) ** 2).reshape(sh)
-        sel = d2 <= radius * radius
-        I, J, K = np.meshgrid(idx[0] % n, idx[1] % n, idx[2] % n, indexing="ij")
-        new = sel & ~inside[I, J, K]
-        inside[I[new], J[new], K[new]] = True
-        count += int(new.sum())
-    return inside.transpose(2, 1, 0).ravel(), k
-
-
 def periodic_setup(n, hex8=False, seed=4, sigma=0.4, spheres=False):
     """Periodic Regular mesh with Tyings::Control + Tyings::Periodic as in
     docs/examples/statics/periodic_elastoplastic.py:28-47; per-element G log-normal, K = 10;
@@ -260,6 +220,8 @@ def periodic_setup(n, hex8=False, seed=4, sigma=0.4, spheres=False):
     G = np.exp(sigma * gf.randn(seed, mesh.nelem))
     nsph = 0
     if spheres:
+        from goosefem_amd.warmstart import stiff_spheres
+
         mask, nsph = stiff_spheres(n)
         G[mask] *= 10.0
     K = np.full(mesh.nelem, 10.0)
@@ -338,9 +300,10 @@ def pcg_cpu_baseline(S, b, iters):
 
 
 def bench_pcg_learned(args):
-    """Wall time of the FP64 Jacobi-PCG to a true relative residual of 1e-10 on the periodic Quad4
-    cell under a simple-shear increment, from the learned warm start (network inference included)
-    and from a zero guess, on modulus fields the network was not trained on."""
+    """Wall time of the FP64 Jacobi-PCG to a true relative residual of 1e-10 on the periodic cell
+    (Quad4 512^2 simple shear, or Hex8 96^3 with stiff spheres under pure shear) from the learned
+    warm start (network inference included) and from a zero guess, on modulus fields the network
+    was not trained on."""
     import torch
 
     import goosefem_amd as gf
@@ -352,23 +315,26 @@ def bench_pcg_learned(args):
         sys.exit(2)
     net = ws.WarmStartNet.load(args.ws_weights)
     n = int(net.meta["n"])
-    pb = ws.periodic_problem(n, sigma=float(net.meta["sigma"]))
-    W = ws.PeriodicShearWarmStart(pb.P, pb.ty, n, net, sigma=pb.sigma)
+    dim = int(net.meta.get("dim", 2))
+    shear = str(net.meta.get("shear", "simple"))
+    pb = ws.periodic_problem(n, sigma=float(net.meta["sigma"]), dim=dim, spheres=dim == 3)
+    W = ws.PeriodicWarmStart(pb.P, pb.ty, n, net, sigma=pb.sigma, dim=dim, shear=shear)
     P = pb.P
-    gamma = 1e-3
+    gamma = 5e-4 if dim == 3 else 1e-3
     Kd = gf.to_device(pb.K)
-    zero_grid = torch.zeros(n * n * 2, dtype=torch.float32, device="cuda")
+    zero_grid = torch.zeros(W.nv * dim, dtype=torch.float32, device="cuda")
 
     def affine_guess():
         x = torch.zeros(pb.ty.nnu, dtype=torch.float64, device="cuda")
-        gf._call("gf_grid_to_dofs", gf._i64(n * n * 2), gf._i32(2), gf._dp(zero_grid), gf._dp(W.idx),
+        gf._call("gf_grid_to_dofs", gf._i64(W.nv * dim), gf._i32(dim), gf._dp(zero_grid), gf._dp(W.idx),
                  gf._dp(W.affine), gf._f64(1.0), gf._f64(gamma), gf._dp(x), gf._stream())
         return x
 
     rows = []
     for k in range(args.ws_tests + 1):
         seed = 900000 + k  # training fields use seeds 1000 + i
-        G = gf.to_device(np.exp(pb.sigma * gf.randn(seed, n * n)))
+        g = ws.sphere_field(n, seed, pb.sigma) if dim == 3 else np.exp(pb.sigma * gf.randn(seed, n * n))
+        G = gf.to_device(g)
         P.set_moduli(Kd, G)
         b = W.rhs(gamma)
         torch.cuda.synchronize()
@@ -392,6 +358,11 @@ def bench_pcg_learned(args):
                          learned_iterations=it1, affine_only_iterations=it2, plain_relres=res0, learned_relres=res1))
     mean = lambda key: float(np.mean([r[key] for r in rows]))  # noqa: E731
     m = net.meta
+    if dim == 3:
+        workload = (f"periodic Hex8 Regular {n}^3, G = exp({pb.sigma} z) x10 in stiff spheres (r 4, 20%), K 10, "
+                    f"pure shear increment {gamma}")
+    else:
+        workload = f"periodic Quad4 Regular {n}^2 plane strain, G = exp({pb.sigma} z), K 10, simple shear increment {gamma}"
     res = {
         "metric": "PCG wall time to true relative residual 1e-10, learned warm start (inference included)",
         "value": mean("learned_s"), "unit": "s", "higher_is_better": False, "n_gpus": 1,
@@ -401,13 +372,13 @@ def bench_pcg_learned(args):
         "inference_ms": mean("inference_s") * 1e3,
         "max_true_relres": max(max(r["plain_relres"], r["learned_relres"]) for r in rows),
         "dtype": "bf16 mfma network (fp32 accumulate), fp64 pcg",
-        "data": "synthetic log-normal moduli (held-out seeds)",
-        "config": {"workload": f"periodic Quad4 Regular {n}^2 plane strain, G = exp({pb.sigma} z), K 10, "
-                               f"simple shear increment {gamma}", "nnu": pb.ty.nnu, "test_fields": len(rows),
-                   "channels": net.ch, "parameters": net.nparams()},
+        "data": "synthetic modulus fields (held-out seeds)",
+        "config": {"workload": workload, "nnu": pb.ty.nnu, "test_fields": len(rows), "channels": net.ch,
+                   "parameters": net.nparams(), "structured_operator": P.structured},
         "training": {"samples": int(m["samples"]), "epochs": int(m["epochs"]), "batch": int(m["batch"]),
                      "data_generation_s": float(m["data_seconds"]), "training_s": float(m["train_seconds"]),
-                     "final_mse": float(m["final_mse"])},
+                     "final_mse": float(m["final_mse"]),
+                     "holdout_rel_error": float(m.get("holdout_rel_error", float("nan")))},
         "per_field": rows,
     }
     print(json.dumps(res))

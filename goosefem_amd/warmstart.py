@@ -41,29 +41,31 @@ def coutp_of(c):
 
 
 class ConvLayer:
-    """3x3 circular convolution: FP32 master weights [9][cout][cin] + bias, Adam state, packed BF16
-    forward weights [9][coutp][cin_s] and data-gradient weights [9][cinp][cout_s]."""
+    """3x3 (9 taps) or 3x3x3 (27 taps) circular convolution: FP32 master weights [ntap][cout][cin] +
+    bias, Adam state, packed BF16 forward weights [ntap][coutp][cin_s] and data-gradient weights
+    [ntap][cinp][cout_s]."""
 
-    def __init__(self, cin, cout, cin_s, cout_s, seed):
+    def __init__(self, cin, cout, cin_s, cout_s, seed, ntap=9):
         torch = _torch()
-        self.cin, self.cout, self.cin_s, self.cout_s = cin, cout, cin_s, cout_s
+        self.cin, self.cout, self.cin_s, self.cout_s, self.ntap = cin, cout, cin_s, cout_s, ntap
         self.coutp = coutp_of(cout_s)
         self.cinp = coutp_of(cin_s)
-        std = np.sqrt(2.0 / (9 * cin))  # He initialisation
-        w = randn(seed, 9 * cout * cin, 0.0, std).astype(np.float32)
+        std = np.sqrt(2.0 / (ntap * cin))  # He initialisation
+        w = randn(seed, ntap * cout * cin, 0.0, std).astype(np.float32)
         self.w = to_device(w, torch.float32)
         self.b = torch.zeros(cout, dtype=torch.float32, device=device())
         self.dw = torch.zeros_like(self.w)
         self.db = torch.zeros_like(self.b)
         self.mw, self.vw = torch.zeros_like(self.w), torch.zeros_like(self.w)
         self.mb, self.vb = torch.zeros_like(self.b), torch.zeros_like(self.b)
-        self.wf = torch.zeros(9 * self.coutp * cin_s, dtype=torch.bfloat16, device=device())
-        self.wb = torch.zeros(9 * self.cinp * cout_s, dtype=torch.bfloat16, device=device())
+        self.wf = torch.zeros(ntap * self.coutp * cin_s, dtype=torch.bfloat16, device=device())
+        self.wb = torch.zeros(ntap * self.cinp * cout_s, dtype=torch.bfloat16, device=device())
         self.pack()
 
     def pack(self):
-        _call("gf_conv_pack", _i32(self.cout), _i32(self.cin), _dp(self.w), _i32(self.coutp), _i32(self.cin_s),
-              _dp(self.wf), _i32(self.cinp), _i32(self.cout_s), _dp(self.wb), _stream())
+        _call("gf_conv_pack3" if self.ntap == 27 else "gf_conv_pack", _i32(self.cout), _i32(self.cin), _dp(self.w),
+              _i32(self.coutp), _i32(self.cin_s), _dp(self.wf), _i32(self.cinp), _i32(self.cout_s), _dp(self.wb),
+              _stream())
 
     def adam(self, lr, step, b1=0.9, b2=0.999, eps=1e-8):
         for w, g, m, v in ((self.w, self.dw, self.mw, self.vw), (self.b, self.db, self.mb, self.vb)):
@@ -73,41 +75,49 @@ class ConvLayer:
 
 
 def conv(x1, c1, P, H, W, layer, y, flags=0, up1=0, x2=None, c2=0, ysplit=None, y2=None, y32=None, mask=None,
-         bias=True, backward=False):
+         bias=True, backward=False, Z=1):
     """y = conv(x) with `layer` (forward weights), or its data gradient (backward=True: tap-flipped,
-    transposed weights, output channels = the layer's input channels)."""
+    transposed weights, output channels = the layer's input channels).  3-D layers take the depth Z."""
     if backward:
         wpk, coutp, cout = layer.wb, layer.cinp, layer.cin_s
     else:
         wpk, coutp, cout = layer.wf, layer.coutp, (layer.cout if y is None else layer.cout_s)
     b = _dp(layer.b) if (bias and not backward) else None
-    _call("gf_conv3x3", _dp(x1), _i32(c1), _i32(up1), _dp(x2), _i32(c2), _i64(P), _i32(H), _i32(W), _dp(wpk),
-          _i32(coutp), b, _i32(cout), _i32(flags), _dp(y), _i32(ysplit if ysplit is not None else cout), _dp(y2),
-          _dp(y32), _dp(mask), _stream())
+    ys = _i32(ysplit if ysplit is not None else cout)
+    if layer.ntap == 27:
+        _call("gf_conv3x3x3", _dp(x1), _i32(c1), _i32(up1), _dp(x2), _i32(c2), _i64(P), _i32(Z), _i32(H), _i32(W),
+              _dp(wpk), _i32(coutp), b, _i32(cout), _i32(flags), _dp(y), ys, _dp(y2), _dp(y32), _dp(mask), _stream())
+    else:
+        _call("gf_conv3x3", _dp(x1), _i32(c1), _i32(up1), _dp(x2), _i32(c2), _i64(P), _i32(H), _i32(W), _dp(wpk),
+              _i32(coutp), b, _i32(cout), _i32(flags), _dp(y), ys, _dp(y2), _dp(y32), _dp(mask), _stream())
 
 
 class WarmStartNet:
-    """Periodic U-Net: per level two conv+ReLU (encoder), 2x2 average pooling between levels,
-    nearest upsampling + skip concatenation + two conv+ReLU (decoder), final conv to 2 channels.
+    """Periodic U-Net on a 2-D (dim = 2) or 3-D (dim = 3) element grid: per level two conv+ReLU
+    (encoder), 2x2(x2) average pooling between levels, nearest upsampling + skip concatenation + two
+    conv+ReLU (decoder), final conv to `nout` channels.
 
-    channels: per-level widths (multiples of 16, at most 128).  Grid H x W must be divisible by
-    2^(levels-1)."""
+    channels: per-level widths (multiples of 16, at most 128).  The grid must be divisible by
+    2^(levels-1) in every dimension.  Grids are (H, W) with Z = 1, or (Z, H, W) given as H, W, Z=."""
 
-    def __init__(self, channels=(16, 32, 64, 128), seed=7):
+    def __init__(self, channels=(16, 32, 64, 128), seed=7, dim=2, nout=None):
         self.ch = list(channels)
+        self.dim = dim
+        self.nout = nout if nout is not None else dim
+        ntap = 27 if dim == 3 else 9
         L = len(self.ch)
-        assert all(c % 16 == 0 and c <= 128 for c in self.ch)
+        assert all(c % 16 == 0 and c <= 128 for c in self.ch) and dim in (2, 3) and 0 < self.nout <= OUT_CH
         s = seed * 1000
         self.enc, self.dec = [], [None] * max(L - 1, 0)
         for l in range(L):
             cin, cin_s = (1, IN_CH) if l == 0 else (self.ch[l - 1], self.ch[l - 1])
-            self.enc.append((ConvLayer(cin, self.ch[l], cin_s, self.ch[l], s + 2 * l),
-                             ConvLayer(self.ch[l], self.ch[l], self.ch[l], self.ch[l], s + 2 * l + 1)))
+            self.enc.append((ConvLayer(cin, self.ch[l], cin_s, self.ch[l], s + 2 * l, ntap),
+                             ConvLayer(self.ch[l], self.ch[l], self.ch[l], self.ch[l], s + 2 * l + 1, ntap)))
         for l in range(L - 2, -1, -1):
             cin = self.ch[l + 1] + self.ch[l]
-            self.dec[l] = (ConvLayer(cin, self.ch[l], cin, self.ch[l], s + 100 + 2 * l),
-                           ConvLayer(self.ch[l], self.ch[l], self.ch[l], self.ch[l], s + 101 + 2 * l))
-        self.out = ConvLayer(self.ch[0], 2, self.ch[0], OUT_CH, s + 500)
+            self.dec[l] = (ConvLayer(cin, self.ch[l], cin, self.ch[l], s + 100 + 2 * l, ntap),
+                           ConvLayer(self.ch[l], self.ch[l], self.ch[l], self.ch[l], s + 101 + 2 * l, ntap))
+        self.out = ConvLayer(self.ch[0], self.nout, self.ch[0], OUT_CH, s + 500, ntap)
         self.step_count = 0
         self._buf = {}
         self.meta = {}
@@ -121,19 +131,25 @@ class WarmStartNet:
         return sum(c.w.numel() + c.b.numel() for c in self.layers())
 
     # ---- buffers ----
-    def buffers(self, B, H, W):
-        key = (B, H, W)
+    def buffers(self, B, H, W, Z=1):
+        key = (B, Z, H, W)
         if key in self._buf:
             return self._buf[key]
         torch = _torch()
         L = len(self.ch)
-        assert H % (1 << (L - 1)) == 0 and W % (1 << (L - 1)) == 0, "grid not divisible by the pooling depth"
+        f = 1 << (L - 1)
+        assert H % f == 0 and W % f == 0 and (self.dim == 2 or Z % f == 0), "grid not divisible by the pooling depth"
+        assert self.dim == 3 or Z == 1
 
         def t(P, C):
             return torch.zeros(P * C, dtype=torch.bfloat16, device=device())
 
-        lv = [(B * (H >> l) * (W >> l), H >> l, W >> l) for l in range(L)]
-        bf = {"lv": lv, "X0": t(lv[0][0], IN_CH), "pred": torch.zeros(lv[0][0] * 2, dtype=torch.float32, device=device()),
+        lv = []
+        for l in range(L):
+            zl = (Z >> l) if self.dim == 3 else 1
+            lv.append((B * zl * (H >> l) * (W >> l), H >> l, W >> l, zl))
+        bf = {"lv": lv, "X0": t(lv[0][0], IN_CH),
+              "pred": torch.zeros(lv[0][0] * self.nout, dtype=torch.float32, device=device()),
               "gY": t(lv[0][0], OUT_CH), "I": [None] * L, "A": [], "E": [], "Bd": [None] * L, "D": [None] * L,
               "gI": [None] * L, "gA": [], "gE": [], "gB": [None] * L, "gD": [None] * L, "gUp": [None] * L}
         for l in range(L):
@@ -153,11 +169,11 @@ class WarmStartNet:
                 bf["gUp"][l] = t(P, self.ch[l + 1])
         # weight-gradient workspace: the largest layer
         need = 0
+        wf = "gf_conv3x3x3_wgrad_work" if self.dim == 3 else "gf_conv3x3_wgrad_work"
         for c in self.layers():
             lvl = self._level_of(c)
             ns, nf = ctypes.c_int(0), ctypes.c_int64(0)
-            _call("gf_conv3x3_wgrad_work", _i64(lv[lvl][0]), _i32(c.cout_s if c is not self.out else OUT_CH),
-                  _i32(c.cin_s), ctypes.byref(ns), ctypes.byref(nf))
+            _call(wf, _i64(lv[lvl][0]), _i32(c.cout_s), _i32(c.cin_s), ctypes.byref(ns), ctypes.byref(nf))
             need = max(need, nf.value)
         bf["work"] = torch.zeros(need, dtype=torch.float32, device=device())
         bf["loss_part"] = torch.zeros((lv[0][0] * OUT_CH + 255) // 256, dtype=torch.float32, device=device())
@@ -175,81 +191,107 @@ class WarmStartNet:
         return 0
 
     def _wgrad(self, layer, bf, l, x1, c1, g, up1=0, x2=None, c2=0):
-        P, H, W = bf["lv"][l]
+        P, H, W, Z = bf["lv"][l]
         work = bf["work"]
         cg = layer.cout_s
-        _call("gf_conv3x3_wgrad", _dp(x1), _i32(c1), _i32(up1), _dp(x2), _i32(c2), _i64(P), _i32(H), _i32(W), _dp(g),
-              _i32(cg), _i32(layer.cout), _i32(layer.cin), _dp(work), _i64(work.numel()), _dp(layer.dw), _dp(layer.db),
-              _stream())
+        tail = (_dp(g), _i32(cg), _i32(layer.cout), _i32(layer.cin), _dp(work), _i64(work.numel()), _dp(layer.dw),
+                _dp(layer.db), _stream())
+        if self.dim == 3:
+            _call("gf_conv3x3x3_wgrad", _dp(x1), _i32(c1), _i32(up1), _dp(x2), _i32(c2), _i64(P), _i32(Z), _i32(H),
+                  _i32(W), *tail)
+        else:
+            _call("gf_conv3x3_wgrad", _dp(x1), _i32(c1), _i32(up1), _dp(x2), _i32(c2), _i64(P), _i32(H), _i32(W),
+                  *tail)
+
+    def _pool(self, kind, src, B, fine, C, dst, act=None, accum=0):
+        """kind 'avg' (fine -> coarse mean), 'unpool' (coarse -> fine, weight 1/2^dim, accumulate,
+        mask), 'sum' (fine -> coarse block sums, mask); `fine` = (H, W, Z) of the fine grid"""
+        H, W, Z = fine
+        if self.dim == 3:
+            if kind == "avg":
+                _call("gf_avgpool2x2x2", _dp(src), _i64(B), _i32(Z), _i32(H), _i32(W), _i32(C), _dp(dst), _stream())
+            elif kind == "unpool":
+                _call("gf_unpool2x2x2", _dp(src), _i64(B), _i32(Z), _i32(H), _i32(W), _i32(C), _f32(0.125),
+                      _i32(accum), _dp(act), _dp(dst), _stream())
+            else:
+                _call("gf_sumpool2x2x2", _dp(src), _i64(B), _i32(Z), _i32(H), _i32(W), _i32(C), _dp(act), _dp(dst),
+                      _stream())
+        else:
+            if kind == "avg":
+                _call("gf_avgpool2", _dp(src), _i64(B), _i32(H), _i32(W), _i32(C), _dp(dst), _stream())
+            elif kind == "unpool":
+                _call("gf_unpool2", _dp(src), _i64(B), _i32(H), _i32(W), _i32(C), _f32(0.25), _i32(accum), _dp(act),
+                      _dp(dst), _stream())
+            else:
+                _call("gf_sumpool2", _dp(src), _i64(B), _i32(H), _i32(W), _i32(C), _dp(act), _dp(dst), _stream())
 
     # ---- forward / backward ----
-    def forward(self, B, H, W):
-        """Network on the input already in buffers(B, H, W)["X0"]; returns the FP32 prediction [B*H*W*2]."""
-        bf = self.buffers(B, H, W)
+    def forward(self, B, H, W, Z=1):
+        """Network on the input already in buffers(B, H, W, Z)["X0"]; returns the FP32 prediction
+        [B*Z*H*W*nout]."""
+        bf = self.buffers(B, H, W, Z)
         L = len(self.ch)
         x, cx = bf["X0"], IN_CH
         for l in range(L):
-            P, h, w = bf["lv"][l]
+            P, h, w, z = bf["lv"][l]
             if l > 0:
-                _call("gf_avgpool2", _dp(bf["E"][l - 1]), _i64(B), _i32(2 * h), _i32(2 * w), _i32(self.ch[l - 1]),
-                      _dp(bf["I"][l]), _stream())
+                fine = bf["lv"][l - 1]
+                self._pool("avg", bf["E"][l - 1], B, (fine[1], fine[2], fine[3]), self.ch[l - 1], bf["I"][l])
                 x, cx = bf["I"][l], self.ch[l - 1]
             c1, c2 = self.enc[l]
-            conv(x, cx, P, h, w, c1, bf["A"][l], flags=CF_RELU)
-            conv(bf["A"][l], self.ch[l], P, h, w, c2, bf["E"][l], flags=CF_RELU)
+            conv(x, cx, P, h, w, c1, bf["A"][l], flags=CF_RELU, Z=z)
+            conv(bf["A"][l], self.ch[l], P, h, w, c2, bf["E"][l], flags=CF_RELU, Z=z)
         d, cd = bf["E"][L - 1], self.ch[L - 1]
         for l in range(L - 2, -1, -1):
-            P, h, w = bf["lv"][l]
+            P, h, w, z = bf["lv"][l]
             c1, c2 = self.dec[l]
-            conv(d, cd, P, h, w, c1, bf["Bd"][l], flags=CF_RELU, up1=1, x2=bf["E"][l], c2=self.ch[l])
-            conv(bf["Bd"][l], self.ch[l], P, h, w, c2, bf["D"][l], flags=CF_RELU)
+            conv(d, cd, P, h, w, c1, bf["Bd"][l], flags=CF_RELU, up1=1, x2=bf["E"][l], c2=self.ch[l], Z=z)
+            conv(bf["Bd"][l], self.ch[l], P, h, w, c2, bf["D"][l], flags=CF_RELU, Z=z)
             d, cd = bf["D"][l], self.ch[l]
-        P, h, w = bf["lv"][0]
-        conv(d, cd, P, h, w, self.out, None, y32=bf["pred"])
+        P, h, w, z = bf["lv"][0]
+        conv(d, cd, P, h, w, self.out, None, y32=bf["pred"], Z=z)
         return bf["pred"]
 
-    def backward(self, B, H, W):
-        """Gradients of all layers from bf["gY"] (dL/dpred in channels 0, 1)."""
-        bf = self.buffers(B, H, W)
+    def backward(self, B, H, W, Z=1):
+        """Gradients of all layers from bf["gY"] (dL/dpred in channels 0..nout-1)."""
+        bf = self.buffers(B, H, W, Z)
         L = len(self.ch)
-        P0, h0, w0 = bf["lv"][0]
+        P0, h0, w0, z0 = bf["lv"][0]
         last = bf["D"][0] if L > 1 else bf["E"][0]
         self._wgrad(self.out, bf, 0, last, self.ch[0], bf["gY"])
         g_last = bf["gD"][0] if L > 1 else bf["gE"][0]
-        conv(bf["gY"], OUT_CH, P0, h0, w0, self.out, g_last, mask=last, backward=True)
+        conv(bf["gY"], OUT_CH, P0, h0, w0, self.out, g_last, mask=last, backward=True, Z=z0)
         for l in range(L - 1):
-            P, h, w = bf["lv"][l]
+            P, h, w, z = bf["lv"][l]
             c1, c2 = self.dec[l]
             self._wgrad(c2, bf, l, bf["Bd"][l], self.ch[l], bf["gD"][l])
-            conv(bf["gD"][l], self.ch[l], P, h, w, c2, bf["gB"][l], mask=bf["Bd"][l], backward=True)
+            conv(bf["gD"][l], self.ch[l], P, h, w, c2, bf["gB"][l], mask=bf["Bd"][l], backward=True, Z=z)
             dn = bf["D"][l + 1] if l + 1 < L - 1 else bf["E"][L - 1]
             self._wgrad(c1, bf, l, dn, self.ch[l + 1], bf["gB"][l], up1=1, x2=bf["E"][l], c2=self.ch[l])
             conv(bf["gB"][l], self.ch[l], P, h, w, c1, bf["gUp"][l], ysplit=self.ch[l + 1], y2=bf["gE"][l],
-                 backward=True)
+                 backward=True, Z=z)
             gdn = bf["gD"][l + 1] if l + 1 < L - 1 else bf["gE"][L - 1]
-            _call("gf_sumpool2", _dp(bf["gUp"][l]), _i64(B), _i32(h), _i32(w), _i32(self.ch[l + 1]), _dp(dn),
-                  _dp(gdn), _stream())
+            self._pool("sum", bf["gUp"][l], B, (h, w, z), self.ch[l + 1], gdn, act=dn)
         for l in range(L - 1, -1, -1):
-            P, h, w = bf["lv"][l]
+            P, h, w, z = bf["lv"][l]
             if l < L - 1:
                 # skip gradient from the decoder + the pooled path of level l + 1, masked by E_l > 0
-                _call("gf_unpool2", _dp(bf["gI"][l + 1]), _i64(B), _i32(h), _i32(w), _i32(self.ch[l]), _f32(0.25),
-                      _i32(1), _dp(bf["E"][l]), _dp(bf["gE"][l]), _stream())
+                self._pool("unpool", bf["gI"][l + 1], B, (h, w, z), self.ch[l], bf["gE"][l], act=bf["E"][l], accum=1)
             c1, c2 = self.enc[l]
             self._wgrad(c2, bf, l, bf["A"][l], self.ch[l], bf["gE"][l])
-            conv(bf["gE"][l], self.ch[l], P, h, w, c2, bf["gA"][l], mask=bf["A"][l], backward=True)
+            conv(bf["gE"][l], self.ch[l], P, h, w, c2, bf["gA"][l], mask=bf["A"][l], backward=True, Z=z)
             x, cx = (bf["X0"], IN_CH) if l == 0 else (bf["I"][l], self.ch[l - 1])
             self._wgrad(c1, bf, l, x, cx, bf["gA"][l])
             if l > 0:
-                conv(bf["gA"][l], self.ch[l], P, h, w, c1, bf["gI"][l], backward=True)
+                conv(bf["gA"][l], self.ch[l], P, h, w, c1, bf["gI"][l], backward=True, Z=z)
 
-    def loss_grad(self, B, H, W, target, scale=1.0):
-        """MSE of the prediction against target [B*H*W*2] FP32; writes gY; returns the device sum of
-        squared errors (a 1-element tensor)."""
-        bf = self.buffers(B, H, W)
+    def loss_grad(self, B, H, W, target, scale=1.0, Z=1):
+        """MSE of the prediction against target [B*Z*H*W*nout] FP32; writes gY; returns the device sum
+        of squared errors (a 1-element tensor)."""
+        bf = self.buffers(B, H, W, Z)
         P0 = bf["lv"][0][0]
-        _call("gf_mse", _i64(P0), _i32(2), _dp(bf["pred"]), _dp(target), _f32(scale), _i32(OUT_CH), _dp(bf["gY"]),
-              _dp(bf["loss_part"]), _dp(bf["loss"]), _stream())
+        _call("gf_mse", _i64(P0), _i32(self.nout), _dp(bf["pred"]), _dp(target), _f32(scale), _i32(OUT_CH),
+              _dp(bf["gY"]), _dp(bf["loss_part"]), _dp(bf["loss"]), _stream())
         return bf["loss"]
 
     def adam(self, lr):
@@ -257,18 +299,18 @@ class WarmStartNet:
         for c in self.layers():
             c.adam(lr, self.step_count)
 
-    def set_input(self, B, H, W, G_list, shift, scale):
-        """Normalised log G of B element fields (device FP64 [H*W] each) into the input buffer."""
-        bf = self.buffers(B, H, W)
-        HW = H * W
+    def set_input(self, B, H, W, G_list, shift, scale, Z=1):
+        """Normalised log G of B element fields (device FP64 [Z*H*W] each) into the input buffer."""
+        bf = self.buffers(B, H, W, Z)
+        V = Z * H * W
         X0 = bf["X0"]
         for b, G in enumerate(G_list):
-            xb = X0[b * HW * IN_CH:(b + 1) * HW * IN_CH]
-            _call("gf_input_logG", _i64(HW), _dp(G), _f64(shift), _f64(scale), _i32(IN_CH), _dp(xb), _stream())
+            xb = X0[b * V * IN_CH:(b + 1) * V * IN_CH]
+            _call("gf_input_logG", _i64(V), _dp(G), _f64(shift), _f64(scale), _i32(IN_CH), _dp(xb), _stream())
 
     # ---- persistence ----
     def state(self):
-        d = {"channels": np.array(self.ch, dtype=np.int64)}
+        d = {"channels": np.array(self.ch, dtype=np.int64), "dim": np.array(self.dim), "nout": np.array(self.nout)}
         for i, c in enumerate(self.layers()):
             d[f"w{i}"] = c.w.cpu().numpy()
             d[f"b{i}"] = c.b.cpu().numpy()
@@ -283,7 +325,9 @@ class WarmStartNet:
     def load(cls, path):
         torch = _torch()
         with np.load(path, allow_pickle=False) as f:
-            net = cls(tuple(int(c) for c in f["channels"]))
+            dim = int(f["dim"]) if "dim" in f.files else 2
+            nout = int(f["nout"]) if "nout" in f.files else 2
+            net = cls(tuple(int(c) for c in f["channels"]), dim=dim, nout=nout)
             for i, c in enumerate(net.layers()):
                 c.w.copy_(torch.from_numpy(f[f"w{i}"]))
                 c.b.copy_(torch.from_numpy(f[f"b{i}"]))
@@ -292,65 +336,82 @@ class WarmStartNet:
         return net
 
 
-class PeriodicShearWarmStart:
-    """Couples a WarmStartNet to the tied PCG of a periodic Quad4 Regular(n, n) cell under a simple
-    shear increment of the control DOF ctrl(0, 1) (docs/examples/statics/periodic_elastoplastic.py:141).
+class PeriodicWarmStart:
+    """Couples a WarmStartNet to the tied PCG of a periodic Regular cell, Quad4 n x n (dim 2) or
+    Hex8 n^3 (dim 3), under a shear increment gamma of the control DOFs: simple shear ctrl(0, 1)
+    (docs/examples/statics/periodic_elastoplastic.py:141) or pure shear ctrl(0, 1) = ctrl(1, 0).
 
-    The independent nodes (i, j), 0 <= i, j < n, are the lower-left nodes of the elements (i, j), so
-    the nodal grid and the element grid coincide.  Unknown x_u of node (i, j), component c:
-    gamma * (u_fluct(i, j, c) + (c == 0) * y_j); the origin node is prescribed."""
+    The independent nodes (i, j[, k]) < n are the lower corner nodes of the elements with the same
+    indices, so the nodal grid and the element grid coincide.  Unknown x_u of node (i, j[, k]),
+    component c: gamma * (u_fluct(c) + affine(c)) with affine u_x = y (and u_y = x for pure shear);
+    the origin node is prescribed."""
 
-    def __init__(self, pcg, tyings, n, net, sigma=0.4, target_scale=None):
+    def __init__(self, pcg, tyings, n, net, sigma=0.4, target_scale=None, dim=2, shear="simple"):
         torch = _torch()
-        self.P, self.ty, self.n, self.net, self.sigma = pcg, tyings, n, net, sigma
+        self.P, self.ty, self.n, self.net, self.sigma, self.dim, self.shear = pcg, tyings, n, net, sigma, dim, shear
         dofs = np.asarray(tyings.dofs)
-        idx = np.full((n, n, 2), -1, dtype=np.int32)
-        aff = np.zeros((n, n, 2))
-        for j in range(n):
-            nodes = j * (n + 1) + np.arange(n)
-            d = dofs[nodes]
-            idx[j] = np.where(d < tyings.nnu, d, -1)
-            aff[j, :, 0] = float(j)  # affine u_x = gamma * y for ctrl(0, 1) = gamma
+        N1 = n + 1
+        r = np.arange(n)
+        if dim == 2:
+            J, I = np.meshgrid(r, r, indexing="ij")
+            nodes = (J * N1 + I).ravel()
+            coords = [I.ravel(), J.ravel()]
+        else:
+            K, J, I = np.meshgrid(r, r, r, indexing="ij")
+            nodes = ((K * N1 + J) * N1 + I).ravel()
+            coords = [I.ravel(), J.ravel(), K.ravel()]
+        d = dofs[nodes][:, :dim]
+        idx = np.where(d < tyings.nnu, d, -1).astype(np.int32)
+        aff = np.zeros((nodes.size, dim))
+        aff[:, 0] = coords[1]  # u_x = gamma * y  (ctrl(0, 1))
+        if shear == "pure":
+            aff[:, 1] = coords[0]  # u_y = gamma * x  (ctrl(1, 0))
         self.idx = to_device(idx.ravel(), torch.int32)
         self.affine = to_device(aff.ravel())
         self.xp_unit = np.zeros(tyings.nnp)
         self.xp_unit[tyings.control[0, 1] - tyings.nnu] = 1.0
+        if shear == "pure":
+            self.xp_unit[tyings.control[1, 0] - tyings.nnu] = 1.0
         self.target_scale = float(target_scale if target_scale is not None else net.meta.get("target_scale", 1.0))
+        self.nv = nodes.size  # grid cells = independent nodes
+
+    def _dims(self):
+        n = self.n
+        return (n, n, n) if self.dim == 3 else (n, n, 1)
 
     def rhs(self, gamma):
         return -self.P.apply(None, gamma * self.xp_unit)
 
     def predict(self, G, gamma, out=None):
-        """Initial guess x_u (device FP64 [nnu]) for the modulus field G (device FP64 [n*n])."""
+        """Initial guess x_u (device FP64 [nnu]) for the modulus field G (device FP64, one per element)."""
         torch = _torch()
-        n = self.n
-        self.net.set_input(1, n, n, [G], 0.0, 1.0 / self.sigma)
-        pred = self.net.forward(1, n, n)
+        H, W, Z = self._dims()
+        self.net.set_input(1, H, W, [G], 0.0, 1.0 / self.sigma, Z=Z)
+        pred = self.net.forward(1, H, W, Z=Z)
         x = out if out is not None else torch.zeros(self.ty.nnu, dtype=torch.float64, device=device())
-        _call("gf_grid_to_dofs", _i64(n * n * 2), _i32(2), _dp(pred), _dp(self.idx), _dp(self.affine),
+        _call("gf_grid_to_dofs", _i64(self.nv * self.dim), _i32(self.dim), _dp(pred), _dp(self.idx), _dp(self.affine),
               _f64(self.target_scale), _f64(gamma), _dp(x), _stream())
         return x
 
     def target(self, x_u, gamma, out):
-        """Centred, scaled fluctuation grid [n*n*2] FP32 of a solution x_u."""
-        n = self.n
-        _call("gf_dofs_to_grid", _i64(n * n * 2), _i32(2), _dp(x_u), _dp(self.idx), _dp(self.affine),
+        """Centred, scaled fluctuation grid [cells * dim] FP32 of a solution x_u."""
+        _call("gf_dofs_to_grid", _i64(self.nv * self.dim), _i32(self.dim), _dp(x_u), _dp(self.idx), _dp(self.affine),
               _f64(self.target_scale), _f64(gamma), _i32(1), _dp(out), _stream())
         return out
 
-    def make_dataset(self, K, seeds, tol=1e-6, progress=None):
-        """PCG solutions for the modulus fields exp(sigma z_s); returns (G [ns, n*n] FP64,
-        targets [ns, n*n*2] FP32, total iterations, seconds)."""
+    def make_dataset(self, K, seeds, tol=1e-6, progress=None, field=None):
+        """PCG solutions for the modulus fields field(seed) (default exp(sigma z_seed)); returns
+        (G [ns, cells] FP64, targets [ns, cells * dim] FP32, total iterations, seconds)."""
         torch = _torch()
-        n = self.n
         ns = len(seeds)
-        Gs = torch.zeros((ns, n * n), dtype=torch.float64, device=device())
-        T = torch.zeros((ns, n * n * 2), dtype=torch.float32, device=device())
+        Gs = torch.zeros((ns, self.nv), dtype=torch.float64, device=device())
+        T = torch.zeros((ns, self.nv * self.dim), dtype=torch.float32, device=device())
         Kd = to_device(K)
         its = 0
         t0 = time.perf_counter()
         for k, s in enumerate(seeds):
-            Gs[k] = to_device(np.exp(self.sigma * randn(s, n * n)))
+            g = field(s) if field is not None else np.exp(self.sigma * randn(s, self.nv))
+            Gs[k] = to_device(g)
             self.P.set_moduli(Kd, Gs[k])
             b = self.rhs(1.0)
             x, it, _ = self.P.solve(b, tol=tol)
@@ -363,27 +424,27 @@ class PeriodicShearWarmStart:
 
     def holdout_error(self, Gs, T):
         """Mean relative L2 error of the prediction (up to a translation) against centred targets."""
-        n = self.n
+        H, W, Z = self._dims()
         errs = []
         for k in range(Gs.shape[0]):
-            self.net.set_input(1, n, n, [Gs[k]], 0.0, 1.0 / self.sigma)
-            p = self.net.forward(1, n, n).reshape(n * n, 2)
+            self.net.set_input(1, H, W, [Gs[k]], 0.0, 1.0 / self.sigma, Z=Z)
+            p = self.net.forward(1, H, W, Z=Z).reshape(self.nv, self.dim)
             p = p - p.mean(dim=0, keepdim=True)
-            t = T[k].reshape(n * n, 2)
+            t = T[k].reshape(self.nv, self.dim)
             errs.append(float(((p - t).norm() / t.norm()).item()))
         return float(np.mean(errs))
 
     def train(self, Gs, T, epochs, batch=4, lr=2e-3, lr_min=1e-5, log_every=1, time_limit=None):
         """Adam with cosine learning-rate decay on (Gs, T); returns per-epoch mean losses."""
         torch = _torch()
-        n = self.n
+        H, W, Z = self._dims()
         ns = Gs.shape[0]
         nb = ns // batch
         total = epochs * nb
         rng = np.random.default_rng(11)
         net = self.net
-        bf = net.buffers(batch, n, n)
-        tgt = torch.zeros(batch * n * n * 2, dtype=torch.float32, device=device())
+        net.buffers(batch, H, W, Z)
+        tgt = torch.zeros(batch * self.nv * self.dim, dtype=torch.float32, device=device())
         hist = []
         t0 = time.perf_counter()
         step = 0
@@ -392,43 +453,107 @@ class PeriodicShearWarmStart:
             acc = torch.zeros(1, dtype=torch.float64, device=device())
             for bi in range(nb):
                 ids = perm[bi * batch:(bi + 1) * batch]
-                net.set_input(batch, n, n, [Gs[i] for i in ids], 0.0, 1.0 / self.sigma)
+                net.set_input(batch, H, W, [Gs[i] for i in ids], 0.0, 1.0 / self.sigma, Z=Z)
                 tgt.copy_(T[torch.from_numpy(ids).to(device())].reshape(-1))
-                net.forward(batch, n, n)
-                loss = net.loss_grad(batch, n, n, tgt)
+                net.forward(batch, H, W, Z=Z)
+                loss = net.loss_grad(batch, H, W, tgt, Z=Z)
                 acc += loss.double()
-                net.backward(batch, n, n)
+                net.backward(batch, H, W, Z=Z)
                 lr_t = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * step / max(total - 1, 1)))
                 net.adam(float(lr_t))
                 step += 1
-            mean = float(acc.item()) / (nb * batch * n * n * 2)
+            mean = float(acc.item()) / (nb * batch * self.nv * self.dim)
             hist.append(mean)
             if log_every and (ep % log_every == 0 or ep == epochs - 1):
                 print(f"epoch {ep}: mse {mean:.4e}  {time.perf_counter() - t0:.1f} s", flush=True)
             if time_limit and time.perf_counter() - t0 > time_limit:
                 break
-        del bf
         return hist
 
 
-def periodic_problem(n, sigma=0.4, seed=4, K=10.0):
-    """Periodic Quad4 Regular(n, n) cell with Tyings::Control + Tyings::Periodic as in
-    docs/examples/statics/periodic_elastoplastic.py:28-47 (iip = control DOFs + origin DOFs),
-    plane strain, per-element G = exp(sigma z), K constant; returns a namespace with the tied PCG."""
+class PeriodicShearWarmStart(PeriodicWarmStart):
+    """The 2-D simple-shear coupling (periodic Quad4 n x n, ctrl(0, 1))."""
+
+    def __init__(self, pcg, tyings, n, net, sigma=0.4, target_scale=None):
+        super().__init__(pcg, tyings, n, net, sigma=sigma, target_scale=target_scale, dim=2, shear="simple")
+
+
+def splitmix_uniform(seed, n):
+    """n uniform [0, 1) numbers from SplitMix64 (53-bit mantissa), identical on every platform."""
+    out = np.empty(n)
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    M = 0xFFFFFFFFFFFFFFFF
+    for i in range(n):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        out[i] = (z >> 11) * (1.0 / (1 << 53))
+    return out
+
+
+def stiff_spheres(n, radius=4.0, fraction=0.2, seed=105):
+    """Element mask (element (i, j, k) at (k n + j) n + i) of periodic spheres in an n^3 cell: centres
+    uniform (seeded), added one at a time until the fraction of element centres inside a sphere
+    reaches `fraction`; returns (mask, number of spheres)."""
+    inside = np.zeros((n, n, n), dtype=bool)  # [i, j, k]
+    u = splitmix_uniform(seed, 3 * 100000)
+    r = int(np.ceil(radius)) + 1
+    off = np.arange(-r, r + 1)
+    count, total, k = 0, n ** 3, 0
+    while count < fraction * total:
+        cen = n * u[3 * k: 3 * k + 3]
+        k += 1
+        idx = [np.floor(cen[a]).astype(int) + off for a in range(3)]
+        d2 = np.zeros((len(off),) * 3)
+        for a in range(3):
+            sh = [1, 1, 1]
+            sh[a] = -1
+            d2 = d2 + ((idx[a] + 0.5 - cen[a]) ** 2).reshape(sh)
+        sel = d2 <= radius * radius
+        I, J, K = np.meshgrid(idx[0] % n, idx[1] % n, idx[2] % n, indexing="ij")
+        new = sel & ~inside[I, J, K]
+        inside[I[new], J[new], K[new]] = True
+        count += int(new.sum())
+    return inside.transpose(2, 1, 0).ravel(), k
+
+
+def sphere_field(n, seed, sigma=0.4, sphere_seed=None):
+    """G = exp(sigma z) (z from `seed`), times 10 inside the stiff spheres of `sphere_seed`."""
+    G = np.exp(sigma * randn(seed, n ** 3))
+    mask, _ = stiff_spheres(n, seed=sphere_seed if sphere_seed is not None else seed + 1000003)
+    G[mask] *= 10.0
+    return G
+
+
+def periodic_problem(n, sigma=0.4, seed=4, K=10.0, dim=2, spheres=False, sphere_seed=105):
+    """Periodic Regular cell, Quad4 n x n plane strain (dim 2) or Hex8 n^3 (dim 3), with
+    Tyings::Control + Tyings::Periodic as in docs/examples/statics/periodic_elastoplastic.py:28-47
+    (iip = control DOFs + origin DOFs); per-element G = exp(sigma z), times 10 inside stiff spheres
+    (dim 3, spheres=True); K constant.  Returns a namespace with the tied PCG."""
     from types import SimpleNamespace
 
     import goosefem_amd as gf
 
-    mesh = gf.Mesh.Quad4.Regular(n, n)
+    mesh = gf.Mesh.Hex8.Regular(n, n, n) if dim == 3 else gf.Mesh.Quad4.Regular(n, n)
     ctrl = gf.Tyings.Control(mesh.coor, mesh.dofs)
     iip = np.concatenate((ctrl.controlDofs.ravel(), ctrl.dofs[mesh.nodesOrigin]))
     ty = gf.Tyings.Periodic(ctrl.coor, ctrl.dofs, ctrl.controlDofs, mesh.nodesPeriodic, iip)
     vec = gf.Vector(mesh.conn, ty.dofs)
-    quad = gf.Element.Quad4.QuadraturePlanar(gf.Vector(mesh.conn, mesh.dofs).AsElement(mesh.coor))
-    G = np.exp(sigma * randn(seed, mesh.nelem))
+    x = gf.Vector(mesh.conn, mesh.dofs).AsElement(mesh.coor)
+    quad = gf.Element.Hex8.Quadrature(x) if dim == 3 else gf.Element.Quad4.QuadraturePlanar(x)
+    nsph = 0
+    if dim == 3 and spheres:
+        G = np.exp(sigma * randn(seed, mesh.nelem))
+        mask, nsph = stiff_spheres(n, seed=sphere_seed)
+        G[mask] *= 10.0
+    else:
+        G = np.exp(sigma * randn(seed, mesh.nelem))
     Kv = np.full(mesh.nelem, K)
     P = gf.TyingsPCG(vec, quad, Kv, G, ty)
-    return SimpleNamespace(mesh=mesh, ty=ty, vec=vec, quad=quad, G=G, K=Kv, P=P, n=n, sigma=sigma)
+    return SimpleNamespace(mesh=mesh, ty=ty, vec=vec, quad=quad, G=G, K=Kv, P=P, n=n, sigma=sigma, dim=dim,
+                           nspheres=nsph)
 
 
 def train_main(argv=None):
@@ -441,6 +566,8 @@ def train_main(argv=None):
     torch = _torch()
     ap = argparse.ArgumentParser(description="train the learned PCG warm start (periodic Quad4 n x n, simple shear)")
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--dim", type=int, default=2, choices=(2, 3),
+                    help="2: Quad4 n^2, simple shear; 3: Hex8 n^3 with stiff spheres, pure shear")
     ap.add_argument("--samples", type=int, default=400)
     ap.add_argument("--holdout", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=40)
@@ -452,26 +579,31 @@ def train_main(argv=None):
     ap.add_argument("--time-limit", type=float, default=None, help="seconds of training per configuration at most")
     ap.add_argument("--out", required=True, help="weights file (.npz); with several configurations a suffix is added")
     a = ap.parse_args(argv)
-    pb = periodic_problem(a.n)
+    pb = periodic_problem(a.n, dim=a.dim, spheres=a.dim == 3)
+    shear = "pure" if a.dim == 3 else "simple"
+    field = (lambda sd: sphere_field(a.n, sd, pb.sigma)) if a.dim == 3 else None
     configs = [tuple(int(c) for c in cfg.split(",")) for cfg in a.channels.split(";")]
     lrs = [float(x) for x in a.lr.split(";")]
     lrs = lrs + [lrs[-1]] * (len(configs) - len(lrs))
-    probe = PeriodicShearWarmStart(pb.P, pb.ty, a.n, WarmStartNet(configs[0]), sigma=pb.sigma, target_scale=1.0)
+    probe = PeriodicWarmStart(pb.P, pb.ty, a.n, WarmStartNet(configs[0], dim=a.dim), sigma=pb.sigma,
+                              target_scale=1.0, dim=a.dim, shear=shear)
     seeds = list(range(a.seed0, a.seed0 + a.samples + a.holdout))
-    Gs, T, its, t_data = probe.make_dataset(pb.K, seeds, tol=a.data_tol, progress=50)
+    Gs, T, its, t_data = probe.make_dataset(pb.K, seeds, tol=a.data_tol, progress=50 if a.dim == 2 else 10,
+                                            field=field)
     scale = float(T[: a.samples].std().item())
     T.div_(scale)
     print(f"data: {len(seeds)} solves, {its} CG iterations, {t_data:.1f} s; target scale {scale:.4e}", flush=True)
     Gtr, Ttr, Gho, Tho = Gs[: a.samples], T[: a.samples], Gs[a.samples:], T[a.samples:]
     for cfg, lr in zip(configs, lrs):
-        net = WarmStartNet(cfg)
-        W = PeriodicShearWarmStart(pb.P, pb.ty, a.n, net, sigma=pb.sigma, target_scale=scale)
+        net = WarmStartNet(cfg, dim=a.dim)
+        W = PeriodicWarmStart(pb.P, pb.ty, a.n, net, sigma=pb.sigma, target_scale=scale, dim=a.dim, shear=shear)
         t0 = time.perf_counter()
         hist = W.train(Gtr, Ttr, a.epochs, batch=a.batch, lr=lr, time_limit=a.time_limit)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
         err = W.holdout_error(Gho, Tho) if a.holdout else float("nan")
-        net.meta = {"target_scale": scale, "sigma": pb.sigma, "n": a.n, "samples": a.samples, "epochs": len(hist),
+        net.meta = {"target_scale": scale, "sigma": pb.sigma, "n": a.n, "dim": a.dim, "shear": shear,
+                    "samples": a.samples, "epochs": len(hist),
                     "batch": a.batch, "lr": lr, "data_seconds": t_data, "train_seconds": t_train, "final_mse": hist[-1],
                     "holdout_rel_error": err, "data_tol": a.data_tol, "seed0": a.seed0, "params": net.nparams()}
         out = a.out

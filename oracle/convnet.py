@@ -11,6 +11,29 @@ central finite-difference check in tests/test_oracle_convnet.py.  No counterpart
 import torch
 
 
+def conv3x3x3(x, w, b=None):
+    """x [N, Z, H, W, Cin], w [27, Cout, Cin] -> [N, Z, H, W, Cout];
+    y[p] = sum_t w[t] x[p + (dz, dy, dx)], t = 9 (dz + 1) + 3 (dy + 1) + dx + 1."""
+    out = None
+    for t in range(27):
+        dz, dy, dx = t // 9 - 1, (t % 9) // 3 - 1, t % 3 - 1
+        xs = torch.roll(x, shifts=(-dz, -dy, -dx), dims=(1, 2, 3))
+        term = torch.einsum("nzhwc,oc->nzhwo", xs, w[t])
+        out = term if out is None else out + term
+    if b is not None:
+        out = out + b
+    return out
+
+
+def avgpool2x2x2(x):
+    N, Z, H, W, C = x.shape
+    return x.reshape(N, Z // 2, 2, H // 2, 2, W // 2, 2, C).mean(dim=(2, 4, 6))
+
+
+def upsample2x2x2(x):
+    return x.repeat_interleave(2, dim=1).repeat_interleave(2, dim=2).repeat_interleave(2, dim=3)
+
+
 def conv3x3(x, w, b=None):
     """x [N, H, W, Cin], w [9, Cout, Cin] -> [N, H, W, Cout]; y[p] = sum_t w[t] x[p + (dy, dx)]."""
     out = None
@@ -39,10 +62,14 @@ def bf16(x):
 
 
 def unet_forward(z, params, channels, round_bf16=False):
-    """z [N, H, W, 1] (normalised log G); params: list of (w, b) in the device network's layer
-    order (encoder pairs per level, decoder pairs per level 0..L-2, output layer).  With
-    round_bf16 the activations are rounded to bfloat16 where the device stores them."""
+    """z [N, H, W, 1] or [N, Z, H, W, 1] (normalised log G); params: list of (w, b) in the device
+    network's layer order (encoder pairs per level, decoder pairs per level 0..L-2, output layer).
+    With round_bf16 the activations are rounded to bfloat16 where the device stores them."""
     L = len(channels)
+    if z.dim() == 5:
+        conv, pool, up = conv3x3x3, avgpool2x2x2, upsample2x2x2
+    else:
+        conv, pool, up = conv3x3, avgpool2, upsample2
     r = bf16 if round_bf16 else (lambda t: t)
     enc = params[: 2 * L]
     dec = params[2 * L: 2 * L + 2 * (L - 1)]
@@ -51,15 +78,15 @@ def unet_forward(z, params, channels, round_bf16=False):
     E = []
     for l in range(L):
         if l > 0:
-            x = r(avgpool2(E[-1]))
+            x = r(pool(E[-1]))
         (w1, b1), (w2, b2) = enc[2 * l], enc[2 * l + 1]
-        a = r(torch.relu(conv3x3(x, w1, b1)))
-        e = r(torch.relu(conv3x3(a, w2, b2)))
+        a = r(torch.relu(conv(x, w1, b1)))
+        e = r(torch.relu(conv(a, w2, b2)))
         E.append(e)
     d = E[-1]
     for l in range(L - 2, -1, -1):
         (w1, b1), (w2, b2) = dec[2 * l], dec[2 * l + 1]
-        cat = torch.cat([upsample2(d), E[l]], dim=3)
-        bb = r(torch.relu(conv3x3(cat, w1, b1)))
-        d = r(torch.relu(conv3x3(bb, w2, b2)))
-    return conv3x3(d, out[0], out[1])
+        cat = torch.cat([up(d), E[l]], dim=-1)
+        bb = r(torch.relu(conv(cat, w1, b1)))
+        d = r(torch.relu(conv(bb, w2, b2)))
+    return conv(d, out[0], out[1])

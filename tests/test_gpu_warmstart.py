@@ -341,3 +341,137 @@ def test_committed_weights_cut_pcg_iterations_at_512(gf, ws):
     _, it1, r1 = pb.P.solve(b, x0=x0, tol=1e-10)
     assert r0 <= 1e-9 and r1 <= 1e-9
     assert it1 < 0.75 * it0, (it0, it1)
+
+
+# ---- 3-D (3x3x3 taps, 2x2x2 pooling) ---------------------------------------------------------------
+
+def packed3(ws, w, cout, cin, cin_s, cout_s):
+    coutp, cinp = ws.coutp_of(cout_s), ws.coutp_of(cin_s)
+    wf = torch.zeros(27 * coutp * cin_s, dtype=torch.bfloat16, device="cuda")
+    wb = torch.zeros(27 * cinp * cout_s, dtype=torch.bfloat16, device="cuda")
+    wd = w.to(torch.float32).cuda().contiguous()
+    ws._call("gf_conv_pack3", ws._i32(cout), ws._i32(cin), ws._dp(wd), ws._i32(coutp), ws._i32(cin_s), ws._dp(wf),
+             ws._i32(cinp), ws._i32(cout_s), ws._dp(wb), ws._stream())
+    return wf, wb, coutp, cinp
+
+
+CONV3_CASES = [
+    # N, Z, H, W, c1, c2, up1, cout
+    (1, 4, 6, 8, 16, 0, 0, 16),
+    (2, 4, 4, 8, 32, 16, 1, 32),
+    (1, 8, 8, 8, 16, 0, 0, 3),
+    (1, 4, 4, 4, 64, 64, 1, 64),
+]
+
+
+@pytest.mark.parametrize("N,Z,H,W,c1,c2,up1,cout", CONV3_CASES)
+def test_conv3d_forward_and_gradients(ws, N, Z, H, W, c1, c2, up1, cout):
+    from oracle import convnet
+
+    cin = c1 + c2
+    cout_s = max(16, (cout + 15) // 16 * 16)
+    x1 = bf16_rand((N, Z >> up1, H >> up1, W >> up1, c1), 31)
+    x2 = bf16_rand((N, Z, H, W, c2), 32) if c2 else None
+    w = (torch.randn(27, cout, cin, dtype=torch.float64, generator=torch.Generator().manual_seed(33)) /
+         np.sqrt(27 * cin)).float()
+    b = torch.linspace(-0.3, 0.3, cout, dtype=torch.float32)
+    wf, wb, coutp, cinp = packed3(ws, w, cout, cin, cin, cout_s)
+    x1d, bd = x1.cuda(), b.cuda()
+    x2d = x2.cuda() if c2 else None
+    P = N * Z * H * W
+    y32 = torch.zeros(P * cout, dtype=torch.float32, device="cuda")
+    ws._call("gf_conv3x3x3", ws._dp(x1d), ws._i32(c1), ws._i32(up1), ws._dp(x2d), ws._i32(c2), ws._i64(P), ws._i32(Z),
+             ws._i32(H), ws._i32(W), ws._dp(wf), ws._i32(coutp), ws._dp(bd), ws._i32(cout), ws._i32(0), None,
+             ws._i32(cout), None, ws._dp(y32), None, ws._stream())
+    a1 = cpu64(x1).requires_grad_()
+    xin = convnet.upsample2x2x2(a1) if up1 else a1
+    a2 = None
+    if c2:
+        a2 = cpu64(x2).requires_grad_()
+        xin = torch.cat([xin, a2], dim=4)
+    wr = convnet.bf16(w.double()).requires_grad_()
+    br = b.double().requires_grad_()
+    ref = convnet.conv3x3x3(xin, wr, br)
+    assert rel(y32.cpu().reshape(N, Z, H, W, cout), ref.detach()) < TOL_F32
+    # gradients
+    gy = torch.zeros(N, Z, H, W, cout_s, dtype=torch.bfloat16)
+    gy[..., :cout] = bf16_rand((N, Z, H, W, cout), 34)
+    ref.backward(cpu64(gy)[..., :cout])
+    gyd = gy.cuda()
+    gfull = torch.zeros(P * cin, dtype=torch.float32, device="cuda")
+    ws._call("gf_conv3x3x3", ws._dp(gyd), ws._i32(cout_s), ws._i32(0), None, ws._i32(0), ws._i64(P), ws._i32(Z),
+             ws._i32(H), ws._i32(W), ws._dp(wb), ws._i32(cinp), None, ws._i32(cin), ws._i32(0), None, ws._i32(cin),
+             None, ws._dp(gfull), None, ws._stream())
+    gfull = gfull.cpu().double().reshape(N, Z, H, W, cin)
+    g1 = gfull[..., :c1]
+    if up1:
+        g1 = g1.reshape(N, Z // 2, 2, H // 2, 2, W // 2, 2, c1).sum(dim=(2, 4, 6))
+    assert rel(g1, a1.grad) < TOL_F32
+    if c2:
+        assert rel(gfull[..., c1:], a2.grad) < TOL_F32
+    ns, nf = ws.ctypes.c_int(0), ws.ctypes.c_int64(0)
+    ws._call("gf_conv3x3x3_wgrad_work", ws._i64(P), ws._i32(cout_s), ws._i32(cin), ws.ctypes.byref(ns),
+             ws.ctypes.byref(nf))
+    work = torch.zeros(nf.value, dtype=torch.float32, device="cuda")
+    dw = torch.zeros(27 * cout * cin, dtype=torch.float32, device="cuda")
+    db = torch.zeros(cout, dtype=torch.float32, device="cuda")
+    ws._call("gf_conv3x3x3_wgrad", ws._dp(x1d), ws._i32(c1), ws._i32(up1), ws._dp(x2d), ws._i32(c2), ws._i64(P),
+             ws._i32(Z), ws._i32(H), ws._i32(W), ws._dp(gyd), ws._i32(cout_s), ws._i32(cout), ws._i32(cin),
+             ws._dp(work), ws._i64(work.numel()), ws._dp(dw), ws._dp(db), ws._stream())
+    assert rel(dw.cpu().reshape(27, cout, cin), wr.grad) < TOL_F32
+    assert rel(db.cpu(), br.grad) < TOL_F32
+
+
+def test_pool3d_kernels(ws):
+    from oracle import convnet
+
+    N, Z, H, W, C = 2, 4, 6, 8, 16
+    x = bf16_rand((N, Z, H, W, C), 41)
+    xd = x.cuda()
+    out = torch.zeros(N * Z * H * W * C // 8, dtype=torch.bfloat16, device="cuda")
+    ws._call("gf_avgpool2x2x2", ws._dp(xd), ws._i64(N), ws._i32(Z), ws._i32(H), ws._i32(W), ws._i32(C), ws._dp(out),
+             ws._stream())
+    ref = convnet.avgpool2x2x2(cpu64(x))
+    assert bool(((cpu64(out).reshape(ref.shape) - ref).abs() <= ULP_BF16 * ref.abs() + 1e-7).all())
+    dout = bf16_rand((N, Z // 2, H // 2, W // 2, C), 42)
+    old = bf16_rand((N, Z, H, W, C), 43)
+    act = bf16_rand((N, Z, H, W, C), 44)
+    din, doutd, actd = old.clone().cuda(), dout.cuda(), act.cuda()
+    ws._call("gf_unpool2x2x2", ws._dp(doutd), ws._i64(N), ws._i32(Z), ws._i32(H), ws._i32(W), ws._i32(C),
+             ws.ctypes.c_float(0.125), ws._i32(1), ws._dp(actd), ws._dp(din), ws._stream())
+    ref = (cpu64(old) + 0.125 * convnet.upsample2x2x2(cpu64(dout))) * (cpu64(act) > 0)
+    assert bool(((cpu64(din) - ref).abs() <= ULP_BF16 * ref.abs() + 1e-7).all())
+    cact = bf16_rand((N, Z // 2, H // 2, W // 2, C), 45)
+    cactd = cact.cuda()
+    so = torch.zeros(N * Z * H * W * C // 8, dtype=torch.bfloat16, device="cuda")
+    ws._call("gf_sumpool2x2x2", ws._dp(xd), ws._i64(N), ws._i32(Z), ws._i32(H), ws._i32(W), ws._i32(C),
+             ws._dp(cactd), ws._dp(so), ws._stream())
+    ref = 8 * convnet.avgpool2x2x2(cpu64(x)) * (cpu64(cact) > 0)
+    assert bool(((cpu64(so).reshape(ref.shape) - ref).abs() <= 2 * ULP_BF16 * ref.abs() + 1e-6).all())
+
+
+def test_network3d_forward_and_gradients_vs_oracle(ws):
+    from oracle import convnet
+
+    channels, B, Z, H, W = (16, 32), 1, 8, 8, 8
+    net = ws.WarmStartNet(channels, seed=3, dim=3)
+    params = []
+    for c in net.layers():
+        c.b.copy_(torch.linspace(-0.05, 0.05, c.cout, device="cuda"))
+        wq = c.w.cpu().double().reshape(27, c.cout, c.cin)
+        params.append((convnet.bf16(wq).requires_grad_(), c.b.cpu().double().requires_grad_()))
+    G = [torch.from_numpy(np.exp(0.4 * np.random.default_rng(2).standard_normal(Z * H * W))).cuda()]
+    net.set_input(B, H, W, G, 0.0, 2.5, Z=Z)
+    pred = net.forward(B, H, W, Z=Z)
+    z = (torch.log(G[0].cpu()) * 2.5).reshape(1, Z, H, W, 1)
+    ref = convnet.unet_forward(convnet.bf16(z), params, channels, round_bf16=True)
+    got = pred.cpu().double().reshape(B, Z, H, W, 3)
+    assert rel(got, ref.detach()) < TOL_NET
+    tgt = torch.randn(B, Z, H, W, 3, dtype=torch.float64, generator=torch.Generator().manual_seed(8))
+    net.loss_grad(B, H, W, tgt.float().cuda().reshape(-1), Z=Z)
+    net.backward(B, H, W, Z=Z)
+    (((ref - tgt) ** 2).mean()).backward()
+    for i, ((wr, br), c) in enumerate(zip(params, net.layers())):
+        e_w = rel(c.dw.cpu().reshape(27, c.cout, c.cin), wr.grad)
+        e_b = rel(c.db.cpu(), br.grad)
+        assert e_w < TOL_NET and e_b < TOL_NET, (i, e_w, e_b)

@@ -86,3 +86,57 @@ def test_unet_gradient_finite_differences():
             lm = loss().item()
             b[0] += h
         assert (lp - lm) / (2 * h) == pytest.approx(b.grad[0].item(), rel=1e-5, abs=1e-9)
+
+
+def test_conv3x3x3_matches_library_circular_conv3d():
+    g = torch.Generator().manual_seed(3)
+    x = torch.randn(2, 4, 6, 8, 3, dtype=torch.float64, generator=g)
+    w = torch.randn(27, 5, 3, dtype=torch.float64, generator=g)
+    b = torch.randn(5, dtype=torch.float64, generator=g)
+    mine = convnet.conv3x3x3(x, w, b)
+    # conv3d cross-correlation, tap t = 9 kz + 3 ky + kx
+    k = w.reshape(3, 3, 3, 5, 3).permute(3, 4, 0, 1, 2)
+    xp = F.pad(x.permute(0, 4, 1, 2, 3), (1, 1, 1, 1, 1, 1), mode="circular")
+    ref = F.conv3d(xp, k, b).permute(0, 2, 3, 4, 1)
+    assert torch.allclose(mine, ref, rtol=1e-13, atol=1e-13)
+    p = convnet.avgpool2x2x2(x[:, :, :, :, :1].repeat(1, 1, 1, 1, 2))
+    assert p.shape == (2, 2, 3, 4, 2)
+    assert torch.allclose(p[0, 1, 2, 3, 0], x[0, 2:4, 4:6, 6:8, 0].mean())
+    u = convnet.upsample2x2x2(p)
+    assert torch.equal(u[:, 1::2, ::2, 1::2], p)
+
+
+def test_unet3d_gradient_finite_differences():
+    ch = (4, 6)
+    g = torch.Generator().manual_seed(4)
+    L = len(ch)
+    shapes = []
+    for l in range(L):
+        cin = 1 if l == 0 else ch[l - 1]
+        shapes += [(ch[l], cin), (ch[l], ch[l])]
+    for l in range(L - 1):
+        shapes += [(ch[l], ch[l + 1] + ch[l]), (ch[l], ch[l])]
+    shapes.append((3, ch[0]))
+    params = []
+    for co, ci in shapes:
+        w = (torch.randn(27, co, ci, dtype=torch.float64, generator=g) * np.sqrt(2.0 / (27 * ci))).requires_grad_()
+        params.append((w, (0.1 * torch.randn(co, dtype=torch.float64, generator=g)).requires_grad_()))
+    z = torch.randn(1, 4, 4, 4, 1, dtype=torch.float64, generator=g)
+    tgt = torch.randn(1, 4, 4, 4, 3, dtype=torch.float64, generator=g)
+
+    def loss():
+        return ((convnet.unet_forward(z, params, ch) - tgt) ** 2).mean()
+
+    loss().backward()
+    h = 1e-6
+    rng = np.random.default_rng(5)
+    for li in (0, 4, len(params) - 1):
+        w, _ = params[li]
+        idx = tuple(int(rng.integers(0, s)) for s in w.shape)
+        with torch.no_grad():
+            w[idx] += h
+            lp = loss().item()
+            w[idx] -= 2 * h
+            lm = loss().item()
+            w[idx] += h
+        assert (lp - lm) / (2 * h) == pytest.approx(w.grad[idx].item(), rel=1e-5, abs=1e-9)
