@@ -44,6 +44,8 @@ struct gf_explicit_s {
     DBuf<float> Ke, Ge;     // per element
     bool k_uniform = false; // all K_e equal: pass the scalar, skip the per-element K stream
     float k_value = 0.0f;
+    bool reg_x = false;     // coordinates are exactly (i sx, j sy): generated in the kernel, not read
+    double sx = 0.0, sy = 0.0;
     DBuf<double> inv_mass, fext;
     bool has_fext = false;
     DBuf<double> u[2], vv[2], a[2];
@@ -201,7 +203,7 @@ __device__ inline void element_force_f32(const ShapeTable& T, const double x[4][
 // ---- structured fused step ----------------------------------------------------------------------
 template <int TX, int TY, bool F32>
 __global__ void __launch_bounds__(TX * TY) k_vv_structured(
-    int nx, int ny, int ntx, int nty, int per_xcd, ShapeTable T, const double* __restrict__ coor,
+    int nx, int ny, int ntx, int nty, int per_xcd, ShapeTable T, const double* __restrict__ coor, double sx, double sy,
     const float* __restrict__ Ke, float Kconst, const float* __restrict__ Ge, const double* __restrict__ inv_mass,
     const double* __restrict__ fext, double dt, double c, double hdt, const double* __restrict__ u0,
     const double* __restrict__ v0, const double* __restrict__ a0, double* __restrict__ u1, double* __restrict__ v1,
@@ -239,7 +241,8 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
         o_u = reinterpret_cast<const double2*>(u0)[on];
         o_v = reinterpret_cast<const double2*>(v0)[on];
         o_a = reinterpret_cast<const double2*>(a0)[on];
-        o_x = reinterpret_cast<const double2*>(coor)[on];
+        o_x = coor ? reinterpret_cast<const double2*>(coor)[on]
+                   : make_double2(static_cast<double>(ogx) * sx, static_cast<double>(ogy) * sy);
         o_im = nt_load2(inv_mass, on);
         if (fext) o_fe = nt_load2(fext, on);
     }
@@ -272,7 +275,8 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
             h_u = reinterpret_cast<const double2*>(u0)[n];
             h_v = reinterpret_cast<const double2*>(v0)[n];
             h_a = reinterpret_cast<const double2*>(a0)[n];
-            h_x = reinterpret_cast<const double2*>(coor)[n];
+            h_x = coor ? reinterpret_cast<const double2*>(coor)[n]
+                       : make_double2(static_cast<double>(gx) * sx, static_cast<double>(gy) * sy);
         }
     }
     double2 o_un;
@@ -554,12 +558,14 @@ void launch_structured(bool f32, unsigned grid, hipStream_t s, int nx, int ny, i
                        int i0, int i1, double* fo)
 {
     if (f32)
-        k_vv_structured<TX, TY, true><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T, x->coor.p,
+        k_vv_structured<TX, TY, true><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T,
+                                                               x->reg_x ? nullptr : x->coor.p, x->sx, x->sy,
                                                                x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
                                                                x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
                                                                x->a[i1].p, fo);
     else
-        k_vv_structured<TX, TY, false><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T, x->coor.p,
+        k_vv_structured<TX, TY, false><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T,
+                                                                x->reg_x ? nullptr : x->coor.p, x->sx, x->sy,
                                                                 x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
                                                                 x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
                                                                 x->a[i1].p, fo);
@@ -609,6 +615,22 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
                 GF_HIP(hipMemset(b->p, 0, v->ndof * sizeof(double)));
             x->fint.alloc(v->ndof);
             x->structured = detect_structured(v, x->nx, x->ny);
+            if (x->structured) {
+                // Regular coordinates (i sx, j sy) bit for bit, as the Regular mesh builds them:
+                // the kernel then generates them instead of streaming 16 bytes per node
+                const int64_t N1 = x->nx + 1, nn = N1 * (x->ny + 1);
+                std::vector<double> ch(static_cast<size_t>(nn) * 2);
+                GF_HIP(hipMemcpy(ch.data(), coor, ch.size() * sizeof(double), hipMemcpyDeviceToHost));
+                const double sx = nn > 1 ? ch[2] : 0.0, sy = x->ny > 0 ? ch[2 * N1 + 1] : 0.0;
+                bool reg = sx > 0.0 && sy > 0.0;
+                for (int64_t n = 0; reg && n < nn; ++n) {
+                    const int64_t i = n % N1, j = n / N1;
+                    reg = ch[2 * n] == static_cast<double>(i) * sx && ch[2 * n + 1] == static_cast<double>(j) * sy;
+                }
+                x->reg_x = reg;
+                x->sx = sx;
+                x->sy = sy;
+            }
             if (!x->structured) {
                 // stored Gauss geometry (QuadraturePlanar(AsElement(coor)))
                 DBuf<double> xe(v->nelem * 8);

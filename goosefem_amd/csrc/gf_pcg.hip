@@ -323,13 +323,14 @@ __global__ void __launch_bounds__(TX * TY * TZ) k_apply_periodic_h8(int n, int n
         }
     }
     const int64_t kn = ((static_cast<int64_t>(z0 + olz) * n + y0 + oly) * n + x0 + olx) * 3;
+    const int ho = ((olz + 1) * HY + oly + 1) * HX + olx + 1; // the owned node in the halo tile
     double dot = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int32_t d = sidx[kn + c];
         if (d >= 0) {
             y[d] = acc[c];
-            dot += x[d] * acc[c];
+            dot += s_x[c][ho] * acc[c];
         }
     }
     if (partial) block_reduce_store_n<NT>(dot, partial);
@@ -1007,6 +1008,8 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         GF_HIP(hipMemcpyAsync(P->state.p + 2, &cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
         check_launch();
         int32_t st[2] = {0, 0};
+        // Quad4: the operator forms the search direction (two launches per iteration).  Hex8 keeps
+        // the separate direction pass: its 3-D halo would read z and p_prev 2.3 times per node.
         const bool fused = P->structured && P->et == GF_QUAD4_PLANAR;
         ShapeQ4 sh;
         for (int q = 0; q < 4; ++q) {

@@ -209,8 +209,10 @@ def test_matrix_csr_bit_exact(gf, et, periodic):
     assert np.array_equal(host(A.Todense()), dense)
 
 
-def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False):
+def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, distort=False):
     coor, conn = oracle.quad4_regular(nx, ny)
+    if distort:  # structured connectivity, perturbed coordinates: the kernel must read them
+        coor = coor + 0.15 * oracle.randn(seed + 300, coor.size).reshape(coor.shape)
     if shuffle:  # same mesh, elements renumbered: defeats the structured-grid detection
         perm = np.random.default_rng(5).permutation(conn.shape[0])
         conn = conn[perm]
@@ -241,10 +243,11 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False):
     return coor, conn, dofs, q, inv, Ke, Ge, u, v, a
 
 
-@pytest.mark.parametrize("nx,ny,shuffle,kvar", [(64, 64, False, False), (130, 17, False, True), (3, 5, False, False),
-                                               (24, 20, True, True)])
-def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar):
-    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle, kvar=kvar)
+@pytest.mark.parametrize("nx,ny,shuffle,kvar,distort", [(64, 64, False, False, False), (130, 17, False, True, False),
+                                                       (3, 5, False, False, False), (24, 20, True, True, False),
+                                                       (40, 24, False, False, True)])
+def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar, distort):
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle, kvar=kvar, distort=distort)
     vec = gf.Vector(conn, dofs)
     X = gf.ExplicitQuad4(vec, coor, Ke, Ge, inv, 0.01)
     assert X.structured == (not shuffle)
@@ -257,9 +260,10 @@ def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar):
         assert np.array_equal(host(X.fint()), fint)
 
 
-@pytest.mark.parametrize("shuffle,kvar", [(False, False), (False, True), (True, False)])
-def test_explicit_fp32_tolerance(gf, shuffle, kvar):
-    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(64, 48, shuffle=shuffle, kvar=kvar)
+@pytest.mark.parametrize("shuffle,kvar,distort", [(False, False, False), (False, True, False), (True, False, False),
+                                                 (False, False, True)])
+def test_explicit_fp32_tolerance(gf, shuffle, kvar, distort):
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(64, 48, shuffle=shuffle, kvar=kvar, distort=distort)
     vec = gf.Vector(conn, dofs)
     X = gf.ExplicitQuad4(vec, coor, Ke, Ge, inv, 0.01)
     X.set_state(u, v, a)

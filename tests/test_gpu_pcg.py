@@ -160,3 +160,7 @@ def test_structured_periodic_hex8_operator_and_solve(gf):
     assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-6
     x2, it2, _ = P.solve(b, tol=1e-10)
     assert it2 == it and np.array_equal(host(x2), x)
+    _, it3, res3 = P.solve(b, tol=1e-12, maxiter=3, raise_on_fail=False)
+    assert it3 == 3 and res3 > 1e-12
+    _, it4, res4 = P.solve(b, x0=xd, tol=1e-10)
+    assert it4 == 0 and res4 < 1e-10
