@@ -313,13 +313,14 @@ __global__ void __launch_bounds__(TX * TY * TZ) k_apply_periodic_h8(int n, int n
         for (int j = 0; j < 3; ++j) {
             const double* rk = kkg + (la * 3 + j) * 24;
             const double* rg = kkg + 576 + (la * 3 + j) * 24;
+            // fused multiply-adds (this operator is not tied to the reference's expression order)
             double sk = 0.0, sg = 0.0;
 #pragma unroll
             for (int c = 0; c < 24; ++c) {
-                sk += rk[c] * ue[c];
-                sg += rg[c] * ue[c];
+                sk = fma(rk[c], ue[c], sk);
+                sg = fma(rg[c], ue[c], sg);
             }
-            acc[j] += k * sk + gm * sg;
+            acc[j] = fma(k, sk, fma(gm, sg, acc[j]));
         }
     }
     const int64_t kn = ((static_cast<int64_t>(z0 + olz) * n + y0 + oly) * n + x0 + olx) * 3;
@@ -695,6 +696,46 @@ void compute_invdiag(gf_pcg P, hipStream_t s)
     check_launch();
 }
 
+// flags[block] = 1 if any element's n values differ from element 0's by more than tol (no atomics:
+// an ordered LDS OR per block, one store per block)
+__global__ void __launch_bounds__(BS) k_uniform_check(int64_t nelem, int n, const double* __restrict__ data, double tol,
+                                                      int32_t* __restrict__ flags)
+{
+    __shared__ int32_t sh[BS];
+    int32_t bad = 0;
+    const int64_t total = nelem * n;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * BS)
+        bad |= fabs(data[i] - data[i % n]) > tol ? 1 : 0;
+    sh[threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] |= sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) flags[blockIdx.x] = sh[0];
+}
+
+// true if every element's n values of `data` equal element 0's to 1e-12 relative; values of element 0
+// are returned in first[0..n)
+bool uniform_per_element(const double* data, int64_t nelem, int n, double* first)
+{
+    GF_HIP(hipMemcpy(first, data, n * sizeof(double), hipMemcpyDeviceToHost));
+    double vmax = 0.0;
+    for (int k = 0; k < n; ++k) vmax = std::max(vmax, std::abs(first[k]));
+    if (!(vmax > 0.0)) return false;
+    constexpr int NBLK = 1024;
+    DBuf<int32_t> flags;
+    flags.alloc(NBLK);
+    k_uniform_check<<<NBLK, BS>>>(nelem, n, data, 1e-12 * vmax, flags.p);
+    check_launch();
+    std::vector<int32_t> h(NBLK);
+    GF_HIP(hipMemcpy(h.data(), flags.p, NBLK * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int32_t f : h)
+        if (f) return false;
+    return true;
+}
+
 // Regular n x n Quad4 cell whose dependent DOFs are periodic copies (C_du rows = one unit entry of
 // the partner's u-DOF, or empty when the partner is prescribed) and whose elements share one
 // geometry: then K' x_u is applied by k_apply_periodic_q4.
@@ -735,19 +776,9 @@ void detect_structured(gf_pcg P, const int64_t* cdu_ptr_h, const int64_t* cdu_co
             }
     // uniform geometry: every element's dN/dx and dV equal element 0's to 1e-12 relative (a Regular
     // mesh gives the same Jacobian everywhere up to the rounding of the coordinate sums, ~1e-16)
-    std::vector<double> g(static_cast<size_t>(P->nelem) * 32), vol(static_cast<size_t>(P->nelem) * 4);
-    GF_HIP(hipMemcpy(g.data(), quad_dNx(P->q), g.size() * sizeof(double), hipMemcpyDeviceToHost));
-    GF_HIP(hipMemcpy(vol.data(), quad_vol(P->q), vol.size() * sizeof(double), hipMemcpyDeviceToHost));
-    double gmax = 0.0, vmax = 0.0;
-    for (int k = 0; k < 32; ++k) gmax = std::max(gmax, std::abs(g[k]));
-    for (int k = 0; k < 4; ++k) vmax = std::max(vmax, std::abs(vol[k]));
-    if (!(gmax > 0.0 && vmax > 0.0)) return;
-    for (int64_t e = 1; e < P->nelem; ++e) {
-        for (int k = 0; k < 32; ++k)
-            if (std::abs(g[e * 32 + k] - g[k]) > 1e-12 * gmax) return;
-        for (int k = 0; k < 4; ++k)
-            if (std::abs(vol[e * 4 + k] - vol[k]) > 1e-12 * vmax) return;
-    }
+    std::vector<double> g(32), vol(4);
+    if (!uniform_per_element(quad_dNx(P->q), P->nelem, 32, g.data())) return;
+    if (!uniform_per_element(quad_vol(P->q), P->nelem, 4, vol.data())) return;
     std::vector<int32_t> sidx(static_cast<size_t>(n * n * 2));
     for (int64_t j = 0; j < n; ++j)
         for (int64_t i = 0; i < n; ++i)
@@ -807,19 +838,9 @@ void detect_structured_hex8(gf_pcg P, const int64_t* cdu_ptr_h, const int64_t* c
                         return;
                 }
     constexpr int NGE = 8 * 8 * 3;
-    std::vector<double> g(static_cast<size_t>(P->nelem) * NGE), vol(static_cast<size_t>(P->nelem) * 8);
-    GF_HIP(hipMemcpy(g.data(), quad_dNx(P->q), g.size() * sizeof(double), hipMemcpyDeviceToHost));
-    GF_HIP(hipMemcpy(vol.data(), quad_vol(P->q), vol.size() * sizeof(double), hipMemcpyDeviceToHost));
-    double gmax = 0.0, vmax = 0.0;
-    for (int t = 0; t < NGE; ++t) gmax = std::max(gmax, std::abs(g[t]));
-    for (int t = 0; t < 8; ++t) vmax = std::max(vmax, std::abs(vol[t]));
-    if (!(gmax > 0.0 && vmax > 0.0)) return;
-    for (int64_t e = 1; e < P->nelem; ++e) {
-        for (int t = 0; t < NGE; ++t)
-            if (std::abs(g[e * NGE + t] - g[t]) > 1e-12 * gmax) return;
-        for (int t = 0; t < 8; ++t)
-            if (std::abs(vol[e * 8 + t] - vol[t]) > 1e-12 * vmax) return;
-    }
+    std::vector<double> g(NGE), vol(8);
+    if (!uniform_per_element(quad_dNx(P->q), P->nelem, NGE, g.data())) return;
+    if (!uniform_per_element(quad_vol(P->q), P->nelem, 8, vol.data())) return;
     // KK(m*3+j, n*3+k) = sum_q dV dNx(m,j) dNx(n,k)
     // KG(m*3+j, n*3+k) = sum_q dV [dNx(m,k) dNx(n,j) + delta_jk dNx(m).dNx(n) - 2/3 dNx(m,j) dNx(n,k)]
     std::vector<double> kkg(2 * 576, 0.0);
