@@ -9,8 +9,9 @@
 // x = hi + lo with hi = bf16(x), lo = bf16(x - hi); the product accumulates hi*hi + hi*lo + lo*hi
 // in FP32 (the dropped lo*lo term and the bf16 rounding of lo are ~2^-16 relative).
 // One wavefront owns one element at a time: C_q (8 x 81), dNx and dV are staged in LDS with
-// coalesced loads, T is built in LDS as bf16 hi/lo pairs, the 15 MFMAs accumulate the 32 x 32 tile,
-// and the 24 x 24 result is written back through LDS as contiguous FP64 rows.
+// coalesced 16-byte loads (the next element's loads are issued before the current one is computed);
+// each MFMA lane evaluates the T entries it feeds, the 15 MFMAs accumulate the 32 x 32 tile, and the
+// 24 x 24 result is stored straight from the accumulators as contiguous FP64 rows.
 //
 // GF_F64: the same per-entry accumulation order as the reference loop (q, i, l for each (j, k)),
 // with one wavefront per element reading C and dNx from LDS (each thread owns one (m, n) block).
@@ -41,9 +42,6 @@ struct WaveLDS {
     float C[NC];
     float g[NG];
     float vol[NQ];
-    __bf16 Thi[KKP][24];
-    __bf16 Tlo[KKP][24];
-    float K[24 * 24];
 };
 
 __device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo)
@@ -120,69 +118,47 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // T[kk][(n,k)], kk = q*9 + j2*3 + i
-        for (int t = lane; t < KKP * 24; t += 64) {
-            const int kk = t / 24, col = t - (t / 24) * 24;
-            float v = 0.0f;
-            if (kk < KK) {
-                const int q = kk / 9, r = kk - q * 9, j2 = r / 3, i = r - j2 * 3;
-                const int n = col / 3, k = col - n * 3;
-                const float* Cq = L.C + q * 81 + ((i * 3 + j2) * 3 + k) * 3;
-                const float* gq = L.g + (q * NN + n) * 3;
-                v = (Cq[0] * gq[0] + Cq[1] * gq[1] + Cq[2] * gq[2]) * L.vol[q];
-            }
-            __bf16 hi, lo;
-            split_bf16(v, hi, lo);
-            L.Thi[kk][col] = hi;
-            L.Tlo[kk][col] = lo;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // MFMA: lane supplies A row (l % 32) and B column (l % 32), contraction slots 8*(l/32)..+7
+        // MFMA: lane supplies A row (l % 32) and B column (l % 32), contraction slots 8*(l/32)..+7.
+        // Row (m, j) of A and column (n, k) of T share the decomposition row = 3 m + j = 3 n + k;
+        // each lane evaluates its own T entries T[kk][(n,k)] = dV_q sum_l C_q(i, j2, k, l) dNx(q,n,l)
+        // straight from the staged C and dNx (kk = q*9 + j2*3 + i).
         const int row = lane % 32, half = lane / 32;
         const int m = row / 3, j = row - (row / 3) * 3;
         floatx16 acc = {};
-#pragma unroll
+#pragma unroll 1
         for (int ch = 0; ch < KKP / 16; ++ch) {
             bf16x8 ahi, alo, bhi, blo;
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const int kk = ch * 16 + half * 8 + s;
-                float av = 0.0f;
+                float av = 0.0f, bv = 0.0f;
                 if (row < 24 && kk < KK) {
                     const int q = kk / 9, r = kk - q * 9, j2 = r / 3, i = r - j2 * 3;
                     av = (j2 == j) ? L.g[(q * NN + m) * 3 + i] : 0.0f;
+                    const float* Cq = L.C + q * 81 + ((i * 3 + j2) * 3 + j) * 3;
+                    const float* gq = L.g + (q * NN + m) * 3;
+                    bv = (Cq[0] * gq[0] + Cq[1] * gq[1] + Cq[2] * gq[2]) * L.vol[q];
                 }
                 __bf16 h, l;
                 split_bf16(av, h, l);
                 ahi[s] = h;
                 alo[s] = l;
-                if (row < 24) {
-                    bhi[s] = L.Thi[kk][row];
-                    blo[s] = L.Tlo[kk][row];
-                }
-                else {
-                    bhi[s] = static_cast<__bf16>(0.0f);
-                    blo[s] = static_cast<__bf16>(0.0f);
-                }
+                split_bf16(bv, h, l);
+                bhi[s] = h;
+                blo[s] = l;
             }
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
         }
-        // accumulator layout: value r of lane l is C[8*(r/4) + 4*(l/32) + r%4][l % 32]
+        // accumulator layout: value r of lane l is K[8*(r/4) + 4*(l/32) + r%4][l % 32]; each store
+        // instruction writes two contiguous 192-byte rows
+        double* Ke = elemmat + e * 576;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
+        for (int r = 0; r < 12; ++r) {
             const int i = 8 * (r / 4) + 4 * half + (r % 4);
-            if (i < 24 && row < 24) L.K[i * 24 + row] = acc[r];
+            if (row < 24) Ke[i * 24 + row] = static_cast<double>(acc[r]);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double2* Ke = reinterpret_cast<double2*>(elemmat + e * 576);
-        for (int t = lane; t < 288; t += 64)
-            Ke[t] = make_double2(static_cast<double>(L.K[2 * t]), static_cast<double>(L.K[2 * t + 1]));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
