@@ -228,6 +228,43 @@ __global__ void k_elemmat_dot(int64_t nrows, int K, const double* __restrict__ K
     out[t] = s;
 }
 
+// the same product for K = 8 or 24: each wave streams 64 rows (64 K contiguous doubles) with
+// coalesced loads, multiplies by the gathered element vector, stages the products in LDS and sums
+// each row in column order (the order of k_elemmat_dot, so the results are identical)
+template <int K>
+__global__ void __launch_bounds__(256) k_elemmat_dot_rows(int64_t nrows, const double* __restrict__ Ke,
+                                                         const double* __restrict__ ue, double* __restrict__ out)
+{
+    constexpr int RP = K + 1; // LDS row pitch
+    __shared__ double sh[4][64 * RP];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * 64;
+    if (r0 >= nrows) return; // uniform per wave
+    const int64_t base = r0 * K, total = nrows * K;
+#pragma unroll 4
+    for (int k = 0; k < K; ++k) {
+        const int loc = k * 64 + l;
+        const int64_t f = base + loc;
+        double p = 0.0;
+        if (f < total) {
+            const int64_t e = f / (K * K);
+            const int c = static_cast<int>(f % K);
+            p = Ke[f] * ue[e * K + c];
+        }
+        sh[w][(loc / K) * RP + loc % K] = p;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t t = r0 + l;
+    if (t < nrows) {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc += sh[w][l * RP + c];
+        out[t] = acc;
+    }
+}
+
 unsigned grid_for(int64_t nelem)
 {
     int64_t g = (nelem + WAVES - 1) / WAVES;
@@ -271,7 +308,13 @@ extern "C" int gf_elemmat_dot(int64_t nelem, int K, const double* elemmat, const
         GF_REQUIRE(elemmat && elemvec && out && K > 0, "invalid argument");
         const int64_t n = nelem * K;
         if (n == 0) return;
-        k_elemmat_dot<<<nblocks(n, 256), 256, 0, S(stream)>>>(n, K, elemmat, elemvec, out);
+        const unsigned gw = static_cast<unsigned>((n + 255) / 256); // 4 waves x 64 rows per block
+        if (K == 24)
+            k_elemmat_dot_rows<24><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
+        else if (K == 8)
+            k_elemmat_dot_rows<8><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
+        else
+            k_elemmat_dot<<<nblocks(n, 256), 256, 0, S(stream)>>>(n, K, elemmat, elemvec, out);
         check_launch();
     });
 }
