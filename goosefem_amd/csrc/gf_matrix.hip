@@ -109,7 +109,39 @@ __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int nne, 
     }
     __syncthreads();
     const int K = nne * nd;
-    if (active) {
+    if (active && K <= L) {
+        // one column per lane: the lookups and values of up to 8 contributions are issued together,
+        // then added in contribution order
+        const int s = lane;
+        const int32_t cend = cptr[row + 1];
+        for (int32_t c0 = cptr[row]; c0 < cend; c0 += 8) {
+            int pos[8];
+            double pv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                pos[j] = -1;
+                pv[j] = 0.0;
+                if (c0 + j < cend && s < K) {
+                    const int64_t k = cidx[c0 + j];
+                    const int64_t e = k / K;
+                    const int r = static_cast<int>(k - e * K);
+                    const int32_t cc = dofs[static_cast<int64_t>(conn[e * nne + s / nd]) * nd + s % nd];
+                    pos[j] = find_col(s_col[g], n, cc);
+                    pv[j] = elemmat[(e * K + r) * K + s];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (pos[j] >= 0) s_acc[g][pos[j]] += pv[j];
+                // contributions of one row are added strictly in order: make this contribution's
+                // LDS writes visible to the next one's reads across the row's lanes
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            }
+        }
+    }
+    else if (active) {
         for (int32_t c = cptr[row]; c < cptr[row + 1]; ++c) {
             const int64_t k = cidx[c];
             const int64_t e = k / K;
