@@ -582,10 +582,10 @@ def train_main(argv=None):
     pb = periodic_problem(a.n, dim=a.dim, spheres=a.dim == 3)
     shear = "pure" if a.dim == 3 else "simple"
     field = (lambda sd: sphere_field(a.n, sd, pb.sigma)) if a.dim == 3 else None
-    configs = [tuple(int(c) for c in cfg.split(",")) for cfg in a.channels.split(";")]
+    cfgs = [tuple(int(c) for c in cfg.split(",")) for cfg in a.channels.split(";")]
     lrs = [float(x) for x in a.lr.split(";")]
-    lrs = lrs + [lrs[-1]] * (len(configs) - len(lrs))
-    probe = PeriodicWarmStart(pb.P, pb.ty, a.n, WarmStartNet(configs[0], dim=a.dim), sigma=pb.sigma,
+    lrs = lrs + [lrs[-1]] * (len(cfgs) - len(lrs))
+    probe = PeriodicWarmStart(pb.P, pb.ty, a.n, WarmStartNet(cfgs[0], dim=a.dim), sigma=pb.sigma,
                               target_scale=1.0, dim=a.dim, shear=shear)
     seeds = list(range(a.seed0, a.seed0 + a.samples + a.holdout))
     Gs, T, its, t_data = probe.make_dataset(pb.K, seeds, tol=a.data_tol, progress=50 if a.dim == 2 else 10,
@@ -594,7 +594,7 @@ def train_main(argv=None):
     T.div_(scale)
     print(f"data: {len(seeds)} solves, {its} CG iterations, {t_data:.1f} s; target scale {scale:.4e}", flush=True)
     Gtr, Ttr, Gho, Tho = Gs[: a.samples], T[: a.samples], Gs[a.samples:], T[a.samples:]
-    for cfg, lr in zip(configs, lrs):
+    for cfg, lr in zip(cfgs, lrs):
         net = WarmStartNet(cfg, dim=a.dim)
         W = PeriodicWarmStart(pb.P, pb.ty, a.n, net, sigma=pb.sigma, target_scale=scale, dim=a.dim, shear=shear)
         t0 = time.perf_counter()
@@ -607,7 +607,7 @@ def train_main(argv=None):
                     "batch": a.batch, "lr": lr, "data_seconds": t_data, "train_seconds": t_train, "final_mse": hist[-1],
                     "holdout_rel_error": err, "data_tol": a.data_tol, "seed0": a.seed0, "params": net.nparams()}
         out = a.out
-        if len(configs) > 1:
+        if len(cfgs) > 1:
             stem, ext = os.path.splitext(a.out)
             out = f"{stem}_{'-'.join(map(str, cfg))}{ext}"
         net.save(out)

@@ -209,6 +209,46 @@ def test_matrix_csr_bit_exact(gf, et, periodic):
     assert np.array_equal(host(A.Todense()), dense)
 
 
+@pytest.mark.parametrize("et", [QUAD4, HEX8])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_single_element_mesh(gf, et, periodic):
+    # the smallest mesh the generators make; periodic ties every node to node 0, so all element
+    # contributions collapse onto ndim DOFs (the longest ordered accumulation per entry)
+    if et == HEX8:
+        c, conn = oracle.hex8_regular(1, 1, 1)
+        ties = oracle.hex8_nodes_periodic(1, 1, 1)
+    else:
+        c, conn = oracle.quad4_regular(1, 1)
+        ties = oracle.quad4_nodes_periodic(1, 1)
+    nd = c.shape[1]
+    dofs = oracle.dofs_periodic(c.shape[0], nd, ties) if periodic else oracle.dofs(c.shape[0], nd)
+    ov = oracle.Vector(conn, dofs)
+    gv = gf.Vector(conn, dofs)
+    assert gv.ndof == ov.ndof == (nd if periodic else c.size)
+    rng = np.random.default_rng(21)
+    ev = rng.standard_normal(ov.conn.shape + (nd,))
+    dv = rng.standard_normal(ov.ndof)
+    assert np.array_equal(host(gv.AssembleDofs(ev)), ov.AssembleDofs(ev))
+    assert np.array_equal(host(gv.AsElement(dv)), ov.AsElement(dv))
+    x = oracle.Vector(conn, oracle.dofs(c.shape[0], nd)).AsElement(c)
+    q = oracle.Quadrature(et, x)
+    gq = gquad(gf, et, x)
+    assert np.array_equal(host(gq.dV), q.dV())
+    C = rng.standard_normal((1, q.nip) + (q.tdim,) * 4)
+    Ke = q.Int_gradN_dot_tensor4_dot_gradNT_dV(C)
+    assert np.array_equal(host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C)), Ke)
+    rowptr, col, val = oracle.matrix_assemble(conn, dofs, Ke)
+    A = gf.Matrix(conn, dofs)
+    A.assemble(Ke)
+    grp, gcol, gval = (host(t) for t in A.csr())
+    assert np.array_equal(grp, rowptr) and np.array_equal(gcol, col) and np.array_equal(gval, val)
+    Kd = Ke * np.eye(Ke.shape[1])  # MatrixDiagonal takes diagonal element matrices
+    diag = oracle.diag_assemble(conn, dofs, Kd)
+    gM = gf.MatrixDiagonal(conn, dofs)
+    gM.assemble(Kd)
+    assert np.array_equal(host(gM.data), diag)
+
+
 def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, distort=False):
     coor, conn = oracle.quad4_regular(nx, ny)
     if distort:  # structured connectivity, perturbed coordinates: the kernel must read them

@@ -1,6 +1,6 @@
 """Tied-system PCG on the GPU: the matrix-free K' and K'_up of MatrixPartitionedTyings against a
 dense restatement built from the oracle's element stiffness, the reordered [u, p, d] dofs and the
-tying matrices; the solve against a direct dense solve with the true residual as judge."""
+tying matrices; the solve against a direct dense solve, with the true residual as the criterion."""
 
 import numpy as np
 import pytest
