@@ -475,11 +475,6 @@ __global__ void k_input_logG(int64_t P, const double* __restrict__ G, double shi
     x[i] = static_cast<__bf16>(c == 0 ? static_cast<float>(scale * (log(G[p]) - shift)) : 0.0f);
 }
 
-bool aligned16(const void* p)
-{
-    return reinterpret_cast<uintptr_t>(p) % 16 == 0;
-}
-
 void check_conv_shapes(int64_t P, int Z, int H, int W, int c1, int up1, int c2, int D)
 {
     GF_REQUIRE(P > 0 && Z > 0 && H > 0 && W > 0 && P % (static_cast<int64_t>(Z) * H * W) == 0,

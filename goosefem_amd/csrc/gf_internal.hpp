@@ -110,6 +110,8 @@ inline unsigned nblocks(int64_t n, int bs) { return static_cast<unsigned>((n + b
 
 inline void check_launch() { GF_HIP(hipGetLastError()); }
 
+inline bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
+
 // Host-side parallel loop over [0, n) in contiguous chunks (set-up code only).
 template <class F>
 inline void parallel_for(int64_t n, F&& f)

@@ -7,7 +7,10 @@
 // ascending order (the DOF-centric list of the Vector), and for each contribution the lanes add
 // the element-matrix row Ke[e, m*nd+i, :] into the row's accumulators (in LDS) at the positions of
 // their columns.  Every CSR value therefore receives its element contributions in triplet order:
-// FP64 bit-identical to the reference's duplicate sum, with no atomics and no E*K*K index table.
+// FP64 bit-identical to the reference's duplicate sum, with no atomics.  The column positions
+// are found once, when the pattern is built (binary search of each element column in the row's
+// sorted column list), and kept as one byte per element-matrix entry (rows of at most 256
+// entries), so an assembly only streams the element matrices, the byte map and the values.
 // SpMV: lanes form the products val*x in parallel, one lane sums them in ascending column order
 // (bit-identical to a sequential row loop).
 #include <algorithm>
@@ -30,6 +33,7 @@ struct gf_matrix_s {
     bool assembled = false;
     DBuf<int32_t> rowptr, col;
     DBuf<double> val;
+    DBuf<uint8_t> posmap; // [contribution c][column s]: position of the column in row c's CSR row
 };
 
 namespace {
@@ -82,18 +86,51 @@ __device__ inline int find_col(const int32_t* cols, int n, int32_t c)
     return lo;
 }
 
-// L lanes per row, CAP = LDS capacity per row (>= max row nnz).
+// Column positions, once per pattern: posmap[c * K + s] is the position, in CSR row d, of column
+// s of contribution c (the c-th entry of the DOF-element list, c in [cptr[d], cptr[d + 1])).
+// L lanes per row, CAP >= the longest row (<= 256, so a position fits one byte).
 template <int L, int CAP>
-__global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int nne, int nd, const int32_t* __restrict__ conn,
-                                                        const int32_t* __restrict__ dofs,
-                                                        const int32_t* __restrict__ cptr,
-                                                        const int32_t* __restrict__ cidx,
-                                                        const int32_t* __restrict__ rowptr,
-                                                        const int32_t* __restrict__ col,
-                                                        const double* __restrict__ elemmat, double* __restrict__ val)
+__global__ void __launch_bounds__(BS) k_matrix_posmap(int64_t nrows, int nne, int nd, const int32_t* __restrict__ conn,
+                                                      const int32_t* __restrict__ dofs,
+                                                      const int32_t* __restrict__ cptr,
+                                                      const int32_t* __restrict__ cidx,
+                                                      const int32_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ col, uint8_t* __restrict__ posmap)
 {
     constexpr int R = BS / L;
     __shared__ int32_t s_col[R][CAP];
+    const int g = threadIdx.x / L, lane = threadIdx.x % L;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * R + g;
+    const bool active = row < nrows;
+    int n = 0;
+    if (active) {
+        const int b = rowptr[row];
+        n = rowptr[row + 1] - b;
+        for (int p = lane; p < n; p += L) s_col[g][p] = col[b + p];
+    }
+    __syncthreads();
+    if (!active) return;
+    const int K = nne * nd;
+    for (int32_t c = cptr[row]; c < cptr[row + 1]; ++c) {
+        const int64_t e = cidx[c] / K;
+        for (int s = lane; s < K; s += L) {
+            const int32_t cc = dofs[static_cast<int64_t>(conn[e * nne + s / nd]) * nd + s % nd];
+            posmap[static_cast<int64_t>(c) * K + s] = static_cast<uint8_t>(find_col(s_col[g], n, cc));
+        }
+    }
+}
+
+// Assembly from the position map.  L lanes per row (K <= L), CAP >= the longest row.  Lane s adds
+// column s of each contribution; the values and positions of up to 8 contributions are loaded
+// together, then added to the row's LDS accumulators strictly in contribution order.
+template <int L, int CAP>
+__global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int K, const int32_t* __restrict__ cptr,
+                                                        const int32_t* __restrict__ cidx,
+                                                        const uint8_t* __restrict__ posmap,
+                                                        const int32_t* __restrict__ rowptr,
+                                                        const double* __restrict__ elemmat, double* __restrict__ val)
+{
+    constexpr int R = BS / L;
     __shared__ double s_acc[R][CAP];
     const int g = threadIdx.x / L, lane = threadIdx.x % L;
     const int64_t row = static_cast<int64_t>(blockIdx.x) * R + g;
@@ -102,17 +139,10 @@ __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int nne, 
     if (active) {
         b = rowptr[row];
         n = rowptr[row + 1] - b;
-        for (int p = lane; p < n; p += L) {
-            s_col[g][p] = col[b + p];
-            s_acc[g][p] = 0.0;
-        }
+        for (int p = lane; p < n; p += L) s_acc[g][p] = 0.0;
     }
     __syncthreads();
-    const int K = nne * nd;
-    if (active && K <= L) {
-        // one column per lane: the lookups and values of up to 8 contributions are issued together,
-        // then added in contribution order
-        const int s = lane;
+    if (active && lane < K) {
         const int32_t cend = cptr[row + 1];
         for (int32_t c0 = cptr[row]; c0 < cend; c0 += 8) {
             int pos[8];
@@ -121,13 +151,10 @@ __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int nne, 
             for (int j = 0; j < 8; ++j) {
                 pos[j] = -1;
                 pv[j] = 0.0;
-                if (c0 + j < cend && s < K) {
-                    const int64_t k = cidx[c0 + j];
-                    const int64_t e = k / K;
-                    const int r = static_cast<int>(k - e * K);
-                    const int32_t cc = dofs[static_cast<int64_t>(conn[e * nne + s / nd]) * nd + s % nd];
-                    pos[j] = find_col(s_col[g], n, cc);
-                    pv[j] = elemmat[(e * K + r) * K + s];
+                if (c0 + j < cend) {
+                    const int64_t c = c0 + j;
+                    pos[j] = posmap[c * K + lane];
+                    pv[j] = elemmat[static_cast<int64_t>(cidx[c]) * K + lane];
                 }
             }
 #pragma unroll
@@ -139,23 +166,6 @@ __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int nne, 
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             }
-        }
-    }
-    else if (active) {
-        for (int32_t c = cptr[row]; c < cptr[row + 1]; ++c) {
-            const int64_t k = cidx[c];
-            const int64_t e = k / K;
-            const int r = static_cast<int>(k - e * K);
-            for (int s = lane; s < K; s += L) {
-                const int32_t cc = dofs[static_cast<int64_t>(conn[e * nne + s / nd]) * nd + s % nd];
-                const int p = find_col(s_col[g], n, cc);
-                s_acc[g][p] += elemmat[(e * K + r) * K + s];
-            }
-            // contributions of one row are added strictly in order: make this contribution's
-            // LDS writes visible to the next one's reads across the row's lanes
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
     }
     __syncthreads();
@@ -429,6 +439,14 @@ int gf_matrix_create(gf_matrix* out, gf_vector v)
             A->rowptr.upload(rowptr);
             A->col.upload(colv);
             A->val.alloc(nnz);
+            if (!A->degenerate && K <= 32 && maxrow <= 256 && ndof > 0) {
+                A->posmap.alloc(v->dof_eidx.n * static_cast<size_t>(K));
+                k_matrix_posmap<32, 256><<<nblocks(ndof, BS / 32), BS, 0, nullptr>>>(
+                    ndof, nne, nd, v->conn.p, v->dofs.p, v->dof_eptr.p, v->dof_eidx.p, A->rowptr.p, A->col.p,
+                    A->posmap.p);
+                check_launch();
+                GF_HIP(hipStreamSynchronize(nullptr));
+            }
         }
         catch (...) {
             delete A;
@@ -461,14 +479,15 @@ int gf_matrix_assemble(gf_matrix A, const double* elemmat, void* stream)
         hipStream_t s = S(stream);
         const int K = v->nne * v->ndim;
         if (A->nrows) {
-            if (!A->degenerate && K <= 8 && A->maxrow <= 64)
+            if (A->posmap.n && K <= 8 && A->maxrow <= 64)
                 k_matrix_assemble<8, 64><<<nblocks(A->nrows, BS / 8), BS, 0, s>>>(
-                    A->nrows, v->nne, v->ndim, v->conn.p, v->dofs.p, v->dof_eptr.p, v->dof_eidx.p, A->rowptr.p,
-                    A->col.p, elemmat, A->val.p);
-            else if (!A->degenerate && K <= 32 && A->maxrow <= 128)
+                    A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
+            else if (A->posmap.n && A->maxrow <= 128)
                 k_matrix_assemble<32, 128><<<nblocks(A->nrows, BS / 32), BS, 0, s>>>(
-                    A->nrows, v->nne, v->ndim, v->conn.p, v->dofs.p, v->dof_eptr.p, v->dof_eidx.p, A->rowptr.p,
-                    A->col.p, elemmat, A->val.p);
+                    A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
+            else if (A->posmap.n)
+                k_matrix_assemble<32, 256><<<nblocks(A->nrows, BS / 32), BS, 0, s>>>(
+                    A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
             else
                 k_matrix_assemble_serial<<<nblocks(A->nrows, BS), BS, 0, s>>>(
                     A->nrows, v->nne, v->ndim, v->conn.p, v->dofs.p, v->dof_eptr.p, v->dof_eidx.p, A->rowptr.p,

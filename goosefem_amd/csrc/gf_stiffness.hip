@@ -31,6 +31,8 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 
 constexpr int WAVES = 4;           // wavefronts per workgroup
 constexpr int NQ = 8, NN = 8;      // quadrature points, nodes
@@ -48,6 +50,22 @@ __device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo)
 {
     hi = static_cast<__bf16>(x);
     lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+// hi/lo split of eight values, converted in pairs (one packed conversion per pair and part)
+__device__ inline void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo)
+{
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const floatx2 v = {x[2 * p], x[2 * p + 1]};
+        const bf16x2 h = __builtin_convertvector(v, bf16x2);
+        const floatx2 r = v - __builtin_convertvector(h, floatx2);
+        const bf16x2 l = __builtin_convertvector(r, bf16x2);
+        hi[2 * p] = h[0];
+        hi[2 * p + 1] = h[1];
+        lo[2 * p] = l[0];
+        lo[2 * p + 1] = l[1];
+    }
 }
 
 // one element's C, dNx, dV in registers (16-byte loads), so that the next element's loads are in
@@ -118,35 +136,47 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // MFMA: lane supplies A row (l % 32) and B column (l % 32), contraction slots 8*(l/32)..+7.
-        // Row (m, j) of A and column (n, k) of T share the decomposition row = 3 m + j = 3 n + k;
-        // each lane evaluates its own T entries T[kk][(n,k)] = dV_q sum_l C_q(i, j2, k, l) dNx(q,n,l)
-        // straight from the staged C and dNx (kk = q*9 + j2*3 + i).
-        const int row = lane % 32, half = lane / 32;
-        const int m = row / 3, j = row - (row / 3) * 3;
+        // MFMA: lane l supplies A row (l % 32) and B column (l % 32) at contraction slots
+        // 8 (l / 32) + s of each 16-wide chunk.  The contraction index kk = (q, j2, i) is laid out so
+        // that each half-wave h owns quadrature points 4h..4h+3: slot s of chunk ch is
+        // t = 8 ch + s, q = 4h + t / 9, (j2, i) = t % 9 (t < 36; the last 4 slots are zero).  Every
+        // index below is then a compile-time constant plus a per-lane base.  Row (m, j) of A and
+        // column (n, k) of T share the decomposition 3 m + j = 3 n + k, so both operands use
+        // dNx(q, m, :):  A = delta(j, j2) dNx(q, m, i),  T = sum_l C_q(i, j2, j, l) dV_q dNx(q, m, l).
+        const int row = lane % 32, h = lane / 32;
+        const int j = row % 3, m = row < 24 ? row / 3 : 7;
+        const bool live = row < 24; // rows 24..31 of the 32 x 32 tile are padding
+        const float* Cb = L.C + h * 4 * 81 + j * 3;
+        const float* gb = L.g + (h * 4 * NN + m) * 3;
+        const float* vb = L.vol + h * 4;
         floatx16 acc = {};
-#pragma unroll 1
+        float gq[3] = {0.0f, 0.0f, 0.0f}, gv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
         for (int ch = 0; ch < KKP / 16; ++ch) {
-            bf16x8 ahi, alo, bhi, blo;
+            float av[8], bv[8];
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
-                const int kk = ch * 16 + half * 8 + s;
-                float av = 0.0f, bv = 0.0f;
-                if (row < 24 && kk < KK) {
-                    const int q = kk / 9, r = kk - q * 9, j2 = r / 3, i = r - j2 * 3;
-                    av = (j2 == j) ? L.g[(q * NN + m) * 3 + i] : 0.0f;
-                    const float* Cq = L.C + q * 81 + ((i * 3 + j2) * 3 + j) * 3;
-                    const float* gq = L.g + (q * NN + m) * 3;
-                    bv = (Cq[0] * gq[0] + Cq[1] * gq[1] + Cq[2] * gq[2]) * L.vol[q];
+                const int t = ch * 8 + s;
+                av[s] = 0.0f;
+                bv[s] = 0.0f;
+                if (t < 36) {
+                    const int ql = t / 9, r = t % 9, j2 = r / 3, i = r % 3;
+                    if (r == 0) {
+                        const float v = vb[ql];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            gq[c] = live ? gb[ql * 24 + c] : 0.0f;
+                            gv[c] = gq[c] * v;
+                        }
+                    }
+                    const float* Cq = Cb + ql * 81 + (i * 3 + j2) * 9;
+                    bv[s] = fmaf(Cq[0], gv[0], fmaf(Cq[1], gv[1], Cq[2] * gv[2]));
+                    av[s] = j == j2 ? gq[i] : 0.0f;
                 }
-                __bf16 h, l;
-                split_bf16(av, h, l);
-                ahi[s] = h;
-                alo[s] = l;
-                split_bf16(bv, h, l);
-                bhi[s] = h;
-                blo[s] = l;
             }
+            bf16x8 ahi, alo, bhi, blo;
+            split8(av, ahi, alo);
+            split8(bv, bhi, blo);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
@@ -156,7 +186,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, 
         double* Ke = elemmat + e * 576;
 #pragma unroll
         for (int r = 0; r < 12; ++r) {
-            const int i = 8 * (r / 4) + 4 * half + (r % 4);
+            const int i = 8 * (r / 4) + 4 * h + (r % 4);
             if (row < 24) Ke[i * 24 + row] = static_cast<double>(acc[r]);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -228,30 +258,44 @@ __global__ void k_elemmat_dot(int64_t nrows, int K, const double* __restrict__ K
     out[t] = s;
 }
 
-// the same product for K = 8 or 24: each wave streams 64 rows (64 K contiguous doubles) with
-// coalesced loads, multiplies by the gathered element vector, stages the products in LDS and sums
-// each row in column order (the order of k_elemmat_dot, so the results are identical)
+// the same product for K = 8 or 24: each wave streams 64 rows (64 K contiguous doubles), issuing
+// all of its 16-byte loads before using any, multiplies by the element vector, stages the products
+// in LDS and sums each row in column order (the order of k_elemmat_dot, so the results are
+// identical).  Ke and ue must be 16-byte aligned.
 template <int K>
 __global__ void __launch_bounds__(256) k_elemmat_dot_rows(int64_t nrows, const double* __restrict__ Ke,
                                                          const double* __restrict__ ue, double* __restrict__ out)
 {
+    static_assert(K % 2 == 0, "pairs of columns");
     constexpr int RP = K + 1; // LDS row pitch
+    constexpr int NV = K / 2; // 16-byte loads per lane
     __shared__ double sh[4][64 * RP];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * 64;
     if (r0 >= nrows) return; // uniform per wave
     const int64_t base = r0 * K, total = nrows * K;
-#pragma unroll 4
-    for (int k = 0; k < K; ++k) {
-        const int loc = k * 64 + l;
+    const double2* K2 = reinterpret_cast<const double2*>(Ke + base);
+    double2 kv[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int loc = 2 * (k * 64 + l);
+        kv[k] = base + loc < total ? K2[k * 64 + l] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int loc = 2 * (k * 64 + l); // even, so loc and loc + 1 lie in one row (K even)
         const int64_t f = base + loc;
-        double p = 0.0;
+        double p0 = 0.0, p1 = 0.0;
         if (f < total) {
             const int64_t e = f / (K * K);
             const int c = static_cast<int>(f % K);
-            p = Ke[f] * ue[e * K + c];
+            const double2 u = *reinterpret_cast<const double2*>(ue + e * K + c);
+            p0 = kv[k].x * u.x;
+            p1 = kv[k].y * u.y;
         }
-        sh[w][(loc / K) * RP + loc % K] = p;
+        double* d = sh[w] + (loc / K) * RP + loc % K;
+        d[0] = p0;
+        d[1] = p1;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -279,7 +323,7 @@ int stiffness_bf16x3(gf_quad q, const double* C, double* elemmat, hipStream_t s)
 {
     return guard([&] {
         GF_REQUIRE(quad_etype(q) == GF_HEX8 && quad_nip(q) == NQ, "GF_BF16X3 needs Hex8 with 8 points");
-        GF_REQUIRE(reinterpret_cast<uintptr_t>(C) % 16 == 0 && reinterpret_cast<uintptr_t>(elemmat) % 16 == 0,
+        GF_REQUIRE(aligned16(C) && aligned16(elemmat),
                    "GF_BF16X3 needs 16-byte aligned qtensor4 and elemmat arrays");
         const int64_t ne = quad_nelem(q);
         if (ne == 0) return;
@@ -309,9 +353,10 @@ extern "C" int gf_elemmat_dot(int64_t nelem, int K, const double* elemmat, const
         const int64_t n = nelem * K;
         if (n == 0) return;
         const unsigned gw = static_cast<unsigned>((n + 255) / 256); // 4 waves x 64 rows per block
-        if (K == 24)
+        const bool a16 = aligned16(elemmat) && aligned16(elemvec);
+        if (K == 24 && a16)
             k_elemmat_dot_rows<24><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
-        else if (K == 8)
+        else if (K == 8 && a16)
             k_elemmat_dot_rows<8><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
         else
             k_elemmat_dot<<<nblocks(n, 256), 256, 0, S(stream)>>>(n, K, elemmat, elemvec, out);

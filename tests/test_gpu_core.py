@@ -209,6 +209,24 @@ def test_matrix_csr_bit_exact(gf, et, periodic):
     assert np.array_equal(host(A.Todense()), dense)
 
 
+@pytest.mark.parametrize("nfan", [20, 40, 70])
+def test_matrix_csr_long_rows(gf, nfan):
+    # a fan of Quad4 elements around hub node 0: the hub's rows hold 2 (3 nfan + 1) columns, which
+    # exercises the 128- and 256-entry row capacities and (nfan 70: 422 columns) the row-serial path
+    conn = np.array([[0, 1 + 3 * i, 2 + 3 * i, 3 + 3 * i] for i in range(nfan)], dtype=np.int64)
+    dofs = oracle.dofs(3 * nfan + 1, 2)
+    rng = np.random.default_rng(nfan)
+    a = rng.standard_normal((nfan, 8, 8))
+    rowptr, col, val = oracle.matrix_assemble(conn, dofs, a)
+    assert rowptr[1] - rowptr[0] == 2 * (3 * nfan + 1)
+    A = gf.Matrix(conn, dofs)
+    A.assemble(a)
+    grp, gcol, gval = (host(t) for t in A.csr())
+    assert np.array_equal(grp, rowptr) and np.array_equal(gcol, col) and np.array_equal(gval, val)
+    x = rng.standard_normal(rowptr.size - 1)
+    assert np.array_equal(host(A.Dot(x)), oracle.csr_dot(rowptr, col, val, x))
+
+
 @pytest.mark.parametrize("et", [QUAD4, HEX8])
 @pytest.mark.parametrize("periodic", [False, True])
 def test_single_element_mesh(gf, et, periodic):
