@@ -122,7 +122,10 @@ __global__ void __launch_bounds__(BS) k_matrix_posmap(int64_t nrows, int nne, in
 
 // Assembly from the position map.  L lanes per row (K <= L), CAP >= the longest row.  Lane s adds
 // column s of each contribution; the values and positions of up to 8 contributions are loaded
-// together, then added to the row's LDS accumulators strictly in contribution order.
+// together, then added to the row's LDS accumulators strictly in contribution order.  Loads and
+// adds are unconditional: indices past the row's end are clamped to its last contribution and
+// their values go to a spare accumulator (slot CAP) that is never stored.  Index products are
+// 32 x 32 -> 64-bit: contribution indices and elemvec positions are below 2^31.
 template <int L, int CAP>
 __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int K, const int32_t* __restrict__ cptr,
                                                         const int32_t* __restrict__ cidx,
@@ -131,7 +134,7 @@ __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int K, co
                                                         const double* __restrict__ elemmat, double* __restrict__ val)
 {
     constexpr int R = BS / L;
-    __shared__ double s_acc[R][CAP];
+    __shared__ double s_acc[R][CAP + 1];
     const int g = threadIdx.x / L, lane = threadIdx.x % L;
     const int64_t row = static_cast<int64_t>(blockIdx.x) * R + g;
     const bool active = row < nrows;
@@ -143,23 +146,24 @@ __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int K, co
     }
     __syncthreads();
     if (active && lane < K) {
+        const uint32_t uK = static_cast<uint32_t>(K);
         const int32_t cend = cptr[row + 1];
         for (int32_t c0 = cptr[row]; c0 < cend; c0 += 8) {
+            uint32_t cc[8], kk[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) cc[j] = static_cast<uint32_t>(min(c0 + j, cend - 1));
+#pragma unroll
+            for (int j = 0; j < 8; ++j) kk[j] = static_cast<uint32_t>(cidx[cc[j]]);
             int pos[8];
             double pv[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                pos[j] = -1;
-                pv[j] = 0.0;
-                if (c0 + j < cend) {
-                    const int64_t c = c0 + j;
-                    pos[j] = posmap[c * K + lane];
-                    pv[j] = elemmat[static_cast<int64_t>(cidx[c]) * K + lane];
-                }
+                pos[j] = c0 + j < cend ? static_cast<int>(posmap[static_cast<uint64_t>(cc[j]) * uK + lane]) : CAP;
+                pv[j] = elemmat[static_cast<uint64_t>(kk[j]) * uK + lane];
             }
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                if (pos[j] >= 0) s_acc[g][pos[j]] += pv[j];
+                s_acc[g][pos[j]] += pv[j];
                 // contributions of one row are added strictly in order: make this contribution's
                 // LDS writes visible to the next one's reads across the row's lanes
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");

@@ -78,22 +78,18 @@ struct Stage {
     double v;
 };
 
+// Unconditional loads (lanes past the end of an array re-read its last entry, which store_stage
+// ignores), so that no branch hides them from the compiler's wait counting.
 __device__ inline void load_stage(Stage& st, const double* __restrict__ C, const double* __restrict__ dNx,
                                   const double* __restrict__ vol, int64_t e, int lane)
 {
     const double2* Ce = reinterpret_cast<const double2*>(C + e * NC);
     const double2* ge = reinterpret_cast<const double2*>(dNx + e * NG);
 #pragma unroll
-    for (int k = 0; k < RC; ++k) {
-        const int t = lane + 64 * k;
-        st.c[k] = t < NC2 ? Ce[t] : make_double2(0.0, 0.0);
-    }
+    for (int k = 0; k < RC; ++k) st.c[k] = Ce[min(lane + 64 * k, NC2 - 1)];
 #pragma unroll
-    for (int k = 0; k < RG; ++k) {
-        const int t = lane + 64 * k;
-        st.g[k] = t < NG2 ? ge[t] : make_double2(0.0, 0.0);
-    }
-    st.v = lane < NQ ? vol[e * NQ + lane] : 0.0;
+    for (int k = 0; k < RG; ++k) st.g[k] = ge[min(lane + 64 * k, NG2 - 1)];
+    st.v = vol[e * NQ + min(lane, NQ - 1)];
 }
 
 __device__ inline void store_stage(const Stage& st, WaveLDS& L, int lane)
@@ -117,6 +113,84 @@ __device__ inline void store_stage(const Stage& st, WaveLDS& L, int lane)
     if (lane < NQ) L.vol[lane] = static_cast<float>(st.v);
 }
 
+// The 32 x 32 FP32 tile whose top-left 24 x 24 block is K_e of the element staged in L.
+__device__ __forceinline__ floatx16 element_tile(const WaveLDS& L, int lane)
+{
+    // MFMA: lane l supplies A row (l % 32) and B column (l % 32) at contraction slots
+    // 8 (l / 32) + s of each 16-wide chunk.  The contraction index kk = (q, j2, i) is laid out so
+    // that each half-wave h owns quadrature points 4h..4h+3: slot s of chunk ch is
+    // t = 8 ch + s, q = 4h + t / 9, (j2, i) = t % 9 (t < 36; the last 4 slots are zero).  Every
+    // index below is then a compile-time constant plus a per-lane base.  Row (m, j) of A and
+    // column (n, k) of T share the decomposition 3 m + j = 3 n + k, so both operands use
+    // dNx(q, m, :):  A = delta(j, j2) dNx(q, m, i),  T = sum_l C_q(i, j2, j, l) dV_q dNx(q, m, l).
+    const int row = lane % 32, h = lane / 32;
+    const int j = row % 3, m = row < 24 ? row / 3 : 7;
+    const bool live = row < 24; // rows 24..31 of the 32 x 32 tile are padding
+    const float* Cb = L.C + h * 4 * 81 + j * 3;
+    const float* gb = L.g + (h * 4 * NN + m) * 3;
+    const float* vb = L.vol + h * 4;
+    floatx16 acc = {};
+    float gq[3] = {0.0f, 0.0f, 0.0f}, gv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ch = 0; ch < KKP / 16; ++ch) {
+        float av[8], bv[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int t = ch * 8 + s;
+            av[s] = 0.0f;
+            bv[s] = 0.0f;
+            if (t < 36) {
+                const int ql = t / 9, r = t % 9, j2 = r / 3, i = r % 3;
+                if (r == 0) {
+                    const float v = vb[ql];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        gq[c] = live ? gb[ql * 24 + c] : 0.0f;
+                        gv[c] = gq[c] * v;
+                    }
+                }
+                const float* Cq = Cb + ql * 81 + (i * 3 + j2) * 9;
+                bv[s] = fmaf(Cq[0], gv[0], fmaf(Cq[1], gv[1], Cq[2] * gv[2]));
+                av[s] = j == j2 ? gq[i] : 0.0f;
+            }
+        }
+        bf16x8 ahi, alo, bhi, blo;
+        split8(av, ahi, alo);
+        split8(bv, bhi, blo);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Store K_e: the tile's 24 x 24 block goes through LDS (into the staged C, which is no longer read)
+// so that each lane writes 9 doubles with full-wave contiguous stores and no lane is masked off.
+// Accumulator layout: value r of lane l is K[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+__device__ __forceinline__ void store_tile(WaveLDS& L, const floatx16& acc, double* __restrict__ Ke, int lane)
+{
+    const int row = lane % 32, h = lane / 32;
+    float* Ks = L.C;
+    wave_sync();
+    if (row < 24) {
+#pragma unroll
+        for (int r = 0; r < 12; ++r) Ks[(8 * (r / 4) + 4 * h + (r % 4)) * 24 + row] = acc[r];
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Ke[lane + 64 * k] = static_cast<double>(Ks[lane + 64 * k]);
+}
+
+// One wavefront per element, grid-stride.  The loop is rotated (store K_e of element e, then
+// wait for element e + stride's loads, which were issued before those stores) so that waiting for
+// the next element's data never waits for the previous element's stores.
 __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, const double* __restrict__ C,
                                                                   const double* __restrict__ dNx,
                                                                   const double* __restrict__ vol,
@@ -126,72 +200,23 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, 
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     WaveLDS& L = lds[w];
     const int64_t stride = static_cast<int64_t>(gridDim.x) * WAVES;
-    const int64_t first = static_cast<int64_t>(blockIdx.x) * WAVES + w;
+    int64_t e = static_cast<int64_t>(blockIdx.x) * WAVES + w;
+    if (e >= nelem) return; // uniform per wave; the kernel has no workgroup barrier
     Stage st;
-    if (first < nelem) load_stage(st, C, dNx, vol, first, lane);
-    for (int64_t e = first; e < nelem; e += stride) {
-        // stage C, dNx, dV of this element in LDS (FP32), then start the next element's loads
+    load_stage(st, C, dNx, vol, e, lane);
+    store_stage(st, L, lane);
+    wave_sync();
+    load_stage(st, C, dNx, vol, min(e + stride, nelem - 1), lane);
+    floatx16 acc = element_tile(L, lane);
+    for (;;) {
+        store_tile(L, acc, elemmat + e * 576, lane);
+        e += stride;
+        if (e >= nelem) break;
+        wave_sync();
         store_stage(st, L, lane);
-        if (e + stride < nelem) load_stage(st, C, dNx, vol, e + stride, lane);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // MFMA: lane l supplies A row (l % 32) and B column (l % 32) at contraction slots
-        // 8 (l / 32) + s of each 16-wide chunk.  The contraction index kk = (q, j2, i) is laid out so
-        // that each half-wave h owns quadrature points 4h..4h+3: slot s of chunk ch is
-        // t = 8 ch + s, q = 4h + t / 9, (j2, i) = t % 9 (t < 36; the last 4 slots are zero).  Every
-        // index below is then a compile-time constant plus a per-lane base.  Row (m, j) of A and
-        // column (n, k) of T share the decomposition 3 m + j = 3 n + k, so both operands use
-        // dNx(q, m, :):  A = delta(j, j2) dNx(q, m, i),  T = sum_l C_q(i, j2, j, l) dV_q dNx(q, m, l).
-        const int row = lane % 32, h = lane / 32;
-        const int j = row % 3, m = row < 24 ? row / 3 : 7;
-        const bool live = row < 24; // rows 24..31 of the 32 x 32 tile are padding
-        const float* Cb = L.C + h * 4 * 81 + j * 3;
-        const float* gb = L.g + (h * 4 * NN + m) * 3;
-        const float* vb = L.vol + h * 4;
-        floatx16 acc = {};
-        float gq[3] = {0.0f, 0.0f, 0.0f}, gv[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int ch = 0; ch < KKP / 16; ++ch) {
-            float av[8], bv[8];
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const int t = ch * 8 + s;
-                av[s] = 0.0f;
-                bv[s] = 0.0f;
-                if (t < 36) {
-                    const int ql = t / 9, r = t % 9, j2 = r / 3, i = r % 3;
-                    if (r == 0) {
-                        const float v = vb[ql];
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            gq[c] = live ? gb[ql * 24 + c] : 0.0f;
-                            gv[c] = gq[c] * v;
-                        }
-                    }
-                    const float* Cq = Cb + ql * 81 + (i * 3 + j2) * 9;
-                    bv[s] = fmaf(Cq[0], gv[0], fmaf(Cq[1], gv[1], Cq[2] * gv[2]));
-                    av[s] = j == j2 ? gq[i] : 0.0f;
-                }
-            }
-            bf16x8 ahi, alo, bhi, blo;
-            split8(av, ahi, alo);
-            split8(bv, bhi, blo);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
-        }
-        // accumulator layout: value r of lane l is K[8*(r/4) + 4*(l/32) + r%4][l % 32]; each store
-        // instruction writes two contiguous 192-byte rows
-        double* Ke = elemmat + e * 576;
-#pragma unroll
-        for (int r = 0; r < 12; ++r) {
-            const int i = 8 * (r / 4) + 4 * h + (r % 4);
-            if (row < 24) Ke[i * 24 + row] = static_cast<double>(acc[r]);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
+        load_stage(st, C, dNx, vol, min(e + stride, nelem - 1), lane);
+        acc = element_tile(L, lane);
     }
 }
 
