@@ -578,6 +578,8 @@ def train_main(argv=None):
     ap.add_argument("--seed0", type=int, default=1000)
     ap.add_argument("--time-limit", type=float, default=None, help="seconds of training per configuration at most")
     ap.add_argument("--out", required=True, help="weights file (.npz); with several configurations a suffix is added")
+    ap.add_argument("--init", default=None,
+                    help="continue training these weights (one configuration; their target scale is kept)")
     a = ap.parse_args(argv)
     pb = periodic_problem(a.n, dim=a.dim, spheres=a.dim == 3)
     shear = "pure" if a.dim == 3 else "simple"
@@ -590,12 +592,19 @@ def train_main(argv=None):
     seeds = list(range(a.seed0, a.seed0 + a.samples + a.holdout))
     Gs, T, its, t_data = probe.make_dataset(pb.K, seeds, tol=a.data_tol, progress=50 if a.dim == 2 else 10,
                                             field=field)
-    scale = float(T[: a.samples].std().item())
+    init = WarmStartNet.load(a.init) if a.init else None
+    if init is not None:
+        cfgs = [tuple(init.ch)]
+        scale = float(init.meta["target_scale"])
+    else:
+        scale = float(T[: a.samples].std().item())
     T.div_(scale)
     print(f"data: {len(seeds)} solves, {its} CG iterations, {t_data:.1f} s; target scale {scale:.4e}", flush=True)
     Gtr, Ttr, Gho, Tho = Gs[: a.samples], T[: a.samples], Gs[a.samples:], T[a.samples:]
     for cfg, lr in zip(cfgs, lrs):
-        net = WarmStartNet(cfg, dim=a.dim)
+        net = init if init is not None else WarmStartNet(cfg, dim=a.dim)
+        prev_epochs = int(net.meta.get("epochs", 0)) if init is not None else 0
+        prev_seconds = float(net.meta.get("train_seconds", 0.0)) if init is not None else 0.0
         W = PeriodicWarmStart(pb.P, pb.ty, a.n, net, sigma=pb.sigma, target_scale=scale, dim=a.dim, shear=shear)
         t0 = time.perf_counter()
         hist = W.train(Gtr, Ttr, a.epochs, batch=a.batch, lr=lr, time_limit=a.time_limit)
@@ -603,8 +612,9 @@ def train_main(argv=None):
         t_train = time.perf_counter() - t0
         err = W.holdout_error(Gho, Tho) if a.holdout else float("nan")
         net.meta = {"target_scale": scale, "sigma": pb.sigma, "n": a.n, "dim": a.dim, "shear": shear,
-                    "samples": a.samples, "epochs": len(hist),
-                    "batch": a.batch, "lr": lr, "data_seconds": t_data, "train_seconds": t_train, "final_mse": hist[-1],
+                    "samples": a.samples, "epochs": prev_epochs + len(hist),
+                    "batch": a.batch, "lr": lr, "data_seconds": t_data, "train_seconds": prev_seconds + t_train,
+                    "final_mse": hist[-1],
                     "holdout_rel_error": err, "data_tol": a.data_tol, "seed0": a.seed0, "params": net.nparams()}
         out = a.out
         if len(cfgs) > 1:
