@@ -246,7 +246,9 @@ int gf_diag_solve_nodevec(gf_diag d, const double* b, double* x, void* stream);
  * reference's triplet order (ascending e), so FP64 values match the reference's duplicate sum.
  * ============================================================================================== */
 
-/* Matrix(conn, dofs)  (Matrix.h:679-693): builds the sparsity pattern and gather lists */
+/* Matrix(conn, dofs)  (Matrix.h:679-693): builds the sparsity pattern and gather lists, and (rows
+ * of at most 256 entries, elements of at most 32 DOFs without a repeated DOF) a device map of one
+ * byte per element-matrix entry: the entry's position in its CSR row (nelem * K * K bytes). */
 int gf_matrix_create(gf_matrix* out, gf_vector v);
 int gf_matrix_destroy(gf_matrix A);
 int gf_matrix_sizes(gf_matrix A, int64_t* nrows, int64_t* nnz);
