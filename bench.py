@@ -141,15 +141,21 @@ def bench_hex8(args):
     Gq = (0.5 + 0.5 * (1.0 + torch.erf(z / np.sqrt(2.0)))).reshape(E, 8)  # U[0.5, 1.5]
     Kq = torch.full((E, 8), 10.0, dtype=torch.float64, device="cuda")
     C = gf.elastic_tangent(Kq, Gq)
-    del z, Gq, Kq
+    C32 = gf.elastic_tangent(Kq, Gq, dtype=torch.float32)
     k64 = torch.empty((E, 24, 24), dtype=torch.float64, device="cuda")
     kbf = torch.empty_like(k64)
+    k32 = torch.empty((E, 24, 24), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
+    t_tan = _timed(lambda: gf.elastic_tangent(Kq, Gq), 3)
+    t_tan32 = _timed(lambda: gf.elastic_tangent(Kq, Gq, dtype=torch.float32), 3)
+    del z, Gq, Kq
     t64 = _timed(lambda: quad.Int_gradN_dot_tensor4_dot_gradNT_dV(C, gf.F64, out=k64), 3)
     tbf = _timed(lambda: quad.Int_gradN_dot_tensor4_dot_gradNT_dV(C, gf.BF16X3, out=kbf), 5)
+    t32 = _timed(lambda: quad.Int_gradN_dot_tensor4_dot_gradNT_dV(C32, gf.BF16X3, out=k32), 5)
     err_ke = float(torch.linalg.vector_norm(kbf - k64) / torch.linalg.vector_norm(k64))
-    del C
+    err_ke32 = float(torch.linalg.vector_norm(k32.double() - k64) / torch.linalg.vector_norm(k64))
+    del C, C32
     t0 = time.perf_counter()
     A = gf.Matrix(vec, None)
     t_pattern = time.perf_counter() - t0
@@ -164,6 +170,11 @@ def bench_hex8(args):
     t_mf = _timed(lambda: matrix_free(k64), 3)
     err_mf = float(torch.linalg.vector_norm(matrix_free(k64) - Ku) / torch.linalg.vector_norm(Ku))
     err_ku_bf = float(torch.linalg.vector_norm(matrix_free(kbf) - Ku) / torch.linalg.vector_norm(Ku))
+    # the same products from the FP32 element matrices (FP64 sums)
+    t_mf32 = _timed(lambda: matrix_free(k32), 3)
+    err_ku32 = float(torch.linalg.vector_norm(matrix_free(k32) - Ku) / torch.linalg.vector_norm(Ku))
+    t_asm32 = _timed(lambda: A.assemble(k32), 3)
+    A.assemble(k64)
     bytes_el = 11392  # FP64 API bytes per element: C 5184 + GradN 1536 + dV 64 + K_e 4608
     flops_el = 69120  # Voigt B^T (D dV) B, 8 points
     ach = bytes_el * E / tbf / 1e9
@@ -179,6 +190,15 @@ def bench_hex8(args):
                      "algorithmic_bytes_per_launch": bytes_el * E, "avg_launch_us": tbf * 1e6,
                      "tflops_voigt": flops_el * E / tbf / 1e12},
         "fp64_stiffness": {"avg_launch_us": t64 * 1e6, "achieved_GBs": bytes_el * E / t64 / 1e9},
+        # FP32 tangent in, FP32 K_e out (the kernel's own operand and accumulator precision):
+        # C 2592 + GradN 1536 + dV 64 + K_e 2304 = 6496 B per element
+        "fp32_io": {"avg_launch_us": t32 * 1e6, "elements_per_s": E / t32,
+                    "achieved_GBs": 6496 * E / t32 / 1e9, "frac": 6496 * E / t32 / 1e9 / HBM_PEAK_GBS,
+                    "Ke_rel_vs_fp64": err_ke32, "assemble_ms": t_asm32 * 1e3,
+                    "assemble_GBs": (E * 2304 + A.nnz * 8) / t_asm32 / 1e9, "matrix_free_ms": t_mf32 * 1e3,
+                    "Ku_rel_vs_fp64": err_ku32},
+        "tangent_ms": {"f64": t_tan * 1e3, "f32": t_tan32 * 1e3,
+                       "f64_GBs": E * 8 * 81 * 8 / t_tan / 1e9, "f32_GBs": E * 8 * 81 * 4 / t_tan32 / 1e9},
         "error_vs_fp64": {"Ke_rel": err_ke, "Ku_rel": err_ku_bf},
         "csr": {"pattern_build_s": t_pattern, "assemble_ms": t_asm * 1e3,
                 "assemble_GBs": (E * 4608 + A.nnz * 8) / t_asm / 1e9,

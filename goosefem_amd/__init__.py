@@ -119,6 +119,14 @@ def to_device(a, dtype=None):
     return t.to(device=device(), dtype=dtype).contiguous()
 
 
+def _is_f32(a):
+    """True for an FP32 numpy array or torch tensor."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        return a.dtype == torch.float32
+    return isinstance(a, np.ndarray) and a.dtype == np.float32
+
+
 def _dp(t):
     """Device pointer of a contiguous torch tensor on the GPU."""
     if t is None:
@@ -586,8 +594,17 @@ class _Quadrature(_Handle):
         return out
 
     def Int_gradN_dot_tensor4_dot_gradNT_dV(self, qtensor, precision=F64, out=None):
-        qt = to_device(qtensor)
+        """Element stiffness.  With precision BF16X3 an FP32 qtensor gives an FP32 elemmat (the MFMA
+        kernel's operands and accumulation are FP32); otherwise input and output are FP64."""
+        torch = _torch()
         K = self.nne * self.ndim
+        if precision == BF16X3 and _is_f32(qtensor):
+            qt = to_device(qtensor, torch.float32)
+            if out is None:
+                out = _empty((self.nelem, K, K), torch.float32)
+            _call("gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV_f32", self._h, _dp(qt), _dp(out), _stream())
+            return out
+        qt = to_device(qtensor)
         if out is None:
             out = _empty((self.nelem, K, K))
         _call("gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV", self._h, _dp(qt), _dp(out), _i32(precision), _stream())
@@ -668,12 +685,15 @@ Element = SimpleNamespace(
 
 
 def elemmat_dot(elemmat, elemvec):
-    """out[e] = elemmat[e] . elemvec[e] for every element (the element part of sum_e K_e u_e)."""
-    Ke = to_device(elemmat)
+    """out[e] = elemmat[e] . elemvec[e] for every element (the element part of sum_e K_e u_e); an
+    FP32 elemmat is widened entry by entry, products and sums are FP64."""
+    f32 = _is_f32(elemmat)
+    Ke = to_device(elemmat, _torch().float32 if f32 else None)
     ue = to_device(elemvec)
     nelem, K = Ke.shape[0], Ke.shape[1]
     out = _empty(tuple(ue.shape))
-    _call("gf_elemmat_dot", _i64(nelem), _i32(K), _dp(Ke), _dp(ue), _dp(out), _stream())
+    _call("gf_elemmat_dot_f32" if f32 else "gf_elemmat_dot", _i64(nelem), _i32(K), _dp(Ke), _dp(ue), _dp(out),
+          _stream())
     return out
 
 
@@ -693,13 +713,16 @@ def elastic_stress(eps, K, G):
     return out
 
 
-def elastic_tangent(K, G, td=3):
-    """C_ijkl = K I(x)I + 2 G (I_sym - I(x)I / 3) per point (FP64)."""
+def elastic_tangent(K, G, td=3, dtype=None):
+    """C_ijkl = K I(x)I + 2 G (I_sym - I(x)I / 3) per point, FP64 (default) or rounded to FP32
+    (dtype torch.float32)."""
     torch = _torch()
     K = to_device(K)
     G = to_device(torch.as_tensor(G).expand(K.shape) if not isinstance(G, torch.Tensor) else G.expand(K.shape))
-    out = _empty(tuple(K.shape) + (td,) * 4)
-    _call("gf_material_elastic_tangent", _i64(K.numel()), _i32(td), _dp(K), _dp(G), _dp(out), _stream())
+    f32 = dtype == torch.float32
+    out = _empty(tuple(K.shape) + (td,) * 4, torch.float32 if f32 else None)
+    _call("gf_material_elastic_tangent_f32" if f32 else "gf_material_elastic_tangent", _i64(K.numel()), _i32(td),
+          _dp(K), _dp(G), _dp(out), _stream())
     return out
 
 
@@ -770,8 +793,10 @@ class Matrix(_Handle):
         self.nrows, self.nnz = nr.value, nz.value
 
     def assemble(self, elemmat):
-        em = to_device(elemmat)
-        _call("gf_matrix_assemble", self._h, _dp(em), _stream())
+        """Matrix::assemble; FP32 element matrices are widened entry by entry and summed in FP64."""
+        f32 = _is_f32(elemmat)
+        em = to_device(elemmat, _torch().float32 if f32 else None)
+        _call("gf_matrix_assemble_f32" if f32 else "gf_matrix_assemble", self._h, _dp(em), _stream())
 
     def Dot(self, x, out=None):
         x = to_device(x)

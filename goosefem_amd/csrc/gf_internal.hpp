@@ -135,6 +135,7 @@ inline void parallel_for(int64_t n, F&& f)
 
 // Hex8 stiffness on MFMA (BF16 hi/lo split, FP32 accumulation), gf_stiffness.hip
 int stiffness_bf16x3(gf_quad q, const double* qtensor4, double* elemmat, hipStream_t s);
+int stiffness_bf16x3_f32(gf_quad q, const float* qtensor4, float* elemmat, hipStream_t s);
 int stiffness_f64_hex8(gf_quad q, const double* qtensor4, double* elemmat, hipStream_t s);
 
 // Narrow an int64 host index array to int32 (device form), checking the range.

@@ -126,12 +126,12 @@ __global__ void __launch_bounds__(BS) k_matrix_posmap(int64_t nrows, int nne, in
 // adds are unconditional: indices past the row's end are clamped to its last contribution and
 // their values go to a spare accumulator (slot CAP) that is never stored.  Index products are
 // 32 x 32 -> 64-bit: contribution indices and elemvec positions are below 2^31.
-template <int L, int CAP>
+template <int L, int CAP, class TE>
 __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int K, const int32_t* __restrict__ cptr,
                                                         const int32_t* __restrict__ cidx,
                                                         const uint8_t* __restrict__ posmap,
                                                         const int32_t* __restrict__ rowptr,
-                                                        const double* __restrict__ elemmat, double* __restrict__ val)
+                                                        const TE* __restrict__ elemmat, double* __restrict__ val)
 {
     constexpr int R = BS / L;
     __shared__ double s_acc[R][CAP + 1];
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int K, co
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 pos[j] = c0 + j < cend ? static_cast<int>(posmap[static_cast<uint64_t>(cc[j]) * uK + lane]) : CAP;
-                pv[j] = elemmat[static_cast<uint64_t>(kk[j]) * uK + lane];
+                pv[j] = static_cast<double>(elemmat[static_cast<uint64_t>(kk[j]) * uK + lane]);
             }
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -178,10 +178,11 @@ __global__ void __launch_bounds__(BS) k_matrix_assemble(int64_t nrows, int K, co
 }
 
 // Fallback: one thread per row, sequential (handles degenerate elements and long rows).
+template <class TE>
 __global__ void k_matrix_assemble_serial(int64_t nrows, int nne, int nd, const int32_t* __restrict__ conn,
                                          const int32_t* __restrict__ dofs, const int32_t* __restrict__ cptr,
                                          const int32_t* __restrict__ cidx, const int32_t* __restrict__ rowptr,
-                                         const int32_t* __restrict__ col, const double* __restrict__ elemmat,
+                                         const int32_t* __restrict__ col, const TE* __restrict__ elemmat,
                                          double* __restrict__ val)
 {
     int64_t row = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -196,7 +197,7 @@ __global__ void k_matrix_assemble_serial(int64_t nrows, int nne, int nd, const i
         for (int s = 0; s < K; ++s) {
             const int32_t cc = dofs[static_cast<int64_t>(conn[e * nne + s / nd]) * nd + s % nd];
             const int p = find_col(col + b, n, cc);
-            val[b + p] += elemmat[(e * K + r) * K + s];
+            val[b + p] += static_cast<double>(elemmat[(e * K + r) * K + s]);
         }
     }
 }
@@ -266,6 +267,34 @@ void spmv(gf_matrix_s* A, const double* x, double* y, hipStream_t s)
 namespace gf {
 void matrix_spmv(gf_matrix A, const double* x, double* y, hipStream_t s) { spmv(A, x, y, s); }
 } // namespace gf
+
+namespace {
+
+template <class TE>
+void matrix_assemble(gf_matrix A, const TE* elemmat, hipStream_t s)
+{
+    auto v = A->v;
+    const int K = v->nne * v->ndim;
+    if (A->nrows) {
+        if (A->posmap.n && K <= 8 && A->maxrow <= 64)
+            k_matrix_assemble<8, 64, TE><<<nblocks(A->nrows, BS / 8), BS, 0, s>>>(
+                A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
+        else if (A->posmap.n && A->maxrow <= 128)
+            k_matrix_assemble<32, 128, TE><<<nblocks(A->nrows, BS / 32), BS, 0, s>>>(
+                A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
+        else if (A->posmap.n)
+            k_matrix_assemble<32, 256, TE><<<nblocks(A->nrows, BS / 32), BS, 0, s>>>(
+                A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
+        else
+            k_matrix_assemble_serial<TE><<<nblocks(A->nrows, BS), BS, 0, s>>>(
+                A->nrows, v->nne, v->ndim, v->conn.p, v->dofs.p, v->dof_eptr.p, v->dof_eidx.p, A->rowptr.p,
+                A->col.p, elemmat, A->val.p);
+        check_launch();
+    }
+    A->assembled = true;
+}
+
+} // namespace
 
 extern "C" {
 
@@ -479,26 +508,15 @@ int gf_matrix_assemble(gf_matrix A, const double* elemmat, void* stream)
 {
     return guard([&] {
         GF_REQUIRE(A && elemmat, "null argument");
-        auto v = A->v;
-        hipStream_t s = S(stream);
-        const int K = v->nne * v->ndim;
-        if (A->nrows) {
-            if (A->posmap.n && K <= 8 && A->maxrow <= 64)
-                k_matrix_assemble<8, 64><<<nblocks(A->nrows, BS / 8), BS, 0, s>>>(
-                    A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
-            else if (A->posmap.n && A->maxrow <= 128)
-                k_matrix_assemble<32, 128><<<nblocks(A->nrows, BS / 32), BS, 0, s>>>(
-                    A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
-            else if (A->posmap.n)
-                k_matrix_assemble<32, 256><<<nblocks(A->nrows, BS / 32), BS, 0, s>>>(
-                    A->nrows, K, v->dof_eptr.p, v->dof_eidx.p, A->posmap.p, A->rowptr.p, elemmat, A->val.p);
-            else
-                k_matrix_assemble_serial<<<nblocks(A->nrows, BS), BS, 0, s>>>(
-                    A->nrows, v->nne, v->ndim, v->conn.p, v->dofs.p, v->dof_eptr.p, v->dof_eidx.p, A->rowptr.p,
-                    A->col.p, elemmat, A->val.p);
-            check_launch();
-        }
-        A->assembled = true;
+        matrix_assemble(A, elemmat, S(stream));
+    });
+}
+
+int gf_matrix_assemble_f32(gf_matrix A, const float* elemmat, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && elemmat, "null argument");
+        matrix_assemble(A, elemmat, S(stream));
     });
 }
 

@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(TX * TY * TZ) k_apply_periodic_h8(int n, int n
 
 // ---- element force from a full dofval, FP64, reference expression order -------------------------
 template <int ET>
-__global__ void k_elem_force(int64_t nelem, const int32_t* __restrict__ conn, const int32_t* __restrict__ dofs,
+__global__ void __launch_bounds__(128) k_elem_force(int64_t nelem, const int32_t* __restrict__ conn, const int32_t* __restrict__ dofs,
                              const double* __restrict__ dNx, const double* __restrict__ vol,
                              const double* __restrict__ Ke, const double* __restrict__ Ge,
                              const double* __restrict__ x, double* __restrict__ fe, const int32_t* __restrict__ state)
@@ -359,6 +359,7 @@ __global__ void k_elem_force(int64_t nelem, const int32_t* __restrict__ conn, co
     for (int m = 0; m < NNE; ++m)
 #pragma unroll
         for (int i = 0; i < ND; ++i) f[m][i] = 0.0;
+#pragma unroll 1
     for (int q = 0; q < NNE; ++q) {
         const double* g = dNx + (e * NNE + q) * NNE * ND;
         const double v = vol[e * NNE + q];

@@ -383,26 +383,28 @@ __global__ void k_elastic_stress(int64_t n, const T* __restrict__ eps, const T* 
         }
 }
 
-template <int TD>
-__global__ void k_elastic_tangent(int64_t n, const double* __restrict__ K, const double* __restrict__ G,
-                                  double* __restrict__ C)
+// Tangent, written as contiguous output: a workgroup owns TP consecutive points (TP TD^4
+// components) and its threads stride over the components, so every store instruction covers
+// contiguous memory.  Component c of a point is C(i, j, a, b) with c = ((i TD + j) TD + a) TD + b,
+// the same expression as the per-point loop of the material law; TO = float rounds the FP64
+// value once.
+constexpr int TP = 64;
+
+template <int TD, class TO>
+__global__ void __launch_bounds__(BS) k_elastic_tangent(int64_t n, const double* __restrict__ K,
+                                                        const double* __restrict__ G, TO* __restrict__ C)
 {
-    int64_t p = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (p >= n) return;
-    double k = K[p], g = G[p];
-    double* c = C + p * TD * TD * TD * TD;
-#pragma unroll
-    for (int i = 0; i < TD; ++i)
-#pragma unroll
-        for (int j = 0; j < TD; ++j)
-#pragma unroll
-            for (int a = 0; a < TD; ++a)
-#pragma unroll
-                for (int b = 0; b < TD; ++b) {
-                    double dij = i == j, dkl = a == b, dik = i == a, djl = j == b, dil = i == b, djk = j == a;
-                    c[((i * TD + j) * TD + a) * TD + b] =
-                        k * dij * dkl + 2.0 * g * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0);
-                }
+    constexpr int T4 = TD * TD * TD * TD;
+    const int64_t p0 = static_cast<int64_t>(blockIdx.x) * TP;
+    const int np = static_cast<int>(min<int64_t>(TP, n - p0));
+    TO* out = C + p0 * T4;
+    for (int t = threadIdx.x; t < np * T4; t += BS) {
+        const int pl = t / T4, c = t - pl * T4;
+        const int i = c / (TD * TD * TD), j = (c / (TD * TD)) % TD, a = (c / TD) % TD, b = c % TD;
+        const double k = K[p0 + pl], g = G[p0 + pl];
+        const double dij = i == j, dkl = a == b, dik = i == a, djl = j == b, dil = i == b, djk = j == a;
+        out[t] = static_cast<TO>(k * dij * dkl + 2.0 * g * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0));
+    }
 }
 
 void compute_geometry(gf_quad_s* q, hipStream_t s)
@@ -710,8 +712,20 @@ int gf_material_elastic_tangent(int64_t nq, int tdim, const double* K, const dou
         GF_REQUIRE(K && G && C, "null argument");
         GF_REQUIRE(tdim == 2 || tdim == 3, "tdim must be 2 or 3");
         if (nq == 0) return;
-        if (tdim == 3) k_elastic_tangent<3><<<nblocks(nq, BS), BS, 0, S(stream)>>>(nq, K, G, C);
-        else k_elastic_tangent<2><<<nblocks(nq, BS), BS, 0, S(stream)>>>(nq, K, G, C);
+        if (tdim == 3) k_elastic_tangent<3, double><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C);
+        else k_elastic_tangent<2, double><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C);
+        check_launch();
+    });
+}
+
+int gf_material_elastic_tangent_f32(int64_t nq, int tdim, const double* K, const double* G, float* C, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(K && G && C, "null argument");
+        GF_REQUIRE(tdim == 2 || tdim == 3, "tdim must be 2 or 3");
+        if (nq == 0) return;
+        if (tdim == 3) k_elastic_tangent<3, float><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C);
+        else k_elastic_tangent<2, float><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C);
         check_launch();
     });
 }

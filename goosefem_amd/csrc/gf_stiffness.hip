@@ -33,6 +33,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int WAVES = 4;           // wavefronts per workgroup
 constexpr int NQ = 8, NN = 8;      // quadrature points, nodes
@@ -40,7 +41,7 @@ constexpr int NC = NQ * 81;        // tensor4 doubles per element
 constexpr int NG = NQ * NN * 3;    // dNx doubles per element
 constexpr int KK = 72, KKP = 80;   // contraction length, padded to 5 x 16
 
-struct WaveLDS {
+struct alignas(16) WaveLDS {
     float C[NC];
     float g[NG];
     float vol[NQ];
@@ -69,39 +70,79 @@ __device__ inline void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo)
 }
 
 // one element's C, dNx, dV in registers (16-byte loads), so that the next element's loads are in
-// flight while the current one is computed
-constexpr int NC2 = NC / 2, NG2 = NG / 2;
-constexpr int RC = (NC2 + 63) / 64, RG = (NG2 + 63) / 64;
+// flight while the current one is computed.  C is FP64 (the reference's qtensor4) or FP32.
+// Loads are unconditional (lanes past the end of an array re-read its last entry, which
+// store_stage ignores), so that no branch hides them from the compiler's wait counting.
+constexpr int NG2 = NG / 2, RG = (NG2 + 63) / 64;
+
+template <class TC>
+struct StageC;
+
+template <>
+struct StageC<double> {
+    static constexpr int N = NC / 2, R = (N + 63) / 64; // double2 loads
+    double2 v[R];
+    __device__ void load(const double* __restrict__ C, int64_t e, int lane)
+    {
+        const double2* p = reinterpret_cast<const double2*>(C + e * NC);
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = p[min(lane + 64 * k, N - 1)];
+    }
+    __device__ void store(float* dst, int lane) const
+    {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int t = lane + 64 * k;
+            if (t < N) {
+                dst[2 * t] = static_cast<float>(v[k].x);
+                dst[2 * t + 1] = static_cast<float>(v[k].y);
+            }
+        }
+    }
+};
+
+template <>
+struct StageC<float> {
+    static constexpr int N = NC / 4, R = (N + 63) / 64; // 16-byte loads of 4 floats
+    floatx4 v[R];
+    __device__ void load(const float* __restrict__ C, int64_t e, int lane)
+    {
+        const floatx4* p = reinterpret_cast<const floatx4*>(C + e * NC);
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = p[min(lane + 64 * k, N - 1)];
+    }
+    __device__ void store(float* dst, int lane) const
+    {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int t = lane + 64 * k;
+            if (t < N) *reinterpret_cast<floatx4*>(dst + 4 * t) = v[k];
+        }
+    }
+};
+
+template <class TC>
 struct Stage {
-    double2 c[RC];
+    StageC<TC> c;
     double2 g[RG];
     double v;
 };
 
-// Unconditional loads (lanes past the end of an array re-read its last entry, which store_stage
-// ignores), so that no branch hides them from the compiler's wait counting.
-__device__ inline void load_stage(Stage& st, const double* __restrict__ C, const double* __restrict__ dNx,
+template <class TC>
+__device__ inline void load_stage(Stage<TC>& st, const TC* __restrict__ C, const double* __restrict__ dNx,
                                   const double* __restrict__ vol, int64_t e, int lane)
 {
-    const double2* Ce = reinterpret_cast<const double2*>(C + e * NC);
+    st.c.load(C, e, lane);
     const double2* ge = reinterpret_cast<const double2*>(dNx + e * NG);
-#pragma unroll
-    for (int k = 0; k < RC; ++k) st.c[k] = Ce[min(lane + 64 * k, NC2 - 1)];
 #pragma unroll
     for (int k = 0; k < RG; ++k) st.g[k] = ge[min(lane + 64 * k, NG2 - 1)];
     st.v = vol[e * NQ + min(lane, NQ - 1)];
 }
 
-__device__ inline void store_stage(const Stage& st, WaveLDS& L, int lane)
+template <class TC>
+__device__ inline void store_stage(const Stage<TC>& st, WaveLDS& L, int lane)
 {
-#pragma unroll
-    for (int k = 0; k < RC; ++k) {
-        const int t = lane + 64 * k;
-        if (t < NC2) {
-            L.C[2 * t] = static_cast<float>(st.c[k].x);
-            L.C[2 * t + 1] = static_cast<float>(st.c[k].y);
-        }
-    }
+    st.c.store(L.C, lane);
 #pragma unroll
     for (int k = 0; k < RG; ++k) {
         const int t = lane + 64 * k;
@@ -174,7 +215,8 @@ __device__ __forceinline__ void wave_sync()
 // Store K_e: the tile's 24 x 24 block goes through LDS (into the staged C, which is no longer read)
 // so that each lane writes 9 doubles with full-wave contiguous stores and no lane is masked off.
 // Accumulator layout: value r of lane l is K[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
-__device__ __forceinline__ void store_tile(WaveLDS& L, const floatx16& acc, double* __restrict__ Ke, int lane)
+template <class TK>
+__device__ __forceinline__ void store_tile(WaveLDS& L, const floatx16& acc, TK* __restrict__ Ke, int lane)
 {
     const int row = lane % 32, h = lane / 32;
     float* Ks = L.C;
@@ -185,16 +227,17 @@ __device__ __forceinline__ void store_tile(WaveLDS& L, const floatx16& acc, doub
     }
     wave_sync();
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Ke[lane + 64 * k] = static_cast<double>(Ks[lane + 64 * k]);
+    for (int k = 0; k < 9; ++k) Ke[lane + 64 * k] = static_cast<TK>(Ks[lane + 64 * k]);
 }
 
 // One wavefront per element, grid-stride.  The loop is rotated (store K_e of element e, then
 // wait for element e + stride's loads, which were issued before those stores) so that waiting for
 // the next element's data never waits for the previous element's stores.
-__global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, const double* __restrict__ C,
+template <class TC, class TK>
+__global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, const TC* __restrict__ C,
                                                                   const double* __restrict__ dNx,
                                                                   const double* __restrict__ vol,
-                                                                  double* __restrict__ elemmat)
+                                                                  TK* __restrict__ elemmat)
 {
     __shared__ WaveLDS lds[WAVES];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -202,7 +245,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, 
     const int64_t stride = static_cast<int64_t>(gridDim.x) * WAVES;
     int64_t e = static_cast<int64_t>(blockIdx.x) * WAVES + w;
     if (e >= nelem) return; // uniform per wave; the kernel has no workgroup barrier
-    Stage st;
+    Stage<TC> st;
     load_stage(st, C, dNx, vol, e, lane);
     store_stage(st, L, lane);
     wave_sync();
@@ -269,58 +312,63 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_f64(int64_t nelem, con
     }
 }
 
-// out[e] = Ke[e] ue[e]  (batched element GEMV), one thread per element row
-__global__ void k_elemmat_dot(int64_t nrows, int K, const double* __restrict__ Ke, const double* __restrict__ ue,
+// out[e] = Ke[e] ue[e]  (batched element GEMV), one thread per element row; an FP32 Ke is widened
+// to FP64 before each product
+template <class TE>
+__global__ void k_elemmat_dot(int64_t nrows, int K, const TE* __restrict__ Ke, const double* __restrict__ ue,
                               double* __restrict__ out)
 {
     int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (t >= nrows) return;
     const int64_t e = t / K;
-    const double* row = Ke + t * K;
+    const TE* row = Ke + t * K;
     const double* x = ue + e * K;
     double s = 0.0;
-    for (int c = 0; c < K; ++c) s += row[c] * x[c];
+    for (int c = 0; c < K; ++c) s += static_cast<double>(row[c]) * x[c];
     out[t] = s;
 }
 
-// the same product for K = 8 or 24: each wave streams 64 rows (64 K contiguous doubles), issuing
-// all of its 16-byte loads before using any, multiplies by the element vector, stages the products
-// in LDS and sums each row in column order (the order of k_elemmat_dot, so the results are
-// identical).  Ke and ue must be 16-byte aligned.
-template <int K>
-__global__ void __launch_bounds__(256) k_elemmat_dot_rows(int64_t nrows, const double* __restrict__ Ke,
+// the same product for K = 8 or 24: each wave streams 64 rows (64 K contiguous entries), issuing
+// all of its 16-byte loads before using any, multiplies by the element vector (FP64 products),
+// stages the products in LDS and sums each row in column order (the order of k_elemmat_dot, so the
+// results are identical).  Ke and ue must be 16-byte aligned.
+template <int K, class TE>
+__global__ void __launch_bounds__(256) k_elemmat_dot_rows(int64_t nrows, const TE* __restrict__ Ke,
                                                          const double* __restrict__ ue, double* __restrict__ out)
 {
-    static_assert(K % 2 == 0, "pairs of columns");
+    constexpr int VW = 16 / static_cast<int>(sizeof(TE)); // entries per 16-byte load
+    static_assert(K % VW == 0, "whole loads per row");
+    typedef TE vec __attribute__((ext_vector_type(VW)));
     constexpr int RP = K + 1; // LDS row pitch
-    constexpr int NV = K / 2; // 16-byte loads per lane
+    constexpr int NV = K / VW;
     __shared__ double sh[4][64 * RP];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * 64;
     if (r0 >= nrows) return; // uniform per wave
     const int64_t base = r0 * K, total = nrows * K;
-    const double2* K2 = reinterpret_cast<const double2*>(Ke + base);
-    double2 kv[NV];
+    const vec* KV = reinterpret_cast<const vec*>(Ke + base);
+    vec kv[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const int loc = 2 * (k * 64 + l);
-        kv[k] = base + loc < total ? K2[k * 64 + l] : make_double2(0.0, 0.0);
+        const int loc = VW * (k * 64 + l);
+        kv[k] = base + loc < total ? KV[k * 64 + l] : vec{};
     }
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-        const int loc = 2 * (k * 64 + l); // even, so loc and loc + 1 lie in one row (K even)
+        const int loc = VW * (k * 64 + l); // a multiple of VW, so loc .. loc + VW - 1 lie in one row
         const int64_t f = base + loc;
-        double p0 = 0.0, p1 = 0.0;
+        double p[VW];
+#pragma unroll
+        for (int v = 0; v < VW; ++v) p[v] = 0.0;
         if (f < total) {
             const int64_t e = f / (K * K);
             const int c = static_cast<int>(f % K);
-            const double2 u = *reinterpret_cast<const double2*>(ue + e * K + c);
-            p0 = kv[k].x * u.x;
-            p1 = kv[k].y * u.y;
+#pragma unroll
+            for (int v = 0; v < VW; ++v) p[v] = static_cast<double>(kv[k][v]) * ue[e * K + c + v];
         }
         double* d = sh[w] + (loc / K) * RP + loc % K;
-        d[0] = p0;
-        d[1] = p1;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) d[v] = p[v];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -352,7 +400,19 @@ int stiffness_bf16x3(gf_quad q, const double* C, double* elemmat, hipStream_t s)
                    "GF_BF16X3 needs 16-byte aligned qtensor4 and elemmat arrays");
         const int64_t ne = quad_nelem(q);
         if (ne == 0) return;
-        k_stiffness_bf16x3<<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);
+        k_stiffness_bf16x3<double, double><<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);
+        check_launch();
+    });
+}
+
+int stiffness_bf16x3_f32(gf_quad q, const float* C, float* elemmat, hipStream_t s)
+{
+    return guard([&] {
+        GF_REQUIRE(quad_etype(q) == GF_HEX8 && quad_nip(q) == NQ, "GF_BF16X3 needs Hex8 with 8 points");
+        GF_REQUIRE(aligned16(C), "GF_BF16X3 needs a 16-byte aligned qtensor4 array");
+        const int64_t ne = quad_nelem(q);
+        if (ne == 0) return;
+        k_stiffness_bf16x3<float, float><<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);
         check_launch();
     });
 }
@@ -370,21 +430,47 @@ int stiffness_f64_hex8(gf_quad q, const double* C, double* elemmat, hipStream_t 
 
 } // namespace gf
 
-extern "C" int gf_elemmat_dot(int64_t nelem, int K, const double* elemmat, const double* elemvec, double* out,
-                              void* stream)
+namespace {
+
+template <class TE>
+int elemmat_dot(int64_t nelem, int K, const TE* elemmat, const double* elemvec, double* out, void* stream)
 {
     return guard([&] {
         GF_REQUIRE(elemmat && elemvec && out && K > 0, "invalid argument");
         const int64_t n = nelem * K;
         if (n == 0) return;
         const unsigned gw = static_cast<unsigned>((n + 255) / 256); // 4 waves x 64 rows per block
-        const bool a16 = aligned16(elemmat) && aligned16(elemvec);
-        if (K == 24 && a16)
-            k_elemmat_dot_rows<24><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
-        else if (K == 8 && a16)
-            k_elemmat_dot_rows<8><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
+        const bool vec = aligned16(elemmat) && aligned16(elemvec);
+        if (K == 24 && vec)
+            k_elemmat_dot_rows<24, TE><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
+        else if (K == 8 && vec)
+            k_elemmat_dot_rows<8, TE><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
         else
-            k_elemmat_dot<<<nblocks(n, 256), 256, 0, S(stream)>>>(n, K, elemmat, elemvec, out);
+            k_elemmat_dot<TE><<<nblocks(n, 256), 256, 0, S(stream)>>>(n, K, elemmat, elemvec, out);
         check_launch();
+    });
+}
+
+} // namespace
+
+extern "C" int gf_elemmat_dot(int64_t nelem, int K, const double* elemmat, const double* elemvec, double* out,
+                              void* stream)
+{
+    return elemmat_dot(nelem, K, elemmat, elemvec, out, stream);
+}
+
+extern "C" int gf_elemmat_dot_f32(int64_t nelem, int K, const float* elemmat, const double* elemvec, double* out,
+                                  void* stream)
+{
+    return elemmat_dot(nelem, K, elemmat, elemvec, out, stream);
+}
+
+extern "C" int gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV_f32(gf_quad q, const float* qtensor4, float* elemmat,
+                                                               void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(q && qtensor4 && elemmat, "null argument");
+        const int rc = gf::stiffness_bf16x3_f32(q, qtensor4, elemmat, S(stream));
+        if (rc != GF_OK) throw Error(rc, gf_last_error());
     });
 }

@@ -201,6 +201,11 @@ int gf_quad_int_gradN_dot_tensor2_dV(gf_quad q, const void* qtensor, void* elemv
  * (Element.h:849-868).  precision GF_F64 (reference order) or GF_BF16X3 (MFMA, Hex8). */
 int gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV(gf_quad q, const double* qtensor4, double* elemmat,
                                                 int precision, void* stream);
+/* The same product (Element.h:849-868) at GF_BF16X3 with FP32 qtensor4 in and FP32 elemmat out
+ * (Hex8, 8 points): the kernel's operands are FP32 before the hi/lo split and it accumulates in
+ * FP32, so FP32 input and output carry every digit it uses, in 57% of the FP64 interface's bytes. */
+int gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV_f32(gf_quad q, const float* qtensor4, float* elemmat,
+                                                    void* stream);
 
 /* Batched element matrix-vector product out[e] = elemmat[e] . elemvec[e] (K = nne*ndim), the
  * element part of the matrix-free product sum_e K_e u_e; assemble with
@@ -208,6 +213,9 @@ int gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV(gf_quad q, const double* qtensor
  * (Matrix.h:804-815) without forming the sparse matrix. */
 int gf_elemmat_dot(int64_t nelem, int K, const double* elemmat, const double* elemvec, double* out,
                    void* stream);
+/* The same with an FP32 elemmat (each entry widened to FP64 before its product; FP64 sums) */
+int gf_elemmat_dot_f32(int64_t nelem, int K, const float* elemmat, const double* elemvec, double* out,
+                       void* stream);
 
 /* Material: linear elastic law of the callers (GMat* in docs/examples; not in the reference):
  * sigma = K tr(eps) I + 2 G dev(eps), per quadrature point; K_q/G_q [nelem*nip] FP64, or FP32
@@ -217,6 +225,9 @@ int gf_material_elastic_stress(int64_t nq, int tdim, const void* eps, const void
 /* Tangent C_ijkl = K I(x)I + 2G (I_sym - I(x)I/3), FP64 [nq, tdim^4] */
 int gf_material_elastic_tangent(int64_t nq, int tdim, const double* K_q, const double* G_q, double* C,
                                 void* stream);
+/* The same tangent rounded to FP32 (the input of gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV_f32) */
+int gf_material_elastic_tangent_f32(int64_t nq, int tdim, const double* K_q, const double* G_q, float* C,
+                                    void* stream);
 
 /* ================================================================================================
  * MatrixDiagonal  (include/GooseFEM/MatrixDiagonal.h)
@@ -254,6 +265,9 @@ int gf_matrix_destroy(gf_matrix A);
 int gf_matrix_sizes(gf_matrix A, int64_t* nrows, int64_t* nnz);
 /* assemble(elemmat)  (Matrix.h:704-727) */
 int gf_matrix_assemble(gf_matrix A, const double* elemmat, void* stream);
+/* assemble(elemmat) from FP32 element matrices: each entry widened to FP64, summed in the same
+ * order as gf_matrix_assemble (Matrix.h:704-727) */
+int gf_matrix_assemble_f32(gf_matrix A, const float* elemmat, void* stream);
 /* device CSR arrays (rowptr [nrows+1], col [nnz] int32; val [nnz] FP64) */
 int gf_matrix_csr(gf_matrix A, const int32_t** rowptr, const int32_t** col, const double** val);
 /* Dot(x)  (Matrix.h:804-815): b = A x on dofval; nodevec variant via AsDofs / AsNode */

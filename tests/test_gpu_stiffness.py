@@ -110,3 +110,61 @@ def test_elemmat_dot_matches_serial_row_sums(gf, K, nelem):
                 s += Ke[e, i, c] * ue[e, c]
             ref[e, i] = s
     assert np.array_equal(got.reshape(nelem, K), ref)
+
+
+@pytest.mark.parametrize("distort", [0.0, 0.15])
+def test_stiffness_bf16x3_fp32_io(gf, distort):
+    # FP32 tangent in, FP32 element matrices out.  The MFMA kernel rounds its FP64 input to FP32
+    # before the split, so fed the same FP32 values through the FP64 interface it must give the
+    # FP32 interface's result widened, bit for bit; both stay within the BF16x3 bound of FP64.
+    import torch
+
+    c, conn, x, q, _ = hex8_problem(5, distort, seed=3)
+    rng = np.random.default_rng(4)
+    Kq = np.full((q.nelem, 8), 10.0)
+    Gq = rng.uniform(0.5, 1.5, (q.nelem, 8))
+    C32 = gf.elastic_tangent(Kq, Gq, dtype=torch.float32)
+    assert C32.dtype == torch.float32
+    C64 = oracle.elastic_tangent(Kq, Gq)
+    assert np.array_equal(host(C32), C64.astype(np.float32))
+    gq = gf.Element.Hex8.Quadrature(x)
+    k32 = gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C32, gf.BF16X3)
+    assert k32.dtype == torch.float32 and tuple(k32.shape) == (q.nelem, 24, 24)
+    kw = host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C32.double(), gf.BF16X3))
+    assert np.array_equal(host(k32).astype(np.float64), kw)
+    ref = q.Int_gradN_dot_tensor4_dot_gradNT_dV(host(C32).astype(np.float64))
+    assert relerr(host(k32).astype(np.float64), ref) < TOL_BF16X3
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fp32_elemmat_assembly_and_product(gf, periodic):
+    # FP32 element matrices: CSR values and the matrix-free product equal the FP64 paths fed the
+    # widened values, bit for bit (widening is exact; sums keep their order)
+    import torch
+
+    n = 3
+    c, conn = oracle.hex8_regular(n, n, n)
+    if periodic:
+        dofs = oracle.dofs_periodic(c.shape[0], 3, oracle.hex8_nodes_periodic(n, n, n))
+    else:
+        dofs = oracle.dofs(c.shape[0], 3)
+    rng = np.random.default_rng(9)
+    a32 = rng.standard_normal((conn.shape[0], 24, 24)).astype(np.float32)
+    a64 = a32.astype(np.float64)
+    rowptr, col, val = oracle.matrix_assemble(conn, dofs, a64)
+    A = gf.Matrix(conn, dofs)
+    A.assemble(torch.from_numpy(a32).cuda())
+    grp, gcol, gval = (host(t) for t in A.csr())
+    assert np.array_equal(grp, rowptr) and np.array_equal(gcol, col) and np.array_equal(gval, val)
+    ue = rng.standard_normal((conn.shape[0], 24))
+    k32 = torch.from_numpy(a32).cuda()
+    assert np.array_equal(host(gf.elemmat_dot(k32, ue)), host(gf.elemmat_dot(a64, ue)))
+    # the row-serial kernels (rows longer than 256 entries) on FP32 input
+    nfan = 70
+    fconn = np.array([[0, 1 + 3 * i, 2 + 3 * i, 3 + 3 * i] for i in range(nfan)], dtype=np.int64)
+    fdofs = oracle.dofs(3 * nfan + 1, 2)
+    f32 = rng.standard_normal((nfan, 8, 8)).astype(np.float32)
+    rp, cl, vl = oracle.matrix_assemble(fconn, fdofs, f32.astype(np.float64))
+    B = gf.Matrix(fconn, fdofs)
+    B.assemble(torch.from_numpy(f32).cuda())
+    assert np.array_equal(host(B.csr()[2]), vl)
