@@ -216,7 +216,23 @@ __global__ void __launch_bounds__(BS) k_spmv(int64_t nrows, const int32_t* __res
     if (row < nrows) {
         b = rowptr[row];
         n = rowptr[row + 1] - b;
-        for (int p = lane; p < n; p += L) s_p[g][p] = val[b + p] * x[col[b + p]];
+        if (n > 0) {
+            // all of the lane's loads are issued together (indices past the row's end are clamped
+            // to its last entry and not stored): the columns first, then the values and x
+            constexpr int NI = CAP / L;
+            int32_t cc[NI];
+#pragma unroll
+            for (int i = 0; i < NI; ++i) cc[i] = col[b + min(lane + i * L, n - 1)];
+            double vv[NI], xv[NI];
+#pragma unroll
+            for (int i = 0; i < NI; ++i) {
+                vv[i] = val[b + min(lane + i * L, n - 1)];
+                xv[i] = x[cc[i]];
+            }
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+                if (lane + i * L < n) s_p[g][lane + i * L] = vv[i] * xv[i];
+        }
     }
     __syncthreads();
     if (row < nrows && lane == 0) {
