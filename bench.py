@@ -95,13 +95,13 @@ def cpu_baseline(S, nx, ny, nsteps):
     return ov.ndof * nsteps / dt, dt
 
 
-def load_traffic(nx):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if it matches this size."""
-    p = os.path.join(ROOT, "profiles", "explicit_vv_quad4_2048_traffic.json")
+def load_traffic(nx, name="explicit_vv_quad4_2048_traffic.json", size_key="nx", key="traffic_bytes_per_launch"):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if it matches this size."""
+    p = os.path.join(ROOT, "profiles", name)
     try:
         d = json.load(open(p))
-        if d.get("nx") == nx:
-            return d.get("traffic_bytes_per_launch")
+        if d.get(size_key) == nx:
+            return d.get(key)
     except (OSError, ValueError):
         pass
     return None
@@ -186,13 +186,15 @@ def bench_hex8(args):
                    "ndof": vec.ndof, "nnz": A.nnz},
         "dtype": "bf16x3 (hi/lo split) mfma, fp32 accumulate; fp64 inputs/outputs",
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_stiffness_bf16x3",
+                     "frac": ach / HBM_PEAK_GBS,
+                     "traffic": load_traffic(n, "hex8_stiffness_128_traffic.json", "n"), "kernel": "k_stiffness_bf16x3",
                      "algorithmic_bytes_per_launch": bytes_el * E, "avg_launch_us": tbf * 1e6,
                      "tflops_voigt": flops_el * E / tbf / 1e12},
         "fp64_stiffness": {"avg_launch_us": t64 * 1e6, "achieved_GBs": bytes_el * E / t64 / 1e9},
         # FP32 tangent in, FP32 K_e out (the kernel's own operand and accumulator precision):
         # C 2592 + GradN 1536 + dV 64 + K_e 2304 = 6496 B per element
         "fp32_io": {"avg_launch_us": t32 * 1e6, "elements_per_s": E / t32,
+                    "traffic": load_traffic(n, "hex8_stiffness_128_traffic.json", "n", "traffic_bytes_per_launch_f32io"),
                     "achieved_GBs": 6496 * E / t32 / 1e9, "frac": 6496 * E / t32 / 1e9 / HBM_PEAK_GBS,
                     "Ke_rel_vs_fp64": err_ke32, "assemble_ms": t_asm32 * 1e3,
                     "assemble_GBs": (E * 2304 + A.nnz * 8) / t_asm32 / 1e9, "matrix_free_ms": t_mf32 * 1e3,
