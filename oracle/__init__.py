@@ -1,7 +1,7 @@
 """FP64 CPU oracle for GooseFEM's element-quadrature + assembly path (TEST INFRASTRUCTURE ONLY).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` import
-this package, and only as the checker.  The product (``goosefem_amd``) never imports it.
+this package, and only as the reference for comparisons.  The product (``goosefem_amd``) never imports it.
 
 The numerics live in ``oracle.cpp`` (a loop-for-loop restatement of the reference files cited
 there); this module wraps them with numpy.  Every function states the reference lines it follows.

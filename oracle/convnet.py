@@ -1,5 +1,5 @@
-"""FP64 CPU restatement of the warm-start network (test infrastructure only: tests/ load it as the
-checker of goosefem_amd/csrc/gf_conv.hip; nothing in goosefem_amd/ imports it).
+"""FP64 CPU restatement of the warm-start network (test infrastructure only: tests/ compare
+goosefem_amd/csrc/gf_conv.hip against it; nothing in goosefem_amd/ imports it).
 
 Same layout and architecture as the device implementation: NHWC activations, weights
 [9][cout][cin] with tap t = 3 (dy + 1) + (dx + 1), 3x3 circular convolution, 2x2 average pooling,

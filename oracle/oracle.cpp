@@ -1,5 +1,5 @@
 // FP64 CPU oracle: a loop-for-loop restatement of GooseFEM's element-quadrature and assembly
-// path, used ONLY as the checker by tests/, __graft_entry__.smoke() and the CPU-baseline leg of
+// path, used ONLY as the reference results in tests/, __graft_entry__.smoke() and the CPU-baseline leg of
 // bench.py.  Nothing in the product (goosefem_amd/) links, loads or calls this file.
 //
 // Every function follows the reference loop order and expression order so that FP64 results are
