@@ -168,3 +168,27 @@ def test_fp32_elemmat_assembly_and_product(gf, periodic):
     B = gf.Matrix(fconn, fdofs)
     B.assemble(torch.from_numpy(f32).cuda())
     assert np.array_equal(host(B.csr()[2]), vl)
+
+
+def test_alignment_rules(gf):
+    # the FP32-I/O stiffness needs a 16-byte aligned tensor4 (clear error otherwise); the element
+    # GEMV falls back to its per-row kernel for unaligned arrays and gives the same results
+    import torch
+
+    c, conn, x, q, C = hex8_problem(2, 0.1, seed=5)
+    gq = gf.Element.Hex8.Quadrature(x)
+    flat = torch.from_numpy(C.astype(np.float32).ravel()).cuda()
+    buf = torch.zeros(flat.numel() + 1, dtype=torch.float32, device="cuda")
+    buf[1:] = flat
+    shifted = buf[1:].view(C.shape)
+    assert shifted.data_ptr() % 16 != 0
+    with pytest.raises(gf.GooseFEMError):
+        gq.Int_gradN_dot_tensor4_dot_gradNT_dV(shifted, gf.BF16X3)
+    rng = np.random.default_rng(6)
+    Ke = rng.standard_normal((q.nelem, 24, 24))
+    ue = rng.standard_normal((q.nelem, 24))
+    ref = host(gf.elemmat_dot(Ke, ue))
+    kb = torch.zeros(Ke.size + 1, dtype=torch.float64, device="cuda")
+    kb[1:] = torch.from_numpy(Ke.ravel()).cuda()
+    assert kb[1:].data_ptr() % 16 != 0
+    assert np.array_equal(host(gf.elemmat_dot(kb[1:].view(Ke.shape), ue)), ref)
