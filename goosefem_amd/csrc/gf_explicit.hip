@@ -200,10 +200,58 @@ __device__ inline void element_force_f32(const ShapeTable& T, const double x[4][
     }
 }
 
+// ---- per-element force, FP32, rectangular element -------------------------------------------------
+// For an axis-aligned sx x sy rectangle the Jacobian is diag(sx/2, sy/2) at every Gauss point, so the
+// displacement gradient is linear in (xi, eta): grad = grad_c + xi grad_xi + eta grad_eta, and so is
+// the stress.  With Gauss points (+-a, +-a), sum_q xi_q = sum_q eta_q = sum_q xi_q eta_q = 0 and
+// sum_q xi_q^2 = sum_q eta_q^2 = 4 a^2 = 4/3, so the sums of element_force_f32 reduce to
+//   S0 = 4 dV I0 sigma_c(0,.),  S1 = 4 dV I1 sigma_c(1,.),
+//   SH = 4/3 dV (I0 sigma_eta(0,.) + I1 sigma_xi(1,.)),   I0 = 2/sx, I1 = 2/sy, dV = sx sy / 4,
+// the same 2x2 Gauss quadrature evaluated in closed form (f as in element_force_f32).
+__device__ inline void element_force_f32_rect(float I0, float I1, float dv, const double u[4][2], float k,
+                                              float gmod, float f[4][2])
+{
+    float ur[4][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        ur[0][i] = 0.0f;
+#pragma unroll
+        for (int m = 1; m < 4; ++m) ur[m][i] = static_cast<float>(u[m][i] - u[0][i]);
+    }
+    float Au[2], Bu[2], Hu[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        Au[i] = 0.25f * (ur[1][i] + ur[2][i] - ur[3][i]);
+        Bu[i] = 0.25f * (ur[2][i] + ur[3][i] - ur[1][i]);
+        Hu[i] = 0.25f * (ur[2][i] - ur[1][i] - ur[3][i]);
+    }
+    const float g2 = 2.0f * gmod, kd = fmaf(g2, 2.0f / 3.0f, k); // k + 4 G / 3
+    // centre: grad(i,j) = I_i du_j/dxi_i
+    const float gxx = I0 * Au[0], gxy = I0 * Au[1], gyx = I1 * Bu[0], gyy = I1 * Bu[1];
+    const float tr = gxx + gyy, kt = k * tr, t3 = tr * (1.0f / 3.0f);
+    const float sxx = fmaf(g2, gxx - t3, kt), syy = fmaf(g2, gyy - t3, kt), sxy = gmod * (gxy + gyx);
+    // eta slope (only grad(0,.) depends on eta) and xi slope (only grad(1,.))
+    const float sxx_e = kd * (I0 * Hu[0]), sxy_e = gmod * (I0 * Hu[1]);
+    const float syy_x = kd * (I1 * Hu[1]), sxy_x = gmod * (I1 * Hu[0]);
+    const float c0 = 4.0f * dv * I0, c1 = 4.0f * dv * I1, ch = (4.0f / 3.0f) * dv;
+    const float S0[2] = {c0 * sxx, c0 * sxy};
+    const float S1[2] = {c1 * sxy, c1 * syy};
+    const float SH[2] = {ch * fmaf(I0, sxx_e, I1 * sxy_x), ch * fmaf(I0, sxy_e, I1 * syy_x)};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float p = 0.25f * S0[j], r = 0.25f * S1[j], h = 0.25f * SH[j];
+        f[0][j] = -p - r + h;
+        f[1][j] = p - r - h;
+        f[2][j] = p + r + h;
+        f[3][j] = -p + r - h;
+    }
+}
+
 // ---- structured fused step ----------------------------------------------------------------------
-template <int TX, int TY, bool F32>
+template <int TX, int TY, bool F32, bool RECT = false>
 __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     int nx, int ny, int ntx, int nty, int per_xcd, ShapeTable T, const double* __restrict__ coor, double sx, double sy,
+    float rI0, float rI1, float rdv,
     const float* __restrict__ Ke, float Kconst, const float* __restrict__ Ge, const double* __restrict__ inv_mass,
     const double* __restrict__ fext, double dt, double c, double hdt, const double* __restrict__ u0,
     const double* __restrict__ v0, const double* __restrict__ a0, double* __restrict__ u1, double* __restrict__ v1,
@@ -285,14 +333,14 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     if (owner) {
         const int t = (oly + 1) * HX + olx + 1;
         s_u[t] = o_un;
-        s_x[t] = o_x;
+        if constexpr (!RECT) s_x[t] = o_x; // the rectangle body needs no coordinates
     }
     if (rt >= 0) {
         double2 hn;
         hn.x = h_u.x + dt * h_v.x + c * h_a.x;
         hn.y = h_u.y + dt * h_v.y + c * h_a.y;
         s_u[rt] = hn;
-        s_x[rt] = h_x;
+        if constexpr (!RECT) s_x[rt] = h_x;
     }
     __syncthreads();
 
@@ -319,7 +367,8 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
             }
             if constexpr (F32) {
                 float f[4][2];
-                element_force_f32(T, x, u, Ke ? Ke[e] : Kconst, Ge[e], f);
+                if constexpr (RECT) element_force_f32_rect(rI0, rI1, rdv, u, Ke ? Ke[e] : Kconst, Ge[e], f);
+                else element_force_f32(T, x, u, Ke ? Ke[e] : Kconst, Ge[e], f);
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     s_f[t][2 * m] = f[m][0];
@@ -538,7 +587,9 @@ struct TileCfg {
     int tx, ty;
 };
 
-TileCfg tile_config(bool f32)
+// Node tile per workgroup (measured on MI355X at 2048^2): FP32 with Regular coordinates (closed-form
+// rectangle body) 64 x 8, other FP32 32 x 16, FP64 32 x 8.  GF_EXPLICIT_TILE=<tx>x<ty> overrides.
+TileCfg tile_config(bool f32, bool rect)
 {
     static const TileCfg over = [] {
         TileCfg c{0, 0};
@@ -549,7 +600,8 @@ TileCfg tile_config(bool f32)
         return c;
     }();
     if (over.tx > 0) return over;
-    return f32 ? TileCfg{32, 16} : TileCfg{32, 8};
+    if (f32) return rect ? TileCfg{64, 8} : TileCfg{32, 16};
+    return TileCfg{32, 8};
 }
 
 template <int TX, int TY>
@@ -557,15 +609,27 @@ void launch_structured(bool f32, unsigned grid, hipStream_t s, int nx, int ny, i
                        const ShapeTable& T, gf_explicit_s* x, const double* fext, double dt, double c, double hdt,
                        int i0, int i1, double* fo)
 {
-    if (f32)
+    if (f32 && x->reg_x)
+        k_vv_structured<TX, TY, true, true><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T,
+                                                               x->reg_x ? nullptr : x->coor.p, x->sx, x->sy,
+                                                               static_cast<float>(2.0 / x->sx), static_cast<float>(2.0 / x->sy),
+                                                               static_cast<float>(x->sx * x->sy / 4.0),
+                                                               x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
+                                                               x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
+                                                               x->a[i1].p, fo);
+    else if (f32)
         k_vv_structured<TX, TY, true><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T,
                                                                x->reg_x ? nullptr : x->coor.p, x->sx, x->sy,
+                                                               static_cast<float>(2.0 / x->sx), static_cast<float>(2.0 / x->sy),
+                                                               static_cast<float>(x->sx * x->sy / 4.0),
                                                                x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
                                                                x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
                                                                x->a[i1].p, fo);
     else
         k_vv_structured<TX, TY, false><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T,
                                                                 x->reg_x ? nullptr : x->coor.p, x->sx, x->sy,
+                                                               static_cast<float>(2.0 / x->sx), static_cast<float>(2.0 / x->sy),
+                                                               static_cast<float>(x->sx * x->sy / 4.0),
                                                                 x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
                                                                 x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
                                                                 x->a[i1].p, fo);
@@ -700,7 +764,7 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
         const bool f32 = precision == GF_F32;
         if (x->structured) {
             static const ShapeTable T = gauss_table();
-            const TileCfg tc = tile_config(f32);
+            const TileCfg tc = tile_config(f32, x->reg_x);
             const int TXr = tc.tx, TYr = tc.ty;
             const int nx = static_cast<int>(x->nx), ny = static_cast<int>(x->ny);
             const int ntx = (nx + 1 + TXr - 1) / TXr, nty = (ny + 1 + TYr - 1) / TYr;
@@ -719,8 +783,15 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
                     launch_structured<64, 4>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
                 else if (TXr == 16 && TYr == 16)
                     launch_structured<16, 16>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                else if (TXr == 128 && TYr == 4)
+                    launch_structured<128, 4>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                else if (TXr == 64 && TYr == 16)
+                    launch_structured<64, 16>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                else if (TXr == 128 && TYr == 8)
+                    launch_structured<128, 8>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
                 else
-                    throw Error(GF_EINVAL, "GF_EXPLICIT_TILE must be one of 32x8, 64x8, 32x16, 64x4, 16x16");
+                    throw Error(GF_EINVAL,
+                                "GF_EXPLICIT_TILE must be one of 32x8, 64x8, 32x16, 64x4, 16x16, 128x4, 64x16, 128x8");
                 check_launch();
                 x->cur = i1;
             }
