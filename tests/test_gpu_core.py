@@ -333,6 +333,29 @@ def test_explicit_fp32_tolerance(gf, shuffle, kvar, distort):
     assert relerr(ga, a) < TOL_F32
 
 
+@pytest.mark.parametrize("nx,ny,sx,sy", [(130, 17, 2.0, 0.5), (64, 48, 0.25, 3.0)])
+def test_explicit_fp32_rectangles(gf, nx, ny, sx, sy):
+    # Regular coordinates (i sx, j sy) with sx != sy: the FP32 step uses the closed-form rectangle
+    # body on partly filled tiles; f_int and a against the FP64 oracle (u is FP64 but its second
+    # step carries the first step's FP32 acceleration, weighted by dt^2 / 2)
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, kvar=True)
+    coor = coor * np.array([sx, sy])
+    vec_o = oracle.Vector(conn, dofs)
+    q = oracle.Quadrature(QUAD4_PLANAR, vec_o.AsElement(coor))
+    xi, w = oracle.quad_rule(QUAD4_PLANAR, True)
+    qn = oracle.Quadrature(QUAD4_PLANAR, vec_o.AsElement(coor), xi, w)
+    inv_all = oracle.diag_inverse(oracle.diag_assemble(conn, dofs, qn.Int_N_scalar_NT_dV(np.ones((q.nelem, 4)))))
+    inv = np.where(inv == 0.0, 0.0, inv_all)
+    X = gf.ExplicitQuad4(gf.Vector(conn, dofs), coor, Ke, Ge, inv, 0.01)
+    assert X.structured
+    X.set_state(u, v, a)
+    fint = oracle.vv_steps(2, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a)
+    X.step(2, gf.F32, store_fint=True)
+    assert relerr(host(X.fint()), fint) < TOL_F32
+    gu, gv_, ga = (host(t) for t in X.state())
+    assert relerr(ga, a) < TOL_F32 and relerr(gu, u) < 1e-8
+
+
 def test_explicit_rejects_bad_input(gf):
     import ctypes
 
