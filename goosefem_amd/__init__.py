@@ -495,6 +495,110 @@ class Vector(_Handle):
         return out
 
 
+class VectorPartitioned(Vector):
+    """GooseFEM::VectorPartitioned (VectorPartitioned.h:72-661): a Vector whose DOFs split into
+    unknown (iiu, ascending) and prescribed (iip) ones; all conversions run on the GPU."""
+
+    def __init__(self, conn, dofs, iip):
+        super().__init__(conn, dofs)
+        iip = _i64a(iip).ravel()
+        h = _vp()
+        _call("gf_vecpart_create", ctypes.byref(h), self._h, _hp(iip), _i64(iip.size))
+        self._ph = h
+        nu, npp = _i64(0), _i64(0)
+        _call("gf_vecpart_sizes", h, ctypes.byref(nu), ctypes.byref(npp))
+        self.nnu, self.nnp = nu.value, npp.value
+        self._iiu = np.empty(self.nnu, dtype=np.int64)
+        self._iip = np.empty(self.nnp, dtype=np.int64)
+        self._part = np.empty((self.nnode, self.ndim), dtype=np.int64)
+        _call("gf_vecpart_index", h, _hp(self._iiu), _hp(self._iip), _hp(self._part))
+
+    def __del__(self):
+        h = getattr(self, "_ph", None)
+        if h is not None and _lib is not None:
+            try:
+                _lib.gf_vecpart_destroy(h)
+            except Exception:
+                pass
+            self._ph = None
+        super().__del__()
+
+    @property
+    def iiu(self):
+        return self._iiu.copy()
+
+    @property
+    def iip(self):
+        return self._iip.copy()
+
+    def dofs_is_u(self):
+        return self._part < self.nnu
+
+    def dofs_is_p(self):
+        return self._part >= self.nnu
+
+    @staticmethod
+    def _kind(t, ndof):
+        if t.dim() == 1:
+            assert t.numel() == ndof
+            return 0
+        return 1 if t.dim() == 2 else 2
+
+    def AsDofs_u(self, arg):
+        """AsDofs_u(dofval | nodevec | elemvec) (VectorPartitioned.h:428-540)."""
+        arg = to_device(arg)
+        out = _empty((self.nnu,))
+        _call("gf_vecpart_as_dofs", self._ph, _i32(self._kind(arg, self.ndof)), _dp(arg), _dp(out), None, _stream())
+        return out
+
+    def AsDofs_p(self, arg):
+        """AsDofs_p(dofval | nodevec | elemvec) (VectorPartitioned.h:549-661)."""
+        arg = to_device(arg)
+        out = _empty((self.nnp,))
+        _call("gf_vecpart_as_dofs", self._ph, _i32(self._kind(arg, self.ndof)), _dp(arg), None, _dp(out), _stream())
+        return out
+
+    def _from(self, kind, shape, dofval_u, dofval_p):
+        u, p = to_device(dofval_u), to_device(dofval_p)
+        assert u.numel() == self.nnu and p.numel() == self.nnp
+        out = _empty(shape)
+        _call("gf_vecpart_from_partitioned", self._ph, _i32(kind), _dp(u), _dp(p), _dp(out), _stream())
+        return out
+
+    def DofsFromParitioned(self, dofval_u, dofval_p):
+        """DofsFromParitioned (sic, VectorPartitioned.h:276-307)."""
+        return self._from(0, self.shape_dofval(), dofval_u, dofval_p)
+
+    def NodeFromPartitioned(self, dofval_u, dofval_p):
+        """NodeFromPartitioned (VectorPartitioned.h:325-358)."""
+        return self._from(1, self.shape_nodevec(), dofval_u, dofval_p)
+
+    def ElementFromPartitioned(self, dofval_u, dofval_p):
+        """ElementFromPartitioned (VectorPartitioned.h:376-412)."""
+        return self._from(2, self.shape_elemvec(), dofval_u, dofval_p)
+
+    def _copy(self, which, src, dest):
+        src = to_device(src)
+        _call("gf_vecpart_copy", self._ph, _i32(which), _dp(src), _dp(dest), _stream())
+        return dest
+
+    def Copy_u(self, nodevec_src, nodevec_dest):
+        """Copy_u (VectorPartitioned.h:176-212): nodevec_dest with its unknown entries from src."""
+        return self._copy(0, nodevec_src, to_device(nodevec_dest).clone())
+
+    def Copy_p(self, nodevec_src, nodevec_dest):
+        """Copy_p (VectorPartitioned.h:229-263): nodevec_dest with its prescribed entries from src."""
+        return self._copy(1, nodevec_src, to_device(nodevec_dest).clone())
+
+    def copy_u(self, nodevec_src, nodevec_dest):
+        """copy_u: in place on a contiguous float64 device tensor nodevec_dest."""
+        return self._copy(0, nodevec_src, nodevec_dest)
+
+    def copy_p(self, nodevec_src, nodevec_dest):
+        """copy_p: in place on a contiguous float64 device tensor nodevec_dest."""
+        return self._copy(1, nodevec_src, nodevec_dest)
+
+
 # -------------------------------------------------------------------------------------------------
 # Quadrature (include/GooseFEM/Element*.h)
 # -------------------------------------------------------------------------------------------------
@@ -776,6 +880,100 @@ class MatrixDiagonal(_Handle):
         p = _vp()
         _call("gf_diag_inverse", self._h, ctypes.byref(p), _stream())
         return p
+
+
+class MatrixDiagonalPartitioned(MatrixDiagonal):
+    """GooseFEM::MatrixDiagonalPartitioned (MatrixDiagonalPartitioned.h:42-452).  assemble, set,
+    data, Dot and Solve's factorisation act on the unpartitioned diagonal (MatrixDiagonal)."""
+
+    def __init__(self, conn, dofs=None, iip=None):
+        self._vec = conn if isinstance(conn, VectorPartitioned) else VectorPartitioned(conn, dofs, iip)
+        h = _vp()
+        _call("gf_diagpart_create", ctypes.byref(h), self._vec._ph)
+        self._dph = h
+        d = _vp()
+        _call("gf_diagpart_diag", h, ctypes.byref(d))
+        self._h = d  # the unpartitioned diagonal, owned by the partitioned matrix
+
+    def __del__(self):
+        h = getattr(self, "_dph", None)
+        if h is not None and _lib is not None:
+            try:
+                _lib.gf_diagpart_destroy(h)
+            except Exception:
+                pass
+        self._dph = None
+        self._h = None
+
+    @property
+    def nnu(self):
+        return self._vec.nnu
+
+    @property
+    def nnp(self):
+        return self._vec.nnp
+
+    def set(self, A):
+        """set(A) (MatrixDiagonalPartitioned.h:121-135)."""
+        Ad = to_device(A)
+        _call("gf_diagpart_set", self._dph, _dp(Ad), _stream())
+
+    @property
+    def data_uu(self):
+        out = _empty((self.nnu,))
+        _call("gf_diagpart_data_part", self._dph, _dp(out), None, _stream())
+        return out
+
+    @property
+    def data_pp(self):
+        out = _empty((self.nnp,))
+        _call("gf_diagpart_data_part", self._dph, None, _dp(out), _stream())
+        return out
+
+    def Dot_u(self, x_u, x_p):
+        """Dot_u (MatrixDiagonalPartitioned.h:248-264): A_uu x_u."""
+        x = to_device(x_u)
+        out = _empty((self.nnu,))
+        _call("gf_diagpart_dot_part", self._dph, _i32(0), _dp(x), _dp(out), _stream())
+        return out
+
+    def Dot_p(self, x_u, x_p):
+        """Dot_p (MatrixDiagonalPartitioned.h:286-300): A_pp x_p."""
+        x = to_device(x_p)
+        out = _empty((self.nnp,))
+        _call("gf_diagpart_dot_part", self._dph, _i32(1), _dp(x), _dp(out), _stream())
+        return out
+
+    def Solve_u(self, b_u, x_p):
+        """Solve_u (MatrixDiagonalPartitioned.h:356-372): A_uu^-1 b_u."""
+        b = to_device(b_u)
+        out = _empty((self.nnu,))
+        _call("gf_diagpart_solve_u", self._dph, _dp(b), _dp(out), _stream())
+        return out
+
+    def Solve(self, b, x=None):
+        """Solve(b) / solve(b, x) (MatrixDiagonal.h:53-105, MatrixDiagonalPartitioned.h:306-337):
+        the unknown entries A_uu^-1 b_u written over a copy of x (zeros when x is not given; the
+        reference leaves the prescribed entries of Solve(b) uninitialised)."""
+        torch = _torch()
+        b = to_device(b)
+        out = torch.zeros_like(b) if x is None else to_device(x).clone()
+        _call("gf_diagpart_apply_sel", self._dph, _i32(0), _i32(0 if b.dim() == 1 else 1), _dp(b), _dp(out),
+              _stream())
+        return out
+
+    def Reaction(self, x, b):
+        """Reaction(x, b) (Matrix.h:490-514, MatrixDiagonalPartitioned.h:378-403): copy of b with
+        its prescribed entries A_pp x_p."""
+        x = to_device(x)
+        out = to_device(b).clone()
+        _call("gf_diagpart_apply_sel", self._dph, _i32(1), _i32(0 if x.dim() == 1 else 1), _dp(x), _dp(out),
+              _stream())
+        return out
+
+    def Reaction_p(self, x_u, x_p):
+        """Reaction_p (MatrixDiagonalPartitioned.h:405-420): A_pp x_p."""
+        return self.Dot_p(x_u, x_p)
 
 
 class Matrix(_Handle):

@@ -53,6 +53,8 @@ typedef struct gf_diag_s* gf_diag;
 typedef struct gf_matrix_s* gf_matrix;
 typedef struct gf_explicit_s* gf_explicit;
 typedef struct gf_pcg_s* gf_pcg;
+typedef struct gf_vecpart_s* gf_vecpart;
+typedef struct gf_diagpart_s* gf_diagpart;
 
 const char* gf_last_error(void);
 int gf_version(int* major, int* minor, int* patch);
@@ -250,6 +252,53 @@ int gf_diag_dot_nodevec(gf_diag d, const double* x, double* b, void* stream);
 /* Solve(b)  (MatrixDiagonal.h:236-259) */
 int gf_diag_solve_dofval(gf_diag d, const double* b, double* x, void* stream);
 int gf_diag_solve_nodevec(gf_diag d, const double* b, double* x, void* stream);
+
+/* ================================================================================================
+ * Dirichlet partitioning: VectorPartitioned (include/GooseFEM/VectorPartitioned.h) and
+ * MatrixDiagonalPartitioned (include/GooseFEM/MatrixDiagonalPartitioned.h).  The DOFs split into
+ * unknown iiu = setdiff1d(dofs, iip) (ascending) and prescribed iip (caller's order); part =
+ * Mesh::Reorder({iiu, iip}).apply(dofs) (Mesh.h:2384-2454).  All results are bit-exact (gathers,
+ * scatters and per-DOF products).  kind: 0 = dofval [ndof], 1 = nodevec [nnode, ndim],
+ * 2 = elemvec [nelem, nne, ndim].
+ * ============================================================================================== */
+
+/* VectorPartitioned(conn, dofs, iip)  (VectorPartitioned.h:72-87); uses v's conn/dofs */
+int gf_vecpart_create(gf_vecpart* out, gf_vector v, const int64_t* iip_h, int64_t nnp);
+int gf_vecpart_destroy(gf_vecpart p);
+/* nnu(), nnp()  (VectorPartitioned.h:92-103) */
+int gf_vecpart_sizes(gf_vecpart p, int64_t* nnu, int64_t* nnp);
+/* iiu(), iip() and the reordered DOFs part [nnode, ndim] as host copies (VectorPartitioned.h:108-119;
+ * dofs_is_u / dofs_is_p, :126-161, are part < nnu / part >= nnu); any output may be NULL */
+int gf_vecpart_index(gf_vecpart p, int64_t* iiu_h, int64_t* iip_h, int64_t* part_h);
+/* AsDofs_u / AsDofs_p of a dofval, nodevec or elemvec (VectorPartitioned.h:428-661; nodevec and
+ * elemvec overwrite in loop order: each DOF gets its last source); either output may be NULL */
+int gf_vecpart_as_dofs(gf_vecpart p, int kind, const double* in, double* dofval_u, double* dofval_p, void* stream);
+/* DofsFromParitioned / NodeFromPartitioned / ElementFromPartitioned (VectorPartitioned.h:276-424) */
+int gf_vecpart_from_partitioned(gf_vecpart p, int kind, const double* dofval_u, const double* dofval_p, double* out,
+                                void* stream);
+/* copy_u (which 0) / copy_p (which 1): nodevec_dest[is_u or is_p] = nodevec_src  (VectorPartitioned.h:198-263) */
+int gf_vecpart_copy(gf_vecpart p, int which, const double* nodevec_src, double* nodevec_dest, void* stream);
+
+/* MatrixDiagonalPartitioned(conn, dofs, iip)  (MatrixDiagonalPartitioned.h:42-68) over p.  Its
+ * unpartitioned diagonal is a gf_diag (gf_diagpart_diag): assemble (:73-98, the same per-DOF sum
+ * in the same order), data (:142-156), Dot (:189-224) and the factorisation (:440-452,
+ * inv = 1 / A) are the gf_diag_* calls on it. */
+int gf_diagpart_create(gf_diagpart* out, gf_vecpart p);
+int gf_diagpart_destroy(gf_diagpart d);
+int gf_diagpart_diag(gf_diagpart d, gf_diag* out);
+/* set(A)  (MatrixDiagonalPartitioned.h:121-135): A [ndof]; entries of DOF numbers that occur in
+ * neither iiu nor iip are not taken (they read 0 in data()) */
+int gf_diagpart_set(gf_diagpart d, const double* A, void* stream);
+/* data_uu(), data_pp()  (MatrixDiagonalPartitioned.h:163-175); either output may be NULL */
+int gf_diagpart_data_part(gf_diagpart d, double* A_uu, double* A_pp, void* stream);
+/* Dot_u (which 0): b_u = A_uu x_u;  Dot_p / Reaction_p (which 1): b_p = A_pp x_p
+ * (MatrixDiagonalPartitioned.h:248-300, :405-420) */
+int gf_diagpart_dot_part(gf_diagpart d, int which, const double* x, double* b, void* stream);
+/* Solve_u: x_u = A_uu^-1 b_u  (MatrixDiagonalPartitioned.h:356-372) */
+int gf_diagpart_solve_u(gf_diagpart d, const double* b_u, double* x_u, void* stream);
+/* Solve (op 0; writes only the unknown entries of out, :306-337) or Reaction (op 1; writes only
+ * the prescribed entries, :378-403) on a dofval (kind 0) or nodevec (kind 1) */
+int gf_diagpart_apply_sel(gf_diagpart d, int op, int kind, const double* in, double* out, void* stream);
 
 /* ================================================================================================
  * Matrix: sparse (CSR, sorted columns)  (include/GooseFEM/Matrix.h:653-860)

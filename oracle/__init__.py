@@ -592,3 +592,199 @@ def randn(seed: int, n: int, mean: float = 0.0, std: float = 1.0) -> np.ndarray:
     f.argtypes = [ctypes.c_uint64, _i64, _dbl, _dbl, ctypes.c_void_p]
     f(seed, n, mean, std, _p(out))
     return out
+
+
+# -------------------------------------------------------------------------------------------------
+# Dirichlet partitioning: VectorPartitioned.h:72-661, MatrixDiagonalPartitioned.h:42-452.  Plain
+# loops in the reference's loop order (small meshes only).
+# -------------------------------------------------------------------------------------------------
+
+
+class VectorPartitioned:
+    """GooseFEM::VectorPartitioned: iiu = setdiff1d(dofs, iip) (VectorPartitioned.h:80),
+    part = Mesh::Reorder({iiu, iip}).apply(dofs) (Mesh.h:2384-2454)."""
+
+    def __init__(self, conn, dofs_, iip):
+        self.conn, self.dofs = _i(conn), _i(dofs_)
+        self.iip = _i(iip).ravel()
+        assert np.unique(self.iip).size == self.iip.size
+        self.nelem, self.nne = self.conn.shape
+        self.nnode, self.ndim = self.dofs.shape
+        self.ndof = int(self.dofs.max()) + 1
+        self.iiu = np.setdiff1d(self.dofs, self.iip)  # sorted unique, like xt::setdiff1d
+        self.nnu, self.nnp = self.iiu.size, self.iip.size
+        renum = np.full(max(self.ndof, int(self.iip.max()) + 1 if self.nnp else 0), -1, dtype=np.int64)
+        i = 0
+        for arr in (self.iiu, self.iip):  # Reorder: running index over the concatenation
+            for j in arr:
+                renum[j] = i
+                i += 1
+        self.part = renum[self.dofs]
+
+    def dofs_is_u(self):  # VectorPartitioned.h:126-143
+        return self.part < self.nnu
+
+    def dofs_is_p(self):  # VectorPartitioned.h:147-161
+        return self.part >= self.nnu
+
+    def _as_dofs(self, arg, want_u):
+        """asDofs_u / asDofs_p (VectorPartitioned.h:443-661): zero fill, then overwrite in loop order."""
+        arg = _f64(arg)
+        n = self.nnu if want_u else self.nnp
+        out = np.zeros(n)
+        sel = (lambda d: d < self.nnu) if want_u else (lambda d: d >= self.nnu)
+        off = 0 if want_u else self.nnu
+        if arg.ndim == 1:
+            idx = self.iiu if want_u else self.iip
+            for d in range(n):
+                out[d] = arg[idx[d]]
+        elif arg.ndim == 2:
+            for m in range(self.nnode):
+                for i in range(self.ndim):
+                    d = self.part[m, i]
+                    if sel(d):
+                        out[d - off] = arg[m, i]
+        else:
+            for e in range(self.nelem):
+                for m in range(self.nne):
+                    for i in range(self.ndim):
+                        d = self.part[self.conn[e, m], i]
+                        if sel(d):
+                            out[d - off] = arg[e, m, i]
+        return out
+
+    def AsDofs_u(self, arg):
+        return self._as_dofs(arg, True)
+
+    def AsDofs_p(self, arg):
+        return self._as_dofs(arg, False)
+
+    def DofsFromParitioned(self, u, p):  # VectorPartitioned.h:293-307
+        out = np.zeros(self.ndof)
+        for d in range(self.nnu):
+            out[self.iiu[d]] = u[d]
+        for d in range(self.nnp):
+            out[self.iip[d]] = p[d]
+        return out
+
+    def _pick(self, d, u, p):
+        return u[d] if d < self.nnu else p[d - self.nnu]
+
+    def NodeFromPartitioned(self, u, p):  # VectorPartitioned.h:344-358
+        out = np.empty((self.nnode, self.ndim))
+        for m in range(self.nnode):
+            for i in range(self.ndim):
+                out[m, i] = self._pick(self.part[m, i], u, p)
+        return out
+
+    def ElementFromPartitioned(self, u, p):  # VectorPartitioned.h:395-412
+        out = np.empty((self.nelem, self.nne, self.ndim))
+        for e in range(self.nelem):
+            for m in range(self.nne):
+                for i in range(self.ndim):
+                    out[e, m, i] = self._pick(self.part[self.conn[e, m], i], u, p)
+        return out
+
+    def Copy_u(self, src, dest):  # VectorPartitioned.h:198-212
+        out = _f64(dest).copy()
+        mask = self.dofs_is_u()
+        out[mask] = _f64(src)[mask]
+        return out
+
+    def Copy_p(self, src, dest):  # VectorPartitioned.h:251-263
+        out = _f64(dest).copy()
+        mask = self.dofs_is_p()
+        out[mask] = _f64(src)[mask]
+        return out
+
+
+class MatrixDiagonalPartitioned:
+    """GooseFEM::MatrixDiagonalPartitioned (MatrixDiagonalPartitioned.h:42-452)."""
+
+    def __init__(self, conn, dofs_, iip):
+        self.v = VectorPartitioned(conn, dofs_, iip)
+        self.Auu = np.zeros(self.v.nnu)
+        self.App = np.zeros(self.v.nnp)
+
+    def assemble(self, elemmat):  # MatrixDiagonalPartitioned.h:73-98
+        v = self.v
+        elemmat = _f64(elemmat)
+        self.Auu[:] = 0.0
+        self.App[:] = 0.0
+        for e in range(v.nelem):
+            for m in range(v.nne):
+                for i in range(v.ndim):
+                    d = v.part[v.conn[e, m], i]
+                    r = m * v.ndim + i
+                    if d < v.nnu:
+                        self.Auu[d] += elemmat[e, r, r]
+                    else:
+                        self.App[d - v.nnu] += elemmat[e, r, r]
+
+    def set(self, A):  # MatrixDiagonalPartitioned.h:121-135
+        A = _f64(A)
+        self.Auu = A[self.v.iiu].copy()
+        self.App = A[self.v.iip].copy()
+
+    def data(self):  # MatrixDiagonalPartitioned.h:142-156
+        out = np.zeros(self.v.ndof)
+        out[self.v.iiu] = self.Auu
+        out[self.v.iip] = self.App
+        return out
+
+    def _diag_of(self, d):
+        return self.Auu[d] if d < self.v.nnu else self.App[d - self.v.nnu]
+
+    def Dot(self, x):  # MatrixDiagonalPartitioned.h:189-224
+        v = self.v
+        x = _f64(x)
+        b = np.zeros_like(x)  # DOF numbers absent from dofs (tied-away holes) are left 0 here
+        if x.ndim == 2:
+            for m in range(v.nnode):
+                for i in range(v.ndim):
+                    b[m, i] = self._diag_of(v.part[m, i]) * x[m, i]
+        else:
+            for d in range(v.nnu):
+                b[v.iiu[d]] = self.Auu[d] * x[v.iiu[d]]
+            for d in range(v.nnp):
+                b[v.iip[d]] = self.App[d] * x[v.iip[d]]
+        return b
+
+    def Dot_u(self, x_u, x_p):  # :248-264
+        return self.Auu * _f64(x_u)
+
+    def Dot_p(self, x_u, x_p):  # :286-300
+        return self.App * _f64(x_p)
+
+    def Solve(self, b, x):  # :306-337 (x: the values kept in the prescribed entries)
+        v = self.v
+        inv_uu = 1.0 / self.Auu  # factorize, :440-452
+        b, out = _f64(b), _f64(x).copy()
+        if b.ndim == 2:
+            for m in range(v.nnode):
+                for i in range(v.ndim):
+                    if v.part[m, i] < v.nnu:
+                        out[m, i] = inv_uu[v.part[m, i]] * b[m, i]
+        else:
+            for d in range(v.nnu):
+                out[v.iiu[d]] = inv_uu[d] * b[v.iiu[d]]
+        return out
+
+    def Solve_u(self, b_u, x_p):  # :356-372
+        return (1.0 / self.Auu) * _f64(b_u)
+
+    def Reaction(self, x, b):  # :378-403 via Matrix.h:490-514
+        v = self.v
+        x, out = _f64(x), _f64(b).copy()
+        if x.ndim == 2:
+            for m in range(v.nnode):
+                for i in range(v.ndim):
+                    if v.part[m, i] >= v.nnu:
+                        out[m, i] = self.App[v.part[m, i] - v.nnu] * x[m, i]
+        else:
+            for d in range(v.nnp):
+                out[v.iip[d]] = self.App[d] * x[v.iip[d]]
+        return out
+
+    def Reaction_p(self, x_u, x_p):  # :405-420
+        return self.App * _f64(x_p)

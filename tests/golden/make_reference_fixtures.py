@@ -13,6 +13,8 @@ Sources (relative to the reference root):
                                         nodaltyings, elem2node(conn, dofsPeriodic)
   tests/basic/MatrixDiagonal.cpp:12-53  lumped mass of Regular(2,2) (Nodal quadrature)
   tests/basic/Vector.cpp:11-167         periodic AsDofs inputs and expected index pairs
+  tests/basic/VectorPartitioned.cpp:10-38          Regular(3,3), iip = arange(8): dofs_is_p
+  tests/basic/MatrixDiagonalPartitioned.cpp:12-53  Regular(2,2), iip = {0, 2}: lumped mass data()
 """
 
 from __future__ import annotations
@@ -122,6 +124,19 @@ def main(root: str) -> None:
             expected.append([int(k), [int(mm.group(1)), int(mm.group(2))]] if mm else [int(k), float(rhs)])
         vec[name] = {"input": arr, "expected": expected}
     fx["Vector_periodic_Regular_2_2"] = vec
+
+    src = open(os.path.join(basic, "VectorPartitioned.cpp")).read()
+    sec = section(src, "basic")
+    m = re.search(r"xt::xtensor<bool, 2> is_p = \{", sec)
+    lit = _braced(sec, m.end() - 1).replace("true", "True").replace("false", "False")
+    fx["VectorPartitioned_is_p_Regular_3_3"] = _cpp_literal(lit)
+
+    src = open(os.path.join(basic, "MatrixDiagonalPartitioned.cpp")).read()
+    sec = section(src, "assemble")
+    m = re.search(r"xt::xtensor<size_t, 1> iip = \{", sec)
+    iip = _cpp_literal(_braced(sec, m.end() - 1))
+    m = re.search(r"xt::xtensor<double, 1> a = \{", sec)
+    fx["MatrixDiagonalPartitioned_lumped_mass_Regular_2_2"] = {"iip": iip, "data": _cpp_literal(_braced(sec, m.end() - 1))}
 
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_fixtures.json")
     with open(out, "w") as f:
