@@ -1023,6 +1023,159 @@ class Matrix(_Handle):
         return rowptr, col, val
 
 
+class MatrixPartitioned(_Handle):
+    """GooseFEM::MatrixPartitioned (MatrixPartitioned.h:34-523): the blocks A_uu, A_up, A_pu, A_pp as
+    device CSR in the partitioned numbering.  assemble is bit-exact with the reference's triplet
+    sums; Dot / Reaction sum each row over the first block's columns, then the second's."""
+
+    _destroy = "gf_matpart_destroy"
+
+    def __init__(self, conn, dofs=None, iip=None):
+        self._vec = conn if isinstance(conn, VectorPartitioned) else VectorPartitioned(conn, dofs, iip)
+        h = _vp()
+        _call("gf_matpart_create", ctypes.byref(h), self._vec._ph)
+        self._h = h
+        nnz = (ctypes.c_int64 * 4)()
+        _call("gf_matpart_sizes", h, nnz)
+        self.nnz_blocks = tuple(nnz)
+        self.nnu, self.nnp = self._vec.nnu, self._vec.nnp
+        self.ndof = self._vec.ndof
+
+    @property
+    def iiu(self):
+        return self._vec.iiu
+
+    @property
+    def iip(self):
+        return self._vec.iip
+
+    def assemble(self, elemmat):
+        """assemble(elemmat) (MatrixPartitioned.h:156-210); FP32 element matrices are widened."""
+        f32 = _is_f32(elemmat)
+        em = to_device(elemmat, _torch().float32 if f32 else None)
+        _call("gf_matpart_assemble_f32" if f32 else "gf_matpart_assemble", self._h, _dp(em), _stream())
+
+    def csr(self, block):
+        """(rowptr, col, val) of block 0 (uu), 1 (up), 2 (pu) or 3 (pp) as GPU tensors (copies)."""
+        torch = _torch()
+        rp, cp, vp = _vp(), _vp(), _vp()
+        _call("gf_matpart_csr", self._h, _i32(block), ctypes.byref(rp), ctypes.byref(cp), ctypes.byref(vp))
+        nrows = self.nnu if block < 2 else self.nnp
+        nnz = self.nnz_blocks[block]
+        rowptr = torch.empty(nrows + 1, dtype=torch.int32, device=device())
+        col = torch.empty(nnz, dtype=torch.int32, device=device())
+        val = _empty((nnz,))
+        _hip_memcpy_d2d(rowptr, rp, 4 * (nrows + 1))
+        if nnz:
+            _hip_memcpy_d2d(col, cp, 4 * nnz)
+            _hip_memcpy_d2d(val, vp, 8 * nnz)
+        return rowptr, col, val
+
+    def _sparse(self, block):
+        torch = _torch()
+        rowptr, col, val = self.csr(block)
+        ncols = self.nnu if block in (0, 2) else self.nnp
+        return torch.sparse_csr_tensor(rowptr, col, val, (rowptr.numel() - 1, ncols))
+
+    @property
+    def data_uu(self):
+        """data_uu() (MatrixPartitioned.h:126-129) as a torch sparse CSR tensor (copy)."""
+        return self._sparse(0)
+
+    @property
+    def data_up(self):
+        return self._sparse(1)
+
+    @property
+    def data_pu(self):
+        return self._sparse(2)
+
+    @property
+    def data_pp(self):
+        return self._sparse(3)
+
+    def Todense(self):
+        """Todense() (MatrixPartitioned.h:324-350): the unpartitioned dense matrix [ndof, ndof]."""
+        full = _vp()
+        _call("gf_matpart_matrix", self._h, ctypes.byref(full))
+        out = _empty((self.ndof, self.ndof))
+        _call("gf_matrix_todense", full, _dp(out), _stream())
+        return out
+
+    def Dot(self, x, out=None):
+        """Dot(x) (MatrixPartitioned.h:354-396) of a dofval or nodevec."""
+        x = to_device(x)
+        if out is None:
+            out = _empty(tuple(x.shape))
+        _call("gf_matpart_apply", self._h, _i32(0), _i32(0 if x.dim() == 1 else 1), _dp(x), _dp(out), _stream())
+        return out
+
+    def Dot_u(self, x_u, x_p):
+        """B_u = A_uu x_u + A_up x_p."""
+        xu, xp = to_device(x_u), to_device(x_p)
+        out = _empty((self.nnu,))
+        _call("gf_matpart_dot_part", self._h, _i32(0), _dp(xu), _dp(xp), _dp(out), _stream())
+        return out
+
+    def Dot_p(self, x_u, x_p):
+        """B_p = A_pu x_u + A_pp x_p."""
+        xu, xp = to_device(x_u), to_device(x_p)
+        out = _empty((self.nnp,))
+        _call("gf_matpart_dot_part", self._h, _i32(1), _dp(xu), _dp(xp), _dp(out), _stream())
+        return out
+
+    def Reaction(self, x, b):
+        """Reaction(x, b) (Matrix.h:490-514, MatrixPartitioned.h:401-433): copy of b with its
+        prescribed entries A_pu x_u + A_pp x_p."""
+        x = to_device(x)
+        out = to_device(b).clone()
+        _call("gf_matpart_apply", self._h, _i32(1), _i32(0 if x.dim() == 1 else 1), _dp(x), _dp(out), _stream())
+        return out
+
+    def Reaction_p(self, x_u, x_p):
+        """Reaction_p(x_u, x_p) (Matrix.h:553-575, MatrixPartitioned.h:436-447)."""
+        return self.Dot_p(x_u, x_p)
+
+
+class MatrixPartitionedSolver:
+    """GooseFEM::MatrixPartitionedSolver (MatrixPartitioned.h:526-640) for a symmetric positive
+    definite A_uu: x_u = A_uu^-1 (b_u - A_up x_p) by an FP64 Jacobi-preconditioned CG on the GPU
+    (relative residual tol) instead of a sparse LDLT factorisation."""
+
+    def __init__(self, tol=1e-12, maxiter=None):
+        self.tol = tol
+        self.maxiter = maxiter
+        self.iterations = 0
+        self.relres = 0.0
+
+    def _run(self, fn, A, *args):
+        it, res = _i64(), _f64()
+        maxiter = self.maxiter or max(100, 10 * A.nnu)
+        _call(fn, A._h, *args, _f64(self.tol), _i64(maxiter), ctypes.byref(it), ctypes.byref(res), _stream())
+        self.iterations, self.relres = it.value, res.value
+
+    def Solve(self, A, b, x, warm=False):
+        """Solve(A, b, x) (MatrixPartitioned.h:583-613): copy of x with its unknown entries solved."""
+        b = to_device(b)
+        out = to_device(x).clone()
+        self._run("gf_matpart_solve", A, _i32(0 if b.dim() == 1 else 1), _dp(b), _dp(out), _i32(int(warm)))
+        return out
+
+    def solve(self, A, b, x, warm=False):
+        """solve(A, b, x): x's unknown entries solved in place (x a float64 GPU tensor)."""
+        b = to_device(b)
+        assert x.dtype == _torch().float64 and x.is_cuda and x.is_contiguous()
+        self._run("gf_matpart_solve", A, _i32(0 if b.dim() == 1 else 1), _dp(b), _dp(x), _i32(int(warm)))
+        return x
+
+    def Solve_u(self, A, b_u, x_p, x_u0=None):
+        """Solve_u(A, b_u, x_p) (MatrixPartitioned.h:545-560, :615-631); x_u0: optional initial guess."""
+        bu, xp = to_device(b_u), to_device(x_p)
+        out = _torch().zeros(A.nnu, dtype=_torch().float64, device=device()) if x_u0 is None else to_device(x_u0).clone()
+        self._run("gf_matpart_solve_u", A, _dp(bu), _dp(xp), _dp(out), _i32(0 if x_u0 is None else 1))
+        return out
+
+
 # -------------------------------------------------------------------------------------------------
 # Explicit dynamics (fused velocity-Verlet step)
 # -------------------------------------------------------------------------------------------------

@@ -28,6 +28,19 @@ struct gf_diagpart_s {
     gf_diag d = nullptr;
 };
 
+// MatrixPartitioned: the four blocks uu, up, pu, pp as CSR in the partitioned numbering (rows and
+// columns ascending), each entry a copy of the unpartitioned CSR entry it comes from (src).
+struct gf_matpart_s {
+    gf_vecpart p = nullptr;
+    gf_matrix full = nullptr;
+    int64_t nrows[4] = {0, 0, 0, 0}, nnz[4] = {0, 0, 0, 0};
+    DBuf<int32_t> rowptr[4], col[4], src[4];
+    DBuf<double> val[4];
+    DBuf<double> work; // x_u, x_p, b_u, b_p, then the PCG vectors
+    DBuf<double> red;  // reduction partials and scalars
+    DBuf<int32_t> state;
+};
+
 namespace {
 
 constexpr int BS = 256;
@@ -134,6 +147,213 @@ __global__ void k_mul_sel_nodevec(int64_t n, int64_t nnu, int which, const int32
     if (j >= n) return;
     const bool is_u = part[j] < nnu;
     if (is_u == (which == 0)) b[j] = A[dofs[j]] * x[j];
+}
+
+// block CSR values from the unpartitioned ones
+__global__ void k_take(int64_t n, const int32_t* __restrict__ src, const double* __restrict__ full,
+                       double* __restrict__ out)
+{
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k < n) out[k] = full[src[k]];
+}
+
+// out[r] = init[r] + sign * (sum over A's row r, then B's row r, of value * x[column]), one running
+// sum over ascending columns of A then of B: the order of Eigen's "dst = A xa; dst += B xb" (and
+// "dst = init; dst -= B xb") for a column-major sparse matrix times a vector.  A, B or init may be
+// absent (null).
+__global__ void k_spmv2(int64_t nrows, const double* __restrict__ init, double sign, const int32_t* __restrict__ ra,
+                        const int32_t* __restrict__ ca, const double* __restrict__ va, const double* __restrict__ xa,
+                        const int32_t* __restrict__ rb, const int32_t* __restrict__ cb, const double* __restrict__ vb,
+                        const double* __restrict__ xb, double* __restrict__ out)
+{
+    const int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (r >= nrows) return;
+    double s = init ? init[r] : 0.0;
+    if (ra)
+        for (int32_t k = ra[r]; k < ra[r + 1]; ++k) s += va[k] * (sign * xa[ca[k]]);
+    if (rb)
+        for (int32_t k = rb[r]; k < rb[r + 1]; ++k) s += vb[k] * (sign * xb[cb[k]]);
+    out[r] = s;
+}
+
+// nodevec b(m, i) from the partitioned b_u / b_p: unknown entries (which 0), prescribed (1) or all (2)
+__global__ void k_node_from_part_sel(int64_t n, int64_t nnu, int which, const int32_t* __restrict__ part,
+                                     const double* __restrict__ u, const double* __restrict__ p,
+                                     double* __restrict__ nodevec)
+{
+    const int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (j >= n) return;
+    const int64_t d = part[j];
+    if (d < nnu) {
+        if (which != 1) nodevec[j] = u[d];
+    }
+    else if (which != 0)
+        nodevec[j] = p[d - nnu];
+}
+
+// ---- Jacobi-PCG on a CSR block ----------------------------------------------------------------
+// Deterministic two-stage reductions: NPB blocks each store one partial sum, every consumer sums
+// the NPB partials in index order.  scal: [0], [1] rz by iteration parity, [2] rr, [3] bb,
+// [4] tol^2 bb.  state: [0] done, [1] iterations, [2] iteration cap.  Every kernel returns at once
+// when done is set, so launches queued past convergence leave the result alone.
+constexpr int PB = 256, NPB = 256;
+
+__device__ inline void block_sum_store(double v, double* out)
+{
+    __shared__ double sh[PB];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = PB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+    __syncthreads();
+}
+
+__device__ inline double sum_partials(const double* partial)
+{
+    double s = 0.0;
+    for (int b = 0; b < NPB; ++b) s += partial[b];
+    return s;
+}
+
+// start (x given): r = b - A x, z = r / diag, p = z; partials of r.z, r.r, b.b
+__global__ void __launch_bounds__(PB) k_cg_init(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
+                                                const double* __restrict__ vl, const double* __restrict__ b,
+                                                const double* __restrict__ x, const double* __restrict__ invd,
+                                                double* __restrict__ r, double* __restrict__ z, double* __restrict__ p,
+                                                double* __restrict__ partial)
+{
+    double arz = 0.0, arr = 0.0, abb = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB) {
+        double s = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += vl[k] * x[cl[k]];
+        const double ri = b[i] - s;
+        const double zi = invd[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        arz += ri * zi;
+        arr += ri * ri;
+        abb += b[i] * b[i];
+    }
+    block_sum_store(arz, partial + blockIdx.x);
+    block_sum_store(arr, partial + NPB + blockIdx.x);
+    block_sum_store(abb, partial + 2 * NPB + blockIdx.x);
+}
+
+__global__ void k_cg_init_scal(const double* __restrict__ partial, double* __restrict__ scal, int32_t* __restrict__ state,
+                               double tol, int32_t maxiter)
+{
+    if (threadIdx.x != 0) return;
+    const double rz = sum_partials(partial), rr = sum_partials(partial + NPB), bb = sum_partials(partial + 2 * NPB);
+    scal[0] = rz;
+    scal[2] = rr;
+    scal[3] = bb;
+    scal[4] = tol * tol * (bb > 0.0 ? bb : 1.0);
+    state[0] = rr <= scal[4] ? 1 : 0;
+    state[1] = 0;
+    state[2] = maxiter;
+}
+
+// q = A p and the partials of p.q
+__global__ void __launch_bounds__(PB) k_cg_apply(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
+                                                 const double* __restrict__ vl, const double* __restrict__ p,
+                                                 double* __restrict__ q, double* __restrict__ partial,
+                                                 const int32_t* __restrict__ state)
+{
+    if (state[0]) return;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB) {
+        double s = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += vl[k] * p[cl[k]];
+        q[i] = s;
+        acc += p[i] * s;
+    }
+    block_sum_store(acc, partial + blockIdx.x);
+}
+
+// alpha = rz / p.q; x += alpha p; r -= alpha q; z = r / diag; partials of r.z and r.r
+__global__ void __launch_bounds__(PB) k_cg_update(int64_t n, const double* __restrict__ pq_partial,
+                                                  const double* __restrict__ scal, const double* __restrict__ invd,
+                                                  const double* __restrict__ p, const double* __restrict__ q,
+                                                  double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                  double* __restrict__ partial, const int32_t* __restrict__ state,
+                                                  int64_t it)
+{
+    if (state[0]) return;
+    const double pq = sum_partials(pq_partial);
+    const double alpha = pq != 0.0 ? scal[it & 1] / pq : 0.0;
+    double arz = 0.0, arr = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        const double zi = invd[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        arz += ri * zi;
+        arr += ri * ri;
+    }
+    block_sum_store(arz, partial + blockIdx.x);
+    block_sum_store(arr, partial + NPB + blockIdx.x);
+}
+
+// beta = rz_new / rz_old; p = z + beta p.  Block 0 stores rz_new (other parity slot), r.r, the
+// iteration count, and raises done at convergence (r.r <= tol^2 b.b) or at the iteration cap.
+__global__ void __launch_bounds__(PB) k_cg_direction(int64_t n, const double* __restrict__ partial,
+                                                     double* __restrict__ scal, const double* __restrict__ z,
+                                                     double* __restrict__ p, int32_t* __restrict__ state, int64_t it)
+{
+    if (state[0]) return;
+    const double rz_old = scal[it & 1];
+    const double rz_new = sum_partials(partial);
+    const double beta = rz_old != 0.0 ? rz_new / rz_old : 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB)
+        p[i] = z[i] + beta * p[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double rr = sum_partials(partial + NPB);
+        scal[(it + 1) & 1] = rz_new;
+        scal[2] = rr;
+        state[1] = static_cast<int32_t>(it + 1);
+        if (rr <= scal[4] || it + 1 >= state[2]) state[0] = 1;
+    }
+}
+
+// |b - A x|^2 partials and b.b partials
+__global__ void __launch_bounds__(PB) k_cg_residual(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
+                                                    const double* __restrict__ vl, const double* __restrict__ b,
+                                                    const double* __restrict__ x, double* __restrict__ partial)
+{
+    double arr = 0.0, abb = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB) {
+        double s = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += vl[k] * x[cl[k]];
+        const double ri = b[i] - s;
+        arr += ri * ri;
+        abb += b[i] * b[i];
+    }
+    block_sum_store(arr, partial + blockIdx.x);
+    block_sum_store(abb, partial + NPB + blockIdx.x);
+}
+
+__global__ void k_cg_relres(const double* __restrict__ partial, double* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    const double rr = sum_partials(partial), bb = sum_partials(partial + NPB);
+    out[0] = sqrt(rr / (bb > 0.0 ? bb : 1.0));
+}
+
+// inverse diagonal of a CSR block
+__global__ void k_csr_invdiag(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
+                              const double* __restrict__ vl, double* __restrict__ invd)
+{
+    const int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (r >= n) return;
+    double d = 0.0;
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
+        if (cl[k] == r) d = vl[k];
+    invd[r] = 1.0 / d;
 }
 
 } // namespace
@@ -380,6 +600,336 @@ int gf_diagpart_apply_sel(gf_diagpart d, int op, int kind, const double* in, dou
         else {
             const int64_t n = v->nnode * v->ndim;
             if (n) k_mul_sel_nodevec<<<nblocks(n, BS), BS, 0, s>>>(n, p->nnu, op, p->part.p, v->dofs.p, M, in, out);
+        }
+        check_launch();
+    });
+}
+
+} // extern "C"
+
+// ---- MatrixPartitioned ----------------------------------------------------------------------------
+
+namespace {
+
+// work layout (doubles): x_u [nnu], x_p [nnp], b_u [nnu], b_p [nnp], then the PCG vectors
+// x, r, z, p, q, invdiag, rhs [nnu each]
+struct MatPartWork {
+    double *xu, *xp, *bu, *bp, *cx, *cr, *cz, *cp, *cq, *invd, *rhs;
+};
+
+MatPartWork work_of(gf_matpart A)
+{
+    const int64_t nu = A->p->nnu, np = A->p->nnp;
+    double* w = A->work.p;
+    MatPartWork W;
+    W.xu = w;
+    W.xp = W.xu + nu;
+    W.bu = W.xp + np;
+    W.bp = W.bu + nu;
+    W.cx = W.bp + np;
+    W.cr = W.cx + nu;
+    W.cz = W.cr + nu;
+    W.cp = W.cz + nu;
+    W.cq = W.cp + nu;
+    W.invd = W.cq + nu;
+    W.rhs = W.invd + nu;
+    return W;
+}
+
+// B_u (which 0) = A_uu x_u + A_up x_p, or B_p (which 1) = A_pu x_u + A_pp x_p
+void matpart_product(gf_matpart A, int which, const double* xu, const double* xp, double* out, hipStream_t s)
+{
+    const int b0 = which == 0 ? 0 : 2, b1 = b0 + 1;
+    const int64_t n = A->nrows[b0];
+    if (!n) return;
+    k_spmv2<<<nblocks(n, BS), BS, 0, s>>>(n, nullptr, 1.0, A->rowptr[b0].p, A->col[b0].p, A->val[b0].p, xu,
+                                          A->rowptr[b1].p, A->col[b1].p, A->val[b1].p, xp, out);
+    check_launch();
+}
+
+void vecpart_as_dofs(gf_vecpart p, int kind, const double* in, double* u, double* pp, void* stream)
+{
+    const int rc = gf_vecpart_as_dofs(p, kind, in, u, pp, stream);
+    if (rc != GF_OK) throw Error(rc, gf_last_error());
+}
+
+// x_u = A_uu^-1 rhs by Jacobi-PCG from the x_u given (warm) or from zero
+void matpart_cg(gf_matpart A, const double* rhs, double* xu, bool warm, double tol, int64_t maxiter, int64_t* iterations,
+                double* relres, hipStream_t s)
+{
+    GF_REQUIRE(tol > 0.0 && maxiter > 0 && maxiter < INT32_MAX, "tol > 0, 0 < maxiter < 2^31");
+    const int64_t n = A->nrows[0];
+    if (n == 0) {
+        if (iterations) *iterations = 0;
+        if (relres) *relres = 0.0;
+        return;
+    }
+    auto W = work_of(A);
+    double* partial = A->red.p;
+    double* scal = partial + 3 * NPB;
+    int32_t* state = A->state.p;
+    const int32_t *rp = A->rowptr[0].p, *cl = A->col[0].p;
+    const double* vl = A->val[0].p;
+    if (warm)
+        GF_HIP(hipMemcpyAsync(W.cx, xu, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    else
+        k_zero<<<nblocks(n, BS), BS, 0, s>>>(n, W.cx);
+    k_cg_init<<<NPB, PB, 0, s>>>(n, rp, cl, vl, rhs, W.cx, W.invd, W.cr, W.cz, W.cp, partial);
+    k_cg_init_scal<<<1, 64, 0, s>>>(partial, scal, state, tol, static_cast<int32_t>(maxiter));
+    check_launch();
+    int32_t st[2] = {0, 0};
+    constexpr int64_t check_every = 16;
+    GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    for (int64_t it = 0; !st[0] && it < maxiter;) {
+        for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
+            k_cg_apply<<<NPB, PB, 0, s>>>(n, rp, cl, vl, W.cp, W.cq, partial + 2 * NPB, state);
+            k_cg_update<<<NPB, PB, 0, s>>>(n, partial + 2 * NPB, scal, W.invd, W.cp, W.cq, W.cx, W.cr, W.cz, partial,
+                                           state, it);
+            k_cg_direction<<<NPB, PB, 0, s>>>(n, partial, scal, W.cz, W.cp, state, it);
+        }
+        check_launch();
+        GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipStreamSynchronize(s));
+    }
+    // the true residual ||rhs - A_uu x_u|| / ||rhs||
+    k_cg_residual<<<NPB, PB, 0, s>>>(n, rp, cl, vl, rhs, W.cx, partial);
+    k_cg_relres<<<1, 64, 0, s>>>(partial, scal + 5);
+    check_launch();
+    GF_HIP(hipMemcpyAsync(xu, W.cx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double res = 0.0;
+    GF_HIP(hipMemcpyAsync(&res, scal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GF_HIP(hipStreamSynchronize(s));
+    if (iterations) *iterations = st[1];
+    if (relres) *relres = res;
+    if (!(res <= 10.0 * tol))
+        throw Error(GF_ENOTCONV, "PCG on A_uu did not reach the tolerance (true residual " + std::to_string(res) + ")");
+}
+
+// block values from the unpartitioned CSR, and the Jacobi preconditioner of A_uu
+void matpart_split(gf_matpart A, hipStream_t s)
+{
+    const int32_t *rp = nullptr, *cl = nullptr;
+    const double* val = nullptr;
+    gf_matrix_csr(A->full, &rp, &cl, &val);
+    for (int b = 0; b < 4; ++b) {
+        const int64_t n = A->nnz[b];
+        if (n) k_take<<<nblocks(n, BS), BS, 0, s>>>(n, A->src[b].p, val, A->val[b].p);
+    }
+    const int64_t nu = A->nrows[0];
+    if (nu) k_csr_invdiag<<<nblocks(nu, BS), BS, 0, s>>>(nu, A->rowptr[0].p, A->col[0].p, A->val[0].p, work_of(A).invd);
+    check_launch();
+}
+
+} // namespace
+
+extern "C" {
+
+int gf_matpart_create(gf_matpart* out, gf_vecpart p)
+{
+    return guard([&] {
+        GF_REQUIRE(out && p, "null argument");
+        auto A = new gf_matpart_s();
+        try {
+            A->p = p;
+            int rc = gf_matrix_create(&A->full, p->v);
+            if (rc != GF_OK) throw Error(rc, gf_last_error());
+            int64_t nrows = 0, nnz = 0;
+            gf_matrix_sizes(A->full, &nrows, &nnz);
+            const int32_t *drp = nullptr, *dcl = nullptr;
+            const double* dval = nullptr;
+            gf_matrix_csr(A->full, &drp, &dcl, &dval);
+            std::vector<int32_t> rp(nrows + 1), cl(nnz);
+            GF_HIP(hipMemcpy(rp.data(), drp, (nrows + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+            if (nnz) GF_HIP(hipMemcpy(cl.data(), dcl, nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+            // renum: DOF -> partitioned number (unknowns first), -1 for holes
+            const int64_t nu = p->nnu, np = p->nnp;
+            std::vector<int64_t> renum(nrows, -1);
+            for (int64_t k = 0; k < nu; ++k) renum[p->iiu_h[k]] = k;
+            for (int64_t k = 0; k < np; ++k) renum[p->iip_h[k]] = nu + k;
+            // one row of the unpartitioned CSR per partitioned row
+            std::vector<int64_t> row_of(nu + np, -1);
+            for (int64_t d = 0; d < nrows; ++d) {
+                if (renum[d] >= 0)
+                    row_of[renum[d]] = d;
+                else
+                    GF_REQUIRE(rp[d + 1] == rp[d], "a DOF number outside iiu and iip has matrix entries");
+            }
+            A->nrows[0] = A->nrows[1] = nu;
+            A->nrows[2] = A->nrows[3] = np;
+            std::vector<int32_t> brp[4], bcl[4], bsrc[4];
+            for (int b = 0; b < 4; ++b) brp[b].assign(A->nrows[b] + 1, 0);
+            std::vector<std::pair<int64_t, int32_t>> ent; // (partitioned column, unpartitioned position)
+            for (int64_t pr = 0; pr < nu + np; ++pr) {
+                const int64_t d = row_of[pr];
+                const int rb = pr < nu ? 0 : 2;
+                const int64_t lr = pr < nu ? pr : pr - nu;
+                ent.clear();
+                if (d >= 0)
+                    for (int32_t k = rp[d]; k < rp[d + 1]; ++k) ent.emplace_back(renum[cl[k]], k);
+                std::sort(ent.begin(), ent.end());
+                for (auto& e : ent) {
+                    const int b = rb + (e.first < nu ? 0 : 1);
+                    bcl[b].push_back(static_cast<int32_t>(e.first < nu ? e.first : e.first - nu));
+                    bsrc[b].push_back(e.second);
+                }
+                brp[rb][lr + 1] = static_cast<int32_t>(bcl[rb].size());
+                brp[rb + 1][lr + 1] = static_cast<int32_t>(bcl[rb + 1].size());
+            }
+            for (int b = 0; b < 4; ++b) {
+                A->nnz[b] = static_cast<int64_t>(bcl[b].size());
+                A->rowptr[b].upload(brp[b]);
+                A->col[b].upload(bcl[b]);
+                A->src[b].upload(bsrc[b]);
+                A->val[b].alloc(A->nnz[b]);
+            }
+            A->work.alloc(2 * (nu + np) + 7 * nu);
+            A->red.alloc(3 * NPB + 8);
+            A->state.alloc(4);
+        }
+        catch (...) {
+            if (A->full) gf_matrix_destroy(A->full);
+            delete A;
+            throw;
+        }
+        *out = A;
+    });
+}
+
+int gf_matpart_destroy(gf_matpart A)
+{
+    if (A && A->full) gf_matrix_destroy(A->full);
+    delete A;
+    return GF_OK;
+}
+
+int gf_matpart_sizes(gf_matpart A, int64_t* nnz)
+{
+    return guard([&] {
+        GF_REQUIRE(A && nnz, "null argument");
+        for (int b = 0; b < 4; ++b) nnz[b] = A->nnz[b];
+    });
+}
+
+int gf_matpart_matrix(gf_matpart A, gf_matrix* full)
+{
+    return guard([&] {
+        GF_REQUIRE(A && full, "null argument");
+        *full = A->full;
+    });
+}
+
+int gf_matpart_csr(gf_matpart A, int block, const int32_t** rowptr, const int32_t** col, const double** val)
+{
+    return guard([&] {
+        GF_REQUIRE(A, "null matrix");
+        GF_REQUIRE(block >= 0 && block < 4, "block must be 0 (uu), 1 (up), 2 (pu) or 3 (pp)");
+        if (rowptr) *rowptr = A->rowptr[block].p;
+        if (col) *col = A->col[block].p;
+        if (val) *val = A->val[block].p;
+    });
+}
+
+int gf_matpart_assemble(gf_matpart A, const double* elemmat, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && elemmat, "null argument");
+        const int rc = gf_matrix_assemble(A->full, elemmat, stream);
+        if (rc != GF_OK) throw Error(rc, gf_last_error());
+        matpart_split(A, S(stream));
+    });
+}
+
+int gf_matpart_assemble_f32(gf_matpart A, const float* elemmat, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && elemmat, "null argument");
+        const int rc = gf_matrix_assemble_f32(A->full, elemmat, stream);
+        if (rc != GF_OK) throw Error(rc, gf_last_error());
+        matpart_split(A, S(stream));
+    });
+}
+
+int gf_matpart_dot_part(gf_matpart A, int which, const double* x_u, const double* x_p, double* b, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && (x_u || A->p->nnu == 0) && (x_p || A->p->nnp == 0) && (b || A->nrows[2 * which] == 0),
+                   "null argument");
+        GF_REQUIRE(which == 0 || which == 1, "which must be 0 (B_u) or 1 (B_p)");
+        matpart_product(A, which, x_u, x_p, b, S(stream));
+    });
+}
+
+int gf_matpart_apply(gf_matpart A, int op, int kind, const double* x, double* b, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && x && b, "null argument");
+        GF_REQUIRE(op == 0 || op == 1, "op must be 0 (Dot) or 1 (Reaction)");
+        GF_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (dofval) or 1 (nodevec)");
+        auto p = A->p;
+        auto W = work_of(A);
+        hipStream_t s = S(stream);
+        vecpart_as_dofs(p, kind, x, W.xu, W.xp, stream);
+        if (op == 0) matpart_product(A, 0, W.xu, W.xp, W.bu, s);
+        matpart_product(A, 1, W.xu, W.xp, W.bp, s);
+        if (kind == 0) {
+            if (op == 0) {
+                const int rc = gf_vecpart_from_partitioned(p, 0, W.bu, W.bp, b, stream);
+                if (rc != GF_OK) throw Error(rc, gf_last_error());
+            }
+            else if (p->nnp)
+                k_scatter<<<nblocks(p->nnp, BS), BS, 0, s>>>(p->nnp, p->iip.p, W.bp, b);
+        }
+        else {
+            const int64_t n = p->v->nnode * p->v->ndim;
+            if (n) k_node_from_part_sel<<<nblocks(n, BS), BS, 0, s>>>(n, p->nnu, op == 0 ? 2 : 1, p->part.p, W.bu, W.bp, b);
+        }
+        check_launch();
+    });
+}
+
+int gf_matpart_solve_u(gf_matpart A, const double* b_u, const double* x_p, double* x_u, int warm, double tol,
+                       int64_t maxiter, int64_t* iterations, double* relres, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && ((b_u && x_u) || A->p->nnu == 0) && (x_p || A->p->nnp == 0), "null argument");
+        auto W = work_of(A);
+        hipStream_t s = S(stream);
+        const int64_t nu = A->nrows[0];
+        // rhs = b_u - A_up x_p
+        if (nu)
+            k_spmv2<<<nblocks(nu, BS), BS, 0, s>>>(nu, b_u, -1.0, nullptr, nullptr, nullptr, nullptr, A->rowptr[1].p,
+                                                   A->col[1].p, A->val[1].p, x_p, W.rhs);
+        check_launch();
+        matpart_cg(A, W.rhs, x_u, warm != 0, tol, maxiter, iterations, relres, s);
+    });
+}
+
+int gf_matpart_solve(gf_matpart A, int kind, const double* b, double* x, int warm, double tol, int64_t maxiter,
+                     int64_t* iterations, double* relres, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && b && x, "null argument");
+        GF_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (dofval) or 1 (nodevec)");
+        auto p = A->p;
+        auto W = work_of(A);
+        hipStream_t s = S(stream);
+        const int64_t nu = A->nrows[0];
+        // B_u from b, X_u (the warm start) and X_p from x; rhs = B_u - A_up X_p
+        vecpart_as_dofs(p, kind, b, W.bu, nullptr, stream);
+        vecpart_as_dofs(p, kind, x, W.xu, W.xp, stream);
+        if (nu)
+            k_spmv2<<<nblocks(nu, BS), BS, 0, s>>>(nu, W.bu, -1.0, nullptr, nullptr, nullptr, nullptr, A->rowptr[1].p,
+                                                   A->col[1].p, A->val[1].p, W.xp, W.rhs);
+        check_launch();
+        matpart_cg(A, W.rhs, W.xu, warm != 0, tol, maxiter, iterations, relres, s);
+        if (kind == 0) {
+            if (nu) k_scatter<<<nblocks(nu, BS), BS, 0, s>>>(nu, p->iiu.p, W.xu, x);
+        }
+        else {
+            const int64_t n = p->v->nnode * p->v->ndim;
+            if (n) k_node_from_part_sel<<<nblocks(n, BS), BS, 0, s>>>(n, p->nnu, 0, p->part.p, W.xu, W.xp, x);
         }
         check_launch();
     });

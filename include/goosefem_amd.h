@@ -55,6 +55,7 @@ typedef struct gf_explicit_s* gf_explicit;
 typedef struct gf_pcg_s* gf_pcg;
 typedef struct gf_vecpart_s* gf_vecpart;
 typedef struct gf_diagpart_s* gf_diagpart;
+typedef struct gf_matpart_s* gf_matpart;
 
 const char* gf_last_error(void);
 int gf_version(int* major, int* minor, int* patch);
@@ -324,6 +325,51 @@ int gf_matrix_dot_dofval(gf_matrix A, const double* x, double* b, void* stream);
 int gf_matrix_dot_nodevec(gf_matrix A, const double* x, double* b, void* stream);
 /* Todense()  (Matrix.h:338-355, :790-802): out [nrows, nrows] */
 int gf_matrix_todense(gf_matrix A, double* out, void* stream);
+
+/* ================================================================================================
+ * MatrixPartitioned / MatrixPartitionedSolver  (include/GooseFEM/MatrixPartitioned.h:34-640)
+ * The blocks A_uu, A_up, A_pu, A_pp are CSR in the partitioned numbering (iiu then iip, columns
+ * ascending).  Every block entry is a copy of the unpartitioned CSR entry (gf_matrix over the same
+ * Vector) it comes from, so it holds the same element contributions summed in the same
+ * (e, m, i, n, j) order as setFromTriplets (:156-210): assembly is bit-exact.  Products run one
+ * sum per row over ascending columns of the first block, then of the second: the order Eigen
+ * evaluates "A_uu X_u + A_up X_p" for a column-major matrix (dst = A X_u; dst += B X_p).
+ * Solve factorises nothing: x_u = A_uu^-1 (b_u - A_up x_p) by a Jacobi-preconditioned CG on A_uu
+ * (FP64, deterministic reductions), so A_uu must be symmetric positive definite (the reference's
+ * SimplicialLDLT also takes indefinite symmetric matrices).  set() / add() of dense sub-matrices
+ * are not provided (the sparsity pattern is the mesh's).
+ * ============================================================================================== */
+
+/* MatrixPartitioned(conn, dofs, iip)  (MatrixPartitioned.h:87-121) over p's Vector and iip.  Holds
+ * an unpartitioned gf_matrix (gf_matpart_matrix: Todense, :324-350, is gf_matrix_todense on it) and
+ * the four blocks; the pattern is built on the host once. */
+int gf_matpart_create(gf_matpart* out, gf_vecpart p);
+int gf_matpart_destroy(gf_matpart A);
+/* nnz of the blocks uu, up, pu, pp (data_uu() ... data_pp(), :126-153) */
+int gf_matpart_sizes(gf_matpart A, int64_t* nnz4);
+int gf_matpart_matrix(gf_matpart A, gf_matrix* full);
+/* device CSR of block 0 (uu), 1 (up), 2 (pu) or 3 (pp); rows nnu or nnp; outputs may be NULL */
+int gf_matpart_csr(gf_matpart A, int block, const int32_t** rowptr, const int32_t** col, const double** val);
+/* assemble(elemmat)  (MatrixPartitioned.h:156-210); the FP32 form widens each entry to FP64 */
+int gf_matpart_assemble(gf_matpart A, const double* elemmat, void* stream);
+int gf_matpart_assemble_f32(gf_matpart A, const float* elemmat, void* stream);
+/* which 0: b = A_uu x_u + A_up x_p  (B_u of :378-396);  which 1: b = A_pu x_u + A_pp x_p
+ * (Reaction_p, :436-447) */
+int gf_matpart_dot_part(gf_matpart A, int which, const double* x_u, const double* x_p, double* b, void* stream);
+/* Dot (op 0; :354-396, every DOF of b written, entries of DOF numbers outside iiu and iip set to 0)
+ * or Reaction (op 1; :401-433, only the prescribed entries of b written) of a dofval (kind 0) or
+ * nodevec (kind 1) */
+int gf_matpart_apply(gf_matpart A, int op, int kind, const double* x, double* b, void* stream);
+/* MatrixPartitionedSolver::Solve_u  (:545-560, :615-631): x_u = A_uu^-1 (b_u - A_up x_p).  The CG
+ * starts from x_u's content when warm != 0, else from zero, and stops when the recursive residual
+ * ||r|| <= tol ||b_u - A_up x_p||; *relres receives the true relative residual, *iterations the CG
+ * iterations (either may be NULL).  Returns GF_ENOTCONV if the true residual exceeds 10 tol. */
+int gf_matpart_solve_u(gf_matpart A, const double* b_u, const double* x_p, double* x_u, int warm, double tol,
+                       int64_t maxiter, int64_t* iterations, double* relres, void* stream);
+/* MatrixPartitionedSolver::Solve / solve  (:583-613): reads b's unknown and x's prescribed entries,
+ * writes x's unknown entries (dofval kind 0, nodevec kind 1); warm: start from x's unknown entries */
+int gf_matpart_solve(gf_matpart A, int kind, const double* b, double* x, int warm, double tol, int64_t maxiter,
+                     int64_t* iterations, double* relres, void* stream);
 
 /* ================================================================================================
  * Explicit dynamics: one fused velocity-Verlet step of the caller loop

@@ -788,3 +788,141 @@ class MatrixDiagonalPartitioned:
 
     def Reaction_p(self, x_u, x_p):  # :405-420
         return self.App * _f64(x_p)
+
+
+class MatrixPartitioned:
+    """GooseFEM::MatrixPartitioned (MatrixPartitioned.h:34-523) and the solve of
+    MatrixPartitionedSolver (:526-640), restated with pure-Python loops (small meshes only).
+
+    Blocks 0..3 = uu, up, pu, pp, each as {(row, col): value} in the partitioned numbering."""
+
+    def __init__(self, conn, dofs_, iip):
+        self.v = VectorPartitioned(conn, dofs_, iip)
+        self.blocks = [{}, {}, {}, {}]
+
+    def _block_of(self, di, dj):
+        nu = self.v.nnu
+        b = (2 if di >= nu else 0) + (1 if dj >= nu else 0)
+        return b, (di - nu if di >= nu else di, dj - nu if dj >= nu else dj)
+
+    def assemble(self, elemmat):
+        """:156-210: triplets in (e, m, i, n, j) order; setFromTriplets sums duplicates in that order."""
+        v = self.v
+        elemmat = _f64(elemmat)
+        self.blocks = [{}, {}, {}, {}]
+        nd = v.ndim
+        for e in range(v.nelem):
+            for m in range(v.nne):
+                for i in range(nd):
+                    di = v.part[v.conn[e, m], i]
+                    for n in range(v.nne):
+                        for j in range(nd):
+                            dj = v.part[v.conn[e, n], j]
+                            b, key = self._block_of(di, dj)
+                            val = elemmat[e, m * nd + i, n * nd + j]
+                            blk = self.blocks[b]
+                            blk[key] = blk[key] + val if key in blk else val
+
+    def csr(self, b):
+        """Block b as CSR (rowptr, col, val): rows and columns ascending."""
+        nrows = self.v.nnu if b < 2 else self.v.nnp
+        keys = sorted(self.blocks[b])
+        rowptr = np.zeros(nrows + 1, dtype=np.int64)
+        for r, _ in keys:
+            rowptr[r + 1] += 1
+        rowptr = np.cumsum(rowptr)
+        col = np.array([c for _, c in keys], dtype=np.int64)
+        val = np.array([self.blocks[b][k] for k in keys], dtype=np.float64)
+        return rowptr, col, val
+
+    def _rows(self, b):
+        nrows = self.v.nnu if b < 2 else self.v.nnp
+        rows = [[] for _ in range(nrows)]
+        for (r, c) in sorted(self.blocks[b]):
+            rows[r].append((c, self.blocks[b][(r, c)]))
+        return rows
+
+    def _product(self, first, second, xa, xb, init=None, sign=1.0):
+        """Row sums over ascending columns of block `first`, then of block `second`, one running sum
+        per row: Eigen's "dst = A xa; dst += B xb" for column-major A, B (SparseDenseProduct's
+        res(i) += value * rhs(j), columns in order)."""
+        ra, rb = self._rows(first), self._rows(second)
+        out = np.empty(len(ra))
+        for r in range(len(ra)):
+            s = 0.0 if init is None else init[r]
+            for c, a in ra[r]:
+                s += a * (sign * xa[c])
+            for c, a in rb[r]:
+                s += a * (sign * xb[c])
+            out[r] = s
+        return out
+
+    def Dot_u(self, x_u, x_p):  # B_u of :378-396
+        return self._product(0, 1, _f64(x_u), _f64(x_p))
+
+    def Dot_p(self, x_u, x_p):  # B_p of :378-396; Reaction_p, :436-447
+        return self._product(2, 3, _f64(x_u), _f64(x_p))
+
+    Reaction_p = Dot_p
+
+    def Todense(self):  # :324-350
+        v = self.v
+        out = np.zeros((v.ndof, v.ndof))
+        rows = (v.iiu, v.iiu, v.iip, v.iip)
+        cols = (v.iiu, v.iip, v.iiu, v.iip)
+        for b in range(4):
+            for (r, c), a in self.blocks[b].items():
+                out[rows[b][r], cols[b][c]] = a
+        return out
+
+    def Dot(self, x):  # :354-396 (entries of DOF numbers outside iiu and iip: 0)
+        v = self.v
+        x = _f64(x)
+        bu = self.Dot_u(v.AsDofs_u(x), v.AsDofs_p(x))
+        bp = self.Dot_p(v.AsDofs_u(x), v.AsDofs_p(x))
+        if x.ndim == 2:
+            return v.NodeFromPartitioned(bu, bp)
+        return v.DofsFromParitioned(bu, bp)
+
+    def Reaction(self, x, b):  # :401-433 via Matrix.h:490-514
+        v = self.v
+        x, out = _f64(x), _f64(b).copy()
+        bp = self.Dot_p(v.AsDofs_u(x), v.AsDofs_p(x))
+        if x.ndim == 2:
+            for m in range(v.nnode):
+                for i in range(v.ndim):
+                    if v.part[m, i] >= v.nnu:
+                        out[m, i] = bp[v.part[m, i] - v.nnu]
+        else:
+            out[v.iip] = bp
+        return out
+
+    def Solve_u(self, b_u, x_p):
+        """:615-631: x_u = A_uu^-1 (b_u - A_up x_p); the dense LU solve stands in for the sparse LDLT
+        (both are direct solves: they agree to rounding)."""
+        v = self.v
+        rows = self._rows(1)
+        rhs = np.empty(v.nnu)
+        b_u, x_p = _f64(b_u), _f64(x_p)
+        for r in range(v.nnu):
+            s = b_u[r]
+            for c, a in rows[r]:
+                s -= a * x_p[c]
+            rhs[r] = s
+        Auu = np.zeros((v.nnu, v.nnu))
+        for (r, c), a in self.blocks[0].items():
+            Auu[r, c] = a
+        return np.linalg.solve(Auu, rhs) if v.nnu else rhs
+
+    def Solve(self, b, x):  # :583-613: unknown entries of a copy of x
+        v = self.v
+        b, out = _f64(b), _f64(x).copy()
+        xu = self.Solve_u(v.AsDofs_u(b), v.AsDofs_p(out))
+        if b.ndim == 2:
+            for m in range(v.nnode):
+                for i in range(v.ndim):
+                    if v.part[m, i] < v.nnu:
+                        out[m, i] = xu[v.part[m, i]]
+        else:
+            out[v.iiu] = xu
+        return out

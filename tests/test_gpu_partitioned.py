@@ -1,13 +1,14 @@
-"""GPU parity of VectorPartitioned / MatrixDiagonalPartitioned (gf_vecpart_*, gf_diagpart_*) with the
-oracle's restatement: bit-exact (gathers, scatters, per-DOF products), on the reference's own tied
-Dirichlet setup (tests/basic/VectorPartitioned.cpp:40-95) and on a periodic Hex8 cube, plus the
-reference's golden arrays."""
+"""GPU parity of VectorPartitioned / MatrixDiagonalPartitioned / MatrixPartitioned (gf_vecpart_*,
+gf_diagpart_*, gf_matpart_*) with the oracle's restatement: bit-exact for gathers, scatters, per-DOF
+products, the sparse blocks and their products, on the reference's own tied Dirichlet setup
+(tests/basic/VectorPartitioned.cpp:40-95) and on a periodic Hex8 cube, plus the reference's golden
+arrays and round trips; the iterative solve within a stated tolerance of the oracle's direct one."""
 
 import numpy as np
 import pytest
 
 import oracle
-from oracle import HEX8, QUAD4
+from oracle import HEX8, QUAD4, QUAD4_PLANAR
 
 pytestmark = pytest.mark.gpu
 
@@ -128,3 +129,131 @@ def test_partitioned_reference_golden(gf, fixtures):
     X = host(A.Solve(B))
     X[g["iip"]] = x[g["iip"]]
     assert np.allclose(B, a * x) and np.allclose(X, x)
+
+
+def elastic_stiffness(et, conn, coor, seed=0):
+    """Element stiffness of a linear elastic body with random shear moduli (FP64, the oracle's)."""
+    nd = coor.shape[1]
+    x = oracle.Vector(conn, oracle.dofs(coor.shape[0], nd)).AsElement(coor)
+    q = oracle.Quadrature(et, x)
+    rng = np.random.default_rng(seed)
+    G = np.exp(0.4 * rng.standard_normal((q.nelem, q.nip)))
+    return q.Int_gradN_dot_tensor4_dot_gradNT_dV(oracle.elastic_tangent(np.full_like(G, 10.0), G))
+
+
+def random_symmetric(rng, nelem, k, spd=False):
+    a = rng.random((nelem, k, k))
+    a = 0.5 * (a + a.transpose(0, 2, 1))
+    if spd:
+        a = a @ a.transpose(0, 2, 1) + k * np.eye(k)
+    return a
+
+
+def no_prescribed():
+    coor, conn = oracle.hex8_regular(3, 2, 2)
+    return coor, conn, oracle.dofs(coor.shape[0], 3), np.zeros(0, dtype=np.int64)
+
+
+@pytest.mark.parametrize("setup", [quad4_tied, hex8_periodic, no_prescribed])
+def test_matrix_partitioned_bit_exact(gf, setup):
+    # assemble (MatrixPartitioned.h:156-210): the four CSR blocks, Todense, Dot / Dot_u / Dot_p,
+    # Reaction / Reaction_p on dofval and nodevec, all bit-exact with the oracle's restatement
+    coor, conn, dofs, iip = setup()
+    rng = np.random.default_rng(7)
+    K = conn.shape[1] * dofs.shape[1]
+    em = random_symmetric(rng, conn.shape[0], K)
+    oA = oracle.MatrixPartitioned(conn, dofs, iip)
+    oA.assemble(em)
+    gA = gf.MatrixPartitioned(conn, dofs, iip)
+    gA.assemble(em)
+    for b in range(4):
+        rp, cl, val = (host(t) for t in gA.csr(b))
+        orp, ocl, oval = oA.csr(b)
+        assert np.array_equal(rp, orp) and np.array_equal(cl, ocl) and np.array_equal(val, oval), b
+    assert gA.nnz_blocks == tuple(len(oA.blocks[b]) for b in range(4))
+    assert np.array_equal(host(gA.Todense()), oA.Todense())
+    v = oA.v
+    x = rng.standard_normal(v.ndof)
+    xn = rng.standard_normal((coor.shape[0], dofs.shape[1]))
+    for arg in (x, xn):
+        assert np.array_equal(host(gA.Dot(arg)), oA.Dot(arg))
+        keep = rng.standard_normal(arg.shape)
+        assert np.array_equal(host(gA.Reaction(arg, keep)), oA.Reaction(arg, keep))
+    x_u, x_p = rng.standard_normal(v.nnu), rng.standard_normal(v.nnp)
+    if v.nnp:
+        assert np.array_equal(host(gA.Dot_u(x_u, x_p)), oA.Dot_u(x_u, x_p))
+        assert np.array_equal(host(gA.Dot_p(x_u, x_p)), oA.Dot_p(x_u, x_p))
+        assert np.array_equal(host(gA.Reaction_p(x_u, x_p)), oA.Reaction_p(x_u, x_p))
+    # the products agree with the dense matrix to rounding
+    Ad = oA.Todense()
+    assert np.allclose(host(gA.Dot(x)), Ad @ x, rtol=1e-13, atol=1e-12)
+
+
+@pytest.mark.parametrize("setup,et", [(quad4_tied, QUAD4_PLANAR), (hex8_periodic, HEX8)])
+def test_matrix_partitioned_solve(gf, setup, et):
+    # MatrixPartitionedSolver::Solve / Solve_u (MatrixPartitioned.h:545-631) on an elastic stiffness
+    # with Dirichlet DOFs: x_u = A_uu^-1 (b_u - A_up x_p) by the device CG, against the oracle's
+    # direct solve.  Tolerance: CG stops at ||r|| <= 1e-12 ||b||; the error bound is cond(A_uu) times
+    # that, cond(A_uu) < 500 for both meshes here, so 1e-7 relative is loose.
+    coor, conn, dofs, iip = setup()
+    em = elastic_stiffness(et, conn, coor)
+    oA = oracle.MatrixPartitioned(conn, dofs, iip)
+    oA.assemble(em)
+    gA = gf.MatrixPartitioned(conn, dofs, iip)
+    gA.assemble(em)
+    v = oA.v
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(v.ndof)
+    b = oA.Dot(x)
+    solver = gf.MatrixPartitionedSolver(tol=1e-12)
+    for bb, xx in ((b, x), (v.NodeFromPartitioned(v.AsDofs_u(b), v.AsDofs_p(b)), v.NodeFromPartitioned(v.AsDofs_u(x), v.AsDofs_p(x)))):
+        ref = oA.Solve(bb, xx)
+        got = host(solver.Solve(gA, bb, xx))
+        assert solver.relres <= 1e-11 and solver.iterations > 0
+        assert np.max(np.abs(got - ref)) <= 1e-7 * np.max(np.abs(ref))
+        assert np.max(np.abs(got - xx)) <= 1e-7 * np.max(np.abs(xx))  # the round trip of Dot
+    b_u, x_p = v.AsDofs_u(b), v.AsDofs_p(x)
+    xu = host(solver.Solve_u(gA, b_u, x_p))
+    ref_u = oA.Solve_u(b_u, x_p)
+    assert np.max(np.abs(xu - ref_u)) <= 1e-7 * np.max(np.abs(ref_u))
+    # warm start from the solution: converged before the first iteration, x_u returned unchanged
+    xw = host(solver.Solve_u(gA, b_u, x_p, x_u0=ref_u))
+    assert solver.iterations == 0 and np.array_equal(xw, ref_u)
+
+
+def test_matrix_partitioned_reference_cases(gf):
+    # tests/basic/MatrixParitioned.cpp:11-44 (Regular(2, 2), iip {0, 5, 7, 13}) and :46-108 (one
+    # 10 x 10 matrix on dofs arange(10), iip {0, 2, 4}), with symmetric positive definite matrices:
+    # the device solve is a CG, which needs A_uu SPD (the reference's LDLT takes any symmetric A)
+    rng = np.random.default_rng(0)
+    m = gf.Mesh.Quad4.Regular(2, 2)
+    A = gf.MatrixPartitioned(m.conn, m.dofs, np.array([0, 5, 7, 13]))
+    A.assemble(random_symmetric(rng, m.nelem, 8, spd=True))
+    solver = gf.MatrixPartitionedSolver()
+    x = rng.random(m.nnode * 2)
+    assert np.allclose(host(solver.Solve(A, A.Dot(x), x)), x)
+    a = rng.random((10, 10))
+    Ad = a @ a.T + 10.0 * np.eye(10)
+    K = gf.MatrixPartitioned(np.arange(5).reshape(1, 5), np.arange(10).reshape(5, 2), np.array([0, 2, 4]))
+    K.assemble(Ad[None])
+    assert np.array_equal(host(K.Todense()), Ad)
+    x = rng.random(10)
+    assert np.allclose(host(K.Dot(x)), Ad @ x)
+    assert np.allclose(host(solver.Solve(K, K.Dot(x), x)), x)
+    xn = x.reshape(5, 2)
+    assert np.allclose(host(K.Dot(xn)), (Ad @ x).reshape(5, 2))
+    assert np.allclose(host(solver.Solve(K, K.Dot(xn), xn)), xn)
+
+
+def test_matrix_partitioned_fp32_assembly(gf):
+    # FP32 element matrices: each entry widened to FP64, then the FP64 assembly (bit-exact with the
+    # FP64 path on the widened input)
+    coor, conn, dofs, iip = quad4_tied()
+    rng = np.random.default_rng(3)
+    em32 = random_symmetric(rng, conn.shape[0], 8).astype(np.float32)
+    a32 = gf.MatrixPartitioned(conn, dofs, iip)
+    a32.assemble(em32)
+    a64 = gf.MatrixPartitioned(conn, dofs, iip)
+    a64.assemble(em32.astype(np.float64))
+    for b in range(4):
+        assert np.array_equal(host(a32.csr(b)[2]), host(a64.csr(b)[2]))

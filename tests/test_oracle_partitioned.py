@@ -1,9 +1,12 @@
-"""Pins the CPU restatement of VectorPartitioned / MatrixDiagonalPartitioned (oracle.VectorPartitioned,
-oracle.MatrixDiagonalPartitioned) against the reference's own tests: the golden arrays of
-tests/basic/VectorPartitioned.cpp:10-38 and tests/basic/MatrixDiagonalPartitioned.cpp:12-53, the
-round trips of VectorPartitioned.cpp:40-95 and MatrixDiagonalPartitioned.cpp:55-94, and the
-identities that fix the rest (a permutation round trip, agreement with the unpartitioned Vector and
-MatrixDiagonal restatements)."""
+"""Pins the CPU restatement of VectorPartitioned / MatrixDiagonalPartitioned / MatrixPartitioned
+(oracle.VectorPartitioned, oracle.MatrixDiagonalPartitioned, oracle.MatrixPartitioned) against the
+reference's own tests: the golden arrays of tests/basic/VectorPartitioned.cpp:10-38 and
+tests/basic/MatrixDiagonalPartitioned.cpp:12-53, the round trips of VectorPartitioned.cpp:40-95,
+MatrixDiagonalPartitioned.cpp:55-94 and MatrixParitioned.cpp:11-108, and the identities that fix the
+rest (a permutation round trip, agreement with the unpartitioned Vector, MatrixDiagonal and Matrix
+restatements, dense products).  The per-row summation order of MatrixPartitioned's products (first
+block's columns, then the second's, one running sum) follows Eigen's evaluation of the reference's
+expressions; no reference output pins those last bits, the tests pin the products to rounding."""
 
 import numpy as np
 
@@ -99,3 +102,83 @@ def test_matrix_dot_solve_reaction():
     plain = oracle.Vector(conn, dofs)
     xn = plain.AsNode(x)
     assert np.array_equal(A.Dot(xn), plain.AsNode(a * x))
+
+
+def random_symmetric(rng, nelem, k, spd=False):
+    a = rng.random((nelem, k, k))
+    a = 0.5 * (a + a.transpose(0, 2, 1))
+    if spd:
+        a = a @ a.transpose(0, 2, 1) + k * np.eye(k)
+    return a
+
+
+def test_matrix_partitioned_blocks_are_the_unpartitioned_entries():
+    # every block entry is the unpartitioned Matrix entry (Matrix.h:704-727) of its DOF pair, bit for
+    # bit (both sum the same triplets in the same (e, m, i, n, j) order), and nothing is lost
+    coor, conn, dofs, iip, _ = tied_dirichlet_9x9()
+    rng = np.random.default_rng(2)
+    K = conn.shape[1] * dofs.shape[1]
+    em = random_symmetric(rng, conn.shape[0], K)
+    A = oracle.MatrixPartitioned(conn, dofs, iip)
+    A.assemble(em)
+    rowptr, col, val = oracle.matrix_assemble(conn, dofs, em)
+    full = {}
+    for r in range(rowptr.size - 1):
+        for k in range(rowptr[r], rowptr[r + 1]):
+            full[(r, col[k])] = val[k]
+    v = A.v
+    rows = (v.iiu, v.iiu, v.iip, v.iip)
+    cols = (v.iiu, v.iip, v.iiu, v.iip)
+    n = 0
+    for b in range(4):
+        for (r, c), a in A.blocks[b].items():
+            assert a == full[(rows[b][r], cols[b][c])]
+        n += len(A.blocks[b])
+    assert n == len(full)
+    assert np.array_equal(A.Todense(), oracle_dense(rowptr, col, val, v.ndof))
+
+
+def oracle_dense(rowptr, col, val, n):
+    out = np.zeros((n, n))
+    for r in range(n):
+        out[r, col[rowptr[r]:rowptr[r + 1]]] = val[rowptr[r]:rowptr[r + 1]]
+    return out
+
+
+def test_matrix_partitioned_reference_solve():
+    # tests/basic/MatrixParitioned.cpp:11-44: Regular(2, 2), iip {0, 5, 7, 13}, random symmetric
+    # element matrices; Solve(A, Dot(x), x) returns x
+    coor, conn = oracle.quad4_regular(2, 2)
+    dofs = oracle.dofs(coor.shape[0], 2)
+    rng = np.random.default_rng(0)
+    A = oracle.MatrixPartitioned(conn, dofs, np.array([0, 5, 7, 13]))
+    A.assemble(random_symmetric(rng, conn.shape[0], 8))
+    x = rng.random(dofs.size)
+    b = A.Dot(x)
+    assert np.allclose(b, A.Todense() @ x, rtol=1e-14, atol=1e-14)
+    assert np.allclose(A.Solve(b, x), x)
+
+
+def test_matrix_partitioned_dense_matrix():
+    # tests/basic/MatrixParitioned.cpp:46-108 fill a 10 x 10 A = a + a^T on dofs arange(10) with
+    # iip {0, 2, 4}; the same matrix enters here as one 5-node element's matrix
+    rng = np.random.default_rng(4)
+    a = rng.random((10, 10))
+    Ad = a + a.T
+    conn = np.arange(5).reshape(1, 5)
+    dofs = np.arange(10).reshape(5, 2)
+    K = oracle.MatrixPartitioned(conn, dofs, np.array([0, 2, 4]))
+    K.assemble(Ad[None])
+    assert np.array_equal(K.Todense(), Ad)
+    x = rng.random(10)
+    assert np.allclose(K.Dot(x), Ad @ x, rtol=1e-14, atol=1e-14)
+    assert np.allclose(K.Solve(K.Dot(x), x), x)
+    xn = x.reshape(5, 2)
+    assert np.allclose(K.Dot(xn), (Ad @ x).reshape(5, 2), rtol=1e-14, atol=1e-14)
+    assert np.allclose(K.Solve(K.Dot(xn), xn), xn)
+    # Reaction: the prescribed rows of A x; Dot_u / Dot_p are the row blocks of A x
+    v = K.v
+    R = K.Reaction(x, np.zeros(10))
+    assert np.allclose(R[v.iip], (Ad @ x)[v.iip]) and np.all(R[v.iiu] == 0.0)
+    assert np.allclose(K.Dot_u(x[v.iiu], x[v.iip]), (Ad @ x)[v.iiu])
+    assert np.allclose(K.Reaction_p(x[v.iiu], x[v.iip]), (Ad @ x)[v.iip])
