@@ -195,6 +195,38 @@ struct dev_out {
     }
 };
 
+/* In-out argument: host containers are uploaded, written on the device and copied back in
+ * finish(); device arrays are used in place. */
+template <class R>
+struct dev_inout {
+    device_array<double> tmp;
+    double* ptr = nullptr;
+    R& ret;
+    explicit dev_inout(R& r) : ret(r)
+    {
+        if constexpr (is_device<R>::value) {
+            ptr = r.data();
+        }
+        else {
+            tmp = device_array<double>(shape_of(r));
+            check(gf_memcpy_h2d(tmp.data(), r.data(), static_cast<int64_t>(tmp.bytes())));
+            ptr = tmp.data();
+        }
+    }
+    void finish()
+    {
+        if constexpr (!is_device<R>::value) check(gf_memcpy_d2h(ret.data(), tmp.data(), static_cast<int64_t>(tmp.bytes())));
+        else check(gf_stream_sync(nullptr));
+    }
+};
+
+/* Compressed sparse rows (in place of the reference's Eigen::SparseMatrix). */
+struct SparseMatrix {
+    size_t rows = 0, cols = 0;
+    std::vector<int64_t> rowptr, col;
+    std::vector<double> val;
+};
+
 template <class C>
 std::vector<int64_t> to_i64(const C& c)
 {
@@ -927,16 +959,401 @@ private:
 };
 
 // =================================================================================================
+// Dirichlet partitioning: VectorPartitioned (include/GooseFEM/VectorPartitioned.h),
+// MatrixDiagonalPartitioned (MatrixDiagonalPartitioned.h), MatrixPartitioned and
+// MatrixPartitionedSolver (MatrixPartitioned.h)
+// =================================================================================================
+
+/* VectorPartitioned(conn, dofs, iip)  (VectorPartitioned.h:72-661) */
+class VectorPartitioned : public Vector {
+public:
+    template <class C, class D, class P>
+    VectorPartitioned(const C& conn, const D& dofs, const P& iip) : Vector(conn, dofs)
+    {
+        auto p = amd::to_i64(iip);
+        amd::check(gf_vecpart_create(&m_ph, handle(), p.data(), static_cast<int64_t>(p.size())));
+        int64_t nu = 0, np = 0;
+        amd::check(gf_vecpart_sizes(m_ph, &nu, &np));
+        m_nnu = static_cast<size_t>(nu);
+        m_nnp = static_cast<size_t>(np);
+        std::vector<int64_t> iiu(m_nnu), iipv(m_nnp), part(nnode() * ndim());
+        amd::check(gf_vecpart_index(m_ph, iiu.data(), iipv.data(), part.data()));
+        m_iiu = amd::from_i64(iiu, {m_nnu});
+        m_iip = amd::from_i64(iipv, {m_nnp});
+        m_part = amd::from_i64(part, {nnode(), ndim()});
+    }
+    ~VectorPartitioned() { gf_vecpart_destroy(m_ph); }
+    size_t nnu() const { return m_nnu; }
+    size_t nnp() const { return m_nnp; }
+    const amd::array<size_t>& iiu() const { return m_iiu; }
+    const amd::array<size_t>& iip() const { return m_iip; }
+    /* the reordered DOFs: unknowns numbered [0, nnu), prescribed [nnu, nnu + nnp) */
+    const amd::array<size_t>& part() const { return m_part; }
+    gf_vecpart part_handle() const { return m_ph; }
+
+    /* AsDofs_u / AsDofs_p of a dofval, nodevec or elemvec  (VectorPartitioned.h:428-661) */
+    template <class T>
+    amd::array<double> AsDofs_u(const T& arg) const { return as_dofs(arg, true); }
+    template <class T>
+    amd::array<double> AsDofs_p(const T& arg) const { return as_dofs(arg, false); }
+    /* DofsFromParitioned / NodeFromPartitioned / ElementFromPartitioned  (VectorPartitioned.h:276-424) */
+    template <class U, class P>
+    amd::array<double> DofsFromParitioned(const U& u, const P& p) const { return from_partitioned(0, u, p, shape_dofval()); }
+    template <class U, class P>
+    amd::array<double> NodeFromPartitioned(const U& u, const P& p) const { return from_partitioned(1, u, p, shape_nodevec()); }
+    template <class U, class P>
+    amd::array<double> ElementFromPartitioned(const U& u, const P& p) const
+    {
+        return from_partitioned(2, u, p, shape_elemvec());
+    }
+    /* Copy_u / Copy_p (VectorPartitioned.h:198-263): dest with its unknown / prescribed entries from src */
+    template <class S, class T>
+    amd::array<double> Copy_u(const S& src, const T& dest) const { return copy(0, src, dest); }
+    template <class S, class T>
+    amd::array<double> Copy_p(const S& src, const T& dest) const { return copy(1, src, dest); }
+
+private:
+    template <class T>
+    static int kind_of(const T& arg) { return static_cast<int>(arg.dimension()) - 1; }
+    template <class T>
+    amd::array<double> as_dofs(const T& arg, bool u) const
+    {
+        amd::array<double> r({u ? m_nnu : m_nnp});
+        amd::dev_in<T> in(arg);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_vecpart_as_dofs(m_ph, kind_of(arg), in.ptr, u ? out.ptr : nullptr, u ? nullptr : out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    template <class U, class P>
+    amd::array<double> from_partitioned(int kind, const U& u, const P& p, std::vector<size_t> shape) const
+    {
+        amd::array<double> r(std::move(shape));
+        amd::dev_in<U> du(u);
+        amd::dev_in<P> dp(p);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_vecpart_from_partitioned(m_ph, kind, du.ptr, dp.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    template <class S, class T>
+    amd::array<double> copy(int which, const S& src, const T& dest) const
+    {
+        amd::array<double> r(amd::shape_of(dest));
+        std::copy(dest.data(), dest.data() + r.size(), r.data());
+        amd::dev_in<S> in(src);
+        amd::dev_inout<amd::array<double>> io(r);
+        amd::check(gf_vecpart_copy(m_ph, which, in.ptr, io.ptr, nullptr));
+        io.finish();
+        return r;
+    }
+    gf_vecpart m_ph = nullptr;
+    size_t m_nnu = 0, m_nnp = 0;
+    amd::array<size_t> m_iiu, m_iip, m_part;
+};
+
+/* MatrixDiagonalPartitioned(conn, dofs, iip)  (MatrixDiagonalPartitioned.h:42-452) */
+class MatrixDiagonalPartitioned {
+public:
+    template <class C, class D, class P>
+    MatrixDiagonalPartitioned(const C& conn, const D& dofs, const P& iip) : m_vec(conn, dofs, iip)
+    {
+        amd::check(gf_diagpart_create(&m_h, m_vec.part_handle()));
+        amd::check(gf_diagpart_diag(m_h, &m_d));
+    }
+    ~MatrixDiagonalPartitioned() { gf_diagpart_destroy(m_h); }
+    size_t ndof() const { return m_vec.ndof(); }
+    size_t nnu() const { return m_vec.nnu(); }
+    size_t nnp() const { return m_vec.nnp(); }
+    const VectorPartitioned& vector() const { return m_vec; }
+    /* assemble(elemmat)  (MatrixDiagonalPartitioned.h:73-98) */
+    template <class T>
+    void assemble(const T& elemmat)
+    {
+        amd::dev_in<T> in(elemmat);
+        amd::check(gf_diag_assemble(m_d, in.ptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    /* set(A)  (:121-135) */
+    template <class T>
+    void set(const T& A)
+    {
+        amd::dev_in<T> in(A);
+        amd::check(gf_diagpart_set(m_h, in.ptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    /* data(), data_uu(), data_pp()  (:142-175) */
+    amd::array<double> data() const
+    {
+        const double* p = nullptr;
+        amd::check(gf_diag_data(m_d, &p));
+        amd::array<double> r({ndof()});
+        amd::check(gf_memcpy_d2h(r.data(), p, static_cast<int64_t>(r.size() * sizeof(double))));
+        return r;
+    }
+    amd::array<double> data_uu() const { return data_part(true); }
+    amd::array<double> data_pp() const { return data_part(false); }
+    /* Dot(x)  (:189-224) */
+    template <class T>
+    amd::array<double> Dot(const T& x) const
+    {
+        amd::array<double> r(amd::shape_of(x));
+        amd::dev_in<T> in(x);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        if (x.dimension() == 1) amd::check(gf_diag_dot_dofval(m_d, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_diag_dot_nodevec(m_d, in.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    /* Dot_u = A_uu x_u, Dot_p = A_pp x_p  (:248-300); Reaction_p (:405-420) is Dot_p */
+    template <class U, class P>
+    amd::array<double> Dot_u(const U& x_u, const P&) const { return dot_part(0, x_u, nnu()); }
+    template <class U, class P>
+    amd::array<double> Dot_p(const U&, const P& x_p) const { return dot_part(1, x_p, nnp()); }
+    template <class U, class P>
+    amd::array<double> Reaction_p(const U& x_u, const P& x_p) const { return Dot_p(x_u, x_p); }
+    /* Solve(b, x) / solve(b, x)  (:306-337): the unknown entries A_uu^-1 b_u */
+    template <class B, class X>
+    amd::array<double> Solve(const B& b, const X& x)
+    {
+        amd::array<double> r(amd::shape_of(x));
+        std::copy(x.data(), x.data() + r.size(), r.data());
+        solve(b, r);
+        return r;
+    }
+    template <class B, class X>
+    void solve(const B& b, X& x)
+    {
+        apply_sel(0, b, x);
+    }
+    /* Solve_u(b_u, x_p)  (:356-372) */
+    template <class B, class P>
+    amd::array<double> Solve_u(const B& b_u, const P&)
+    {
+        amd::array<double> r({nnu()});
+        amd::dev_in<B> in(b_u);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_diagpart_solve_u(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    /* Reaction(x, b) / reaction(x, b)  (Matrix.h:490-514, :378-403) */
+    template <class X, class B>
+    amd::array<double> Reaction(const X& x, const B& b) const
+    {
+        amd::array<double> r(amd::shape_of(b));
+        std::copy(b.data(), b.data() + r.size(), r.data());
+        reaction(x, r);
+        return r;
+    }
+    template <class X, class B>
+    void reaction(const X& x, B& b) const
+    {
+        apply_sel(1, x, b);
+    }
+
+private:
+    amd::array<double> data_part(bool u) const
+    {
+        amd::array<double> r({u ? nnu() : nnp()});
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_diagpart_data_part(m_h, u ? out.ptr : nullptr, u ? nullptr : out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    template <class T>
+    amd::array<double> dot_part(int which, const T& x, size_t n) const
+    {
+        amd::array<double> r({n});
+        amd::dev_in<T> in(x);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_diagpart_dot_part(m_h, which, in.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    template <class T, class R>
+    void apply_sel(int op, const T& in_arg, R& out_arg) const
+    {
+        amd::dev_in<T> in(in_arg);
+        amd::dev_inout<R> io(out_arg);
+        amd::check(gf_diagpart_apply_sel(m_h, op, in_arg.dimension() == 1 ? 0 : 1, in.ptr, io.ptr, nullptr));
+        io.finish();
+    }
+    VectorPartitioned m_vec;
+    gf_diagpart m_h = nullptr;
+    gf_diag m_d = nullptr;
+};
+
+/* MatrixPartitioned(conn, dofs, iip)  (MatrixPartitioned.h:34-523): CSR blocks uu, up, pu, pp */
+class MatrixPartitioned {
+public:
+    template <class C, class D, class P>
+    MatrixPartitioned(const C& conn, const D& dofs, const P& iip) : m_vec(conn, dofs, iip)
+    {
+        amd::check(gf_matpart_create(&m_h, m_vec.part_handle()));
+        amd::check(gf_matpart_sizes(m_h, m_nnz));
+    }
+    ~MatrixPartitioned() { gf_matpart_destroy(m_h); }
+    size_t ndof() const { return m_vec.ndof(); }
+    size_t nnu() const { return m_vec.nnu(); }
+    size_t nnp() const { return m_vec.nnp(); }
+    const amd::array<size_t>& iiu() const { return m_vec.iiu(); }
+    const amd::array<size_t>& iip() const { return m_vec.iip(); }
+    const VectorPartitioned& vector() const { return m_vec; }
+    gf_matpart handle() const { return m_h; }
+    /* assemble(elemmat)  (MatrixPartitioned.h:156-210) */
+    template <class T>
+    void assemble(const T& elemmat)
+    {
+        amd::dev_in<T> in(elemmat);
+        amd::check(gf_matpart_assemble(m_h, in.ptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    /* data_uu() ... data_pp()  (:126-153) as host CSR copies */
+    amd::SparseMatrix data_uu() const { return block(0); }
+    amd::SparseMatrix data_up() const { return block(1); }
+    amd::SparseMatrix data_pu() const { return block(2); }
+    amd::SparseMatrix data_pp() const { return block(3); }
+    /* Todense()  (:324-350) */
+    amd::array<double> Todense() const
+    {
+        gf_matrix full = nullptr;
+        amd::check(gf_matpart_matrix(m_h, &full));
+        amd::array<double> r({ndof(), ndof()});
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_matrix_todense(full, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    /* Dot(x)  (:354-396) */
+    template <class T>
+    amd::array<double> Dot(const T& x) const
+    {
+        amd::array<double> r(amd::shape_of(x));
+        amd::dev_in<T> in(x);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_matpart_apply(m_h, 0, x.dimension() == 1 ? 0 : 1, in.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    /* Reaction(x, b) / reaction(x, b)  (Matrix.h:490-514, :401-433) */
+    template <class X, class B>
+    amd::array<double> Reaction(const X& x, const B& b) const
+    {
+        amd::array<double> r(amd::shape_of(b));
+        std::copy(b.data(), b.data() + r.size(), r.data());
+        reaction(x, r);
+        return r;
+    }
+    template <class X, class B>
+    void reaction(const X& x, B& b) const
+    {
+        amd::dev_in<X> in(x);
+        amd::dev_inout<B> io(b);
+        amd::check(gf_matpart_apply(m_h, 1, x.dimension() == 1 ? 0 : 1, in.ptr, io.ptr, nullptr));
+        io.finish();
+    }
+    /* Reaction_p(x_u, x_p)  (:436-447) */
+    template <class U, class P>
+    amd::array<double> Reaction_p(const U& x_u, const P& x_p) const
+    {
+        amd::array<double> r({nnp()});
+        amd::dev_in<U> du(x_u);
+        amd::dev_in<P> dp(x_p);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_matpart_dot_part(m_h, 1, du.ptr, dp.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+
+private:
+    amd::SparseMatrix block(int b) const
+    {
+        const int32_t *rp = nullptr, *cl = nullptr;
+        const double* val = nullptr;
+        amd::check(gf_matpart_csr(m_h, b, &rp, &cl, &val));
+        amd::SparseMatrix M;
+        M.rows = b < 2 ? nnu() : nnp();
+        M.cols = (b % 2 == 0) ? nnu() : nnp();
+        const size_t nnz = static_cast<size_t>(m_nnz[b]);
+        std::vector<int32_t> r32(M.rows + 1), c32(nnz);
+        M.val.resize(nnz);
+        amd::check(gf_memcpy_d2h(r32.data(), rp, static_cast<int64_t>(r32.size() * sizeof(int32_t))));
+        if (nnz) {
+            amd::check(gf_memcpy_d2h(c32.data(), cl, static_cast<int64_t>(nnz * sizeof(int32_t))));
+            amd::check(gf_memcpy_d2h(M.val.data(), val, static_cast<int64_t>(nnz * sizeof(double))));
+        }
+        M.rowptr.assign(r32.begin(), r32.end());
+        M.col.assign(c32.begin(), c32.end());
+        return M;
+    }
+    VectorPartitioned m_vec;
+    gf_matpart m_h = nullptr;
+    int64_t m_nnz[4] = {0, 0, 0, 0};
+};
+
+/* MatrixPartitionedSolver  (MatrixPartitioned.h:526-640): x_u = A_uu^-1 (b_u - A_up x_p) by an FP64
+ * Jacobi-preconditioned CG on the device (A_uu symmetric positive definite) in place of the
+ * reference's SimplicialLDLT.  Throws if the true relative residual exceeds 10 tol. */
+class MatrixPartitionedSolver {
+public:
+    explicit MatrixPartitionedSolver(double tol = 1e-12, int64_t maxiter = 0) : m_tol(tol), m_maxiter(maxiter) {}
+    int64_t iterations() const { return m_it; }
+    double relres() const { return m_res; }
+    /* Solve(A, b, x) / solve(A, b, x)  (:583-613) */
+    template <class B, class X>
+    amd::array<double> Solve(MatrixPartitioned& A, const B& b, const X& x)
+    {
+        amd::array<double> r(amd::shape_of(x));
+        std::copy(x.data(), x.data() + r.size(), r.data());
+        solve(A, b, r);
+        return r;
+    }
+    template <class B, class X>
+    void solve(MatrixPartitioned& A, const B& b, X& x)
+    {
+        amd::dev_in<B> in(b);
+        amd::dev_inout<X> io(x);
+        amd::check(gf_matpart_solve(A.handle(), b.dimension() == 1 ? 0 : 1, in.ptr, io.ptr, 0, m_tol, cap(A), &m_it,
+                                    &m_res, nullptr));
+        io.finish();
+    }
+    /* Solve_u(A, b_u, x_p) / solve_u(A, b_u, x_p, x_u)  (:545-580, :615-631) */
+    template <class B, class P>
+    amd::array<double> Solve_u(MatrixPartitioned& A, const B& b_u, const P& x_p)
+    {
+        amd::array<double> r({A.nnu()});
+        solve_u(A, b_u, x_p, r);
+        return r;
+    }
+    template <class B, class P, class U>
+    void solve_u(MatrixPartitioned& A, const B& b_u, const P& x_p, U& x_u)
+    {
+        amd::dev_in<B> db(b_u);
+        amd::dev_in<P> dp(x_p);
+        amd::dev_out<U> out(x_u, {A.nnu()});
+        amd::check(gf_matpart_solve_u(A.handle(), db.ptr, dp.ptr, out.ptr, 0, m_tol, cap(A), &m_it, &m_res, nullptr));
+        out.finish();
+    }
+
+private:
+    int64_t cap(const MatrixPartitioned& A) const
+    {
+        return m_maxiter > 0 ? m_maxiter : std::max<int64_t>(100, 10 * static_cast<int64_t>(A.nnu()));
+    }
+    double m_tol;
+    int64_t m_maxiter;
+    int64_t m_it = 0;
+    double m_res = 0.0;
+};
+
+// =================================================================================================
 // Tyings (include/GooseFEM/TyingsPeriodic.h)
 // =================================================================================================
 namespace Tyings {
 
 /* Compressed sparse rows of a tying matrix (in place of the reference's Eigen::SparseMatrix). */
-struct SparseMatrix {
-    size_t rows = 0, cols = 0;
-    std::vector<int64_t> rowptr, col;
-    std::vector<double> val;
-};
+using SparseMatrix = amd::SparseMatrix;
 
 /* Tyings::Control  (TyingsPeriodic.h:361-439) */
 class Control {

@@ -1,6 +1,7 @@
 // Drop-in check of the C++ face (include/GooseFEM/GooseFEM.h): the same calls and checks as the
 // reference's C++ unit tests (tests/basic/ElementQuad4Planar.cpp, ElementHex8.cpp,
-// MatrixDiagonal.cpp, Matrix.cpp, Vector.cpp, MeshQuad4.cpp), against the GPU path.
+// MatrixDiagonal.cpp, Matrix.cpp, Vector.cpp, MeshQuad4.cpp, VectorPartitioned.cpp,
+// MatrixDiagonalPartitioned.cpp, MatrixParitioned.cpp), against the GPU path.
 #include <GooseFEM/GooseFEM.h>
 
 #include <cmath>
@@ -110,6 +111,105 @@ int main()
             for (size_t j = 0; j < A.ndof(); ++j) s += K(i, j) * x[j];
             ok &= std::abs(s - b[i]) < 1e-12 && std::abs(K(i, (i + 3) % A.ndof()) - K((i + 3) % A.ndof(), i)) < 1e-14;
         }
+        REQUIRE(ok);
+    }
+    // VectorPartitioned.cpp:40-95: tied Dirichlet mesh, nodevec -> partitioned -> nodevec round trip
+    {
+        GF::Mesh::Quad4::Regular mesh(9, 9);
+        auto dofs = mesh.dofs();
+        auto left = mesh.nodesLeftOpenEdge();
+        auto right = mesh.nodesRightOpenEdge();
+        for (size_t k = 0; k < left.size(); ++k)
+            for (size_t i = 0; i < 2; ++i) dofs(right[k], i) = dofs(left[k], i);
+        auto top = mesh.nodesTopEdge();
+        auto bottom = mesh.nodesBottomEdge();
+        GF::array_type::tensor<size_t, 1> iip({2 * (top.size() + bottom.size())});
+        size_t k = 0;
+        for (size_t i = 0; i < 2; ++i)
+            for (size_t n = 0; n < top.size(); ++n) iip[k++] = dofs(top[n], i);
+        for (size_t i = 0; i < 2; ++i)
+            for (size_t n = 0; n < bottom.size(); ++n) iip[k++] = dofs(bottom[n], i);
+        GF::VectorPartitioned vec(mesh.conn(), dofs, iip);
+        GF::array_type::tensor<double, 2> u(vec.shape_nodevec());
+        std::mt19937 rng(5);
+        std::uniform_real_distribution<double> U(0.0, 1.0);
+        for (auto& v : u) v = U(rng);
+        for (size_t n = 0; n < left.size(); ++n)
+            for (size_t i = 0; i < 2; ++i) u(right[n], i) = u(left[n], i);
+        auto u_u = vec.AsDofs_u(u);
+        auto u_p = vec.AsDofs_p(u);
+        REQUIRE(u_u.size() == vec.nnu() && u_p.size() == vec.nnp() && vec.nnp() == iip.size());
+        auto back = vec.NodeFromPartitioned(u_u, u_p);
+        bool ok = true;
+        for (size_t i = 0; i < u.size(); ++i) ok &= back[i] == u[i];
+        REQUIRE(ok);
+        GF::array_type::tensor<double, 2> zero(vec.shape_nodevec());
+        auto c = vec.Copy_p(u, vec.Copy_u(u, zero));
+        ok = true;
+        for (size_t i = 0; i < u.size(); ++i) ok &= c[i] == u[i];
+        REQUIRE(ok);
+        // MatrixDiagonalPartitioned.cpp:55-94: set, Dot, Solve, Reaction
+        GF::MatrixDiagonalPartitioned D(mesh.conn(), dofs, iip);
+        GF::array_type::tensor<double, 1> a({vec.ndof()}), x({vec.ndof()});
+        for (auto& v : a) v = 0.5 + U(rng);
+        for (auto& v : x) v = U(rng);
+        D.set(a);
+        auto b = D.Dot(x);
+        auto X = D.Solve(b, x);
+        ok = true;
+        for (size_t d = 0; d < vec.nnu(); ++d) ok &= std::abs(X[vec.iiu()[d]] - x[vec.iiu()[d]]) < 1e-14;
+        REQUIRE(ok);
+        auto R = D.Reaction(x, b);
+        ok = true;
+        for (size_t d = 0; d < vec.nnp(); ++d) ok &= R[vec.iip()[d]] == a[vec.iip()[d]] * x[vec.iip()[d]];
+        REQUIRE(ok);
+    }
+    // MatrixParitioned.cpp:11-44 and :46-108: assemble, Todense, Dot, Solve (SPD matrices: the device
+    // solve is a CG)
+    {
+        GF::Mesh::Quad4::Regular mesh(2, 2);
+        GF::array_type::tensor<size_t, 1> iip({4});
+        iip[0] = 0;
+        iip[1] = 5;
+        iip[2] = 7;
+        iip[3] = 13;
+        GF::MatrixPartitioned A(mesh.conn(), mesh.dofs(), iip);
+        std::mt19937 rng(6);
+        std::uniform_real_distribution<double> U(0.0, 1.0);
+        GF::array_type::tensor<double, 3> a({mesh.nelem(), 8, 8});
+        for (size_t e = 0; e < mesh.nelem(); ++e) {
+            double r[8][8];
+            for (auto& row : r)
+                for (auto& v : row) v = U(rng);
+            for (size_t i = 0; i < 8; ++i)
+                for (size_t j = 0; j < 8; ++j) {
+                    double s = i == j ? 8.0 : 0.0;
+                    for (size_t k = 0; k < 8; ++k) s += r[i][k] * r[j][k];
+                    a(e, i, j) = s;
+                }
+        }
+        A.assemble(a);
+        GF::array_type::tensor<double, 1> x({A.ndof()});
+        for (auto& v : x) v = U(rng);
+        auto b = A.Dot(x);
+        auto K = A.Todense();
+        bool ok = true;
+        for (size_t i = 0; i < A.ndof(); ++i) {
+            double s = 0.0;
+            for (size_t j = 0; j < A.ndof(); ++j) s += K(i, j) * x[j];
+            ok &= std::abs(s - b[i]) < 1e-12;
+        }
+        REQUIRE(ok);
+        GF::MatrixPartitionedSolver solver;
+        auto xc = solver.Solve(A, b, x);
+        ok = true;
+        for (size_t i = 0; i < x.size(); ++i) ok &= std::abs(xc[i] - x[i]) < 1e-9;
+        REQUIRE(ok && solver.iterations() > 0 && solver.relres() < 1e-11);
+        auto uu = A.data_uu();
+        REQUIRE(uu.rows == A.nnu() && uu.cols == A.nnu() && uu.rowptr.back() == static_cast<int64_t>(uu.val.size()));
+        auto R = A.Reaction(x, b);
+        ok = true;
+        for (size_t d = 0; d < A.nnp(); ++d) ok &= std::abs(R[A.iip()[d]] - b[A.iip()[d]]) < 1e-12;
         REQUIRE(ok);
     }
     // ElementHex8.cpp: dV and periodic lumped mass of ones; BF16x3 stiffness close to FP64
