@@ -144,6 +144,39 @@ def _empty(shape, dtype=None):
     return torch.empty(shape, dtype=dtype or torch.float64, device=device())
 
 
+def _out(out, shape, dtype=None):
+    """The caller's output tensor (checked) or a new one."""
+    if out is None:
+        return _empty(shape, dtype)
+    torch = _torch()
+    want = dtype or torch.float64
+    assert out.is_cuda and out.is_contiguous(), "output must be a contiguous GPU tensor"
+    assert tuple(out.shape) == tuple(shape) and out.dtype == want, f"output must be {want} of shape {tuple(shape)}"
+    return out
+
+
+def _inplace(ret, fn):
+    """Run fn(device_tensor) so that it writes into ret: a contiguous FP64 GPU tensor is written
+    directly; a numpy array goes through a device copy that is copied back into it."""
+    torch = _torch()
+    if isinstance(ret, torch.Tensor) and ret.is_cuda:
+        fn(ret)
+        return ret
+    assert isinstance(ret, np.ndarray) and ret.dtype == np.float64, "ret: FP64 GPU tensor or numpy array"
+    d = to_device(ret)
+    fn(d)
+    torch.cuda.current_stream().synchronize()
+    ret[...] = d.cpu().numpy().reshape(ret.shape)
+    return ret
+
+
+def _np_of(a):
+    """Host numpy view of a scalar, numpy array or torch tensor."""
+    if hasattr(a, "detach"):
+        return a.detach().cpu().numpy()
+    return np.asarray(a, dtype=np.float64)
+
+
 def _dtype_code(t):
     torch = _torch()
     if t.dtype == torch.float64:
@@ -443,10 +476,10 @@ class Vector(_Handle):
     def shape_dofval(self):
         return (self.ndof,)
 
-    def AsElement(self, arg):
+    def AsElement(self, arg, out=None):
         """AsElement(nodevec | dofval) (Vector.h:193-211)."""
         arg = to_device(arg)
-        out = _empty(self.shape_elemvec())
+        out = _out(out, self.shape_elemvec())
         if arg.dim() == 2:
             _call("gf_vector_as_element_nodevec", self._h, _dp(arg), _dp(out), _stream())
         else:
@@ -454,31 +487,31 @@ class Vector(_Handle):
             _call("gf_vector_as_element_dofval", self._h, _dp(arg), _dp(out), _stream())
         return out
 
-    def AsDofs(self, arg):
+    def AsDofs(self, arg, out=None):
         """AsDofs(nodevec | elemvec) (Vector.h:141-159): overwrite, last in loop order wins."""
         arg = to_device(arg)
-        out = _empty(self.shape_dofval())
+        out = _out(out, self.shape_dofval())
         if arg.dim() == 2:
             _call("gf_vector_as_dofs_nodevec", self._h, _dp(arg), _dp(out), _stream())
         else:
             _call("gf_vector_as_dofs_elemvec", self._h, _dp(arg), _dp(out), _stream())
         return out
 
-    def AsNode(self, arg):
+    def AsNode(self, arg, out=None):
         """AsNode(dofval | elemvec) (Vector.h:167-185)."""
         arg = to_device(arg)
-        out = _empty(self.shape_nodevec())
+        out = _out(out, self.shape_nodevec())
         if arg.dim() == 1:
             _call("gf_vector_as_node_dofval", self._h, _dp(arg), _dp(out), _stream())
         else:
             _call("gf_vector_as_node_elemvec", self._h, _dp(arg), _dp(out), _stream())
         return out
 
-    def AssembleDofs(self, arg):
+    def AssembleDofs(self, arg, out=None):
         """AssembleDofs(nodevec | elemvec) (Vector.h:219-237); elemvec may be float32."""
         torch = _torch()
         arg = arg if isinstance(arg, torch.Tensor) and arg.is_cuda else to_device(arg)
-        out = _empty(self.shape_dofval())
+        out = _out(out, self.shape_dofval())
         if arg.dim() == 2:
             _call("gf_vector_assemble_dofs_nodevec", self._h, _dp(arg.double().contiguous()), _dp(out), _stream())
         else:
@@ -486,13 +519,30 @@ class Vector(_Handle):
             _call("gf_vector_assemble_dofs_elemvec", self._h, _dp(arg), _i32(_dtype_code(arg)), _dp(out), _stream())
         return out
 
-    def AssembleNode(self, arg):
+    def AssembleNode(self, arg, out=None):
         """AssembleNode(elemvec) (Vector.h:245-263) = AsNode(AssembleDofs(elemvec))."""
         torch = _torch()
         arg = arg.contiguous() if isinstance(arg, torch.Tensor) and arg.is_cuda else to_device(arg)
-        out = _empty(self.shape_nodevec())
+        out = _out(out, self.shape_nodevec())
         _call("gf_vector_assemble_node_elemvec", self._h, _dp(arg), _i32(_dtype_code(arg)), _dp(out), _stream())
         return out
+
+    # write-into forms of the reference (Vector.h: asElement, asDofs, asNode, assembleDofs,
+    # assembleNode): ret is a contiguous FP64 GPU tensor, or a numpy array updated in place
+    def asElement(self, arg, ret):
+        return _inplace(ret, lambda r: self.AsElement(arg, out=r))
+
+    def asDofs(self, arg, ret):
+        return _inplace(ret, lambda r: self.AsDofs(arg, out=r))
+
+    def asNode(self, arg, ret):
+        return _inplace(ret, lambda r: self.AsNode(arg, out=r))
+
+    def assembleDofs(self, arg, ret):
+        return _inplace(ret, lambda r: self.AssembleDofs(arg, out=r))
+
+    def assembleNode(self, arg, ret):
+        return _inplace(ret, lambda r: self.AssembleNode(arg, out=r))
 
 
 class VectorPartitioned(Vector):
@@ -591,12 +641,12 @@ class VectorPartitioned(Vector):
         return self._copy(1, nodevec_src, to_device(nodevec_dest).clone())
 
     def copy_u(self, nodevec_src, nodevec_dest):
-        """copy_u: in place on a contiguous float64 device tensor nodevec_dest."""
-        return self._copy(0, nodevec_src, nodevec_dest)
+        """copy_u: in place on nodevec_dest (FP64 GPU tensor or numpy array)."""
+        return _inplace(nodevec_dest, lambda r: self._copy(0, nodevec_src, r))
 
     def copy_p(self, nodevec_src, nodevec_dest):
-        """copy_p: in place on a contiguous float64 device tensor nodevec_dest."""
-        return self._copy(1, nodevec_src, nodevec_dest)
+        """copy_p: in place on nodevec_dest (FP64 GPU tensor or numpy array)."""
+        return _inplace(nodevec_dest, lambda r: self._copy(1, nodevec_src, r))
 
 
 # -------------------------------------------------------------------------------------------------
@@ -653,21 +703,21 @@ class _Quadrature(_Handle):
     def shape_qtensor(self, rank=2):
         return (self.nelem, self.nip) + (self.tdim,) * rank
 
-    def _grad(self, fn, elemvec):
+    def _grad(self, fn, elemvec, out=None):
         torch = _torch()
         ev = elemvec.contiguous() if isinstance(elemvec, torch.Tensor) and elemvec.is_cuda else to_device(elemvec)
-        out = _empty(self.shape_qtensor(2), dtype=ev.dtype)
+        out = _out(out, self.shape_qtensor(2), dtype=ev.dtype)
         _call(fn, self._h, _dp(ev), _dp(out), _i32(_dtype_code(ev)), _stream())
         return out
 
-    def GradN_vector(self, elemvec):
-        return self._grad("gf_quad_gradN_vector", elemvec)
+    def GradN_vector(self, elemvec, out=None):
+        return self._grad("gf_quad_gradN_vector", elemvec, out)
 
-    def GradN_vector_T(self, elemvec):
-        return self._grad("gf_quad_gradN_vector_T", elemvec)
+    def GradN_vector_T(self, elemvec, out=None):
+        return self._grad("gf_quad_gradN_vector_T", elemvec, out)
 
-    def SymGradN_vector(self, elemvec):
-        return self._grad("gf_quad_symgradN_vector", elemvec)
+    def SymGradN_vector(self, elemvec, out=None):
+        return self._grad("gf_quad_symgradN_vector", elemvec, out)
 
     def InterpQuad_vector(self, elemvec):
         ev = to_device(elemvec)
@@ -690,10 +740,10 @@ class _Quadrature(_Handle):
         _call("gf_quad_int_N_scalar_NT_dV", self._h, _dp(qs), _dp(out), _stream())
         return out
 
-    def Int_gradN_dot_tensor2_dV(self, qtensor):
+    def Int_gradN_dot_tensor2_dV(self, qtensor, out=None):
         torch = _torch()
         qt = qtensor.contiguous() if isinstance(qtensor, torch.Tensor) and qtensor.is_cuda else to_device(qtensor)
-        out = _empty((self.nelem, self.nne, self.ndim), dtype=qt.dtype)
+        out = _out(out, (self.nelem, self.nne, self.ndim), dtype=qt.dtype)
         _call("gf_quad_int_gradN_dot_tensor2_dV", self._h, _dp(qt), _dp(out), _i32(_dtype_code(qt)), _stream())
         return out
 
@@ -713,6 +763,23 @@ class _Quadrature(_Handle):
             out = _empty((self.nelem, K, K))
         _call("gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV", self._h, _dp(qt), _dp(out), _i32(precision), _stream())
         return out
+
+    # write-into forms of the reference (Element.h: gradN_vector, gradN_vector_T, symGradN_vector,
+    # int_gradN_dot_tensor2_dV, int_gradN_dot_tensor4_dot_gradNT_dV)
+    def gradN_vector(self, elemvec, qtensor):
+        return _inplace(qtensor, lambda r: self.GradN_vector(elemvec, out=r))
+
+    def gradN_vector_T(self, elemvec, qtensor):
+        return _inplace(qtensor, lambda r: self.GradN_vector_T(elemvec, out=r))
+
+    def symGradN_vector(self, elemvec, qtensor):
+        return _inplace(qtensor, lambda r: self.SymGradN_vector(elemvec, out=r))
+
+    def int_gradN_dot_tensor2_dV(self, qtensor, elemvec):
+        return _inplace(elemvec, lambda r: self.Int_gradN_dot_tensor2_dV(qtensor, out=r))
+
+    def int_gradN_dot_tensor4_dot_gradNT_dV(self, qtensor, elemmat):
+        return _inplace(elemmat, lambda r: self.Int_gradN_dot_tensor4_dot_gradNT_dV(qtensor, out=r))
 
 
 def _hip_memcpy_d2d(dst, src_ptr, nbytes):
@@ -1162,11 +1229,10 @@ class MatrixPartitionedSolver:
         return out
 
     def solve(self, A, b, x, warm=False):
-        """solve(A, b, x): x's unknown entries solved in place (x a float64 GPU tensor)."""
+        """solve(A, b, x): x's unknown entries solved in place (FP64 GPU tensor or numpy array)."""
         b = to_device(b)
-        assert x.dtype == _torch().float64 and x.is_cuda and x.is_contiguous()
-        self._run("gf_matpart_solve", A, _i32(0 if b.dim() == 1 else 1), _dp(b), _dp(x), _i32(int(warm)))
-        return x
+        kind = _i32(0 if b.dim() == 1 else 1)
+        return _inplace(x, lambda r: self._run("gf_matpart_solve", A, kind, _dp(b), _dp(r), _i32(int(warm))))
 
     def Solve_u(self, A, b_u, x_p, x_u0=None):
         """Solve_u(A, b_u, x_p) (MatrixPartitioned.h:545-560, :615-631); x_u0: optional initial guess."""
@@ -1174,6 +1240,109 @@ class MatrixPartitionedSolver:
         out = _torch().zeros(A.nnu, dtype=_torch().float64, device=device()) if x_u0 is None else to_device(x_u0).clone()
         self._run("gf_matpart_solve_u", A, _dp(bu), _dp(xp), _dp(out), _i32(0 if x_u0 is None else 1))
         return out
+
+
+# -------------------------------------------------------------------------------------------------
+# Material laws per quadrature point, with the API the examples use from GMat (Eps, Sig, C,
+# refresh, increment)
+# -------------------------------------------------------------------------------------------------
+
+
+class _MaterialArrays:
+    def _init_arrays(self, shape):
+        torch = _torch()
+        self.shape = tuple(shape)
+        self._Eps = torch.zeros(self.shape + (3, 3), dtype=torch.float64, device=device())
+        self._Sig = torch.zeros_like(self._Eps)
+        self._C = torch.zeros(self.shape + (3, 3, 3, 3), dtype=torch.float64, device=device())
+
+    @property
+    def Eps(self):
+        """Strain [..., 3, 3] (device, written in place by e.g. quad.symGradN_vector(ue, mat.Eps))."""
+        return self._Eps
+
+    @Eps.setter
+    def Eps(self, value):
+        self._Eps.copy_(to_device(value).reshape(self._Eps.shape))
+
+    @property
+    def Sig(self):
+        return self._Sig
+
+    @property
+    def C(self):
+        return self._C
+
+
+class Elastic(_MaterialArrays):
+    """Linear elastic law of the examples (GMatElastic.Cartesian3d.Elastic2d(K, G),
+    docs/examples/statics/partial-periodic_elastic.py:63): Sig = K tr(Eps) I + 2G dev(Eps)."""
+
+    def __init__(self, K, G):
+        self.K = to_device(K)
+        self.G = to_device(_torch().as_tensor(np.broadcast_to(_np_of(G), tuple(self.K.shape)).copy()))
+        self._init_arrays(self.K.shape)
+        _call("gf_material_elastic_tangent", _i64(self.K.numel()), _i32(3), _dp(self.K), _dp(self.G), _dp(self._C),
+              _stream())
+
+    def refresh(self, compute_tangent=True):
+        _call("gf_material_elastic_stress", _i64(self.K.numel()), _i32(3), _dp(self._Eps), _dp(self.K), _dp(self.G),
+              _dp(self._Sig), _i32(_dtype_code(self._Eps)), _stream())
+
+    def increment(self):
+        pass
+
+
+class LinearHardening(_MaterialArrays, _Handle):
+    """J2 elasto-plasticity with linear isotropic hardening on the GPU, with the API of
+    GMatElastoPlastic.Cartesian3d.LinearHardening2d(K, G, sigy0, H) as
+    docs/examples/statics/periodic_elastoplastic.py:71-104 drives it: set Eps, refresh(), read Sig
+    and C; increment() commits the plastic state (Epsp, epsp)."""
+
+    _destroy = "gf_plastic_destroy"
+
+    def __init__(self, K, G, sigy0, H):
+        torch = _torch()
+        self.K = to_device(K)
+        shape = tuple(self.K.shape)
+
+        def full(a):
+            return to_device(torch.as_tensor(np.broadcast_to(_np_of(a), shape).copy()))
+
+        self.G, self.sigy0, self.H = full(G), full(sigy0), full(H)
+        self._init_arrays(shape)
+        h = _vp()
+        _call("gf_plastic_create", ctypes.byref(h), _i64(self.K.numel()), _dp(self.K), _dp(self.G), _dp(self.sigy0),
+              _dp(self.H), _stream())
+        self._h = h
+
+    def refresh(self, compute_tangent=True):
+        _call("gf_plastic_refresh", self._h, _dp(self._Eps), _dp(self._Sig), _dp(self._C) if compute_tangent else None,
+              _stream())
+
+    def increment(self):
+        _call("gf_plastic_increment", self._h, _stream())
+
+    def _state(self, which, shape):
+        ptrs = [_vp(), _vp(), _vp(), _vp()]
+        _call("gf_plastic_state", self._h, *[ctypes.byref(p) for p in ptrs])
+        out = _empty(shape)
+        _hip_memcpy_d2d(out, ptrs[which], 8 * out.numel())
+        return out
+
+    @property
+    def Epsp(self):
+        """Plastic strain [..., 3, 3] (copy)."""
+        return self._state(0, self.shape + (3, 3))
+
+    @property
+    def epsp(self):
+        """Equivalent plastic strain [...] (copy)."""
+        return self._state(1, self.shape)
+
+
+Elastic2d = Elastic
+LinearHardening2d = LinearHardening
 
 
 # -------------------------------------------------------------------------------------------------

@@ -1,0 +1,186 @@
+// J2 elasto-plasticity with linear isotropic hardening (small strain, 3-d tensors) per quadrature
+// point: the material law the statics examples call through
+// GMatElastoPlastic.Cartesian3d.LinearHardening2d (docs/examples/statics/periodic_elastoplastic.py:
+// 71-104; python-gmatelastoplastic >= 0.3.1, environment.yaml:16).  GMat is not part of the
+// reference; the law is the published radial return with its consistent tangent (de Souza Neto,
+// Peric, Owen 2008, Box 7.3 and eq. 7.120):
+//   Ee = Eps - Epsp_t;  sigm = K tr(Ee);  s = 2G dev(Ee);  q = sqrt(3/2 s:s)
+//   phi = q - (sigy0 + H epsp_t);  if phi > 0:  dg = phi / (3G + H)
+//   Sig = sigm I + f s,  f = 1 - 3G dg / q;   Epsp = Epsp_t + (3/2 dg / q) s;   epsp = epsp_t + dg
+//   C = K I(x)I + 2G f I_d + c n(x)n,  c = 6G^2 (dg / q - 1 / (3G + H)),  n = s / |s|
+// (f = 1, c = 0, dg = 0 when phi <= 0).  The committed state (Epsp_t, epsp_t) changes only in
+// increment(), as in GMat, so Newton iterations within an increment re-evaluate from it.
+//
+// Layout: Eps, Sig, Epsp [nq, 3, 3] and C [nq, 3, 3, 3, 3] row-major FP64 (the quadrature
+// kernels' qtensor layout); K, G, sigy0, H, epsp [nq].  One block handles TP points: the tensors
+// are staged through LDS so every global access is a contiguous, coalesced stream, and the 81
+// tangent entries per point are written by the whole block from per-point scalars kept in LDS.
+// The arithmetic keeps the expression order of the oracle's restatement (FP64, no contraction).
+#include "gf_internal.hpp"
+
+using namespace gf;
+
+struct gf_plastic_s {
+    int64_t nq = 0;
+    DBuf<double> K, G, sy0, H;
+    DBuf<double> Epsp, Epsp_t, epsp, epsp_t;
+};
+
+namespace {
+
+constexpr int TP = 128;
+
+__global__ void __launch_bounds__(TP)
+    k_plastic_refresh(int64_t nq, const double* __restrict__ Eps, const double* __restrict__ Kp,
+                      const double* __restrict__ Gp, const double* __restrict__ Sy, const double* __restrict__ Hp,
+                      const double* __restrict__ Epsp_t, const double* __restrict__ epsp_t,
+                      double* __restrict__ Epsp, double* __restrict__ epsp, double* __restrict__ Sig,
+                      double* __restrict__ C)
+{
+    __shared__ double sE[TP * 9], sP[TP * 9], sN[TP * 9], sK[TP], sF[TP], sC[TP];
+    const int64_t p0 = blockIdx.x * static_cast<int64_t>(TP);
+    const int np = static_cast<int>(nq - p0 < TP ? nq - p0 : TP);
+    const int t = threadIdx.x;
+    const double* gE = Eps + 9 * p0;
+    const double* gP = Epsp_t + 9 * p0;
+    for (int i = t; i < np * 9; i += TP) {
+        sE[i] = gE[i];
+        sP[i] = gP[i];
+    }
+    __syncthreads();
+    if (t < np) {
+        const int64_t p = p0 + t;
+        double* e = sE + 9 * t;
+        double* ep = sP + 9 * t;
+        double* n = sN + 9 * t;
+        double ee[9], s[9];
+        for (int k = 0; k < 9; ++k) ee[k] = e[k] - ep[k];
+        const double tr = ee[0] + ee[4] + ee[8];
+        const double epsm = tr / 3.0;
+        const double k_ = Kp[p], g = Gp[p], h = Hp[p], ept = epsp_t[p];
+        double ss = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            s[k] = 2.0 * g * (k % 4 == 0 ? ee[k] - epsm : ee[k]);
+            ss += s[k] * s[k];
+        }
+        const double q = sqrt(1.5 * ss);
+        const double phi = q - (Sy[p] + h * ept);
+        double f = 1.0, c = 0.0, dg = 0.0;
+        if (phi > 0.0) {
+            dg = phi / (3.0 * g + h);
+            f = 1.0 - 3.0 * g * dg / q;
+            c = 6.0 * g * g * (dg / q - 1.0 / (3.0 * g + h));
+            const double nrm = sqrt(ss);
+            for (int k = 0; k < 9; ++k) n[k] = s[k] / nrm;
+        }
+        else {
+            for (int k = 0; k < 9; ++k) n[k] = 0.0;
+        }
+        const double sigm = k_ * tr;
+        const double a = 1.5 * dg / q;
+        for (int k = 0; k < 9; ++k) {
+            e[k] = f * s[k] + (k % 4 == 0 ? sigm : 0.0);
+            if (phi > 0.0) ep[k] = ep[k] + a * s[k];
+        }
+        epsp[p] = ept + dg;
+        sK[t] = k_;
+        sF[t] = 2.0 * g * f;
+        sC[t] = c;
+    }
+    __syncthreads();
+    double* oS = Sig + 9 * p0;
+    double* oP = Epsp + 9 * p0;
+    for (int i = t; i < np * 9; i += TP) {
+        oS[i] = sE[i];
+        oP[i] = sP[i];
+    }
+    if (!C) return;
+    double* oC = C + 81 * p0;
+    for (int i = t; i < np * 81; i += TP) {
+        const int pl = i / 81, r = i - 81 * pl, ij = r / 9, kl = r - 9 * ij;
+        const int ii = ij / 3, jj = ij - 3 * ii, aa = kl / 3, bb = kl - 3 * aa;
+        const double dij = ii == jj, dkl = aa == bb, dik = ii == aa, djl = jj == bb, dil = ii == bb, djk = jj == aa;
+        oC[i] = sK[pl] * dij * dkl + sF[pl] * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0) +
+                sC[pl] * (sN[9 * pl + ij] * sN[9 * pl + kl]);
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int gf_plastic_create(gf_plastic* out, int64_t nq, const double* K, const double* G, const double* sigy0,
+                      const double* H, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(out && K && G && sigy0 && H, "null argument");
+        GF_REQUIRE(nq > 0, "nq must be positive");
+        auto M = new gf_plastic_s();
+        try {
+            hipStream_t s = S(stream);
+            M->nq = nq;
+            const size_t b1 = nq * sizeof(double), b9 = 9 * b1;
+            DBuf<double>* par[4] = {&M->K, &M->G, &M->sy0, &M->H};
+            const double* src[4] = {K, G, sigy0, H};
+            for (int k = 0; k < 4; ++k) {
+                par[k]->alloc(nq);
+                GF_HIP(hipMemcpyAsync(par[k]->p, src[k], b1, hipMemcpyDeviceToDevice, s));
+            }
+            M->Epsp.alloc(9 * nq);
+            M->Epsp_t.alloc(9 * nq);
+            M->epsp.alloc(nq);
+            M->epsp_t.alloc(nq);
+            GF_HIP(hipMemsetAsync(M->Epsp.p, 0, b9, s));
+            GF_HIP(hipMemsetAsync(M->Epsp_t.p, 0, b9, s));
+            GF_HIP(hipMemsetAsync(M->epsp.p, 0, b1, s));
+            GF_HIP(hipMemsetAsync(M->epsp_t.p, 0, b1, s));
+            GF_HIP(hipStreamSynchronize(s));
+        }
+        catch (...) {
+            delete M;
+            throw;
+        }
+        *out = M;
+    });
+}
+
+int gf_plastic_destroy(gf_plastic m)
+{
+    delete m;
+    return GF_OK;
+}
+
+int gf_plastic_refresh(gf_plastic m, const double* Eps, double* Sig, double* C, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(m && Eps && Sig, "null argument");
+        hipStream_t s = S(stream);
+        k_plastic_refresh<<<nblocks(m->nq, TP), TP, 0, s>>>(m->nq, Eps, m->K.p, m->G.p, m->sy0.p, m->H.p, m->Epsp_t.p,
+                                                            m->epsp_t.p, m->Epsp.p, m->epsp.p, Sig, C);
+        check_launch();
+    });
+}
+
+int gf_plastic_increment(gf_plastic m, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(m, "null argument");
+        hipStream_t s = S(stream);
+        GF_HIP(hipMemcpyAsync(m->Epsp_t.p, m->Epsp.p, 9 * m->nq * sizeof(double), hipMemcpyDeviceToDevice, s));
+        GF_HIP(hipMemcpyAsync(m->epsp_t.p, m->epsp.p, m->nq * sizeof(double), hipMemcpyDeviceToDevice, s));
+    });
+}
+
+int gf_plastic_state(gf_plastic m, const double** Epsp, const double** epsp, const double** Epsp_t,
+                     const double** epsp_t)
+{
+    return guard([&] {
+        GF_REQUIRE(m, "null argument");
+        if (Epsp) *Epsp = m->Epsp.p;
+        if (epsp) *epsp = m->epsp.p;
+        if (Epsp_t) *Epsp_t = m->Epsp_t.p;
+        if (epsp_t) *epsp_t = m->epsp_t.p;
+    });
+}
+
+} // extern "C"

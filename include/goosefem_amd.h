@@ -56,6 +56,7 @@ typedef struct gf_pcg_s* gf_pcg;
 typedef struct gf_vecpart_s* gf_vecpart;
 typedef struct gf_diagpart_s* gf_diagpart;
 typedef struct gf_matpart_s* gf_matpart;
+typedef struct gf_plastic_s* gf_plastic;
 
 const char* gf_last_error(void);
 int gf_version(int* major, int* minor, int* patch);
@@ -231,6 +232,24 @@ int gf_material_elastic_tangent(int64_t nq, int tdim, const double* K_q, const d
 /* The same tangent rounded to FP32 (the input of gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV_f32) */
 int gf_material_elastic_tangent_f32(int64_t nq, int tdim, const double* K_q, const double* G_q, float* C,
                                     void* stream);
+
+/* Material: J2 elasto-plasticity with linear isotropic hardening, the law the statics examples call
+ * as GMatElastoPlastic.Cartesian3d.LinearHardening2d(K, G, sigy0, H)
+ * (docs/examples/statics/periodic_elastoplastic.py:71-104; external package, not in the
+ * reference).  Radial return and consistent tangent of de Souza Neto et al. (2008) Box 7.3 and
+ * eq. 7.120.  All arrays are device FP64: K, G, sigy0, H [nq] (copied at create); Eps, Sig, Epsp
+ * [nq, 3, 3]; C [nq, 3, 3, 3, 3]; epsp [nq].  The state starts at zero plastic strain. */
+int gf_plastic_create(gf_plastic* out, int64_t nq, const double* K, const double* G, const double* sigy0,
+                      const double* H, void* stream);
+int gf_plastic_destroy(gf_plastic m);
+/* refresh(): Sig (and C unless NULL) from Eps and the committed state; updates the trial state
+ * Epsp, epsp */
+int gf_plastic_refresh(gf_plastic m, const double* Eps, double* Sig, double* C, void* stream);
+/* increment(): commit the state (Epsp_t = Epsp, epsp_t = epsp) */
+int gf_plastic_increment(gf_plastic m, void* stream);
+/* device pointers to the state arrays; any output may be NULL */
+int gf_plastic_state(gf_plastic m, const double** Epsp, const double** epsp, const double** Epsp_t,
+                     const double** epsp_t);
 
 /* ================================================================================================
  * MatrixDiagonal  (include/GooseFEM/MatrixDiagonal.h)

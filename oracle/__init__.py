@@ -558,6 +558,48 @@ def elastic_tangent(K, G, td=3):
     return out
 
 
+def plastic_refresh(Eps, K, G, sigy0, H, Epsp_t, epsp_t, tangent=True):
+    """J2 radial return with linear isotropic hardening (or_plastic_refresh; the law of
+    GMatElastoPlastic.Cartesian3d.LinearHardening2d used at periodic_elastoplastic.py:71-104).
+    Eps, Epsp_t: [..., 3, 3]; K, G, sigy0, H, epsp_t: [...].  Returns (Sig, C or None, Epsp, epsp)."""
+    Eps = _f64(Eps)
+    shape = Eps.shape[:-2]
+    nq = int(np.prod(shape)) if shape else 1
+    args = [_f64(np.broadcast_to(a, shape)) for a in (K, G, sigy0, H)]
+    Epsp_t, epsp_t = _f64(Epsp_t), _f64(epsp_t)
+    Sig = np.empty_like(Eps)
+    Epsp = np.empty_like(Eps)
+    epsp = np.empty(shape)
+    C = np.empty(shape + (3, 3, 3, 3)) if tangent else None
+    _call("or_plastic_refresh", nq, Eps, *args, Epsp_t, epsp_t, Epsp, epsp, Sig, C)
+    return Sig, C, Epsp, epsp
+
+
+class LinearHardening:
+    """The state machine of GMatElastoPlastic.Cartesian3d.LinearHardening2d as the statics example
+    drives it (periodic_elastoplastic.py:85-104): set Eps, refresh(), read Sig / C; increment()
+    commits the plastic state."""
+
+    def __init__(self, K, G, sigy0, H):
+        self.K, self.G, self.sigy0, self.H = (_f64(a) for a in (K, G, sigy0, H))
+        shape = self.K.shape
+        self.Eps = np.zeros(shape + (3, 3))
+        self.Epsp = np.zeros(shape + (3, 3))
+        self.Epsp_t = np.zeros(shape + (3, 3))
+        self.epsp = np.zeros(shape)
+        self.epsp_t = np.zeros(shape)
+        self.Sig = np.zeros(shape + (3, 3))
+        self.C = None
+
+    def refresh(self, tangent=True):
+        self.Sig, self.C, self.Epsp, self.epsp = plastic_refresh(
+            self.Eps, self.K, self.G, self.sigy0, self.H, self.Epsp_t, self.epsp_t, tangent)
+
+    def increment(self):
+        self.Epsp_t = self.Epsp.copy()
+        self.epsp_t = self.epsp.copy()
+
+
 def vv_steps(nsteps, et, conn, dofs_, dNx, vol, Ke, Ge, inv_mass, dt, u, v, a, fext=None):
     """Velocity-Verlet steps (Elastic-VelocityVerlet/main.cpp:110-157); u, v, a updated in place.
 

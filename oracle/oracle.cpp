@@ -14,8 +14,8 @@
 //   vector conversions                   golden periodic values tests/basic/Vector.cpp + brute force
 //   sparse assemble / dot                dense brute force (numpy) on tiny meshes
 //   tyings / K' / CG                     scipy spsolve on K' built from dense blocks (tiny meshes)
-//   material law                         parity unpinned by the reference (external GMat); closed
-//                                        form identities only
+//   material laws                        parity unpinned by the reference (external GMat); closed
+//                                        form identities only (elastic; J2 linear hardening)
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -991,6 +991,75 @@ void or_elastic_tangent(i64 nq, int td, const double* K, const double* G, double
                         double dij = i == j, dkl = a == b, dik = i == a, djl = j == b, dil = i == b, djk = j == a;
                         c[((i * td + j) * td + a) * td + b] =
                             k * dij * dkl + 2.0 * g * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0);
+                    }
+    }
+}
+
+// ================================================================================================
+// J2 elasto-plasticity with linear isotropic hardening, small strain, 3-d tensors: the law behind
+// GMatElastoPlastic.Cartesian3d.LinearHardening2d (python-gmatelastoplastic >= 0.3.1,
+// environment.yaml:16; called at docs/examples/statics/periodic_elastoplastic.py:71-104).  GMat is
+// not part of the reference, so this restates the published radial-return algorithm and its
+// consistent tangent (de Souza Neto, Peric, Owen, "Computational Methods for Plasticity", 2008,
+// Box 7.3 and eq. 7.120; Simo and Hughes, "Computational Inelasticity", 1998, Box 3.2):
+//   Ee = Eps - Epsp_t;  sigm = K tr(Ee);  s = 2G dev(Ee);  q = sqrt(3/2 s:s)
+//   phi = q - (sigy0 + H epsp_t);  elastic if phi <= 0, else
+//   dg = phi / (3G + H);  Sig = sigm I + (1 - 3G dg / q) s;  Epsp = Epsp_t + (3/2 dg / q) s;
+//   epsp = epsp_t + dg
+//   C = K I (x) I + 2G f I_d + c n (x) n,  f = 1 - 3G dg / q,  c = 6G^2 (dg / q - 1 / (3G + H)),
+//   n = s / |s|  (f = 1, c = 0 in the elastic branch);  I_d = I_s - I (x) I / 3.
+// Parity with GMat's own implementation is unpinned (absent package); the tests pin this
+// restatement by the identities of the law (consistency, incompressible plastic flow, exact radial
+// return under proportional loading, the tangent as the derivative of the stress, elastic
+// unloading, the elastic limit).
+// ================================================================================================
+
+void or_plastic_refresh(i64 nq, const double* Eps, const double* K, const double* G, const double* sy0,
+                        const double* H, const double* Epsp_t, const double* epsp_t, double* Epsp, double* epsp,
+                        double* Sig, double* C)
+{
+    for (i64 p = 0; p < nq; ++p) {
+        const double* e = Eps + 9 * p;
+        const double* ept = Epsp_t + 9 * p;
+        double ee[9], s[9], n[9];
+        for (int k = 0; k < 9; ++k) ee[k] = e[k] - ept[k];
+        const double tr = ee[0] + ee[4] + ee[8];
+        const double epsm = tr / 3.0;
+        const double k_ = K[p], g = G[p];
+        double ss = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            s[k] = 2.0 * g * (k % 4 == 0 ? ee[k] - epsm : ee[k]);
+            ss += s[k] * s[k];
+        }
+        const double q = std::sqrt(1.5 * ss);
+        const double phi = q - (sy0[p] + H[p] * epsp_t[p]);
+        double f = 1.0, c = 0.0, dg = 0.0;
+        for (int k = 0; k < 9; ++k) n[k] = 0.0;
+        if (phi > 0.0) {
+            dg = phi / (3.0 * g + H[p]);
+            f = 1.0 - 3.0 * g * dg / q;
+            c = 6.0 * g * g * (dg / q - 1.0 / (3.0 * g + H[p]));
+            const double nrm = std::sqrt(ss);
+            for (int k = 0; k < 9; ++k) n[k] = s[k] / nrm;
+        }
+        const double sigm = k_ * tr;
+        const double a = 1.5 * dg / q;
+        for (int k = 0; k < 9; ++k) {
+            Sig[9 * p + k] = f * s[k] + (k % 4 == 0 ? sigm : 0.0);
+            Epsp[9 * p + k] = phi > 0.0 ? ept[k] + a * s[k] : ept[k];
+        }
+        epsp[p] = epsp_t[p] + dg;
+        if (!C) continue;
+        const double g2f = 2.0 * g * f;
+        double* cp = C + 81 * p;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int a_ = 0; a_ < 3; ++a_)
+                    for (int b = 0; b < 3; ++b) {
+                        const double dij = i == j, dkl = a_ == b, dik = i == a_, djl = j == b, dil = i == b, djk = j == a_;
+                        cp[((i * 3 + j) * 3 + a_) * 3 + b] = k_ * dij * dkl +
+                                                             g2f * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0) +
+                                                             c * (n[i * 3 + j] * n[a_ * 3 + b]);
                     }
     }
 }
