@@ -414,6 +414,68 @@ def bench_pcg_learned(args):
     print(json.dumps(res))
 
 
+def bench_plastic(args):
+    """J2 linear-hardening refresh (stress, consistent tangent, plastic state) at every quadrature
+    point of a Hex8 Regular n^3 mesh (8 points per element): the per-point step of the statics
+    examples' Newton loop.  Strains are random symmetric tensors, a third of them scaled to stay
+    elastic; moduli K = 10, G = 1, sigy0 = 0.05, H = 0.4."""
+    import torch
+
+    import goosefem_amd as gf
+
+    n = args.hex_n
+    nq = n * n * n * 8
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    E = torch.randn((nq, 3, 3), dtype=torch.float64, device="cuda", generator=g) * 0.05
+    E = 0.5 * (E + E.transpose(1, 2))
+    E[::3] *= 0.01
+    mat = gf.LinearHardening(torch.full((nq,), 10.0, dtype=torch.float64, device="cuda"), 1.0, 0.05, 0.4)
+    mat.Eps = E
+    del E
+    torch.cuda.synchronize()
+    t_full = _timed(lambda: mat.refresh(True), 10)
+    t_sig = _timed(lambda: mat.refresh(False), 10)
+    frac_plastic = float((mat.epsp > 0).double().mean())
+    # algorithmic bytes per point: read Eps 72 + Epsp_t 72 + (K, G, sigy0, H) 32 + epsp_t 8; write
+    # Sig 72 + Epsp 72 + epsp 8, and C 648 with the tangent
+    b_full, b_sig = 984, 336
+    ach = b_full * nq / t_full / 1e9
+    res = {
+        "metric": "J2 linear-hardening refresh throughput (stress + consistent tangent)",
+        "value": nq / t_full,
+        "unit": "points/s",
+        "config": {"workload": f"Hex8 Regular {n}^3 quadrature points (x8), J2 linear hardening", "points": nq,
+                   "plastic_fraction": frac_plastic},
+        "dtype": "f64",
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "kernel": "k_plastic_refresh",
+                     "traffic": load_traffic(n, "plastic_refresh_hex8_128_traffic.json", "n"),
+                     "algorithmic_bytes_per_launch": b_full * nq, "avg_launch_us": t_full * 1e6},
+        "stress_only": {"avg_launch_us": t_sig * 1e6, "points_per_s": nq / t_sig,
+                        "achieved_GBs": b_sig * nq / t_sig / 1e9, "frac": b_sig * nq / t_sig / 1e9 / HBM_PEAK_GBS},
+    }
+    if not args.no_cpu:
+        import oracle
+
+        rng = np.random.default_rng(0)
+        m = 1 << 20
+        Es = rng.standard_normal((m, 3, 3)) * 0.05
+        Es = 0.5 * (Es + Es.transpose(0, 2, 1))
+        Es[::3] *= 0.01
+        z9, z1 = np.zeros((m, 3, 3)), np.zeros(m)
+        done, dt = 0, 0.0
+        while dt < 10.0:
+            t0 = time.perf_counter()
+            oracle.plastic_refresh(Es, 10.0, 1.0, 0.05, 0.4, z9, z1)
+            dt += time.perf_counter() - t0
+            done += m
+        res["cpu_baseline"] = {"value": done / dt, "unit": "points/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} points (batches of {m}), FP64 oracle restatement, one thread, "
+                                         f"{dt:.1f} s"}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,7 +485,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
-    ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned"],
+    ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned",
+                                           "plastic-refresh"],
                     default="quad4-explicit")
     ap.add_argument("--pcg-n", type=int, default=512)
     ap.add_argument("--pcg-hex", action="store_true", help="periodic Hex8 n^3 instead of Quad4 n^2")
@@ -462,6 +525,9 @@ def main():
         return
     if args.workload == "pcg-learned":
         bench_pcg_learned(args)
+        return
+    if args.workload == "plastic-refresh":
+        bench_plastic(args)
         return
 
     nx = ny = args.nx

@@ -14,7 +14,8 @@
 // Layout: Eps, Sig, Epsp [nq, 3, 3] and C [nq, 3, 3, 3, 3] row-major FP64 (the quadrature
 // kernels' qtensor layout); K, G, sigy0, H, epsp [nq].  One block handles TP points: the tensors
 // are staged through LDS so every global access is a contiguous, coalesced stream, and the 81
-// tangent entries per point are written by the whole block from per-point scalars kept in LDS.
+// tangent entries per point are written by the whole block from per-point scalars kept in LDS,
+// two entries per 16-byte streaming store (the outputs are not re-read by this kernel).
 // The arithmetic keeps the expression order of the oracle's restatement (FP64, no contraction).
 #include "gf_internal.hpp"
 
@@ -28,14 +29,15 @@ struct gf_plastic_s {
 
 namespace {
 
-constexpr int TP = 128;
+constexpr int TP = 128, BS = 256; // points and threads per block (the copy loops use every thread)
+typedef double dbl2 __attribute__((ext_vector_type(2)));
 
-__global__ void __launch_bounds__(TP)
+__global__ void __launch_bounds__(BS)
     k_plastic_refresh(int64_t nq, const double* __restrict__ Eps, const double* __restrict__ Kp,
                       const double* __restrict__ Gp, const double* __restrict__ Sy, const double* __restrict__ Hp,
                       const double* __restrict__ Epsp_t, const double* __restrict__ epsp_t,
                       double* __restrict__ Epsp, double* __restrict__ epsp, double* __restrict__ Sig,
-                      double* __restrict__ C)
+                      double* __restrict__ C, bool pairs)
 {
     __shared__ double sE[TP * 9], sP[TP * 9], sN[TP * 9], sK[TP], sF[TP], sC[TP];
     const int64_t p0 = blockIdx.x * static_cast<int64_t>(TP);
@@ -43,7 +45,7 @@ __global__ void __launch_bounds__(TP)
     const int t = threadIdx.x;
     const double* gE = Eps + 9 * p0;
     const double* gP = Epsp_t + 9 * p0;
-    for (int i = t; i < np * 9; i += TP) {
+    for (int i = t; i < np * 9; i += BS) {
         sE[i] = gE[i];
         sP[i] = gP[i];
     }
@@ -88,20 +90,46 @@ __global__ void __launch_bounds__(TP)
         sC[t] = c;
     }
     __syncthreads();
+    // outputs: pairs of doubles with streaming (non-temporal) stores where the bases allow it; the
+    // block's slices of Sig, Epsp and C are contiguous
     double* oS = Sig + 9 * p0;
     double* oP = Epsp + 9 * p0;
-    for (int i = t; i < np * 9; i += TP) {
-        oS[i] = sE[i];
-        oP[i] = sP[i];
+    const int n9 = np * 9;
+    if (pairs) {
+        for (int i = t; i < n9 / 2; i += BS) {
+            __builtin_nontemporal_store(dbl2{sE[2 * i], sE[2 * i + 1]}, reinterpret_cast<dbl2*>(oS) + i);
+            __builtin_nontemporal_store(dbl2{sP[2 * i], sP[2 * i + 1]}, reinterpret_cast<dbl2*>(oP) + i);
+        }
+        if ((n9 & 1) && t == 0) {
+            oS[n9 - 1] = sE[n9 - 1];
+            oP[n9 - 1] = sP[n9 - 1];
+        }
+    }
+    else {
+        for (int i = t; i < n9; i += BS) {
+            oS[i] = sE[i];
+            oP[i] = sP[i];
+        }
     }
     if (!C) return;
+    // tangent: two entries per 16-byte streaming store over the block's contiguous slice (faster
+    // than one 8-byte store per entry, even with a component per thread)
     double* oC = C + 81 * p0;
-    for (int i = t; i < np * 81; i += TP) {
+    auto entry = [&](int i) {
         const int pl = i / 81, r = i - 81 * pl, ij = r / 9, kl = r - 9 * ij;
         const int ii = ij / 3, jj = ij - 3 * ii, aa = kl / 3, bb = kl - 3 * aa;
         const double dij = ii == jj, dkl = aa == bb, dik = ii == aa, djl = jj == bb, dil = ii == bb, djk = jj == aa;
-        oC[i] = sK[pl] * dij * dkl + sF[pl] * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0) +
-                sC[pl] * (sN[9 * pl + ij] * sN[9 * pl + kl]);
+        return sK[pl] * dij * dkl + sF[pl] * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0) +
+               sC[pl] * (sN[9 * pl + ij] * sN[9 * pl + kl]);
+    };
+    const int n81 = np * 81;
+    if (pairs) {
+        for (int i = t; i < n81 / 2; i += BS)
+            __builtin_nontemporal_store(dbl2{entry(2 * i), entry(2 * i + 1)}, reinterpret_cast<dbl2*>(oC) + i);
+        if ((n81 & 1) && t == 0) oC[n81 - 1] = entry(n81 - 1);
+    }
+    else {
+        for (int i = t; i < n81; i += BS) oC[i] = entry(i);
     }
 }
 
@@ -155,8 +183,10 @@ int gf_plastic_refresh(gf_plastic m, const double* Eps, double* Sig, double* C, 
     return guard([&] {
         GF_REQUIRE(m && Eps && Sig, "null argument");
         hipStream_t s = S(stream);
-        k_plastic_refresh<<<nblocks(m->nq, TP), TP, 0, s>>>(m->nq, Eps, m->K.p, m->G.p, m->sy0.p, m->H.p, m->Epsp_t.p,
-                                                            m->epsp_t.p, m->Epsp.p, m->epsp.p, Sig, C);
+        // 16-byte stores need 16-byte aligned bases (each block's offset is a multiple of 16 bytes)
+        const bool pairs = aligned16(Sig) && aligned16(m->Epsp.p) && (!C || aligned16(C));
+        k_plastic_refresh<<<nblocks(m->nq, TP), BS, 0, s>>>(m->nq, Eps, m->K.p, m->G.p, m->sy0.p, m->H.p, m->Epsp_t.p,
+                                                            m->epsp_t.p, m->Epsp.p, m->epsp.p, Sig, C, pairs);
         check_launch();
     });
 }

@@ -392,18 +392,47 @@ constexpr int TP = 64;
 
 template <int TD, class TO>
 __global__ void __launch_bounds__(BS) k_elastic_tangent(int64_t n, const double* __restrict__ K,
-                                                        const double* __restrict__ G, TO* __restrict__ C)
+                                                        const double* __restrict__ G, TO* __restrict__ C, bool pairs)
 {
-    constexpr int T4 = TD * TD * TD * TD;
+    // The block's slice of C is contiguous and is written with streaming stores (not re-read).
+    // FP64: two entries per 16-byte store.  FP32: each thread owns one component c and walks the
+    // block's points PPI at a time, so its two constant factors are computed once and the
+    // PPI * T4 active threads of a pass write one contiguous run.  (Measured on Hex8 128^3: the
+    // FP64 pairs beat the component mapping, whose stores are 8 bytes; for FP32 the component
+    // mapping wins, the entry arithmetic being the larger cost.)
+    typedef TO V2 __attribute__((ext_vector_type(2)));
+    constexpr int T4 = TD * TD * TD * TD, PPI = BS / T4;
     const int64_t p0 = static_cast<int64_t>(blockIdx.x) * TP;
     const int np = static_cast<int>(min<int64_t>(TP, n - p0));
     TO* out = C + p0 * T4;
-    for (int t = threadIdx.x; t < np * T4; t += BS) {
-        const int pl = t / T4, c = t - pl * T4;
-        const int i = c / (TD * TD * TD), j = (c / (TD * TD)) % TD, a = (c / TD) % TD, b = c % TD;
+    if constexpr (sizeof(TO) == 8) {
+        auto entry = [&](int t) {
+            const int pl = t / T4, c = t - pl * T4;
+            const int i = c / (TD * TD * TD), j = (c / (TD * TD)) % TD, a = (c / TD) % TD, b = c % TD;
+            const double k = K[p0 + pl], g = G[p0 + pl];
+            const double dij = i == j, dkl = a == b, dik = i == a, djl = j == b, dil = i == b, djk = j == a;
+            return static_cast<TO>(k * dij * dkl + 2.0 * g * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0));
+        };
+        const int nt = np * T4;
+        if (pairs) {
+            for (int t = threadIdx.x; t < nt / 2; t += BS)
+                __builtin_nontemporal_store(V2{entry(2 * t), entry(2 * t + 1)}, reinterpret_cast<V2*>(out) + t);
+            if ((nt & 1) && threadIdx.x == 0) out[nt - 1] = entry(nt - 1);
+        }
+        else {
+            for (int t = threadIdx.x; t < nt; t += BS) out[t] = entry(t);
+        }
+        return;
+    }
+    if (threadIdx.x >= PPI * T4) return;
+    const int c = threadIdx.x % T4, q = threadIdx.x / T4;
+    const int i = c / (TD * TD * TD), j = (c / (TD * TD)) % TD, a = (c / TD) % TD, b = c % TD;
+    const double dij = i == j, dkl = a == b, dik = i == a, djl = j == b, dil = i == b, djk = j == a;
+    const double a1 = dij * dkl, a2 = 0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0;
+    for (int pl = q; pl < np; pl += PPI) {
         const double k = K[p0 + pl], g = G[p0 + pl];
-        const double dij = i == j, dkl = a == b, dik = i == a, djl = j == b, dil = i == b, djk = j == a;
-        out[t] = static_cast<TO>(k * dij * dkl + 2.0 * g * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0));
+        // k * a1 equals k * dij * dkl (a1 is 0 or 1), so the value is the same expression's
+        __builtin_nontemporal_store(static_cast<TO>(k * a1 + 2.0 * g * a2), out + pl * T4 + c);
     }
 }
 
@@ -712,8 +741,8 @@ int gf_material_elastic_tangent(int64_t nq, int tdim, const double* K, const dou
         GF_REQUIRE(K && G && C, "null argument");
         GF_REQUIRE(tdim == 2 || tdim == 3, "tdim must be 2 or 3");
         if (nq == 0) return;
-        if (tdim == 3) k_elastic_tangent<3, double><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C);
-        else k_elastic_tangent<2, double><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C);
+        if (tdim == 3) k_elastic_tangent<3, double><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C, aligned16(C));
+        else k_elastic_tangent<2, double><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C, aligned16(C));
         check_launch();
     });
 }
@@ -724,8 +753,8 @@ int gf_material_elastic_tangent_f32(int64_t nq, int tdim, const double* K, const
         GF_REQUIRE(K && G && C, "null argument");
         GF_REQUIRE(tdim == 2 || tdim == 3, "tdim must be 2 or 3");
         if (nq == 0) return;
-        if (tdim == 3) k_elastic_tangent<3, float><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C);
-        else k_elastic_tangent<2, float><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C);
+        if (tdim == 3) k_elastic_tangent<3, float><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C, aligned16(C));
+        else k_elastic_tangent<2, float><<<nblocks(nq, TP), BS, 0, S(stream)>>>(nq, K, G, C, aligned16(C));
         check_launch();
     });
 }
