@@ -237,6 +237,64 @@ class _RegularBase:
         return out
 
 
+class _Quad4FineLayer:
+    """Mesh::Quad4::FineLayer (include/GooseFEM/MeshQuad4.h:200-1135): fine element layers around
+    the middle, coarsened by 3 in x towards the bottom and top edges (host set-up)."""
+
+    def __init__(self, nelx, nely, h=1.0, nfine=1):
+        self._args = (_i64(int(nelx)), _i64(int(nely)), _f64(float(h)), _i64(int(nfine)))
+        self.h = float(h)
+        self.nne, self.ndim = 4, 2
+        sizes = np.zeros(4, dtype=np.int64)
+        _call("gf_mesh_quad4_finelayer", *self._args, _hp(sizes), None, None)
+        self.nelem, self.nnode, self.nelx, self.nely = (int(v) for v in sizes)
+
+    def _set(self, name):
+        n = _i64(0)
+        _call("gf_mesh_quad4_finelayer_set", *self._args, name.encode(), None, ctypes.byref(n))
+        out = np.empty(n.value, dtype=np.int64)
+        _call("gf_mesh_quad4_finelayer_set", *self._args, name.encode(), _hp(out), ctypes.byref(n))
+        return out
+
+    def __getattr__(self, name):
+        if name.startswith("nodes") or name.startswith("elements"):
+            r = self._set(name)
+            return int(r[0]) if name.endswith("Corner") or name == "nodesOrigin" else r
+        raise AttributeError(name)
+
+    @property
+    def coor(self):
+        out = np.empty((self.nnode, 2))
+        _call("gf_mesh_quad4_finelayer", *self._args, None, _hp(out), None)
+        return out
+
+    @property
+    def conn(self):
+        out = np.empty((self.nelem, 4), dtype=np.int64)
+        _call("gf_mesh_quad4_finelayer", *self._args, None, None, _hp(out))
+        return out
+
+    @property
+    def dofs(self):
+        out = np.empty((self.nnode, self.ndim), dtype=np.int64)
+        _call("gf_mesh_dofs", _i64(self.nnode), _i32(self.ndim), _hp(out))
+        return out
+
+    @property
+    def nodesPeriodic(self):
+        return self._set("nodesPeriodic").reshape(-1, 2)
+
+    @property
+    def dofsPeriodic(self):
+        ties = self.nodesPeriodic
+        out = np.empty((self.nnode, self.ndim), dtype=np.int64)
+        _call("gf_mesh_dofs_periodic", _i64(self.nnode), _i32(self.ndim), _hp(ties), _i64(ties.shape[0]), _hp(out))
+        return out
+
+    def elementsMiddleLayer(self):
+        return self._set("elementsMiddleLayer")
+
+
 class _Quad4Regular(_RegularBase):
     """Mesh::Quad4::Regular (include/GooseFEM/MeshQuad4.h:32-195)."""
 
@@ -335,7 +393,7 @@ def _nodaltyings(dofs):
 
 
 Mesh = SimpleNamespace(
-    Quad4=SimpleNamespace(Regular=_Quad4Regular),
+    Quad4=SimpleNamespace(Regular=_Quad4Regular, FineLayer=_Quad4FineLayer),
     Hex8=SimpleNamespace(Regular=_Hex8Regular),
     dofs=lambda nnode, ndim: np.arange(nnode * ndim, dtype=np.int64).reshape(nnode, ndim),
     renumber=_renumber,

@@ -89,6 +89,18 @@ int gf_mesh_nodeset(int64_t nelx, int64_t nely, int64_t nelz, const char* name, 
 /* RegularBase::nodesPeriodic()  (Mesh.h:511-540 for 2-D, Mesh.h:1577-1692 for 3-D).
  * out_h [nties, 2] (independent, dependent); *nties receives the count. */
 int gf_mesh_nodes_periodic(int64_t nelx, int64_t nely, int64_t nelz, int64_t* out_h, int64_t* nties);
+/* Mesh::Quad4::FineLayer(nelx, nely, h, nfine)  (include/GooseFEM/MeshQuad4.h:200-1135): element
+ * layers finest around the middle, coarsened by 3 in x towards the bottom and top.  sizes (any
+ * output may be NULL) receives {nelem, nnode, nelx(), nely()} (nelx() the finest layer's element
+ * count, nely() the height in fine elements, MeshQuad4.h:571-579); coor_h [nnode, 2], conn_h
+ * [nelem, 4]. */
+int gf_mesh_quad4_finelayer(int64_t nelx, int64_t nely, double h, int64_t nfine, int64_t* sizes, double* coor_h,
+                            int64_t* conn_h);
+/* Named sets of that mesh: "nodesBottomEdge" ... "nodesRightOpenEdge", the corners,
+ * "nodesOrigin", "elementsMiddleLayer" (MeshQuad4.h:287-292, 747-880) and "nodesPeriodic" (pairs,
+ * flattened [nties * 2], the rule of Mesh.h:511-540).  *n receives the count. */
+int gf_mesh_quad4_finelayer_set(int64_t nelx, int64_t nely, double h, int64_t nfine, const char* name,
+                                int64_t* out_h, int64_t* n);
 /* Mesh::dofs(nnode, ndim)  (Mesh.h:93-96) */
 int gf_mesh_dofs(int64_t nnode, int ndim, int64_t* dofs_h);
 /* RegularBase::dofsPeriodic()  (Mesh.h:291-304): dependent nodes take their independent node's

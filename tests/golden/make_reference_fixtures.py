@@ -7,6 +7,7 @@ committed and is all that tests read at run time:
 
 Sources (relative to the reference root):
   tests/basic/MeshQuad4.cpp:10-100      Regular(5,3): coor, conn, dofs, node sets, periodic
+  tests/basic/MeshQuad4.cpp:103-256     FineLayer(9,18): coor, conn, dofs, node sets, periodic
   tests/basic/MeshHex8.cpp:8-188        Regular(3,3,3): coor, conn, node sets, dofs, periodic
   tests/basic/Mesh.cpp:199-236          Regular(3,3): coordination, elem2node
   tests/test_Mesh.py:23-171             Regular(4,4) periodic: coordination, node2dof,
@@ -85,6 +86,8 @@ def main(root: str) -> None:
 
     src = open(os.path.join(basic, "MeshQuad4.cpp")).read()
     fx["MeshQuad4_Regular_5_3"] = cpp_arrays(section(src, "Regular"))
+
+    fx["MeshQuad4_FineLayer_9_18"] = cpp_arrays(section(src, "FineLayer"))
 
     src = open(os.path.join(basic, "MeshHex8.cpp")).read()
     fx["MeshHex8_Regular_3_3_3"] = cpp_arrays(section(src, "Regular"))
