@@ -888,6 +888,107 @@ class _Hex8Quadrature(_Quadrature):
         super().__init__(x, xi, w)
 
 
+class _Quad4QuadratureAxisymmetric(_Handle):
+    """Element::Quad4::QuadratureAxisymmetric (ElementQuad4Axisymmetric.h:33-467): nodal coordinates
+    (z, r), tensors 3 x 3 in (r, theta, z) order; all operators on the GPU, bit-exact in FP64."""
+
+    _destroy = "gf_axi_destroy"
+
+    def __init__(self, x, xi=None, w=None):
+        if xi is None:
+            xi, w = _rule(2, False)
+        self._xi = np.ascontiguousarray(xi, dtype=np.float64)
+        self._w = np.ascontiguousarray(w, dtype=np.float64)
+        self._x = to_device(x)
+        self.nelem, self.nne, self.ndim, self.tdim = int(self._x.shape[0]), 4, 2, 3
+        self.nip = int(self._w.size)
+        h = _vp()
+        _call("gf_axi_create", ctypes.byref(h), _i64(self.nelem), _i32(self.nip), _dp(self._x),
+              self._xi.ctypes.data_as(ctypes.c_void_p), self._w.ctypes.data_as(ctypes.c_void_p), _stream())
+        self._h = h
+
+    def update_x(self, x):
+        self._x = to_device(x)
+        _call("gf_axi_update_x", self._h, _dp(self._x), _stream())
+
+    def shape_qtensor(self, rank=2):
+        return (self.nelem, self.nip) + (3,) * rank
+
+    def _geometry(self):
+        G, dV = _vp(), _vp()
+        _call("gf_axi_geometry", self._h, ctypes.byref(G), ctypes.byref(dV))
+        return G, dV
+
+    @property
+    def dV(self):
+        """dV() [nelem, nip] (copy)."""
+        out = _empty((self.nelem, self.nip))
+        _hip_memcpy_d2d(out, self._geometry()[1], 8 * out.numel())
+        return out
+
+    def B(self):
+        """B() [nelem, nip, nne, 3, 3, 3] (ElementQuad4Axisymmetric.h:401-438), built from the five
+        non-zero components per node."""
+        torch = _torch()
+        G = _empty((self.nelem, self.nip, 4, 3))
+        _hip_memcpy_d2d(G, self._geometry()[0], 8 * G.numel())
+        B = torch.zeros((self.nelem, self.nip, 4, 3, 3, 3), dtype=torch.float64, device=device())
+        B[..., 0, 0, 0] = G[..., 0]
+        B[..., 0, 2, 2] = G[..., 0]
+        B[..., 1, 1, 0] = G[..., 1]
+        B[..., 2, 0, 0] = G[..., 2]
+        B[..., 2, 2, 2] = G[..., 2]
+        return B
+
+    def _grad(self, mode, elemvec, out=None):
+        ev = to_device(elemvec)
+        out = _out(out, self.shape_qtensor(2))
+        _call("gf_axi_grad", self._h, _i32(mode), _dp(ev), _dp(out), _stream())
+        return out
+
+    def GradN_vector(self, elemvec, out=None):
+        return self._grad(0, elemvec, out)
+
+    def GradN_vector_T(self, elemvec, out=None):
+        return self._grad(1, elemvec, out)
+
+    def SymGradN_vector(self, elemvec, out=None):
+        return self._grad(2, elemvec, out)
+
+    def Int_N_scalar_NT_dV(self, qscalar, out=None):
+        qs = to_device(qscalar)
+        out = _out(out, (self.nelem, 8, 8))
+        _call("gf_axi_int_N_scalar_NT_dV", self._h, _dp(qs), _dp(out), _stream())
+        return out
+
+    def Int_gradN_dot_tensor2_dV(self, qtensor, out=None):
+        qt = to_device(qtensor)
+        out = _out(out, (self.nelem, 4, 2))
+        _call("gf_axi_int_gradN_dot_tensor2_dV", self._h, _dp(qt), _dp(out), _stream())
+        return out
+
+    def Int_gradN_dot_tensor4_dot_gradNT_dV(self, qtensor, out=None):
+        qt = to_device(qtensor)
+        out = _out(out, (self.nelem, 8, 8))
+        _call("gf_axi_int_gradN_dot_tensor4_dot_gradNT_dV", self._h, _dp(qt), _dp(out), _stream())
+        return out
+
+    def gradN_vector(self, elemvec, qtensor):
+        return _inplace(qtensor, lambda r: self.GradN_vector(elemvec, out=r))
+
+    def gradN_vector_T(self, elemvec, qtensor):
+        return _inplace(qtensor, lambda r: self.GradN_vector_T(elemvec, out=r))
+
+    def symGradN_vector(self, elemvec, qtensor):
+        return _inplace(qtensor, lambda r: self.SymGradN_vector(elemvec, out=r))
+
+    def int_gradN_dot_tensor2_dV(self, qtensor, elemvec):
+        return _inplace(elemvec, lambda r: self.Int_gradN_dot_tensor2_dV(qtensor, out=r))
+
+    def int_gradN_dot_tensor4_dot_gradNT_dV(self, qtensor, elemmat):
+        return _inplace(elemmat, lambda r: self.Int_gradN_dot_tensor4_dot_gradNT_dV(qtensor, out=r))
+
+
 def _rule(nd, nodal):
     a = 1.0 if nodal else 1.0 / np.sqrt(3.0)
     if nd == 2:
@@ -902,6 +1003,7 @@ Element = SimpleNamespace(
     Quad4=SimpleNamespace(
         Quadrature=_Quad4Quadrature,
         QuadraturePlanar=_Quad4QuadraturePlanar,
+        QuadratureAxisymmetric=_Quad4QuadratureAxisymmetric,
         Gauss=SimpleNamespace(xi=lambda: _rule(2, False)[0], w=lambda: _rule(2, False)[1], nip=lambda: 4),
         Nodal=SimpleNamespace(xi=lambda: _rule(2, True)[0], w=lambda: _rule(2, True)[1], nip=lambda: 4),
     ),

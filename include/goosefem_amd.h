@@ -57,6 +57,7 @@ typedef struct gf_vecpart_s* gf_vecpart;
 typedef struct gf_diagpart_s* gf_diagpart;
 typedef struct gf_matpart_s* gf_matpart;
 typedef struct gf_plastic_s* gf_plastic;
+typedef struct gf_axi_s* gf_axi;
 
 const char* gf_last_error(void);
 int gf_version(int* major, int* minor, int* patch);
@@ -244,6 +245,31 @@ int gf_material_elastic_tangent(int64_t nq, int tdim, const double* K_q, const d
 /* The same tangent rounded to FP32 (the input of gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV_f32) */
 int gf_material_elastic_tangent_f32(int64_t nq, int tdim, const double* K_q, const double* G_q, float* C,
                                     void* stream);
+
+/* ================================================================================================
+ * Element::Quad4::QuadratureAxisymmetric  (include/GooseFEM/ElementQuad4Axisymmetric.h:33-467)
+ * Nodal coordinates (z, r); tensors 3 x 3 in (r, theta, z) order.  The geometry is kept as
+ * G [nelem, nip, 4, 3] = (dN/dr, N / r_q, dN/dz) per node, the five non-zero components of the
+ * reference's B (:401-438), and dV [nelem, nip] = w J 2 pi r_q.  Results equal the reference's bit
+ * for bit (same FP64 expression order).  x [nelem, 4, 2] device; xi_h [nip, 2], w_h [nip] host
+ * (nip <= 16).
+ * ============================================================================================== */
+int gf_axi_create(gf_axi* out, int64_t nelem, int nip, const double* x, const double* xi_h, const double* w_h,
+                  void* stream);
+int gf_axi_destroy(gf_axi m);
+/* update_x (Element.h:563-587 for this element) */
+int gf_axi_update_x(gf_axi m, const double* x, void* stream);
+/* device pointers to G and dV (dV(), :446); either output may be NULL */
+int gf_axi_geometry(gf_axi m, const double** G, const double** dV);
+/* mode 0: gradN_vector (:140-169); 1: gradN_vector_T (:171-200); 2: symGradN_vector (:202-233);
+ * elemvec [nelem, 4, 2] -> qtensor [nelem, nip, 3, 3] */
+int gf_axi_grad(gf_axi m, int mode, const double* elemvec, double* qtensor, void* stream);
+/* int_N_scalar_NT_dV (:235-266): qscalar [nelem, nip] -> elemmat [nelem, 8, 8] */
+int gf_axi_int_N_scalar_NT_dV(gf_axi m, const double* qscalar, double* elemmat, void* stream);
+/* int_gradN_dot_tensor2_dV (:268-296): qtensor [nelem, nip, 3, 3] -> elemvec [nelem, 4, 2] */
+int gf_axi_int_gradN_dot_tensor2_dV(gf_axi m, const double* qtensor, double* elemvec, void* stream);
+/* int_gradN_dot_tensor4_dot_gradNT_dV (:298-398): qtensor [nelem, nip, 3, 3, 3, 3] -> elemmat [nelem, 8, 8] */
+int gf_axi_int_gradN_dot_tensor4_dot_gradNT_dV(gf_axi m, const double* qtensor, double* elemmat, void* stream);
 
 /* Material: J2 elasto-plasticity with linear isotropic hardening, the law the statics examples call
  * as GMatElastoPlastic.Cartesian3d.LinearHardening2d(K, G, sigy0, H)

@@ -968,3 +968,69 @@ class MatrixPartitioned:
         else:
             out[v.iiu] = xu
         return out
+
+
+class QuadratureAxisymmetric:
+    """Element::Quad4::QuadratureAxisymmetric (ElementQuad4Axisymmetric.h:33-467): nodal coordinates
+    (z, r), tensors in (r, theta, z) order; G[e, q, m] = (dN/dr, N / r_q, dN/dz)."""
+
+    def __init__(self, x, xi=None, w=None):
+        if xi is None:
+            xi, w = quad_rule(QUAD4)
+        self.xi, self.w = _f64(xi), _f64(w)
+        self.nip, self.nne, self.ndim, self.tdim = self.w.size, 4, 2, 3
+        self.N = np.empty((self.nip, 4))
+        self.dNxi = np.empty((self.nip, 4, 2))
+        f = lib().or_quad_shape
+        f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        f(QUAD4, self.nip, _p(self.xi), _p(self.N), _p(self.dNxi))
+        self.update_x(x)
+
+    def update_x(self, x):
+        self.x = _f64(x)
+        self.nelem = self.x.shape[0]
+        self.G = np.empty((self.nelem, self.nip, 4, 3))
+        self.vol = np.empty((self.nelem, self.nip))
+        _call("or_axi_geometry", self.nelem, _c(self.nip), self.x, self.N, self.dNxi, self.w, self.G, self.vol)
+
+    def dV(self):
+        return self.vol
+
+    def B(self):
+        """The reference's B [nelem, nip, nne, 3, 3, 3] (ElementQuad4Axisymmetric.h:401-438)."""
+        B = np.zeros((self.nelem, self.nip, 4, 3, 3, 3))
+        B[:, :, :, 0, 0, 0] = self.G[..., 0]
+        B[:, :, :, 0, 2, 2] = self.G[..., 0]
+        B[:, :, :, 1, 1, 0] = self.G[..., 1]
+        B[:, :, :, 2, 0, 0] = self.G[..., 2]
+        B[:, :, :, 2, 2, 2] = self.G[..., 2]
+        return B
+
+    def _grad(self, mode, elemvec):
+        out = np.empty((self.nelem, self.nip, 3, 3))
+        _call("or_axi_grad", _c(mode), self.nelem, _c(self.nip), self.G, _f64(elemvec), out)
+        return out
+
+    def GradN_vector(self, elemvec):
+        return self._grad(0, elemvec)
+
+    def GradN_vector_T(self, elemvec):
+        return self._grad(1, elemvec)
+
+    def SymGradN_vector(self, elemvec):
+        return self._grad(2, elemvec)
+
+    def Int_N_scalar_NT_dV(self, qscalar):
+        out = np.empty((self.nelem, 8, 8))
+        _call("or_axi_int_N_scalar_NT_dV", self.nelem, _c(self.nip), self.N, self.vol, _f64(qscalar), out)
+        return out
+
+    def Int_gradN_dot_tensor2_dV(self, qtensor):
+        out = np.empty((self.nelem, 4, 2))
+        _call("or_axi_int_tensor2", self.nelem, _c(self.nip), self.G, self.vol, _f64(qtensor), out)
+        return out
+
+    def Int_gradN_dot_tensor4_dot_gradNT_dV(self, qtensor):
+        out = np.empty((self.nelem, 8, 8))
+        _call("or_axi_int_tensor4", self.nelem, _c(self.nip), self.G, self.vol, _f64(qtensor), out)
+        return out

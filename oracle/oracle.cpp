@@ -996,6 +996,134 @@ void or_elastic_tangent(i64 nq, int td, const double* K, const double* G, double
 }
 
 // ================================================================================================
+// Element::Quad4::QuadratureAxisymmetric (include/GooseFEM/ElementQuad4Axisymmetric.h:33-467).
+// Nodal coordinates (z, r); tensors in (r, theta, z) order.  Of the B-matrix only five components
+// per node are non-zero (:401-438): B(m,r,r,r) = B(m,r,z,z) = dN/dr, B(m,t,t,r) = N / r_q,
+// B(m,z,r,r) = B(m,z,z,z) = dN/dz; dV = w J 2 pi r_q.  Geometry is stored as
+// G[e, q, m, {dNdr, Nr, dNdz}] and vol[e, q].
+// ================================================================================================
+
+void or_axi_geometry(i64 nelem, int nip, const double* x, const double* N, const double* dNxi, const double* w,
+                     double* G, double* vol)
+{
+    for (i64 e = 0; e < nelem; ++e) {
+        const double* xe = x + e * 8;
+        for (int q = 0; q < nip; ++q) {
+            const double* g = dNxi + q * 8;
+            const double* n = N + q * 4;
+            double J[2][2], Ji[2][2];
+            J[0][0] = g[0] * xe[0] + g[2] * xe[2] + g[4] * xe[4] + g[6] * xe[6];
+            J[0][1] = g[0] * xe[1] + g[2] * xe[3] + g[4] * xe[5] + g[6] * xe[7];
+            J[1][0] = g[1] * xe[0] + g[3] * xe[2] + g[5] * xe[4] + g[7] * xe[6];
+            J[1][1] = g[1] * xe[1] + g[3] * xe[3] + g[5] * xe[5] + g[7] * xe[7];
+            const double det = inv2(J, Ji);
+            const double rq = n[0] * xe[1] + n[1] * xe[3] + n[2] * xe[5] + n[3] * xe[7];
+            double* o = G + (e * nip + q) * 12;
+            for (int m = 0; m < 4; ++m) {
+                o[3 * m + 0] = Ji[1][0] * g[2 * m] + Ji[1][1] * g[2 * m + 1];
+                o[3 * m + 1] = 1.0 / rq * n[m];
+                o[3 * m + 2] = Ji[0][0] * g[2 * m] + Ji[0][1] * g[2 * m + 1];
+            }
+            vol[e * nip + q] = w[q] * det * 2.0 * M_PI * rq;
+        }
+    }
+}
+
+// gradN_vector (mode 0, :140-169), gradN_vector_T (1, :171-200), symGradN_vector (2, :202-233)
+void or_axi_grad(int mode, i64 nelem, int nip, const double* G, const double* elemvec, double* out)
+{
+    for (i64 e = 0; e < nelem; ++e) {
+        const double* u = elemvec + e * 8; // u[m*2 + 0] = u_z, u[m*2 + 1] = u_r
+        for (int q = 0; q < nip; ++q) {
+            const double* b = G + (e * nip + q) * 12;
+            double* t = out + (e * nip + q) * 9;
+            for (int k = 0; k < 9; ++k) t[k] = 0.0;
+            // sums over the four nodes in node order
+            const double rr = b[0] * u[1] + b[3] * u[3] + b[6] * u[5] + b[9] * u[7];
+            const double tt = b[1] * u[1] + b[4] * u[3] + b[7] * u[5] + b[10] * u[7];
+            const double zz = b[2] * u[0] + b[5] * u[2] + b[8] * u[4] + b[11] * u[6];
+            t[0] = rr;
+            t[4] = tt;
+            t[8] = zz;
+            if (mode == 2) {
+                const double s = 0.5 * (b[0] * u[0] + b[3] * u[2] + b[6] * u[4] + b[9] * u[6] + b[2] * u[1] +
+                                        b[5] * u[3] + b[8] * u[5] + b[11] * u[7]);
+                t[6] = s;
+                t[2] = s;
+            }
+            else {
+                const double dzr = b[0] * u[0] + b[3] * u[2] + b[6] * u[4] + b[9] * u[6]; // B(m,r,z,z) u_z
+                const double drz = b[2] * u[1] + b[5] * u[3] + b[8] * u[5] + b[11] * u[7]; // B(m,z,r,r) u_r
+                t[mode == 0 ? 2 : 6] = dzr;
+                t[mode == 0 ? 6 : 2] = drz;
+            }
+        }
+    }
+}
+
+// int_N_scalar_NT_dV (:235-266)
+void or_axi_int_N_scalar_NT_dV(i64 nelem, int nip, const double* N, const double* vol, const double* qs,
+                               double* M)
+{
+    for (i64 e = 0; e < nelem; ++e) {
+        double* Me = M + e * 64;
+        for (int k = 0; k < 64; ++k) Me[k] = 0.0;
+        for (int q = 0; q < nip; ++q) {
+            const double* n = N + q * 4;
+            const double v = vol[e * nip + q], rho = qs[e * nip + q];
+            for (int m = 0; m < 4; ++m)
+                for (int k = 0; k < 4; ++k) {
+                    Me[(m * 2) * 8 + k * 2] += n[m] * rho * n[k] * v;
+                    Me[(m * 2 + 1) * 8 + k * 2 + 1] += n[m] * rho * n[k] * v;
+                }
+        }
+    }
+}
+
+// int_gradN_dot_tensor2_dV (:268-296)
+void or_axi_int_tensor2(i64 nelem, int nip, const double* G, const double* vol, const double* sig, double* f)
+{
+    for (i64 e = 0; e < nelem; ++e) {
+        double* fe = f + e * 8;
+        for (int k = 0; k < 8; ++k) fe[k] = 0.0;
+        for (int q = 0; q < nip; ++q) {
+            const double* b = G + (e * nip + q) * 12;
+            const double* s = sig + (e * nip + q) * 9;
+            const double v = vol[e * nip + q];
+            for (int m = 0; m < 4; ++m) {
+                fe[2 * m] += v * (b[3 * m + 2] * s[8] + b[3 * m] * s[2]);
+                fe[2 * m + 1] += v * (b[3 * m] * s[0] + b[3 * m + 1] * s[4] + b[3 * m + 2] * s[6]);
+            }
+        }
+    }
+}
+
+// int_gradN_dot_tensor4_dot_gradNT_dV (:298-398).  The 25 products per node pair run over the
+// five non-zero B slots a (outer) and b (inner), slot = (i, j, value, nodal component):
+//   (r,r: dNdr, u_r) (t,t: Nr, u_r) (z,z: dNdz, u_z) (r,z: dNdr, u_z) (z,r: dNdz, u_r);
+// K(m, ca; n, cb) += B_a(m) C(i_a, j_a, j_b, i_b) B_b(n) dV.
+void or_axi_int_tensor4(i64 nelem, int nip, const double* G, const double* vol, const double* C, double* K)
+{
+    const int si[5] = {0, 1, 2, 0, 2}, sj[5] = {0, 1, 2, 2, 0}, sv[5] = {0, 1, 2, 0, 2}, sc[5] = {1, 1, 0, 0, 1};
+    for (i64 e = 0; e < nelem; ++e) {
+        double* Ke = K + e * 64;
+        for (int k = 0; k < 64; ++k) Ke[k] = 0.0;
+        for (int q = 0; q < nip; ++q) {
+            const double* b = G + (e * nip + q) * 12;
+            const double* c = C + (e * nip + q) * 81;
+            const double v = vol[e * nip + q];
+            for (int m = 0; m < 4; ++m)
+                for (int n = 0; n < 4; ++n)
+                    for (int a = 0; a < 5; ++a)
+                        for (int bb = 0; bb < 5; ++bb) {
+                            const double cab = c[((si[a] * 3 + sj[a]) * 3 + sj[bb]) * 3 + si[bb]];
+                            Ke[(m * 2 + sc[a]) * 8 + n * 2 + sc[bb]] += b[3 * m + sv[a]] * cab * b[3 * n + sv[bb]] * v;
+                        }
+        }
+    }
+}
+
+// ================================================================================================
 // J2 elasto-plasticity with linear isotropic hardening, small strain, 3-d tensors: the law behind
 // GMatElastoPlastic.Cartesian3d.LinearHardening2d (python-gmatelastoplastic >= 0.3.1,
 // environment.yaml:16; called at docs/examples/statics/periodic_elastoplastic.py:71-104).  GMat is
