@@ -476,6 +476,61 @@ def bench_plastic(args):
     print(json.dumps(res))
 
 
+def bench_newton(args):
+    """One pass of the statics Newton iteration with J2 linear hardening on a Quad4 plane-strain
+    Regular n^2 mesh (4 points per element): strain, return map, element force and element
+    stiffness.  Fused (gf_plastic_newton_q4: Eps and C stay on chip) against the unfused calls
+    (symGradN_vector, refresh, Int_gradN_dot_tensor2_dV, Int_gradN_dot_tensor4_dot_gradNT_dV)."""
+    import torch
+
+    import goosefem_amd as gf
+
+    n = args.newton_n
+    mesh = gf.Mesh.Quad4.Regular(n, n)
+    vec = gf.Vector(mesh.conn, mesh.dofs)
+    quad = gf.Element.Quad4.QuadraturePlanar(vec.AsElement(mesh.coor))
+    ne = mesh.nelem
+    u = torch.from_numpy(gf.randn(11, 2 * mesh.nnode, 0.0, 2e-3)).cuda().reshape(mesh.nnode, 2)
+    ue = vec.AsElement(u)
+    Kq = torch.full((ne, 4), 10.0, dtype=torch.float64, device="cuda")
+    fused = gf.LinearHardening(Kq, 1.0, 0.01, 0.4)
+    plain = gf.LinearHardening(Kq, 1.0, 0.01, 0.4)
+    fe = torch.empty((ne, 4, 2), dtype=torch.float64, device="cuda")
+    Ke = torch.empty((ne, 8, 8), dtype=torch.float64, device="cuda")
+    fe2, Ke2 = torch.empty_like(fe), torch.empty_like(Ke)
+    torch.cuda.synchronize()
+    t_fused = _timed(lambda: fused.newton_q4(quad, ue, fe, Ke), 10)
+
+    def unfused():
+        quad.symGradN_vector(ue, plain.Eps)
+        plain.refresh()
+        quad.Int_gradN_dot_tensor2_dV(plain.Sig, out=fe2)
+        quad.Int_gradN_dot_tensor4_dot_gradNT_dV(plain.C, out=Ke2)
+
+    t_plain = _timed(unfused, 5)
+    same = bool(torch.equal(fe, fe2) and torch.equal(Ke, Ke2))
+    frac_plastic = float((fused.epsp > 0).double().mean())
+    # algorithmic bytes per element (FP64): reads dNx 256, dV 32, u_e 64, (K, G, sigy0, H) 128,
+    # Epsp_t 288, epsp_t 32; writes Epsp 288, epsp 32, f_e 64, K_e 512
+    b_el = 1696
+    b_plain = 8576  # the unfused chain: Eps, Sig and C written and read back
+    ach = b_el * ne / t_fused / 1e9
+    res = {
+        "metric": "J2 Newton pass throughput (fused strain + return map + f_e + K_e)",
+        "value": ne / t_fused,
+        "unit": "elements/s",
+        "config": {"workload": f"Quad4 plane strain Regular {n}^2, J2 linear hardening, 4 points per element",
+                   "nelem": ne, "plastic_fraction": frac_plastic},
+        "dtype": "f64",
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "kernel": "k_plastic_newton_q4", "traffic": None,
+                     "algorithmic_bytes_per_launch": b_el * ne, "avg_launch_us": t_fused * 1e6},
+        "unfused": {"ms": t_plain * 1e3, "algorithmic_GB": b_plain * ne / 1e9, "speedup_fused": t_plain / t_fused,
+                    "bit_identical": same},
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -486,13 +541,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned",
-                                           "plastic-refresh"],
+                                           "plastic-refresh", "newton-q4"],
                     default="quad4-explicit")
     ap.add_argument("--pcg-n", type=int, default=512)
     ap.add_argument("--pcg-hex", action="store_true", help="periodic Hex8 n^3 instead of Quad4 n^2")
     ap.add_argument("--spheres", action="store_true",
                     help="stiff periodic spheres (G x10, radius 4, 20%% of elements) and pure shear 5e-4")
     ap.add_argument("--hex-n", type=int, default=128)
+    ap.add_argument("--newton-n", type=int, default=2048)
     ap.add_argument("--ws-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "goosefem_amd",
                                                          "data", "warmstart_quad4_512.npz"))
     ap.add_argument("--ws-tests", type=int, default=4, help="held-out modulus fields timed")
@@ -528,6 +584,9 @@ def main():
         return
     if args.workload == "plastic-refresh":
         bench_plastic(args)
+        return
+    if args.workload == "newton-q4":
+        bench_newton(args)
         return
 
     nx = ny = args.nx

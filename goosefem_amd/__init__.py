@@ -1483,6 +1483,19 @@ class LinearHardening(_MaterialArrays, _Handle):
     def increment(self):
         _call("gf_plastic_increment", self._h, _stream())
 
+    def newton_q4(self, quad, elemvec, fe=None, Ke=None, Sig=None):
+        """Fused pass of the statics Newton iteration on a Quad4 QuadraturePlanar (4 points):
+        strain from elemvec, refresh, element force and stiffness, without storing Eps or C.
+        Equal bit for bit to symGradN_vector -> refresh() -> Int_gradN_dot_tensor2_dV(Sig),
+        Int_gradN_dot_tensor4_dot_gradNT_dV(C).  Returns (fe, Ke); Sig is written if given (and
+        then also left in self.Sig when it is that tensor)."""
+        ev = to_device(elemvec)
+        ne = quad.nelem
+        fe = _out(fe, (ne, 4, 2))
+        Ke = _out(Ke, (ne, 8, 8))
+        _call("gf_plastic_newton_q4", self._h, quad._h, _dp(ev), _dp(fe), _dp(Ke), _dp(Sig), _stream())
+        return fe, Ke
+
     def _state(self, which, shape):
         ptrs = [_vp(), _vp(), _vp(), _vp()]
         _call("gf_plastic_state", self._h, *[ctypes.byref(p) for p in ptrs])

@@ -133,6 +133,135 @@ __global__ void __launch_bounds__(BS)
     }
 }
 
+// ---- the law fused into the Quad4 plane-strain element kernels -----------------------------------
+// One pass of a Newton iteration of the statics examples (periodic_elastoplastic.py:87-104):
+//   Eps = symGradN_vector(ue); refresh(); fe = int_gradN_dot_tensor2_dV(Sig);
+//   Ke = int_gradN_dot_tensor4_dot_gradNT_dV(C)
+// without writing Eps or C to memory.  A block owns EB elements: first one thread per quadrature
+// point forms the strain, runs the return map (updating Epsp, epsp) and leaves in LDS the point's
+// geometry, the in-plane stress and the four scalars that define its tangent; then one thread per
+// (element, node m, node n) forms the 2 x 2 block K(m., n.) and one thread per (element, node) the
+// force.  Every value follows the expression and summation order of the unfused calls
+// (Element.h:1102-1176 loops for the integrals), so the results are theirs bit for bit.
+constexpr int EB = 64, NB = 256; // elements and threads per block (EB * 4 quadrature points = NB)
+
+__global__ void __launch_bounds__(NB)
+    k_plastic_newton_q4(int64_t nelem, const double* __restrict__ dNx, const double* __restrict__ vol,
+                        const double* __restrict__ ue, const double* __restrict__ Kp, const double* __restrict__ Gp,
+                        const double* __restrict__ Sy, const double* __restrict__ Hp,
+                        const double* __restrict__ Epsp_t, const double* __restrict__ epsp_t,
+                        double* __restrict__ Epsp, double* __restrict__ epsp, double* __restrict__ fe,
+                        double* __restrict__ Ke, double* __restrict__ Sig)
+{
+    __shared__ double sG[NB][8], sV[NB], sS[NB][4], sT[NB][3], sNn[NB][4];
+    const int64_t e0 = blockIdx.x * static_cast<int64_t>(EB);
+    const int ne = static_cast<int>(nelem - e0 < EB ? nelem - e0 : EB);
+    const int t = threadIdx.x;
+    if (t < 4 * ne) {
+        const int64_t p = 4 * e0 + t; // quadrature point (element-major)
+        const double* g = dNx + 8 * p;
+        const double* u = ue + 8 * (e0 + t / 4);
+        double gv[8], uv[8];
+        for (int k = 0; k < 8; ++k) {
+            gv[k] = g[k];
+            uv[k] = u[k];
+            sG[t][k] = gv[k];
+        }
+        const double v = vol[p];
+        sV[t] = v;
+        // plane-strain symmetric gradient (the planar symGradN_vector)
+        double ee[9];
+        for (int k = 0; k < 9; ++k) ee[k] = 0.0;
+        ee[0] = gv[0] * uv[0] + gv[2] * uv[2] + gv[4] * uv[4] + gv[6] * uv[6];
+        ee[4] = gv[1] * uv[1] + gv[3] * uv[3] + gv[5] * uv[5] + gv[7] * uv[7];
+        ee[1] = 0.5 * (gv[0] * uv[1] + gv[2] * uv[3] + gv[4] * uv[5] + gv[6] * uv[7] + gv[1] * uv[0] + gv[3] * uv[2] +
+                       gv[5] * uv[4] + gv[7] * uv[6]);
+        ee[3] = ee[1];
+        // return map, as in k_plastic_refresh
+        const double* ept_ = Epsp_t + 9 * p;
+        double ep[9];
+        for (int k = 0; k < 9; ++k) {
+            ep[k] = ept_[k];
+            ee[k] = ee[k] - ep[k];
+        }
+        const double tr = ee[0] + ee[4] + ee[8];
+        const double epsm = tr / 3.0;
+        const double k_ = Kp[p], gm = Gp[p], h = Hp[p], ept = epsp_t[p];
+        double s[9], ss = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            s[k] = 2.0 * gm * (k % 4 == 0 ? ee[k] - epsm : ee[k]);
+            ss += s[k] * s[k];
+        }
+        const double q = sqrt(1.5 * ss);
+        const double phi = q - (Sy[p] + h * ept);
+        double f = 1.0, c = 0.0, dg = 0.0, n[4] = {0.0, 0.0, 0.0, 0.0};
+        if (phi > 0.0) {
+            dg = phi / (3.0 * gm + h);
+            f = 1.0 - 3.0 * gm * dg / q;
+            c = 6.0 * gm * gm * (dg / q - 1.0 / (3.0 * gm + h));
+            const double nrm = sqrt(ss);
+            n[0] = s[0] / nrm;
+            n[1] = s[1] / nrm;
+            n[2] = s[3] / nrm;
+            n[3] = s[4] / nrm;
+        }
+        const double sigm = k_ * tr;
+        const double a = 1.5 * dg / q;
+        double* oP = Epsp + 9 * p;
+        for (int k = 0; k < 9; ++k) oP[k] = phi > 0.0 ? ep[k] + a * s[k] : ep[k];
+        epsp[p] = ept + dg;
+        double sg[9];
+        for (int k = 0; k < 9; ++k) sg[k] = f * s[k] + (k % 4 == 0 ? sigm : 0.0);
+        if (Sig)
+            for (int k = 0; k < 9; ++k) Sig[9 * p + k] = sg[k];
+        sS[t][0] = sg[0];
+        sS[t][1] = sg[1];
+        sS[t][2] = sg[3];
+        sS[t][3] = sg[4];
+        sT[t][0] = k_;
+        sT[t][1] = 2.0 * gm * f;
+        sT[t][2] = c;
+        for (int k = 0; k < 4; ++k) sNn[t][k] = n[k];
+    }
+    __syncthreads();
+    // K(m *, n *): 16 node pairs per element, four per thread
+    for (int it = t; it < 16 * ne; it += NB) {
+        const int el = it / 16, m = (it % 16) / 4, nn = it % 4;
+        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        for (int qp = 0; qp < 4; ++qp) {
+            const int r = 4 * el + qp;
+            const double* g = sG[r];
+            const double v = sV[r], K0 = sT[r][0], F0 = sT[r][1], C0 = sT[r][2];
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    for (int k = 0; k < 2; ++k)
+                        for (int l = 0; l < 2; ++l) {
+                            const double dij = i == j, dkl = k == l, dik = i == k, djl = j == l, dil = i == l, djk = j == k;
+                            const double C = K0 * dij * dkl + F0 * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0) +
+                                             C0 * (sNn[r][2 * i + j] * sNn[r][2 * k + l]);
+                            acc[j][k] += g[m * 2 + i] * C * g[nn * 2 + l] * v;
+                        }
+        }
+        // two 16-byte streaming stores: K(2m + j, 2n .. 2n + 1); four neighbouring threads fill a row
+        dbl2* K = reinterpret_cast<dbl2*>(Ke + 64 * (e0 + el));
+        __builtin_nontemporal_store(dbl2{acc[0][0], acc[0][1]}, K + (m * 2) * 4 + nn);
+        __builtin_nontemporal_store(dbl2{acc[1][0], acc[1][1]}, K + (m * 2 + 1) * 4 + nn);
+    }
+    // fe(m, *): one thread per (element, node)
+    if (t < 4 * ne) {
+        const int el = t / 4, m = t % 4;
+        double f0 = 0.0, f1 = 0.0;
+        for (int qp = 0; qp < 4; ++qp) {
+            const int r = 4 * el + qp;
+            const double* g = sG[r];
+            const double v = sV[r];
+            f0 += (g[m * 2 + 0] * sS[r][0] + g[m * 2 + 1] * sS[r][2]) * v;
+            f1 += (g[m * 2 + 0] * sS[r][1] + g[m * 2 + 1] * sS[r][3]) * v;
+        }
+        __builtin_nontemporal_store(dbl2{f0, f1}, reinterpret_cast<dbl2*>(fe + 8 * (e0 + el)) + m);
+    }
+}
+
 } // namespace
 
 extern "C" {
@@ -210,6 +339,30 @@ int gf_plastic_state(gf_plastic m, const double** Epsp, const double** epsp, con
         if (epsp) *epsp = m->epsp.p;
         if (Epsp_t) *Epsp_t = m->Epsp_t.p;
         if (epsp_t) *epsp_t = m->epsp_t.p;
+    });
+}
+
+int gf_plastic_newton_q4(gf_plastic m, gf_quad q, const double* elemvec, double* fe, double* Ke, double* Sig,
+                         void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(m && q && elemvec && fe && Ke, "null argument");
+        int64_t nelem = 0;
+        int nne = 0, ndim = 0, tdim = 0, nip = 0;
+        int rc = gf_quad_sizes(q, &nelem, &nne, &ndim, &tdim, &nip);
+        if (rc != GF_OK) throw Error(rc, gf_last_error());
+        GF_REQUIRE(nne == 4 && ndim == 2 && tdim == 3 && nip == 4,
+                   "needs a Quad4 QuadraturePlanar with four quadrature points");
+        GF_REQUIRE(m->nq == 4 * nelem, "the material must hold one point per quadrature point");
+        GF_REQUIRE(aligned16(fe) && aligned16(Ke), "fe and Ke must be 16-byte aligned");
+        const double *dNx = nullptr, *dV = nullptr;
+        gf_quad_gradN(q, &dNx);
+        gf_quad_dV(q, &dV);
+        if (nelem)
+            k_plastic_newton_q4<<<nblocks(nelem, EB), NB, 0, S(stream)>>>(nelem, dNx, dV, elemvec, m->K.p, m->G.p, m->sy0.p,
+                                                                          m->H.p, m->Epsp_t.p, m->epsp_t.p, m->Epsp.p,
+                                                                          m->epsp.p, fe, Ke, Sig);
+        check_launch();
     });
 }
 

@@ -428,6 +428,75 @@ public:
     size_t nodesTopRightCorner() const { return node("nodesTopRightCorner"); }
 };
 
+/* Mesh::Quad4::FineLayer  (include/GooseFEM/MeshQuad4.h:200-1135): layers finest around the
+ * middle, coarsened by 3 in x towards the bottom and top; node sets by name via nodeset(name) */
+class FineLayer {
+public:
+    FineLayer(size_t nelx, size_t nely, double h = 1.0, size_t nfine = 1)
+        : m_nx(static_cast<int64_t>(nelx)), m_ny(static_cast<int64_t>(nely)), m_nf(static_cast<int64_t>(nfine)), m_h(h)
+    {
+        int64_t sz[4] = {0, 0, 0, 0};
+        amd::check(gf_mesh_quad4_finelayer(m_nx, m_ny, m_h, m_nf, sz, nullptr, nullptr));
+        m_nelem = static_cast<size_t>(sz[0]);
+        m_nnode = static_cast<size_t>(sz[1]);
+        m_nelx = static_cast<size_t>(sz[2]);
+        m_nely = static_cast<size_t>(sz[3]);
+    }
+    size_t nelem() const { return m_nelem; }
+    size_t nnode() const { return m_nnode; }
+    size_t nne() const { return 4; }
+    size_t ndim() const { return 2; }
+    size_t nelx() const { return m_nelx; }
+    size_t nely() const { return m_nely; }
+    double h() const { return m_h; }
+    amd::array<double> coor() const
+    {
+        amd::array<double> c({m_nnode, 2});
+        amd::check(gf_mesh_quad4_finelayer(m_nx, m_ny, m_h, m_nf, nullptr, c.data(), nullptr));
+        return c;
+    }
+    amd::array<size_t> conn() const
+    {
+        std::vector<int64_t> c(m_nelem * 4);
+        amd::check(gf_mesh_quad4_finelayer(m_nx, m_ny, m_h, m_nf, nullptr, nullptr, c.data()));
+        return amd::from_i64(c, {m_nelem, 4});
+    }
+    amd::array<size_t> dofs() const { return Mesh::dofs(m_nnode, 2); }
+    amd::array<size_t> nodeset(const std::string& name) const
+    {
+        int64_t n = 0;
+        amd::check(gf_mesh_quad4_finelayer_set(m_nx, m_ny, m_h, m_nf, name.c_str(), nullptr, &n));
+        std::vector<int64_t> r(static_cast<size_t>(n));
+        amd::check(gf_mesh_quad4_finelayer_set(m_nx, m_ny, m_h, m_nf, name.c_str(), r.data(), &n));
+        return amd::from_i64(r, {static_cast<size_t>(n)});
+    }
+    size_t node(const std::string& name) const { return nodeset(name)[0]; }
+    amd::array<size_t> elementsMiddleLayer() const { return nodeset("elementsMiddleLayer"); }
+    amd::array<size_t> nodesBottomEdge() const { return nodeset("nodesBottomEdge"); }
+    amd::array<size_t> nodesTopEdge() const { return nodeset("nodesTopEdge"); }
+    amd::array<size_t> nodesLeftEdge() const { return nodeset("nodesLeftEdge"); }
+    amd::array<size_t> nodesRightEdge() const { return nodeset("nodesRightEdge"); }
+    size_t nodesOrigin() const { return node("nodesOrigin"); }
+    amd::array<size_t> nodesPeriodic() const
+    {
+        auto r = nodeset("nodesPeriodic");
+        return amd::from_i64(amd::to_i64(r), {r.size() / 2, 2});
+    }
+    amd::array<size_t> dofsPeriodic() const
+    {
+        auto t = amd::to_i64(nodesPeriodic());
+        std::vector<int64_t> r(m_nnode * 2);
+        amd::check(gf_mesh_dofs_periodic(static_cast<int64_t>(m_nnode), 2, t.data(), static_cast<int64_t>(t.size() / 2),
+                                         r.data()));
+        return amd::from_i64(r, {m_nnode, 2});
+    }
+
+private:
+    int64_t m_nx, m_ny, m_nf;
+    double m_h;
+    size_t m_nelem = 0, m_nnode = 0, m_nelx = 0, m_nely = 0;
+};
+
 } // namespace Quad4
 
 namespace Hex8 {
@@ -836,6 +905,167 @@ public:
 };
 } // namespace Hex8
 } // namespace Element
+
+namespace Element {
+namespace Quad4 {
+
+/* Element::Quad4::QuadratureAxisymmetric  (include/GooseFEM/ElementQuad4Axisymmetric.h:33-467):
+ * nodal coordinates (z, r), tensors 3 x 3 in (r, theta, z) order */
+class QuadratureAxisymmetric {
+public:
+    template <class T>
+    explicit QuadratureAxisymmetric(const T& x) : QuadratureAxisymmetric(x, Gauss::xi(), Gauss::w())
+    {
+    }
+    template <class T, class X, class W>
+    QuadratureAxisymmetric(const T& x, const X& xi, const W& w)
+    {
+        m_nelem = x.shape()[0];
+        m_nip = w.size();
+        std::vector<double> xih(xi.data(), xi.data() + xi.size()), wh(w.data(), w.data() + w.size());
+        amd::dev_in<T> in(x);
+        amd::check(gf_axi_create(&m_h, static_cast<int64_t>(m_nelem), static_cast<int>(m_nip), in.ptr, xih.data(),
+                                 wh.data(), nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    QuadratureAxisymmetric(const QuadratureAxisymmetric&) = delete;
+    QuadratureAxisymmetric& operator=(const QuadratureAxisymmetric&) = delete;
+    ~QuadratureAxisymmetric() { gf_axi_destroy(m_h); }
+    size_t nelem() const { return m_nelem; }
+    size_t nne() const { return 4; }
+    size_t ndim() const { return 2; }
+    size_t tdim() const { return 3; }
+    size_t nip() const { return m_nip; }
+    std::vector<size_t> shape_qtensor2() const { return {m_nelem, m_nip, 3, 3}; }
+    std::vector<size_t> shape_qtensor4() const { return {m_nelem, m_nip, 3, 3, 3, 3}; }
+    template <class T>
+    void update_x(const T& x)
+    {
+        amd::dev_in<T> in(x);
+        amd::check(gf_axi_update_x(m_h, in.ptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    amd::array<double> dV() const
+    {
+        const double* p = nullptr;
+        amd::check(gf_axi_geometry(m_h, nullptr, &p));
+        amd::array<double> r({m_nelem, m_nip});
+        amd::check(gf_memcpy_d2h(r.data(), p, static_cast<int64_t>(r.size() * sizeof(double))));
+        return r;
+    }
+    template <class T>
+    amd::array<double> GradN_vector(const T& ue) const { return grad(0, ue); }
+    template <class T>
+    amd::array<double> GradN_vector_T(const T& ue) const { return grad(1, ue); }
+    template <class T>
+    amd::array<double> SymGradN_vector(const T& ue) const { return grad(2, ue); }
+    template <class T, class R>
+    void symGradN_vector(const T& ue, R& eps) const { grad_into(2, ue, eps); }
+    template <class T>
+    amd::array<double> Int_N_scalar_NT_dV(const T& qs) const { return integrate(0, qs, {m_nelem, 8, 8}); }
+    template <class T>
+    amd::array<double> Int_gradN_dot_tensor2_dV(const T& qt) const { return integrate(1, qt, {m_nelem, 4, 2}); }
+    template <class T>
+    amd::array<double> Int_gradN_dot_tensor4_dot_gradNT_dV(const T& qt) const { return integrate(2, qt, {m_nelem, 8, 8}); }
+
+private:
+    template <class T>
+    amd::array<double> grad(int mode, const T& ue) const
+    {
+        amd::array<double> r(shape_qtensor2());
+        grad_into(mode, ue, r);
+        return r;
+    }
+    template <class T, class R>
+    void grad_into(int mode, const T& ue, R& ret) const
+    {
+        amd::dev_in<T> in(ue);
+        amd::dev_out<R> out(ret, shape_qtensor2());
+        amd::check(gf_axi_grad(m_h, mode, in.ptr, out.ptr, nullptr));
+        out.finish();
+    }
+    template <class T>
+    amd::array<double> integrate(int which, const T& q, std::vector<size_t> shape) const
+    {
+        amd::array<double> r(std::move(shape));
+        amd::dev_in<T> in(q);
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        if (which == 0) amd::check(gf_axi_int_N_scalar_NT_dV(m_h, in.ptr, out.ptr, nullptr));
+        else if (which == 1) amd::check(gf_axi_int_gradN_dot_tensor2_dV(m_h, in.ptr, out.ptr, nullptr));
+        else amd::check(gf_axi_int_gradN_dot_tensor4_dot_gradNT_dV(m_h, in.ptr, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+    gf_axi m_h = nullptr;
+    size_t m_nelem = 0, m_nip = 0;
+};
+
+} // namespace Quad4
+} // namespace Element
+
+// =================================================================================================
+// Material laws the examples take from GMat (not part of GooseFEM), with GMat's interface:
+// Eps(), Sig(), C() as device arrays, refresh(), increment()
+// =================================================================================================
+namespace material {
+
+/* J2 elasto-plasticity with linear isotropic hardening, as
+ * GMatElastoPlastic::Cartesian3d::LinearHardening (docs/examples/statics/periodic_elastoplastic.py:71-104) */
+class LinearHardening {
+public:
+    template <class K, class G, class S, class H>
+    LinearHardening(const K& kappa, const G& mu, const S& sigy0, const H& hard)
+    {
+        m_shape = amd::shape_of(kappa);
+        m_nq = kappa.size();
+        amd::dev_in<K> k(kappa);
+        amd::dev_in<G> g(mu);
+        amd::dev_in<S> sy(sigy0);
+        amd::dev_in<H> hh(hard);
+        amd::check(gf_plastic_create(&m_h, static_cast<int64_t>(m_nq), k.ptr, g.ptr, sy.ptr, hh.ptr, nullptr));
+        auto s2 = m_shape, s4 = m_shape;
+        s2.insert(s2.end(), {3, 3});
+        s4.insert(s4.end(), {3, 3, 3, 3});
+        m_Eps = amd::device_array<double>(s2);
+        m_Sig = amd::device_array<double>(s2);
+        m_C = amd::device_array<double>(s4);
+        amd::array<double> z(s2);
+        amd::check(gf_memcpy_h2d(m_Eps.data(), z.data(), static_cast<int64_t>(m_Eps.bytes())));
+    }
+    LinearHardening(const LinearHardening&) = delete;
+    LinearHardening& operator=(const LinearHardening&) = delete;
+    ~LinearHardening() { gf_plastic_destroy(m_h); }
+    amd::device_array<double>& Eps() { return m_Eps; }
+    const amd::device_array<double>& Sig() const { return m_Sig; }
+    const amd::device_array<double>& C() const { return m_C; }
+    void refresh(bool compute_tangent = true)
+    {
+        amd::check(gf_plastic_refresh(m_h, m_Eps.data(), m_Sig.data(), compute_tangent ? m_C.data() : nullptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    void increment()
+    {
+        amd::check(gf_plastic_increment(m_h, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    /* equivalent plastic strain (host copy) */
+    amd::array<double> epsp() const
+    {
+        const double* p = nullptr;
+        amd::check(gf_plastic_state(m_h, nullptr, &p, nullptr, nullptr));
+        amd::array<double> r(m_shape);
+        amd::check(gf_memcpy_d2h(r.data(), p, static_cast<int64_t>(r.size() * sizeof(double))));
+        return r;
+    }
+
+private:
+    gf_plastic m_h = nullptr;
+    std::vector<size_t> m_shape;
+    size_t m_nq = 0;
+    amd::device_array<double> m_Eps, m_Sig, m_C;
+};
+
+} // namespace material
 
 // =================================================================================================
 // MatrixDiagonal (include/GooseFEM/MatrixDiagonal.h) and Matrix (include/GooseFEM/Matrix.h)

@@ -1,7 +1,8 @@
 // Drop-in check of the C++ face (include/GooseFEM/GooseFEM.h): the same calls and checks as the
 // reference's C++ unit tests (tests/basic/ElementQuad4Planar.cpp, ElementHex8.cpp,
 // MatrixDiagonal.cpp, Matrix.cpp, Vector.cpp, MeshQuad4.cpp, VectorPartitioned.cpp,
-// MatrixDiagonalPartitioned.cpp, MatrixParitioned.cpp), against the GPU path.
+// MatrixDiagonalPartitioned.cpp, MatrixParitioned.cpp), the FineLayer mesh, the axisymmetric
+// element and the examples' J2 material law, against the GPU path.
 #include <GooseFEM/GooseFEM.h>
 
 #include <cmath>
@@ -210,6 +211,53 @@ int main()
         auto R = A.Reaction(x, b);
         ok = true;
         for (size_t d = 0; d < A.nnp(); ++d) ok &= std::abs(R[A.iip()[d]] - b[A.iip()[d]]) < 1e-12;
+        REQUIRE(ok);
+    }
+    // MeshQuad4.cpp:103-256: FineLayer(9, 18) sizes, middle layer, periodic pairs
+    {
+        GF::Mesh::Quad4::FineLayer mesh(9, 18);
+        REQUIRE(mesh.nelem() == 81 && mesh.nnode() == 96 && mesh.nelx() == 9 && mesh.nely() == 21);
+        auto mid = mesh.elementsMiddleLayer();
+        REQUIRE(mid.size() == 9 && mid[0] == 36 && mid[8] == 44);
+        auto per = mesh.nodesPeriodic();
+        REQUIRE(per.shape(0) == 15 && per(1, 1) == 95 && per(14, 0) == 88 && per(14, 1) == 91);
+        auto conn = mesh.conn();
+        REQUIRE(conn(7, 1) == 21 && conn(63, 3) == 84);
+    }
+    // QuadratureAxisymmetric: the volume of a hollow cylinder, and the material law's consistency
+    {
+        GF::Mesh::Quad4::Regular mesh(4, 3);
+        auto coor = mesh.coor();
+        for (size_t n = 0; n < mesh.nnode(); ++n) {
+            coor(n, 0) = 0.5 * coor(n, 0);       // z in [0, 2]
+            coor(n, 1) = 0.5 + coor(n, 1) / 3.0; // r in [0.5, 1.5]
+        }
+        GF::Vector vec(mesh.conn(), mesh.dofs());
+        GF::Element::Quad4::QuadratureAxisymmetric quad(vec.AsElement(coor));
+        auto dV = quad.dV();
+        double V = 0.0;
+        for (size_t i = 0; i < dV.size(); ++i) V += dV[i];
+        REQUIRE(std::abs(V - M_PI * (1.5 * 1.5 - 0.25) * 2.0) < 1e-12);
+        GF::array_type::tensor<double, 2> K({quad.nelem(), quad.nip()}, 10.0), G({quad.nelem(), quad.nip()}, 1.0),
+            sy({quad.nelem(), quad.nip()}, 0.01), H({quad.nelem(), quad.nip()}, 0.5);
+        GF::material::LinearHardening mat(K, G, sy, H);
+        GF::array_type::tensor<double, 2> u(vec.shape_nodevec());
+        for (size_t n = 0; n < mesh.nnode(); ++n) u(n, 1) = 0.02 * coor(n, 1); // radial expansion
+        quad.symGradN_vector(vec.AsElement(u), mat.Eps());
+        mat.refresh();
+        auto ep = mat.epsp();
+        auto sig = mat.Sig().to_host();
+        bool ok = true;
+        for (size_t i = 0; i < ep.size(); ++i) {
+            const double* s9 = sig.data() + 9 * i;
+            const double m = (s9[0] + s9[4] + s9[8]) / 3.0;
+            double ss = 0.0;
+            for (int k = 0; k < 9; ++k) {
+                const double d = s9[k] - (k % 4 == 0 ? m : 0.0);
+                ss += d * d;
+            }
+            ok &= ep[i] > 0.0 && std::abs(std::sqrt(1.5 * ss) - (0.01 + 0.5 * ep[i])) < 1e-12;
+        }
         REQUIRE(ok);
     }
     // ElementHex8.cpp: dV and periodic lumped mass of ones; BF16x3 stiffness close to FP64

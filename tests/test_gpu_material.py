@@ -223,3 +223,48 @@ def test_elastoplastic_newton_against_oracle(gf):
     assert max(it_cpu) <= 6
     assert np.max(np.abs(d_gpu - d_cpu)) <= 1e-9 * np.max(np.abs(d_cpu))
     assert np.max(np.abs(ep_gpu - ep_cpu)) <= 1e-7 * np.max(ep_cpu)
+
+
+@pytest.mark.parametrize("nx,ny", [(13, 9), (64, 33)])
+def test_fused_newton_pass_bit_exact(gf, nx, ny):
+    # gf_plastic_newton_q4 equals the unfused chain bit for bit (fe, Ke, Sig, plastic state), over two
+    # increments, on a distorted mesh with mixed elastic / plastic points; and the oracle's chain
+    coor, conn = oracle.quad4_regular(nx, ny)
+    rng = np.random.default_rng(7)
+    coor = coor + 0.2 * rng.uniform(-1, 1, coor.shape)
+    dofs = oracle.dofs(coor.shape[0], 2)
+    vec = gf.Vector(conn, dofs)
+    x = vec.AsElement(coor)
+    quad = gf.Element.Quad4.QuadraturePlanar(x)
+    shape = (conn.shape[0], 4)
+    K = rng.uniform(5, 15, shape)
+    G = rng.uniform(0.5, 1.5, shape)
+    sy = rng.uniform(0.005, 0.02, shape)
+    H = rng.uniform(0.0, 1.0, shape)
+    fused = gf.LinearHardening(K, G, sy, H)
+    plain = gf.LinearHardening(K, G, sy, H)
+    ref = oracle.LinearHardening(K, G, sy, H)
+    oq = oracle.Quadrature(QUAD4_PLANAR, host(x))
+    import torch
+
+    for inc in range(2):
+        u = rng.standard_normal(coor.shape) * 0.002 * (inc + 1)
+        ue = vec.AsElement(u)
+        fused.increment()
+        plain.increment()
+        ref.increment()
+        Sig = torch.empty_like(fused.Sig)
+        fe, Ke = fused.newton_q4(quad, ue, Sig=Sig)
+        quad.symGradN_vector(ue, plain.Eps)
+        plain.refresh()
+        fe2 = quad.Int_gradN_dot_tensor2_dV(plain.Sig)
+        Ke2 = quad.Int_gradN_dot_tensor4_dot_gradNT_dV(plain.C)
+        assert np.array_equal(host(fe), host(fe2)) and np.array_equal(host(Ke), host(Ke2))
+        assert np.array_equal(host(Sig), host(plain.Sig))
+        assert np.array_equal(host(fused.Epsp), host(plain.Epsp)) and np.array_equal(host(fused.epsp), host(plain.epsp))
+        ref.Eps = oq.SymGradN_vector(host(ue))
+        ref.refresh()
+        assert np.array_equal(host(fe), oq.Int_gradN_dot_tensor2_dV(ref.Sig))
+        assert np.array_equal(host(Ke), oq.Int_gradN_dot_tensor4_dot_gradNT_dV(ref.C))
+    ep = host(fused.epsp)
+    assert 0.1 < np.mean(ep > 0) < 0.95
