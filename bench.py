@@ -523,7 +523,7 @@ def bench_newton(args):
                    "nelem": ne, "plastic_fraction": frac_plastic},
         "dtype": "f64",
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "kernel": "k_plastic_newton_q4", "traffic": None,
+                     "kernel": "k_plastic_newton_q4", "traffic": load_traffic(n, "newton_q4_2048_traffic.json", "n"),
                      "algorithmic_bytes_per_launch": b_el * ne, "avg_launch_us": t_fused * 1e6},
         "unfused": {"ms": t_plain * 1e3, "algorithmic_GB": b_plain * ne / 1e9, "speedup_fused": t_plain / t_fused,
                     "bit_identical": same},
