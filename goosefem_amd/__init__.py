@@ -392,11 +392,34 @@ def _nodaltyings(dofs):
     return _csr_lists(ptr, idx)
 
 
+def _nodal_mass(coor, conn):
+    """Mesh::nodal_mass (Mesh.h:2732-2752): the lumped (nodal-quadrature) mass of unit density per
+    node and direction, Quad4 only (the reference's only implemented type), on the GPU."""
+    coor = np.asarray(coor, dtype=np.float64)
+    conn = _i64a(conn)
+    assert conn.shape[1] == 4 and coor.shape[1] == 2, "nodal_mass: Quad4 meshes only"
+    dofs = np.arange(coor.size, dtype=np.int64).reshape(coor.shape)
+    vec = Vector(conn, dofs)
+    M = MatrixDiagonal(conn, dofs)
+    quad = Element.Quad4.Quadrature(vec.AsElement(coor), Element.Quad4.Nodal.xi(), Element.Quad4.Nodal.w())
+    M.assemble(quad.Int_N_scalar_NT_dV(np.ones(conn.shape)))
+    return vec.AsNode(M.data)
+
+
+def _center_of_gravity(coor, conn):
+    """Mesh::center_of_gravity (Mesh.h:2809-2831): nodal-mass weighted mean of the coordinates."""
+    m = _nodal_mass(coor, conn)
+    c = to_device(coor)
+    return (c * m).sum(0) / m.sum(0)
+
+
 Mesh = SimpleNamespace(
     Quad4=SimpleNamespace(Regular=_Quad4Regular, FineLayer=_Quad4FineLayer),
     Hex8=SimpleNamespace(Regular=_Hex8Regular),
     dofs=lambda nnode, ndim: np.arange(nnode * ndim, dtype=np.int64).reshape(nnode, ndim),
     renumber=_renumber,
+    nodal_mass=_nodal_mass,
+    center_of_gravity=_center_of_gravity,
     coordination=_coordination,
     elem2node=_elem2node,
     nodaltyings=_nodaltyings,
@@ -712,7 +735,55 @@ class VectorPartitioned(Vector):
 # -------------------------------------------------------------------------------------------------
 
 
-class _Quadrature(_Handle):
+class _QuadShapes:
+    """Shapes, allocation and AsTensor of QuadratureBase (Element.h:201-420), on the GPU."""
+
+    def shape_elemvec(self, arg=None):
+        return (self.nelem, self.nne, self.ndim if arg is None else arg)
+
+    def shape_elemmat(self):
+        return (self.nelem, self.nne * self.ndim, self.nne * self.ndim)
+
+    def shape_qscalar(self):
+        return (self.nelem, self.nip)
+
+    def shape_qvector(self, arg=None):
+        return (self.nelem, self.nip, self.tdim if arg is None else arg)
+
+    def shape_qtensor(self, rank=2, arg=None):
+        return (self.nelem, self.nip) + ((self.tdim if arg is None else arg),) * rank
+
+    def _alloc(self, shape, val):
+        torch = _torch()
+        if val is None:
+            return _empty(shape)
+        return torch.full(shape, float(val), dtype=torch.float64, device=device())
+
+    def allocate_elemvec(self, val=None):
+        return self._alloc(self.shape_elemvec(), val)
+
+    def allocate_elemmat(self, val=None):
+        return self._alloc(self.shape_elemmat(), val)
+
+    def allocate_qscalar(self, val=None):
+        return self._alloc(self.shape_qscalar(), val)
+
+    def allocate_qtensor(self, rank=2, val=None):
+        return self._alloc(self.shape_qtensor(rank), val)
+
+    def AsTensor(self, rank, qscalar):
+        """AsTensor(rank, qscalar) (Element.h:234-245, Allocate.h:158-170): every component of the
+        rank-`rank` tensor at a point equals that point's scalar."""
+        qs = to_device(qscalar)
+        assert tuple(qs.shape) == self.shape_qscalar()
+        return qs.reshape(qs.shape + (1,) * rank).expand(self.shape_qtensor(rank)).contiguous()
+
+    def asTensor(self, qscalar, qtensor):
+        rank = qtensor.dim() - 2
+        return _inplace(qtensor, lambda r: r.copy_(self.AsTensor(rank, qscalar)))
+
+
+class _Quadrature(_QuadShapes, _Handle):
     _destroy = "gf_quad_destroy"
     _etype = None
 
@@ -758,9 +829,6 @@ class _Quadrature(_Handle):
         xd = to_device(x)  # held until the call has enqueued its kernels
         _call("gf_quad_update_x", self._h, _dp(xd), _stream())
 
-    def shape_qtensor(self, rank=2):
-        return (self.nelem, self.nip) + (self.tdim,) * rank
-
     def _grad(self, fn, elemvec, out=None):
         torch = _torch()
         ev = elemvec.contiguous() if isinstance(elemvec, torch.Tensor) and elemvec.is_cuda else to_device(elemvec)
@@ -777,26 +845,35 @@ class _Quadrature(_Handle):
     def SymGradN_vector(self, elemvec, out=None):
         return self._grad("gf_quad_symgradN_vector", elemvec, out)
 
-    def InterpQuad_vector(self, elemvec):
+    def InterpQuad_vector(self, elemvec, out=None):
         ev = to_device(elemvec)
         n = ev.shape[2]
-        out = _empty((self.nelem, self.nip, n))
+        out = _out(out, (self.nelem, self.nip, n))
         _call("gf_quad_interp_quad_vector", self._h, _dp(ev), _i32(n), _dp(out), _stream())
         return out
 
-    def Int_N_vector_dV(self, qvector):
+    def Int_N_vector_dV(self, qvector, out=None):
         qv = to_device(qvector)
         n = qv.shape[2]
-        out = _empty((self.nelem, self.nne, n))
+        out = _out(out, (self.nelem, self.nne, n))
         _call("gf_quad_int_N_vector_dV", self._h, _dp(qv), _i32(n), _dp(out), _stream())
         return out
 
-    def Int_N_scalar_NT_dV(self, qscalar):
+    def Int_N_scalar_NT_dV(self, qscalar, out=None):
         qs = to_device(qscalar)
         K = self.nne * self.ndim
-        out = _empty((self.nelem, K, K))
+        out = _out(out, (self.nelem, K, K))
         _call("gf_quad_int_N_scalar_NT_dV", self._h, _dp(qs), _dp(out), _stream())
         return out
+
+    def interpQuad_vector(self, elemvec, qvector):
+        return _inplace(qvector, lambda r: self.InterpQuad_vector(elemvec, out=r))
+
+    def int_N_vector_dV(self, qvector, elemvec):
+        return _inplace(elemvec, lambda r: self.Int_N_vector_dV(qvector, out=r))
+
+    def int_N_scalar_NT_dV(self, qscalar, elemmat):
+        return _inplace(elemmat, lambda r: self.Int_N_scalar_NT_dV(qscalar, out=r))
 
     def Int_gradN_dot_tensor2_dV(self, qtensor, out=None):
         torch = _torch()
@@ -888,7 +965,7 @@ class _Hex8Quadrature(_Quadrature):
         super().__init__(x, xi, w)
 
 
-class _Quad4QuadratureAxisymmetric(_Handle):
+class _Quad4QuadratureAxisymmetric(_QuadShapes, _Handle):
     """Element::Quad4::QuadratureAxisymmetric (ElementQuad4Axisymmetric.h:33-467): nodal coordinates
     (z, r), tensors 3 x 3 in (r, theta, z) order; all operators on the GPU, bit-exact in FP64."""
 
@@ -910,9 +987,6 @@ class _Quad4QuadratureAxisymmetric(_Handle):
     def update_x(self, x):
         self._x = to_device(x)
         _call("gf_axi_update_x", self._h, _dp(self._x), _stream())
-
-    def shape_qtensor(self, rank=2):
-        return (self.nelem, self.nip) + (3,) * rank
 
     def _geometry(self):
         G, dV = _vp(), _vp()

@@ -268,3 +268,26 @@ def test_fused_newton_pass_bit_exact(gf, nx, ny):
         assert np.array_equal(host(Ke), oq.Int_gradN_dot_tensor4_dot_gradNT_dV(ref.C))
     ep = host(fused.epsp)
     assert 0.1 < np.mean(ep > 0) < 0.95
+
+
+def test_quadrature_shapes_and_astensor(gf):
+    # QuadratureBase shapes, allocation, AsTensor (Element.h:201-420, Allocate.h:158-170) and the
+    # write-into forms with numpy outputs, as the examples use them (fixed-displacement_elastic.py)
+    mesh = gf.Mesh.Quad4.Regular(3, 2)
+    vec = gf.Vector(mesh.conn, mesh.dofs)
+    q = gf.Element.Quad4.QuadraturePlanar(vec.AsElement(mesh.coor))
+    assert q.shape_elemvec() == (6, 4, 2) and q.shape_elemvec(3) == (6, 4, 3)
+    assert q.shape_elemmat() == (6, 8, 8) and q.shape_qscalar() == (6, 4)
+    assert q.shape_qvector() == (6, 4, 3) and q.shape_qtensor(2) == (6, 4, 3, 3) and q.shape_qtensor(4, 2) == (6, 4, 2, 2, 2, 2)
+    assert tuple(q.allocate_qtensor(2, 0.0).shape) == (6, 4, 3, 3) and float(q.allocate_elemmat(1.5).sum()) == 6 * 64 * 1.5
+    dV = host(q.dV)
+    T = host(q.AsTensor(2, q.dV))
+    assert T.shape == (6, 4, 3, 3) and np.all(T == dV[:, :, None, None])
+    M = np.zeros((6, 8, 8))
+    q.int_N_scalar_NT_dV(np.ones((6, 4)), M)
+    assert np.isclose(M.sum(), 2 * 3 * 2)  # twice the area
+    ue = np.zeros((6, 4, 2))
+    vec.asElement(mesh.coor, ue)
+    assert np.array_equal(ue, host(vec.AsElement(mesh.coor)))
+    ax = gf.Element.Quad4.QuadratureAxisymmetric(vec.AsElement(mesh.coor + np.array([0.0, 1.0])))
+    assert ax.shape_qtensor(2) == (6, 4, 3, 3) and tuple(ax.AsTensor(2, ax.dV).shape) == (6, 4, 3, 3)

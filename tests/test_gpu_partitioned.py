@@ -257,3 +257,21 @@ def test_matrix_partitioned_fp32_assembly(gf):
     a64.assemble(em32.astype(np.float64))
     for b in range(4):
         assert np.array_equal(host(a32.csr(b)[2]), host(a64.csr(b)[2]))
+
+
+def test_reference_python_mesh_partitioned_and_center_of_gravity(gf):
+    # tests/test_MeshPartitioned.py:10-39 (MatrixPartitioned vs Matrix Todense; the solve round trip,
+    # with SPD element matrices for the device CG) and tests/test_Mesh.py:12-21 (center of gravity)
+    mesh = gf.Mesh.Quad4.Regular(2, 2)
+    rng = np.random.default_rng(9)
+    a = random_symmetric(rng, mesh.nelem, 8, spd=True)
+    A = gf.MatrixPartitioned(mesh.conn, mesh.dofs, [0, 5, 7, 13])
+    B = gf.Matrix(mesh.conn, mesh.dofs)
+    A.assemble(a)
+    B.assemble(a)
+    assert np.array_equal(host(A.Todense()), host(B.Todense()))
+    x = rng.random(mesh.nnode * 2)
+    assert np.allclose(host(gf.MatrixPartitionedSolver().Solve(A, A.Dot(x), x)), x)
+    m3 = gf.Mesh.Quad4.Regular(3, 3)
+    assert np.allclose(host(gf.Mesh.center_of_gravity(m3.coor, m3.conn)), [1.5, 1.5])
+    assert np.isclose(float(gf.Mesh.nodal_mass(m3.coor, m3.conn).sum()), 2 * 9.0)
