@@ -413,6 +413,35 @@ def _center_of_gravity(coor, conn):
     return (c * m).sum(0) / m.sum(0)
 
 
+class _StopList:
+    """Iterate::StopList (include/GooseFEM/Iterate.h): the last n residuals of an iteration, for a
+    stopping test (all below a tolerance, and optionally descending)."""
+
+    def __init__(self, n=1):
+        self.reset(n)
+
+    def reset(self, n=None):
+        if n is not None:
+            self._n = int(n)
+        self._res = [float("inf")] * self._n
+
+    def roll_insert(self, res):
+        self._res = self._res[1:] + [float(res)]
+
+    def descending(self):
+        return all(a >= b for a, b in zip(self._res, self._res[1:]))
+
+    def all_less(self, tol):
+        return not any(r >= tol for r in self._res)
+
+    @property
+    def data(self):
+        return list(self._res)
+
+
+Iterate = SimpleNamespace(StopList=_StopList)
+
+
 Mesh = SimpleNamespace(
     Quad4=SimpleNamespace(Regular=_Quad4Regular, FineLayer=_Quad4FineLayer),
     Hex8=SimpleNamespace(Regular=_Hex8Regular),

@@ -20,6 +20,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <initializer_list>
+#include <limits>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -1674,6 +1675,45 @@ private:
 };
 
 } // namespace Tyings
+
+// =================================================================================================
+// Iterate::StopList  (include/GooseFEM/Iterate.h): the last n residuals of an iteration
+// =================================================================================================
+namespace Iterate {
+
+class StopList {
+public:
+    explicit StopList(size_t n = 1) { reset(n); }
+    void reset() { std::fill(m_res.begin(), m_res.end(), std::numeric_limits<double>::infinity()); }
+    void reset(size_t n)
+    {
+        m_res.assign(n, std::numeric_limits<double>::infinity());
+    }
+    void roll_insert(double res)
+    {
+        if (m_res.empty()) return;
+        for (size_t i = 1; i < m_res.size(); ++i) m_res[i - 1] = m_res[i];
+        m_res.back() = res;
+    }
+    bool descending() const
+    {
+        for (size_t i = 1; i < m_res.size(); ++i)
+            if (m_res[i] > m_res[i - 1]) return false;
+        return true;
+    }
+    bool all_less(double tol) const
+    {
+        for (double r : m_res)
+            if (r >= tol) return false;
+        return true;
+    }
+    const std::vector<double>& data() const { return m_res; }
+
+private:
+    std::vector<double> m_res;
+};
+
+} // namespace Iterate
 
 } // namespace GooseFEM
 
