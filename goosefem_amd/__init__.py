@@ -392,6 +392,50 @@ def _nodaltyings(dofs):
     return _csr_lists(ptr, idx)
 
 
+class _Tri3Regular(_Quad4Regular):
+    """Mesh::Tri3::Regular (include/GooseFEM/MeshTri3.h:26-232): the Quad4 Regular grid's nodes,
+    node sets and periodic pairs, each square split into two triangles (host set-up)."""
+
+    def __init__(self, nelx, nely, h=1.0):
+        super().__init__(nelx, nely, h)
+        self.nne = 3
+        self.nelem = 2 * self.nelx * self.nely
+
+    @property
+    def conn(self):
+        nx = self.nelx
+        iy, ix = np.meshgrid(np.arange(self.nely), np.arange(nx), indexing="ij")
+        n0 = (iy * (nx + 1) + ix).ravel()
+        out = np.empty((self.nelem, 3), dtype=np.int64)
+        out[0::2] = np.stack((n0, n0 + 1, n0 + nx + 1), axis=1)
+        out[1::2] = np.stack((n0 + 1, n0 + nx + 2, n0 + nx + 1), axis=1)
+        return out
+
+
+def _tri3_get_orientation(coor, conn):
+    """Mesh::Tri3::getOrientation (MeshTri3.h:240-268): -1 for clockwise elements, +1 otherwise."""
+    coor, conn = np.asarray(coor, dtype=np.float64), _i64a(conn)
+    v1 = coor[conn[:, 0]] - coor[conn[:, 1]]
+    v2 = coor[conn[:, 2]] - coor[conn[:, 1]]
+    k = v1[:, 0] * v2[:, 1] - v2[:, 0] * v1[:, 1]
+    return np.where(k < 0, -1, 1).astype(np.int32)
+
+
+def _tri3_set_orientation(coor, conn, *args):
+    """Mesh::Tri3::setOrientation(coor, conn[, val], orientation=-1) (MeshTri3.h:279-326): swap the
+    last two nodes of the elements whose orientation differs from the requested one."""
+    if len(args) == 2:
+        val, orientation = args
+    else:
+        orientation = args[0] if args else -1
+        val = _tri3_get_orientation(coor, conn)
+    assert orientation in (-1, 1)
+    out = _i64a(conn).copy()
+    flip = (np.asarray(val) > 0) if orientation == -1 else (np.asarray(val) < 0)
+    out[flip, 1], out[flip, 2] = out[flip, 2].copy(), out[flip, 1].copy()
+    return out
+
+
 def _nodal_mass(coor, conn):
     """Mesh::nodal_mass (Mesh.h:2732-2752): the lumped (nodal-quadrature) mass of unit density per
     node and direction, Quad4 only (the reference's only implemented type), on the GPU."""
@@ -445,6 +489,8 @@ Iterate = SimpleNamespace(StopList=_StopList)
 Mesh = SimpleNamespace(
     Quad4=SimpleNamespace(Regular=_Quad4Regular, FineLayer=_Quad4FineLayer),
     Hex8=SimpleNamespace(Regular=_Hex8Regular),
+    Tri3=SimpleNamespace(Regular=_Tri3Regular, getOrientation=_tri3_get_orientation,
+                         setOrientation=_tri3_set_orientation),
     dofs=lambda nnode, ndim: np.arange(nnode * ndim, dtype=np.int64).reshape(nnode, ndim),
     renumber=_renumber,
     nodal_mass=_nodal_mass,

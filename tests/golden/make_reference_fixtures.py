@@ -8,6 +8,7 @@ committed and is all that tests read at run time:
 Sources (relative to the reference root):
   tests/basic/MeshQuad4.cpp:10-100      Regular(5,3): coor, conn, dofs, node sets, periodic
   tests/basic/MeshQuad4.cpp:103-256     FineLayer(9,18): coor, conn, dofs, node sets, periodic
+  tests/basic/MeshTri3.cpp:6-88         Tri3 Regular(5,3): coor, conn, dofs, node sets, periodic
   tests/basic/MeshHex8.cpp:8-188        Regular(3,3,3): coor, conn, node sets, dofs, periodic
   tests/basic/Mesh.cpp:199-236          Regular(3,3): coordination, elem2node
   tests/test_Mesh.py:23-171             Regular(4,4) periodic: coordination, node2dof,
@@ -88,6 +89,9 @@ def main(root: str) -> None:
     fx["MeshQuad4_Regular_5_3"] = cpp_arrays(section(src, "Regular"))
 
     fx["MeshQuad4_FineLayer_9_18"] = cpp_arrays(section(src, "FineLayer"))
+
+    src = open(os.path.join(basic, "MeshTri3.cpp")).read()
+    fx["MeshTri3_Regular_5_3"] = cpp_arrays(section(src, "Regular"))
 
     src = open(os.path.join(basic, "MeshHex8.cpp")).read()
     fx["MeshHex8_Regular_3_3_3"] = cpp_arrays(section(src, "Regular"))

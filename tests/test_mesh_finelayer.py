@@ -72,3 +72,25 @@ def test_finelayer_invariants(nelx, nely, nfine):
     assert np.all(np.isclose(np.abs(d[:, 0]), Lx) | np.isclose(d[:, 0], 0))
     assert np.all(np.isclose(np.abs(d[:, 1]), Ly) | np.isclose(d[:, 1], 0))
     assert P.shape[0] == 3 + m.nodesBottomOpenEdge.size + m.nodesLeftOpenEdge.size
+
+
+def test_tri3_regular_golden(fixtures):
+    # tests/basic/MeshTri3.cpp:6-88, and the orientation helpers (MeshTri3.h:240-326)
+    g = fixtures["MeshTri3_Regular_5_3"]
+    m = gf.Mesh.Tri3.Regular(5, 3)
+    # nelem_ is written "15 * 2" in the reference test (MeshTri3.cpp:21)
+    assert (m.nelem, m.nnode, m.nne, m.ndim) == (30, g["nnode_"], g["nne_"], g["ndim_"])
+    assert np.allclose(m.coor, np.array(g["coor_"]))
+    assert np.array_equal(m.conn, np.array(g["conn_"], dtype=np.int64))
+    assert np.array_equal(m.dofs, np.array(g["dofs_"]))
+    for name in SETS:
+        assert np.array_equal(getattr(m, name), np.array(g[name + "_"])), name
+    for name in CORNERS:
+        assert getattr(m, name) == g[name + "_"], name
+    assert np.array_equal(m.nodesPeriodic, np.array(g["nodesPeriodic_"]))
+    assert np.array_equal(m.dofsPeriodic, np.array(g["dofsPeriodic_"]))
+    o = gf.Mesh.Tri3.getOrientation(m.coor, m.conn)
+    assert np.all(o == 1)  # counter-clockwise
+    cw = gf.Mesh.Tri3.setOrientation(m.coor, m.conn, -1)
+    assert np.all(gf.Mesh.Tri3.getOrientation(m.coor, cw) == -1)
+    assert np.array_equal(gf.Mesh.Tri3.setOrientation(m.coor, cw, +1), m.conn)
