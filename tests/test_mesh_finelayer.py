@@ -89,8 +89,11 @@ def test_tri3_regular_golden(fixtures):
         assert getattr(m, name) == g[name + "_"], name
     assert np.array_equal(m.nodesPeriodic, np.array(g["nodesPeriodic_"]))
     assert np.array_equal(m.dofsPeriodic, np.array(g["dofsPeriodic_"]))
+    # k = (x0 - x1) x (x2 - x1) is negative for the mesh's counter-clockwise triangles: -1
     o = gf.Mesh.Tri3.getOrientation(m.coor, m.conn)
-    assert np.all(o == 1)  # counter-clockwise
-    cw = gf.Mesh.Tri3.setOrientation(m.coor, m.conn, -1)
-    assert np.all(gf.Mesh.Tri3.getOrientation(m.coor, cw) == -1)
-    assert np.array_equal(gf.Mesh.Tri3.setOrientation(m.coor, cw, +1), m.conn)
+    assert np.all(o == -1)
+    flipped = gf.Mesh.Tri3.setOrientation(m.coor, m.conn, +1)
+    assert np.all(gf.Mesh.Tri3.getOrientation(m.coor, flipped) == 1)
+    assert np.array_equal(flipped[:, 0], m.conn[:, 0]) and np.array_equal(flipped[:, 1], m.conn[:, 2])
+    assert np.array_equal(gf.Mesh.Tri3.setOrientation(m.coor, flipped, -1), m.conn)
+    assert np.array_equal(gf.Mesh.Tri3.setOrientation(m.coor, m.conn, o, -1), m.conn)
