@@ -531,6 +531,90 @@ def bench_newton(args):
     print(json.dumps(res))
 
 
+def bench_statics(args):
+    """One load increment of the statics example on the device: a Quad4 plane-strain n^2 plate, J2
+    linear hardening, bottom edge fixed, top edge sheared; Newton iterations of the fused element
+    pass, MatrixPartitioned assembly (CSR + the four blocks) and the partitioned Jacobi-PCG solve."""
+    import torch
+
+    import goosefem_amd as gf
+
+    n = args.statics_n
+    mesh = gf.Mesh.Quad4.Regular(n, n)
+    dofs = mesh.dofs
+    iip = np.concatenate((dofs[mesh.nodesBottomEdge].ravel(), dofs[mesh.nodesTopEdge].ravel()))
+    vector = gf.VectorPartitioned(mesh.conn, dofs, iip)
+    quad = gf.Element.Quad4.QuadraturePlanar(vector.AsElement(mesh.coor))
+    ne = mesh.nelem
+    mat = gf.LinearHardening(torch.full((ne, 4), 10.0, dtype=torch.float64, device="cuda"), 1.0, 0.005, 0.4)
+    t0 = time.perf_counter()
+    A = gf.MatrixPartitioned(vector)
+    t_pattern = time.perf_counter() - t0
+    solver = gf.MatrixPartitionedSolver(tol=1e-10)
+    disp = torch.zeros((mesh.nnode, 2), dtype=torch.float64, device="cuda")
+    fe = torch.empty((ne, 4, 2), dtype=torch.float64, device="cuda")
+    Ke = torch.empty((ne, 8, 8), dtype=torch.float64, device="cuda")
+    t_el = t_asm = t_solve = 0.0
+    its, newton, resid = [], 0, []
+    mat.increment()
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for it in range(20):
+        ta = time.perf_counter()
+        mat.newton_q4(quad, vector.AsElement(disp), fe, Ke)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        A.assemble(Ke)
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        fint = vector.AssembleNode(fe)
+        fres = -fint
+        if it > 0:
+            fres[torch.from_numpy(vector.dofs_is_p()).cuda()] = 0.0
+            rel = float(fres.abs().sum()) / float(fint.abs().sum())
+            resid.append(rel)
+            if rel < 1e-8:
+                break
+        du = torch.zeros_like(disp)
+        if it == 0:
+            du[torch.from_numpy(mesh.nodesTopEdge).cuda(), 0] = 0.005 * n
+        solver.solve(A, fres, du)
+        torch.cuda.synchronize()
+        td = time.perf_counter()
+        disp += du
+        t_el += tb - ta
+        t_asm += tc - tb
+        t_solve += td - tc
+        its.append(solver.iterations)
+        newton += 1
+    t_all = time.perf_counter() - t_all
+    nnz_uu = A.nnz_blocks[0]
+    nnu = vector.nnu
+    cg_its = sum(its)
+    t_it = t_solve / max(cg_its, 1)
+    # per CG iteration: A_uu p (12 B/nnz + p, q), the update (p, q, diag, x, r read; x, r, z
+    # written) and the direction (z, p read; p written): 12 nnz + 15 * 8 nnu bytes
+    b_it = 12 * nnz_uu + 120 * nnu
+    ach = b_it / t_it / 1e9
+    res = {
+        "metric": "Statics load increment (J2 plate, Newton + partitioned PCG) wall time",
+        "value": t_all,
+        "unit": "s",
+        "higher_is_better": False,
+        "config": {"workload": f"Quad4 plane strain Regular {n}^2, J2 linear hardening, bottom fixed, top sheared",
+                   "nelem": ne, "nnu": nnu, "nnz_uu": nnz_uu, "newton_iterations": newton, "cg_iterations": its,
+                   "newton_residuals": resid,
+                   "plastic_fraction": float((mat.epsp > 0).double().mean())},
+        "dtype": "f64",
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "kernel": "partitioned Jacobi-PCG iteration (k_cg_apply + k_cg_update + k_cg_direction)",
+                     "traffic": None, "algorithmic_bytes_per_launch": b_it, "avg_launch_us": t_it * 1e6},
+        "breakdown_s": {"fused_element_pass": t_el, "matrix_assemble": t_asm, "solve": t_solve,
+                        "pattern_build": t_pattern},
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -541,7 +625,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned",
-                                           "plastic-refresh", "newton-q4"],
+                                           "plastic-refresh", "newton-q4", "statics-q4"],
                     default="quad4-explicit")
     ap.add_argument("--pcg-n", type=int, default=512)
     ap.add_argument("--pcg-hex", action="store_true", help="periodic Hex8 n^3 instead of Quad4 n^2")
@@ -549,6 +633,7 @@ def main():
                     help="stiff periodic spheres (G x10, radius 4, 20%% of elements) and pure shear 5e-4")
     ap.add_argument("--hex-n", type=int, default=128)
     ap.add_argument("--newton-n", type=int, default=2048)
+    ap.add_argument("--statics-n", type=int, default=256)
     ap.add_argument("--ws-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "goosefem_amd",
                                                          "data", "warmstart_quad4_512.npz"))
     ap.add_argument("--ws-tests", type=int, default=4, help="held-out modulus fields timed")
@@ -587,6 +672,9 @@ def main():
         return
     if args.workload == "newton-q4":
         bench_newton(args)
+        return
+    if args.workload == "statics-q4":
+        bench_statics(args)
         return
 
     nx = ny = args.nx

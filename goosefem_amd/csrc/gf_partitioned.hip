@@ -218,6 +218,25 @@ __device__ inline double sum_partials(const double* partial)
     return s;
 }
 
+// The sum of count partials for a whole block, formed by its first wave in a fixed order (lane l
+// adds partials l, l + 64, ... in turn, then a butterfly whose lane-0 result is taken) and broadcast
+// through LDS; every thread of the block must call it.  Deterministic, and count / 64 + 6 dependent
+// steps instead of count.
+__device__ inline double block_sum_partials(const double* partial, int count)
+{
+    __shared__ double total;
+    if (threadIdx.x < 64) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < count; b += 64) v += partial[b];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (threadIdx.x == 0) total = v;
+    }
+    __syncthreads();
+    const double r = total;
+    __syncthreads();
+    return r;
+}
+
 // start (x given): r = b - A x, z = r / diag, p = z; partials of r.z, r.r, b.b
 __global__ void __launch_bounds__(PB) k_cg_init(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
                                                 const double* __restrict__ vl, const double* __restrict__ b,
@@ -257,25 +276,35 @@ __global__ void k_cg_init_scal(const double* __restrict__ partial, double* __res
     state[2] = maxiter;
 }
 
-// q = A p and the partials of p.q
+// q = A p and the per-block partials of p.q.  LPR lanes share a row (entries j, j + LPR, ...),
+// combined in a fixed order; the grid covers every row, so the gathers of p have enough waves in
+// flight to hide their latency.
+constexpr int LPR = 4;
+
 __global__ void __launch_bounds__(PB) k_cg_apply(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
                                                  const double* __restrict__ vl, const double* __restrict__ p,
                                                  double* __restrict__ q, double* __restrict__ partial,
                                                  const int32_t* __restrict__ state)
 {
     if (state[0]) return;
-    double acc = 0.0;
-    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB) {
-        double s = 0.0;
-        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += vl[k] * p[cl[k]];
-        q[i] = s;
-        acc += p[i] * s;
+    const int64_t g = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x;
+    const int64_t r = g / LPR;
+    const int j = static_cast<int>(g % LPR);
+    double s = 0.0;
+    if (r < n)
+        for (int32_t k = rp[r] + j; k < rp[r + 1]; k += LPR) s += vl[k] * p[cl[k]];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    double c = 0.0;
+    if (r < n && j == 0) {
+        q[r] = s;
+        c = p[r] * s;
     }
-    block_sum_store(acc, partial + blockIdx.x);
+    block_sum_store(c, partial + blockIdx.x);
 }
 
 // alpha = rz / p.q; x += alpha p; r -= alpha q; z = r / diag; partials of r.z and r.r
-__global__ void __launch_bounds__(PB) k_cg_update(int64_t n, const double* __restrict__ pq_partial,
+__global__ void __launch_bounds__(PB) k_cg_update(int64_t n, const double* __restrict__ pq_partial, int npq,
                                                   const double* __restrict__ scal, const double* __restrict__ invd,
                                                   const double* __restrict__ p, const double* __restrict__ q,
                                                   double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
@@ -283,7 +312,7 @@ __global__ void __launch_bounds__(PB) k_cg_update(int64_t n, const double* __res
                                                   int64_t it)
 {
     if (state[0]) return;
-    const double pq = sum_partials(pq_partial);
+    const double pq = block_sum_partials(pq_partial, npq);
     const double alpha = pq != 0.0 ? scal[it & 1] / pq : 0.0;
     double arz = 0.0, arr = 0.0;
     for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB) {
@@ -307,7 +336,7 @@ __global__ void __launch_bounds__(PB) k_cg_direction(int64_t n, const double* __
 {
     if (state[0]) return;
     const double rz_old = scal[it & 1];
-    const double rz_new = sum_partials(partial);
+    const double rz_new = block_sum_partials(partial, NPB);
     const double beta = rz_old != 0.0 ? rz_new / rz_old : 0.0;
     for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB)
         p[i] = z[i] + beta * p[i];
@@ -679,13 +708,15 @@ void matpart_cg(gf_matpart A, const double* rhs, double* xu, bool warm, double t
     check_launch();
     int32_t st[2] = {0, 0};
     constexpr int64_t check_every = 16;
+    const unsigned napply = nblocks(LPR * n, PB);
+    double* pq_part = partial + 3 * NPB + 8; // one p.q partial per apply block
     GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
     for (int64_t it = 0; !st[0] && it < maxiter;) {
         for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
-            k_cg_apply<<<NPB, PB, 0, s>>>(n, rp, cl, vl, W.cp, W.cq, partial + 2 * NPB, state);
-            k_cg_update<<<NPB, PB, 0, s>>>(n, partial + 2 * NPB, scal, W.invd, W.cp, W.cq, W.cx, W.cr, W.cz, partial,
-                                           state, it);
+            k_cg_apply<<<napply, PB, 0, s>>>(n, rp, cl, vl, W.cp, W.cq, pq_part, state);
+            k_cg_update<<<NPB, PB, 0, s>>>(n, pq_part, static_cast<int>(napply), scal, W.invd, W.cp, W.cq, W.cx, W.cr,
+                                           W.cz, partial, state, it);
             k_cg_direction<<<NPB, PB, 0, s>>>(n, partial, scal, W.cz, W.cp, state, it);
         }
         check_launch();
@@ -785,7 +816,7 @@ int gf_matpart_create(gf_matpart* out, gf_vecpart p)
                 A->val[b].alloc(A->nnz[b]);
             }
             A->work.alloc(2 * (nu + np) + 7 * nu);
-            A->red.alloc(3 * NPB + 8);
+            A->red.alloc(3 * NPB + 8 + nblocks(LPR * nu, PB));
             A->state.alloc(4);
         }
         catch (...) {
