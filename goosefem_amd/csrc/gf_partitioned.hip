@@ -218,21 +218,21 @@ __device__ inline double sum_partials(const double* partial)
     return s;
 }
 
-// The sum of count partials for a whole block, formed by its first wave in a fixed order (lane l
-// adds partials l, l + 64, ... in turn, then a butterfly whose lane-0 result is taken) and broadcast
-// through LDS; every thread of the block must call it.  Deterministic, and count / 64 + 6 dependent
-// steps instead of count.
+// The sum of count partials for a whole block: thread t adds partials t, t + PB, ... in turn, then
+// a fixed-order tree in LDS; every thread of the block must call it.  Deterministic, with
+// count / PB loads per thread instead of count.
 __device__ inline double block_sum_partials(const double* partial, int count)
 {
-    __shared__ double total;
-    if (threadIdx.x < 64) {
-        double v = 0.0;
-        for (int b = threadIdx.x; b < count; b += 64) v += partial[b];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (threadIdx.x == 0) total = v;
-    }
+    __shared__ double sh[PB];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < count; b += PB) v += partial[b];
+    sh[threadIdx.x] = v;
     __syncthreads();
-    const double r = total;
+    for (int h = PB / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    const double r = sh[0];
     __syncthreads();
     return r;
 }
@@ -340,8 +340,9 @@ __global__ void __launch_bounds__(PB) k_cg_direction(int64_t n, const double* __
     const double beta = rz_old != 0.0 ? rz_new / rz_old : 0.0;
     for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB)
         p[i] = z[i] + beta * p[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const double rr = sum_partials(partial + NPB);
+    if (blockIdx.x != 0) return;
+    const double rr = block_sum_partials(partial + NPB, NPB); // block 0 only (uniform in the block)
+    if (threadIdx.x == 0) {
         scal[(it + 1) & 1] = rz_new;
         scal[2] = rr;
         state[1] = static_cast<int32_t>(it + 1);
