@@ -192,6 +192,8 @@ __global__ void k_node_from_part_sel(int64_t n, int64_t nnu, int which, const in
 }
 
 // ---- Jacobi-PCG on a CSR block ----------------------------------------------------------------
+// Two launches per iteration: the product (which also forms the search direction from the
+// previous iteration's z and p) and the x / r / z update.
 // Deterministic two-stage reductions: NPB blocks each store one partial sum, every consumer sums
 // the NPB partials in index order.  scal: [0], [1] rz by iteration parity, [2] rr, [3] bb,
 // [4] tol^2 bb.  state: [0] done, [1] iterations, [2] iteration cap.  Every kernel returns at once
@@ -303,6 +305,51 @@ __global__ void __launch_bounds__(PB) k_cg_apply(int64_t n, const int32_t* __res
     block_sum_store(c, partial + blockIdx.x);
 }
 
+// Iterations after the first: the direction update fused into the product.  beta = rz_new / rz_old
+// from the previous update's partials; every gathered p(c) is formed as z(c) + beta p_prev(c) (the
+// value the separate direction pass would have stored), the block's own rows are written to p_new,
+// and q = A p_new with the partials of p_new.q.  Block 0 also stores rz_new, r.r, the iteration
+// count, and raises done at convergence (r.r <= tol^2 b.b) or at the iteration cap.
+__global__ void __launch_bounds__(PB)
+    k_cg_apply_dir(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
+                   const double* __restrict__ vl, const double* __restrict__ z, const double* __restrict__ pprev,
+                   double* __restrict__ pnew, double* __restrict__ q, const double* __restrict__ rz_rr,
+                   double* __restrict__ scal, double* __restrict__ partial, int32_t* __restrict__ state, int64_t it)
+{
+    if (state[0]) return;
+    const double rz_old = scal[(it - 1) & 1];
+    const double rz_new = block_sum_partials(rz_rr, NPB);
+    const double beta = rz_old != 0.0 ? rz_new / rz_old : 0.0;
+    if (blockIdx.x == 0) {
+        const double rr = block_sum_partials(rz_rr + NPB, NPB);
+        if (threadIdx.x == 0) {
+            scal[it & 1] = rz_new;
+            scal[2] = rr;
+            state[1] = static_cast<int32_t>(it);
+            if (rr <= scal[4] || it >= state[2]) state[0] = 1;
+        }
+    }
+    const int64_t g = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x;
+    const int64_t r = g / LPR;
+    const int j = static_cast<int>(g % LPR);
+    double sum = 0.0;
+    if (r < n)
+        for (int32_t k = rp[r] + j; k < rp[r + 1]; k += LPR) {
+            const int32_t c = cl[k];
+            sum += vl[k] * (z[c] + beta * pprev[c]);
+        }
+    sum += __shfl_xor(sum, 1);
+    sum += __shfl_xor(sum, 2);
+    double contrib = 0.0;
+    if (r < n && j == 0) {
+        const double pr = z[r] + beta * pprev[r];
+        pnew[r] = pr;
+        q[r] = sum;
+        contrib = pr * sum;
+    }
+    block_sum_store(contrib, partial + blockIdx.x);
+}
+
 // alpha = rz / p.q; x += alpha p; r -= alpha q; z = r / diag; partials of r.z and r.r
 __global__ void __launch_bounds__(PB) k_cg_update(int64_t n, const double* __restrict__ pq_partial, int npq,
                                                   const double* __restrict__ scal, const double* __restrict__ invd,
@@ -326,28 +373,6 @@ __global__ void __launch_bounds__(PB) k_cg_update(int64_t n, const double* __res
     }
     block_sum_store(arz, partial + blockIdx.x);
     block_sum_store(arr, partial + NPB + blockIdx.x);
-}
-
-// beta = rz_new / rz_old; p = z + beta p.  Block 0 stores rz_new (other parity slot), r.r, the
-// iteration count, and raises done at convergence (r.r <= tol^2 b.b) or at the iteration cap.
-__global__ void __launch_bounds__(PB) k_cg_direction(int64_t n, const double* __restrict__ partial,
-                                                     double* __restrict__ scal, const double* __restrict__ z,
-                                                     double* __restrict__ p, int32_t* __restrict__ state, int64_t it)
-{
-    if (state[0]) return;
-    const double rz_old = scal[it & 1];
-    const double rz_new = block_sum_partials(partial, NPB);
-    const double beta = rz_old != 0.0 ? rz_new / rz_old : 0.0;
-    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB)
-        p[i] = z[i] + beta * p[i];
-    if (blockIdx.x != 0) return;
-    const double rr = block_sum_partials(partial + NPB, NPB); // block 0 only (uniform in the block)
-    if (threadIdx.x == 0) {
-        scal[(it + 1) & 1] = rz_new;
-        scal[2] = rr;
-        state[1] = static_cast<int32_t>(it + 1);
-        if (rr <= scal[4] || it + 1 >= state[2]) state[0] = 1;
-    }
 }
 
 // |b - A x|^2 partials and b.b partials
@@ -642,9 +667,9 @@ int gf_diagpart_apply_sel(gf_diagpart d, int op, int kind, const double* in, dou
 namespace {
 
 // work layout (doubles): x_u [nnu], x_p [nnp], b_u [nnu], b_p [nnp], then the PCG vectors
-// x, r, z, p, q, invdiag, rhs [nnu each]
+// x, r, z, p, q, invdiag, rhs, second p [nnu each]
 struct MatPartWork {
-    double *xu, *xp, *bu, *bp, *cx, *cr, *cz, *cp, *cq, *invd, *rhs;
+    double *xu, *xp, *bu, *bp, *cx, *cr, *cz, *cp, *cq, *invd, *rhs, *cp2;
 };
 
 MatPartWork work_of(gf_matpart A)
@@ -663,6 +688,7 @@ MatPartWork work_of(gf_matpart A)
     W.cq = W.cp + nu;
     W.invd = W.cq + nu;
     W.rhs = W.invd + nu;
+    W.cp2 = W.rhs + nu;
     return W;
 }
 
@@ -713,13 +739,21 @@ void matpart_cg(gf_matpart A, const double* rhs, double* xu, bool warm, double t
     double* pq_part = partial + 3 * NPB + 8; // one p.q partial per apply block
     GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     GF_HIP(hipStreamSynchronize(s));
-    for (int64_t it = 0; !st[0] && it < maxiter;) {
+    double* pb[2] = {W.cp, W.cp2};
+    int64_t it = 0;
+    while (!st[0] && it < maxiter) {
         for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
-            k_cg_apply<<<napply, PB, 0, s>>>(n, rp, cl, vl, W.cp, W.cq, pq_part, state);
-            k_cg_update<<<NPB, PB, 0, s>>>(n, pq_part, static_cast<int>(napply), scal, W.invd, W.cp, W.cq, W.cx, W.cr,
-                                           W.cz, partial, state, it);
-            k_cg_direction<<<NPB, PB, 0, s>>>(n, partial, scal, W.cz, W.cp, state, it);
+            if (it == 0)
+                k_cg_apply<<<napply, PB, 0, s>>>(n, rp, cl, vl, pb[0], W.cq, pq_part, state);
+            else
+                k_cg_apply_dir<<<napply, PB, 0, s>>>(n, rp, cl, vl, W.cz, pb[(it - 1) & 1], pb[it & 1], W.cq, partial,
+                                                     scal, pq_part, state, it);
+            k_cg_update<<<NPB, PB, 0, s>>>(n, pq_part, static_cast<int>(napply), scal, W.invd, pb[it & 1], W.cq, W.cx,
+                                           W.cr, W.cz, partial, state, it);
         }
+        if (it == maxiter) // the convergence test and iteration count of the last update
+            k_cg_apply_dir<<<napply, PB, 0, s>>>(n, rp, cl, vl, W.cz, pb[(it - 1) & 1], pb[it & 1], W.cq, partial, scal,
+                                                 pq_part, state, it);
         check_launch();
         GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         GF_HIP(hipStreamSynchronize(s));
@@ -816,7 +850,7 @@ int gf_matpart_create(gf_matpart* out, gf_vecpart p)
                 A->src[b].upload(bsrc[b]);
                 A->val[b].alloc(A->nnz[b]);
             }
-            A->work.alloc(2 * (nu + np) + 7 * nu);
+            A->work.alloc(2 * (nu + np) + 8 * nu);
             A->red.alloc(3 * NPB + 8 + nblocks(LPR * nu, PB));
             A->state.alloc(4);
         }
