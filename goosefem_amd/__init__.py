@@ -1513,6 +1513,155 @@ class MatrixPartitioned(_Handle):
         return self.Dot_p(x_u, x_p)
 
 
+def _csr_of(M, ncol=None):
+    """Host CSR (rowptr int64, col int64, val float64) of a dense array or a (rowptr, col, val)."""
+    if isinstance(M, tuple):
+        rp, cl, v = M
+        return _i64a(rp), _i64a(cl), np.ascontiguousarray(v, dtype=np.float64)
+    M = np.asarray(M, dtype=np.float64)
+    rows, cols = np.nonzero(M)
+    rp = np.zeros(M.shape[0] + 1, dtype=np.int64)
+    np.add.at(rp, rows + 1, 1)
+    return np.cumsum(rp), cols.astype(np.int64), np.ascontiguousarray(M[rows, cols])
+
+
+def _csr_transpose(rp, cl, v, ncol):
+    order = np.lexsort((np.repeat(np.arange(rp.size - 1), np.diff(rp)), cl))
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))[order]
+    trp = np.zeros(ncol + 1, dtype=np.int64)
+    np.add.at(trp, cl[order] + 1, 1)
+    return np.cumsum(trp), rows.astype(np.int64), v[order]
+
+
+class VectorPartitionedTyings(Vector):
+    """GooseFEM::VectorPartitionedTyings (VectorPartitionedTyings.h:26-251): a Vector over the tied
+    numbering [u | p | d] of Tyings::Periodic::dofs(), with the dependent DOFs d tied to the
+    independent ones i = [u | p] by C_di = [C_du, C_dp].  The tying matrices may be dense arrays (as
+    Tyings.Periodic.Cdu ...) or host CSR triples."""
+
+    def __init__(self, conn, dofs, Cdu, Cdp, Cdi):
+        super().__init__(conn, dofs)
+        Cdu, Cdp, Cdi = (np.asarray(C, dtype=np.float64) if not isinstance(C, tuple) else C for C in (Cdu, Cdp, Cdi))
+        self._du = _csr_of(Cdu)
+        self._dp = _csr_of(Cdp)
+        di = _csr_of(Cdi)
+        self.nnd = self._du[0].size - 1
+        self.nnu = Cdu.shape[1] if not isinstance(Cdu, tuple) else int(self.ndof - self.nnd - Cdp.shape[1])
+        self.nni = self.ndof - self.nnd
+        self.nnp = self.nni - self.nnu
+        torch = _torch()
+        trp, tcl, tv = _csr_transpose(*di, self.nni)  # C_id, rows i
+        self._id = (torch.as_tensor(trp.astype(np.int32), device=device()),
+                    torch.as_tensor(tcl.astype(np.int32), device=device()),
+                    torch.as_tensor(tv, device=device()))
+
+    @property
+    def iiu(self):
+        return np.arange(self.nnu)
+
+    @property
+    def iip(self):
+        return np.arange(self.nnu, self.nni)
+
+    @property
+    def iii(self):
+        return np.arange(self.nni)
+
+    @property
+    def iid(self):
+        return np.arange(self.nni, self.ndof)
+
+    def AsDofs_i(self, nodevec, apply_tyings=True):
+        """AsDofs_i (VectorPartitionedTyings.h:209-251): the independent DOFs of a nodevec, plus
+        C_id times its dependent DOFs (the tied contributions) unless apply_tyings is False."""
+        d = self.AsDofs(nodevec)
+        out = d[: self.nni].clone()
+        if apply_tyings and self.nnd:
+            rp, cl, v = self._id
+            xd = d[self.nni:].contiguous()
+            _call("gf_csr_addmv", _i64(self.nni), _vp(rp.data_ptr()), _vp(cl.data_ptr()), _vp(v.data_ptr()), _dp(xd),
+                  _dp(out), _stream())
+        return out
+
+    def asDofs_i(self, nodevec, dofval_i, apply_tyings=True):
+        return _inplace(dofval_i, lambda r: r.copy_(self.AsDofs_i(nodevec, apply_tyings)))
+
+    def copy_p(self, dofval_src, dofval_dest):
+        """copy_p (VectorPartitionedTyings.h:188-199): dofval_dest[iip] = dofval_src[iip]."""
+        src = dofval_src
+        if isinstance(dofval_dest, np.ndarray):
+            dofval_dest[self.nnu:self.nni] = np.asarray(_np_of(src))[self.nnu:self.nni]
+        else:
+            dofval_dest[self.nnu:self.nni] = to_device(src)[self.nnu:self.nni]
+        return dofval_dest
+
+
+class MatrixPartitionedTyings(_Handle):
+    """GooseFEM::MatrixPartitionedTyings (MatrixPartitionedTyings.h:34-523) for any mesh and element
+    matrices: K assembled as one CSR over the tied numbering, the condensed operator applied
+    matrix-free inside MatrixPartitionedTyingsSolver's CG."""
+
+    _destroy = "gf_mattyings_destroy"
+
+    def __init__(self, conn, dofs, Cdu, Cdp):
+        self._vec = Vector(conn, dofs)
+        Cdu = np.asarray(Cdu, dtype=np.float64) if not isinstance(Cdu, tuple) else Cdu
+        Cdp = np.asarray(Cdp, dtype=np.float64) if not isinstance(Cdp, tuple) else Cdp
+        self._du, self._dp = _csr_of(Cdu), _csr_of(Cdp)
+        self.nnd = self._du[0].size - 1
+        ndof = self._vec.ndof
+        self.nnp = Cdp.shape[1] if not isinstance(Cdp, tuple) else int(self._dp[1].max(initial=-1) + 1)
+        self.nnu = ndof - self.nnd - self.nnp
+        self.nni = self.nnu + self.nnp
+        self.ndof = ndof
+        h = _vp()
+        _call("gf_mattyings_create", ctypes.byref(h), self._vec._h, _i64(self.nnu), _i64(self.nnp), _i64(self.nnd),
+              _hp(self._du[0]), _hp(self._du[1]), _hp(self._du[2]), _hp(self._dp[0]), _hp(self._dp[1]),
+              _hp(self._dp[2]))
+        self._h = h
+
+    def assemble(self, elemmat):
+        """assemble(elemmat) (MatrixPartitionedTyings.h:210-305)."""
+        em = to_device(elemmat)
+        _call("gf_mattyings_assemble", self._h, _dp(em), _stream())
+
+    def Todense(self):
+        K = _vp()
+        _call("gf_mattyings_matrix", self._h, ctypes.byref(K))
+        out = _empty((self.ndof, self.ndof))
+        _call("gf_matrix_todense", K, _dp(out), _stream())
+        return out
+
+
+class MatrixPartitionedTyingsSolver:
+    """GooseFEM::MatrixPartitionedTyingsSolver (MatrixPartitionedTyings.h:527-675): x_u from the
+    condensed system by an FP64 Jacobi-PCG (tol relative), x_d from the tyings; K' must be SPD."""
+
+    def __init__(self, tol=1e-12, maxiter=None):
+        self.tol = tol
+        self.maxiter = maxiter
+        self.iterations = 0
+        self.relres = 0.0
+
+    def solve(self, A, b, x):
+        """solve(A, b, x) (:540-564) on nodevecs: x's u and d entries solved in place."""
+        bd = to_device(b)
+        it, res = _i64(), _f64()
+        maxiter = self.maxiter or max(100, 10 * A.nnu)
+
+        def run(r):
+            _call("gf_mattyings_solve", A._h, _dp(bd), _dp(r), _f64(self.tol), _i64(maxiter), ctypes.byref(it),
+                  ctypes.byref(res), _stream())
+
+        out = _inplace(x, run)
+        self.iterations, self.relres = it.value, res.value
+        return out
+
+    def Solve(self, A, b, x):
+        xc = to_device(x).clone()
+        return self.solve(A, b, xc)
+
+
 class MatrixPartitionedSolver:
     """GooseFEM::MatrixPartitionedSolver (MatrixPartitioned.h:526-640) for a symmetric positive
     definite A_uu: x_u = A_uu^-1 (b_u - A_up x_p) by an FP64 Jacobi-preconditioned CG on the GPU

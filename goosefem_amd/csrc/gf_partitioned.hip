@@ -41,6 +41,19 @@ struct gf_matpart_s {
     DBuf<int32_t> state;
 };
 
+// MatrixPartitionedTyings: the unpartitioned CSR K over the tied DOF numbering [u | p | d] and the
+// tying matrices C_du, C_dp (rows d) with C_ud = C_du^T (rows u), all CSR on the device.
+struct gf_mattyings_s {
+    gf_vector v = nullptr;
+    gf_matrix K = nullptr;
+    int64_t nnu = 0, nnp = 0, nnd = 0, ndof = 0;
+    DBuf<int32_t> du_rp, du_cl, dp_rp, dp_cl, ud_rp, ud_cl;
+    DBuf<double> du_v, dp_v, ud_v;
+    DBuf<double> work; // x, r, z, p, q, invd, rhs [nnu]; xfull, yfull, kdiag [ndof]
+    DBuf<double> red;
+    DBuf<int32_t> state;
+};
+
 namespace {
 
 constexpr int BS = 256;
@@ -409,6 +422,148 @@ __global__ void k_csr_invdiag(int64_t n, const int32_t* __restrict__ rp, const i
     for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
         if (cl[k] == r) d = vl[k];
     invd[r] = 1.0 / d;
+}
+
+// ---- tied systems: expand, fold and a CG on the condensed operator --------------------------------
+
+// xfull = (x_u, x_p, C_du x_u + C_dp x_p); x_u or x_p may be absent (zero)
+__global__ void k_ty_expand(int64_t nnu, int64_t nnp, int64_t nnd, const double* __restrict__ xu,
+                            const double* __restrict__ xp, const int32_t* __restrict__ du_rp,
+                            const int32_t* __restrict__ du_cl, const double* __restrict__ du_v,
+                            const int32_t* __restrict__ dp_rp, const int32_t* __restrict__ dp_cl,
+                            const double* __restrict__ dp_v, double* __restrict__ xfull)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const int64_t ni = nnu + nnp;
+    if (t < nnu) xfull[t] = xu ? xu[t] : 0.0;
+    else if (t < ni) xfull[t] = xp ? xp[t - nnu] : 0.0;
+    else if (t < ni + nnd) {
+        const int64_t d = t - ni;
+        double s = 0.0;
+        if (xu)
+            for (int32_t k = du_rp[d]; k < du_rp[d + 1]; ++k) s += du_v[k] * xu[du_cl[k]];
+        if (xp)
+            for (int32_t k = dp_rp[d]; k < dp_rp[d + 1]; ++k) s += dp_v[k] * xp[dp_cl[k]];
+        xfull[t] = s;
+    }
+}
+
+// q_u = y_u + C_ud y_d (the condensed product) and the per-block partials of p.q (p may be null)
+__global__ void __launch_bounds__(PB) k_ty_fold(int64_t nnu, int64_t nni, const double* __restrict__ y,
+                                                const int32_t* __restrict__ ud_rp, const int32_t* __restrict__ ud_cl,
+                                                const double* __restrict__ ud_v, const double* __restrict__ p,
+                                                double* __restrict__ q, double* __restrict__ partial)
+{
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < nnu; i += static_cast<int64_t>(NPB) * PB) {
+        double s = y[i];
+        for (int32_t k = ud_rp[i]; k < ud_rp[i + 1]; ++k) s += ud_v[k] * y[nni + ud_cl[k]];
+        q[i] = s;
+        if (p) acc += p[i] * s;
+    }
+    if (partial) block_sum_store(acc, partial + blockIdx.x);
+}
+
+// the diagonal of each CSR row (0 where the row has none)
+__global__ void k_csr_diag(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
+                           const double* __restrict__ vl, double* __restrict__ dg)
+{
+    const int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (r >= n) return;
+    double d = 0.0;
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k)
+        if (cl[k] == r) d = vl[k];
+    dg[r] = d;
+}
+
+// Jacobi preconditioner of the condensed operator: 1 / (K_ii + sum_d C_ud(i, d)^2 K_dd)
+__global__ void k_ty_invdiag(int64_t nnu, int64_t nni, const double* __restrict__ kd, const int32_t* __restrict__ ud_rp,
+                             const int32_t* __restrict__ ud_cl, const double* __restrict__ ud_v,
+                             double* __restrict__ invd)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= nnu) return;
+    double d = kd[i];
+    for (int32_t k = ud_rp[i]; k < ud_rp[i + 1]; ++k) d += ud_v[k] * ud_v[k] * kd[nni + ud_cl[k]];
+    invd[i] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// rhs = b_u + C_ud b_d - t  (t: the condensed product of the prescribed part)
+__global__ void k_ty_rhs(int64_t nnu, int64_t nni, const double* __restrict__ bfull, const int32_t* __restrict__ ud_rp,
+                         const int32_t* __restrict__ ud_cl, const double* __restrict__ ud_v, const double* __restrict__ t,
+                         double* __restrict__ rhs)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= nnu) return;
+    double s = bfull[i];
+    for (int32_t k = ud_rp[i]; k < ud_rp[i + 1]; ++k) s += ud_v[k] * bfull[nni + ud_cl[k]];
+    rhs[i] = s - t[i];
+}
+
+// y += A x for a CSR A (rows in order, entries in order)
+__global__ void k_csr_addmv(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
+                            const double* __restrict__ vl, const double* __restrict__ x, double* __restrict__ y)
+{
+    const int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (r >= n) return;
+    double s = y[r];
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) s += vl[k] * x[cl[k]];
+    y[r] = s;
+}
+
+// |b - q|^2 and |b|^2 partials (q = A x formed by the caller)
+__global__ void __launch_bounds__(PB) k_cg_residual_vec(int64_t n, const double* __restrict__ b,
+                                                        const double* __restrict__ q, double* __restrict__ partial)
+{
+    double arr = 0.0, abb = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB) {
+        const double ri = b[i] - q[i];
+        arr += ri * ri;
+        abb += b[i] * b[i];
+    }
+    block_sum_store(arr, partial + blockIdx.x);
+    block_sum_store(abb, partial + NPB + blockIdx.x);
+}
+
+// beta = rz_new / rz_old; p = z + beta p; block 0 stores rz_new, r.r, the count and the done flag
+__global__ void __launch_bounds__(PB) k_ty_direction(int64_t n, const double* __restrict__ partial,
+                                                     double* __restrict__ scal, const double* __restrict__ z,
+                                                     double* __restrict__ p, int32_t* __restrict__ state, int64_t it)
+{
+    if (state[0]) return;
+    const double rz_old = scal[it & 1];
+    const double rz_new = block_sum_partials(partial, NPB);
+    const double beta = rz_old != 0.0 ? rz_new / rz_old : 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB)
+        p[i] = z[i] + beta * p[i];
+    if (blockIdx.x != 0) return;
+    const double rr = block_sum_partials(partial + NPB, NPB);
+    if (threadIdx.x == 0) {
+        scal[(it + 1) & 1] = rz_new;
+        scal[2] = rr;
+        state[1] = static_cast<int32_t>(it + 1);
+        if (rr <= scal[4] || it + 1 >= state[2]) state[0] = 1;
+    }
+}
+
+// start from x = 0: r = z... (r = rhs, z = r / diag, p = z); partials of r.z, r.r, b.b
+__global__ void __launch_bounds__(PB) k_ty_init(int64_t n, const double* __restrict__ b, const double* __restrict__ invd,
+                                                double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                double* __restrict__ p, double* __restrict__ partial)
+{
+    double arz = 0.0, arr = 0.0;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB) {
+        const double ri = b[i], zi = invd[i] * ri;
+        x[i] = 0.0;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        arz += ri * zi;
+        arr += ri * ri;
+    }
+    block_sum_store(arz, partial + blockIdx.x);
+    block_sum_store(arr, partial + NPB + blockIdx.x);
+    block_sum_store(arr, partial + 2 * NPB + blockIdx.x);
 }
 
 } // namespace
@@ -997,6 +1152,208 @@ int gf_matpart_solve(gf_matpart A, int kind, const double* b, double* x, int war
             const int64_t n = p->v->nnode * p->v->ndim;
             if (n) k_node_from_part_sel<<<nblocks(n, BS), BS, 0, s>>>(n, p->nnu, 0, p->part.p, W.xu, W.xp, x);
         }
+        check_launch();
+    });
+}
+
+// ---- MatrixPartitionedTyings -------------------------------------------------------------------
+
+static void to_i32(const int64_t* a, size_t n, std::vector<int32_t>& out, const char* what)
+{
+    out.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        GF_REQUIRE(a[i] >= 0 && a[i] < INT32_MAX, std::string(what) + " entry outside the int32 range");
+        out[i] = static_cast<int32_t>(a[i]);
+    }
+}
+
+int gf_mattyings_create(gf_mattyings* out, gf_vector v, int64_t nnu, int64_t nnp, int64_t nnd, const int64_t* du_rp,
+                        const int64_t* du_cl, const double* du_v, const int64_t* dp_rp, const int64_t* dp_cl,
+                        const double* dp_v)
+{
+    return guard([&] {
+        GF_REQUIRE(out && v && du_rp && dp_rp, "null argument");
+        GF_REQUIRE(nnu >= 0 && nnp >= 0 && nnd >= 0 && nnu + nnp + nnd == v->ndof,
+                   "nnu + nnp + nnd must equal the number of DOFs (numbering [u | p | d])");
+        auto A = new gf_mattyings_s();
+        try {
+            A->v = v;
+            A->nnu = nnu;
+            A->nnp = nnp;
+            A->nnd = nnd;
+            A->ndof = v->ndof;
+            int rc = gf_matrix_create(&A->K, v);
+            if (rc != GF_OK) throw Error(rc, gf_last_error());
+            std::vector<int32_t> rp, cl;
+            // C_du, C_dp as given (rows d, columns ascending within a row as produced by the tyings)
+            to_i32(du_rp, nnd + 1, rp, "C_du rowptr");
+            A->du_rp.upload(rp);
+            const int64_t nz_du = du_rp[nnd];
+            to_i32(du_cl, nz_du, cl, "C_du col");
+            for (int32_t c : cl) GF_REQUIRE(c < nnu, "C_du column outside [0, nnu)");
+            A->du_cl.upload(cl);
+            A->du_v.upload(du_v, nz_du);
+            to_i32(dp_rp, nnd + 1, rp, "C_dp rowptr");
+            A->dp_rp.upload(rp);
+            const int64_t nz_dp = dp_rp[nnd];
+            to_i32(dp_cl, nz_dp, cl, "C_dp col");
+            for (int32_t c : cl) GF_REQUIRE(c < nnp, "C_dp column outside [0, nnp)");
+            A->dp_cl.upload(cl);
+            A->dp_v.upload(dp_v, nz_dp);
+            // C_ud = C_du^T: rows u, entries in ascending d
+            std::vector<int32_t> trp(nnu + 1, 0), tcl(nz_du);
+            std::vector<double> tv(nz_du);
+            for (int64_t k = 0; k < nz_du; ++k) ++trp[du_cl[k] + 1];
+            for (int64_t i = 0; i < nnu; ++i) trp[i + 1] += trp[i];
+            std::vector<int32_t> fill(trp.begin(), trp.end() - 1);
+            for (int64_t d = 0; d < nnd; ++d)
+                for (int64_t k = du_rp[d]; k < du_rp[d + 1]; ++k) {
+                    const int32_t at = fill[du_cl[k]]++;
+                    tcl[at] = static_cast<int32_t>(d);
+                    tv[at] = du_v[k];
+                }
+            A->ud_rp.upload(trp);
+            A->ud_cl.upload(tcl);
+            A->ud_v.upload(tv);
+            A->work.alloc(7 * nnu + 3 * A->ndof);
+            A->red.alloc(3 * NPB + 8);
+            A->state.alloc(4);
+        }
+        catch (...) {
+            if (A->K) gf_matrix_destroy(A->K);
+            delete A;
+            throw;
+        }
+        *out = A;
+    });
+}
+
+int gf_mattyings_destroy(gf_mattyings A)
+{
+    if (A && A->K) gf_matrix_destroy(A->K);
+    delete A;
+    return GF_OK;
+}
+
+int gf_mattyings_matrix(gf_mattyings A, gf_matrix* K)
+{
+    return guard([&] {
+        GF_REQUIRE(A && K, "null argument");
+        *K = A->K;
+    });
+}
+
+int gf_mattyings_assemble(gf_mattyings A, const double* elemmat, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && elemmat, "null argument");
+        const int rc = gf_matrix_assemble(A->K, elemmat, stream);
+        if (rc != GF_OK) throw Error(rc, gf_last_error());
+        const int32_t *rp = nullptr, *cl = nullptr;
+        const double* val = nullptr;
+        gf_matrix_csr(A->K, &rp, &cl, &val);
+        hipStream_t s = S(stream);
+        double* kd = A->work.p + 7 * A->nnu + 2 * A->ndof;
+        double* invd = A->work.p + 5 * A->nnu;
+        k_csr_diag<<<nblocks(A->ndof, BS), BS, 0, s>>>(A->ndof, rp, cl, val, kd);
+        if (A->nnu)
+            k_ty_invdiag<<<nblocks(A->nnu, BS), BS, 0, s>>>(A->nnu, A->nnu + A->nnp, kd, A->ud_rp.p, A->ud_cl.p, A->ud_v.p,
+                                                             invd);
+        check_launch();
+    });
+}
+
+// the condensed product of a full-vector expansion: q_u = [K (x_u, x_p, C_du x_u + C_dp x_p)]_u +
+// C_ud [K (...)]_d; p (for the partials) may be null
+static void ty_product(gf_mattyings A, const double* xu, const double* xp, double* q, const double* p, double* partial,
+                       hipStream_t s)
+{
+    double* xfull = A->work.p + 7 * A->nnu;
+    double* yfull = xfull + A->ndof;
+    k_ty_expand<<<nblocks(A->ndof, BS), BS, 0, s>>>(A->nnu, A->nnp, A->nnd, xu, xp, A->du_rp.p, A->du_cl.p, A->du_v.p,
+                                                     A->dp_rp.p, A->dp_cl.p, A->dp_v.p, xfull);
+    check_launch();
+    const int rc = gf_matrix_dot_dofval(A->K, xfull, yfull, s);
+    if (rc != GF_OK) throw Error(rc, gf_last_error());
+    k_ty_fold<<<NPB, PB, 0, s>>>(A->nnu, A->nnu + A->nnp, yfull, A->ud_rp.p, A->ud_cl.p, A->ud_v.p, p, q, partial);
+    check_launch();
+}
+
+int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, int64_t maxiter, int64_t* iterations,
+                       double* relres, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && b && x, "null argument");
+        GF_REQUIRE(tol > 0.0 && maxiter > 0 && maxiter < INT32_MAX, "tol > 0, 0 < maxiter < 2^31");
+        hipStream_t s = S(stream);
+        const int64_t nu = A->nnu, ni = A->nnu + A->nnp, nd = A->ndof;
+        double* w = A->work.p;
+        double *cx = w, *cr = w + nu, *cz = w + 2 * nu, *cp = w + 3 * nu, *cq = w + 4 * nu, *invd = w + 5 * nu,
+               *rhs = w + 6 * nu, *xfull = w + 7 * nu, *yfull = xfull + nd;
+        double* partial = A->red.p;
+        double* scal = partial + 3 * NPB;
+        int32_t* state = A->state.p;
+        // b, x as dofval (nodevec -> dofval, overwrite in loop order); X_p lives in the work's yfull
+        // until the prescribed product has been formed
+        DBuf<double> bfull(nd), xin(nd);
+        int rc = gf_vector_as_dofs_nodevec(A->v, b, bfull.p, stream);
+        if (rc != GF_OK) throw Error(rc, gf_last_error());
+        rc = gf_vector_as_dofs_nodevec(A->v, x, xin.p, stream);
+        if (rc != GF_OK) throw Error(rc, gf_last_error());
+        const double* Xp = xin.p + nu;
+        // rhs = B_u + C_ud B_d - ACup X_p
+        ty_product(A, nullptr, Xp, cq, nullptr, nullptr, s);
+        if (nu) k_ty_rhs<<<nblocks(nu, BS), BS, 0, s>>>(nu, ni, bfull.p, A->ud_rp.p, A->ud_cl.p, A->ud_v.p, cq, rhs);
+        check_launch();
+        int32_t st[2] = {0, 0};
+        if (nu) {
+            k_ty_init<<<NPB, PB, 0, s>>>(nu, rhs, invd, cx, cr, cz, cp, partial);
+            k_cg_init_scal<<<1, 64, 0, s>>>(partial, scal, state, tol, static_cast<int32_t>(maxiter));
+            check_launch();
+            GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            GF_HIP(hipStreamSynchronize(s));
+            constexpr int64_t check_every = 16;
+            for (int64_t it = 0; !st[0] && it < maxiter;) {
+                for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
+                    ty_product(A, cp, nullptr, cq, cp, partial + 2 * NPB, s);
+                    k_cg_update<<<NPB, PB, 0, s>>>(nu, partial + 2 * NPB, NPB, scal, invd, cp, cq, cx, cr, cz, partial, state,
+                                                   it);
+                    k_ty_direction<<<NPB, PB, 0, s>>>(nu, partial, scal, cz, cp, state, it);
+                }
+                check_launch();
+                GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                GF_HIP(hipStreamSynchronize(s));
+            }
+            // true residual of the condensed system
+            ty_product(A, cx, nullptr, cq, nullptr, nullptr, s);
+            k_cg_residual_vec<<<NPB, PB, 0, s>>>(nu, rhs, cq, partial);
+            k_cg_relres<<<1, 64, 0, s>>>(partial, scal + 5);
+            check_launch();
+        }
+        // x = AsNode(X_u, X_p, C_du X_u + C_dp X_p): u and d entries solved, p entries as given
+        k_ty_expand<<<nblocks(nd, BS), BS, 0, s>>>(nu, A->nnp, A->nnd, cx, Xp, A->du_rp.p, A->du_cl.p, A->du_v.p,
+                                                    A->dp_rp.p, A->dp_cl.p, A->dp_v.p, xfull);
+        check_launch();
+        rc = gf_vector_as_node_dofval(A->v, xfull, x, stream);
+        if (rc != GF_OK) throw Error(rc, gf_last_error());
+        double res = 0.0;
+        if (nu) GF_HIP(hipMemcpyAsync(&res, scal + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipStreamSynchronize(s));
+        if (iterations) *iterations = nu ? st[1] : 0;
+        if (relres) *relres = res;
+        if (!(res <= 10.0 * tol))
+            throw Error(GF_ENOTCONV, "PCG on the condensed system did not reach the tolerance (true residual " +
+                                         std::to_string(res) + ")");
+    });
+}
+
+int gf_csr_addmv(int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val, const double* x, double* y,
+                 void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(rowptr && (x || nrows == 0) && (y || nrows == 0), "null argument");
+        if (nrows) k_csr_addmv<<<nblocks(nrows, BS), BS, 0, S(stream)>>>(nrows, rowptr, col, val, x, y);
         check_launch();
     });
 }

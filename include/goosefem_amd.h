@@ -58,6 +58,7 @@ typedef struct gf_diagpart_s* gf_diagpart;
 typedef struct gf_matpart_s* gf_matpart;
 typedef struct gf_plastic_s* gf_plastic;
 typedef struct gf_axi_s* gf_axi;
+typedef struct gf_mattyings_s* gf_mattyings;
 
 const char* gf_last_error(void);
 int gf_version(int* major, int* minor, int* patch);
@@ -245,6 +246,36 @@ int gf_material_elastic_tangent(int64_t nq, int tdim, const double* K_q, const d
 /* The same tangent rounded to FP32 (the input of gf_quad_int_gradN_dot_tensor4_dot_gradNT_dV_f32) */
 int gf_material_elastic_tangent_f32(int64_t nq, int tdim, const double* K_q, const double* G_q, float* C,
                                     void* stream);
+
+/* ================================================================================================
+ * MatrixPartitionedTyings + MatrixPartitionedTyingsSolver for any mesh and element matrices
+ * (include/GooseFEM/MatrixPartitionedTyings.h:34-675).  K is assembled as one CSR over the tied
+ * numbering [u | p | d] (a gf_matrix over v, whose dofs are Tyings::Periodic's dofs()); the
+ * condensed operator K'_uu = K_uu + K_ud C_du + C_ud K_du + C_ud K_dd C_du (:661-663) is applied
+ * matrix-free (expand x_d = C_du x_u + C_dp x_p, one CSR product, fold y_u + C_ud y_d) inside an
+ * FP64 Jacobi-preconditioned CG instead of an LDLT factorisation, so K'_uu must be symmetric
+ * positive definite.  C_du [nnd, nnu] and C_dp [nnd, nnp] are host CSR (int64 rowptr / col, FP64
+ * values), as Tyings::Periodic::Cdu() / Cdp() (TyingsPeriodic.h:244-256).
+ * ============================================================================================== */
+int gf_mattyings_create(gf_mattyings* out, gf_vector v, int64_t nnu, int64_t nnp, int64_t nnd, const int64_t* du_rowptr,
+                        const int64_t* du_col, const double* du_val, const int64_t* dp_rowptr, const int64_t* dp_col,
+                        const double* dp_val);
+int gf_mattyings_destroy(gf_mattyings A);
+/* the unpartitioned CSR (Todense via gf_matrix_todense) */
+int gf_mattyings_matrix(gf_mattyings A, gf_matrix* K);
+/* assemble(elemmat)  (MatrixPartitionedTyings.h:210-305): K, and the Jacobi diagonal of K' */
+int gf_mattyings_assemble(gf_mattyings A, const double* elemmat, void* stream);
+/* MatrixPartitionedTyingsSolver::solve(A, b, x)  (:540-564) on nodevecs: B_u = b_u + C_ud b_d,
+ * X_u = K'^-1 (B_u - K'_up X_p), X_d = C_du X_u + C_dp X_p; writes x's u and d entries, keeps its
+ * p entries.  Stops at ||r|| <= tol ||rhs||; GF_ENOTCONV if the true residual exceeds 10 tol. */
+int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, int64_t maxiter, int64_t* iterations,
+                       double* relres, void* stream);
+
+/* y += A x for a device CSR A (int32 rowptr [nrows + 1], col; FP64 values), each row summed in
+ * entry order onto y's value.  No counterpart in the reference: it applies the transposed tying
+ * matrix in VectorPartitionedTyings::asDofs_i (VectorPartitionedTyings.h:224-251). */
+int gf_csr_addmv(int64_t nrows, const int32_t* rowptr, const int32_t* col, const double* val, const double* x, double* y,
+                 void* stream);
 
 /* ================================================================================================
  * Element::Quad4::QuadratureAxisymmetric  (include/GooseFEM/ElementQuad4Axisymmetric.h:33-467)
