@@ -615,6 +615,81 @@ def bench_statics(args):
     print(json.dumps(res))
 
 
+def bench_periodic_j2(args):
+    """The reference's periodic elasto-plastic example (periodic_elastoplastic.py) at n^2 elements:
+    J2 linear hardening (hard 2x2-element corner), shear increments of 0.01 through the control
+    node, Newton with the assembled tangent and the tied solve (MatrixPartitionedTyings +
+    MatrixPartitionedTyingsSolver: matrix-free condensed CG), all on the device."""
+    import torch
+
+    import goosefem_amd as gf
+
+    n = args.periodic_n
+    mesh = gf.Mesh.Quad4.Regular(n, n)
+    control = gf.Tyings.Control(mesh.coor, mesh.dofs)
+    coor, dofs = control.coor, control.dofs
+    cdofs, cnodes = control.controlDofs, control.controlNodes
+    iip = np.concatenate((cdofs.ravel(), dofs[mesh.nodesOrigin, :].ravel()))
+    ty = gf.Tyings.Periodic(coor, dofs, cdofs, mesh.nodesPeriodic, iip)
+    dofs = ty.dofs
+    Cdu, Cdp, Cdi = ty.Cdu_csr, ty.Cdp_csr, ty.Cdi_csr
+    vector = gf.VectorPartitionedTyings(mesh.conn, dofs, Cdu, Cdp, Cdi)
+    elem = gf.Element.Quad4.QuadraturePlanar(vector.AsElement(coor))
+    ne = mesh.nelem
+    sy = np.full((ne, 4), 0.05)
+    sy[mesh.elementgrid[:2, :2].ravel(), :] = 100.0
+    mat = gf.LinearHardening(np.ones((ne, 4)), 1.0, sy, 1.0)
+    K = gf.MatrixPartitionedTyings(mesh.conn, dofs, Cdu, Cdp)
+    solver = gf.MatrixPartitionedTyingsSolver(tol=1e-10)
+    disp = torch.zeros((coor.shape[0], 2), dtype=torch.float64, device="cuda")
+    fext = torch.zeros_like(disp)
+    fe = torch.empty((ne, 4, 2), dtype=torch.float64, device="cuda")
+    Ke = torch.empty((ne, 8, 8), dtype=torch.float64, device="cuda")
+    newton, cg, t_solve = [], [], 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for inc in range(args.periodic_steps):
+        mat.increment()
+        for it in range(50):
+            mat.newton_q4(elem, vector.AsElement(disp), fe, Ke)
+            fint = vector.AssembleNode(fe)
+            K.assemble(Ke)
+            fres = fext - fint
+            if it > 0:
+                Fext, Fint = vector.AsDofs_i(fext), vector.AsDofs_i(fint)
+                Fext[ty.nnu:ty.nni] = Fint[ty.nnu:ty.nni]
+                nfext = float(Fext.abs().sum())
+                res = float((Fext - Fint).abs().sum()) / (nfext if nfext else 1.0)
+                if res < 1e-5:
+                    newton.append(it)
+                    break
+            du = torch.zeros_like(disp)
+            if it == 0:
+                du[int(cnodes[0]), 1] = 0.01
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            solver.solve(K, fres, du)
+            torch.cuda.synchronize()
+            t_solve += time.perf_counter() - ts
+            cg.append(solver.iterations)
+            disp += du
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res = {
+        "metric": "Periodic elasto-plastic example (Newton + tied solve) wall time",
+        "value": wall,
+        "unit": "s",
+        "higher_is_better": False,
+        "config": {"workload": f"periodic_elastoplastic.py on Quad4 Regular {n}^2, {args.periodic_steps} shear increments of 0.01",
+                   "nelem": ne, "nnu": ty.nnu, "newton_iterations": newton, "cg_iterations": cg,
+                   "plastic_fraction": float((mat.epsp > 0).double().mean())},
+        "dtype": "f64",
+        "us_per_cg_iteration": t_solve / max(sum(cg), 1) * 1e6,
+        "solve_s": t_solve,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -625,7 +700,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned",
-                                           "plastic-refresh", "newton-q4", "statics-q4"],
+                                           "plastic-refresh", "newton-q4", "statics-q4", "periodic-j2"],
                     default="quad4-explicit")
     ap.add_argument("--pcg-n", type=int, default=512)
     ap.add_argument("--pcg-hex", action="store_true", help="periodic Hex8 n^3 instead of Quad4 n^2")
@@ -634,6 +709,8 @@ def main():
     ap.add_argument("--hex-n", type=int, default=128)
     ap.add_argument("--newton-n", type=int, default=2048)
     ap.add_argument("--statics-n", type=int, default=256)
+    ap.add_argument("--periodic-n", type=int, default=128)
+    ap.add_argument("--periodic-steps", type=int, default=4)
     ap.add_argument("--ws-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "goosefem_amd",
                                                          "data", "warmstart_quad4_512.npz"))
     ap.add_argument("--ws-tests", type=int, default=4, help="held-out modulus fields timed")
@@ -675,6 +752,9 @@ def main():
         return
     if args.workload == "statics-q4":
         bench_statics(args)
+        return
+    if args.workload == "periodic-j2":
+        bench_periodic_j2(args)
         return
 
     nx = ny = args.nx

@@ -560,9 +560,10 @@ class _Periodic:
             out[r, col[rowptr[r]:rowptr[r + 1]]] += val[rowptr[r]:rowptr[r + 1]]
         return out
 
-    Cdi_csr = property(lambda self: self._csr(0))
-    Cdu_csr = property(lambda self: self._csr(1))
-    Cdp_csr = property(lambda self: self._csr(2))
+    # (rowptr, col, val, ncols): the form the tied classes take without densifying
+    Cdi_csr = property(lambda self: self._csr(0) + (self.nni,))
+    Cdu_csr = property(lambda self: self._csr(1) + (self.nnu,))
+    Cdp_csr = property(lambda self: self._csr(2) + (self.nnp,))
     Cdi = property(lambda self: self._dense(0, self.nni))
     Cdu = property(lambda self: self._dense(1, self.nnu))
     Cdp = property(lambda self: self._dense(2, self.nnp))
@@ -1513,10 +1514,19 @@ class MatrixPartitioned(_Handle):
         return self.Dot_p(x_u, x_p)
 
 
-def _csr_of(M, ncol=None):
-    """Host CSR (rowptr int64, col int64, val float64) of a dense array or a (rowptr, col, val)."""
+def _ncols(M):
+    """Column count of a dense array or a (rowptr, col, val, ncols) CSR tuple."""
     if isinstance(M, tuple):
-        rp, cl, v = M
+        assert len(M) == 4, "CSR tying matrices are passed as (rowptr, col, val, ncols)"
+        return int(M[3])
+    return int(np.shape(M)[1])
+
+
+def _csr_of(M, ncol=None):
+    """Host CSR (rowptr int64, col int64, val float64) of a dense array or a (rowptr, col, val[,
+    ncols]) tuple."""
+    if isinstance(M, tuple):
+        rp, cl, v = M[:3]
         return _i64a(rp), _i64a(cl), np.ascontiguousarray(v, dtype=np.float64)
     M = np.asarray(M, dtype=np.float64)
     rows, cols = np.nonzero(M)
@@ -1546,9 +1556,10 @@ class VectorPartitionedTyings(Vector):
         self._dp = _csr_of(Cdp)
         di = _csr_of(Cdi)
         self.nnd = self._du[0].size - 1
-        self.nnu = Cdu.shape[1] if not isinstance(Cdu, tuple) else int(self.ndof - self.nnd - Cdp.shape[1])
-        self.nni = self.ndof - self.nnd
-        self.nnp = self.nni - self.nnu
+        self.nnu = _ncols(Cdu)
+        self.nnp = _ncols(Cdp)
+        self.nni = self.nnu + self.nnp
+        assert self.nni + self.nnd == self.ndof, "tying matrices do not match the DOF numbering"
         torch = _torch()
         trp, tcl, tv = _csr_transpose(*di, self.nni)  # C_id, rows i
         self._id = (torch.as_tensor(trp.astype(np.int32), device=device()),
@@ -1610,8 +1621,9 @@ class MatrixPartitionedTyings(_Handle):
         self._du, self._dp = _csr_of(Cdu), _csr_of(Cdp)
         self.nnd = self._du[0].size - 1
         ndof = self._vec.ndof
-        self.nnp = Cdp.shape[1] if not isinstance(Cdp, tuple) else int(self._dp[1].max(initial=-1) + 1)
-        self.nnu = ndof - self.nnd - self.nnp
+        self.nnp = _ncols(Cdp)
+        self.nnu = _ncols(Cdu)
+        assert self.nnu + self.nnp + self.nnd == ndof, "tying matrices do not match the DOF numbering"
         self.nni = self.nnu + self.nnp
         self.ndof = ndof
         h = _vp()
