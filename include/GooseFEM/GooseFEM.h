@@ -1677,6 +1677,89 @@ private:
 } // namespace Tyings
 
 // =================================================================================================
+// MatrixPartitionedTyings + MatrixPartitionedTyingsSolver (include/GooseFEM/MatrixPartitionedTyings.h)
+// for any element matrices: K as one CSR over the tied numbering [u | p | d], the condensed system
+// solved by a matrix-free FP64 Jacobi-PCG (K' symmetric positive definite) instead of LDLT
+// =================================================================================================
+
+class MatrixPartitionedTyings {
+public:
+    template <class C, class D>
+    MatrixPartitionedTyings(const C& conn, const D& dofs, const Tyings::SparseMatrix& Cdu, const Tyings::SparseMatrix& Cdp)
+        : m_vec(conn, dofs)
+    {
+        m_nnu = Cdu.cols;
+        m_nnp = Cdp.cols;
+        m_nnd = Cdu.rows;
+        amd::check(gf_mattyings_create(&m_h, m_vec.handle(), static_cast<int64_t>(m_nnu), static_cast<int64_t>(m_nnp),
+                                       static_cast<int64_t>(m_nnd), Cdu.rowptr.data(), Cdu.col.data(), Cdu.val.data(),
+                                       Cdp.rowptr.data(), Cdp.col.data(), Cdp.val.data()));
+    }
+    MatrixPartitionedTyings(const MatrixPartitionedTyings&) = delete;
+    MatrixPartitionedTyings& operator=(const MatrixPartitionedTyings&) = delete;
+    ~MatrixPartitionedTyings() { gf_mattyings_destroy(m_h); }
+    size_t nnu() const { return m_nnu; }
+    size_t nnp() const { return m_nnp; }
+    size_t nnd() const { return m_nnd; }
+    size_t ndof() const { return m_vec.ndof(); }
+    gf_mattyings handle() const { return m_h; }
+    /* assemble(elemmat)  (MatrixPartitionedTyings.h:210-305) */
+    template <class T>
+    void assemble(const T& elemmat)
+    {
+        amd::dev_in<T> in(elemmat);
+        amd::check(gf_mattyings_assemble(m_h, in.ptr, nullptr));
+        amd::check(gf_stream_sync(nullptr));
+    }
+    amd::array<double> Todense() const
+    {
+        gf_matrix K = nullptr;
+        amd::check(gf_mattyings_matrix(m_h, &K));
+        amd::array<double> r({ndof(), ndof()});
+        amd::dev_out<amd::array<double>> out(r, r.shape());
+        amd::check(gf_matrix_todense(K, out.ptr, nullptr));
+        out.finish();
+        return r;
+    }
+
+private:
+    Vector m_vec;
+    gf_mattyings m_h = nullptr;
+    size_t m_nnu = 0, m_nnp = 0, m_nnd = 0;
+};
+
+class MatrixPartitionedTyingsSolver {
+public:
+    explicit MatrixPartitionedTyingsSolver(double tol = 1e-12, int64_t maxiter = 0) : m_tol(tol), m_maxiter(maxiter) {}
+    int64_t iterations() const { return m_it; }
+    double relres() const { return m_res; }
+    /* solve(A, b, x) / Solve(A, b, x)  (MatrixPartitionedTyings.h:540-564) on nodevecs */
+    template <class B, class X>
+    void solve(MatrixPartitionedTyings& A, const B& b, X& x)
+    {
+        amd::dev_in<B> in(b);
+        amd::dev_inout<X> io(x);
+        const int64_t cap = m_maxiter > 0 ? m_maxiter : std::max<int64_t>(100, 10 * static_cast<int64_t>(A.nnu()));
+        amd::check(gf_mattyings_solve(A.handle(), in.ptr, io.ptr, m_tol, cap, &m_it, &m_res, nullptr));
+        io.finish();
+    }
+    template <class B, class X>
+    amd::array<double> Solve(MatrixPartitionedTyings& A, const B& b, const X& x)
+    {
+        amd::array<double> r(amd::shape_of(x));
+        std::copy(x.data(), x.data() + r.size(), r.data());
+        solve(A, b, r);
+        return r;
+    }
+
+private:
+    double m_tol;
+    int64_t m_maxiter;
+    int64_t m_it = 0;
+    double m_res = 0.0;
+};
+
+// =================================================================================================
 // Iterate::StopList  (include/GooseFEM/Iterate.h): the last n residuals of an iteration
 // =================================================================================================
 namespace Iterate {

@@ -331,6 +331,72 @@ int main()
         auto Cdu = tyings.Cdu();
         REQUIRE(Cdu.rows == tyings.nnd() && Cdu.cols == tyings.nnu());
     }
+    // MatrixPartitionedTyings: a homogeneous elastic periodic cell sheared through a control DOF
+    // deforms affinely (zero fluctuation), so the tied solve must return u = A x + c at every node
+    {
+        GF::Mesh::Quad4::Regular mesh(6, 6);
+        GF::Tyings::Control control(mesh.coor(), mesh.dofs());
+        auto cd = control.controlDofs();
+        auto cn = control.controlNodes();
+        auto dofs0 = control.dofs();
+        GF::array_type::tensor<size_t, 1> iip({6});
+        for (size_t i = 0; i < 4; ++i) iip[i] = cd[i];
+        iip[4] = dofs0(mesh.nodesOrigin(), 0);
+        iip[5] = dofs0(mesh.nodesOrigin(), 1);
+        GF::Tyings::Periodic tyings(control.coor(), dofs0, cd, mesh.nodesPeriodic(), iip);
+        auto dofs = tyings.dofs();
+        auto coor = control.coor();
+        GF::Vector vec(mesh.conn(), dofs);
+        GF::Element::Quad4::QuadraturePlanar quad(vec.AsElement(coor));
+        GF::array_type::tensor<double, 6> C(quad.shape_qtensor4());
+        for (size_t e = 0; e < quad.nelem(); ++e)
+            for (size_t q = 0; q < quad.nip(); ++q)
+                for (size_t i = 0; i < 3; ++i)
+                    for (size_t j = 0; j < 3; ++j)
+                        for (size_t k = 0; k < 3; ++k)
+                            for (size_t l = 0; l < 3; ++l)
+                                C(e, q, i, j, k, l) = 10.0 * (i == j) * (k == l) +
+                                                      2.0 * ((0.5 * ((i == k) * (j == l) + (i == l) * (j == k))) -
+                                                             (i == j) * (k == l) / 3.0);
+        GF::MatrixPartitionedTyings K(mesh.conn(), dofs, tyings.Cdu(), tyings.Cdp());
+        K.assemble(quad.Int_gradN_dot_tensor4_dot_gradNT_dV(C, GF_F64));
+        GF::array_type::tensor<double, 2> b(vec.shape_nodevec()), u(vec.shape_nodevec());
+        u(cn[0], 1) = 0.01;
+        GF::MatrixPartitionedTyingsSolver solver;
+        solver.solve(K, b, u);
+        REQUIRE(solver.relres() < 1e-11);
+        // least-squares affine fit per component over the mesh nodes (not the control nodes)
+        bool ok = true;
+        for (size_t c = 0; c < 2; ++c) {
+            double S[3][3] = {{0}}, R[3] = {0};
+            for (size_t n = 0; n < mesh.nnode(); ++n) {
+                const double f[3] = {coor(n, 0), coor(n, 1), 1.0};
+                for (int a = 0; a < 3; ++a) {
+                    R[a] += f[a] * u(n, c);
+                    for (int bb = 0; bb < 3; ++bb) S[a][bb] += f[a] * f[bb];
+                }
+            }
+            // solve the 3 x 3 normal equations by Cramer's rule
+            auto det3 = [](double M[3][3]) {
+                return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+            };
+            double coef[3];
+            const double D = det3(S);
+            for (int a = 0; a < 3; ++a) {
+                double M[3][3];
+                for (int r = 0; r < 3; ++r)
+                    for (int cc = 0; cc < 3; ++cc) M[r][cc] = cc == a ? R[r] : S[r][cc];
+                coef[a] = det3(M) / D;
+            }
+            for (size_t n = 0; n < mesh.nnode(); ++n)
+                ok &= std::abs(coef[0] * coor(n, 0) + coef[1] * coor(n, 1) + coef[2] - u(n, c)) < 1e-10;
+        }
+        REQUIRE(ok);
+        double umax = 0.0;
+        for (size_t n = 0; n < mesh.nnode(); ++n) umax = std::max(umax, std::abs(u(n, 0)) + std::abs(u(n, 1)));
+        REQUIRE(umax > 1e-3); // the cell did deform
+    }
     if (g_fail) {
         std::printf("%d check(s) failed\n", g_fail);
         return 1;
