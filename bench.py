@@ -607,7 +607,7 @@ def bench_statics(args):
                    "plastic_fraction": float((mat.epsp > 0).double().mean())},
         "dtype": "f64",
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "kernel": "partitioned Jacobi-PCG iteration (k_cg_apply + k_cg_update + k_cg_direction)",
+                     "kernel": "partitioned Jacobi-PCG iteration (k_cg_apply_dir + k_cg_update)",
                      "traffic": None, "algorithmic_bytes_per_launch": b_it, "avg_launch_us": t_it * 1e6},
         "breakdown_s": {"fused_element_pass": t_el, "matrix_assemble": t_asm, "solve": t_solve,
                         "pattern_build": t_pattern},
