@@ -295,6 +295,38 @@ class _Quad4FineLayer:
         return self._set("elementsMiddleLayer")
 
 
+class _Hex8FineLayer(_Quad4FineLayer):
+    """Mesh::Hex8::FineLayer (include/GooseFEM/MeshHex8.h:613-2296): fine element layers around
+    the middle, coarsened by 3 in x or z towards the bottom and top faces (host set-up)."""
+
+    def __init__(self, nelx, nely, nelz, h=1.0, nfine=1):
+        self._args = (_i64(int(nelx)), _i64(int(nely)), _i64(int(nelz)), _f64(float(h)), _i64(int(nfine)))
+        self.h = float(h)
+        self.nne, self.ndim = 8, 3
+        sizes = np.zeros(5, dtype=np.int64)
+        _call("gf_mesh_hex8_finelayer", *self._args, _hp(sizes), None, None)
+        self.nelem, self.nnode, self.nelx, self.nely, self.nelz = (int(v) for v in sizes)
+
+    def _set(self, name):
+        n = _i64(0)
+        _call("gf_mesh_hex8_finelayer_set", *self._args, name.encode(), None, ctypes.byref(n))
+        out = np.empty(n.value, dtype=np.int64)
+        _call("gf_mesh_hex8_finelayer_set", *self._args, name.encode(), _hp(out), ctypes.byref(n))
+        return out
+
+    @property
+    def coor(self):
+        out = np.empty((self.nnode, 3))
+        _call("gf_mesh_hex8_finelayer", *self._args, None, _hp(out), None)
+        return out
+
+    @property
+    def conn(self):
+        out = np.empty((self.nelem, 8), dtype=np.int64)
+        _call("gf_mesh_hex8_finelayer", *self._args, None, None, _hp(out))
+        return out
+
+
 class _Quad4Regular(_RegularBase):
     """Mesh::Quad4::Regular (include/GooseFEM/MeshQuad4.h:32-195)."""
 
@@ -488,7 +520,7 @@ Iterate = SimpleNamespace(StopList=_StopList)
 
 Mesh = SimpleNamespace(
     Quad4=SimpleNamespace(Regular=_Quad4Regular, FineLayer=_Quad4FineLayer),
-    Hex8=SimpleNamespace(Regular=_Hex8Regular),
+    Hex8=SimpleNamespace(Regular=_Hex8Regular, FineLayer=_Hex8FineLayer),
     Tri3=SimpleNamespace(Regular=_Tri3Regular, getOrientation=_tri3_get_orientation,
                          setOrientation=_tri3_set_orientation),
     dofs=lambda nnode, ndim: np.arange(nnode * ndim, dtype=np.int64).reshape(nnode, ndim),

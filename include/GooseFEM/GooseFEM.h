@@ -531,6 +531,81 @@ public:
     }
 };
 
+/* Mesh::Hex8::FineLayer  (include/GooseFEM/MeshHex8.h:613-2296): layers finest around the
+ * middle, coarsened by 3 in x or z towards the bottom and top; node sets by name via
+ * nodeset(name) ("nodesFront", "nodesFrontFace", "nodesFrontBottomEdge", ...) */
+class FineLayer {
+public:
+    FineLayer(size_t nelx, size_t nely, size_t nelz, double h = 1.0, size_t nfine = 1)
+        : m_nx(static_cast<int64_t>(nelx)), m_ny(static_cast<int64_t>(nely)), m_nz(static_cast<int64_t>(nelz)),
+          m_nf(static_cast<int64_t>(nfine)), m_h(h)
+    {
+        int64_t sz[5] = {0, 0, 0, 0, 0};
+        amd::check(gf_mesh_hex8_finelayer(m_nx, m_ny, m_nz, m_h, m_nf, sz, nullptr, nullptr));
+        m_nelem = static_cast<size_t>(sz[0]);
+        m_nnode = static_cast<size_t>(sz[1]);
+        m_nelx = static_cast<size_t>(sz[2]);
+        m_nely = static_cast<size_t>(sz[3]);
+        m_nelz = static_cast<size_t>(sz[4]);
+    }
+    size_t nelem() const { return m_nelem; }
+    size_t nnode() const { return m_nnode; }
+    size_t nne() const { return 8; }
+    size_t ndim() const { return 3; }
+    size_t nelx() const { return m_nelx; }
+    size_t nely() const { return m_nely; }
+    size_t nelz() const { return m_nelz; }
+    double h() const { return m_h; }
+    amd::array<double> coor() const
+    {
+        amd::array<double> c({m_nnode, 3});
+        amd::check(gf_mesh_hex8_finelayer(m_nx, m_ny, m_nz, m_h, m_nf, nullptr, c.data(), nullptr));
+        return c;
+    }
+    amd::array<size_t> conn() const
+    {
+        std::vector<int64_t> c(m_nelem * 8);
+        amd::check(gf_mesh_hex8_finelayer(m_nx, m_ny, m_nz, m_h, m_nf, nullptr, nullptr, c.data()));
+        return amd::from_i64(c, {m_nelem, 8});
+    }
+    amd::array<size_t> dofs() const { return Mesh::dofs(m_nnode, 3); }
+    amd::array<size_t> nodeset(const std::string& name) const
+    {
+        int64_t n = 0;
+        amd::check(gf_mesh_hex8_finelayer_set(m_nx, m_ny, m_nz, m_h, m_nf, name.c_str(), nullptr, &n));
+        std::vector<int64_t> r(static_cast<size_t>(n));
+        amd::check(gf_mesh_hex8_finelayer_set(m_nx, m_ny, m_nz, m_h, m_nf, name.c_str(), r.data(), &n));
+        return amd::from_i64(r, {static_cast<size_t>(n)});
+    }
+    size_t node(const std::string& name) const { return nodeset(name)[0]; }
+    amd::array<size_t> elementsMiddleLayer() const { return nodeset("elementsMiddleLayer"); }
+    amd::array<size_t> nodesFront() const { return nodeset("nodesFront"); }
+    amd::array<size_t> nodesBack() const { return nodeset("nodesBack"); }
+    amd::array<size_t> nodesLeft() const { return nodeset("nodesLeft"); }
+    amd::array<size_t> nodesRight() const { return nodeset("nodesRight"); }
+    amd::array<size_t> nodesBottom() const { return nodeset("nodesBottom"); }
+    amd::array<size_t> nodesTop() const { return nodeset("nodesTop"); }
+    size_t nodesOrigin() const { return node("nodesOrigin"); }
+    amd::array<size_t> nodesPeriodic() const
+    {
+        auto r = nodeset("nodesPeriodic");
+        return amd::from_i64(amd::to_i64(r), {r.size() / 2, 2});
+    }
+    amd::array<size_t> dofsPeriodic() const
+    {
+        auto t = amd::to_i64(nodesPeriodic());
+        std::vector<int64_t> r(m_nnode * 3);
+        amd::check(gf_mesh_dofs_periodic(static_cast<int64_t>(m_nnode), 3, t.data(), static_cast<int64_t>(t.size() / 2),
+                                         r.data()));
+        return amd::from_i64(r, {m_nnode, 3});
+    }
+
+private:
+    int64_t m_nx, m_ny, m_nz, m_nf;
+    double m_h;
+    size_t m_nelem = 0, m_nnode = 0, m_nelx = 0, m_nely = 0, m_nelz = 0;
+};
+
 } // namespace Hex8
 } // namespace Mesh
 

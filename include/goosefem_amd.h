@@ -103,6 +103,20 @@ int gf_mesh_quad4_finelayer(int64_t nelx, int64_t nely, double h, int64_t nfine,
  * flattened [nties * 2], the rule of Mesh.h:511-540).  *n receives the count. */
 int gf_mesh_quad4_finelayer_set(int64_t nelx, int64_t nely, double h, int64_t nfine, const char* name,
                                 int64_t* out_h, int64_t* n);
+/* Mesh::Hex8::FineLayer(nelx, nely, nelz, h, nfine)  (include/GooseFEM/MeshHex8.h:613-2296):
+ * element layers finest around the middle, coarsened by 3 in x or in z towards the bottom and top.
+ * sizes (any output may be NULL) receives {nelem, nnode, nelx(), nely(), nelz()} (nelx(), nelz()
+ * the finest layer's element counts, nely() the height in fine elements, MeshHex8.h:874-886);
+ * coor_h [nnode, 3], conn_h [nelem, 8]. */
+int gf_mesh_hex8_finelayer(int64_t nelx, int64_t nely, int64_t nelz, double h, int64_t nfine, int64_t* sizes,
+                           double* coor_h, int64_t* conn_h);
+/* Named sets of that mesh: the faces ("nodesFront" ...), face interiors ("nodesFrontFace" ...),
+ * edges and open edges ("nodesFrontBottomEdge", "nodesFrontBottomOpenEdge" ..., the words in any
+ * order), corners, "nodesOrigin", "elementsMiddleLayer" (MeshHex8.h:855-870, 1331-2275) and
+ * "nodesPeriodic" (pairs, flattened [nties * 2], the rule of Mesh.h:1577-1692).  *n receives the
+ * count. */
+int gf_mesh_hex8_finelayer_set(int64_t nelx, int64_t nely, int64_t nelz, double h, int64_t nfine, const char* name,
+                               int64_t* out_h, int64_t* n);
 /* Mesh::dofs(nnode, ndim)  (Mesh.h:93-96) */
 int gf_mesh_dofs(int64_t nnode, int ndim, int64_t* dofs_h);
 /* RegularBase::dofsPeriodic()  (Mesh.h:291-304): dependent nodes take their independent node's

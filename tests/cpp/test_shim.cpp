@@ -1,7 +1,7 @@
 // Drop-in check of the C++ face (include/GooseFEM/GooseFEM.h): the same calls and checks as the
 // reference's C++ unit tests (tests/basic/ElementQuad4Planar.cpp, ElementHex8.cpp,
 // MatrixDiagonal.cpp, Matrix.cpp, Vector.cpp, MeshQuad4.cpp, VectorPartitioned.cpp,
-// MatrixDiagonalPartitioned.cpp, MatrixParitioned.cpp), the FineLayer mesh, the axisymmetric
+// MatrixDiagonalPartitioned.cpp, MatrixParitioned.cpp), the FineLayer meshes, the axisymmetric
 // element and the examples' J2 material law, against the GPU path.
 #include <GooseFEM/GooseFEM.h>
 
@@ -223,6 +223,23 @@ int main()
         REQUIRE(per.shape(0) == 15 && per(1, 1) == 95 && per(14, 0) == 88 && per(14, 1) == 91);
         auto conn = mesh.conn();
         REQUIRE(conn(7, 1) == 21 && conn(63, 3) == 84);
+    }
+    // MeshHex8.cpp:192-377: FineLayer(3, 3, 3) sizes, connectivity, node sets and periodic pairs;
+    // a mesh with transition layers keeps nelx, nelz and the middle layer's size
+    {
+        GF::Mesh::Hex8::FineLayer mesh(3, 3, 3);
+        REQUIRE(mesh.nelem() == 27 && mesh.nnode() == 64 && mesh.nely() == 3);
+        auto conn = mesh.conn();
+        REQUIRE(conn(26, 0) == 42 && conn(26, 3) == 58 && conn(26, 7) == 62);
+        auto bot = mesh.nodesBottom();
+        REQUIRE(bot.size() == 16 && bot[1] == 4 && bot[4] == 1);
+        auto face = mesh.nodeset("nodesBottomFace");
+        REQUIRE(face.size() == 4 && face[0] == 5 && face[1] == 9 && face[2] == 6);
+        auto per = mesh.nodesPeriodic();
+        REQUIRE(per.shape(0) == 37 && per(0, 1) == 3 && per(36, 0) == 10 && per(36, 1) == 58);
+        REQUIRE(mesh.dofsPeriodic()(63, 2) == 2 && mesh.nodesOrigin() == 0);
+        GF::Mesh::Hex8::FineLayer fine(9, 17, 27);
+        REQUIRE(fine.nelx() == 9 && fine.nelz() == 27 && fine.elementsMiddleLayer().size() == 243);
     }
     // QuadratureAxisymmetric: the volume of a hollow cylinder, and the material law's consistency
     {

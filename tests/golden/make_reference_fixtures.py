@@ -10,6 +10,7 @@ Sources (relative to the reference root):
   tests/basic/MeshQuad4.cpp:103-256     FineLayer(9,18): coor, conn, dofs, node sets, periodic
   tests/basic/MeshTri3.cpp:6-88         Tri3 Regular(5,3): coor, conn, dofs, node sets, periodic
   tests/basic/MeshHex8.cpp:8-188        Regular(3,3,3): coor, conn, node sets, dofs, periodic
+  tests/basic/MeshHex8.cpp:192-377      FineLayer(3,3,3): coor, conn, node sets, dofs, periodic
   tests/basic/Mesh.cpp:199-236          Regular(3,3): coordination, elem2node
   tests/test_Mesh.py:23-171             Regular(4,4) periodic: coordination, node2dof,
                                         nodaltyings, elem2node(conn, dofsPeriodic)
@@ -95,6 +96,7 @@ def main(root: str) -> None:
 
     src = open(os.path.join(basic, "MeshHex8.cpp")).read()
     fx["MeshHex8_Regular_3_3_3"] = cpp_arrays(section(src, "Regular"))
+    fx["MeshHex8_FineLayer_3_3_3"] = cpp_arrays(section(src, "FineLayer - basic"))
 
     src = open(os.path.join(basic, "Mesh.cpp")).read()
     sec = section(src, "coordination")

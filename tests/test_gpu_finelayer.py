@@ -118,3 +118,30 @@ def test_finelayer_partitioned_statics(gf):
     ref = oA.Solve(np.zeros_like(disp), disp)
     got = host(gf.MatrixPartitionedSolver(tol=1e-12).Solve(gA, np.zeros_like(disp), disp))
     assert np.max(np.abs(got - ref)) <= 1e-8 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("nelx,nely,nelz", [(9, 17, 27), (27, 24, 9)])
+def test_hex8_finelayer_element_path_bit_exact(gf, nelx, nely, nelz):
+    """Mesh::Hex8::FineLayer (MeshHex8.h:613-2296): transition layers of wedge-like hexahedra."""
+    from oracle import HEX8
+
+    m = gf.Mesh.Hex8.FineLayer(nelx, nely, nelz)
+    coor, conn, dofs = m.coor, m.conn, m.dofs
+    ov = oracle.Vector(conn, dofs)
+    oq = oracle.Quadrature(HEX8, ov.AsElement(coor))
+    gv = gf.Vector(conn, dofs)
+    gq = gf.Element.Hex8.Quadrature(gv.AsElement(coor))
+    assert np.array_equal(host(gq.dV), oq.dV())
+    assert len(np.unique(np.round(oq.dV(), 12))) > 2
+    assert abs(oq.dV().sum() - np.prod(coor.max(axis=0))) < 1e-9 * np.prod(coor.max(axis=0))
+    rng = np.random.default_rng(1)
+    ue = ov.AsElement(rng.standard_normal(coor.shape) * 1e-2)
+    eps = oq.SymGradN_vector(ue)
+    assert np.array_equal(host(gq.SymGradN_vector(ue)), eps)
+    K = np.full((oq.nelem, oq.nip), 10.0)
+    G = np.exp(0.3 * rng.standard_normal((oq.nelem, oq.nip)))
+    fe = oq.Int_gradN_dot_tensor2_dV(oracle.elastic_stress(eps, K, G))
+    assert np.array_equal(host(gq.Int_gradN_dot_tensor2_dV(oracle.elastic_stress(eps, K, G))), fe)
+    assert np.array_equal(host(gv.AssembleNode(fe)), ov.AssembleNode(fe))
+    C = oracle.elastic_tangent(K, G)
+    assert np.array_equal(host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C)), oq.Int_gradN_dot_tensor4_dot_gradNT_dV(C))
