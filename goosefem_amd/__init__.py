@@ -424,6 +424,52 @@ def _nodaltyings(dofs):
     return _csr_lists(ptr, idx)
 
 
+def _quad4_only(coor, conn):
+    coor, conn = np.asarray(coor, dtype=np.float64), _i64a(conn)
+    if coor.ndim != 2 or conn.ndim != 2 or coor.shape[1] != 2 or conn.shape[1] != 4:
+        raise NotImplementedError("Element-type not implemented")
+    if conn.size and conn.max() >= coor.shape[0]:
+        raise ValueError("conn refers to a node beyond coor")
+    return coor, conn
+
+
+def _edgesize(coor, conn, elementType=None):
+    """Mesh::edgesize (Mesh.h:2571-2615): the length of each Quad4 element edge, [nelem, 4],
+    edge i running from local node i to local node i + 1 (host set-up)."""
+    coor, conn = _quad4_only(coor, conn)
+    x, y = coor[conn, 0], coor[conn, 1]
+    dx, dy = np.roll(x, -1, axis=1) - x, np.roll(y, -1, axis=1) - y
+    return np.sqrt(dx * dx + dy * dy)
+
+
+def _centers(coor, conn, elementType=None):
+    """Mesh::centers (Mesh.h:2626-2659): the mean of each Quad4 element's nodal coordinates, the
+    four nodes summed in local order, then divided by 4 (host set-up)."""
+    coor, conn = _quad4_only(coor, conn)
+    ret = np.zeros((conn.shape[0], coor.shape[1]))
+    for i in range(4):
+        ret += coor[conn[:, i]]
+    return ret / 4.0
+
+
+def _elemmap2nodemap(elem_map, coor, conn, elementType=None):
+    """Mesh::elemmap2nodemap (Mesh.h:2670-2709): the node map that goes with an element map
+    (new_elemvar = elemvar[elem_map]) on a Quad4 mesh.  Local node column i = 0..3 maps the nodes
+    in that column of conn to the same column of conn[elem_map] (the last element that lists a
+    node wins); a node keeps the first column's mapping that reaches it (host set-up)."""
+    coor, conn = _quad4_only(coor, conn)
+    elem_map = _i64a(elem_map)
+    if elem_map.shape != (conn.shape[0],):
+        raise ValueError("elem_map must have one entry per element")
+    N = coor.shape[0]
+    ret = np.full(N, N, dtype=np.int64)
+    for i in range(4):
+        t = np.full(N, N, dtype=np.int64)
+        t[conn[:, i]] = conn[elem_map, i]
+        ret = np.where(ret == N, t, ret)
+    return ret
+
+
 class _Tri3Regular(_Quad4Regular):
     """Mesh::Tri3::Regular (include/GooseFEM/MeshTri3.h:26-232): the Quad4 Regular grid's nodes,
     node sets and periodic pairs, each square split into two triangles (host set-up)."""
@@ -527,6 +573,9 @@ Mesh = SimpleNamespace(
     renumber=_renumber,
     nodal_mass=_nodal_mass,
     center_of_gravity=_center_of_gravity,
+    centers=_centers,
+    edgesize=_edgesize,
+    elemmap2nodemap=_elemmap2nodemap,
     coordination=_coordination,
     elem2node=_elem2node,
     nodaltyings=_nodaltyings,
