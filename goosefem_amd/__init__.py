@@ -294,6 +294,13 @@ class _Quad4FineLayer:
     def elementsMiddleLayer(self):
         return self._set("elementsMiddleLayer")
 
+    def roll(self, n):
+        """FineLayer::roll (MeshQuad4.h:539-564): element map shifting every layer by n coarse
+        elements in x; new_elemvar = elemvar[roll(n)]."""
+        out = np.empty(self.nelem, dtype=np.int64)
+        _call("gf_mesh_quad4_finelayer_roll", *self._args, _i64(int(n)), _hp(out))
+        return out
+
 
 class _Hex8FineLayer(_Quad4FineLayer):
     """Mesh::Hex8::FineLayer (include/GooseFEM/MeshHex8.h:613-2296): fine element layers around

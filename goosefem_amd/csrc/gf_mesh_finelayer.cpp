@@ -236,4 +236,21 @@ int gf_mesh_quad4_finelayer_set(int64_t nelx, int64_t nely, double h, int64_t nf
     });
 }
 
+// FineLayer::roll (MeshQuad4.h:539-564): the element map that shifts every layer by n elements of
+// the coarsest layer in x (periodically); a layer k times finer shifts by k n elements, a
+// transition layer (four elements per coarse element) by 4 k n.  out[nelem]: new_elemvar =
+// elemvar[out].
+int gf_mesh_quad4_finelayer_roll(int64_t nelx, int64_t nely, double h, int64_t nfine, int64_t n, int64_t* out)
+{
+    return guard([&] {
+        GF_REQUIRE(n >= 0 && out != nullptr, "roll: n >= 0 and an output");
+        const auto m = plan(nelx, nely, h, nfine);
+        for (int64_t k = 0; k < m.layers(); ++k) {
+            const int64_t per = m.refine[k] == -1 ? 1 : 4;
+            const int64_t nel = per * m.lnx[k], shift = (n * (m.lnx[k] / m.lnx[0]) * per) % nel;
+            for (int64_t i = 0; i < nel; ++i) out[m.start_elem[k] + (i + shift) % nel] = m.start_elem[k] + i;
+        }
+    });
+}
+
 } // extern "C"

@@ -473,6 +473,12 @@ public:
     }
     size_t node(const std::string& name) const { return nodeset(name)[0]; }
     amd::array<size_t> elementsMiddleLayer() const { return nodeset("elementsMiddleLayer"); }
+    amd::array<size_t> roll(size_t n) const
+    {
+        std::vector<int64_t> r(m_nelem);
+        amd::check(gf_mesh_quad4_finelayer_roll(m_nx, m_ny, m_h, m_nf, static_cast<int64_t>(n), r.data()));
+        return amd::from_i64(r, {m_nelem});
+    }
     amd::array<size_t> nodesBottomEdge() const { return nodeset("nodesBottomEdge"); }
     amd::array<size_t> nodesTopEdge() const { return nodeset("nodesTopEdge"); }
     amd::array<size_t> nodesLeftEdge() const { return nodeset("nodesLeftEdge"); }

@@ -103,6 +103,10 @@ int gf_mesh_quad4_finelayer(int64_t nelx, int64_t nely, double h, int64_t nfine,
  * flattened [nties * 2], the rule of Mesh.h:511-540).  *n receives the count. */
 int gf_mesh_quad4_finelayer_set(int64_t nelx, int64_t nely, double h, int64_t nfine, const char* name,
                                 int64_t* out_h, int64_t* n);
+/* FineLayer::roll(n)  (include/GooseFEM/MeshQuad4.h:539-564): the element map (out_h [nelem],
+ * new_elemvar = elemvar[out_h]) that shifts every element layer periodically in x by n elements
+ * of the coarsest layer. */
+int gf_mesh_quad4_finelayer_roll(int64_t nelx, int64_t nely, double h, int64_t nfine, int64_t n, int64_t* out_h);
 /* Mesh::Hex8::FineLayer(nelx, nely, nelz, h, nfine)  (include/GooseFEM/MeshHex8.h:613-2296):
  * element layers finest around the middle, coarsened by 3 in x or in z towards the bottom and top.
  * sizes (any output may be NULL) receives {nelem, nnode, nelx(), nely(), nelz()} (nelx(), nelz()

@@ -13,6 +13,7 @@ Sources (relative to the reference root):
   tests/basic/MeshHex8.cpp:192-377      FineLayer(3,3,3): coor, conn, node sets, dofs, periodic
   tests/basic/Mesh.cpp:199-236          Regular(3,3): coordination, elem2node
   tests/basic/Mesh.cpp:207-213, 236-258 Regular(2,2,h=2) centers; Regular(3,3) elemmap2nodemap
+  tests/basic/Mesh.cpp:260-382          FineLayer(3,3) roll + elemmap2nodemap, examples 1 and 2
   tests/test_Mesh.py:23-171             Regular(4,4) periodic: coordination, node2dof,
                                         nodaltyings, elem2node(conn, dofsPeriodic)
   tests/basic/MatrixDiagonal.cpp:12-53  lumped mass of Regular(2,2) (Nodal quadrature)
@@ -104,6 +105,9 @@ def main(root: str) -> None:
     fx["Mesh_coordination_Regular_3_3"] = cpp_arrays(sec)["ret"]
     fx["Mesh_centers_Regular_2_2_h2"] = cpp_arrays(section(src, "centers"))["c"]
     fx["Mesh_elemmap2nodemap_Regular_3_3"] = cpp_arrays(section(src, "elemmap2nodemap"))
+    for k in (1, 2):
+        fx[f"Mesh_elemmap2nodemap_FineLayer_3_3_example{k}"] = cpp_arrays(
+            section(src, f"elemmap2nodemap - example {k}"))
     sec = section(src, "elem2node")
     lists = re.findall(r"tonode\[(\d+)\] == std::vector<size_t>\{([^}]*)\}", sec)
     fx["Mesh_elem2node_Regular_3_3"] = [
