@@ -82,9 +82,73 @@ _i32 = ctypes.c_int
 _f64 = ctypes.c_double
 
 
-def _call(name, *args):
+_CTYPE_OF = {
+    "int": ctypes.c_int,
+    "int32_t": ctypes.c_int32,
+    "int64_t": ctypes.c_int64,
+    "uint64_t": ctypes.c_uint64,
+    "double": ctypes.c_double,
+    "float": ctypes.c_float,
+}
+_signatures = None
+
+
+def signatures():
+    """{name: [ctypes argument types]} of every function the C ABI header declares.
+
+    Parsed from include/goosefem_amd.h, so that each call is checked for its argument count and
+    scalar types before it crosses the ABI (a mismatch raises TypeError instead of passing garbage
+    through a register).  Pointers and handles are void*, `const char*` is a C string.
+    """
+    global _signatures
+    if _signatures is None:
+        import re
+
+        hdr = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "goosefem_amd.h")
+        text = open(hdr).read()
+        text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", " ", text)
+        sigs = {}
+        for m in re.finditer(r"\bint\s+(gf_\w+)\s*\(([^)]*)\)\s*;", text):
+            name, params = m.group(1), " ".join(m.group(2).split())
+            types = []
+            if params not in ("", "void"):
+                for p in params.split(","):
+                    p = p.strip()
+                    if "*" in p:
+                        is_str = re.match(r"const\s+char\s*\*\s*\w+$", p) is not None
+                        types.append(ctypes.c_char_p if is_str else ctypes.c_void_p)
+                        continue
+                    words = p.replace("const ", "").split()
+                    base = words[0]
+                    if base in _CTYPE_OF:
+                        types.append(_CTYPE_OF[base])
+                    elif base.startswith("gf_"):  # opaque handle
+                        types.append(ctypes.c_void_p)
+                    else:
+                        raise ImportError(f"goosefem_amd.h: unknown parameter type {p!r} of {name}")
+            sigs[name] = types
+        _signatures = sigs
+    return _signatures
+
+
+def _fn(name):
+    """The ABI function `name` with its argument types and int status return set."""
     f = getattr(lib(), name)
-    f.restype = ctypes.c_int
+    if getattr(f, "_gf_typed", False) is False:
+        sig = signatures().get(name)
+        if sig is None:
+            raise AttributeError(f"{name} is not declared in include/goosefem_amd.h")
+        f.argtypes = sig
+        f.restype = ctypes.c_int
+        f._gf_typed = True
+    return f
+
+
+def _call(name, *args):
+    f = _fn(name)
+    if len(args) != len(f.argtypes):
+        raise TypeError(f"{name} takes {len(f.argtypes)} arguments ({len(args)} given)")
     _check(f(*args))
 
 
@@ -320,6 +384,11 @@ class _Hex8FineLayer(_Quad4FineLayer):
         out = np.empty(n.value, dtype=np.int64)
         _call("gf_mesh_hex8_finelayer_set", *self._args, name.encode(), _hp(out), ctypes.byref(n))
         return out
+
+    def roll(self, n):
+        """The reference's Hex8 FineLayer has no roll (only Mesh::Quad4::FineLayer::roll,
+        MeshQuad4.h:539-564, exists)."""
+        raise NotImplementedError("Mesh.Hex8.FineLayer has no roll (the reference defines it for Quad4 only)")
 
     @property
     def coor(self):
@@ -663,10 +732,7 @@ Tyings = SimpleNamespace(Control=_Control, Periodic=_Periodic)
 def randn(seed: int, n: int, mean: float = 0.0, std: float = 1.0) -> np.ndarray:
     """Portable SplitMix64 + Box-Muller normal samples (host)."""
     out = np.empty(n)
-    f = lib().gf_randn
-    f.argtypes = [ctypes.c_uint64, _i64, _f64, _f64, _vp]
-    f.restype = ctypes.c_int
-    _check(f(seed, n, mean, std, _hp(out)))
+    _call("gf_randn", seed, n, mean, std, _hp(out))
     return out
 
 
@@ -2056,9 +2122,7 @@ class TyingsPCG(_Handle):
         if x0 is None:
             x.zero_()
         it, res = _i64(), _f64()
-        f = lib().gf_pcg_solve
-        f.restype = ctypes.c_int
-        rc = f(self._h, _dp(b), _dp(x), _f64(tol), _i64(maxiter), _i64(check_every), ctypes.byref(it),
+        rc = _fn("gf_pcg_solve")(self._h, _dp(b), _dp(x), _f64(tol), _i64(maxiter), _i64(check_every), ctypes.byref(it),
                ctypes.byref(res), _stream())
         if rc != GF_OK and (raise_on_fail or rc != GF_ENOTCONV):
             _check(rc)

@@ -18,8 +18,9 @@
 //   * elements run bottom to top, z outer, x inner, four per coarse element in a transition layer.
 // Node sets are selected on the node coordinates (which lie exactly on the bounding planes) and
 // ordered as the reference orders them: by node number, except the bottom and top faces, which
-// run x outer, z inner (MeshHex8.h:1531-1573, 1851-1892).  nodesPeriodic pairs corners, then
-// edges, then faces (Mesh.h:1577-1692).
+// run x outer, z inner (MeshHex8.h:1531-1573, 1851-1892).  The open side faces leave out the
+// intermediate nodes of the two outermost layers, as the reference's layer loops do.
+// nodesPeriodic pairs corners, then edges, then faces (Mesh.h:1577-1692).
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -311,6 +312,16 @@ bool named(const FineLayerH8& m, const std::string& name, std::vector<int64_t>& 
     const double tol = 1e-9 * m.h;
     auto on = [&](double v, int side, int a) { return std::abs(v - (side ? hi[a] : 0.0)) <= tol; };
     auto inside = [&](double v, int a) { return v > tol && v < hi[a] - tol; };
+    // The reference's open Front / Back / Left / Right faces run over the node rows 1 .. L - 1 and
+    // the intermediate nodes of the transition layers 1 .. L - 2 only (MeshHex8.h:1575-1850): the
+    // intermediate nodes of the outermost layers 0 and L - 1, which lie inside those faces, are not
+    // part of them, and so are not tied by nodesPeriodic either.
+    auto outer_intermediate = [&](int64_t n) {
+        for (int64_t k : {int64_t(0), L - 1})
+            if (m.refine[k] != -1 && n >= m.start_node[k] + m.nnd[k] && n < m.start_node[k + 1]) return true;
+        return false;
+    };
+    const bool side_face = kind == "Face" && fixed[1] == -1;
     for (int64_t n = 0; n < m.nnode; ++n) {
         bool ok = true;
         for (int a = 0; a < 3 && ok; ++a) {
@@ -318,6 +329,7 @@ bool named(const FineLayerH8& m, const std::string& name, std::vector<int64_t>& 
             if (fixed[a] != -1) ok = on(v, fixed[a], a);
             else if (open) ok = inside(v, a);
         }
+        if (ok && side_face && outer_intermediate(n)) ok = false;
         if (ok) r.push_back(n);
     }
     // the bottom and top faces run x outer, z inner over their node row

@@ -7,6 +7,7 @@ the box surface, every node used once, node sets equal to the nodes on their pla
 reference's order, periodic pairs that differ by box vectors and tie every node of the back, right
 and top faces, and a middle layer of nelx * nelz elements centred at Ly / 2."""
 
+import ctypes
 import itertools
 import json
 import os
@@ -93,13 +94,31 @@ def test_hex8_finelayer_invariants(nelx, nely, nelz, nfine):
     assert np.array_equal(m.nodesBottomFrontEdge, fb)
     assert np.array_equal(m.nodesFrontBottomOpenEdge, fb[1:-1])
     assert m.nodesBackTopRightCorner == int(np.where((abs(c - [Lx, Ly, Lz]) < tol).all(axis=1))[0][0])
+    # open side faces: the nodes inside the plane, without the intermediate nodes of the two
+    # outermost element layers (the reference's loops run over the layers 1 .. L - 2 only,
+    # MeshHex8.h:1575-1850); the node rows are the y values of a vertical edge
+    rows = c[m.nodesFrontLeftEdge, 1]
+    assert np.all(np.diff(rows) > 0)
+    outer = (c[:, 1] > tol) & (c[:, 1] < rows[1] - tol) | (c[:, 1] > rows[-2] + tol) & (c[:, 1] < Ly - tol)
+    box = (Lx, Ly, Lz)
+    for name, (a, v) in planes.items():
+        if name in ("Bottom", "Top"):
+            continue
+        o = 2 - a  # the other horizontal axis
+        inside = (abs(c[:, a] - v) < tol) & (c[:, o] > tol) & (c[:, o] < box[o] - tol)
+        inside &= (c[:, 1] > tol) & (c[:, 1] < Ly - tol)
+        assert np.array_equal(getattr(m, "nodes" + name + "Face"), np.where(inside & ~outer)[0]), name
     t = m.nodesPeriodic
     d = c[t[:, 1]] - c[t[:, 0]]
     for a, L in enumerate((Lx, Ly, Lz)):
         assert np.all((abs(d[:, a]) < tol) | (abs(d[:, a] - L) < tol))
     dep = set(t[:, 1].tolist())
     tied = (abs(c[:, 0] - Lx) < tol) | (abs(c[:, 1] - Ly) < tol) | (abs(c[:, 2] - Lz) < tol)
-    assert len(dep) == len(t) and dep == set(np.where(tied)[0].tolist())
+    # every back / right / top node is tied except the outer-layer intermediate nodes that the
+    # reference's open faces leave out
+    loose = outer & ((abs(c[:, 0] - Lx) < tol) | (abs(c[:, 2] - Lz) < tol))
+    loose &= (c[:, 0] > tol) & (c[:, 0] < Lx - tol) | (c[:, 2] > tol) & (c[:, 2] < Lz - tol)
+    assert len(dep) == len(t) and dep == set(np.where(tied & ~loose)[0].tolist())
     assert set(t[:, 0].tolist()).isdisjoint(dep)
     assert m.dofsPeriodic.max() + 1 == 3 * (m.nnode - len(t))
     ml = m.elementsMiddleLayer()
@@ -113,3 +132,60 @@ def test_hex8_finelayer_unknown_set():
     m = gf.Mesh.Hex8.FineLayer(3, 3, 3)
     with pytest.raises(Exception):
         m.nodesFrontBackEdge
+
+
+def _reference_face_count(m, face):
+    """Size of nodes<face>Face by the reference's count loops (MeshHex8.h:1575-1600 Front/Back,
+    1713-1740 Left/Right): over the layers iy = 1 .. (L + 1) / 2 - 1 and (L - 1) / 2 .. L - 2,
+    3 n - 1 nodes for a layer refined across that face (n coarse elements along it), else n - 1.
+    The layer data are read off the mesh: the coarse node row of layer k is its bottom row below
+    the middle layer and its top row from the middle up; a layer is refined across the Front face
+    when it has nodes at z = 0 strictly between its two rows."""
+    c = m.coor
+    tol = 1e-9
+    a, o = (2, 0) if face in ("Front", "Back") else (0, 2)
+    v = 0.0 if face in ("Front", "Left") else c[:, a].max()
+    rows = c[m.nodesFrontLeftEdge, 1]
+    L = len(rows) - 1
+    plane = abs(c[:, a] - v) < tol
+
+    def layer(k):
+        coarse = rows[k] if k < (L - 1) // 2 else rows[k + 1]
+        n = int((plane & (abs(c[:, 1] - coarse) < tol)).sum()) - 1
+        refined = bool((plane & (c[:, 1] > rows[k] + tol) & (c[:, 1] < rows[k + 1] - tol)).any())
+        return 3 * n - 1 if refined else n - 1
+
+    return sum(layer(k) for k in range(1, (L + 1) // 2)) + sum(layer(k) for k in range((L - 1) // 2, L - 1))
+
+
+@pytest.mark.parametrize("nelx,nely,nelz,nfine", CASES + [(3, 9, 3, 1)])
+def test_hex8_finelayer_face_counts(nelx, nely, nelz, nfine):
+    """The open side faces have the reference's sizes, also when the outermost layers are
+    transition layers (e.g. (3, 9, 3) and (9, 31, 27): 12 and 72 front-face nodes, where the
+    nodes inside the face planes number 16 and 76)."""
+    m = gf.Mesh.Hex8.FineLayer(nelx, nely, nelz, 1.0, nfine)
+    for face in ("Front", "Back", "Left", "Right"):
+        assert len(getattr(m, "nodes" + face + "Face")) == _reference_face_count(m, face), face
+    known = {(3, 9, 3): 12, (9, 31, 27): 72}
+    if (nelx, nely, nelz) in known:
+        assert len(m.nodesFrontFace) == known[(nelx, nely, nelz)]
+        assert len(m.nodesBackFace) == known[(nelx, nely, nelz)]
+
+
+def test_hex8_finelayer_has_no_roll():
+    """Only the Quad4 FineLayer has roll (MeshQuad4.h:539-564); the Hex8 one raises cleanly."""
+    m = gf.Mesh.Hex8.FineLayer(3, 3, 3)
+    with pytest.raises(NotImplementedError):
+        m.roll(1)
+    with pytest.raises(NotImplementedError):
+        m.roll(0)
+
+
+def test_abi_arity_is_checked():
+    """Every call through the Python face is checked against the header's parameter list."""
+    with pytest.raises(TypeError):
+        gf._call("gf_mesh_quad4_finelayer_roll", 3, 3, 1.0, 1, 1)
+    with pytest.raises(TypeError):
+        gf._call("gf_mesh_quad4_finelayer_roll", 3, 3, 1.0, 1, 1, None, None)
+    with pytest.raises(ctypes.ArgumentError):
+        gf._call("gf_mesh_dofs", 4.5, 2, None)  # a float where the header has int64_t
