@@ -111,6 +111,58 @@ def test_tyings_vs_oracle(gf, nx, ny):
             assert np.array_equal(x, y), name
 
 
+@pytest.mark.parametrize("nx,ny,nz", [(4, 3, 5), (3, 3, 3)])
+def test_tyings_vs_oracle_hex8(gf, nx, ny, nz):
+    """Tyings::Control + Tyings::Periodic on a 3-D mesh (TyingsPeriodic.h:82-121, 238-343,
+    361-392): reordered dofs, control dofs and nodes, sizes and the Cdi / Cdu / Cdp CSR arrays
+    equal the oracle's, entry for entry."""
+    m = gf.Mesh.Hex8.Regular(nx, ny, nz)
+    ctrl = gf.Tyings.Control(m.coor, m.dofs)
+    co, do, cd, cn = oracle.tyings_control(m.coor, m.dofs)
+    assert np.array_equal(ctrl.coor, co) and np.array_equal(ctrl.dofs, do)
+    assert np.array_equal(ctrl.controlDofs, cd) and np.array_equal(ctrl.controlNodes, cn)
+    ties = m.nodesPeriodic
+    assert np.array_equal(ties, oracle.hex8_nodes_periodic(nx, ny, nz))
+    iip = np.concatenate((cd.ravel(), do[m.nodesOrigin]))
+    ty = gf.Tyings.Periodic(ctrl.coor, ctrl.dofs, ctrl.controlDofs, ties, iip)
+    ref = oracle.tyings_periodic(co, do, cd, ties, iip)
+    assert np.array_equal(ty.dofs, ref["dofs"]) and np.array_equal(ty.control, ref["control"])
+    assert (ty.nnu, ty.nnp, ty.nni, ty.nnd) == (ref["nnu"], ref["nnp"], ref["nni"], ref["nnd"])
+    assert ty.nnd == 3 * len(ties) and ty.nnp == 12
+    for name in ("Cdi", "Cdu", "Cdp"):
+        a, b = getattr(ty, name + "_csr"), ref[name + "_csr"]
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), name
+
+
+def test_tyings_hex8_affine_closed_form(gf):
+    """u_d = u_i + (x_d - x_i) . ctrl (TyingsPeriodic.h:244-256) in 3-D, on the product's own Cdi:
+    an affine field u = F x with the control node j holding row j of F is reproduced exactly on
+    every dependent DOF, and Cdi = [Cdu | Cdp]."""
+    nx, ny, nz = 4, 3, 5
+    m = gf.Mesh.Hex8.Regular(nx, ny, nz)
+    ctrl = gf.Tyings.Control(m.coor, m.dofs)
+    iip = np.concatenate((ctrl.controlDofs.ravel(), ctrl.dofs[m.nodesOrigin]))
+    ty = gf.Tyings.Periodic(ctrl.coor, ctrl.dofs, ctrl.controlDofs, m.nodesPeriodic, iip)
+    F = np.array([[0.01, 0.03, -0.007], [-0.02, 0.005, 0.011], [0.004, -0.013, 0.021]])
+    co = ctrl.coor
+    nnode = m.coor.shape[0]
+    u = np.zeros((co.shape[0], 3))
+    u[:nnode] = m.coor @ F.T
+    u[nnode:] = F
+    d = ty.dofs
+    ui = np.zeros(ty.nni)
+    ud = np.zeros(ty.nnd)
+    indep = d < ty.nni
+    ui[d[indep]] = u[indep]
+    ud[d[~indep] - ty.nni] = u[~indep]
+    Cdi, Cdu, Cdp = ty.Cdi, ty.Cdu, ty.Cdp
+    assert np.abs(Cdi @ ui - ud).max() < 1e-14
+    assert np.array_equal(Cdi, np.hstack((Cdu, Cdp)))
+    # a sign, index or transposition error in the dependent rows breaks the affine reproduction
+    assert np.abs(Cdi @ (ui * (1 + (np.arange(ty.nni) % 2))) - ud).max() > 1e-3
+
+
 def test_randn_matches_specification(gf):
     assert np.array_equal(gf.randn(3, 1001, 0.5, 2.0), oracle.randn(3, 1001, 0.5, 2.0))
 

@@ -350,6 +350,34 @@ def test_tyings_structure_and_closed_form():
     assert np.allclose(ty["Cdi"], np.hstack((ty["Cdu"], ty["Cdp"])))
 
 
+def test_tyings_structure_and_closed_form_hex8():
+    """The 3-D restatement: sizes, a permutation of all DOFs, and u_d = u_i + (x_d - x_i) . ctrl
+    for an affine field (TyingsPeriodic.h:244-256)."""
+    nx, ny, nz = 4, 3, 5
+    coor, conn = oracle.hex8_regular(nx, ny, nz)
+    dofs = oracle.dofs(coor.shape[0], 3)
+    co, do, cdofs, cnodes = oracle.tyings_control(coor, dofs)
+    iip = np.concatenate((cdofs.ravel(), do[0]))
+    ties = oracle.hex8_nodes_periodic(nx, ny, nz)
+    ty = oracle.tyings_periodic(co, do, cdofs, ties, iip)
+    assert len(ties) == (nx + 1) * (ny + 1) * (nz + 1) - nx * ny * nz
+    assert ty["nnd"] == 3 * len(ties) and ty["nnp"] == 12
+    assert ty["nni"] == 3 * (nx * ny * nz + 3)
+    assert np.array_equal(np.sort(ty["dofs"].ravel()), np.arange(ty["nni"] + ty["nnd"]))
+    F = np.array([[0.01, 0.03, -0.007], [-0.02, 0.005, 0.011], [0.004, -0.013, 0.021]])
+    u = np.zeros((co.shape[0], 3))
+    u[: coor.shape[0]] = coor @ F.T
+    u[coor.shape[0]:] = F
+    d = ty["dofs"]
+    indep = d < ty["nni"]
+    ui = np.zeros(ty["nni"])
+    ud = np.zeros(ty["nnd"])
+    ui[d[indep]] = u[indep]
+    ud[d[~indep] - ty["nni"]] = u[~indep]
+    assert np.abs(ty["Cdi"] @ ui - ud).max() < 1e-14
+    assert np.allclose(ty["Cdi"], np.hstack((ty["Cdu"], ty["Cdp"])))
+
+
 def test_Kprime_cg_vs_scipy_direct():
     scipy = pytest.importorskip("scipy.sparse.linalg")
     nx, ny = 6, 5
