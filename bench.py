@@ -3,10 +3,16 @@ per-element log-normal shear modulus (sigma 0.3), K = 10, lumped mass (density 1
 fixed, top edge driven at x-velocity 1e-4, dt = 0.01.  One step = one velocity-Verlet step over the
 whole mesh (fused kernel).  Metric: DOF-updates/s = ndof * steps / wall time.
 
-    python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--nx 2048] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--nx 2048] [--no-cpu] [--no-blocks]
 
 Prints one JSON line.  Inputs are synthetic (SplitMix64/Box-Muller generator, seed 2), resident
-in HBM before the timed region.
+in HBM before the timed region.  Beside the headline (FP32 element body) the line carries the FP64
+reference-order and FP64-fast legs, their errors over 1000 steps, the roofline against the datasheet
+peak and against a measured copy ceiling, the general-mesh path, the one-thread and all-cores CPU
+baselines, and the metric's second half: learned vs plain PCG on the periodic Quad4 512^2 cell
+(pcg_learned) and the Hex8 128^3 element stiffness on MFMA (hex8_stiffness).  Other workloads:
+--workload hex8-stiffness | pcg-periodic | pcg-learned | plastic-refresh | newton-q4 | statics-q4 |
+periodic-j2.
 """
 
 from __future__ import annotations
@@ -126,11 +132,19 @@ def _timed(fn, reps):
 def bench_hex8(args):
     """Hex8 Regular n^3, per-point isotropic tangent (K = 10, G ~ U[0.5, 1.5]): element stiffness in
     FP64 vs BF16x3 MFMA, CSR assembly, assembled Dot vs matrix-free sum_e K_e u_e."""
+    print(json.dumps(hex8_stiffness(args.hex_n, cpu=not args.no_cpu, csr=True)))
+
+
+# one v_mfma_f32_32x32x16_bf16 keeps its SIMD's matrix core busy 32 cycles; 15 per element
+MFMA_PER_ELEMENT, MFMA_CYCLES, SIMDS, CLOCK_HZ = 15, 32, 1024, 2.4e9
+
+
+def hex8_stiffness(n, cpu=True, csr=True, cpu_seconds=10.0):
+    """The Hex8 element-stiffness workload as a dict (see bench_hex8)."""
     import torch
 
     import goosefem_amd as gf
 
-    n = args.hex_n
     t0 = time.perf_counter()
     mesh = gf.Mesh.Hex8.Regular(n, n, n)
     coor, conn, dofs = mesh.coor, mesh.conn, mesh.dofs
@@ -156,25 +170,6 @@ def bench_hex8(args):
     err_ke = float(torch.linalg.vector_norm(kbf - k64) / torch.linalg.vector_norm(k64))
     err_ke32 = float(torch.linalg.vector_norm(k32.double() - k64) / torch.linalg.vector_norm(k64))
     del C, C32
-    t0 = time.perf_counter()
-    A = gf.Matrix(vec, None)
-    t_pattern = time.perf_counter() - t0
-    t_asm = _timed(lambda: A.assemble(k64), 3)
-    u = torch.from_numpy(gf.randn(4, vec.ndof)).cuda()
-    Ku = torch.empty_like(u)
-    t_dot = _timed(lambda: A.Dot(u, out=Ku), 10)
-
-    def matrix_free(K):
-        return vec.AssembleDofs(gf.elemmat_dot(K, vec.AsElement(u)))
-
-    t_mf = _timed(lambda: matrix_free(k64), 3)
-    err_mf = float(torch.linalg.vector_norm(matrix_free(k64) - Ku) / torch.linalg.vector_norm(Ku))
-    err_ku_bf = float(torch.linalg.vector_norm(matrix_free(kbf) - Ku) / torch.linalg.vector_norm(Ku))
-    # the same products from the FP32 element matrices (FP64 sums)
-    t_mf32 = _timed(lambda: matrix_free(k32), 3)
-    err_ku32 = float(torch.linalg.vector_norm(matrix_free(k32) - Ku) / torch.linalg.vector_norm(Ku))
-    t_asm32 = _timed(lambda: A.assemble(k32), 3)
-    A.assemble(k64)
     bytes_el = 11392  # FP64 API bytes per element: C 5184 + GradN 1536 + dV 64 + K_e 4608
     flops_el = 69120  # Voigt B^T (D dV) B, 8 points
     ach = bytes_el * E / tbf / 1e9
@@ -182,40 +177,65 @@ def bench_hex8(args):
         "metric": "Hex8 element stiffness (BF16x3 MFMA) throughput",
         "value": E / tbf,
         "unit": "elements/s",
-        "config": {"workload": f"Hex8 Regular {n}^3 element stiffness + CSR assembly + K.u", "nelem": E,
-                   "ndof": vec.ndof, "nnz": A.nnz},
+        "config": {"workload": f"Hex8 Regular {n}^3 element stiffness" + (" + CSR assembly + K.u" if csr else ""),
+                   "nelem": E, "ndof": vec.ndof},
         "dtype": "bf16x3 (hi/lo split) mfma, fp32 accumulate; fp64 inputs/outputs",
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS,
                      "traffic": load_traffic(n, "hex8_stiffness_128_traffic.json", "n"), "kernel": "k_stiffness_bf16x3",
                      "algorithmic_bytes_per_launch": bytes_el * E, "avg_launch_us": tbf * 1e6,
                      "tflops_voigt": flops_el * E / tbf / 1e12},
+        # matrix cores busy: MFMA issue cycles of the launch over the chip's SIMD cycles in its duration
+        "mfma_busy_est": MFMA_PER_ELEMENT * MFMA_CYCLES * E / (SIMDS * CLOCK_HZ * tbf),
         "fp64_stiffness": {"avg_launch_us": t64 * 1e6, "achieved_GBs": bytes_el * E / t64 / 1e9},
         # FP32 tangent in, FP32 K_e out (the kernel's own operand and accumulator precision):
         # C 2592 + GradN 1536 + dV 64 + K_e 2304 = 6496 B per element
         "fp32_io": {"avg_launch_us": t32 * 1e6, "elements_per_s": E / t32,
                     "traffic": load_traffic(n, "hex8_stiffness_128_traffic.json", "n", "traffic_bytes_per_launch_f32io"),
                     "achieved_GBs": 6496 * E / t32 / 1e9, "frac": 6496 * E / t32 / 1e9 / HBM_PEAK_GBS,
-                    "Ke_rel_vs_fp64": err_ke32, "assemble_ms": t_asm32 * 1e3,
-                    "assemble_GBs": (E * 2304 + A.nnz * 8) / t_asm32 / 1e9, "matrix_free_ms": t_mf32 * 1e3,
-                    "Ku_rel_vs_fp64": err_ku32},
+                    "mfma_busy_est": MFMA_PER_ELEMENT * MFMA_CYCLES * E / (SIMDS * CLOCK_HZ * t32),
+                    "Ke_rel_vs_fp64": err_ke32},
         "tangent_ms": {"f64": t_tan * 1e3, "f32": t_tan32 * 1e3,
                        "f64_GBs": E * 8 * 81 * 8 / t_tan / 1e9, "f32_GBs": E * 8 * 81 * 4 / t_tan32 / 1e9},
-        "error_vs_fp64": {"Ke_rel": err_ke, "Ku_rel": err_ku_bf},
-        "csr": {"pattern_build_s": t_pattern, "assemble_ms": t_asm * 1e3,
-                "assemble_GBs": (E * 4608 + A.nnz * 8) / t_asm / 1e9,
-                "dot_ms": t_dot * 1e3, "dot_GBs": (A.nnz * 12 + vec.ndof * 16) / t_dot / 1e9,
-                "matrix_free_ms": t_mf * 1e3, "matrix_free_vs_dot_rel": err_mf},
+        "error_vs_fp64": {"Ke_rel": err_ke},
         "setup_s": t_setup,
     }
-    if not args.no_cpu:
+    if csr:
+        t0 = time.perf_counter()
+        A = gf.Matrix(vec, None)
+        t_pattern = time.perf_counter() - t0
+        t_asm = _timed(lambda: A.assemble(k64), 3)
+        u = torch.from_numpy(gf.randn(4, vec.ndof)).cuda()
+        Ku = torch.empty_like(u)
+        t_dot = _timed(lambda: A.Dot(u, out=Ku), 10)
+
+        def matrix_free(K):
+            return vec.AssembleDofs(gf.elemmat_dot(K, vec.AsElement(u)))
+
+        t_mf = _timed(lambda: matrix_free(k64), 3)
+        err_mf = float(torch.linalg.vector_norm(matrix_free(k64) - Ku) / torch.linalg.vector_norm(Ku))
+        err_ku_bf = float(torch.linalg.vector_norm(matrix_free(kbf) - Ku) / torch.linalg.vector_norm(Ku))
+        # the same products from the FP32 element matrices (FP64 sums)
+        t_mf32 = _timed(lambda: matrix_free(k32), 3)
+        err_ku32 = float(torch.linalg.vector_norm(matrix_free(k32) - Ku) / torch.linalg.vector_norm(Ku))
+        t_asm32 = _timed(lambda: A.assemble(k32), 3)
+        A.assemble(k64)
+        res["config"]["nnz"] = A.nnz
+        res["error_vs_fp64"]["Ku_rel"] = err_ku_bf
+        res["fp32_io"].update({"assemble_ms": t_asm32 * 1e3, "assemble_GBs": (E * 2304 + A.nnz * 8) / t_asm32 / 1e9,
+                               "matrix_free_ms": t_mf32 * 1e3, "Ku_rel_vs_fp64": err_ku32})
+        res["csr"] = {"pattern_build_s": t_pattern, "assemble_ms": t_asm * 1e3,
+                      "assemble_GBs": (E * 4608 + A.nnz * 8) / t_asm / 1e9,
+                      "dot_ms": t_dot * 1e3, "dot_GBs": (A.nnz * 12 + vec.ndof * 16) / t_dot / 1e9,
+                      "matrix_free_ms": t_mf * 1e3, "matrix_free_vs_dot_rel": err_mf}
+    if cpu:
         import oracle
 
-        # bounded CPU sample: chunks of 50,000 elements of the same mesh until about 10 s of work
+        # bounded CPU sample: chunks of 50,000 elements of the same mesh until about cpu_seconds of work
         m, dt, e0 = 50000, 0.0, 0
         Gs = 0.5 + np.random.default_rng(0).random((m, 8))
         Cs = oracle.elastic_tangent(np.full((m, 8), 10.0), Gs)
-        while dt < 10.0 and e0 + m <= conn.shape[0]:
+        while dt < cpu_seconds and e0 + m <= conn.shape[0]:
             ov = oracle.Vector(conn[e0:e0 + m], oracle.dofs(coor.shape[0], 3))
             oq = oracle.Quadrature(oracle.HEX8, ov.AsElement(coor))
             t0 = time.perf_counter()
@@ -225,7 +245,7 @@ def bench_hex8(args):
         res["cpu_baseline"] = {"value": e0 / dt, "unit": "elements/s", "cores": 1, "kind": "port",
                                "sample": f"{e0} elements of the same mesh, FP64 oracle reference loop, one thread, "
                                          f"{dt:.1f} s"}
-    print(json.dumps(res))
+    return res
 
 
 def periodic_setup(n, hex8=False, seed=4, sigma=0.4, spheres=False):
@@ -333,17 +353,23 @@ def bench_pcg_learned(args):
     """Wall time of the FP64 Jacobi-PCG to a true relative residual of 1e-10 on the periodic cell
     (Quad4 512^2 simple shear, or Hex8 96^3 with stiff spheres under pure shear) from the learned
     warm start (network inference included) and from a zero guess, on modulus fields the network
-    was not trained on."""
+    was not trained on; with --increments N also the quasi-static loop of N load increments."""
+    if not os.path.exists(args.ws_weights):
+        print(json.dumps({"error": f"no trained weights at {args.ws_weights}; run "
+                                   "python -m goosefem_amd.warmstart --out <file> first"}))
+        sys.exit(2)
+    print(json.dumps(pcg_learned(args.ws_weights, args.ws_tests, cpu=not args.no_cpu, increments=args.increments)))
+
+
+def pcg_learned(weights, ntests, cpu=False, increments=0, cpu_seconds=10.0):
+    """Learned warm start vs plain PCG (see bench_pcg_learned) as a dict."""
     import torch
 
     import goosefem_amd as gf
     from goosefem_amd import warmstart as ws
 
-    if not os.path.exists(args.ws_weights):
-        print(json.dumps({"error": f"no trained weights at {args.ws_weights}; run "
-                                   "python -m goosefem_amd.warmstart --out <file> first"}))
-        sys.exit(2)
-    net = ws.WarmStartNet.load(args.ws_weights)
+    t_setup = time.perf_counter()
+    net = ws.WarmStartNet.load(weights)
     n = int(net.meta["n"])
     dim = int(net.meta.get("dim", 2))
     shear = str(net.meta.get("shear", "simple"))
@@ -353,6 +379,7 @@ def bench_pcg_learned(args):
     gamma = 5e-4 if dim == 3 else 1e-3
     Kd = gf.to_device(pb.K)
     zero_grid = torch.zeros(W.nv * dim, dtype=torch.float32, device="cuda")
+    t_setup = time.perf_counter() - t_setup
 
     def affine_guess():
         x = torch.zeros(pb.ty.nnu, dtype=torch.float64, device="cuda")
@@ -360,32 +387,30 @@ def bench_pcg_learned(args):
                  gf._dp(W.affine), gf._f64(1.0), gf._f64(gamma), gf._dp(x), gf._stream())
         return x
 
+    def clock(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
     rows = []
-    for k in range(args.ws_tests + 1):
+    for k in range(ntests + 1):
         seed = 900000 + k  # training fields use seeds 1000 + i
         g = ws.sphere_field(n, seed, pb.sigma) if dim == 3 else np.exp(pb.sigma * gf.randn(seed, n * n))
         G = gf.to_device(g)
         P.set_moduli(Kd, G)
         b = W.rhs(gamma)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, it0, res0 = P.solve(b, tol=1e-10)
-        torch.cuda.synchronize()
-        t_plain = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        xg = W.predict(G, gamma)
-        torch.cuda.synchronize()
-        t_inf = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        xg = W.predict(G, gamma)
-        _, it1, res1 = P.solve(b, x0=xg, tol=1e-10)
-        torch.cuda.synchronize()
-        t_learned = time.perf_counter() - t0
+        t_plain, (_, it0, res0) = clock(lambda: P.solve(b, tol=1e-10))
+        t_inf, _ = clock(lambda: W.predict(G, gamma))
+        t_learned, (_, it1, res1) = clock(lambda: P.solve(b, x0=W.predict(G, gamma), tol=1e-10))
+        start = P.start_used
         _, it2, _ = P.solve(b, x0=affine_guess(), tol=1e-10)
         if k == 0:
             continue  # first field: warm-up of kernels and buffers
         rows.append(dict(seed=seed, plain_s=t_plain, learned_s=t_learned, inference_s=t_inf, plain_iterations=it0,
-                         learned_iterations=it1, affine_only_iterations=it2, plain_relres=res0, learned_relres=res1))
+                         learned_iterations=it1, affine_only_iterations=it2, plain_relres=res0, learned_relres=res1,
+                         learned_start_kept=start == gf.GF_START_GIVEN))
     mean = lambda key: float(np.mean([r[key] for r in rows]))  # noqa: E731
     m = net.meta
     if dim == 3:
@@ -396,7 +421,8 @@ def bench_pcg_learned(args):
     res = {
         "metric": "PCG wall time to true relative residual 1e-10, learned warm start (inference included)",
         "value": mean("learned_s"), "unit": "s", "higher_is_better": False, "n_gpus": 1,
-        "plain_s": mean("plain_s"), "speedup_vs_plain": mean("plain_s") / mean("learned_s"),
+        "plain_s": mean("plain_s"), "learned_s": mean("learned_s"),
+        "speedup_vs_plain": mean("plain_s") / mean("learned_s"),
         "iterations": {"plain": mean("plain_iterations"), "learned": mean("learned_iterations"),
                        "affine_only": mean("affine_only_iterations")},
         "inference_ms": mean("inference_s") * 1e3,
@@ -409,9 +435,45 @@ def bench_pcg_learned(args):
                      "data_generation_s": float(m["data_seconds"]), "training_s": float(m["train_seconds"]),
                      "final_mse": float(m["final_mse"]),
                      "holdout_rel_error": float(m.get("holdout_rel_error", float("nan")))},
+        "setup_s": t_setup,
         "per_field": rows,
     }
-    print(json.dumps(res))
+    if increments > 0:
+        # the quasi-static loop of the reference's periodic example (periodic_elastoplastic.py:89-146)
+        # for a linear material: every increment solves K' du_u = -K'_up du_p for the same du_p, so
+        # the previous increment's du_u is an exact start; four start strategies, each to 1e-10
+        g = np.exp(pb.sigma * gf.randn(900001, n * n)) if dim == 2 else ws.sphere_field(n, 900001, pb.sigma)
+        G = gf.to_device(g)
+        P.set_moduli(Kd, G)
+        b = W.rhs(gamma)
+        loops = {}
+        for name in ("plain_zero", "plain_previous", "learned", "learned_then_previous"):
+            du = None
+            its = []
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for inc in range(increments):
+                if name == "plain_zero" or (name == "plain_previous" and inc == 0):
+                    x0 = None
+                elif name == "learned" or inc == 0:
+                    x0 = W.predict(G, gamma)
+                else:
+                    x0 = du
+                du, it, r = P.solve(b, x0=x0, tol=1e-10)
+                its.append(it)
+            torch.cuda.synchronize()
+            loops[name] = {"s": time.perf_counter() - t0, "iterations_total": int(np.sum(its)),
+                           "iterations_first": int(its[0]), "iterations_rest_mean": float(np.mean(its[1:]))
+                           if len(its) > 1 else 0.0}
+        res["quasi_static"] = {"increments": increments, "increment": gamma, "field_seed": 900001, **loops,
+                               "speedup_learned_vs_plain_zero": loops["plain_zero"]["s"] / loops["learned"]["s"],
+                               "speedup_learned_then_previous_vs_plain_previous":
+                                   loops["plain_previous"]["s"] / loops["learned_then_previous"]["s"]}
+    if cpu:
+        S = dict(ty=pb.ty, conn=pb.mesh.conn, coor=pb.mesh.coor, K=pb.K, G=np.asarray(g, dtype=np.float64),
+                 mesh=pb.mesh)
+        res["cpu_baseline"] = pcg_cpu_baseline(S, W.rhs(gamma), target_s=cpu_seconds)
+    return res
 
 
 def bench_plastic(args):
@@ -696,8 +758,11 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=2048)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-blocks", action="store_true",
+                    help="skip the learned-PCG (512^2 periodic) and Hex8 stiffness (128^3) blocks")
+    ap.add_argument("--no-general", action="store_true", help="skip the general-mesh explicit row")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned",
                                            "plastic-refresh", "newton-q4", "statics-q4", "periodic-j2"],
@@ -714,6 +779,8 @@ def main():
     ap.add_argument("--ws-weights", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "goosefem_amd",
                                                          "data", "warmstart_quad4_512.npz"))
     ap.add_argument("--ws-tests", type=int, default=4, help="held-out modulus fields timed")
+    ap.add_argument("--increments", type=int, default=0,
+                    help="pcg-learned: also run the quasi-static loop of this many load increments")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC counter calibration) and exit")
     args = ap.parse_args()
@@ -762,34 +829,51 @@ def main():
     gf, X = S["gf"], S["X"]
     prec = gf.F32 if args.precision == "f32" else gf.F64
     ndof = S["vec"].ndof
+    t_all = time.perf_counter()
 
-    # FP64 reference trajectory of the same length (error of the FP32 element kernels vs FP64)
-    X.step(args.steps, gf.F64, store_fint=True)
-    torch.cuda.synchronize()
-    u64 = X.state()[0].cpu().numpy()
-    f64 = X.fint().cpu().numpy()
-    X.set_state(S["u0"], S["v0"], S["a0"])
-    X.step(args.steps, gf.F32, store_fint=True)
-    torch.cuda.synchronize()
-    u32 = X.state()[0].cpu().numpy()
-    f32 = X.fint().cpu().numpy()
-    err_u = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
-    err_f = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
+    # errors of the FP32 and FP64-fast element bodies against the FP64 reference-order path over the
+    # workload's 1000 steps (and over --steps when that differs)
+    err = {}
+    for nst in sorted({1000, args.steps}):
+        traj = {}
+        for p_ in (gf.F64, gf.F32, gf.F64_FAST):
+            X.set_state(S["u0"], S["v0"], S["a0"])
+            X.step(nst, p_, store_fint=True)
+            torch.cuda.synchronize()
+            traj[p_] = (X.state()[0].cpu().numpy(), X.fint().cpu().numpy())
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+        u64, f64 = traj[gf.F64]
+        err[str(nst)] = {"f32": {"u_rel": rel(traj[gf.F32][0], u64), "fint_rel": rel(traj[gf.F32][1], f64)},
+                         "f64_fast": {"u_rel": rel(traj[gf.F64_FAST][0], u64),
+                                      "fint_rel": rel(traj[gf.F64_FAST][1], f64)}}
+    main_err = err["1000"]["f32" if prec == gf.F32 else "f64_fast"]
 
-    # timed run: warmup, then exactly `steps` steps
-    X.set_state(S["u0"], S["v0"], S["a0"])
-    X.step(args.warmup, prec)
-    torch.cuda.synchronize()
-    wall, ktime = time_steps(X, args.steps, prec, gf)
-    other = gf.F64 if prec == gf.F32 else gf.F32
-    X.set_state(S["u0"], S["v0"], S["a0"])
-    X.step(args.warmup, other)
-    wall_o, ktime_o = time_steps(X, args.steps, other, gf)
+    # timed runs: warmup, then exactly `steps` steps, for each precision
+    timing = {}
+    for p_ in (prec, gf.F64 if prec == gf.F32 else gf.F32, gf.F64_FAST):
+        X.set_state(S["u0"], S["v0"], S["a0"])
+        X.step(args.warmup, p_)
+        torch.cuda.synchronize()
+        timing[p_] = time_steps(X, args.steps, p_, gf)
+    wall, ktime = timing[prec]
 
     B = canonical_bytes_per_step(nx, ny)
     per_launch = ktime / args.steps
     achieved = B / per_launch / 1e9
     value = ndof * args.steps / wall
+    traffic = load_traffic(nx)
+    ceiling = copy_ceiling(gf)
+    name = {gf.F64: "f64", gf.F32: "f32", gf.F64_FAST: "f64_fast"}
+    legs = {}
+    for p_, (w_, k_) in timing.items():
+        if p_ == prec:
+            continue
+        legs[name[p_]] = {"value": ndof * args.steps / w_, "avg_launch_us": k_ / args.steps * 1e6,
+                          "achieved_GBs": B / (k_ / args.steps) / 1e9,
+                          "frac": B / (k_ / args.steps) / 1e9 / HBM_PEAK_GBS,
+                          "body": {"f64": "reference expression order, bit-exact with the oracle",
+                                   "f32": "FP32 element body, FP64 assembly and state",
+                                   "f64_fast": "FP64 closed-form rectangle body with FMA (1e-14 of f64)"}[name[p_]]}
     res = {
         "metric": "DOF-updates/s (Quad4 explicit velocity-Verlet)",
         "value": value,
@@ -806,21 +890,114 @@ def main():
         "config": {"workload": f"Quad4 Regular {nx}x{ny} plane strain, 2x2 Gauss, velocity-Verlet dt=0.01",
                    "ndof": ndof, "nelem": nx * ny, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(nx),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_vv_structured", "algorithmic_bytes_per_launch": B,
-                     "avg_launch_us": per_launch * 1e6},
-        "other_precision": {"dtype": "f64" if prec == gf.F32 else "f32",
-                            "value": ndof * args.steps / wall_o,
-                            "avg_launch_us": ktime_o / args.steps * 1e6,
-                            "achieved_GBs": B / (ktime_o / args.steps) / 1e9},
-        "error_vs_fp64": {"u_rel": err_u, "fint_rel": err_f, "steps": args.steps},
+                     "avg_launch_us": per_launch * 1e6,
+                     # the bytes the kernel moves (counter traffic, which is below B: Regular coordinates
+                     # are generated, a uniform K is a scalar, no connectivity is read) over its time
+                     "moved_GBs": traffic / per_launch / 1e9 if traffic else None,
+                     "frac_moved": traffic / per_launch / 1e9 / HBM_PEAK_GBS if traffic else None,
+                     "copy_ceiling_GBs": ceiling,
+                     "frac_of_copy_ceiling": achieved / ceiling,
+                     "frac_moved_of_copy_ceiling": traffic / per_launch / 1e9 / ceiling if traffic else None},
+        "other_precision": legs,
+        "error_vs_fp64": {"u_rel": main_err["u_rel"], "fint_rel": main_err["fint_rel"], "steps": 1000,
+                          "by_steps": err},
     }
+    if not args.no_general:
+        res["general_mesh"] = explicit_general(nx, ny, args.steps, args.warmup)
     if not args.no_cpu:
         v_cpu, t_cpu = cpu_baseline(S, nx, ny, args.cpu_steps)
         res["cpu_baseline"] = {"value": v_cpu, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
                                "sample": f"{args.cpu_steps} velocity-Verlet steps of the same {nx}x{ny} "
                                          f"workload, FP64 oracle, one thread, {t_cpu:.1f} s"}
+        v_par, t_par, nthr = cpu_baseline_parallel(S, args.cpu_steps * 2)
+        res["cpu_baseline_all_cores"] = {"value": v_par, "unit": "DOF-updates/s", "cores": nthr, "kind": "port",
+                                         "host_cpus": os.cpu_count(),
+                                         "sample": f"{args.cpu_steps * 2} velocity-Verlet steps of the same "
+                                                   f"{nx}x{ny} workload, FP64 oracle on {nthr} OpenMP threads "
+                                                   f"(bit-identical to the one-thread run), {t_par:.1f} s"}
+    del S, X
+    torch.cuda.empty_cache()
+    if not args.no_blocks:
+        # the metric's second half (configs 4 and 3): learned vs plain PCG and the Hex8 stiffness
+        if os.path.exists(args.ws_weights):
+            pl = pcg_learned(args.ws_weights, 2, cpu=not args.no_cpu, cpu_seconds=5.0)
+            res["pcg_learned"] = {k: pl[k] for k in ("plain_s", "learned_s", "speedup_vs_plain", "iterations",
+                                                     "inference_ms", "max_true_relres", "config", "per_field")}
+            res["pcg_learned"]["unit"] = "s to true relative residual 1e-10 (inference included)"
+            if "cpu_baseline" in pl:
+                res["pcg_learned"]["cpu_baseline"] = pl["cpu_baseline"]
+            torch.cuda.empty_cache()
+        hx = hex8_stiffness(args.hex_n, cpu=not args.no_cpu, csr=False, cpu_seconds=5.0)
+        res["hex8_stiffness"] = {k: hx[k] for k in ("value", "unit", "config", "roofline", "mfma_busy_est",
+                                                    "fp64_stiffness", "fp32_io", "tangent_ms", "error_vs_fp64")}
+        if "cpu_baseline" in hx:
+            res["hex8_stiffness"]["cpu_baseline"] = hx["cpu_baseline"]
+    res["bench_wall_s"] = time.perf_counter() - t_all
     print(json.dumps(res))
+
+
+def copy_ceiling(gf, gib=4):
+    """Measured copy ceiling (GB/s, read + write bytes): the best of the plain and the non-temporal
+    variant of the library's streaming copy kernel over a 4 GiB buffer."""
+    import torch
+
+    n = (gib << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    best = 0.0
+    for mode in (0, 1):
+        t = _timed(lambda: gf._call("gf_copy_stream", gf._dp(src), gf._dp(dst), n, mode, gf._stream()), 10)
+        best = max(best, 2 * 8 * n / t / 1e9)
+    del src, dst
+    return best
+
+
+def explicit_general(nx, ny, steps, warmup):
+    """The general-mesh explicit path (displacement update, element kernel with geometry recomputed
+    from the coordinates, ordered per-DOF gather + update) on the same Regular mesh with its elements
+    renumbered (which defeats the structured-grid detection), FP32 and FP64 element bodies."""
+    import torch
+
+    import goosefem_amd as gf
+
+    mesh = gf.Mesh.Quad4.Regular(nx, ny)
+    coor, dofs = mesh.coor, mesh.dofs
+    conn = mesh.conn[np.random.default_rng(5).permutation(mesh.nelem)]
+    vec = gf.Vector(conn, dofs)
+    inv = np.ones(vec.ndof)
+    G = np.exp(0.3 * gf.randn(2, mesh.nelem)).astype(np.float32)
+    K = np.full(mesh.nelem, 10.0, dtype=np.float32)
+    X = gf.ExplicitQuad4(vec, coor, K, G, inv, 0.01)
+    assert not X.structured
+    u0 = 1e-3 * gf.randn(1, vec.ndof)
+    B = canonical_bytes_per_step(nx, ny)
+    out = {"workload": f"Quad4 Regular {nx}x{ny}, elements renumbered (random permutation), general path",
+           "algorithmic_bytes_per_step": B}
+    for p_, nm in ((gf.F32, "f32"), (gf.F64, "f64"), (gf.F64_FAST, "f64_fast")):
+        X.set_state(u0, np.zeros(vec.ndof), np.zeros(vec.ndof))
+        X.step(warmup, p_)
+        wall, kt = time_steps(X, steps, p_, gf)
+        out[nm] = {"ms_per_step": kt / steps * 1e3, "DOF_updates_per_s": vec.ndof * steps / wall,
+                   "achieved_GBs": B / (kt / steps) / 1e9, "frac": B / (kt / steps) / 1e9 / HBM_PEAK_GBS}
+    del X, vec
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline_parallel(S, nsteps):
+    """The oracle's FP64 restatement on all host threads (OpenMP), same workload."""
+    import oracle
+
+    ov = oracle.Vector(S["conn"], S["dofs"])
+    q = oracle.Quadrature(oracle.QUAD4_PLANAR, ov.AsElement(S["coor"]))
+    u, v, a = S["u0"].copy(), S["v0"].copy(), S["a0"].copy()
+    t0 = time.perf_counter()
+    _, nthr = oracle.vv_steps_parallel(nsteps, oracle.QUAD4_PLANAR, S["conn"], S["dofs"], q.dNx, q.dV(),
+                                       S["K"].astype(np.float64), S["G"].astype(np.float64), S["inv"], 0.01, u, v, a)
+    dt = time.perf_counter() - t0
+    return ov.ndof * nsteps / dt, dt, nthr
 
 
 if __name__ == "__main__":

@@ -26,7 +26,8 @@ LIB_PATH = os.path.join(_HERE, "libgoosefem_amd.so")
 GF_OK, GF_EINVAL, GF_EHIP, GF_ENOMEM, GF_ESTATE, GF_ENOTCONV = 0, -1, -2, -3, -4, -5
 QUAD4, QUAD4_PLANAR, HEX8 = 0, 1, 2
 GAUSS, NODAL = 0, 1
-F64, F32, BF16X3 = 0, 1, 2
+F64, F32, BF16X3, F64_FAST = 0, 1, 2, 3
+GF_START_GIVEN, GF_START_PREVIOUS, GF_START_ZERO = 0, 1, 2
 
 _lib = None
 
@@ -2127,3 +2128,12 @@ class TyingsPCG(_Handle):
         if rc != GF_OK and (raise_on_fail or rc != GF_ENOTCONV):
             _check(rc)
         return x, it.value, res.value
+
+    @property
+    def start_used(self):
+        """Start of the last solve: GF_START_GIVEN (the caller's x0), GF_START_PREVIOUS (the
+        previous converged solution, taken when x0's residual was not finite or above ||b||) or
+        GF_START_ZERO."""
+        v = _i32(0)
+        _call("gf_pcg_start_used", self._h, ctypes.byref(v))
+        return v.value

@@ -18,8 +18,8 @@
 // its own previous tiles loaded).
 //
 // General path (any Quad4 mesh / dof numbering): three kernels per step -- displacement update,
-// element kernel (gather + strain + stress + force, FP64 or FP32), DOF-centric ordered gather +
-// update.
+// element kernel (gather of coordinates and displacement, geometry recomputed, strain + stress +
+// force in FP64 reference order, FP32, or FP64 with FMA), DOF-centric ordered gather + update.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,11 +28,6 @@
 #include "gf_internal.hpp"
 
 using namespace gf;
-
-namespace gf {
-const double* quad_dNx(gf_quad q);
-const double* quad_vol(gf_quad q);
-} // namespace gf
 
 struct gf_explicit_s {
     gf_vector v = nullptr;
@@ -52,10 +47,9 @@ struct gf_explicit_s {
     int cur = 0;
     DBuf<double> fint;
     bool fint_valid = false;
-    // general path
-    gf_quad quad = nullptr;
+    // general path: element forces of the last step, [nelem, 4, 2]
     DBuf<double> fe64;
-    DBuf<float> fe32, dNx32, vol32;
+    DBuf<float> fe32;
 };
 
 namespace {
@@ -119,80 +113,86 @@ __device__ inline void element_force_f64(const ShapeTable& T, const double x[4][
     }
 }
 
-// ---- per-element force, FP32 --------------------------------------------------------------------
-// Same 2x2 Gauss quadrature of the bilinear element, evaluated through the bilinear structure of
-// the shape functions instead of per-node loops.  With node signs (s_m, t_m) = (-1,-1), (1,-1),
-// (1,1), (-1,1):  dN_m/dxi = s_m (1 + t_m eta) / 4,  dN_m/deta = t_m (1 + s_m xi) / 4, so for any
-// nodal field w:  sum_m dN_m/dxi w_m = (A_w + eta H_w) / 4,  sum_m dN_m/deta w_m = (B_w + xi H_w) / 4
-// with A_w = -w0 + w1 + w2 - w3, B_w = -w0 - w1 + w2 + w3, H_w = w0 - w1 + w2 - w3.  The nodal force
-// f(m,j) = sum_q dV_q sum_k dN_m/dxi_k (J^-T sigma)_kj collapses to three per-element sums.
-// Differences to node 0 are taken in FP64 before rounding to FP32 (no cancellation of large
-// coordinates or rigid displacements).  The Gauss weights are 1 (w[] kept for generality).
-__device__ inline void element_force_f32(const ShapeTable& T, const double x[4][2], const double u[4][2], float k,
-                                         float gmod, float f[4][2])
+// ---- element modes ---------------------------------------------------------------------------------
+// With node signs (s_m, t_m) = (-1,-1), (1,-1), (1,1), (-1,1) the bilinear shape functions give
+// dN_m/dxi = s_m (1 + t_m eta) / 4 and dN_m/deta = t_m (1 + s_m xi) / 4, so for a nodal field w
+//   sum_m dN_m/dxi w_m = (A_w + eta H_w) / 4,   sum_m dN_m/deta w_m = (B_w + xi H_w) / 4,
+// A_w = -w0 + w1 + w2 - w3, B_w = -w0 - w1 + w2 + w3, H_w = w0 - w1 + w2 - w3.  The modes (here
+// A / 4, B / 4, H / 4) are formed from the FP64 nodal values in FP64, as sums of two differences
+// (no cancellation of large coordinates or rigid displacements), and rounded once to the body's
+// arithmetic type T.  For the FP32 bodies this single rounding replaces four (differences, then
+// two FP32 sums); it is the dominant error term of the nodal forces, whose element
+// contributions cancel.
+template <class T>
+__device__ inline void element_modes(const double w[4][2], T A[2], T B[2], T H[2])
 {
-    float xr[4][2], ur[4][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        xr[0][i] = 0.0f;
-        ur[0][i] = 0.0f;
-#pragma unroll
-        for (int m = 1; m < 4; ++m) {
-            xr[m][i] = static_cast<float>(x[m][i] - x[0][i]);
-            ur[m][i] = static_cast<float>(u[m][i] - u[0][i]);
-        }
+        A[i] = static_cast<T>(0.25 * ((w[1][i] - w[0][i]) + (w[2][i] - w[3][i])));
+        B[i] = static_cast<T>(0.25 * ((w[2][i] - w[1][i]) + (w[3][i] - w[0][i])));
+        H[i] = static_cast<T>(0.25 * ((w[0][i] - w[1][i]) + (w[2][i] - w[3][i])));
     }
-    float Ax[2], Bx[2], Hx[2], Au[2], Bu[2], Hu[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        Ax[i] = 0.25f * (xr[1][i] + xr[2][i] - xr[3][i]);
-        Bx[i] = 0.25f * (xr[2][i] + xr[3][i] - xr[1][i]);
-        Hx[i] = 0.25f * (xr[2][i] - xr[1][i] - xr[3][i]);
-        Au[i] = 0.25f * (ur[1][i] + ur[2][i] - ur[3][i]);
-        Bu[i] = 0.25f * (ur[2][i] + ur[3][i] - ur[1][i]);
-        Hu[i] = 0.25f * (ur[2][i] - ur[1][i] - ur[3][i]);
-    }
-    const float g2 = 2.0f * gmod;
+}
+
+__device__ inline float fmaT(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ inline double fmaT(double a, double b, double c) { return fma(a, b, c); }
+
+// ---- per-element force, algebraic form (FP32, or FP64 with FMA) -----------------------------------
+// The 2x2 Gauss quadrature of the bilinear element through the element modes: at each point J, its
+// inverse (one division), the displacement gradient and the stress come from the modes, and the
+// nodal force f(m,j) = sum_q dV_q sum_k dN_m/dxi_k (J^-T sigma)_kj collapses to three per-element
+// sums.  Same quadrature as the reference's per-node loops (ElementQuad4Planar.h:191-278), not the
+// same expression order.  The Gauss weights are 1 (w[] kept for generality).
+template <class T>
+__device__ inline void element_force_alg(const ShapeTable& Tb, const double x[4][2], const double u[4][2], T k,
+                                         T gmod, T f[4][2])
+{
+    T Ax[2], Bx[2], Hx[2], Au[2], Bu[2], Hu[2];
+    element_modes(x, Ax, Bx, Hx);
+    element_modes(u, Au, Bu, Hu);
+    const T g2 = T(2) * gmod;
     // accumulated P_q = dV_q J^-T sigma: S0 = sum P(0,.), S1 = sum P(1,.), SH = sum (eta P(0,.) + xi P(1,.))
-    float S0[2] = {0.f, 0.f}, S1[2] = {0.f, 0.f}, SH[2] = {0.f, 0.f};
+    T S0[2] = {T(0), T(0)}, S1[2] = {T(0), T(0)}, SH[2] = {T(0), T(0)};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         // Gauss points (-a,-a), (a,-a), (a,a), (-a,a) with a = 1/sqrt(3)
-        const float a = 0.57735026918962576f;
-        const float xi = (q == 1 || q == 2) ? a : -a;
-        const float eta = (q >= 2) ? a : -a;
+        const T a = T(0.57735026918962576451);
+        const T xi = (q == 1 || q == 2) ? a : -a;
+        const T eta = (q >= 2) ? a : -a;
         // J(i,j) = dx_j / dxi_i
-        const float J00 = fmaf(eta, Hx[0], Ax[0]), J01 = fmaf(eta, Hx[1], Ax[1]);
-        const float J10 = fmaf(xi, Hx[0], Bx[0]), J11 = fmaf(xi, Hx[1], Bx[1]);
-        const float det = fmaf(J00, J11, -J01 * J10);
-        const float rdet = __builtin_amdgcn_rcpf(det);
-        const float I00 = J11 * rdet, I01 = -J01 * rdet, I10 = -J10 * rdet, I11 = J00 * rdet;
+        const T J00 = fmaT(eta, Hx[0], Ax[0]), J01 = fmaT(eta, Hx[1], Ax[1]);
+        const T J10 = fmaT(xi, Hx[0], Bx[0]), J11 = fmaT(xi, Hx[1], Bx[1]);
+        const T det = fmaT(J00, J11, -J01 * J10);
+        T rdet;
+        if constexpr (sizeof(T) == 4) rdet = __builtin_amdgcn_rcpf(det);
+        else rdet = T(1) / det;
+        const T I00 = J11 * rdet, I01 = -J01 * rdet, I10 = -J10 * rdet, I11 = J00 * rdet;
         // du_j / dxi_k
-        const float d0x = fmaf(eta, Hu[0], Au[0]), d0y = fmaf(eta, Hu[1], Au[1]);
-        const float d1x = fmaf(xi, Hu[0], Bu[0]), d1y = fmaf(xi, Hu[1], Bu[1]);
+        const T d0x = fmaT(eta, Hu[0], Au[0]), d0y = fmaT(eta, Hu[1], Au[1]);
+        const T d1x = fmaT(xi, Hu[0], Bu[0]), d1y = fmaT(xi, Hu[1], Bu[1]);
         // grad(i,j) = sum_k Jinv(i,k) du_j/dxi_k
-        const float gxx = fmaf(I00, d0x, I01 * d1x), gxy = fmaf(I00, d0y, I01 * d1y);
-        const float gyx = fmaf(I10, d0x, I11 * d1x), gyy = fmaf(I10, d0y, I11 * d1y);
-        const float tr = gxx + gyy;
-        const float kt = k * tr, t3 = tr * (1.0f / 3.0f);
-        const float vol = static_cast<float>(T.w[q]) * det;
-        const float sxx = fmaf(g2, gxx - t3, kt) * vol;
-        const float syy = fmaf(g2, gyy - t3, kt) * vol;
-        const float sxy = gmod * (gxy + gyx) * vol;
+        const T gxx = fmaT(I00, d0x, I01 * d1x), gxy = fmaT(I00, d0y, I01 * d1y);
+        const T gyx = fmaT(I10, d0x, I11 * d1x), gyy = fmaT(I10, d0y, I11 * d1y);
+        const T tr = gxx + gyy;
+        const T kt = k * tr, t3 = tr * (T(1) / T(3));
+        const T vol = static_cast<T>(Tb.w[q]) * det;
+        const T sxx = fmaT(g2, gxx - t3, kt) * vol;
+        const T syy = fmaT(g2, gyy - t3, kt) * vol;
+        const T sxy = gmod * (gxy + gyx) * vol;
         // P(k,j) = sum_i Jinv(i,k) sigma(i,j) dV
-        const float P0x = fmaf(I00, sxx, I10 * sxy), P0y = fmaf(I00, sxy, I10 * syy);
-        const float P1x = fmaf(I01, sxx, I11 * sxy), P1y = fmaf(I01, sxy, I11 * syy);
+        const T P0x = fmaT(I00, sxx, I10 * sxy), P0y = fmaT(I00, sxy, I10 * syy);
+        const T P1x = fmaT(I01, sxx, I11 * sxy), P1y = fmaT(I01, sxy, I11 * syy);
         S0[0] += P0x;
         S0[1] += P0y;
         S1[0] += P1x;
         S1[1] += P1y;
-        SH[0] = fmaf(eta, P0x, fmaf(xi, P1x, SH[0]));
-        SH[1] = fmaf(eta, P0y, fmaf(xi, P1y, SH[1]));
+        SH[0] = fmaT(eta, P0x, fmaT(xi, P1x, SH[0]));
+        SH[1] = fmaT(eta, P0y, fmaT(xi, P1y, SH[1]));
     }
     // f(m,j) = (s_m S0_j + t_m S1_j + s_m t_m SH_j) / 4
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const float p = 0.25f * S0[j], r = 0.25f * S1[j], h = 0.25f * SH[j];
+        const T p = T(0.25) * S0[j], r = T(0.25) * S1[j], h = T(0.25) * SH[j];
         f[0][j] = -p - r + h;
         f[1][j] = p - r - h;
         f[2][j] = p + r + h;
@@ -200,46 +200,34 @@ __device__ inline void element_force_f32(const ShapeTable& T, const double x[4][
     }
 }
 
-// ---- per-element force, FP32, rectangular element -------------------------------------------------
+// ---- per-element force, rectangular element (FP32, or FP64 with FMA) ------------------------------
 // For an axis-aligned sx x sy rectangle the Jacobian is diag(sx/2, sy/2) at every Gauss point, so the
 // displacement gradient is linear in (xi, eta): grad = grad_c + xi grad_xi + eta grad_eta, and so is
 // the stress.  With Gauss points (+-a, +-a), sum_q xi_q = sum_q eta_q = sum_q xi_q eta_q = 0 and
-// sum_q xi_q^2 = sum_q eta_q^2 = 4 a^2 = 4/3, so the sums of element_force_f32 reduce to
+// sum_q xi_q^2 = sum_q eta_q^2 = 4 a^2 = 4/3, so the sums of element_force_alg reduce to
 //   S0 = 4 dV I0 sigma_c(0,.),  S1 = 4 dV I1 sigma_c(1,.),
 //   SH = 4/3 dV (I0 sigma_eta(0,.) + I1 sigma_xi(1,.)),   I0 = 2/sx, I1 = 2/sy, dV = sx sy / 4,
-// the same 2x2 Gauss quadrature evaluated in closed form (f as in element_force_f32).
-__device__ inline void element_force_f32_rect(float I0, float I1, float dv, const double u[4][2], float k,
-                                              float gmod, float f[4][2])
+// the same 2x2 Gauss quadrature evaluated in closed form (f as in element_force_alg).
+template <class T>
+__device__ inline void element_force_rect(T I0, T I1, T dv, const double u[4][2], T k, T gmod, T f[4][2])
 {
-    float ur[4][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        ur[0][i] = 0.0f;
-#pragma unroll
-        for (int m = 1; m < 4; ++m) ur[m][i] = static_cast<float>(u[m][i] - u[0][i]);
-    }
-    float Au[2], Bu[2], Hu[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        Au[i] = 0.25f * (ur[1][i] + ur[2][i] - ur[3][i]);
-        Bu[i] = 0.25f * (ur[2][i] + ur[3][i] - ur[1][i]);
-        Hu[i] = 0.25f * (ur[2][i] - ur[1][i] - ur[3][i]);
-    }
-    const float g2 = 2.0f * gmod, kd = fmaf(g2, 2.0f / 3.0f, k); // k + 4 G / 3
+    T Au[2], Bu[2], Hu[2];
+    element_modes(u, Au, Bu, Hu);
+    const T g2 = T(2) * gmod, kd = fmaT(g2, T(2) / T(3), k); // k + 4 G / 3
     // centre: grad(i,j) = I_i du_j/dxi_i
-    const float gxx = I0 * Au[0], gxy = I0 * Au[1], gyx = I1 * Bu[0], gyy = I1 * Bu[1];
-    const float tr = gxx + gyy, kt = k * tr, t3 = tr * (1.0f / 3.0f);
-    const float sxx = fmaf(g2, gxx - t3, kt), syy = fmaf(g2, gyy - t3, kt), sxy = gmod * (gxy + gyx);
+    const T gxx = I0 * Au[0], gxy = I0 * Au[1], gyx = I1 * Bu[0], gyy = I1 * Bu[1];
+    const T tr = gxx + gyy, kt = k * tr, t3 = tr * (T(1) / T(3));
+    const T sxx = fmaT(g2, gxx - t3, kt), syy = fmaT(g2, gyy - t3, kt), sxy = gmod * (gxy + gyx);
     // eta slope (only grad(0,.) depends on eta) and xi slope (only grad(1,.))
-    const float sxx_e = kd * (I0 * Hu[0]), sxy_e = gmod * (I0 * Hu[1]);
-    const float syy_x = kd * (I1 * Hu[1]), sxy_x = gmod * (I1 * Hu[0]);
-    const float c0 = 4.0f * dv * I0, c1 = 4.0f * dv * I1, ch = (4.0f / 3.0f) * dv;
-    const float S0[2] = {c0 * sxx, c0 * sxy};
-    const float S1[2] = {c1 * sxy, c1 * syy};
-    const float SH[2] = {ch * fmaf(I0, sxx_e, I1 * sxy_x), ch * fmaf(I0, sxy_e, I1 * syy_x)};
+    const T sxx_e = kd * (I0 * Hu[0]), sxy_e = gmod * (I0 * Hu[1]);
+    const T syy_x = kd * (I1 * Hu[1]), sxy_x = gmod * (I1 * Hu[0]);
+    const T c0 = T(4) * dv * I0, c1 = T(4) * dv * I1, ch = (T(4) / T(3)) * dv;
+    const T S0[2] = {c0 * sxx, c0 * sxy};
+    const T S1[2] = {c1 * sxy, c1 * syy};
+    const T SH[2] = {ch * fmaT(I0, sxx_e, I1 * sxy_x), ch * fmaT(I0, sxy_e, I1 * syy_x)};
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const float p = 0.25f * S0[j], r = 0.25f * S1[j], h = 0.25f * SH[j];
+        const T p = T(0.25) * S0[j], r = T(0.25) * S1[j], h = T(0.25) * SH[j];
         f[0][j] = -p - r + h;
         f[1][j] = p - r - h;
         f[2][j] = p + r + h;
@@ -247,11 +235,29 @@ __device__ inline void element_force_f32_rect(float I0, float I1, float dv, cons
     }
 }
 
+// Element bodies of the fused step: the reference-order FP64 body (bit-exact), and the algebraic /
+// closed-form rectangle bodies in FP32 or in FP64 with FMA.
+enum Body : int { B_REF64 = 0, B_F32 = 1, B_F32_RECT = 2, B_F64 = 3, B_F64_RECT = 4 };
+constexpr bool body_rect(int b) { return b == B_F32_RECT || b == B_F64_RECT; }
+constexpr bool body_f32(int b) { return b == B_F32 || b == B_F32_RECT; }
+
+template <int BODY, class T>
+__device__ inline void element_force_body(const ShapeTable& Tb, T rI0, T rI1, T rdv, const double x[4][2],
+                                          const double u[4][2], float k, float gmod, T f[4][2])
+{
+    if constexpr (BODY == B_REF64)
+        element_force_f64(Tb, x, u, static_cast<double>(k), static_cast<double>(gmod), f);
+    else if constexpr (body_rect(BODY))
+        element_force_rect<T>(rI0, rI1, rdv, u, static_cast<T>(k), static_cast<T>(gmod), f);
+    else
+        element_force_alg<T>(Tb, x, u, static_cast<T>(k), static_cast<T>(gmod), f);
+}
+
 // ---- structured fused step ----------------------------------------------------------------------
-template <int TX, int TY, bool F32, bool RECT = false>
+template <int TX, int TY, int BODY>
 __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     int nx, int ny, int ntx, int nty, int per_xcd, ShapeTable T, const double* __restrict__ coor, double sx, double sy,
-    float rI0, float rI1, float rdv,
+    double rI0, double rI1, double rdv,
     const float* __restrict__ Ke, float Kconst, const float* __restrict__ Ge, const double* __restrict__ inv_mass,
     const double* __restrict__ fext, double dt, double c, double hdt, const double* __restrict__ u0,
     const double* __restrict__ v0, const double* __restrict__ a0, double* __restrict__ u1, double* __restrict__ v1,
@@ -261,9 +267,10 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     constexpr int EX = TX + 1, EY = TY + 1;   // element tile
     constexpr int NT = TX * TY;
     constexpr int NB = (EX * EY + NT - 1) / NT; // elements per thread
-    using FT = typename std::conditional<F32, float, double>::type;
+    constexpr bool RECT = body_rect(BODY);
+    using FT = typename std::conditional<body_f32(BODY), float, double>::type;
     __shared__ double2 s_u[HY * HX];
-    __shared__ double2 s_x[HY * HX];
+    __shared__ double2 s_x[RECT ? 1 : HY * HX]; // the rectangle body needs no coordinates
     __shared__ FT s_f[EY * EX][8];
 
     // XCD-aware tile order: hardware places block b on XCD b % 8; give each XCD a contiguous band
@@ -359,30 +366,19 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
             double x[4][2], u[4][2];
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                const double2 xx = s_x[nid[m]], uu = s_u[nid[m]];
+                const double2 xx = RECT ? make_double2(0.0, 0.0) : s_x[nid[m]], uu = s_u[nid[m]];
                 x[m][0] = xx.x;
                 x[m][1] = xx.y;
                 u[m][0] = uu.x;
                 u[m][1] = uu.y;
             }
-            if constexpr (F32) {
-                float f[4][2];
-                if constexpr (RECT) element_force_f32_rect(rI0, rI1, rdv, u, Ke ? Ke[e] : Kconst, Ge[e], f);
-                else element_force_f32(T, x, u, Ke ? Ke[e] : Kconst, Ge[e], f);
+            FT f[4][2];
+            element_force_body<BODY, FT>(T, static_cast<FT>(rI0), static_cast<FT>(rI1), static_cast<FT>(rdv), x, u,
+                                         Ke ? Ke[e] : Kconst, Ge[e], f);
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    s_f[t][2 * m] = f[m][0];
-                    s_f[t][2 * m + 1] = f[m][1];
-                }
-            }
-            else {
-                double f[4][2];
-                element_force_f64(T, x, u, static_cast<double>(Ke ? Ke[e] : Kconst), static_cast<double>(Ge[e]), f);
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    s_f[t][2 * m] = f[m][0];
-                    s_f[t][2 * m + 1] = f[m][1];
-                }
+            for (int m = 0; m < 4; ++m) {
+                s_f[t][2 * m] = f[m][0];
+                s_f[t][2 * m + 1] = f[m][1];
             }
         }
     }
@@ -438,88 +434,38 @@ __global__ void k_update_u(int64_t n, double dt, double c, const double* __restr
     if (d < n) u1[d] = u0[d] + dt * v0[d] + c * a0[d];
 }
 
-template <bool F32>
-__global__ void k_element_general(int64_t nelem, const int32_t* __restrict__ conn, const int32_t* __restrict__ dofs,
-                                  const double* __restrict__ dNx, const double* __restrict__ vol,
-                                  const float* __restrict__ Ke, const float* __restrict__ Ge,
-                                  const double* __restrict__ u, double* __restrict__ fe64, float* __restrict__ fe32)
+// One thread per element: gathers the element's coordinates (nodevec) and updated displacement
+// (dofval through dofs), recomputes the geometry from the coordinates and writes the element force
+// (FP64 reference order, FP32, or FP64 with FMA) as an elemvec [nelem, 4, 2].  The connectivity row of
+// an element is one 16-byte load; consecutive threads read consecutive rows.
+template <int BODY>
+__global__ void __launch_bounds__(BS) k_element_general(int64_t nelem, ShapeTable T, const int32_t* __restrict__ conn,
+                                                        const int32_t* __restrict__ dofs, const double* __restrict__ coor,
+                                                        const float* __restrict__ Ke, const float* __restrict__ Ge,
+                                                        const double* __restrict__ u,
+                                                        typename std::conditional<body_f32(BODY), float, double>::type* __restrict__ fe)
 {
-    int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    using FT = typename std::conditional<body_f32(BODY), float, double>::type;
+    const int64_t e = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x;
     if (e >= nelem) return;
-    double ue[4][2];
+    const int4 cn = reinterpret_cast<const int4*>(conn)[e];
+    const int nodes[4] = {cn.x, cn.y, cn.z, cn.w};
+    double x[4][2], ue[4][2];
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) ue[m][i] = u[dofs[static_cast<int64_t>(conn[e * 4 + m]) * 2 + i]];
-    if constexpr (!F32) {
-        // reference order: SymGradN_vector -> stress -> Int_gradN_dot_tensor2_dV (stored geometry)
-        double f[4][2];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) f[m][0] = f[m][1] = 0.0;
-        const double k = Ke[e], gmod = Ge[e];
-        for (int q = 0; q < 4; ++q) {
-            const double* d = dNx + (e * 4 + q) * 8;
-            double e00 = d[0] * ue[0][0] + d[2] * ue[1][0] + d[4] * ue[2][0] + d[6] * ue[3][0];
-            double e11 = d[1] * ue[0][1] + d[3] * ue[1][1] + d[5] * ue[2][1] + d[7] * ue[3][1];
-            double e01 = 0.5 * (d[0] * ue[0][1] + d[2] * ue[1][1] + d[4] * ue[2][1] + d[6] * ue[3][1] +
-                                d[1] * ue[0][0] + d[3] * ue[1][0] + d[5] * ue[2][0] + d[7] * ue[3][0]);
-            double e22 = 0.0;
-            double tr = e00 + e11 + e22;
-            double s00 = k * tr + 2.0 * gmod * (e00 - tr / 3.0);
-            double s11 = k * tr + 2.0 * gmod * (e11 - tr / 3.0);
-            double s01 = 2.0 * gmod * e01;
-            double s10 = 2.0 * gmod * e01;
-            double v = vol[e * 4 + q];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                f[m][0] += (d[2 * m] * s00 + d[2 * m + 1] * s10) * v;
-                f[m][1] += (d[2 * m] * s01 + d[2 * m + 1] * s11) * v;
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            fe64[e * 8 + 2 * m] = f[m][0];
-            fe64[e * 8 + 2 * m + 1] = f[m][1];
-        }
+    for (int m = 0; m < 4; ++m) {
+        const double2 xx = reinterpret_cast<const double2*>(coor)[nodes[m]];
+        const int2 dd = reinterpret_cast<const int2*>(dofs)[nodes[m]];
+        x[m][0] = xx.x;
+        x[m][1] = xx.y;
+        ue[m][0] = u[dd.x];
+        ue[m][1] = u[dd.y];
     }
-    else {
-        float ur[4][2];
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int i = 0; i < 2; ++i) ur[m][i] = static_cast<float>(ue[m][i] - ue[0][i]);
-        float f[4][2];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) f[m][0] = f[m][1] = 0.0f;
-        const float k = Ke[e], gmod = Ge[e];
-        for (int q = 0; q < 4; ++q) {
-            float d[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) d[j] = static_cast<float>(dNx[(e * 4 + q) * 8 + j]);
-            float du00 = 0.f, du01 = 0.f, du10 = 0.f, du11 = 0.f;
-#pragma unroll
-            for (int m = 1; m < 4; ++m) {
-                du00 = fmaf(d[2 * m], ur[m][0], du00);
-                du01 = fmaf(d[2 * m], ur[m][1], du01);
-                du10 = fmaf(d[2 * m + 1], ur[m][0], du10);
-                du11 = fmaf(d[2 * m + 1], ur[m][1], du11);
-            }
-            float tr = du00 + du11, t3 = tr * (1.0f / 3.0f), g2 = 2.0f * gmod, kt = k * tr;
-            float vq = static_cast<float>(vol[e * 4 + q]);
-            float sv00 = fmaf(g2, du00 - t3, kt) * vq, sv11 = fmaf(g2, du11 - t3, kt) * vq;
-            float sv01 = g2 * 0.5f * (du01 + du10) * vq;
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                f[m][0] += fmaf(d[2 * m], sv00, d[2 * m + 1] * sv01);
-                f[m][1] += fmaf(d[2 * m], sv01, d[2 * m + 1] * sv11);
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            fe32[e * 8 + 2 * m] = f[m][0];
-            fe32[e * 8 + 2 * m + 1] = f[m][1];
-        }
-    }
+    FT f[4][2];
+    element_force_body<BODY, FT>(T, FT(0), FT(0), FT(0), x, ue, Ke[e], Ge[e], f);
+    typedef FT v4 __attribute__((ext_vector_type(4)));
+    v4* out = reinterpret_cast<v4*>(fe + e * 8);
+    out[0] = v4{f[0][0], f[0][1], f[1][0], f[1][1]};
+    out[1] = v4{f[2][0], f[2][1], f[3][0], f[3][1]};
 }
 
 template <class T>
@@ -587,9 +533,9 @@ struct TileCfg {
     int tx, ty;
 };
 
-// Node tile per workgroup (measured on MI355X at 2048^2): FP32 with Regular coordinates (closed-form
-// rectangle body) 64 x 8, other FP32 32 x 16, FP64 32 x 8.  GF_EXPLICIT_TILE=<tx>x<ty> overrides.
-TileCfg tile_config(bool f32, bool rect)
+// Node tile per workgroup (measured on MI355X at 2048^2): the closed-form rectangle bodies 64 x 8,
+// the FP32 algebraic body 32 x 16, the FP64 bodies 32 x 8.  GF_EXPLICIT_TILE=<tx>x<ty> overrides.
+TileCfg tile_config(int body)
 {
     static const TileCfg over = [] {
         TileCfg c{0, 0};
@@ -600,39 +546,43 @@ TileCfg tile_config(bool f32, bool rect)
         return c;
     }();
     if (over.tx > 0) return over;
-    if (f32) return rect ? TileCfg{64, 8} : TileCfg{32, 16};
-    return TileCfg{32, 8};
+    switch (body) {
+        case B_F32_RECT:
+        case B_F64_RECT: return TileCfg{64, 8};
+        case B_F32: return TileCfg{32, 16};
+        default: return TileCfg{32, 8};
+    }
+}
+
+template <int TX, int TY, int BODY>
+void launch_body(unsigned grid, hipStream_t s, int nx, int ny, int ntx, int nty, int per_xcd, const ShapeTable& T,
+                 gf_explicit_s* x, const double* fext, double dt, double c, double hdt, int i0, int i1, double* fo)
+{
+    k_vv_structured<TX, TY, BODY><<<grid, TX * TY, 0, s>>>(
+        nx, ny, ntx, nty, per_xcd, T, x->reg_x ? nullptr : x->coor.p, x->sx, x->sy, 2.0 / x->sx, 2.0 / x->sy,
+        x->sx * x->sy / 4.0, x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt,
+        x->u[i0].p, x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p, x->a[i1].p, fo);
 }
 
 template <int TX, int TY>
-void launch_structured(bool f32, unsigned grid, hipStream_t s, int nx, int ny, int ntx, int nty, int per_xcd,
+void launch_structured(int body, unsigned grid, hipStream_t s, int nx, int ny, int ntx, int nty, int per_xcd,
                        const ShapeTable& T, gf_explicit_s* x, const double* fext, double dt, double c, double hdt,
                        int i0, int i1, double* fo)
 {
-    if (f32 && x->reg_x)
-        k_vv_structured<TX, TY, true, true><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T,
-                                                               x->reg_x ? nullptr : x->coor.p, x->sx, x->sy,
-                                                               static_cast<float>(2.0 / x->sx), static_cast<float>(2.0 / x->sy),
-                                                               static_cast<float>(x->sx * x->sy / 4.0),
-                                                               x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
-                                                               x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
-                                                               x->a[i1].p, fo);
-    else if (f32)
-        k_vv_structured<TX, TY, true><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T,
-                                                               x->reg_x ? nullptr : x->coor.p, x->sx, x->sy,
-                                                               static_cast<float>(2.0 / x->sx), static_cast<float>(2.0 / x->sy),
-                                                               static_cast<float>(x->sx * x->sy / 4.0),
-                                                               x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
-                                                               x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
-                                                               x->a[i1].p, fo);
-    else
-        k_vv_structured<TX, TY, false><<<grid, TX * TY, 0, s>>>(nx, ny, ntx, nty, per_xcd, T,
-                                                                x->reg_x ? nullptr : x->coor.p, x->sx, x->sy,
-                                                               static_cast<float>(2.0 / x->sx), static_cast<float>(2.0 / x->sy),
-                                                               static_cast<float>(x->sx * x->sy / 4.0),
-                                                                x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt, x->u[i0].p,
-                                                                x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p,
-                                                                x->a[i1].p, fo);
+    switch (body) {
+        case B_REF64:
+            return launch_body<TX, TY, B_REF64>(grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+        case B_F32:
+            return launch_body<TX, TY, B_F32>(grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+        case B_F32_RECT:
+            return launch_body<TX, TY, B_F32_RECT>(grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1,
+                                                   fo);
+        case B_F64:
+            return launch_body<TX, TY, B_F64>(grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+        default:
+            return launch_body<TX, TY, B_F64_RECT>(grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1,
+                                                   fo);
+    }
 }
 
 } // namespace
@@ -696,21 +646,13 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
                 x->sy = sy;
             }
             if (!x->structured) {
-                // stored Gauss geometry (QuadraturePlanar(AsElement(coor)))
-                DBuf<double> xe(v->nelem * 8);
-                int rc = gf_vector_as_element_nodevec(v, x->coor.p, xe.p, nullptr);
-                if (rc) throw Error(rc, gf_last_error());
-                rc = gf_quad_create(&x->quad, GF_QUAD4_PLANAR, xe.p, v->nelem, GF_GAUSS, nullptr, nullptr, 0, 1.0,
-                                    nullptr);
-                if (rc) throw Error(rc, gf_last_error());
-                GF_HIP(hipDeviceSynchronize());
+                GF_REQUIRE(v->nelem < (int64_t(1) << 28), "general path: nelem < 2^28");
                 x->fe64.alloc(v->nelem * 8);
                 x->fe32.alloc(v->nelem * 8);
             }
             GF_HIP(hipDeviceSynchronize());
         }
         catch (...) {
-            if (x->quad) gf_quad_destroy(x->quad);
             delete x;
             throw;
         }
@@ -720,7 +662,6 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
 
 int gf_explicit_destroy(gf_explicit x)
 {
-    if (x && x->quad) gf_quad_destroy(x->quad);
     delete x;
     return GF_OK;
 }
@@ -756,15 +697,18 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
 {
     return guard([&] {
         GF_REQUIRE(x, "null integrator");
-        GF_REQUIRE(precision == GF_F64 || precision == GF_F32, "precision must be GF_F64 or GF_F32");
+        GF_REQUIRE(precision == GF_F64 || precision == GF_F32 || precision == GF_F64_FAST,
+                   "precision must be GF_F64, GF_F32 or GF_F64_FAST");
         GF_REQUIRE(nsteps >= 0, "nsteps >= 0");
         hipStream_t s = S(stream);
         const double dt = x->dt, c = 0.5 * dt * dt, hdt = 0.5 * dt;
         const double* fext = x->has_fext ? x->fext.p : nullptr;
-        const bool f32 = precision == GF_F32;
+        static const ShapeTable T = gauss_table();
         if (x->structured) {
-            static const ShapeTable T = gauss_table();
-            const TileCfg tc = tile_config(f32, x->reg_x);
+            const int body = precision == GF_F64 ? B_REF64
+                             : precision == GF_F32 ? (x->reg_x ? B_F32_RECT : B_F32)
+                                                   : (x->reg_x ? B_F64_RECT : B_F64);
+            const TileCfg tc = tile_config(body);
             const int TXr = tc.tx, TYr = tc.ty;
             const int nx = static_cast<int>(x->nx), ny = static_cast<int>(x->ny);
             const int ntx = (nx + 1 + TXr - 1) / TXr, nty = (ny + 1 + TYr - 1) / TYr;
@@ -774,50 +718,46 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
                 const int i0 = x->cur, i1 = 1 - x->cur;
                 double* fo = (store_fint && k == nsteps - 1) ? x->fint.p : nullptr;
                 if (TXr == 32 && TYr == 8)
-                    launch_structured<32, 8>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                    launch_structured<32, 8>(body, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
                 else if (TXr == 64 && TYr == 8)
-                    launch_structured<64, 8>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                    launch_structured<64, 8>(body, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
                 else if (TXr == 32 && TYr == 16)
-                    launch_structured<32, 16>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                    launch_structured<32, 16>(body, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
                 else if (TXr == 64 && TYr == 4)
-                    launch_structured<64, 4>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                    launch_structured<64, 4>(body, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
                 else if (TXr == 16 && TYr == 16)
-                    launch_structured<16, 16>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
-                else if (TXr == 128 && TYr == 4)
-                    launch_structured<128, 4>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
-                else if (TXr == 64 && TYr == 16)
-                    launch_structured<64, 16>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
-                else if (TXr == 128 && TYr == 8)
-                    launch_structured<128, 8>(f32, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
+                    launch_structured<16, 16>(body, grid, s, nx, ny, ntx, nty, per_xcd, T, x, fext, dt, c, hdt, i0, i1, fo);
                 else
-                    throw Error(GF_EINVAL,
-                                "GF_EXPLICIT_TILE must be one of 32x8, 64x8, 32x16, 64x4, 16x16, 128x4, 64x16, 128x8");
+                    throw Error(GF_EINVAL, "GF_EXPLICIT_TILE must be one of 32x8, 64x8, 32x16, 64x4, 16x16");
                 check_launch();
                 x->cur = i1;
             }
         }
         else {
             auto v = x->v;
-            const double* dNx = gf::quad_dNx(x->quad);
-            const double* vol = gf::quad_vol(x->quad);
+            const unsigned ge = nblocks(x->nelem, BS), gd = nblocks(x->ndof, BS);
             for (int64_t k = 0; k < nsteps; ++k) {
                 const int i0 = x->cur, i1 = 1 - x->cur;
                 double* fo = (store_fint && k == nsteps - 1) ? x->fint.p : nullptr;
-                k_update_u<<<nblocks(x->ndof, BS), BS, 0, s>>>(x->ndof, dt, c, x->u[i0].p, x->vv[i0].p, x->a[i0].p,
-                                                             x->u[i1].p);
-                if (f32) {
-                    k_element_general<true><<<nblocks(x->nelem, BS), BS, 0, s>>>(
-                        x->nelem, v->conn.p, v->dofs.p, dNx, vol, x->Ke.p, x->Ge.p, x->u[i1].p, nullptr, x->fe32.p);
-                    k_node_update<float><<<nblocks(x->ndof, BS), BS, 0, s>>>(
-                        x->ndof, v->dof_eptr.p, v->dof_eidx.p, x->fe32.p, x->inv_mass.p, fext, hdt, x->vv[i0].p,
-                        x->a[i0].p, x->vv[i1].p, x->a[i1].p, fo);
+                k_update_u<<<gd, BS, 0, s>>>(x->ndof, dt, c, x->u[i0].p, x->vv[i0].p, x->a[i0].p, x->u[i1].p);
+                const double* un = x->u[i1].p;
+                if (precision == GF_F32) {
+                    k_element_general<B_F32><<<ge, BS, 0, s>>>(x->nelem, T, v->conn.p, v->dofs.p, x->coor.p, x->Ke.p,
+                                                                x->Ge.p, un, x->fe32.p);
+                    k_node_update<float><<<gd, BS, 0, s>>>(x->ndof, v->dof_eptr.p, v->dof_eidx.p, x->fe32.p,
+                                                           x->inv_mass.p, fext, hdt, x->vv[i0].p, x->a[i0].p,
+                                                           x->vv[i1].p, x->a[i1].p, fo);
                 }
                 else {
-                    k_element_general<false><<<nblocks(x->nelem, BS), BS, 0, s>>>(
-                        x->nelem, v->conn.p, v->dofs.p, dNx, vol, x->Ke.p, x->Ge.p, x->u[i1].p, x->fe64.p, nullptr);
-                    k_node_update<double><<<nblocks(x->ndof, BS), BS, 0, s>>>(
-                        x->ndof, v->dof_eptr.p, v->dof_eidx.p, x->fe64.p, x->inv_mass.p, fext, hdt, x->vv[i0].p,
-                        x->a[i0].p, x->vv[i1].p, x->a[i1].p, fo);
+                    if (precision == GF_F64)
+                        k_element_general<B_REF64><<<ge, BS, 0, s>>>(x->nelem, T, v->conn.p, v->dofs.p, x->coor.p,
+                                                                      x->Ke.p, x->Ge.p, un, x->fe64.p);
+                    else
+                        k_element_general<B_F64><<<ge, BS, 0, s>>>(x->nelem, T, v->conn.p, v->dofs.p, x->coor.p,
+                                                                    x->Ke.p, x->Ge.p, un, x->fe64.p);
+                    k_node_update<double><<<gd, BS, 0, s>>>(x->ndof, v->dof_eptr.p, v->dof_eidx.p, x->fe64.p,
+                                                            x->inv_mass.p, fext, hdt, x->vv[i0].p, x->a[i0].p,
+                                                            x->vv[i1].p, x->a[i1].p, fo);
                 }
                 check_launch();
                 x->cur = i1;

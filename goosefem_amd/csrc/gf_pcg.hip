@@ -9,13 +9,17 @@
 //   gather  y = AssembleDofs(f)                          (ordered DOF-centric gather, Vector.h:723-741)
 //   fold    out_u = y_u + C_du^T y_d
 // Reductions are two-stage and deterministic: every block sums its grid-stride range in a fixed
-// order and a fixed LDS tree; consumers reduce the block partials in block order.  The iteration
+// order, its wavefronts with DPP lane moves and its waves in order; consumers reduce the block
+// partials the same way.  The iteration
 // runs with no host synchronisation: scalars live on the device and a device flag stops the
 // iteration (kernels launched after convergence return immediately).
 #include <algorithm>
 #include <cmath>
+#include <tuple>
+#include <utility>
 
 #include "gf_internal.hpp"
+#include "gf_wave.hpp"
 
 using namespace gf;
 
@@ -58,40 +62,28 @@ struct gf_pcg_s {
     int nty = 0;
     DBuf<double> ke8;   // Hex8: [2][24][24] element matrices per unit K and per unit G
     DBuf<double> p2;    // Quad4: second search-direction buffer of the fused iteration
+    DBuf<double> xprev; // the last converged solution (the previous-step start of the fallback)
+    bool has_prev = false;
+    int last_start = 0; // start of the last solve: GF_START_GIVEN / _PREVIOUS / _ZERO
 };
 
 namespace {
 
-// block reduction of one value per thread (block of NTH threads) into partial[blockIdx]; fixed tree
+// block reduction of one value per thread (block of NTH threads) into partial[blockIdx]: DPP
+// wavefront sums, then the waves in order (deterministic)
 template <int NTH>
 __device__ inline void block_reduce_store_n(double v, double* partial)
 {
-    __shared__ double shr[NTH];
-    shr[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = NTH / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) shr[threadIdx.x] += shr[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = shr[0];
-    __syncthreads();
+    const double t = block_sum<NTH>(v);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
 // sum of `count` partials in fixed order (every block computes the same value)
 __device__ inline double reduce_partials(const double* partial, int count = NBR)
 {
-    __shared__ double sh[BS];
     double v = 0.0;
     for (int i = threadIdx.x; i < count; i += BS) v += partial[i];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = BS / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    double r = sh[0];
-    __syncthreads();
-    return r;
+    return block_sum<BS>(v);
 }
 
 // ---- one Gauss point of the plane-strain Quad4 element force, FP64, reference expression order ----
@@ -438,19 +430,8 @@ __device__ inline double row_sum(const int32_t* ptr, const int32_t* idx, const d
     return s;
 }
 
-// block reduction of one value per thread into partial[blockIdx]; fixed tree
-__device__ inline void block_reduce_store(double v, double* partial)
-{
-    __shared__ double sh[BS];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = BS / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
-    __syncthreads();
-}
+// block reduction of one value per thread into partial[blockIdx]
+__device__ inline void block_reduce_store(double v, double* partial) { block_reduce_store_n<BS>(v, partial); }
 
 
 // out_u = y_u + C_du^T y_d with y = ordered gather of fe; optional partial of dot(out, w)
@@ -598,6 +579,12 @@ __global__ void __launch_bounds__(BS) k_pcg_direction(int64_t n, const double* _
         state[1] = static_cast<int32_t>(it + 1);
         if (rr <= scal[4] || it + 1 >= state[2]) state[0] = 1;
     }
+}
+
+__global__ void __launch_bounds__(BS) k_fill_zero(int64_t n, double* __restrict__ a)
+{
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * BS)
+        a[i] = 0.0;
 }
 
 __global__ void __launch_bounds__(BS) k_residual(int64_t n, const double* __restrict__ b, const double* __restrict__ q,
@@ -1021,14 +1008,42 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         if (check_every <= 0) check_every = 32;
         hipStream_t s = S(stream);
         const int64_t n = P->nnu;
-        // x0 -> u-part of x_full (the search direction lives there during the iteration)
-        GF_HIP(hipMemcpyAsync(P->xfull.p, x_u, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        apply(P, nullptr, nullptr, P->qv.p, true, s, nullptr, nullptr, nullptr);
-        k_pcg_init<<<NBR, BS, 0, s>>>(n, b_u, P->qv.p, P->invdiag.p, P->r.p, P->z.p, P->x.p, P->xfull.p, P->partial.p);
-        k_pcg_init_scalars<<<1, BS, 0, s>>>(P->partial.p, P->scal.p, P->state.p, tol);
+        // Initial residual of a start (nullptr: zero) -> host (||r0||^2, ||b||^2).  The start goes to
+        // the u-part of x_full, where the search direction lives during the iteration.
+        auto init = [&](const double* x0) {
+            if (x0) GF_HIP(hipMemcpyAsync(P->xfull.p, x0, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            else k_fill_zero<<<std::min<unsigned>(nblocks(n, BS), 4096), BS, 0, s>>>(n, P->xfull.p);
+            apply(P, nullptr, nullptr, P->qv.p, true, s, nullptr, nullptr, nullptr);
+            k_pcg_init<<<NBR, BS, 0, s>>>(n, b_u, P->qv.p, P->invdiag.p, P->r.p, P->z.p, P->x.p, P->xfull.p,
+                                          P->partial.p);
+            k_pcg_init_scalars<<<1, BS, 0, s>>>(P->partial.p, P->scal.p, P->state.p, tol);
+            check_launch();
+            double sc[2];
+            GF_HIP(hipMemcpyAsync(sc, P->scal.p + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            GF_HIP(hipStreamSynchronize(s));
+            return std::make_pair(sc[1], sc[0]); // (rr, bb)
+        };
+        // Fallback of a bad start (a learned prediction gone wrong: NaN, huge, garbage): a start whose
+        // residual is not finite or larger than ||b|| (worse than a zero guess) is replaced by the
+        // previous solve's solution, when that one does better than zero, else by zero.
+        auto [rr, bb] = init(x_u);
+        if (!std::isfinite(bb))
+            throw Error(GF_EINVAL, "gf_pcg_solve: the right-hand side is not finite (or its squared norm overflows)");
+        P->last_start = GF_START_GIVEN;
+        if (!(std::isfinite(rr) && rr <= bb)) {
+            bool ok = false;
+            if (P->has_prev) {
+                std::tie(rr, bb) = init(P->xprev.p);
+                ok = std::isfinite(rr) && rr <= bb;
+                P->last_start = GF_START_PREVIOUS;
+            }
+            if (!ok) {
+                std::tie(rr, bb) = init(nullptr);
+                P->last_start = GF_START_ZERO;
+            }
+        }
         int32_t cap = static_cast<int32_t>(maxiter);
         GF_HIP(hipMemcpyAsync(P->state.p + 2, &cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
-        check_launch();
         int32_t st[2] = {0, 0};
         // Quad4: the operator forms the search direction (two launches per iteration).  Hex8 keeps
         // the separate direction pass: its 3-D halo would read z and p_prev 2.3 times per node.
@@ -1039,7 +1054,8 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
             sh.v[q] = P->sh_v[q];
         }
         double* pb[2] = {P->xfull.p, P->p2.p};
-        for (int64_t it = 0; it < maxiter;) {
+        const bool done0 = rr <= tol * tol * (bb > 0.0 ? bb : 1.0);
+        for (int64_t it = 0; it < maxiter && !done0;) {
             for (int64_t k = 0; fused && k < check_every && it < maxiter; ++k, ++it) {
                 // two launches per iteration: q = K' p with p = z + beta p_prev built in the operator,
                 // then the x / r / z update
@@ -1076,6 +1092,18 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         if (relres) *relres = res;
         if (!(res <= tol * 10.0)) throw Error(GF_ENOTCONV, "PCG did not reach the tolerance (true residual " +
                                                                std::to_string(res) + ")");
+        // keep the solution as the previous-step start of a later fallback
+        if (P->xprev.n != static_cast<size_t>(n)) P->xprev.alloc(n);
+        GF_HIP(hipMemcpyAsync(P->xprev.p, P->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        P->has_prev = true;
+    });
+}
+
+int gf_pcg_start_used(gf_pcg P, int* start)
+{
+    return guard([&] {
+        GF_REQUIRE(P && start, "null argument");
+        *start = P->last_start;
     });
 }
 

@@ -46,6 +46,9 @@ extern "C" {
 #define GF_F64 0    /* FP64 throughout, reference expression order, no FMA contraction */
 #define GF_F32 1    /* FP32 per-quadrature-point work (inputs/outputs FP32 where typed void*) */
 #define GF_BF16X3 2 /* stiffness only: BF16 hi/lo split operands on MFMA, FP32 accumulation */
+#define GF_F64_FAST 3 /* explicit step only: FP64 throughout with fused multiply-add, one division per
+                       * Gauss point and the closed-form rectangle body on Regular coordinates;
+                       * within 1e-14 of GF_F64, not bit for bit */
 
 typedef struct gf_vector_s* gf_vector;
 typedef struct gf_quad_s* gf_quad;
@@ -71,6 +74,10 @@ int gf_device_free(void* ptr);
 int gf_memcpy_h2d(void* dst, const void* src_h, int64_t bytes);
 int gf_memcpy_d2h(void* dst_h, const void* src, int64_t bytes);
 int gf_memcpy_d2d(void* dst, const void* src, int64_t bytes);
+/* Streaming device copy of n doubles (n even, 16-byte aligned): the measured copy ceiling that
+ * roofline fractions are reported against beside the datasheet peak.  mode 0: plain loads and
+ * stores; 1: non-temporal.  No counterpart in the reference (measurement utility). */
+int gf_copy_stream(const double* src, double* dst, int64_t n, int mode, void* stream);
 
 /* ================================================================================================
  * Mesh bookkeeping (host, set-up only).  Integer outputs are bit-exact with the reference.
@@ -501,9 +508,9 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
 int gf_explicit_destroy(gf_explicit x);
 /* state arrays (device, [ndof] each); writable between steps */
 int gf_explicit_state(gf_explicit x, double** u, double** v, double** a);
-/* run nsteps steps; precision GF_F64 (bit-exact with the serial reference order) or GF_F32
- * (FP32 strain/stress/force, FP64 assembly and update).  store_fint != 0 keeps the internal force
- * of the last step (one extra [ndof] write). */
+/* run nsteps steps; precision GF_F64 (bit-exact with the serial reference order), GF_F32 (FP32
+ * strain/stress/force, FP64 assembly and update) or GF_F64_FAST (FP64 element work with FMA).
+ * store_fint != 0 keeps the internal force of the last step (one extra [ndof] write). */
 int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fint, void* stream);
 /* internal force (dofval) of the last stored step's displacement, FP64 [ndof] */
 int gf_explicit_fint(gf_explicit x, const double** fint);
@@ -544,9 +551,18 @@ int gf_pcg_inverse_diagonal(gf_pcg p, const double** invdiag);
 /* Solve K' x_u = b_u.  x_u: initial guess in, solution out.  Stops when the recursive residual
  * ||r|| <= tol ||b||; *relres receives the true residual ||b - K' x|| / ||b|| recomputed in FP64,
  * *iterations the CG iterations.  The host checks the device convergence flag every check_every
- * iterations (<= 0: 32).  Returns GF_ENOTCONV if the true residual exceeds 10 tol. */
+ * iterations (<= 0: 32).  Returns GF_ENOTCONV if the true residual exceeds 10 tol, GF_EINVAL at
+ * once if b_u is not finite.  A start whose residual is not finite or exceeds ||b|| (worse than a
+ * zero guess) is replaced by the previous converged solution of this solver, if that one does
+ * better than zero, else by zero; gf_pcg_start_used reports which start ran.  Replaces the tied
+ * direct solve (MatrixPartitionedTyingsSolver::solve, MatrixPartitionedTyings.h:528-564). */
 int gf_pcg_solve(gf_pcg p, const double* b_u, double* x_u, double tol, int64_t maxiter, int64_t check_every,
                  int64_t* iterations, double* relres, void* stream);
+#define GF_START_GIVEN 0    /* the caller's x_u */
+#define GF_START_PREVIOUS 1 /* the previous converged solution (fallback) */
+#define GF_START_ZERO 2     /* zero (fallback) */
+/* start used by the last gf_pcg_solve: GF_START_GIVEN, GF_START_PREVIOUS or GF_START_ZERO */
+int gf_pcg_start_used(gf_pcg p, int* start);
 
 /* ================================================================================================
  * Learned warm start of the tied PCG (no counterpart in the reference): kernels of a periodic

@@ -617,6 +617,23 @@ def vv_steps(nsteps, et, conn, dofs_, dNx, vol, Ke, Ge, inv_mass, dt, u, v, a, f
     return fint
 
 
+def vv_steps_parallel(nsteps, et, conn, dofs_, dNx, vol, Ke, Ge, inv_mass, dt, u, v, a, fext=None):
+    """vv_steps on all host threads (OpenMP; the all-cores CPU baseline), bit-identical to vv_steps.
+
+    Returns (fint of the last step, threads used).
+    """
+    conn, dofs_ = _i(conn), _i(dofs_)
+    ndof = int(dofs_.max()) + 1
+    fint = np.empty(ndof)
+    c = 0.5 * dt * dt
+    for arr in (u, v, a):
+        assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"]
+    nt = _call("or_vv_steps_parallel", int(nsteps), _c(et), conn, dofs_, conn.shape[0], ndof, _f64(dNx),
+               _f64(vol), _f64(Ke), _f64(Ge), _f64(inv_mass), None if fext is None else _f64(fext), float(dt),
+               float(c), u, v, a, fint)
+    return fint, int(nt)
+
+
 def pcg_csr(rowptr, col, val, diag, b, x0=None, tol=1e-10, maxiter=100000):
     """Plain FP64 Jacobi-PCG (not in the reference). Returns (x, iterations, true rel. residual)."""
     n = rowptr.size - 1

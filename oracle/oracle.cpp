@@ -22,6 +22,9 @@
 #include <cstring>
 #include <numeric>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef int64_t i64;
 
@@ -1232,6 +1235,75 @@ void or_vv_steps(i64 nsteps, int et, const i64* conn, const i64* dofs, i64 nelem
     }
     if (fint_out)
         for (i64 d = 0; d < ndof; ++d) fint_out[d] = fint[d];
+}
+
+// The same steps on all host threads (OpenMP): the element passes run the functions above on
+// contiguous element chunks, and the assembly is a per-DOF gather of the element-vector entries in
+// ascending (e, m, i) order, the order of the serial scatter, so the results are bit-identical to
+// or_vv_steps.  Returns the number of threads used.  This is the "all host cores" CPU baseline; the
+// reference itself cannot run its OpenMP loops (Element.h:122-145 returns from inside a parallel
+// loop, which GCC rejects with -fopenmp).
+i64 or_vv_steps_parallel(i64 nsteps, int et, const i64* conn, const i64* dofs, i64 nelem, i64 ndof,
+                         const double* dNx, const double* vol, const double* Ke, const double* Ge,
+                         const double* inv_mass, const double* fext, double dt, double c, double* u, double* v,
+                         double* a, double* fint_out)
+{
+    const int nne = et_nne(et), nd = et_ndim(et), td = et_tdim(et), nip = nne;
+    std::vector<double> ue(nelem * nne * nd), eps(nelem * nip * td * td), sig(nelem * nip * td * td),
+        fe(nelem * nne * nd), fint(ndof), Kq(nelem * nip), Gq(nelem * nip);
+    for (i64 e = 0; e < nelem; ++e)
+        for (int q = 0; q < nip; ++q) {
+            Kq[e * nip + q] = Ke[e];
+            Gq[e * nip + q] = Ge[e];
+        }
+    // DOF -> element-vector entries k = (e nne + m) nd + i, ascending (counting sort)
+    std::vector<i64> ptr(ndof + 1, 0), idx(nelem * nne * nd);
+    for (i64 k = 0; k < nelem * nne * nd; ++k) ++ptr[dofs[conn[k / nd] * nd + k % nd] + 1];
+    for (i64 d = 0; d < ndof; ++d) ptr[d + 1] += ptr[d];
+    {
+        std::vector<i64> fill(ptr.begin(), ptr.end() - 1);
+        for (i64 k = 0; k < nelem * nne * nd; ++k) idx[fill[dofs[conn[k / nd] * nd + k % nd]]++] = k;
+    }
+    const i64 chunk = 4096;
+    const i64 nchunk = (nelem + chunk - 1) / chunk;
+    int nthreads = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        {
+#ifdef _OPENMP
+            nthreads = omp_get_num_threads();
+#endif
+        }
+    }
+    for (i64 s = 0; s < nsteps; ++s) {
+#pragma omp parallel for schedule(static)
+        for (i64 d = 0; d < ndof; ++d) u[d] = u[d] + dt * v[d] + c * a[d];
+#pragma omp parallel for schedule(dynamic, 1)
+        for (i64 b = 0; b < nchunk; ++b) {
+            const i64 e0 = b * chunk, n = std::min(chunk, nelem - e0);
+            or_vec_as_element_dofval(conn + e0 * nne, dofs, n, nne, nd, u, ue.data() + e0 * nne * nd);
+            or_quad_grad(et, 2, n, nip, dNx + e0 * nip * nne * nd, ue.data() + e0 * nne * nd,
+                         eps.data() + e0 * nip * td * td);
+            or_elastic_stress(n * nip, td, eps.data() + e0 * nip * td * td, Kq.data() + e0 * nip,
+                              Gq.data() + e0 * nip, sig.data() + e0 * nip * td * td);
+            or_quad_int_gradN_dot_tensor2_dV(et, n, nip, dNx + e0 * nip * nne * nd, vol + e0 * nip,
+                                             sig.data() + e0 * nip * td * td, fe.data() + e0 * nne * nd);
+        }
+#pragma omp parallel for schedule(static)
+        for (i64 d = 0; d < ndof; ++d) {
+            double f = 0.0;
+            for (i64 j = ptr[d]; j < ptr[d + 1]; ++j) f += fe[idx[j]];
+            fint[d] = f;
+            double fres = (fext ? fext[d] : 0.0) - f;
+            double an = inv_mass[d] * fres;
+            v[d] = v[d] + 0.5 * dt * (a[d] + an);
+            a[d] = an;
+        }
+    }
+    if (fint_out)
+        for (i64 d = 0; d < ndof; ++d) fint_out[d] = fint[d];
+    return nthreads;
 }
 
 // ================================================================================================

@@ -10,7 +10,7 @@ from oracle import HEX8, QUAD4, QUAD4_PLANAR
 pytestmark = pytest.mark.gpu
 
 # FP32 element kernels vs FP64: norm-wise relative error bound (stated tolerance)
-TOL_F32 = 2e-6
+TOL_F32 = 1e-6
 
 
 @pytest.fixture(scope="module")
@@ -354,6 +354,131 @@ def test_explicit_fp32_rectangles(gf, nx, ny, sx, sy):
     assert relerr(host(X.fint()), fint) < TOL_F32
     gu, gv_, ga = (host(t) for t in X.state())
     assert relerr(ga, a) < TOL_F32 and relerr(gu, u) < 1e-8
+
+
+@pytest.mark.parametrize("nx,ny,shuffle,kvar,distort,sx", [(64, 64, False, False, False, 1.0),
+                                                          (130, 17, False, True, False, 0.37),
+                                                          (24, 20, True, True, False, 1.0),
+                                                          (40, 24, False, False, True, 1.0)])
+def test_explicit_fp64_fast(gf, nx, ny, shuffle, kvar, distort, sx):
+    """GF_F64_FAST (FP64 with FMA, one division per Gauss point, the closed-form rectangle body on
+    Regular coordinates) against the reference-order oracle: f_int and a within 1e-14 relative after
+    one step and after ten."""
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle, kvar=kvar, distort=distort)
+    if sx != 1.0:
+        coor = coor * np.array([sx, 1.0])
+        vec_o = oracle.Vector(conn, dofs)
+        q = oracle.Quadrature(QUAD4_PLANAR, vec_o.AsElement(coor))
+        xi, w = oracle.quad_rule(QUAD4_PLANAR, True)
+        qn = oracle.Quadrature(QUAD4_PLANAR, vec_o.AsElement(coor), xi, w)
+        inv_all = oracle.diag_inverse(oracle.diag_assemble(conn, dofs, qn.Int_N_scalar_NT_dV(np.ones((q.nelem, 4)))))
+        inv = np.where(inv == 0.0, 0.0, inv_all)
+    X = gf.ExplicitQuad4(gf.Vector(conn, dofs), coor, Ke, Ge, inv, 0.01)
+    assert X.structured == (not shuffle)
+    X.set_state(u, v, a)
+    for nsteps in (1, 9):
+        fint = oracle.vv_steps(nsteps, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a)
+        X.step(nsteps, gf.F64_FAST, store_fint=True)
+        gu, gv_, ga = (host(t) for t in X.state())
+        assert relerr(host(X.fint()), fint) < 1e-14
+        assert relerr(ga, a) < 1e-14 and relerr(gv_, v) < 1e-14 and relerr(gu, u) < 1e-14
+
+
+def _fint_extended(u, G, K, nx, ny, rel=1e-25):
+    """f_int of the Regular nx x ny (h = 1) plane-strain mesh at displacement u, in extended precision
+    (long double) with the exact geometry (J = I / 2, dN/dx = 2 dN/dxi, dV = 1 / 4), 2x2 Gauss.  Only
+    element rows touching a node with |u| > rel max|u| are evaluated: the nodes elsewhere carry
+    forces below rel of the norm.  Returns (f, node mask of the evaluated rows)."""
+    L = np.longdouble
+    U = u.reshape(ny + 1, nx + 1, 2)
+    moving = np.where(np.abs(U).max(axis=(1, 2)) > rel * np.abs(U).max())[0]
+    ey0, ey1 = max(int(moving.min()) - 1, 0), min(int(moving.max()) + 1, ny)  # element rows [ey0, ey1)
+    ex = np.arange(nx)
+    ey = np.arange(ey0, ey1)
+    n0 = (ey[:, None] * (nx + 1) + ex[None, :]).ravel()
+    nodes = np.stack((n0, n0 + 1, n0 + nx + 2, n0 + nx + 1), axis=1)  # Mesh::Quad4::Regular order
+    e = (ey[:, None] * nx + ex[None, :]).ravel()
+    ue = u.reshape(-1, 2)[nodes].astype(L)
+    KL, GL = K[e].astype(L), G[e].astype(L)
+    a = L(1) / np.sqrt(L(3))
+    sm = np.array([-1, 1, 1, -1], dtype=L)
+    tm = np.array([-1, -1, 1, 1], dtype=L)
+    f = np.zeros(ue.shape, dtype=L)
+    for xi, eta in ((-a, -a), (a, -a), (a, a), (-a, a)):
+        dx = 2 * sm * (1 + tm * eta) / 4
+        dy = 2 * tm * (1 + sm * xi) / 4
+        gxx, gyy = ue[:, :, 0] @ dx, ue[:, :, 1] @ dy
+        gxy, gyx = ue[:, :, 1] @ dx, ue[:, :, 0] @ dy
+        tr = gxx + gyy
+        sxx = KL * tr + 2 * GL * (gxx - tr / 3)
+        syy = KL * tr + 2 * GL * (gyy - tr / 3)
+        sxy = GL * (gxy + gyx)
+        f[:, :, 0] += (np.outer(sxx, dx) + np.outer(sxy, dy)) / 4
+        f[:, :, 1] += (np.outer(sxy, dx) + np.outer(syy, dy)) / 4
+    out = np.zeros(u.size, dtype=L)
+    np.add.at(out, (2 * nodes[:, :, None] + np.arange(2)).ravel(), f.ravel())
+    mask = np.zeros(u.size // 2, dtype=bool)
+    mask[ey0 * (nx + 1):(ey1 + 1) * (nx + 1)] = True
+    return out.astype(np.float64), np.repeat(mask, 2)
+
+
+def test_explicit_full_size_fp32_vs_fp64(gf):
+    """The Quad4 2048 x 2048 explicit workload (log-normal G, sigma 0.3; K 10; bottom edge fixed, top
+    edge driven at v_x 1e-4; dt 0.01), 1000 velocity-Verlet steps: the FP32 element body against the
+    FP64 reference-order path (bit-exact with the oracle on the smaller meshes above), u and f_int
+    within 1e-6 relative; the FP64 fast body within 1e-12 of it.  At the same displacement the fast
+    body is within 1e-14 of an extended-precision evaluation with exact geometry, while the
+    reference order is not: it forms the Jacobian from absolute coordinates (up to 2048), which
+    costs it about |x| eps of relative error in dN/dx."""
+    import torch
+
+    n = 2048
+    mesh = gf.Mesh.Quad4.Regular(n, n)
+    coor, conn, dofs = mesh.coor, mesh.conn, mesh.dofs
+    vec = gf.Vector(conn, dofs)
+    nodal = gf.Element.Quad4.QuadraturePlanar(vec.AsElement(coor), gf.Element.Quad4.Nodal.xi(),
+                                              gf.Element.Quad4.Nodal.w())
+    M = gf.MatrixDiagonal(vec, None)
+    M.assemble(nodal.Int_N_scalar_NT_dV(torch.ones((mesh.nelem, 4), dtype=torch.float64, device="cuda")))
+    inv = M.Solve(torch.ones(vec.ndof, dtype=torch.float64, device="cuda"))
+    del nodal
+    top = mesh.nodesTopEdge
+    fixed = np.concatenate((dofs[mesh.nodesBottomEdge].ravel(), dofs[top].ravel()))
+    inv[torch.from_numpy(fixed).cuda()] = 0.0
+    G = np.exp(0.3 * gf.randn(2, mesh.nelem)).astype(np.float32)
+    K = np.full(mesh.nelem, 10.0, dtype=np.float32)
+    u0 = np.zeros(vec.ndof)
+    v0 = np.zeros(vec.ndof)
+    v0[dofs[top, 0]] = 1e-4
+    X = gf.ExplicitQuad4(vec, coor, K, G, inv, 0.01)
+    out = {}
+    for prec in (gf.F64, gf.F32, gf.F64_FAST):
+        X.set_state(u0, v0, np.zeros(vec.ndof))
+        X.step(999, prec)
+        s999 = [host(t).copy() for t in X.state()]
+        X.step(1, prec, store_fint=True)
+        out[prec] = (host(X.state()[0]), host(X.fint()), s999)
+    u64, f64, s999 = out[gf.F64]
+    assert relerr(out[gf.F32][0], u64) < 1e-6
+    assert relerr(out[gf.F32][1], f64) < 1e-6
+    assert relerr(out[gf.F64_FAST][0], u64) < 1e-12
+    assert relerr(out[gf.F64_FAST][1], f64) < 1e-12
+    # one step of the fast body from the reference path's state: the same displacement
+    X.set_state(*s999)
+    X.step(1, gf.F64_FAST, store_fint=True)
+    assert np.array_equal(host(X.state()[0]), u64)
+    ffast = host(X.fint())
+    fext, mask = _fint_extended(u64, G.astype(np.float64), K.astype(np.float64), n, n)
+    nrm = np.linalg.norm(fext)
+    assert np.linalg.norm(f64[~mask]) < 1e-20 * nrm and np.linalg.norm(ffast[~mask]) < 1e-20 * nrm
+    e_fast = np.linalg.norm(ffast[mask] - fext[mask]) / nrm
+    e_ref = np.linalg.norm(f64[mask] - fext[mask]) / nrm
+    e_f32 = relerr(out[gf.F32][1], f64)
+    print(f"\n2048^2 x 1000: FP32 u {relerr(out[gf.F32][0], u64):.2e} f_int {e_f32:.2e}; FP64 fast u "
+          f"{relerr(out[gf.F64_FAST][0], u64):.2e} f_int {relerr(out[gf.F64_FAST][1], f64):.2e}; at u_1000 vs "
+          f"extended precision: fast {e_fast:.2e}, reference order {e_ref:.2e}")
+    assert e_fast < 1e-14
+    assert relerr(ffast, f64) < 1e-12
 
 
 def test_explicit_rejects_bad_input(gf):

@@ -164,3 +164,52 @@ def test_structured_periodic_hex8_operator_and_solve(gf):
     assert it3 == 3 and res3 > 1e-12
     _, it4, res4 = P.solve(b, x0=xd, tol=1e-10)
     assert it4 == 0 and res4 < 1e-10
+
+
+@pytest.mark.parametrize("et,n,square", [(QUAD4_PLANAR, 32, True), (QUAD4_PLANAR, 10, False), (HEX8, 8, False)])
+def test_bad_start_falls_back(gf, et, n, square):
+    """A learned (or any) start whose residual is not finite or larger than ||b|| is replaced by the
+    previous converged solution, if that one does better than zero, else by zero: NaN, 1e6-scaled
+    and random starts reach the true residual 1e-10 in the plain solve's iteration count (the zero
+    fallback repeats the plain solve bit for bit); a non-finite right-hand side fails at once."""
+    mesh, ty, P, Kp, Kup = periodic_system(gf, et, n, square=square)
+    xp = np.zeros(ty.nnp)
+    xp[ty.control[0, 1] - ty.nnu] = 1e-3
+    b = -Kup @ xp
+    xd = np.linalg.solve(Kp, b)
+    rng = np.random.default_rng(11)
+    starts = {
+        "nan": np.full(ty.nnu, np.nan),
+        "scaled": 1e6 * xd,
+        "random": np.abs(xd).max() * rng.standard_normal(ty.nnu),
+    }
+    x_plain = it_plain = None
+    for name, x0 in starts.items():
+        # a fresh solver has no previous solution: the fallback is zero, i.e. the plain solve
+        _, _, Q, _, _ = periodic_system(gf, et, n, square=square)
+        if x_plain is None:
+            xz, it_plain, _ = Q.solve(b, tol=1e-10)
+            x_plain = host(xz)
+            assert Q.start_used == gf.GF_START_GIVEN
+            _, _, Q, _, _ = periodic_system(gf, et, n, square=square)
+        x, it, res = Q.solve(b, x0=x0, tol=1e-10)
+        x = host(x)
+        assert Q.start_used == gf.GF_START_ZERO, name
+        assert res <= 1e-10 and np.linalg.norm(b - Kp @ x) / np.linalg.norm(b) < 1e-9, name
+        assert it == it_plain and np.array_equal(x, x_plain), name
+    # with a previous solution: a doubled load from a NaN start restarts from that solution, whose
+    # residual is half of ||b||, and needs no more iterations than the plain solve
+    P.solve(b, tol=1e-10)
+    x, it, res = P.solve(2.0 * b, x0=np.full(ty.nnu, np.nan), tol=1e-10)
+    assert P.start_used == gf.GF_START_PREVIOUS
+    assert res <= 1e-10 and it <= it_plain
+    assert np.linalg.norm(host(x) - 2.0 * xd) / np.linalg.norm(2.0 * xd) < 1e-6
+    # a good start is kept
+    P.solve(b, x0=xd + 1e-6 * np.abs(xd).max() * rng.standard_normal(ty.nnu), tol=1e-10)
+    assert P.start_used == gf.GF_START_GIVEN
+    # a non-finite right-hand side is rejected before any iteration
+    bad = b.copy()
+    bad[3] = np.inf
+    with pytest.raises(gf.GooseFEMError) as e:
+        P.solve(bad, tol=1e-10)
+    assert e.value.code == gf.GF_EINVAL

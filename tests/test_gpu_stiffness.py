@@ -11,7 +11,7 @@ from oracle import HEX8
 pytestmark = pytest.mark.gpu
 
 # BF16x3 (hi*hi + hi*lo + lo*hi, FP32 accumulation) vs FP64: norm-wise relative error bound
-TOL_BF16X3 = 2e-5
+TOL_BF16X3 = 1e-5
 
 
 @pytest.fixture(scope="module")

@@ -467,3 +467,26 @@ def test_randn_moments_and_determinism():
     z = oracle.randn(7, 200001)
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
     assert np.array_equal(z[:1000], oracle.randn(7, 1000))
+
+
+def test_vv_steps_parallel_bit_identical():
+    """The all-cores CPU baseline (OpenMP element chunks + ordered per-DOF gather) repeats the serial
+    restatement bit for bit, on a renumbered mesh as well."""
+    nx, ny = 37, 21
+    coor, conn = oracle.quad4_regular(nx, ny)
+    rng = np.random.default_rng(4)
+    conn = conn[rng.permutation(conn.shape[0])]
+    dofs = oracle.dofs(coor.shape[0], 2)[rng.permutation(coor.shape[0])]
+    q = oracle.Quadrature(QUAD4_PLANAR, oracle.Vector(conn, dofs).AsElement(coor))
+    E = conn.shape[0]
+    K = np.full(E, 10.0)
+    G = np.exp(0.3 * rng.standard_normal(E))
+    inv = 1.0 / (1.0 + rng.random(2 * coor.shape[0]))
+    u0 = 1e-3 * rng.standard_normal(inv.size)
+    out = []
+    for fn in (oracle.vv_steps, oracle.vv_steps_parallel):
+        u, v, a = u0.copy(), np.zeros_like(u0), np.zeros_like(u0)
+        r = fn(5, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), K, G, inv, 0.01, u, v, a)
+        out.append((u, v, a, r[0] if isinstance(r, tuple) else r))
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
