@@ -763,6 +763,7 @@ def main():
     ap.add_argument("--no-blocks", action="store_true",
                     help="skip the learned-PCG (512^2 periodic) and Hex8 stiffness (128^3) blocks")
     ap.add_argument("--no-general", action="store_true", help="skip the general-mesh explicit row")
+    ap.add_argument("--no-errors", action="store_true", help="skip the FP32 / FP64-fast error trajectories (profiling)")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned",
                                            "plastic-refresh", "newton-q4", "statics-q4", "periodic-j2"],
@@ -834,7 +835,7 @@ def main():
     # errors of the FP32 and FP64-fast element bodies against the FP64 reference-order path over the
     # workload's 1000 steps (and over --steps when that differs)
     err = {}
-    for nst in sorted({1000, args.steps}):
+    for nst in ([] if args.no_errors else sorted({1000, args.steps})):
         traj = {}
         for p_ in (gf.F64, gf.F32, gf.F64_FAST):
             X.set_state(S["u0"], S["v0"], S["a0"])
@@ -846,7 +847,7 @@ def main():
         err[str(nst)] = {"f32": {"u_rel": rel(traj[gf.F32][0], u64), "fint_rel": rel(traj[gf.F32][1], f64)},
                          "f64_fast": {"u_rel": rel(traj[gf.F64_FAST][0], u64),
                                       "fint_rel": rel(traj[gf.F64_FAST][1], f64)}}
-    main_err = err["1000"]["f32" if prec == gf.F32 else "f64_fast"]
+    main_err = err["1000"]["f32" if prec == gf.F32 else "f64_fast"] if err else {"u_rel": None, "fint_rel": None}
 
     # timed runs: warmup, then exactly `steps` steps, for each precision
     timing = {}
@@ -862,7 +863,7 @@ def main():
     achieved = B / per_launch / 1e9
     value = ndof * args.steps / wall
     traffic = load_traffic(nx)
-    ceiling = copy_ceiling(gf)
+    ceiling, ceilings = copy_ceiling(gf)
     name = {gf.F64: "f64", gf.F32: "f32", gf.F64_FAST: "f64_fast"}
     legs = {}
     for p_, (w_, k_) in timing.items():
@@ -897,7 +898,7 @@ def main():
                      # are generated, a uniform K is a scalar, no connectivity is read) over its time
                      "moved_GBs": traffic / per_launch / 1e9 if traffic else None,
                      "frac_moved": traffic / per_launch / 1e9 / HBM_PEAK_GBS if traffic else None,
-                     "copy_ceiling_GBs": ceiling,
+                     "copy_ceiling_GBs": ceiling, "stream_probes_GBs": ceilings,
                      "frac_of_copy_ceiling": achieved / ceiling,
                      "frac_moved_of_copy_ceiling": traffic / per_launch / 1e9 / ceiling if traffic else None},
         "other_precision": legs,
@@ -939,50 +940,63 @@ def main():
 
 
 def copy_ceiling(gf, gib=4):
-    """Measured copy ceiling (GB/s, read + write bytes): the best of the plain and the non-temporal
-    variant of the library's streaming copy kernel over a 4 GiB buffer."""
+    """Measured HBM ceilings (GB/s of bytes read + written) of the library's streaming probes over
+    4 GiB arrays: copy (best of plain and non-temporal), add (2 reads : 1 write) and read-only.
+    Returns (ceiling, all): the ceiling is the faster of copy and add, the two mixes that bracket the
+    explicit step's 1.4 : 1 read : write traffic."""
     import torch
 
     n = (gib << 30) // 8
-    src = torch.ones(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    best = 0.0
-    for mode in (0, 1):
-        t = _timed(lambda: gf._call("gf_copy_stream", gf._dp(src), gf._dp(dst), n, mode, gf._stream()), 10)
-        best = max(best, 2 * 8 * n / t / 1e9)
-    del src, dst
-    return best
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.ones(n, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+    rate = {}
+    for mode, name, nbytes in ((0, "copy", 16 * n), (1, "copy_nt", 16 * n), (2, "add", 24 * n), (3, "read", 8 * n)):
+        t = _timed(lambda: gf._call("gf_stream_probe", mode, gf._dp(a), gf._dp(b), gf._dp(c), n, gf._stream()), 10)
+        rate[name] = nbytes / t / 1e9
+    del a, b, c
+    torch.cuda.empty_cache()
+    rate["copy"] = max(rate["copy"], rate.pop("copy_nt"))
+    return max(rate["copy"], rate["add"]), rate
 
 
 def explicit_general(nx, ny, steps, warmup):
-    """The general-mesh explicit path (displacement update, element kernel with geometry recomputed
-    from the coordinates, ordered per-DOF gather + update) on the same Regular mesh with its elements
-    renumbered (which defeats the structured-grid detection), FP32 and FP64 element bodies."""
+    """The general-mesh explicit path (element kernel with the geometry recomputed from the
+    coordinates over the elements in locality order, then the ordered per-DOF gather + update) on
+    (1) the same Regular mesh with its elements renumbered by a random permutation (which defeats
+    the structured-grid detection) and (2) a FineLayer mesh of about the same size (3.95 M
+    elements, coarsening layers above and below 1800 fine layers).  Algorithmic bytes per step:
+    64 ndof + 20 nelem, as for the headline."""
     import torch
 
     import goosefem_amd as gf
 
-    mesh = gf.Mesh.Quad4.Regular(nx, ny)
-    coor, dofs = mesh.coor, mesh.dofs
-    conn = mesh.conn[np.random.default_rng(5).permutation(mesh.nelem)]
-    vec = gf.Vector(conn, dofs)
-    inv = np.ones(vec.ndof)
-    G = np.exp(0.3 * gf.randn(2, mesh.nelem)).astype(np.float32)
-    K = np.full(mesh.nelem, 10.0, dtype=np.float32)
-    X = gf.ExplicitQuad4(vec, coor, K, G, inv, 0.01)
-    assert not X.structured
-    u0 = 1e-3 * gf.randn(1, vec.ndof)
-    B = canonical_bytes_per_step(nx, ny)
-    out = {"workload": f"Quad4 Regular {nx}x{ny}, elements renumbered (random permutation), general path",
-           "algorithmic_bytes_per_step": B}
-    for p_, nm in ((gf.F32, "f32"), (gf.F64, "f64"), (gf.F64_FAST, "f64_fast")):
-        X.set_state(u0, np.zeros(vec.ndof), np.zeros(vec.ndof))
-        X.step(warmup, p_)
-        wall, kt = time_steps(X, steps, p_, gf)
-        out[nm] = {"ms_per_step": kt / steps * 1e3, "DOF_updates_per_s": vec.ndof * steps / wall,
-                   "achieved_GBs": B / (kt / steps) / 1e9, "frac": B / (kt / steps) / 1e9 / HBM_PEAK_GBS}
-    del X, vec
-    torch.cuda.empty_cache()
+    out = {}
+    meshes = (("renumbered", gf.Mesh.Quad4.Regular(nx, ny), True),
+              ("finelayer", gf.Mesh.Quad4.FineLayer(2187, 2400, 1.0, 1800), False))
+    for key, mesh, shuffle in meshes:
+        coor, dofs = mesh.coor, mesh.dofs
+        conn = mesh.conn[np.random.default_rng(5).permutation(mesh.nelem)] if shuffle else mesh.conn
+        vec = gf.Vector(conn, dofs)
+        inv = np.ones(vec.ndof)
+        G = np.exp(0.3 * gf.randn(2, mesh.nelem)).astype(np.float32)
+        K = np.full(mesh.nelem, 10.0, dtype=np.float32)
+        X = gf.ExplicitQuad4(vec, coor, K, G, inv, 0.01)
+        assert not X.structured
+        u0 = 1e-3 * gf.randn(1, vec.ndof)
+        B = 64 * vec.ndof + 20 * mesh.nelem
+        row = {"nelem": mesh.nelem, "ndof": vec.ndof, "algorithmic_bytes_per_step": B}
+        for p_, nm in ((gf.F32, "f32"), (gf.F64, "f64"), (gf.F64_FAST, "f64_fast")):
+            X.set_state(u0, np.zeros(vec.ndof), np.zeros(vec.ndof))
+            X.step(warmup, p_)
+            wall, kt = time_steps(X, steps, p_, gf)
+            row[nm] = {"ms_per_step": kt / steps * 1e3, "DOF_updates_per_s": vec.ndof * steps / wall,
+                       "achieved_GBs": B / (kt / steps) / 1e9, "frac": B / (kt / steps) / 1e9 / HBM_PEAK_GBS}
+        out[key] = row
+        del X, vec
+        torch.cuda.empty_cache()
+    out["workload"] = (f"general path: Quad4 Regular {nx}x{ny} with elements renumbered (random permutation); "
+                       "Quad4 FineLayer(2187, 2400, nfine 1800)")
     return out
 
 

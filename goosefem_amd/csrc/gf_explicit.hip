@@ -17,9 +17,10 @@
 // Tiles are ordered so each XCD sweeps a contiguous band of the mesh (its L2 sees the halo rows
 // its own previous tiles loaded).
 //
-// General path (any Quad4 mesh / dof numbering): three kernels per step -- displacement update,
-// element kernel (gather of coordinates and displacement, geometry recomputed, strain + stress +
-// force in FP64 reference order, FP32, or FP64 with FMA), DOF-centric ordered gather + update.
+// General path (any Quad4 mesh / dof numbering): two kernels per step -- element kernel (gather of
+// coordinates and displacement, geometry recomputed, strain + stress + force in FP64 reference
+// order, FP32, or FP64 with FMA) over the elements sorted by their lowest node, then the
+// DOF-centric ordered gather + update, which also forms the next step's displacement.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -47,7 +48,12 @@ struct gf_explicit_s {
     int cur = 0;
     DBuf<double> fint;
     bool fint_valid = false;
-    // general path: element forces of the last step, [nelem, 4, 2]
+    // general path: elements in locality order (sorted by their lowest node), with their
+    // connectivity and moduli in that order, and the per-DOF gather lists pointing at the element
+    // forces of that order (each list still in the reference's (e, m, i) order); element forces of
+    // the last step, [nelem, 4, 2]
+    DBuf<int32_t> gconn, geidx;
+    DBuf<float> gK, gG;
     DBuf<double> fe64;
     DBuf<float> fe32;
 };
@@ -468,21 +474,40 @@ __global__ void __launch_bounds__(BS) k_element_general(int64_t nelem, ShapeTabl
     out[1] = v4{f[2][0], f[2][1], f[3][0], f[3][1]};
 }
 
+// Ordered per-DOF gather of the element forces (the reference's serial scatter order) and the
+// velocity-Verlet update of v and a; unless this is the last step of the call, also the next
+// step's displacement u_next = u + dt v_new + c a_new (the displacement update of the next step,
+// same expression order as k_update_u).
 template <class T>
-__global__ void k_node_update(int64_t ndof, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
-                              const T* __restrict__ fe, const double* __restrict__ inv_mass,
-                              const double* __restrict__ fext, double hdt, const double* __restrict__ v0,
-                              const double* __restrict__ a0, double* __restrict__ v1, double* __restrict__ a1,
-                              double* __restrict__ fint_out)
+__global__ void __launch_bounds__(BS) k_node_update(int64_t ndof, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ idx, const T* __restrict__ fe,
+                                                    const double* __restrict__ inv_mass, const double* __restrict__ fext,
+                                                    double dt, double c, double hdt, const double* __restrict__ v0,
+                                                    const double* __restrict__ a0, double* __restrict__ v1,
+                                                    double* __restrict__ a1, const double* __restrict__ u,
+                                                    double* __restrict__ u_next, double* __restrict__ fint_out)
 {
     int64_t d = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (d >= ndof) return;
     double s = 0.0;
     for (int32_t j = ptr[d]; j < ptr[d + 1]; ++j) s += static_cast<double>(fe[idx[j]]);
     double an = inv_mass[d] * ((fext ? fext[d] : 0.0) - s);
-    v1[d] = v0[d] + hdt * (a0[d] + an);
+    double vn = v0[d] + hdt * (a0[d] + an);
+    v1[d] = vn;
     a1[d] = an;
+    if (u_next) u_next[d] = u[d] + dt * vn + c * an;
     if (fint_out) fint_out[d] = s;
+}
+
+// gather list entry k = (e nne + m) nd + i of the original element e -> the same entry of e's
+// position rank[e] in the locality order
+__global__ void k_remap_eidx(int64_t n, const int32_t* __restrict__ idx, const int32_t* __restrict__ rank,
+                             int32_t* __restrict__ out)
+{
+    int64_t j = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (j >= n) return;
+    const int32_t k = idx[j];
+    out[j] = rank[k / 8] * 8 + k % 8;
 }
 
 ShapeTable gauss_table()
@@ -647,8 +672,38 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
             }
             if (!x->structured) {
                 GF_REQUIRE(v->nelem < (int64_t(1) << 28), "general path: nelem < 2^28");
-                x->fe64.alloc(v->nelem * 8);
-                x->fe32.alloc(v->nelem * 8);
+                const int64_t E = v->nelem;
+                // locality order: elements sorted by their lowest node (stable)
+                std::vector<int32_t> order(E), rank(E), cp(E * 4);
+                std::vector<int64_t> key(E);
+                for (int64_t e = 0; e < E; ++e) {
+                    order[e] = static_cast<int32_t>(e);
+                    key[e] = *std::min_element(v->conn_h.begin() + 4 * e, v->conn_h.begin() + 4 * e + 4);
+                }
+                std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+                for (int64_t j = 0; j < E; ++j) {
+                    rank[order[j]] = static_cast<int32_t>(j);
+                    for (int m = 0; m < 4; ++m) cp[4 * j + m] = static_cast<int32_t>(v->conn_h[4 * order[j] + m]);
+                }
+                x->gconn.upload(cp);
+                std::vector<float> kh(E), gh(E), kp(E), gp(E);
+                GF_HIP(hipMemcpy(kh.data(), K_e, E * sizeof(float), hipMemcpyDeviceToHost));
+                GF_HIP(hipMemcpy(gh.data(), G_e, E * sizeof(float), hipMemcpyDeviceToHost));
+                for (int64_t j = 0; j < E; ++j) {
+                    kp[j] = kh[order[j]];
+                    gp[j] = gh[order[j]];
+                }
+                x->gK.upload(kp);
+                x->gG.upload(gp);
+                DBuf<int32_t> drank;
+                drank.upload(rank);
+                const int64_t nidx = E * 8;
+                x->geidx.alloc(nidx);
+                k_remap_eidx<<<nblocks(nidx, BS), BS>>>(nidx, v->dof_eidx.p, drank.p, x->geidx.p);
+                check_launch();
+                GF_HIP(hipDeviceSynchronize());
+                x->fe64.alloc(E * 8);
+                x->fe32.alloc(E * 8);
             }
             GF_HIP(hipDeviceSynchronize());
         }
@@ -736,28 +791,33 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
         else {
             auto v = x->v;
             const unsigned ge = nblocks(x->nelem, BS), gd = nblocks(x->ndof, BS);
+            // the displacement of the first step; each node update then writes the next one
+            if (nsteps > 0)
+                k_update_u<<<gd, BS, 0, s>>>(x->ndof, dt, c, x->u[x->cur].p, x->vv[x->cur].p, x->a[x->cur].p,
+                                            x->u[1 - x->cur].p);
             for (int64_t k = 0; k < nsteps; ++k) {
                 const int i0 = x->cur, i1 = 1 - x->cur;
-                double* fo = (store_fint && k == nsteps - 1) ? x->fint.p : nullptr;
-                k_update_u<<<gd, BS, 0, s>>>(x->ndof, dt, c, x->u[i0].p, x->vv[i0].p, x->a[i0].p, x->u[i1].p);
-                const double* un = x->u[i1].p;
+                const bool last = k == nsteps - 1;
+                double* fo = (store_fint && last) ? x->fint.p : nullptr;
+                const double* un = x->u[i1].p; // this step's displacement
+                double* unext = last ? nullptr : x->u[i0].p;
                 if (precision == GF_F32) {
-                    k_element_general<B_F32><<<ge, BS, 0, s>>>(x->nelem, T, v->conn.p, v->dofs.p, x->coor.p, x->Ke.p,
-                                                                x->Ge.p, un, x->fe32.p);
-                    k_node_update<float><<<gd, BS, 0, s>>>(x->ndof, v->dof_eptr.p, v->dof_eidx.p, x->fe32.p,
-                                                           x->inv_mass.p, fext, hdt, x->vv[i0].p, x->a[i0].p,
-                                                           x->vv[i1].p, x->a[i1].p, fo);
+                    k_element_general<B_F32><<<ge, BS, 0, s>>>(x->nelem, T, x->gconn.p, v->dofs.p, x->coor.p, x->gK.p,
+                                                                x->gG.p, un, x->fe32.p);
+                    k_node_update<float><<<gd, BS, 0, s>>>(x->ndof, v->dof_eptr.p, x->geidx.p, x->fe32.p,
+                                                           x->inv_mass.p, fext, dt, c, hdt, x->vv[i0].p, x->a[i0].p,
+                                                           x->vv[i1].p, x->a[i1].p, un, unext, fo);
                 }
                 else {
                     if (precision == GF_F64)
-                        k_element_general<B_REF64><<<ge, BS, 0, s>>>(x->nelem, T, v->conn.p, v->dofs.p, x->coor.p,
-                                                                      x->Ke.p, x->Ge.p, un, x->fe64.p);
+                        k_element_general<B_REF64><<<ge, BS, 0, s>>>(x->nelem, T, x->gconn.p, v->dofs.p, x->coor.p,
+                                                                      x->gK.p, x->gG.p, un, x->fe64.p);
                     else
-                        k_element_general<B_F64><<<ge, BS, 0, s>>>(x->nelem, T, v->conn.p, v->dofs.p, x->coor.p,
-                                                                    x->Ke.p, x->Ge.p, un, x->fe64.p);
-                    k_node_update<double><<<gd, BS, 0, s>>>(x->ndof, v->dof_eptr.p, v->dof_eidx.p, x->fe64.p,
-                                                            x->inv_mass.p, fext, hdt, x->vv[i0].p, x->a[i0].p,
-                                                            x->vv[i1].p, x->a[i1].p, fo);
+                        k_element_general<B_F64><<<ge, BS, 0, s>>>(x->nelem, T, x->gconn.p, v->dofs.p, x->coor.p,
+                                                                    x->gK.p, x->gG.p, un, x->fe64.p);
+                    k_node_update<double><<<gd, BS, 0, s>>>(x->ndof, v->dof_eptr.p, x->geidx.p, x->fe64.p,
+                                                            x->inv_mass.p, fext, dt, c, hdt, x->vv[i0].p, x->a[i0].p,
+                                                            x->vv[i1].p, x->a[i1].p, un, unext, fo);
                 }
                 check_launch();
                 x->cur = i1;

@@ -1,7 +1,11 @@
-// Streaming copy: the measured HBM ceiling that the bandwidth-bound kernels' roofline fractions are
-// reported against next to the datasheet peak.  No counterpart in the reference (measurement
-// utility).  Each thread moves 4 x 16 bytes per grid-stride step with full-wave contiguous 16-byte
-// loads and stores; mode 1 uses non-temporal (streaming) loads and stores.
+// Streaming probes: the measured HBM ceilings that the bandwidth-bound kernels' roofline fractions
+// are reported against next to the datasheet peak.  No counterpart in the reference (measurement
+// utility).  Each workgroup streams one contiguous 16 KiB chunk per array with full-wave
+// contiguous 16-byte accesses (4 per thread, 256 threads), one pass over the arrays.
+//   mode 0  copy   c = a          (1 read : 1 write)
+//   mode 1  copy   c = a          non-temporal loads and stores
+//   mode 2  add    c = a + b      (2 reads : 1 write), non-temporal
+//   mode 3  read   c[block] = sum of the block's chunk of a   (read only)
 #include "gf_internal.hpp"
 
 using namespace gf;
@@ -9,27 +13,58 @@ using namespace gf;
 namespace {
 
 typedef double dvec2 __attribute__((ext_vector_type(2)));
+constexpr int TPB = 256, U = 4, CHUNK = TPB * U; // dvec2 per workgroup
 
 template <bool NT>
-__global__ void __launch_bounds__(256) k_copy_stream(int64_t n2, const dvec2* __restrict__ src, dvec2* __restrict__ dst)
+__device__ inline dvec2 ld(const dvec2* p)
 {
-    constexpr int U = 4;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * 256;
-    int64_t i = blockIdx.x * static_cast<int64_t>(256) + threadIdx.x;
-    for (; i + (U - 1) * stride < n2; i += U * stride) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <bool NT>
+__device__ inline void st(dvec2 v, dvec2* p)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(TPB) k_stream(int64_t n2, const dvec2* __restrict__ a, const dvec2* __restrict__ b,
+                                                dvec2* __restrict__ c)
+{
+    constexpr bool NT = MODE != 0;
+    const int64_t base = blockIdx.x * static_cast<int64_t>(CHUNK) + threadIdx.x;
+    if (base + (U - 1) * TPB < n2) {
         dvec2 v[U];
 #pragma unroll
-        for (int k = 0; k < U; ++k) v[k] = NT ? __builtin_nontemporal_load(src + i + k * stride) : src[i + k * stride];
+        for (int k = 0; k < U; ++k) v[k] = ld<NT>(a + base + k * TPB);
+        if constexpr (MODE == 2) {
+            dvec2 w[U];
 #pragma unroll
-        for (int k = 0; k < U; ++k) {
-            if (NT) __builtin_nontemporal_store(v[k], dst + i + k * stride);
-            else dst[i + k * stride] = v[k];
+            for (int k = 0; k < U; ++k) w[k] = ld<NT>(b + base + k * TPB);
+#pragma unroll
+            for (int k = 0; k < U; ++k) v[k] += w[k];
         }
+        if constexpr (MODE == 3) {
+            dvec2 t = v[0] + v[1] + v[2] + v[3];
+            if (t.x == 12345.678) c[blockIdx.x] = t; // keeps the loads; the probe's data never matches
+        }
+        else {
+#pragma unroll
+            for (int k = 0; k < U; ++k) st<NT>(v[k], c + base + k * TPB);
+        }
+        return;
     }
-    for (; i < n2; i += stride) {
-        const dvec2 v = NT ? __builtin_nontemporal_load(src + i) : src[i];
-        if (NT) __builtin_nontemporal_store(v, dst + i);
-        else dst[i] = v;
+    for (int k = 0; k < U; ++k) {
+        const int64_t i = base + k * TPB;
+        if (i >= n2) break;
+        dvec2 v = ld<NT>(a + i);
+        if constexpr (MODE == 2) v += ld<NT>(b + i);
+        if constexpr (MODE == 3) {
+            if (v.x == 12345.678) c[blockIdx.x] = v;
+        }
+        else st<NT>(v, c + i);
     }
 }
 
@@ -37,20 +72,26 @@ __global__ void __launch_bounds__(256) k_copy_stream(int64_t n2, const dvec2* __
 
 extern "C" {
 
-int gf_copy_stream(const double* src, double* dst, int64_t n, int mode, void* stream)
+int gf_stream_probe(int mode, const double* a, const double* b, double* c, int64_t n, void* stream)
 {
     return guard([&] {
-        GF_REQUIRE(src && dst && n >= 0 && n % 2 == 0, "src, dst non-null; n >= 0 and even");
-        GF_REQUIRE(aligned16(src) && aligned16(dst), "16-byte aligned buffers");
-        GF_REQUIRE(mode == 0 || mode == 1, "mode 0 (plain) or 1 (non-temporal)");
+        GF_REQUIRE(mode >= 0 && mode <= 3, "mode 0 (copy), 1 (copy, non-temporal), 2 (add), 3 (read)");
+        GF_REQUIRE(a && c && (mode != 2 || b) && n >= 0 && n % 2 == 0, "a, c (and b for add) non-null; n even");
+        GF_REQUIRE(aligned16(a) && aligned16(c) && (mode != 2 || aligned16(b)), "16-byte aligned arrays");
         if (n == 0) return;
         const int64_t n2 = n / 2;
-        // 8 workgroups per CU of 256 CUs, grid-stride over the rest
-        const unsigned grid = static_cast<unsigned>(std::min<int64_t>(nblocks(n2, 256), 2048));
-        if (mode) k_copy_stream<true><<<grid, 256, 0, S(stream)>>>(n2, reinterpret_cast<const dvec2*>(src),
-                                                                   reinterpret_cast<dvec2*>(dst));
-        else k_copy_stream<false><<<grid, 256, 0, S(stream)>>>(n2, reinterpret_cast<const dvec2*>(src),
-                                                               reinterpret_cast<dvec2*>(dst));
+        const int64_t grid = (n2 + CHUNK - 1) / CHUNK;
+        GF_REQUIRE(grid < (int64_t(1) << 31), "n too large");
+        auto A = reinterpret_cast<const dvec2*>(a);
+        auto B = reinterpret_cast<const dvec2*>(b);
+        auto C = reinterpret_cast<dvec2*>(c);
+        const unsigned g = static_cast<unsigned>(grid);
+        switch (mode) {
+            case 0: k_stream<0><<<g, TPB, 0, S(stream)>>>(n2, A, B, C); break;
+            case 1: k_stream<1><<<g, TPB, 0, S(stream)>>>(n2, A, B, C); break;
+            case 2: k_stream<2><<<g, TPB, 0, S(stream)>>>(n2, A, B, C); break;
+            default: k_stream<3><<<g, TPB, 0, S(stream)>>>(n2, A, B, C); break;
+        }
         check_launch();
     });
 }

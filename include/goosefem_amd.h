@@ -74,10 +74,12 @@ int gf_device_free(void* ptr);
 int gf_memcpy_h2d(void* dst, const void* src_h, int64_t bytes);
 int gf_memcpy_d2h(void* dst_h, const void* src, int64_t bytes);
 int gf_memcpy_d2d(void* dst, const void* src, int64_t bytes);
-/* Streaming device copy of n doubles (n even, 16-byte aligned): the measured copy ceiling that
- * roofline fractions are reported against beside the datasheet peak.  mode 0: plain loads and
- * stores; 1: non-temporal.  No counterpart in the reference (measurement utility). */
-int gf_copy_stream(const double* src, double* dst, int64_t n, int mode, void* stream);
+/* Streaming probes over n doubles (n even, 16-byte aligned arrays of n doubles each): the measured
+ * HBM ceilings that roofline fractions are reported against beside the datasheet peak.  mode 0:
+ * c = a; 1: c = a with non-temporal accesses; 2: c = a + b (non-temporal); 3: read a only (c is
+ * written only for data that the probe never uses).  No counterpart in the reference (measurement
+ * utility). */
+int gf_stream_probe(int mode, const double* a, const double* b, double* c, int64_t n, void* stream);
 
 /* ================================================================================================
  * Mesh bookkeeping (host, set-up only).  Integer outputs are bit-exact with the reference.
