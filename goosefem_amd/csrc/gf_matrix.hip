@@ -14,10 +14,18 @@
 // SpMV: lanes form the products val*x in parallel, one lane sums them in ascending column order
 // (bit-identical to a sequential row loop).
 #include <algorithm>
+#include <mutex>
 
 #include "gf_internal.hpp"
 
 using namespace gf;
+
+// Staging of gf_matrix_dot_nodevec_noalloc: the dofval copies of x and of A x (2 x at most 2^20
+// doubles, 16 MiB).  Every device the library runs on holds its own copy for the life of the
+// process.  In host code `scratch` is a host-side stand-in, not a device address: the device
+// address comes from hipGetSymbolAddress (scratch_ptr below).
+constexpr int64_t NOALLOC_MAX_DOFS = int64_t(1) << 20;
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_DOFS];
 
 struct gf_diag_s {
     gf_vector v = nullptr;
@@ -568,6 +576,55 @@ int gf_matrix_dot_nodevec(gf_matrix A, const double* x, double* b, void* stream)
         rc = gf_vector_as_node_dofval(A->v, bd.p, b, stream);
         if (rc) throw Error(rc, gf_last_error());
         GF_HIP(hipStreamSynchronize(S(stream)));
+    });
+}
+
+} // extern "C"
+
+namespace {
+
+std::mutex g_scratch_mu;
+constexpr int MAX_DEVICES = 64;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per device (with that device
+// current) and kept.
+double* scratch_ptr()
+{
+    int dev = 0;
+    GF_HIP(hipGetDevice(&dev));
+    GF_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "device index below 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        GF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+} // namespace
+
+extern "C" {
+
+int gf_matrix_dot_nodevec_noalloc(gf_matrix A, const double* x, double* b, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(A && x && b, "null argument");
+        if (!A->assembled) throw Error(GF_ESTATE, "matrix not assembled");
+        if (A->nrows > NOALLOC_MAX_DOFS)
+            throw Error(GF_EINVAL, "gf_matrix_dot_nodevec_noalloc: " + std::to_string(A->nrows) +
+                                       " DOFs exceed the static staging of " + std::to_string(NOALLOC_MAX_DOFS) +
+                                       " (use gf_matrix_dot_nodevec)");
+        double* xd = scratch_ptr();
+        double* bd = xd + NOALLOC_MAX_DOFS;
+        // the same three kernels as gf_matrix_dot_nodevec; k_row_last writes every DOF of xd, and
+        // the product every row of bd, so nothing left by an earlier call is read
+        int rc = gf_vector_as_dofs_nodevec(A->v, x, xd, stream);
+        if (rc) throw Error(rc, gf_last_error());
+        spmv(A, xd, bd, S(stream));
+        rc = gf_vector_as_node_dofval(A->v, bd, b, stream);
+        if (rc) throw Error(rc, gf_last_error());
     });
 }
 

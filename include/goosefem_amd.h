@@ -446,6 +446,13 @@ int gf_matrix_csr(gf_matrix A, const int32_t** rowptr, const int32_t** col, cons
 /* Dot(x)  (Matrix.h:804-815): b = A x on dofval; nodevec variant via AsDofs / AsNode */
 int gf_matrix_dot_dofval(gf_matrix A, const double* x, double* b, void* stream);
 int gf_matrix_dot_nodevec(gf_matrix A, const double* x, double* b, void* stream);
+/* gf_matrix_dot_nodevec without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The dofval staging of x and A x lives in a static device array of the
+ * library, 2 x 2^20 doubles (16 MiB) per device: matrices above 2^20 rows fail with GF_EINVAL.
+ * Calls that overlap on one GPU share that array and race; the stream must belong to the current
+ * device.  Results are bit-identical to gf_matrix_dot_nodevec, which allocates and synchronises. */
+int gf_matrix_dot_nodevec_noalloc(gf_matrix A, const double* x, double* b, void* stream);
 /* Todense()  (Matrix.h:338-355, :790-802): out [nrows, nrows] */
 int gf_matrix_todense(gf_matrix A, double* out, void* stream);
 
