@@ -12,7 +12,7 @@ peak and against a measured copy ceiling, the general-mesh path, the one-thread 
 baselines, and the metric's second half: learned vs plain PCG on the periodic Quad4 512^2 cell
 (pcg_learned) and the Hex8 128^3 element stiffness on MFMA (hex8_stiffness).  Other workloads:
 --workload hex8-stiffness | pcg-periodic | pcg-learned | plastic-refresh | newton-q4 | statics-q4 |
-periodic-j2.
+periodic-j2 | quad-api.
 """
 
 from __future__ import annotations
@@ -766,7 +766,7 @@ def main():
     ap.add_argument("--no-errors", action="store_true", help="skip the FP32 / FP64-fast error trajectories (profiling)")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--workload", choices=["quad4-explicit", "hex8-stiffness", "pcg-periodic", "pcg-learned",
-                                           "plastic-refresh", "newton-q4", "statics-q4", "periodic-j2"],
+                                           "plastic-refresh", "newton-q4", "statics-q4", "periodic-j2", "quad-api"],
                     default="quad4-explicit")
     ap.add_argument("--pcg-n", type=int, default=512)
     ap.add_argument("--pcg-hex", action="store_true", help="periodic Hex8 n^3 instead of Quad4 n^2")
@@ -823,6 +823,13 @@ def main():
         return
     if args.workload == "periodic-j2":
         bench_periodic_j2(args)
+        return
+    if args.workload == "quad-api":
+        import goosefem_amd as gf
+
+        ceiling, probes = copy_ceiling(gf)
+        print(json.dumps({"quad4": api_kernels("quad4", args.nx), "hex8": api_kernels("hex8", args.hex_n),
+                          "copy_ceiling_GBs": ceiling, "stream_probes_GBs": probes}))
         return
 
     nx = ny = args.nx
@@ -930,6 +937,7 @@ def main():
             if "cpu_baseline" in pl:
                 res["pcg_learned"]["cpu_baseline"] = pl["cpu_baseline"]
             torch.cuda.empty_cache()
+        res["api_kernels"] = {"quad4": api_kernels("quad4", nx), "hex8": api_kernels("hex8", args.hex_n)}
         hx = hex8_stiffness(args.hex_n, cpu=not args.no_cpu, csr=False, cpu_seconds=5.0)
         res["hex8_stiffness"] = {k: hx[k] for k in ("value", "unit", "config", "roofline", "mfma_busy_est",
                                                     "fp64_stiffness", "fp32_io", "tangent_ms", "error_vs_fp64")}
@@ -997,6 +1005,58 @@ def explicit_general(nx, ny, steps, warmup):
         torch.cuda.empty_cache()
     out["workload"] = (f"general path: Quad4 Regular {nx}x{ny} with elements renumbered (random permutation); "
                        "Quad4 FineLayer(2187, 2400, nfine 1800)")
+    return out
+
+
+def api_kernels(kind, n):
+    """The unfused quadrature calls of the reference's API (SymGradN_vector, GradN_vector,
+    Int_gradN_dot_tensor2_dV; Element.h:634-717, 807-825) on Quad4 planar n x n or Hex8 n^3, FP64
+    and FP32 element data, each timed with HIP events.  Algorithmic bytes per element (FP64; half
+    for FP32): Quad4 gradient dN/dx 256 + elemvec 64 read, qtensor 288 written; integral dN/dx 256 +
+    dV 32 + qtensor 288 read, elemvec 64 written.  Hex8: 1536 + 192 / 576 and 1536 + 64 + 576 / 192."""
+    import torch
+
+    import goosefem_amd as gf
+
+    mesh = gf.Mesh.Hex8.Regular(n, n, n) if kind == "hex8" else gf.Mesh.Quad4.Regular(n, n)
+    vec = gf.Vector(mesh.conn, mesh.dofs)
+    x = vec.AsElement(mesh.coor)
+    quad = gf.Element.Hex8.Quadrature(x) if kind == "hex8" else gf.Element.Quad4.QuadraturePlanar(x)
+    del x
+    E, nip, nne, nd = mesh.nelem, quad.nip, quad.nne, quad.ndim
+    bytes_grad = nip * nne * nd * 8 + nne * nd * 8 + nip * 9 * 8
+    bytes_int = nip * nne * nd * 8 + nip * 8 + nip * 9 * 8 + nne * nd * 8
+    out = {"workload": f"{'Hex8 ' + str(n) + '^3' if kind == 'hex8' else 'Quad4 planar ' + str(n) + '^2'}",
+           "nelem": E, "bytes_per_element_f64": {"gradient": bytes_grad, "integral": bytes_int}}
+    u = torch.from_numpy(gf.randn(11, E * nne * nd)).cuda().reshape(E, nne, nd)
+    for dt, nm, scale in ((torch.float64, "f64", 1.0), (torch.float32, "f32", 0.5)):
+        ue = u.to(dt)
+        eps = quad.SymGradN_vector(ue)
+        fe = quad.Int_gradN_dot_tensor2_dV(eps)
+        t_sym = _timed(lambda: quad.SymGradN_vector(ue, out=eps), 10)
+        t_grad = _timed(lambda: quad.GradN_vector(ue, out=eps), 10)
+        t_int = _timed(lambda: quad.Int_gradN_dot_tensor2_dV(eps, out=fe), 10)
+        row = {}
+        for key, t, b in (("SymGradN_vector", t_sym, bytes_grad), ("GradN_vector", t_grad, bytes_grad),
+                          ("Int_gradN_dot_tensor2_dV", t_int, bytes_int)):
+            gbs = scale * b * E / t / 1e9
+            row[key] = {"us": t * 1e6, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        out[nm] = row
+        del eps, fe, ue
+    del u, quad, vec
+    if kind == "hex8":
+        # the per-point isotropic tangent (81 entries per point) that feeds the stiffness
+        Kq = torch.full((E, nip), 10.0, dtype=torch.float64, device="cuda")
+        Gq = 0.5 + torch.rand((E, nip), dtype=torch.float64, device="cuda")
+        for dt, nm, sz in ((None, "f64", 8), (torch.float32, "f32", 4)):
+            C = gf.elastic_tangent(Kq, Gq, dtype=dt)
+            t = _timed(lambda: gf._call("gf_material_elastic_tangent_f32" if sz == 4 else "gf_material_elastic_tangent",
+                                        E * nip, 3, gf._dp(Kq), gf._dp(Gq), gf._dp(C), gf._stream()), 10)
+            b = E * nip * (81 * sz + 16)  # writes C, reads K and G
+            out[nm]["elastic_tangent"] = {"us": t * 1e6, "achieved_GBs": b / t / 1e9, "frac": b / t / 1e9 / HBM_PEAK_GBS}
+            del C
+        del Kq, Gq
+    torch.cuda.empty_cache()
     return out
 
 

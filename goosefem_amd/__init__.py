@@ -2049,6 +2049,16 @@ class ExplicitQuad4(_Handle):
         _hip_memcpy_d2d(o, p, 8 * self._vec.ndof)
         return o
 
+    def energy(self, mass, out=None):
+        """(Ekin, Epot) of the current state as a device tensor of 2 (written into out[0:2] if given):
+        Ekin = sum_d mass_d v_d^2 / 2 (mass: the lumped mass of every DOF), Epot = u . f_int / 2 of
+        the last step run with store_fint (Elastic-VelocityVerlet/main.cpp:155-160)."""
+        if not hasattr(self, "_mass") or self._mass_src is not mass:
+            self._mass, self._mass_src = to_device(mass), mass
+        out = _empty((2,)) if out is None else out
+        _call("gf_explicit_energy", self._h, _dp(self._mass), _dp(out), _vp(out.data_ptr() + 8), _stream())
+        return out
+
 
 # -------------------------------------------------------------------------------------------------
 # Tied-system PCG (MatrixPartitionedTyings operator, FP64 Jacobi-PCG)

@@ -56,8 +56,35 @@ def _compile(src: str) -> str:
     return obj
 
 
+H5_SRC = os.path.join(HERE, "csrc_h5", "gf_h5.cpp")
+H5_LIB = os.path.join(HERE, "libgoosefem_amd_h5.so")
+HDF5_DIR = os.environ.get("GF_HDF5_DIR", "/opt/conda")
+
+
+def build_h5(verbose: bool = True) -> str | None:
+    """The HDF5 output library (host C++, over the HDF5 C library in $GF_HDF5_DIR, /opt/conda by
+    default, linked with an rpath to it).  Skipped, with a message, where HDF5 is absent."""
+    inc, libdir = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print(f"HDF5 not found under {HDF5_DIR}: {H5_LIB} not built")
+        return None
+    cxx = shutil.which("g++") or "g++"
+    tmp = H5_LIB + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{inc}", H5_SRC, "-o", tmp, f"-L{libdir}",
+           "-lhdf5", f"-Wl,-rpath,{libdir}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {H5_SRC}:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, H5_LIB)
+    if verbose:
+        print(f"built {H5_LIB}")
+    return H5_LIB
+
+
 def build(verbose: bool = True) -> str:
-    """Compile all sources and link the shared library.  Always rebuilds."""
+    """Compile all sources and link the shared library (and the HDF5 output library).  Always
+    rebuilds."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
@@ -70,6 +97,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
+    build_h5(verbose)
     return LIB
 
 

@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "gf_internal.hpp"
+#include "gf_wave.hpp"
 
 using namespace gf;
 
@@ -56,6 +57,7 @@ struct gf_explicit_s {
     DBuf<float> gK, gG;
     DBuf<double> fe64;
     DBuf<float> fe32;
+    DBuf<double> epart; // energy reduction partials [2][NEB]
 };
 
 namespace {
@@ -510,6 +512,45 @@ __global__ void k_remap_eidx(int64_t n, const int32_t* __restrict__ idx, const i
     out[j] = rank[k / 8] * 8 + k % 8;
 }
 
+// Energies of the current state: Ekin = 1/2 sum_d m_d v_d^2 (the example's 0.5 sqdot(mass, V),
+// Elastic-VelocityVerlet/main.cpp:10-19,159) and Epot = 1/2 sum_d u_d fint_d, which for the linear
+// elastic law equals sum_q W_q dV_q with W = sigma : eps / 2 on the same quadrature.  Block partials
+// (DPP wavefront sums, waves in order), then one block adds them in order: deterministic.
+constexpr int NEB = 512;
+
+__global__ void __launch_bounds__(BS) k_energy_partial(int64_t n, const double* __restrict__ mass,
+                                                       const double* __restrict__ v, const double* __restrict__ u,
+                                                       const double* __restrict__ f, double* __restrict__ part)
+{
+    double ek = 0.0, ep = 0.0;
+    for (int64_t d = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; d < n; d += static_cast<int64_t>(gridDim.x) * BS) {
+        ek += mass[d] * v[d] * v[d];
+        ep += u[d] * f[d];
+    }
+    ek = block_sum<BS>(ek);
+    ep = block_sum<BS>(ep);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ek;
+        part[NEB + blockIdx.x] = ep;
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_energy_final(const double* __restrict__ part, double* __restrict__ ekin,
+                                                     double* __restrict__ epot)
+{
+    double ek = 0.0, ep = 0.0;
+    for (int i = threadIdx.x; i < NEB; i += BS) {
+        ek += part[i];
+        ep += part[NEB + i];
+    }
+    ek = block_sum<BS>(ek);
+    ep = block_sum<BS>(ep);
+    if (threadIdx.x == 0) {
+        *ekin = 0.5 * ek;
+        *epot = 0.5 * ep;
+    }
+}
+
 ShapeTable gauss_table()
 {
     ShapeTable T;
@@ -745,6 +786,19 @@ int gf_explicit_fint(gf_explicit x, const double** fint)
         GF_REQUIRE(x && fint, "null argument");
         if (!x->fint_valid) throw Error(GF_ESTATE, "no step run with store_fint");
         *fint = x->fint.p;
+    });
+}
+
+int gf_explicit_energy(gf_explicit x, const double* mass, double* ekin, double* epot, void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(x && mass && ekin && epot, "null argument");
+        if (!x->fint_valid) throw Error(GF_ESTATE, "no step run with store_fint: f_int of the current state unknown");
+        if (x->epart.n != static_cast<size_t>(2 * NEB)) x->epart.alloc(2 * NEB);
+        hipStream_t s = S(stream);
+        k_energy_partial<<<NEB, BS, 0, s>>>(x->ndof, mass, x->vv[x->cur].p, x->u[x->cur].p, x->fint.p, x->epart.p);
+        k_energy_final<<<1, BS, 0, s>>>(x->epart.p, ekin, epot);
+        check_launch();
     });
 }
 

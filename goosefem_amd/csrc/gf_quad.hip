@@ -169,79 +169,164 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
     if (t < n) b[t] = static_cast<float>(a[t]);
 }
 
-// ---- gradients: mode 0 GradN_vector, 1 GradN_vector_T, 2 SymGradN_vector -------------------------
-
-// 2-D node vectors (Quad4 tdim 2 / planar tdim 3): ElementQuad4.h:348-434, ElementQuad4Planar.h:129-221
-template <class T, int TD>
-__global__ void k_grad_2d(int64_t n, int nip, int mode, const T* __restrict__ dNx, const T* __restrict__ elemvec,
-                          T* __restrict__ qtensor)
+// ---- LDS staging of contiguous slices ---------------------------------------------------------------
+// The quadrature kernels read and write per-point / per-element records (dN/dx rows, element
+// vectors, tensors) of 8 to 72 bytes.  A workgroup moves its contiguous slice of each array between
+// HBM and LDS with full-wave contiguous 16-byte accesses (when the array is 16-byte aligned; element
+// by element otherwise) and the threads compute on LDS, so no global access is strided per lane.
+template <class T, int NT, int KB>
+__device__ inline void stage_in(T* __restrict__ lds, const T* __restrict__ g, int count)
 {
-    int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (t >= n) return;
-    int64_t e = t / nip;
-    const T* u = elemvec + e * 8;
-    const T* g = dNx + t * 8;
-    T a00 = g[0] * u[0] + g[2] * u[2] + g[4] * u[4] + g[6] * u[6];
-    T a01 = g[0] * u[1] + g[2] * u[3] + g[4] * u[5] + g[6] * u[7];
-    T a10 = g[1] * u[0] + g[3] * u[2] + g[5] * u[4] + g[7] * u[6];
-    T a11 = g[1] * u[1] + g[3] * u[3] + g[5] * u[5] + g[7] * u[7];
-    T r[TD * TD];
+    // KB: 16-byte loads in flight per thread before their LDS writes (the call site's typical count)
+    constexpr int VW = 16 / static_cast<int>(sizeof(T));
+    typedef T V __attribute__((ext_vector_type(VW)));
+    int done = 0;
+    if ((reinterpret_cast<uintptr_t>(g) & 15) == 0) {
+        const int nv = count / VW;
+        for (int base = 0; base < nv; base += KB * NT) {
+            V r[KB];
+            // unconditional loads (an index past the end re-reads the last chunk), so that all KB
+            // are issued before the first wait
 #pragma unroll
-    for (int k = 0; k < TD * TD; ++k) r[k] = T(0);
-    if (mode == 0) {
-        r[0] = a00;
-        r[1] = a01;
-        r[TD] = a10;
-        r[TD + 1] = a11;
-    }
-    else if (mode == 1) {
-        r[0] = a00;
-        r[TD] = a01;
-        r[1] = a10;
-        r[TD + 1] = a11;
-    }
-    else {
-        r[0] = a00;
-        r[TD + 1] = a11;
-        r[1] = T(0.5) * (g[0] * u[1] + g[2] * u[3] + g[4] * u[5] + g[6] * u[7] + g[1] * u[0] + g[3] * u[2] +
-                         g[5] * u[4] + g[7] * u[6]);
-        r[TD] = r[1];
-    }
-    T* o = qtensor + t * TD * TD;
+            for (int k = 0; k < KB; ++k) {
+                const int i = base + k * NT + threadIdx.x;
+                r[k] = reinterpret_cast<const V*>(g)[i < nv ? i : nv - 1];
+            }
 #pragma unroll
-    for (int k = 0; k < TD * TD; ++k) o[k] = r[k];
+            for (int k = 0; k < KB; ++k) {
+                const int i = base + k * NT + threadIdx.x;
+                if (i < nv) reinterpret_cast<V*>(lds)[i] = r[k];
+            }
+        }
+        done = nv * VW;
+    }
+    for (int i = done + threadIdx.x; i < count; i += NT) lds[i] = g[i];
 }
 
-// Hex8: generic loops Element.h:924-1022
-template <class T>
-__global__ void k_grad_3d(int64_t n, int nip, int mode, const T* __restrict__ dNx, const T* __restrict__ elemvec,
-                          T* __restrict__ qtensor)
+template <class T, int NT>
+__device__ inline void stage_out(T* __restrict__ g, const T* __restrict__ lds, int count)
 {
-    int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (t >= n) return;
-    int64_t e = t / nip;
-    const T* u = elemvec + e * 24;
-    const T* g = dNx + t * 24;
-    T r[9];
+    constexpr int VW = 16 / static_cast<int>(sizeof(T));
+    typedef T V __attribute__((ext_vector_type(VW)));
+    int done = 0;
+    if ((reinterpret_cast<uintptr_t>(g) & 15) == 0) {
+        const int nv = count / VW;
+        for (int i = threadIdx.x; i < nv; i += NT)
+            __builtin_nontemporal_store(reinterpret_cast<const V*>(lds)[i], reinterpret_cast<V*>(g) + i);
+        done = nv * VW;
+    }
+    for (int i = done + threadIdx.x; i < count; i += NT) g[i] = lds[i];
+}
+
+// ---- gradients: mode 0 GradN_vector, 1 GradN_vector_T, 2 SymGradN_vector -------------------------
+
+// GradN_vector (mode 0), GradN_vector_T (1), SymGradN_vector (2): 2-D node vectors (Quad4 tdim 2 /
+// planar tdim 3, ElementQuad4.h:348-434, ElementQuad4Planar.h:129-221) and Hex8 (the generic loops of
+// Element.h:924-1022), in the reference's expression order.  A workgroup of QB threads owns QB
+// consecutive points (e, q); LDS holds their dN/dx rows [QB][nne nd], the element vectors of the
+// elements they span, and the output tensors [QB][TD TD].
+template <int ND>
+inline size_t grad_lds_bytes(int qb, int nip, int td, size_t sz)
+{
+    const int nn = ND == 2 ? 8 : 24;
+    return (static_cast<size_t>(qb) * nn + static_cast<size_t>(qb) * td * td + (qb / nip + 2) * static_cast<size_t>(nn)) * sz;
+}
+
+template <class T, int TD>
+__global__ void __launch_bounds__(256) k_grad_2d_lds(int64_t n, int nip, int mode, const T* __restrict__ dNx,
+                                                     const T* __restrict__ elemvec, T* __restrict__ qtensor)
+{
+    constexpr int QB = 256;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* s_o = reinterpret_cast<T*>(smem); // [QB][TD*TD]
+    T* s_g = s_o + QB * TD * TD;           // [QB][8]
+    T* s_u = s_g + QB * 8;           // [elements][8]
+    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * QB;
+    const int nq = static_cast<int>(min<int64_t>(QB, n - t0));
+    const int64_t e0 = t0 / nip, e1 = (t0 + nq - 1) / nip;
+    stage_in<T, QB, 8 / (16 / sizeof(T))>(s_g, dNx + t0 * 8, nq * 8);
+    stage_in<T, QB, 1>(s_u, elemvec + e0 * 8, static_cast<int>(e1 - e0 + 1) * 8);
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        const int64_t e = (t0 + threadIdx.x) / nip;
+        const T* u = s_u + (e - e0) * 8;
+        const T* g = s_g + threadIdx.x * 8;
+        T a00 = g[0] * u[0] + g[2] * u[2] + g[4] * u[4] + g[6] * u[6];
+        T a01 = g[0] * u[1] + g[2] * u[3] + g[4] * u[5] + g[6] * u[7];
+        T a10 = g[1] * u[0] + g[3] * u[2] + g[5] * u[4] + g[7] * u[6];
+        T a11 = g[1] * u[1] + g[3] * u[3] + g[5] * u[5] + g[7] * u[7];
+        T r[TD * TD];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) r[k] = T(0);
+        for (int k = 0; k < TD * TD; ++k) r[k] = T(0);
+        if (mode == 0) {
+            r[0] = a00;
+            r[1] = a01;
+            r[TD] = a10;
+            r[TD + 1] = a11;
+        }
+        else if (mode == 1) {
+            r[0] = a00;
+            r[TD] = a01;
+            r[1] = a10;
+            r[TD + 1] = a11;
+        }
+        else {
+            r[0] = a00;
+            r[TD + 1] = a11;
+            r[1] = T(0.5) * (g[0] * u[1] + g[2] * u[3] + g[4] * u[5] + g[6] * u[7] + g[1] * u[0] + g[3] * u[2] +
+                             g[5] * u[4] + g[7] * u[6]);
+            r[TD] = r[1];
+        }
+        T* o = s_o + threadIdx.x * TD * TD;
 #pragma unroll
-    for (int m = 0; m < 8; ++m)
+        for (int k = 0; k < TD * TD; ++k) o[k] = r[k];
+    }
+    __syncthreads();
+    stage_out<T, QB>(qtensor + t0 * TD * TD, s_o, nq * TD * TD);
+}
+
+template <class T>
+__global__ void __launch_bounds__(128) k_grad_3d_lds(int64_t n, int nip, int mode, const T* __restrict__ dNx,
+                                                     const T* __restrict__ elemvec, T* __restrict__ qtensor)
+{
+    constexpr int QB = 128;
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* s_o = reinterpret_cast<T*>(smem); // [QB][9]
+    T* s_g = s_o + QB * 9;           // [QB][24]
+    T* s_u = s_g + QB * 24;           // [elements][24]
+    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * QB;
+    const int nq = static_cast<int>(min<int64_t>(QB, n - t0));
+    const int64_t e0 = t0 / nip, e1 = (t0 + nq - 1) / nip;
+    stage_in<T, QB, 24 / (16 / sizeof(T))>(s_g, dNx + t0 * 24, nq * 24);
+    stage_in<T, QB, 1>(s_u, elemvec + e0 * 24, static_cast<int>(e1 - e0 + 1) * 24);
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        const int64_t e = (t0 + threadIdx.x) / nip;
+        const T* u = s_u + (e - e0) * 24;
+        const T* g = s_g + threadIdx.x * 24;
+        T r[9];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 9; ++k) r[k] = T(0);
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                if (mode == 0) r[i * 3 + j] += g[m * 3 + i] * u[m * 3 + j];
-                else if (mode == 1) r[j * 3 + i] += g[m * 3 + i] * u[m * 3 + j];
-                else {
-                    T h = T(0.5) * g[m * 3 + i] * u[m * 3 + j];
-                    r[i * 3 + j] += h;
-                    r[j * 3 + i] += h;
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    if (mode == 0) r[i * 3 + j] += g[m * 3 + i] * u[m * 3 + j];
+                    else if (mode == 1) r[j * 3 + i] += g[m * 3 + i] * u[m * 3 + j];
+                    else {
+                        T h = T(0.5) * g[m * 3 + i] * u[m * 3 + j];
+                        r[i * 3 + j] += h;
+                        r[j * 3 + i] += h;
+                    }
                 }
-            }
-    T* o = qtensor + t * 9;
+        T* o = s_o + threadIdx.x * 9;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) o[k] = r[k];
+        for (int k = 0; k < 9; ++k) o[k] = r[k];
+    }
+    __syncthreads();
+    stage_out<T, QB>(qtensor + t0 * 9, s_o, nq * 9);
 }
 
 // ---- interpolation / integration ----------------------------------------------------------------
@@ -325,6 +410,55 @@ __global__ void k_int_tensor2(int64_t n, int nip, int nne, const T* __restrict__
     for (int j = 0; j < ND; ++j) elemvec[t * ND + j] = f[j];
 }
 
+// Staged form of k_int_tensor2: a workgroup of EB NNE threads owns EB consecutive elements; LDS
+// holds their dN/dx [EB nip NNE ND], dV [EB nip], tensors [EB nip TD TD] and the output element
+// vectors [EB NNE ND].  Thread (e, m) sums over the points in the order of k_int_tensor2.
+template <class T, int NNE, int ND, int TD, int EB>
+__global__ void __launch_bounds__(EB* NNE) k_int_tensor2_lds(int64_t nelem, int nip, const T* __restrict__ dNx,
+                                                             const T* __restrict__ vol, const T* __restrict__ qtensor,
+                                                             T* __restrict__ elemvec)
+{
+    constexpr int NT = EB * NNE;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t e0 = static_cast<int64_t>(blockIdx.x) * EB;
+    const int ne = static_cast<int>(min<int64_t>(EB, nelem - e0));
+    T* s_g = reinterpret_cast<T*>(smem); // EB nip NNE ND
+    T* s_s = s_g + EB * nip * NNE * ND;   // EB nip TD TD
+    T* s_o = s_s + EB * nip * TD * TD;    // EB NNE ND
+    T* s_v = s_o + EB * NNE * ND;         // EB nip
+    stage_in<T, NT, NNE * ND / (16 / sizeof(T))>(s_g, dNx + e0 * nip * NNE * ND, ne * nip * NNE * ND);
+    stage_in<T, NT, (TD * TD + (16 / sizeof(T)) - 1) / (16 / sizeof(T))>(s_s, qtensor + e0 * nip * TD * TD, ne * nip * TD * TD);
+    stage_in<T, NT, 1>(s_v, vol + e0 * nip, ne * nip);
+    __syncthreads();
+    const int el = threadIdx.x / NNE, m = threadIdx.x - el * NNE;
+    if (el < ne) {
+        T f[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) f[j] = T(0);
+        for (int q = 0; q < nip; ++q) {
+            const T* g = s_g + ((el * nip + q) * NNE + m) * ND;
+            const T* sq = s_s + (el * nip + q) * TD * TD;
+            T v = s_v[el * nip + q];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                T a = g[0] * sq[0 * TD + j] + g[1] * sq[1 * TD + j];
+                if (ND == 3) a = a + g[2] * sq[2 * TD + j];
+                f[j] += a * v;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < ND; ++j) s_o[(el * NNE + m) * ND + j] = f[j];
+    }
+    __syncthreads();
+    stage_out<T, NT>(elemvec + e0 * NNE * ND, s_o, ne * NNE * ND);
+}
+
+template <class T, int NNE, int ND, int TD, int EB>
+inline size_t int2_lds_bytes(int nip)
+{
+    return static_cast<size_t>(EB) * (nip * NNE * ND + nip * TD * TD + NNE * ND + nip) * sizeof(T);
+}
+
 // Int_gradN_dot_tensor4_dot_gradNT_dV FP64 (Element.h:1131-1176): thread per (e, m, n) block
 template <int NNE, int ND, int TD>
 __global__ void k_int_tensor4_f64(int64_t n, int nip, const double* __restrict__ dNx, const double* __restrict__ vol,
@@ -383,56 +517,79 @@ __global__ void k_elastic_stress(int64_t n, const T* __restrict__ eps, const T* 
         }
 }
 
-// Tangent, written as contiguous output: a workgroup owns TP consecutive points (TP TD^4
-// components) and its threads stride over the components, so every store instruction covers
-// contiguous memory.  Component c of a point is C(i, j, a, b) with c = ((i TD + j) TD + a) TD + b,
-// the same expression as the per-point loop of the material law; TO = float rounds the FP64
-// value once.
-constexpr int TP = 64;
+// Tangent C_ijab = K d_ij d_ab + 2 G (0.5 (d_ia d_jb + d_ib d_ja) - d_ij d_ab / 3) per point.  Every
+// point has four distinct entries: K + 2G (1 - 1/3) (i = j = a = b), K + 2G (-1/3) (i = j, a = b,
+// i != a), G ({a, b} = {i, j}, i != j) and 0, each formed from the same FP64 expression as the
+// per-point loop of the material law (TO = float rounds it once).  A workgroup computes the four
+// entries of its TP points into LDS, then writes its contiguous slice of C with full-wave
+// contiguous 16-byte streaming stores: GP points (GP TD^4 entries) are NV stores, and a small LDS
+// table gives, for each store of a group, the point and entry of each of its ES lanes.
+constexpr int TP = 256;
+
+// entry code of component c = ((i TD + j) TD + a) TD + b
+template <int TD>
+__device__ inline int tangent_code(int c)
+{
+    const int i = c / (TD * TD * TD), j = (c / (TD * TD)) % TD, a = (c / TD) % TD, b = c % TD;
+    if (i == j && a == b) return i == a ? 0 : 1;
+    if (i != j && ((i == a && j == b) || (i == b && j == a))) return 2;
+    return 3;
+}
 
 template <int TD, class TO>
 __global__ void __launch_bounds__(BS) k_elastic_tangent(int64_t n, const double* __restrict__ K,
-                                                        const double* __restrict__ G, TO* __restrict__ C, bool pairs)
+                                                        const double* __restrict__ G, TO* __restrict__ C, bool vec)
 {
-    // The block's slice of C is contiguous and is written with streaming stores (not re-read).
-    // FP64: two entries per 16-byte store.  FP32: each thread owns one component c and walks the
-    // block's points PPI at a time, so its two constant factors are computed once and the
-    // PPI * T4 active threads of a pass write one contiguous run.  (Measured on Hex8 128^3: the
-    // FP64 pairs beat the component mapping, whose stores are 8 bytes; for FP32 the component
-    // mapping wins, the entry arithmetic being the larger cost.)
-    typedef TO V2 __attribute__((ext_vector_type(2)));
-    constexpr int T4 = TD * TD * TD * TD, PPI = BS / T4;
+    static_assert(BS == TP, "one point per thread in the first phase");
+    constexpr int T4 = TD * TD * TD * TD;
+    constexpr int ES = 16 / static_cast<int>(sizeof(TO));  // entries per 16-byte store
+    constexpr int GP = T4 % ES == 0 ? 1 : ES;               // points per group (GP T4 a multiple of ES)
+    constexpr int NV = GP * T4 / ES;                        // stores per group
+    constexpr int NS = TP / GP * NV;                        // stores per workgroup
+    typedef TO V __attribute__((ext_vector_type(ES)));
+    __shared__ TO s_val[TP][4];
+    __shared__ uint16_t s_tab[NV];
     const int64_t p0 = static_cast<int64_t>(blockIdx.x) * TP;
     const int np = static_cast<int>(min<int64_t>(TP, n - p0));
-    TO* out = C + p0 * T4;
-    if constexpr (sizeof(TO) == 8) {
-        auto entry = [&](int t) {
-            const int pl = t / T4, c = t - pl * T4;
-            const int i = c / (TD * TD * TD), j = (c / (TD * TD)) % TD, a = (c / TD) % TD, b = c % TD;
-            const double k = K[p0 + pl], g = G[p0 + pl];
-            const double dij = i == j, dkl = a == b, dik = i == a, djl = j == b, dil = i == b, djk = j == a;
-            return static_cast<TO>(k * dij * dkl + 2.0 * g * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0));
-        };
-        const int nt = np * T4;
-        if (pairs) {
-            for (int t = threadIdx.x; t < nt / 2; t += BS)
-                __builtin_nontemporal_store(V2{entry(2 * t), entry(2 * t + 1)}, reinterpret_cast<V2*>(out) + t);
-            if ((nt & 1) && threadIdx.x == 0) out[nt - 1] = entry(nt - 1);
-        }
-        else {
-            for (int t = threadIdx.x; t < nt; t += BS) out[t] = entry(t);
-        }
-        return;
+    if (threadIdx.x < np) {
+        const double k = K[p0 + threadIdx.x], g = G[p0 + threadIdx.x];
+        const double a2_0 = 0.5 * (1.0 * 1.0 + 1.0 * 1.0) - 1.0 * 1.0 / 3.0; // i = j = a = b
+        const double a2_1 = 0.5 * (0.0 * 0.0 + 0.0 * 0.0) - 1.0 * 1.0 / 3.0; // i = j != a = b
+        s_val[threadIdx.x][0] = static_cast<TO>(k * 1.0 * 1.0 + 2.0 * g * a2_0);
+        s_val[threadIdx.x][1] = static_cast<TO>(k * 1.0 * 1.0 + 2.0 * g * a2_1);
+        s_val[threadIdx.x][2] = static_cast<TO>(k * 0.0 * 0.0 + 2.0 * g * (0.5 * (1.0 * 1.0 + 0.0 * 0.0) - 0.0 * 0.0 / 3.0));
+        s_val[threadIdx.x][3] = static_cast<TO>(k * 0.0 * 0.0 + 2.0 * g * (0.5 * (0.0 * 0.0 + 0.0 * 0.0) - 0.0 * 0.0 / 3.0));
     }
-    if (threadIdx.x >= PPI * T4) return;
-    const int c = threadIdx.x % T4, q = threadIdx.x / T4;
-    const int i = c / (TD * TD * TD), j = (c / (TD * TD)) % TD, a = (c / TD) % TD, b = c % TD;
-    const double dij = i == j, dkl = a == b, dik = i == a, djl = j == b, dil = i == b, djk = j == a;
-    const double a1 = dij * dkl, a2 = 0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0;
-    for (int pl = q; pl < np; pl += PPI) {
-        const double k = K[p0 + pl], g = G[p0 + pl];
-        // k * a1 equals k * dij * dkl (a1 is 0 or 1), so the value is the same expression's
-        __builtin_nontemporal_store(static_cast<TO>(k * a1 + 2.0 * g * a2), out + pl * T4 + c);
+    for (int v = threadIdx.x; v < NV; v += BS) {
+        uint16_t t = 0;
+#pragma unroll
+        for (int l = 0; l < ES; ++l) {
+            const int f = v * ES + l, off = f / T4;
+            t |= static_cast<uint16_t>((off | (tangent_code<TD>(f - off * T4) << 2)) << (4 * l));
+        }
+        s_tab[v] = t;
+    }
+    __syncthreads();
+    TO* out = C + p0 * T4;
+    if (vec && np == TP) {
+        V* o = reinterpret_cast<V*>(out);
+        for (int sidx = threadIdx.x; sidx < NS; sidx += BS) {
+            const int grp = sidx / NV, v = sidx - grp * NV;
+            const unsigned t = s_tab[v];
+            V w;
+#pragma unroll
+            for (int l = 0; l < ES; ++l) {
+                const unsigned e = (t >> (4 * l)) & 15u;
+                w[l] = s_val[grp * GP + (e & 3u)][e >> 2];
+            }
+            __builtin_nontemporal_store(w, o + sidx);
+        }
+    }
+    else {
+        for (int f = threadIdx.x; f < np * T4; f += BS) {
+            const int pl = f / T4;
+            out[f] = s_val[pl][tangent_code<TD>(f - pl * T4)];
+        }
     }
 }
 
@@ -469,22 +626,24 @@ int grad_dispatch(gf_quad q, const void* elemvec, void* qtensor, int dtype, int 
         int64_t n = q->nelem * q->nip;
         if (n == 0) return;
         hipStream_t s = S(stream);
-        if (dtype == GF_F64) {
-            auto u = static_cast<const double*>(elemvec);
-            auto o = static_cast<double*>(qtensor);
-            if (q->et == GF_HEX8) k_grad_3d<double><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, mode, q->dNx.p, u, o);
+        auto launch = [&](auto tag, const auto* u, auto* o, const auto* g) {
+            using T = decltype(tag);
+            const size_t sz = sizeof(T);
+            if (q->et == GF_HEX8)
+                k_grad_3d_lds<T><<<nblocks(n, 128), 128, grad_lds_bytes<3>(128, q->nip, 3, sz), s>>>(n, q->nip, mode, g,
+                                                                                                     u, o);
             else if (q->et == GF_QUAD4_PLANAR)
-                k_grad_2d<double, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, mode, q->dNx.p, u, o);
-            else k_grad_2d<double, 2><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, mode, q->dNx.p, u, o);
-        }
+                k_grad_2d_lds<T, 3><<<nblocks(n, 256), 256, grad_lds_bytes<2>(256, q->nip, 3, sz), s>>>(n, q->nip, mode,
+                                                                                                        g, u, o);
+            else
+                k_grad_2d_lds<T, 2><<<nblocks(n, 256), 256, grad_lds_bytes<2>(256, q->nip, 2, sz), s>>>(n, q->nip, mode,
+                                                                                                        g, u, o);
+        };
+        if (dtype == GF_F64)
+            launch(double(0), static_cast<const double*>(elemvec), static_cast<double*>(qtensor), q->dNx.p);
         else {
             ensure32(q, s);
-            auto u = static_cast<const float*>(elemvec);
-            auto o = static_cast<float*>(qtensor);
-            if (q->et == GF_HEX8) k_grad_3d<float><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, mode, q->dNx32.p, u, o);
-            else if (q->et == GF_QUAD4_PLANAR)
-                k_grad_2d<float, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, mode, q->dNx32.p, u, o);
-            else k_grad_2d<float, 2><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, mode, q->dNx32.p, u, o);
+            launch(float(0), static_cast<const float*>(elemvec), static_cast<float*>(qtensor), q->dNx32.p);
         }
         check_launch();
     });
@@ -647,29 +806,37 @@ int gf_quad_int_gradN_dot_tensor2_dV(gf_quad q, const void* qtensor, void* elemv
     return guard([&] {
         GF_REQUIRE(q && qtensor && elemvec, "null argument");
         GF_REQUIRE(dtype == GF_F64 || dtype == GF_F32, "dtype must be GF_F64 or GF_F32");
-        int64_t n = q->nelem * q->nne;
-        if (n == 0) return;
+        const int64_t E = q->nelem;
+        if (E == 0) return;
         hipStream_t s = S(stream);
-        if (dtype == GF_F64) {
-            auto in = static_cast<const double*>(qtensor);
-            auto o = static_cast<double*>(elemvec);
+        auto launch = [&](auto tag, const auto* in, auto* o, const auto* g, const auto* v) {
+            using T = decltype(tag);
+            // rules with many points (LDS need above 64 KiB): the unstaged kernel, thread per (e, m)
+            const size_t need = q->et == GF_HEX8 ? int2_lds_bytes<T, 8, 3, 3, 16>(q->nip)
+                                                 : int2_lds_bytes<T, 4, 2, 3, 64>(q->nip);
+            if (need > 65536) {
+                const int64_t n = E * q->nne;
+                if (q->et == GF_HEX8) k_int_tensor2<T, 3, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, g, v, in, o);
+                else if (q->et == GF_QUAD4_PLANAR)
+                    k_int_tensor2<T, 2, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, g, v, in, o);
+                else k_int_tensor2<T, 2, 2><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, g, v, in, o);
+                return;
+            }
             if (q->et == GF_HEX8)
-                k_int_tensor2<double, 3, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, q->dNx.p, q->vol.p, in, o);
+                k_int_tensor2_lds<T, 8, 3, 3, 16><<<nblocks(E, 16), 128, int2_lds_bytes<T, 8, 3, 3, 16>(q->nip), s>>>(
+                    E, q->nip, g, v, in, o);
             else if (q->et == GF_QUAD4_PLANAR)
-                k_int_tensor2<double, 2, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, q->dNx.p, q->vol.p, in, o);
+                k_int_tensor2_lds<T, 4, 2, 3, 64><<<nblocks(E, 64), 256, int2_lds_bytes<T, 4, 2, 3, 64>(q->nip), s>>>(
+                    E, q->nip, g, v, in, o);
             else
-                k_int_tensor2<double, 2, 2><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, q->dNx.p, q->vol.p, in, o);
-        }
+                k_int_tensor2_lds<T, 4, 2, 2, 64><<<nblocks(E, 64), 256, int2_lds_bytes<T, 4, 2, 2, 64>(q->nip), s>>>(
+                    E, q->nip, g, v, in, o);
+        };
+        if (dtype == GF_F64)
+            launch(double(0), static_cast<const double*>(qtensor), static_cast<double*>(elemvec), q->dNx.p, q->vol.p);
         else {
             ensure32(q, s);
-            auto in = static_cast<const float*>(qtensor);
-            auto o = static_cast<float*>(elemvec);
-            if (q->et == GF_HEX8)
-                k_int_tensor2<float, 3, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, q->dNx32.p, q->vol32.p, in, o);
-            else if (q->et == GF_QUAD4_PLANAR)
-                k_int_tensor2<float, 2, 3><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, q->dNx32.p, q->vol32.p, in, o);
-            else
-                k_int_tensor2<float, 2, 2><<<nblocks(n, BS), BS, 0, s>>>(n, q->nip, q->nne, q->dNx32.p, q->vol32.p, in, o);
+            launch(float(0), static_cast<const float*>(qtensor), static_cast<float*>(elemvec), q->dNx32.p, q->vol32.p);
         }
         check_launch();
     });

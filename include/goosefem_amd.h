@@ -521,6 +521,12 @@ int gf_explicit_state(gf_explicit x, double** u, double** v, double** a);
  * strain/stress/force, FP64 assembly and update) or GF_F64_FAST (FP64 element work with FMA).
  * store_fint != 0 keeps the internal force of the last step (one extra [ndof] write). */
 int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fint, void* stream);
+/* Energies of the current state (the per-increment output of Elastic-VelocityVerlet/main.cpp:155-160),
+ * enqueued on stream into the device scalars *ekin, *epot: Ekin = 1/2 sum_d mass_d v_d^2 (mass: device
+ * dofval, the lumped mass of every DOF, prescribed ones included) and Epot = 1/2 sum_d u_d fint_d, which
+ * for the linear elastic law equals sum_q W_q dV_q with W = sigma : eps / 2 (same quadrature).  Needs the
+ * last step run with store_fint (GF_ESTATE otherwise).  Deterministic (fixed-order reductions). */
+int gf_explicit_energy(gf_explicit x, const double* mass, double* ekin, double* epot, void* stream);
 /* internal force (dofval) of the last stored step's displacement, FP64 [ndof] */
 int gf_explicit_fint(gf_explicit x, const double** fint);
 /* 1 if the structured-grid fused kernel is used (conn is a Regular Quad4 grid), else 0 */

@@ -15,13 +15,10 @@ import oracle
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    syms = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
-        src = open(h).read()
-        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        syms |= set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(gf_\w+)\s*\(", src, flags=re.M))
-    return sorted(syms)
+def declared_symbols(header="goosefem_amd.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(gf_\w+)\s*\(", src, flags=re.M)))
 
 
 @pytest.fixture(scope="module")
@@ -32,10 +29,25 @@ def gf():
     return goosefem_amd
 
 
+def test_every_header_belongs_to_a_library():
+    assert sorted(os.path.basename(h) for h in glob.glob(os.path.join(ROOT, "include", "*.h"))) == [
+        "goosefem_amd.h", "goosefem_amd_h5.h"]
+
+
 def test_library_exports_every_declared_symbol(gf):
     syms = declared_symbols()
     assert len(syms) > 60
     lib = gf.lib()
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_h5_library_exports_every_declared_symbol():
+    from goosefem_amd import h5
+
+    syms = declared_symbols("goosefem_amd_h5.h")
+    assert len(syms) == 8
+    lib = h5.lib()
     missing = [s for s in syms if not hasattr(lib, s)]
     assert not missing, missing
 

@@ -115,6 +115,23 @@ def test_material_bit_exact(gf):
     assert np.array_equal(host(gf.elastic_tangent(K, G)), oracle.elastic_tangent(K, G))
 
 
+@pytest.mark.parametrize("td", [2, 3])
+def test_elastic_tangent_streams(gf, td):
+    """The tangent kernel's vector path (full 256-point workgroups, 16-byte stores) and its tail,
+    FP64 bit-exact with the oracle and FP32 equal to the oracle's FP64 rounded once."""
+    import torch
+
+    rng = np.random.default_rng(7)
+    K = rng.uniform(5.0, 15.0, (301, 8))  # 2408 points: 9 full workgroups and a tail of 104
+    G = rng.uniform(0.5, 1.5, (301, 8))
+    ref = oracle.elastic_tangent(K, G, td)
+    Kd, Gd = gf.to_device(K), gf.to_device(G)
+    assert np.array_equal(host(gf.elastic_tangent(Kd, Gd, td=td)), ref)
+    c32 = gf.elastic_tangent(Kd, Gd, td=td, dtype=torch.float32)
+    assert c32.dtype == torch.float32
+    assert np.array_equal(host(c32), ref.astype(np.float32))
+
+
 @pytest.mark.parametrize("periodic", [False, True])
 @pytest.mark.parametrize("et", [QUAD4, HEX8])
 def test_vector_bit_exact(gf, et, periodic):
@@ -479,6 +496,45 @@ def test_explicit_full_size_fp32_vs_fp64(gf):
           f"extended precision: fast {e_fast:.2e}, reference order {e_ref:.2e}")
     assert e_fast < 1e-14
     assert relerr(ffast, f64) < 1e-12
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_explicit_energy(gf, shuffle, tmp_path):
+    """Per-step energies of the explicit step (Elastic-VelocityVerlet/main.cpp:155-160) against
+    independent evaluations: Ekin = sum m v^2 / 2 (numpy) and Epot = sum_q sigma : eps dV / 2 from the
+    oracle's quadrature of the same displacement; then the example's HDF5 output of the series."""
+    from goosefem_amd import h5
+
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(24, 20, shuffle=shuffle, kvar=True)
+    mass = np.where(inv > 0, 1.0 / np.where(inv > 0, inv, 1.0), 0.75)  # any positive mass of every DOF
+    X = gf.ExplicitQuad4(gf.Vector(conn, dofs), coor, Ke, Ge, inv, 0.01)
+    X.set_state(u, v, a)
+    with pytest.raises(gf.GooseFEMError):
+        X.energy(mass)  # no f_int stored yet
+    hist = gf.to_device(np.zeros((6, 2)))
+    for inc in range(6):
+        X.step(1, gf.F64, store_fint=True)
+        X.energy(mass, out=hist[inc])
+    gu, gv, _ = (host(t) for t in X.state())
+    ek, ep = host(hist)[-1]
+    assert abs(ek - 0.5 * np.sum(mass * gv * gv)) <= 1e-13 * abs(ek)
+    ov = oracle.Vector(conn, dofs)
+    eps = q.SymGradN_vector(ov.AsElement(gu.reshape(-1, 2)))
+    sig = oracle.elastic_stress(eps, np.repeat(Ke[:, None], 4, 1), np.repeat(Ge[:, None], 4, 1))
+    w = 0.5 * np.einsum("eqij,eqji->eq", sig, eps)
+    assert abs(ep - np.sum(w * q.dV())) <= 1e-12 * abs(ep)
+    path = str(tmp_path / "example.hdf5")
+    with h5.File(path, h5.File.Overwrite) as f:
+        H = host(hist)
+        for inc in range(6):
+            h5.dump(f, "/global/Ekin", H[inc, 0], inc)
+            h5.dump(f, "/global/Epot", H[inc, 1], inc)
+        h5.dump(f, "/mesh/conn", conn.astype(np.uint64))
+        h5.dump(f, "/mesh/coor", coor)
+        h5.dump(f, "/mesh/disp", gu.reshape(-1, 2))
+    with h5.File(path, "r") as f:
+        assert np.array_equal(h5.load(f, "/global/Epot"), H[:, 1])
+        assert np.array_equal(h5.load(f, "/mesh/disp"), gu.reshape(-1, 2))
 
 
 def test_explicit_rejects_bad_input(gf):
