@@ -340,7 +340,43 @@ def test_committed_weights_cut_pcg_iterations_at_512(gf, ws):
     x0 = W.predict(G, 1e-3)
     _, it1, r1 = pb.P.solve(b, x0=x0, tol=1e-10)
     assert r0 <= 1e-9 and r1 <= 1e-9
+    assert pb.P.start_used == gf.GF_START_GIVEN  # the prediction is kept, not replaced by the fallback
     assert it1 < 0.75 * it0, (it0, it1)
+    # what the network adds beyond the affine part of the start (which alone removes about 20%)
+    _, it2, _ = pb.P.solve(b, x0=_affine_start(gf, W, 2, 1e-3), tol=1e-10)
+    assert it1 < 0.85 * it2, (it2, it1)
+
+
+def _affine_start(gf, W, dim, gamma):
+    """The start the network's prediction is added to: the affine part alone (a zero fluctuation)."""
+    grid = torch.zeros(W.nv * dim, dtype=torch.float32, device="cuda")
+    x = torch.zeros(W.ty.nnu, dtype=torch.float64, device="cuda")
+    gf._call("gf_grid_to_dofs", gf._i64(W.nv * dim), gf._i32(dim), gf._dp(grid), gf._dp(W.idx), gf._dp(W.affine),
+             gf._f64(1.0), gf._f64(gamma), gf._dp(x), gf._stream())
+    return x
+
+
+def test_committed_weights_3d_beat_affine_start(gf, ws):
+    """The committed Hex8 96^3 network (stiff spheres, pure shear) on a field outside its training
+    seeds: fewer PCG iterations than the affine part of the start alone and than a zero guess, to the
+    same true residual."""
+    import os
+
+    path = os.path.join(os.path.dirname(gf.__file__), "data", "warmstart_hex8_96.npz")
+    net = ws.WarmStartNet.load(path)
+    n, dim = int(net.meta["n"]), int(net.meta.get("dim", 3))
+    assert dim == 3
+    pb = ws.periodic_problem(n, sigma=float(net.meta["sigma"]), dim=3, spheres=True)
+    W = ws.PeriodicWarmStart(pb.P, pb.ty, n, net, sigma=pb.sigma, dim=3, shear=str(net.meta.get("shear", "pure")))
+    G = gf.to_device(ws.sphere_field(n, 900001, pb.sigma))
+    pb.P.set_moduli(gf.to_device(pb.K), G)
+    gamma = 5e-4
+    b = W.rhs(gamma)
+    _, it0, r0 = pb.P.solve(b, tol=1e-10)
+    _, it1, r1 = pb.P.solve(b, x0=W.predict(G, gamma), tol=1e-10)
+    _, it2, r2 = pb.P.solve(b, x0=_affine_start(gf, W, 3, gamma), tol=1e-10)
+    assert max(r0, r1, r2) <= 1e-9
+    assert it1 < 0.85 * it2 and it1 < 0.8 * it0, (it0, it2, it1)
 
 
 # ---- 3-D (3x3x3 taps, 2x2x2 pooling) ---------------------------------------------------------------
