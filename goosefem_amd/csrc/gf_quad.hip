@@ -11,6 +11,8 @@
 // in long contiguous runs.
 #include <cmath>
 
+#include <type_traits>
+
 #include "gf_internal.hpp"
 
 using namespace gf;
@@ -117,15 +119,13 @@ __global__ void k_compute_dN_quad4(int64_t n, int nip, int planar, double thick,
     vol[t] = planar ? w[q] * det * thick : w[q] * det;
 }
 
-__global__ void k_compute_dN_hex8(int64_t n, int nip, const double* __restrict__ x, const double* __restrict__ dNxi,
-                                  const double* __restrict__ w, double* __restrict__ dNx, double* __restrict__ vol)
+// dN/dx [8][3] and det J of one Hex8 point: J = dN/dxi^T x_e, its cofactor inverse (detail.h:40-70),
+// dN/dx = J^-1 dN/dxi, in the generic compute_dN order (Element.h:1178-1229, with the Q1 chain rule
+// corrected).  Shared by the set-up kernel and the gradient kernel that recomputes the geometry, so
+// both produce the same bits.
+__device__ inline void hex8_point_geometry(const double* __restrict__ xe, const double* __restrict__ g, double* o,
+                                           double& det)
 {
-    int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (t >= n) return;
-    int64_t e = t / nip;
-    int q = static_cast<int>(t - e * nip);
-    const double* xe = x + e * 24;
-    const double* g = dNxi + q * 24;
     double A[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -137,9 +137,8 @@ __global__ void k_compute_dN_hex8(int64_t n, int nip, const double* __restrict__
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) A[i][j] += g[m * 3 + i] * xe[m * 3 + j];
-    // detail.h:40-70
-    double det = (A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1]) -
-                 (A[0][2] * A[1][1] * A[2][0] + A[0][1] * A[1][0] * A[2][2] + A[0][0] * A[1][2] * A[2][1]);
+    det = (A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1]) -
+          (A[0][2] * A[1][1] * A[2][0] + A[0][1] * A[1][0] * A[2][2] + A[0][0] * A[1][2] * A[2][1]);
     double B[3][3];
     B[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
     B[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
@@ -150,7 +149,6 @@ __global__ void k_compute_dN_hex8(int64_t n, int nip, const double* __restrict__
     B[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
     B[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
     B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
-    double* o = dNx + t * 24;
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
@@ -160,6 +158,17 @@ __global__ void k_compute_dN_hex8(int64_t n, int nip, const double* __restrict__
             for (int j = 0; j < 3; ++j) s += B[i][j] * g[m * 3 + j];
             o[m * 3 + i] = s;
         }
+}
+
+__global__ void k_compute_dN_hex8(int64_t n, int nip, const double* __restrict__ x, const double* __restrict__ dNxi,
+                                  const double* __restrict__ w, double* __restrict__ dNx, double* __restrict__ vol)
+{
+    int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= n) return;
+    int64_t e = t / nip;
+    int q = static_cast<int>(t - e * nip);
+    double det;
+    hex8_point_geometry(x + e * 24, dNxi + q * 24, dNx + t * 24, det);
     vol[t] = w[q] * det;
 }
 
@@ -232,8 +241,8 @@ inline size_t grad_lds_bytes(int qb, int nip, int td, size_t sz)
     return (static_cast<size_t>(qb) * nn + static_cast<size_t>(qb) * td * td + (qb / nip + 2) * static_cast<size_t>(nn)) * sz;
 }
 
-template <class T, int TD>
-__global__ void __launch_bounds__(256) k_grad_2d_lds(int64_t n, int nip, int mode, const T* __restrict__ dNx,
+template <class T, int TD, int mode>
+__global__ void __launch_bounds__(256) k_grad_2d_lds(int64_t n, int nip, const T* __restrict__ dNx,
                                                      const T* __restrict__ elemvec, T* __restrict__ qtensor)
 {
     constexpr int QB = 256;
@@ -258,13 +267,13 @@ __global__ void __launch_bounds__(256) k_grad_2d_lds(int64_t n, int nip, int mod
         T r[TD * TD];
 #pragma unroll
         for (int k = 0; k < TD * TD; ++k) r[k] = T(0);
-        if (mode == 0) {
+        if constexpr (mode == 0) {
             r[0] = a00;
             r[1] = a01;
             r[TD] = a10;
             r[TD + 1] = a11;
         }
-        else if (mode == 1) {
+        else if constexpr (mode == 1) {
             r[0] = a00;
             r[TD] = a01;
             r[1] = a10;
@@ -285,8 +294,8 @@ __global__ void __launch_bounds__(256) k_grad_2d_lds(int64_t n, int nip, int mod
     stage_out<T, QB>(qtensor + t0 * TD * TD, s_o, nq * TD * TD);
 }
 
-template <class T>
-__global__ void __launch_bounds__(128) k_grad_3d_lds(int64_t n, int nip, int mode, const T* __restrict__ dNx,
+template <class T, int mode>
+__global__ void __launch_bounds__(128) k_grad_3d_lds(int64_t n, int nip, const T* __restrict__ dNx,
                                                      const T* __restrict__ elemvec, T* __restrict__ qtensor)
 {
     constexpr int QB = 128;
@@ -313,8 +322,8 @@ __global__ void __launch_bounds__(128) k_grad_3d_lds(int64_t n, int nip, int mod
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    if (mode == 0) r[i * 3 + j] += g[m * 3 + i] * u[m * 3 + j];
-                    else if (mode == 1) r[j * 3 + i] += g[m * 3 + i] * u[m * 3 + j];
+                    if constexpr (mode == 0) r[i * 3 + j] += g[m * 3 + i] * u[m * 3 + j];
+                    else if constexpr (mode == 1) r[j * 3 + i] += g[m * 3 + i] * u[m * 3 + j];
                     else {
                         T h = T(0.5) * g[m * 3 + i] * u[m * 3 + j];
                         r[i * 3 + j] += h;
@@ -327,6 +336,73 @@ __global__ void __launch_bounds__(128) k_grad_3d_lds(int64_t n, int nip, int mod
     }
     __syncthreads();
     stage_out<T, QB>(qtensor + t0 * 9, s_o, nq * 9);
+}
+
+// Hex8 gradients with the geometry recomputed: the stored dN/dx of a Hex8 quadrature is 1536 bytes
+// per element against 192 bytes of element coordinates, so this kernel reads the coordinates (kept by
+// the quadrature) and forms each point's dN/dx with hex8_point_geometry, the function that produced the
+// stored ones: the results are bit-identical to the stored-geometry form (FP32: the FP64 dN/dx rounded
+// once, as the stored FP32 copy is).  A workgroup of QB threads owns QB consecutive points; LDS holds
+// the coordinates and element vectors of the elements they span, the reference-element gradients and
+// the output tensors.
+constexpr int QB3 = 256;
+
+inline size_t grad3x_lds_bytes(int nip, size_t sz)
+{
+    const size_t ne = QB3 / nip + 2;
+    return static_cast<size_t>(QB3) * 9 * sz + ne * 24 * sz + ne * 24 * 8 + static_cast<size_t>(nip) * 24 * 8;
+}
+
+template <class T, int mode>
+__global__ void __launch_bounds__(QB3) k_grad_3d_x(int64_t n, int nip, const double* __restrict__ x,
+                                                   const double* __restrict__ dNxi, const T* __restrict__ elemvec,
+                                                   T* __restrict__ qtensor)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * QB3;
+    const int nq = static_cast<int>(min<int64_t>(QB3, n - t0));
+    const int64_t e0 = t0 / nip, e1 = (t0 + nq - 1) / nip;
+    const int ne = static_cast<int>(e1 - e0 + 1), nemax = QB3 / nip + 2;
+    T* s_o = reinterpret_cast<T*>(smem);       // [QB3][9]
+    T* s_u = s_o + QB3 * 9;                      // [nemax][24]
+    double* s_x = reinterpret_cast<double*>(s_u + nemax * 24); // [nemax][24]
+    double* s_d = s_x + nemax * 24;              // [nip][24]
+    stage_in<T, QB3, 1>(s_u, elemvec + e0 * 24, ne * 24);
+    stage_in<double, QB3, 1>(s_x, x + e0 * 24, ne * 24);
+    for (int i = threadIdx.x; i < nip * 24; i += QB3) s_d[i] = dNxi[i];
+    __syncthreads();
+    if (threadIdx.x < nq) {
+        const int64_t t = t0 + threadIdx.x, e = t / nip;
+        const int q = static_cast<int>(t - e * nip);
+        double gd[24], det;
+        hex8_point_geometry(s_x + (e - e0) * 24, s_d + q * 24, gd, det);
+        T g[24];
+#pragma unroll
+        for (int k = 0; k < 24; ++k) g[k] = static_cast<T>(gd[k]);
+        const T* u = s_u + (e - e0) * 24;
+        T r[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) r[k] = T(0);
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    if constexpr (mode == 0) r[i * 3 + j] += g[m * 3 + i] * u[m * 3 + j];
+                    else if constexpr (mode == 1) r[j * 3 + i] += g[m * 3 + i] * u[m * 3 + j];
+                    else {
+                        T h = T(0.5) * g[m * 3 + i] * u[m * 3 + j];
+                        r[i * 3 + j] += h;
+                        r[j * 3 + i] += h;
+                    }
+                }
+        T* o = s_o + threadIdx.x * 9;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) o[k] = r[k];
+    }
+    __syncthreads();
+    stage_out<T, QB3>(qtensor + t0 * 9, s_o, nq * 9);
 }
 
 // ---- interpolation / integration ----------------------------------------------------------------
@@ -626,24 +702,36 @@ int grad_dispatch(gf_quad q, const void* elemvec, void* qtensor, int dtype, int 
         int64_t n = q->nelem * q->nip;
         if (n == 0) return;
         hipStream_t s = S(stream);
-        auto launch = [&](auto tag, const auto* u, auto* o, const auto* g) {
+        auto launch = [&](auto tag, auto mtag, const auto* u, auto* o, const auto* g) {
             using T = decltype(tag);
+            constexpr int M = decltype(mtag)::value;
             const size_t sz = sizeof(T);
-            if (q->et == GF_HEX8)
-                k_grad_3d_lds<T><<<nblocks(n, 128), 128, grad_lds_bytes<3>(128, q->nip, 3, sz), s>>>(n, q->nip, mode, g,
-                                                                                                     u, o);
+            // Hex8 FP64: the geometry recomputed from the coordinates (1536 -> 192 bytes of dN/dx per
+            // element); FP32 reads the stored FP32 copy (768 bytes per element), cheaper than the
+            // FP64 recomputation; rules with few points per element: the stored geometry
+            if (q->et == GF_HEX8 && sizeof(T) == 8 && grad3x_lds_bytes(q->nip, sz) <= 65536)
+                k_grad_3d_x<T, M><<<nblocks(n, QB3), QB3, grad3x_lds_bytes(q->nip, sz), s>>>(n, q->nip, q->x.p,
+                                                                                             q->dNxi_d.p, u, o);
+            else if (q->et == GF_HEX8)
+                k_grad_3d_lds<T, M><<<nblocks(n, 128), 128, grad_lds_bytes<3>(128, q->nip, 3, sz), s>>>(n, q->nip, g,
+                                                                                                        u, o);
             else if (q->et == GF_QUAD4_PLANAR)
-                k_grad_2d_lds<T, 3><<<nblocks(n, 256), 256, grad_lds_bytes<2>(256, q->nip, 3, sz), s>>>(n, q->nip, mode,
-                                                                                                        g, u, o);
+                k_grad_2d_lds<T, 3, M><<<nblocks(n, 256), 256, grad_lds_bytes<2>(256, q->nip, 3, sz), s>>>(n, q->nip,
+                                                                                                           g, u, o);
             else
-                k_grad_2d_lds<T, 2><<<nblocks(n, 256), 256, grad_lds_bytes<2>(256, q->nip, 2, sz), s>>>(n, q->nip, mode,
-                                                                                                        g, u, o);
+                k_grad_2d_lds<T, 2, M><<<nblocks(n, 256), 256, grad_lds_bytes<2>(256, q->nip, 2, sz), s>>>(n, q->nip,
+                                                                                                           g, u, o);
+        };
+        auto by_mode = [&](auto tag, const auto* u, auto* o, const auto* g) {
+            if (mode == 0) launch(tag, std::integral_constant<int, 0>(), u, o, g);
+            else if (mode == 1) launch(tag, std::integral_constant<int, 1>(), u, o, g);
+            else launch(tag, std::integral_constant<int, 2>(), u, o, g);
         };
         if (dtype == GF_F64)
-            launch(double(0), static_cast<const double*>(elemvec), static_cast<double*>(qtensor), q->dNx.p);
+            by_mode(double(0), static_cast<const double*>(elemvec), static_cast<double*>(qtensor), q->dNx.p);
         else {
             ensure32(q, s);
-            launch(float(0), static_cast<const float*>(elemvec), static_cast<float*>(qtensor), q->dNx32.p);
+            by_mode(float(0), static_cast<const float*>(elemvec), static_cast<float*>(qtensor), q->dNx32.p);
         }
         check_launch();
     });

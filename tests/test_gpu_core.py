@@ -82,6 +82,37 @@ def test_quadrature_fp64_bit_exact(gf, et, distort):
     assert np.array_equal(host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C)), q.Int_gradN_dot_tensor4_dot_gradNT_dV(C))
 
 
+@pytest.mark.parametrize("rule", ["centroid", "gauss3"])
+def test_hex8_gradients_custom_rules(gf, rule):
+    """Hex8 gradients of rules with 1 point (the stored-geometry kernel) and 27 points (geometry
+    recomputed from the coordinates) bit-exact with the oracle, FP64; FP32 equal to FP32 arithmetic on
+    the FP64 dN/dx rounded once."""
+    import torch
+
+    c, conn = mesh(HEX8, 4, 0.2)
+    vec = oracle.Vector(conn, oracle.dofs(c.shape[0], 3))
+    x = vec.AsElement(c)
+    if rule == "centroid":
+        xi, w = np.zeros((1, 3)), np.array([8.0])
+    else:
+        a = np.sqrt(3.0 / 5.0)
+        p1, w1 = np.array([-a, 0.0, a]), np.array([5.0, 8.0, 5.0]) / 9.0
+        xi = np.array([[p, q, r] for p in p1 for q in p1 for r in p1])
+        w = np.array([wp * wq * wr for wp in w1 for wq in w1 for wr in w1])
+    q = oracle.Quadrature(HEX8, x, xi, w)
+    gq = gf.Element.Hex8.Quadrature(x, xi, w)
+    ue = np.random.default_rng(8).standard_normal((q.nelem, 8, 3))
+    for name in ("GradN_vector", "GradN_vector_T", "SymGradN_vector"):
+        assert np.array_equal(host(getattr(gq, name)(ue)), getattr(q, name)(ue)), name
+    u32 = ue.astype(np.float32)
+    g32 = q.GradN().astype(np.float32)
+    got = host(gq.GradN_vector(torch.from_numpy(u32).cuda()))
+    want = np.zeros((q.nelem, q.nip, 3, 3), dtype=np.float32)
+    for m in range(8):  # the kernel's accumulation order, in FP32
+        want += g32[:, :, m, :, None] * u32[:, None, m, None, :]
+    assert np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("et", [QUAD4_PLANAR, HEX8])
 def test_quadrature_fp32_tolerance(gf, et):
     import torch
