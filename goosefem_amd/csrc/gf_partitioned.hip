@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "gf_internal.hpp"
+#include "gf_wave.hpp"
 
 using namespace gf;
 
@@ -213,17 +214,11 @@ __global__ void k_node_from_part_sel(int64_t n, int64_t nnu, int which, const in
 // when done is set, so launches queued past convergence leave the result alone.
 constexpr int PB = 256, NPB = 256;
 
+// the workgroup's sum (DPP wavefront sums, waves added in order) stored by thread 0
 __device__ inline void block_sum_store(double v, double* out)
 {
-    __shared__ double sh[PB];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = PB / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
-    __syncthreads();
+    const double t = block_sum<PB>(v);
+    if (threadIdx.x == 0) *out = t;
 }
 
 __device__ inline double sum_partials(const double* partial)
@@ -234,22 +229,13 @@ __device__ inline double sum_partials(const double* partial)
 }
 
 // The sum of count partials for a whole block: thread t adds partials t, t + PB, ... in turn, then
-// a fixed-order tree in LDS; every thread of the block must call it.  Deterministic, with
+// the fixed-order workgroup sum; every thread of the block must call it.  Deterministic, with
 // count / PB loads per thread instead of count.
 __device__ inline double block_sum_partials(const double* partial, int count)
 {
-    __shared__ double sh[PB];
     double v = 0.0;
     for (int b = threadIdx.x; b < count; b += PB) v += partial[b];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int h = PB / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
+    return block_sum<PB>(v);
 }
 
 // start (x given): r = b - A x, z = r / diag, p = z; partials of r.z, r.r, b.b
@@ -286,7 +272,8 @@ __global__ void k_cg_init_scal(const double* __restrict__ partial, double* __res
     scal[2] = rr;
     scal[3] = bb;
     scal[4] = tol * tol * (bb > 0.0 ? bb : 1.0);
-    state[0] = rr <= scal[4] ? 1 : 0;
+    // 2: non-finite right-hand side, 3: non-finite residual of the start (both stop the kernels)
+    state[0] = !isfinite(bb) ? 2 : (!isfinite(rr) ? 3 : (rr <= scal[4] ? 1 : 0));
     state[1] = 0;
     state[2] = maxiter;
 }
@@ -881,19 +868,24 @@ void matpart_cg(gf_matpart A, const double* rhs, double* xu, bool warm, double t
     int32_t* state = A->state.p;
     const int32_t *rp = A->rowptr[0].p, *cl = A->col[0].p;
     const double* vl = A->val[0].p;
-    if (warm)
-        GF_HIP(hipMemcpyAsync(W.cx, xu, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    else
-        k_zero<<<nblocks(n, BS), BS, 0, s>>>(n, W.cx);
-    k_cg_init<<<NPB, PB, 0, s>>>(n, rp, cl, vl, rhs, W.cx, W.invd, W.cr, W.cz, W.cp, partial);
-    k_cg_init_scal<<<1, 64, 0, s>>>(partial, scal, state, tol, static_cast<int32_t>(maxiter));
-    check_launch();
     int32_t st[2] = {0, 0};
+    auto init = [&](bool from_x) {
+        if (from_x)
+            GF_HIP(hipMemcpyAsync(W.cx, xu, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        else
+            k_zero<<<nblocks(n, BS), BS, 0, s>>>(n, W.cx);
+        k_cg_init<<<NPB, PB, 0, s>>>(n, rp, cl, vl, rhs, W.cx, W.invd, W.cr, W.cz, W.cp, partial);
+        k_cg_init_scal<<<1, 64, 0, s>>>(partial, scal, state, tol, static_cast<int32_t>(maxiter));
+        check_launch();
+        GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipStreamSynchronize(s));
+    };
+    init(warm);
+    if (st[0] == 2) throw Error(GF_EINVAL, "non-finite right-hand side");
+    if (st[0] == 3) init(false); // a start with non-finite entries: solve from zero instead
     constexpr int64_t check_every = 16;
     const unsigned napply = nblocks(LPR * n, PB);
     double* pq_part = partial + 3 * NPB + 8; // one p.q partial per apply block
-    GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GF_HIP(hipStreamSynchronize(s));
     double* pb[2] = {W.cp, W.cp2};
     int64_t it = 0;
     while (!st[0] && it < maxiter) {
@@ -1312,6 +1304,7 @@ int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, i
             check_launch();
             GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
             GF_HIP(hipStreamSynchronize(s));
+            if (st[0] == 2) throw Error(GF_EINVAL, "non-finite right-hand side");
             constexpr int64_t check_every = 16;
             for (int64_t it = 0; !st[0] && it < maxiter;) {
                 for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {

@@ -221,6 +221,29 @@ def test_matrix_partitioned_solve(gf, setup, et):
     assert solver.iterations == 0 and np.array_equal(xw, ref_u)
 
 
+def test_partitioned_solve_rejects_bad_input(gf):
+    # a non-finite right-hand side fails at once with GF_EINVAL (before any CG iteration); a warm
+    # start with non-finite entries is replaced by the zero start, so the solve equals the cold one
+    coor, conn, dofs, iip = quad4_tied()
+    gA = gf.MatrixPartitioned(conn, dofs, iip)
+    gA.assemble(elastic_stiffness(QUAD4_PLANAR, conn, coor))
+    x = np.random.default_rng(12).standard_normal(gA.ndof)
+    b = host(gA.Dot(x))
+    solver = gf.MatrixPartitionedSolver(tol=1e-12)
+    bad = b.copy()
+    bad[int(np.asarray(gA.iiu)[0])] = np.nan  # an unknown DOF (a prescribed one is not read)
+    with pytest.raises(gf.GooseFEMError) as e:
+        solver.Solve(gA, bad, x)
+    assert e.value.code == gf.GF_EINVAL
+    cold = host(solver.Solve(gA, b, x))
+    it_cold = solver.iterations
+    assert it_cold > 0
+    xw = x.copy()
+    xw[np.asarray(gA.iiu)] = np.nan  # the start (the unknown entries); the numbering has holes
+    warm = host(solver.Solve(gA, b, xw, warm=True))
+    assert solver.iterations == it_cold and np.array_equal(warm, cold)
+
+
 def test_matrix_partitioned_reference_cases(gf):
     # tests/basic/MatrixParitioned.cpp:11-44 (Regular(2, 2), iip {0, 5, 7, 13}) and :46-108 (one
     # 10 x 10 matrix on dofs arange(10), iip {0, 2, 4}), with symmetric positive definite matrices:

@@ -85,6 +85,12 @@ def test_condensed_solve_against_dense(gf):
     ref = ov.AsNode(dense_solve(Kd, ty.Cdu, ty.Cdp, ty.nnu, ty.nnp, ov.AsDofs(b), ov.AsDofs(xin)))
     assert solver.relres < 1e-11 and solver.iterations > 0
     assert np.max(np.abs(got - ref)) < 1e-8 * np.max(np.abs(ref))
+    # a non-finite right-hand side fails at once (GF_EINVAL), not after the iteration limit
+    bad = b.copy()
+    bad[1, 0] = np.inf
+    with pytest.raises(gf.GooseFEMError) as e:
+        solver.solve(A, bad, xin.copy())
+    assert e.value.code == gf.GF_EINVAL
     # AsDofs_i: independent DOFs plus the tied contributions C_id b_d
     vec = gf.VectorPartitionedTyings(conn, dofs, ty.Cdu, ty.Cdp, ty.Cdi)
     bd = ov.AsDofs(b)
@@ -166,5 +172,6 @@ def test_periodic_elastoplastic_example(gf):
     assert it_gpu == it_cpu  # the same Newton path
     # the example stops Newton at a 1e-5 relative residual, so the two runs differ by the linear
     # solvers' rounding (CG to 1e-13 against a direct solve) carried through that last iterate
-    assert np.max(np.abs(d_gpu - d_cpu)) <= 1e-6 * np.max(np.abs(d_cpu))
+    # (measured 2.5e-8 relative)
+    assert np.max(np.abs(d_gpu - d_cpu)) <= 1e-7 * np.max(np.abs(d_cpu))
     assert np.max(np.abs(ep_gpu - ep_cpu)) <= 1e-5 * np.max(ep_cpu)
