@@ -112,6 +112,29 @@ inline void check_launch() { GF_HIP(hipGetLastError()); }
 
 inline bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; }
 
+// Ordered gather sum: src[idx[j]] over j in [b, e), added in ascending j (the reference's serial
+// scatter order) in double.  The loads of NB entries are issued together -- NB indices, clamped to
+// the list so that every address is in bounds, then NB values -- and only then added, the valid ones
+// in order; a loop that loads one entry per iteration waits twice per entry.  Same bits as that loop.
+template <int NB, class T>
+__device__ inline double ordered_gather_sum(int32_t b, int32_t e, const int32_t* __restrict__ idx,
+                                            const T* __restrict__ src)
+{
+    double s = 0.0;
+    for (int32_t j = b; j < e; j += NB) {
+        int32_t k[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) k[q] = idx[min(j + q, e - 1)];
+        T v[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) v[q] = src[k[q]];
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+            if (j + q < e) s += static_cast<double>(v[q]);
+    }
+    return s;
+}
+
 // Host-side parallel loop over [0, n) in contiguous chunks (set-up code only).
 template <class F>
 inline void parallel_for(int64_t n, F&& f)

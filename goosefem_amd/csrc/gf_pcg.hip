@@ -425,9 +425,7 @@ __global__ void k_elem_diag(int64_t nelem, const double* __restrict__ dNx, const
 
 __device__ inline double row_sum(const int32_t* ptr, const int32_t* idx, const double* fe, int64_t r)
 {
-    double s = 0.0;
-    for (int32_t j = ptr[r]; j < ptr[r + 1]; ++j) s += fe[idx[j]];
-    return s;
+    return ordered_gather_sum<4>(ptr[r], ptr[r + 1], idx, fe);
 }
 
 // block reduction of one value per thread into partial[blockIdx]

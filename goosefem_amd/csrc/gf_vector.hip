@@ -76,9 +76,7 @@ __global__ void k_row_sum(int64_t nrows, const int32_t* __restrict__ ptr, const 
 {
     int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (r >= nrows) return;
-    double s = 0.0;
-    for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) s += static_cast<double>(src[idx[k]]);
-    dst[r] = s;
+    dst[r] = ordered_gather_sum<4>(ptr[r], ptr[r + 1], idx, src);
 }
 
 // AsNode(elemvec): per node component, the last (e, m) of the node's row
@@ -112,9 +110,7 @@ __global__ void k_assemble_node(int64_t n, const int32_t* __restrict__ dofs, con
     int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (k >= n) return;
     int32_t d = dofs[k];
-    double s = 0.0;
-    for (int32_t j = ptr[d]; j < ptr[d + 1]; ++j) s += static_cast<double>(elemvec[idx[j]]);
-    nodevec[k] = s;
+    nodevec[k] = ordered_gather_sum<4>(ptr[d], ptr[d + 1], idx, elemvec);
 }
 
 } // namespace
