@@ -928,9 +928,9 @@ def main():
     del S, X
     torch.cuda.empty_cache()
     if not args.no_blocks:
-        # the metric's second half (configs 4 and 3): learned vs plain PCG and the Hex8 stiffness
+        # the metric's other workloads: learned vs plain PCG (all held-out fields) and the Hex8 stiffness
         if os.path.exists(args.ws_weights):
-            pl = pcg_learned(args.ws_weights, 2, cpu=not args.no_cpu, cpu_seconds=5.0)
+            pl = pcg_learned(args.ws_weights, args.ws_tests, cpu=not args.no_cpu, cpu_seconds=5.0)
             res["pcg_learned"] = {k: pl[k] for k in ("plain_s", "learned_s", "speedup_vs_plain", "iterations",
                                                      "inference_ms", "max_true_relres", "config", "per_field")}
             res["pcg_learned"]["unit"] = "s to true relative residual 1e-10 (inference included)"
@@ -969,12 +969,12 @@ def copy_ceiling(gf, gib=4):
 
 
 def explicit_general(nx, ny, steps, warmup):
-    """The general-mesh explicit path (element kernel with the geometry recomputed from the
-    coordinates over the elements in locality order, then the ordered per-DOF gather + update) on
-    (1) the same Regular mesh with its elements renumbered by a random permutation (which defeats
-    the structured-grid detection) and (2) a FineLayer mesh of about the same size (3.95 M
-    elements, coarsening layers above and below 1800 fine layers).  Algorithmic bytes per step:
-    64 ndof + 20 nelem, as for the headline."""
+    """The general-mesh explicit path (one kernel per step: per spatially compact block of 256
+    nodes, the forces of the elements touching them, geometry recomputed from the coordinates, into
+    LDS, then each node's ordered sum and update) on (1) the same Regular mesh with its elements
+    renumbered by a random permutation (which defeats the structured-grid detection) and (2) a
+    FineLayer mesh of about the same size (3.95 M elements, coarsening layers above and below 1800
+    fine layers).  Algorithmic bytes per step: 64 ndof + 20 nelem, as for the headline."""
     import torch
 
     import goosefem_amd as gf
@@ -993,7 +993,7 @@ def explicit_general(nx, ny, steps, warmup):
         assert not X.structured
         u0 = 1e-3 * gf.randn(1, vec.ndof)
         B = 64 * vec.ndof + 20 * mesh.nelem
-        row = {"nelem": mesh.nelem, "ndof": vec.ndof, "algorithmic_bytes_per_step": B}
+        row = {"nelem": mesh.nelem, "ndof": vec.ndof, "algorithmic_bytes_per_step": B, "path": X.path}
         for p_, nm in ((gf.F32, "f32"), (gf.F64, "f64"), (gf.F64_FAST, "f64_fast")):
             X.set_state(u0, np.zeros(vec.ndof), np.zeros(vec.ndof))
             X.step(warmup, p_)
@@ -1041,6 +1041,12 @@ def api_kernels(kind, n):
                           ("Int_gradN_dot_tensor2_dV", t_int, bytes_int)):
             gbs = scale * b * E / t / 1e9
             row[key] = {"us": t * 1e6, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+            if kind == "hex8" and nm == "f64" and key != "Int_gradN_dot_tensor2_dV":
+                # the FP64 Hex8 gradients recompute dN/dx from the element coordinates: they move
+                # coordinates 192 + u_e 192 + qtensor 576 B per element, not the stored 1536 B of dN/dx
+                moved = nne * nd * 8 * 2 + nip * 9 * 8
+                row[key].update(geometry="recomputed from the element coordinates",
+                                moved_bytes_per_element=moved, frac_moved=moved * E / t / 1e9 / HBM_PEAK_GBS)
         out[nm] = row
         del eps, fe, ue
     del u, quad, vec

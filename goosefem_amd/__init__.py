@@ -2013,6 +2013,14 @@ class ExplicitQuad4(_Handle):
         _call("gf_explicit_is_structured", self._h, ctypes.byref(s))
         return bool(s.value)
 
+    @property
+    def path(self):
+        """'structured' (tile kernel), 'blocks' (general mesh, fused per node block) or 'gather'
+        (general mesh with tied DOFs: element kernel + ordered per-DOF gather)."""
+        s = _i32()
+        _call("gf_explicit_path", self._h, ctypes.byref(s))
+        return ("structured", "blocks", "gather")[s.value]
+
     def _state_ptrs(self):
         u, v, a = _vp(), _vp(), _vp()
         _call("gf_explicit_state", self._h, ctypes.byref(u), ctypes.byref(v), ctypes.byref(a))

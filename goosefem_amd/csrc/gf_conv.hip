@@ -275,6 +275,108 @@ __global__ void __launch_bounds__(256) k_sum_block(int64_t n, const float* __res
     if (threadIdx.x == 0) out[0] = static_cast<float>(sh[0]);
 }
 
+// ---- multi-scale mean-squared error -----------------------------------------------------------------
+// e = pred - target on grids [B][Z][H][W][nc] (Z = 1 for D = 2).  Level s of the pyramid holds the
+// mean of e over cells 2^s wide, level s + 1 the mean of 2^D level-s cells.  The gradient of
+// L = sum e^2 + lam sum_s (2^D)^s sum_cells (mean_s - mu)^2 (mu: the grid's mean) is
+// 2 e + 2 lam sum_s (mean_s(cell of i) - mu) at fine cell i, since the centred cell means of a level
+// sum to zero; and (2^D)^s sum_cells m^2 = sum_i m(cell of i)^2, so the loss is a sum over fine cells.
+
+// level 1 from the fine error (src_fine != 0: src holds pred, src2 target) or level s + 1 from level s
+template <int D>
+__global__ void __launch_bounds__(256) k_err_pool(int64_t ncoarse, int Zc, int Hc, int Wc, int nc, int fine,
+                                                  const float* __restrict__ src, const float* __restrict__ src2,
+                                                  float* __restrict__ dst)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= ncoarse * nc) return;
+    const int c = static_cast<int>(t % nc);
+    int64_t r = t / nc;
+    const int x = static_cast<int>(r % Wc);
+    r /= Wc;
+    const int y = static_cast<int>(r % Hc);
+    r /= Hc;
+    const int z = static_cast<int>(r % Zc);
+    const int64_t b = r / Zc;
+    const int Wf = 2 * Wc, Hf = 2 * Hc, Zf = D == 3 ? 2 * Zc : 1;
+    float acc = 0.0f;
+#pragma unroll
+    for (int dz = 0; dz < (D == 3 ? 2 : 1); ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int64_t i =
+                    ((((b * Zf + (D == 3 ? 2 * z + dz : 0)) * Hf + 2 * y + dy) * Wf) + 2 * x + dx) * nc + c;
+                acc += fine ? src[i] - src2[i] : src[i];
+            }
+    dst[t] = acc * (D == 3 ? 0.125f : 0.25f);
+}
+
+// per (grid, channel) mean of the coarsest level (ncell cells per grid)
+__global__ void __launch_bounds__(64) k_err_mean(int64_t ngrid, int ncell, int nc, const float* __restrict__ top,
+                                                 float* __restrict__ mu)
+{
+    const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (t >= ngrid * nc) return;
+    const int64_t b = t / nc;
+    const int c = static_cast<int>(t % nc);
+    float acc = 0.0f;
+    for (int k = 0; k < ncell; ++k) acc += top[(b * ncell + k) * nc + c];
+    mu[t] = acc / static_cast<float>(ncell);
+}
+
+constexpr int MS_MAX_LEVELS = 12;
+struct MsLevels {
+    const float* lv[MS_MAX_LEVELS]; // level s + 1 at lv[s]
+};
+
+template <int D>
+__global__ void __launch_bounds__(256) k_mse_multiscale(int64_t P, int Z, int H, int W, int nc, int S, float lam,
+                                                        const float* __restrict__ pred,
+                                                        const float* __restrict__ target, MsLevels L,
+                                                        const float* __restrict__ mu, float scale, int cg,
+                                                        __bf16* __restrict__ g, float* __restrict__ part)
+{
+    __shared__ float sh[256];
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    float sq = 0.0f;
+    if (i < P * cg) {
+        const int64_t p = i / cg;
+        const int c = static_cast<int>(i - p * cg);
+        float gv = 0.0f;
+        if (c < nc) {
+            const float d = pred[p * nc + c] - target[p * nc + c];
+            int64_t r = p;
+            const int x = static_cast<int>(r % W);
+            r /= W;
+            const int y = static_cast<int>(r % H);
+            r /= H;
+            const int z = static_cast<int>(r % Z);
+            const int64_t b = r / Z;
+            const float m0 = mu[b * nc + c];
+            float ms = 0.0f, msq = 0.0f;
+            for (int s = 1; s <= S; ++s) {
+                const int Ws = W >> s, Hs = H >> s, Zs = D == 3 ? Z >> s : 1;
+                const int64_t cell = ((b * Zs + (D == 3 ? z >> s : 0)) * Hs + (y >> s)) * Ws + (x >> s);
+                const float m = L.lv[s - 1][cell * nc + c] - m0;
+                ms += m;
+                msq += m * m;
+            }
+            sq = d * d + lam * msq;
+            gv = scale * (2.0f * d + 2.0f * lam * ms) / static_cast<float>(P * nc);
+        }
+        g[i] = static_cast<__bf16>(gv);
+    }
+    sh[threadIdx.x] = sq;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 // ---- 2x2 (D = 2) / 2x2x2 (D = 3) pooling --------------------------------------------------------------
 // grids [N, Z, H, W, C] with Z = 1 for D = 2; the coarse grid halves every pooled dimension
 template <int D>
@@ -713,6 +815,69 @@ int gf_mse(int64_t P, int nc, const float* pred, const float* target, float scal
         hipStream_t s = S(stream);
         k_mse<<<nb, 256, 0, s>>>(P, nc, pred, target, scale, cg, static_cast<__bf16*>(g), part);
         k_sum_block<<<1, 256, 0, s>>>(nb, part, loss_sum);
+        check_launch();
+    });
+}
+
+namespace {
+// floats of the pyramid (levels 1..S), the means and the loss partials
+int64_t mse_multiscale_floats(int dim, int64_t B, int Z, int H, int W, int nc, int S, int cg)
+{
+    int64_t n = 0;
+    for (int s = 1; s <= S; ++s) n += B * (dim == 3 ? (Z >> s) : 1) * (H >> s) * (W >> s) * nc;
+    const int64_t P = B * Z * H * W;
+    return n + B * nc + (P * cg + 255) / 256;
+}
+} // namespace
+
+int gf_mse_multiscale_work(int dim, int64_t B, int Z, int H, int W, int nc, int S, int cg, int64_t* nfloats)
+{
+    return guard([&] {
+        GF_REQUIRE(nfloats && (dim == 2 || dim == 3) && B > 0 && nc > 0 && cg >= nc && S >= 1 && S <= MS_MAX_LEVELS,
+                   "bad argument");
+        *nfloats = mse_multiscale_floats(dim, B, dim == 3 ? Z : 1, H, W, nc, S, cg);
+    });
+}
+
+int gf_mse_multiscale(int dim, int64_t B, int Z, int H, int W, int nc, int levels, float lam, const float* pred,
+                      const float* target, float scale, int cg, void* g, float* work, int64_t nwork, float* loss_sum,
+                      void* stream)
+{
+    return guard([&] {
+        GF_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+        if (dim == 2) Z = 1;
+        GF_REQUIRE(B > 0 && Z > 0 && H > 0 && W > 0 && nc > 0 && cg >= nc && levels >= 1 && levels <= MS_MAX_LEVELS,
+                   "bad argument");
+        const int f = 1 << levels;
+        GF_REQUIRE(H % f == 0 && W % f == 0 && (dim == 2 || Z % f == 0), "2^S must divide every grid dimension");
+        GF_REQUIRE(pred && target && g && work && loss_sum, "null argument");
+        GF_REQUIRE(nwork >= mse_multiscale_floats(dim, B, Z, H, W, nc, levels, cg), "workspace too small");
+        hipStream_t s = S(stream);
+        MsLevels L{};
+        float* w = work;
+        for (int k = 1; k <= levels; ++k) {
+            const int Zc = dim == 3 ? Z >> k : 1, Hc = H >> k, Wc = W >> k;
+            const int64_t ncoarse = B * Zc * Hc * Wc;
+            const float* src = k == 1 ? pred : L.lv[k - 2];
+            if (dim == 3)
+                k_err_pool<3><<<nblocks(ncoarse * nc, 256), 256, 0, s>>>(ncoarse, Zc, Hc, Wc, nc, k == 1, src, target, w);
+            else
+                k_err_pool<2><<<nblocks(ncoarse * nc, 256), 256, 0, s>>>(ncoarse, Zc, Hc, Wc, nc, k == 1, src, target, w);
+            L.lv[k - 1] = w;
+            w += ncoarse * nc;
+        }
+        float* mu = w;
+        w += B * nc;
+        const int ncell = (dim == 3 ? Z >> levels : 1) * (H >> levels) * (W >> levels);
+        k_err_mean<<<nblocks(B * nc, 64), 64, 0, s>>>(B, ncell, nc, L.lv[levels - 1], mu);
+        const int64_t P = B * Z * H * W, nb = nblocks(P * cg, 256);
+        if (dim == 3)
+            k_mse_multiscale<3><<<nb, 256, 0, s>>>(P, Z, H, W, nc, levels, lam, pred, target, L, mu, scale, cg,
+                                                   static_cast<__bf16*>(g), w);
+        else
+            k_mse_multiscale<2><<<nb, 256, 0, s>>>(P, Z, H, W, nc, levels, lam, pred, target, L, mu, scale, cg,
+                                                   static_cast<__bf16*>(g), w);
+        k_sum_block<<<1, 256, 0, s>>>(nb, w, loss_sum);
         check_launch();
     });
 }

@@ -22,6 +22,7 @@
 // order, FP32, or FP64 with FMA) over the elements sorted by their lowest node, then the
 // DOF-centric ordered gather + update, which also forms the next step's displacement.
 #include <cmath>
+#include <functional>
 #include <cstdio>
 #include <cstdlib>
 #include <type_traits>
@@ -57,6 +58,17 @@ struct gf_explicit_s {
     DBuf<float> gK, gG;
     DBuf<double> fe64;
     DBuf<float> fe32;
+    // fused general step (when each DOF belongs to exactly one node component): node blocks of BS
+    // spatially compact nodes, each with its local nodes (its own, then the halo nodes its elements
+    // reach), the elements that touch its nodes (ascending locality position) with their
+    // connectivity in local node numbers and their moduli, and per own node its (element slot, m)
+    // contributions in the reference's (e, m) order as slot * 4 + m
+    bool blocks = false;
+    int64_t nblk = 0;
+    int64_t lds64 = 0, lds32 = 0;                   // largest LDS need of a block, FP64 / FP32 forces
+    DBuf<int32_t> bkptr, blptr, bnloc, beptr, bnptr; // own nodes, local nodes (own, then halo), element slots
+    DBuf<uint16_t> blconn, bnent;              // element slots' local connectivity; node contributions
+    DBuf<float> bK, bG;                        // moduli per element slot
     DBuf<double> epart; // energy reduction partials [2][NEB]
 };
 
@@ -491,14 +503,124 @@ __global__ void __launch_bounds__(BS) k_node_update(int64_t ndof, const int32_t*
 {
     int64_t d = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (d >= ndof) return;
-    double s = 0.0;
-    for (int32_t j = ptr[d]; j < ptr[d + 1]; ++j) s += static_cast<double>(fe[idx[j]]);
+    const double s = ordered_gather_sum<4>(ptr[d], ptr[d + 1], idx, fe);
     double an = inv_mass[d] * ((fext ? fext[d] : 0.0) - s);
     double vn = v0[d] + hdt * (a0[d] + an);
     v1[d] = vn;
     a1[d] = an;
     if (u_next) u_next[d] = u[d] + dt * vn + c * an;
     if (fint_out) fint_out[d] = s;
+}
+
+// The general step fused per node block.  (0) The block's local nodes -- its own BS nodes, then the
+// halo nodes of its elements -- are staged in LDS: coordinates and displacements, one contiguous list
+// per block.  (1) The forces of its elements (halo elements, which touch two blocks, are computed by
+// each) go to LDS, computed from the staged nodes through the local connectivity.  (2) Each thread
+// sums its own node's contributions from LDS in the reference's order and updates the node's two
+// DOFs as k_node_update does.  No element-force array and no per-DOF gather list cross HBM.
+template <int BODY>
+__global__ void __launch_bounds__(BS) k_explicit_blocks(
+    ShapeTable T, const int32_t* __restrict__ bkptr, const int32_t* __restrict__ blptr, const int32_t* __restrict__ bnloc,
+    const int32_t* __restrict__ beptr, const uint16_t* __restrict__ blconn, const float* __restrict__ bK,
+    const float* __restrict__ bG, const int32_t* __restrict__ dofs, const double* __restrict__ coor,
+    const int32_t* __restrict__ bnptr, const uint16_t* __restrict__ bnent, const double* __restrict__ inv_mass,
+    const double* __restrict__ fext, double dt, double c, double hdt, const double* __restrict__ v0,
+    const double* __restrict__ a0, double* __restrict__ v1, double* __restrict__ a1, const double* __restrict__ u,
+    double* __restrict__ u_next, double* __restrict__ fint_out)
+{
+    using FT = typename std::conditional<body_f32(BODY), float, double>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int32_t lb = blptr[blockIdx.x], nl = blptr[blockIdx.x + 1] - lb;
+    const int32_t eb = beptr[blockIdx.x], ne = beptr[blockIdx.x + 1] - eb;
+    const int32_t kb = bkptr[blockIdx.x], own = bkptr[blockIdx.x + 1] - kb;
+    FT* s_f = reinterpret_cast<FT*>(smem);                          // [ne][4][2] element forces
+    double2* s_x = reinterpret_cast<double2*>(smem + ne * 8 * sizeof(FT)); // [nl] coordinates
+    double2* s_u = s_x + nl;                                        // [nl] displacements
+    const int64_t k = static_cast<int64_t>(kb) + threadIdx.x;
+    const bool active = static_cast<int>(threadIdx.x) < own;
+    // (0) local nodes; thread t's own node is local node t
+    int2 dme = make_int2(0, 0);
+    for (int base = 0; base < nl; base += 2 * BS) {
+        // two local nodes per thread and pass, their loads issued together (indices clamped)
+        const int l0 = base + threadIdx.x, l1 = l0 + BS;
+        const int32_t n0 = bnloc[lb + min(l0, nl - 1)], n1 = bnloc[lb + min(l1, nl - 1)];
+        const double2 x0 = reinterpret_cast<const double2*>(coor)[n0], x1 = reinterpret_cast<const double2*>(coor)[n1];
+        const int2 d0 = reinterpret_cast<const int2*>(dofs)[n0], d1 = reinterpret_cast<const int2*>(dofs)[n1];
+        const double2 u0 = make_double2(u[d0.x], u[d0.y]), u1 = make_double2(u[d1.x], u[d1.y]);
+        if (l0 < nl) {
+            s_x[l0] = x0;
+            s_u[l0] = u0;
+        }
+        if (l1 < nl) {
+            s_x[l1] = x1;
+            s_u[l1] = u1;
+        }
+        if (base == 0) dme = d0;
+    }
+    __syncthreads();
+    // the own node's operands, in flight while the elements are computed
+    const int d[2] = {dme.x, dme.y};
+    double im[2] = {0.0, 0.0}, vo[2] = {0.0, 0.0}, ao[2] = {0.0, 0.0}, fx[2] = {0.0, 0.0};
+    int32_t j0 = 0, j1 = 0;
+    if (active) {
+        j0 = bnptr[k];
+        j1 = bnptr[k + 1];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            im[i] = inv_mass[d[i]];
+            vo[i] = v0[d[i]];
+            ao[i] = a0[d[i]];
+            fx[i] = fext ? fext[d[i]] : 0.0;
+        }
+    }
+    // (1) element forces from the staged nodes
+    for (int le = threadIdx.x; le < ne; le += BS) {
+        const ushort4 lc = reinterpret_cast<const ushort4*>(blconn)[eb + le];
+        const int ln[4] = {lc.x, lc.y, lc.z, lc.w};
+        double x[4][2], ue[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double2 xx = s_x[ln[m]], uu = s_u[ln[m]];
+            x[m][0] = xx.x;
+            x[m][1] = xx.y;
+            ue[m][0] = uu.x;
+            ue[m][1] = uu.y;
+        }
+        FT f[4][2];
+        element_force_body<BODY, FT>(T, FT(0), FT(0), FT(0), x, ue, bK[eb + le], bG[eb + le], f);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            s_f[le * 8 + m * 2] = f[m][0];
+            s_f[le * 8 + m * 2 + 1] = f[m][1];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    // (2) the own node's ordered sums and update
+    double s0 = 0.0, s1 = 0.0;
+    for (int32_t j = j0; j < j1; j += 4) {
+        int ent[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ent[q] = bnent[min(j + q, j1 - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (j + q < j1) {
+                const FT* f = s_f + (ent[q] >> 2) * 8 + (ent[q] & 3) * 2;
+                s0 += static_cast<double>(f[0]);
+                s1 += static_cast<double>(f[1]);
+            }
+    }
+    const double2 uo = s_u[threadIdx.x];
+    const double sum[2] = {s0, s1}, uu[2] = {uo.x, uo.y};
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double an = im[i] * (fx[i] - sum[i]);
+        const double vn = vo[i] + hdt * (ao[i] + an);
+        v1[d[i]] = vn;
+        a1[d[i]] = an;
+        if (u_next) u_next[d[i]] = uu[i] + dt * vn + c * an;
+        if (fint_out) fint_out[d[i]] = sum[i];
+    }
 }
 
 // gather list entry k = (e nne + m) nd + i of the original element e -> the same entry of e's
@@ -651,6 +773,151 @@ void launch_structured(int body, unsigned grid, hipStream_t s, int nx, int ny, i
     }
 }
 
+// Node blocks of the fused general step (k_explicit_blocks).  Used when every DOF belongs to
+// exactly one (node, component) -- a bijection onto [0, ndof) -- so that a DOF's contributions are
+// those of one node; tied meshes keep the element kernel + per-DOF gather.  The blocks come from a
+// recursive coordinate bisection of the nodes: a set is split across its longer extent with a
+// multiple of BS nodes on the lower side, until it holds at most BS, so every block is a compact
+// patch of BS nodes (one block may hold fewer).  Each block's nodes are then put in ascending
+// number so that neighbouring threads touch neighbouring DOFs.  rank[e]: position of original
+// element e in the locality order; cp, kp, gp: connectivity [4] and moduli in that order.  Blocks
+// whose local nodes and element forces would not fit in 64 KiB of LDS keep the two-kernel path.
+void build_node_blocks(gf_explicit_s* x, gf_vector v, const double* coor_d, const std::vector<int32_t>& rank,
+                       const std::vector<int32_t>& cp, const std::vector<float>& kp, const std::vector<float>& gp)
+{
+    const int64_t nn = v->nnode, E = v->nelem, nd = v->ndof;
+    if (nd != 2 * nn || nn == 0 || E == 0) return;
+    {
+        std::vector<char> seen(nd, 0);
+        for (int64_t k = 0; k < 2 * nn; ++k) {
+            const int64_t d = v->dofs_h[k];
+            if (d < 0 || d >= nd || seen[d]) return;
+            seen[d] = 1;
+        }
+    }
+    std::vector<double> xy(2 * nn);
+    GF_HIP(hipMemcpy(xy.data(), coor_d, xy.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int32_t> node(nn);
+    for (int64_t n = 0; n < nn; ++n) node[n] = static_cast<int32_t>(n);
+    // recursive coordinate bisection, iteratively over a stack of node ranges
+    std::vector<std::pair<int64_t, int64_t>> todo{{0, nn}};
+    while (!todo.empty()) {
+        const auto [b, e] = todo.back();
+        todo.pop_back();
+        const int64_t n = e - b;
+        if (n <= BS) continue;
+        double l[2] = {xy[2 * node[b]], xy[2 * node[b] + 1]}, h[2] = {l[0], l[1]};
+        for (int64_t k = b; k < e; ++k)
+            for (int i = 0; i < 2; ++i) {
+                l[i] = std::min(l[i], xy[2 * node[k] + i]);
+                h[i] = std::max(h[i], xy[2 * node[k] + i]);
+            }
+        const int dim = (h[0] - l[0] >= h[1] - l[1]) ? 0 : 1;
+        const int64_t lower = std::max<int64_t>(BS, ((n + BS) / (2 * BS)) * BS); // a multiple of BS, < n
+        std::nth_element(node.begin() + b, node.begin() + b + lower, node.begin() + e, [&](int32_t p, int32_t q) {
+            const double xp = xy[2 * p + dim], xq = xy[2 * q + dim];
+            return xp < xq || (xp == xq && p < q);
+        });
+        todo.push_back({b, b + lower});
+        todo.push_back({b + lower, e});
+    }
+    // (e, m) of every node, e ascending then m: the reference's scatter order for each component
+    std::vector<int64_t> nptr(nn + 1, 0);
+    for (int64_t k = 0; k < 4 * E; ++k) ++nptr[v->conn_h[k] + 1];
+    for (int64_t n = 0; n < nn; ++n) nptr[n + 1] += nptr[n];
+    std::vector<int32_t> nel(4 * E);
+    {
+        std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
+        for (int64_t k = 0; k < 4 * E; ++k) nel[fill[v->conn_h[k]]++] = static_cast<int32_t>(k); // k = e * 4 + m
+    }
+    std::vector<int32_t> bkptr{0}, beptr{0}, blptr{0}, bnloc, bnptr(nn + 1, 0);
+    std::vector<uint16_t> bnent, blconn;
+    std::vector<float> bK, bG;
+    bnent.reserve(4 * E);
+    blconn.reserve(4 * (E + E / 4));
+    bnloc.reserve(nn + nn / 3);
+    int64_t lds64 = 0, lds32 = 0;
+    std::vector<int32_t> el, halo, lidx(nn, -1);
+    // node[k0, k1) as one block, or bisected again while its local nodes and element forces need more
+    // than LDS_CAP bytes (FP64); blocks are emitted in node-array order
+    constexpr int64_t LDS_CAP = 40960;
+    std::function<bool(int64_t, int64_t)> emit = [&](int64_t k0, int64_t k1) -> bool {
+        el.clear();
+        for (int64_t k = k0; k < k1; ++k)
+            for (int64_t j = nptr[node[k]]; j < nptr[node[k] + 1]; ++j) el.push_back(rank[nel[j] / 4]);
+        std::sort(el.begin(), el.end());
+        el.erase(std::unique(el.begin(), el.end()), el.end());
+        for (int64_t k = k0; k < k1; ++k) lidx[node[k]] = 0;
+        halo.clear();
+        for (int32_t r : el)
+            for (int m = 0; m < 4; ++m)
+                if (lidx[cp[4 * r + m]] < 0) halo.push_back(cp[4 * r + m]);
+        for (int64_t k = k0; k < k1; ++k) lidx[node[k]] = -1;
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        const int64_t own = k1 - k0, nl = own + static_cast<int64_t>(halo.size()), ne = static_cast<int64_t>(el.size());
+        if (nl * 32 + ne * 64 > LDS_CAP && own > 1) {
+            double l[2] = {xy[2 * node[k0]], xy[2 * node[k0] + 1]}, h[2] = {l[0], l[1]};
+            for (int64_t k = k0; k < k1; ++k)
+                for (int i = 0; i < 2; ++i) {
+                    l[i] = std::min(l[i], xy[2 * node[k] + i]);
+                    h[i] = std::max(h[i], xy[2 * node[k] + i]);
+                }
+            const int dim = (h[0] - l[0] >= h[1] - l[1]) ? 0 : 1;
+            const int64_t mid = k0 + own / 2;
+            std::nth_element(node.begin() + k0, node.begin() + mid, node.begin() + k1, [&](int32_t p, int32_t q) {
+                const double xp = xy[2 * p + dim], xq = xy[2 * q + dim];
+                return xp < xq || (xp == xq && p < q);
+            });
+            return emit(k0, mid) && emit(mid, k1);
+        }
+        if (nl > 65535 || nl * 32 + ne * 64 > 65536) return false; // beyond LDS: the two-kernel path
+        // own nodes in ascending number (local number k - k0), then the halo nodes, ascending
+        std::sort(node.begin() + k0, node.begin() + k1);
+        for (int64_t k = k0; k < k1; ++k) lidx[node[k]] = static_cast<int32_t>(k - k0);
+        for (size_t h = 0; h < halo.size(); ++h) lidx[halo[h]] = static_cast<int32_t>(own + h);
+        lds64 = std::max(lds64, nl * 32 + ne * 64);
+        lds32 = std::max(lds32, nl * 32 + ne * 32);
+        bnloc.insert(bnloc.end(), node.begin() + k0, node.begin() + k1);
+        bnloc.insert(bnloc.end(), halo.begin(), halo.end());
+        blptr.push_back(static_cast<int32_t>(bnloc.size()));
+        for (int32_t r : el) {
+            for (int m = 0; m < 4; ++m) blconn.push_back(static_cast<uint16_t>(lidx[cp[4 * r + m]]));
+            bK.push_back(kp[r]);
+            bG.push_back(gp[r]);
+        }
+        beptr.push_back(beptr.back() + static_cast<int32_t>(ne));
+        bkptr.push_back(static_cast<int32_t>(k1));
+        for (int64_t k = k0; k < k1; ++k) {
+            for (int64_t j = nptr[node[k]]; j < nptr[node[k] + 1]; ++j) {
+                const int32_t e = nel[j] / 4, m = nel[j] % 4;
+                const int32_t slot =
+                    static_cast<int32_t>(std::lower_bound(el.begin(), el.end(), rank[e]) - el.begin());
+                bnent.push_back(static_cast<uint16_t>(slot * 4 + m));
+            }
+            bnptr[k + 1] = static_cast<int32_t>(bnent.size());
+        }
+        for (int64_t k = k0; k < k1; ++k) lidx[node[k]] = -1;
+        for (int32_t n : halo) lidx[n] = -1;
+        return true;
+    };
+    for (int64_t k0 = 0; k0 < nn; k0 += BS)
+        if (!emit(k0, std::min(nn, k0 + BS))) return;
+    x->bkptr.upload(bkptr);
+    x->blptr.upload(blptr);
+    x->bnloc.upload(bnloc);
+    x->beptr.upload(beptr);
+    x->blconn.upload(blconn);
+    x->bK.upload(bK);
+    x->bG.upload(bG);
+    x->bnptr.upload(bnptr);
+    x->bnent.upload(bnent);
+    x->nblk = static_cast<int64_t>(bkptr.size()) - 1;
+    x->lds64 = lds64;
+    x->lds32 = lds32;
+    x->blocks = true;
+}
+
 } // namespace
 
 extern "C" {
@@ -736,15 +1003,18 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
                 }
                 x->gK.upload(kp);
                 x->gG.upload(gp);
-                DBuf<int32_t> drank;
-                drank.upload(rank);
-                const int64_t nidx = E * 8;
-                x->geidx.alloc(nidx);
-                k_remap_eidx<<<nblocks(nidx, BS), BS>>>(nidx, v->dof_eidx.p, drank.p, x->geidx.p);
-                check_launch();
-                GF_HIP(hipDeviceSynchronize());
-                x->fe64.alloc(E * 8);
-                x->fe32.alloc(E * 8);
+                build_node_blocks(x, v, coor, rank, cp, kp, gp);
+                if (!x->blocks) {
+                    DBuf<int32_t> drank;
+                    drank.upload(rank);
+                    const int64_t nidx = E * 8;
+                    x->geidx.alloc(nidx);
+                    k_remap_eidx<<<nblocks(nidx, BS), BS>>>(nidx, v->dof_eidx.p, drank.p, x->geidx.p);
+                    check_launch();
+                    GF_HIP(hipDeviceSynchronize());
+                    x->fe64.alloc(E * 8);
+                    x->fe32.alloc(E * 8);
+                }
             }
             GF_HIP(hipDeviceSynchronize());
         }
@@ -769,6 +1039,14 @@ int gf_explicit_state(gf_explicit x, double** u, double** v, double** a)
         if (u) *u = x->u[x->cur].p;
         if (v) *v = x->vv[x->cur].p;
         if (a) *a = x->a[x->cur].p;
+    });
+}
+
+int gf_explicit_path(gf_explicit x, int* path)
+{
+    return guard([&] {
+        GF_REQUIRE(x && path, "null argument");
+        *path = x->structured ? 0 : (x->blocks ? 1 : 2);
     });
 }
 
@@ -849,7 +1127,29 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
             if (nsteps > 0)
                 k_update_u<<<gd, BS, 0, s>>>(x->ndof, dt, c, x->u[x->cur].p, x->vv[x->cur].p, x->a[x->cur].p,
                                             x->u[1 - x->cur].p);
-            for (int64_t k = 0; k < nsteps; ++k) {
+            for (int64_t k = 0; k < nsteps && x->blocks; ++k) {
+                const int i0 = x->cur, i1 = 1 - x->cur;
+                const bool last = k == nsteps - 1;
+                double* fo = (store_fint && last) ? x->fint.p : nullptr;
+                const double* un = x->u[i1].p; // this step's displacement
+                double* unext = last ? nullptr : x->u[i0].p;
+                const unsigned g = static_cast<unsigned>(x->nblk);
+                auto run = [&](auto body) {
+                    constexpr int B = decltype(body)::value;
+                    const size_t lds = static_cast<size_t>(body_f32(B) ? x->lds32 : x->lds64);
+                    k_explicit_blocks<B><<<g, BS, lds, s>>>(T, x->bkptr.p, x->blptr.p, x->bnloc.p, x->beptr.p,
+                                                            x->blconn.p, x->bK.p, x->bG.p, v->dofs.p, x->coor.p,
+                                                            x->bnptr.p, x->bnent.p, x->inv_mass.p, fext, dt, c, hdt,
+                                                            x->vv[i0].p, x->a[i0].p, x->vv[i1].p, x->a[i1].p, un, unext,
+                                                            fo);
+                };
+                if (precision == GF_F32) run(std::integral_constant<int, B_F32>());
+                else if (precision == GF_F64) run(std::integral_constant<int, B_REF64>());
+                else run(std::integral_constant<int, B_F64>());
+                check_launch();
+                x->cur = i1;
+            }
+            for (int64_t k = 0; k < nsteps && !x->blocks; ++k) {
                 const int i0 = x->cur, i1 = 1 - x->cur;
                 const bool last = k == nsteps - 1;
                 double* fo = (store_fint && last) ? x->fint.p : nullptr;

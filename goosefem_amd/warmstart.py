@@ -285,11 +285,26 @@ class WarmStartNet:
             if l > 0:
                 conv(bf["gA"][l], self.ch[l], P, h, w, c1, bf["gI"][l], backward=True, Z=z)
 
-    def loss_grad(self, B, H, W, target, scale=1.0, Z=1):
-        """MSE of the prediction against target [B*Z*H*W*nout] FP32; writes gY; returns the device sum
-        of squared errors (a 1-element tensor)."""
+    def loss_grad(self, B, H, W, target, scale=1.0, Z=1, lowpass=0.0, levels=0):
+        """Loss of the prediction against target [B*Z*H*W*nout] FP32; writes gY; returns the device
+        loss (a 1-element tensor).  lowpass = 0: the sum of squared errors (gf_mse).  lowpass > 0:
+        gf_mse_multiscale, which adds lowpass times the squared centred means of the error over cells
+        2, 4, ..., 2^levels wide, each weighted by its cell size, so that error smooth over a cell
+        counts (levels + 1) times and white error barely more than once."""
         bf = self.buffers(B, H, W, Z)
         P0 = bf["lv"][0][0]
+        if lowpass > 0.0:
+            torch = _torch()
+            zz = Z if self.dim == 3 else 1
+            nf = _i64()
+            _call("gf_mse_multiscale_work", _i32(self.dim), _i64(B), _i32(zz), _i32(H), _i32(W), _i32(self.nout),
+                  _i32(levels), _i32(OUT_CH), ctypes.byref(nf))
+            if bf.get("ms_work") is None or bf["ms_work"].numel() < nf.value:
+                bf["ms_work"] = torch.zeros(nf.value, dtype=torch.float32, device=device())
+            _call("gf_mse_multiscale", _i32(self.dim), _i64(B), _i32(zz), _i32(H), _i32(W), _i32(self.nout),
+                  _i32(levels), _f32(lowpass), _dp(bf["pred"]), _dp(target), _f32(scale), _i32(OUT_CH), _dp(bf["gY"]),
+                  _dp(bf["ms_work"]), _i64(bf["ms_work"].numel()), _dp(bf["loss"]), _stream())
+            return bf["loss"]
         _call("gf_mse", _i64(P0), _i32(self.nout), _dp(bf["pred"]), _dp(target), _f32(scale), _i32(OUT_CH),
               _dp(bf["gY"]), _dp(bf["loss_part"]), _dp(bf["loss"]), _stream())
         return bf["loss"]
@@ -434,8 +449,10 @@ class PeriodicWarmStart:
             errs.append(float(((p - t).norm() / t.norm()).item()))
         return float(np.mean(errs))
 
-    def train(self, Gs, T, epochs, batch=4, lr=2e-3, lr_min=1e-5, log_every=1, time_limit=None):
-        """Adam with cosine learning-rate decay on (Gs, T); returns per-epoch mean losses."""
+    def train(self, Gs, T, epochs, batch=4, lr=2e-3, lr_min=1e-5, log_every=1, time_limit=None, lowpass=0.0,
+              levels=0):
+        """Adam with cosine learning-rate decay on (Gs, T); returns per-epoch mean losses.  lowpass,
+        levels: the multi-scale loss of WarmStartNet.loss_grad."""
         torch = _torch()
         H, W, Z = self._dims()
         ns = Gs.shape[0]
@@ -456,7 +473,7 @@ class PeriodicWarmStart:
                 net.set_input(batch, H, W, [Gs[i] for i in ids], 0.0, 1.0 / self.sigma, Z=Z)
                 tgt.copy_(T[torch.from_numpy(ids).to(device())].reshape(-1))
                 net.forward(batch, H, W, Z=Z)
-                loss = net.loss_grad(batch, H, W, tgt, Z=Z)
+                loss = net.loss_grad(batch, H, W, tgt, Z=Z, lowpass=lowpass, levels=levels)
                 acc += loss.double()
                 net.backward(batch, H, W, Z=Z)
                 lr_t = lr_min + 0.5 * (lr - lr_min) * (1.0 + np.cos(np.pi * step / max(total - 1, 1)))
@@ -580,6 +597,12 @@ def train_main(argv=None):
     ap.add_argument("--out", required=True, help="weights file (.npz); with several configurations a suffix is added")
     ap.add_argument("--init", default=None,
                     help="continue training these weights (one configuration; their target scale is kept)")
+    ap.add_argument("--lowpass", type=float, default=0.0,
+                    help="weight of the multi-scale loss term (gf_mse_multiscale): error smooth over 2^s-wide "
+                         "cells, s = 1..--lowpass-levels, counts this much more per level")
+    ap.add_argument("--lowpass-levels", type=int, default=0,
+                    help="levels of the multi-scale term (default: the deepest the grid allows, at most 8)")
+    ap.add_argument("--lr-min", type=float, default=1e-5, help="final learning rate of the cosine decay")
     a = ap.parse_args(argv)
     pb = periodic_problem(a.n, dim=a.dim, spheres=a.dim == 3)
     shear = "pure" if a.dim == 3 else "simple"
@@ -607,7 +630,13 @@ def train_main(argv=None):
         prev_seconds = float(net.meta.get("train_seconds", 0.0)) if init is not None else 0.0
         W = PeriodicWarmStart(pb.P, pb.ty, a.n, net, sigma=pb.sigma, target_scale=scale, dim=a.dim, shear=shear)
         t0 = time.perf_counter()
-        hist = W.train(Gtr, Ttr, a.epochs, batch=a.batch, lr=lr, time_limit=a.time_limit)
+        levels = a.lowpass_levels
+        if a.lowpass > 0.0 and levels <= 0:
+            levels = 0
+            while levels < 8 and a.n % (2 << levels) == 0:
+                levels += 1
+        hist = W.train(Gtr, Ttr, a.epochs, batch=a.batch, lr=lr, lr_min=a.lr_min, time_limit=a.time_limit,
+                       lowpass=a.lowpass, levels=levels)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
         err = W.holdout_error(Gho, Tho) if a.holdout else float("nan")
@@ -615,7 +644,10 @@ def train_main(argv=None):
                     "samples": a.samples, "epochs": prev_epochs + len(hist),
                     "batch": a.batch, "lr": lr, "data_seconds": t_data, "train_seconds": prev_seconds + t_train,
                     "final_mse": hist[-1],
-                    "holdout_rel_error": err, "data_tol": a.data_tol, "seed0": a.seed0, "params": net.nparams()}
+                    "holdout_rel_error": err, "data_tol": a.data_tol, "seed0": a.seed0, "params": net.nparams(),
+                    "lowpass": a.lowpass, "lowpass_levels": levels,
+                    "lowpass_epochs": (int(net.meta.get("lowpass_epochs", 0)) if init is not None else 0)
+                    + (len(hist) if a.lowpass > 0.0 else 0)}
         out = a.out
         if len(cfgs) > 1:
             stem, ext = os.path.splitext(a.out)

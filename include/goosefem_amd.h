@@ -531,6 +531,10 @@ int gf_explicit_energy(gf_explicit x, const double* mass, double* ekin, double* 
 int gf_explicit_fint(gf_explicit x, const double** fint);
 /* 1 if the structured-grid fused kernel is used (conn is a Regular Quad4 grid), else 0 */
 int gf_explicit_is_structured(gf_explicit x, int* structured);
+/* which kernels run the step (no counterpart in the reference; a diagnostic): 0 the structured-grid
+ * tile kernel, 1 the general mesh fused per node block (every DOF belongs to one node component),
+ * 2 the general mesh as element kernel + ordered per-DOF gather (tied DOFs) */
+int gf_explicit_path(gf_explicit x, int* path);
 
 /* ================================================================================================
  * Tied-system solver: FP64 Jacobi-preconditioned CG on the operator of MatrixPartitionedTyings
@@ -624,6 +628,18 @@ int gf_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, f
  * floats of workspace */
 int gf_mse(int64_t P, int nc, const float* pred, const float* target, float scale, int cg, void* g, float* part,
            float* loss_sum, void* stream);
+/* multi-scale mean squared error of pred vs target on B grids [B][Z][H][W][nc] FP32 (dim 2: Z = 1;
+ * dim 3: cells 2^s wide in all three directions): with e = pred - target, mu each grid's and
+ * channel's mean of e and m_s the mean of e over the cell of width 2^s (s = 1..S),
+ *   L = sum e^2 + lam sum_s (2^dim)^s sum_cells (m_s - mu)^2,
+ * which weights error that is smooth over a cell of width 2^s like the plain term and white error
+ * (2^dim)^s times less.  Writes g = scale dL/de / (P nc) as bfloat16 into channels 0..nc-1 of
+ * [P][cg] (others zero) and *loss_sum (device) = L.  2^S must divide H, W (and Z).  work: at least
+ * gf_mse_multiscale_work(...) floats.  No counterpart in the reference (training of the warm start). */
+int gf_mse_multiscale_work(int dim, int64_t B, int Z, int H, int W, int nc, int S, int cg, int64_t* nfloats);
+int gf_mse_multiscale(int dim, int64_t B, int Z, int H, int W, int nc, int S, float lam, const float* pred,
+                      const float* target, float scale, int cg, void* g, float* work, int64_t nwork, float* loss_sum,
+                      void* stream);
 /* x_u[idx[k]] = gamma * (scale * (grid[k] - grid[k % nc]) + affine[k]) where idx[k] >= 0 */
 int gf_grid_to_dofs(int64_t n, int nc, const float* grid, const int32_t* idx, const double* affine, double scale,
                     double gamma, double* xu, void* stream);

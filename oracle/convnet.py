@@ -90,3 +90,28 @@ def unet_forward(z, params, channels, round_bf16=False):
         bb = r(torch.relu(conv(cat, w1, b1)))
         d = r(torch.relu(conv(bb, w2, b2)))
     return conv(d, out[0], out[1])
+
+
+def mse_multiscale(pred, target, lam, levels, dim=2):
+    """Multi-scale mean squared error of the warm start's training (gf_mse_multiscale), float64.
+    pred, target [B, H, W, C] (dim 2) or [B, Z, H, W, C] (dim 3).  With e = pred - target, mu each
+    grid's per-channel mean of e and m_s the mean of e over cells 2^s wide (s = 1..levels):
+        L = sum e^2 + lam sum_s (2^dim)^s sum_cells (m_s - mu)^2.
+    Returns (L, dL/de) with the gradient from autograd; pinned by finite differences and closed
+    forms in tests/test_oracle_convnet.py."""
+    e = (torch.as_tensor(pred, dtype=torch.float64) - torch.as_tensor(target, dtype=torch.float64)).clone()
+    e.requires_grad_(True)
+    sp = tuple(range(1, dim + 1))
+    mu = e.mean(dim=sp, keepdim=True)
+    loss = (e * e).sum()
+    for s in range(1, levels + 1):
+        f = 1 << s
+        if dim == 2:
+            B, H, W, C = e.shape
+            m = e.reshape(B, H // f, f, W // f, f, C).mean(dim=(2, 4))
+        else:
+            B, Z, H, W, C = e.shape
+            m = e.reshape(B, Z // f, f, H // f, f, W // f, f, C).mean(dim=(2, 4, 6))
+        loss = loss + lam * float((1 << dim) ** s) * ((m - mu) ** 2).sum()
+    (g,) = torch.autograd.grad(loss, e)
+    return float(loss.detach()), g.detach()

@@ -315,7 +315,7 @@ def test_single_element_mesh(gf, et, periodic):
     assert np.array_equal(host(gM.data), diag)
 
 
-def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, distort=False):
+def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, distort=False, tied=False):
     coor, conn = oracle.quad4_regular(nx, ny)
     if distort:  # structured connectivity, perturbed coordinates: the kernel must read them
         coor = coor + 0.15 * oracle.randn(seed + 300, coor.size).reshape(coor.shape)
@@ -323,6 +323,8 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, disto
         perm = np.random.default_rng(5).permutation(conn.shape[0])
         conn = conn[perm]
     dofs = oracle.dofs(coor.shape[0], 2)
+    if tied:  # periodic DOFs: a DOF gathers the contributions of several nodes
+        dofs = oracle.dofs_periodic(coor.shape[0], 2, oracle.quad4_nodes_periodic(nx, ny))
     vec = oracle.Vector(conn, dofs)
     x = vec.AsElement(coor)
     q = oracle.Quadrature(QUAD4_PLANAR, x)
@@ -339,7 +341,7 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, disto
     if kvar:
         Ke = (10.0 + oracle.randn(seed + 200, q.nelem)).astype(np.float32).astype(np.float64)
     Ge = np.exp(0.3 * oracle.randn(seed + 100, q.nelem)).astype(np.float32).astype(np.float64)
-    u = oracle.randn(seed, dofs.size, 0.0, 1e-3)
+    u = oracle.randn(seed, int(dofs.max()) + 1, 0.0, 1e-3)
     v = np.zeros_like(u)
     if fixed:
         v[dofs[top, 0]] = 1e-4
@@ -349,14 +351,20 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, disto
     return coor, conn, dofs, q, inv, Ke, Ge, u, v, a
 
 
-@pytest.mark.parametrize("nx,ny,shuffle,kvar,distort", [(64, 64, False, False, False), (130, 17, False, True, False),
-                                                       (3, 5, False, False, False), (24, 20, True, True, False),
-                                                       (40, 24, False, False, True)])
-def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar, distort):
-    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle, kvar=kvar, distort=distort)
+@pytest.mark.parametrize("nx,ny,shuffle,kvar,distort,tied", [(64, 64, False, False, False, False),
+                                                            (130, 17, False, True, False, False),
+                                                            (3, 5, False, False, False, False),
+                                                            (24, 20, True, True, False, False),
+                                                            (40, 24, False, False, True, False),
+                                                            (70, 45, True, False, True, False),
+                                                            (24, 20, True, True, False, True)])
+def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar, distort, tied):
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle, kvar=kvar, distort=distort,
+                                                                tied=tied)
     vec = gf.Vector(conn, dofs)
     X = gf.ExplicitQuad4(vec, coor, Ke, Ge, inv, 0.01)
     assert X.structured == (not shuffle)
+    assert X.path == ("structured" if not shuffle else "gather" if tied else "blocks")
     X.set_state(u, v, a)
     for nsteps in (1, 9):
         fint = oracle.vv_steps(nsteps, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a)

@@ -511,3 +511,42 @@ def test_network3d_forward_and_gradients_vs_oracle(ws):
         e_w = rel(c.dw.cpu().reshape(27, c.cout, c.cin), wr.grad)
         e_b = rel(c.db.cpu(), br.grad)
         assert e_w < TOL_NET and e_b < TOL_NET, (i, e_w, e_b)
+
+
+@pytest.mark.parametrize("dim,shape,levels,lam", [(2, (2, 32, 32, 2), 4, 1.0), (2, (1, 16, 48, 2), 3, 4.0),
+                                                  (3, (2, 8, 8, 8, 3), 2, 1.0), (3, (1, 4, 8, 16, 3), 2, 0.5)])
+def test_mse_multiscale_matches_oracle(ws, dim, shape, levels, lam):
+    # gf_mse_multiscale against oracle.convnet.mse_multiscale (itself pinned by closed forms and
+    # finite differences): the loss to FP32 summation accuracy, the gradient to one bfloat16 rounding
+    from oracle import convnet
+
+    rng = np.random.default_rng(sum(shape) + levels)
+    pred = rng.standard_normal(shape).astype(np.float32)
+    tgt = (pred + 0.3 * rng.standard_normal(shape) + 0.2).astype(np.float32)  # an offset: mu != 0
+    L_ref, g_ref = convnet.mse_multiscale(pred, tgt, lam, levels, dim=dim)
+    B, nc, cg = shape[0], shape[-1], 16
+    Z, H, W = (1,) + shape[1:3] if dim == 2 else shape[1:4]
+    P = B * Z * H * W
+    nf = ws._i64()
+    ws._call("gf_mse_multiscale_work", ws._i32(dim), ws._i64(B), ws._i32(Z), ws._i32(H), ws._i32(W), ws._i32(nc),
+             ws._i32(levels), ws._i32(cg), __import__("ctypes").byref(nf))
+    work = torch.zeros(nf.value, dtype=torch.float32, device="cuda")
+    g = torch.full((P * cg,), 7.0, dtype=torch.bfloat16, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    pd, td = torch.from_numpy(pred).cuda().reshape(-1), torch.from_numpy(tgt).cuda().reshape(-1)
+    scale = 3.0
+    ws._call("gf_mse_multiscale", ws._i32(dim), ws._i64(B), ws._i32(Z), ws._i32(H), ws._i32(W), ws._i32(nc),
+             ws._i32(levels), ws._f32(lam), ws._dp(pd), ws._dp(td), ws._f32(scale), ws._i32(cg), ws._dp(g),
+             ws._dp(work), ws._i64(work.numel()), ws._dp(loss), ws._stream())
+    torch.cuda.synchronize()
+    assert abs(loss.item() - L_ref) <= 1e-5 * L_ref
+    got = cpu64(g).reshape(P, cg)
+    want = (scale * g_ref / (P * nc)).reshape(P, nc)
+    assert torch.all(got[:, nc:] == 0)
+    err = (got[:, :nc] - want).abs()
+    assert torch.all(err <= ULP_BF16 * want.abs() + 1e-6 * want.abs().max())
+    # a bad level count (2^levels must divide every dimension) is refused
+    with pytest.raises(Exception):
+        ws._call("gf_mse_multiscale", ws._i32(dim), ws._i64(B), ws._i32(Z), ws._i32(H), ws._i32(W), ws._i32(nc),
+                 ws._i32(levels + 4), ws._f32(lam), ws._dp(pd), ws._dp(td), ws._f32(scale), ws._i32(cg), ws._dp(g),
+                 ws._dp(work), ws._i64(work.numel()), ws._dp(loss), ws._stream())

@@ -140,3 +140,50 @@ def test_unet3d_gradient_finite_differences():
             lm = loss().item()
             w[idx] += h
         assert (lp - lm) / (2 * h) == pytest.approx(w.grad[idx].item(), rel=1e-5, abs=1e-9)
+
+
+def test_mse_multiscale_closed_forms():
+    # e constant: the centred cell means vanish, L = sum e^2
+    B, H, W, C = 2, 8, 8, 2
+    z = np.zeros((B, H, W, C))
+    L, g = convnet.mse_multiscale(z + 0.5, z, 3.0, 3)
+    assert abs(L - 0.25 * z.size) < 1e-12 and torch.allclose(g, torch.full_like(g, 1.0))
+    # finest checkerboard: every cell mean is zero, L = sum e^2 = N
+    y, x = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    chk = ((-1.0) ** (x + y))[None, :, :, None] * np.ones((B, 1, 1, C))
+    L, _ = convnet.mse_multiscale(chk, z, 3.0, 3)
+    assert abs(L - z.size) < 1e-12
+    # +-1 constant on 2x2 cells in a checkerboard of cells: level-1 means +-1, coarser ones zero,
+    # so L = N + lam 4 (N / 4) = N (1 + lam)
+    chk2 = ((-1.0) ** (x // 2 + y // 2))[None, :, :, None] * np.ones((B, 1, 1, C))
+    lam = 3.0
+    L, g = convnet.mse_multiscale(chk2, z, lam, 3)
+    assert abs(L - z.size * (1.0 + lam)) < 1e-9
+    assert torch.allclose(g, torch.as_tensor(2.0 * (1.0 + lam) * chk2))
+    # 3-D: +-1 on 2x2x2 cells in a 3-D checkerboard: L = N (1 + lam)
+    Z = 4
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(4), np.arange(4), indexing="ij")
+    c3 = ((-1.0) ** (xx // 2 + yy // 2 + zz // 2))[None, ..., None] * np.ones((1, 1, 1, 1, 3))
+    L, _ = convnet.mse_multiscale(c3, np.zeros_like(c3), lam, 2, dim=3)
+    assert abs(L - c3.size * (1.0 + lam)) < 1e-9
+    # lam = 0 is the plain squared error
+    rng = np.random.default_rng(0)
+    p, t = rng.standard_normal((B, H, W, C)), rng.standard_normal((B, H, W, C))
+    L, g = convnet.mse_multiscale(p, t, 0.0, 3)
+    assert abs(L - ((p - t) ** 2).sum()) < 1e-9 and torch.allclose(g, torch.as_tensor(2.0 * (p - t)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mse_multiscale_gradient_finite_differences(dim):
+    rng = np.random.default_rng(1)
+    shape = (2, 8, 8, 2) if dim == 2 else (1, 4, 4, 4, 3)
+    p, t = rng.standard_normal(shape), rng.standard_normal(shape)
+    lam, levels = 0.7, (3 if dim == 2 else 2)
+    _, g = convnet.mse_multiscale(p, t, lam, levels, dim=dim)
+    h = 1e-6
+    for k in rng.choice(p.size, 12, replace=False):
+        pp, pm = p.copy(), p.copy()
+        pp.flat[k] += h
+        pm.flat[k] -= h
+        fd = (convnet.mse_multiscale(pp, t, lam, levels, dim=dim)[0] - convnet.mse_multiscale(pm, t, lam, levels, dim=dim)[0]) / (2 * h)
+        assert abs(fd - g.flatten()[k].item()) < 1e-6 * max(1.0, abs(fd))
