@@ -66,6 +66,7 @@ struct gf_explicit_s {
     bool blocks = false;
     int64_t nblk = 0;
     int64_t lds64 = 0, lds32 = 0;                   // largest LDS need of a block, FP64 / FP32 forces
+    bool dofs_ident = false;                        // dofs(n, i) = 2 n + i
     DBuf<int32_t> bkptr, blptr, bnloc, beptr, bnptr; // own nodes, local nodes (own, then halo), element slots
     DBuf<uint16_t> blconn, bnent;              // element slots' local connectivity; node contributions
     DBuf<float> bK, bG;                        // moduli per element slot
@@ -518,9 +519,9 @@ __global__ void __launch_bounds__(BS) k_node_update(int64_t ndof, const int32_t*
 // each) go to LDS, computed from the staged nodes through the local connectivity.  (2) Each thread
 // sums its own node's contributions from LDS in the reference's order and updates the node's two
 // DOFs as k_node_update does.  No element-force array and no per-DOF gather list cross HBM.
-template <int BODY>
+template <int BODY, bool IDENT>
 __global__ void __launch_bounds__(BS) k_explicit_blocks(
-    ShapeTable T, const int32_t* __restrict__ bkptr, const int32_t* __restrict__ blptr, const int32_t* __restrict__ bnloc,
+    int nblk, int per_xcd, ShapeTable T, const int32_t* __restrict__ bkptr, const int32_t* __restrict__ blptr, const int32_t* __restrict__ bnloc,
     const int32_t* __restrict__ beptr, const uint16_t* __restrict__ blconn, const float* __restrict__ bK,
     const float* __restrict__ bG, const int32_t* __restrict__ dofs, const double* __restrict__ coor,
     const int32_t* __restrict__ bnptr, const uint16_t* __restrict__ bnent, const double* __restrict__ inv_mass,
@@ -530,9 +531,13 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
 {
     using FT = typename std::conditional<body_f32(BODY), float, double>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int32_t lb = blptr[blockIdx.x], nl = blptr[blockIdx.x + 1] - lb;
-    const int32_t eb = beptr[blockIdx.x], ne = beptr[blockIdx.x + 1] - eb;
-    const int32_t kb = bkptr[blockIdx.x], own = bkptr[blockIdx.x + 1] - kb;
+    // XCD-aware: workgroup b runs on XCD b mod 8; each XCD takes a contiguous range of patches, which
+    // are spatial neighbours (bisection order), so halo nodes hit that XCD's L2
+    const int pb = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
+    if (pb >= nblk) return; // uniform per workgroup
+    const int32_t lb = blptr[pb], nl = blptr[pb + 1] - lb;
+    const int32_t eb = beptr[pb], ne = beptr[pb + 1] - eb;
+    const int32_t kb = bkptr[pb], own = bkptr[pb + 1] - kb;
     FT* s_f = reinterpret_cast<FT*>(smem);                          // [ne][4][2] element forces
     double2* s_x = reinterpret_cast<double2*>(smem + ne * 8 * sizeof(FT)); // [nl] coordinates
     double2* s_u = s_x + nl;                                        // [nl] displacements
@@ -545,8 +550,20 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
         const int l0 = base + threadIdx.x, l1 = l0 + BS;
         const int32_t n0 = bnloc[lb + min(l0, nl - 1)], n1 = bnloc[lb + min(l1, nl - 1)];
         const double2 x0 = reinterpret_cast<const double2*>(coor)[n0], x1 = reinterpret_cast<const double2*>(coor)[n1];
-        const int2 d0 = reinterpret_cast<const int2*>(dofs)[n0], d1 = reinterpret_cast<const int2*>(dofs)[n1];
-        const double2 u0 = make_double2(u[d0.x], u[d0.y]), u1 = make_double2(u[d1.x], u[d1.y]);
+        int2 d0, d1;
+        double2 u0, u1;
+        if constexpr (IDENT) { // dofs(n, i) = 2 n + i: a node's displacement is one 16-byte load
+            d0 = make_int2(2 * n0, 2 * n0 + 1);
+            d1 = make_int2(2 * n1, 2 * n1 + 1);
+            u0 = reinterpret_cast<const double2*>(u)[n0];
+            u1 = reinterpret_cast<const double2*>(u)[n1];
+        }
+        else {
+            d0 = reinterpret_cast<const int2*>(dofs)[n0];
+            d1 = reinterpret_cast<const int2*>(dofs)[n1];
+            u0 = make_double2(u[d0.x], u[d0.y]);
+            u1 = make_double2(u[d1.x], u[d1.y]);
+        }
         if (l0 < nl) {
             s_x[l0] = x0;
             s_u[l0] = u0;
@@ -565,12 +582,31 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
     if (active) {
         j0 = bnptr[k];
         j1 = bnptr[k + 1];
+        if constexpr (IDENT) {
+            const int n = d[0] / 2;
+            const double2 im2 = reinterpret_cast<const double2*>(inv_mass)[n];
+            const double2 vo2 = reinterpret_cast<const double2*>(v0)[n];
+            const double2 ao2 = reinterpret_cast<const double2*>(a0)[n];
+            im[0] = im2.x;
+            im[1] = im2.y;
+            vo[0] = vo2.x;
+            vo[1] = vo2.y;
+            ao[0] = ao2.x;
+            ao[1] = ao2.y;
+            if (fext) {
+                const double2 f2 = reinterpret_cast<const double2*>(fext)[n];
+                fx[0] = f2.x;
+                fx[1] = f2.y;
+            }
+        }
+        else {
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            im[i] = inv_mass[d[i]];
-            vo[i] = v0[d[i]];
-            ao[i] = a0[d[i]];
-            fx[i] = fext ? fext[d[i]] : 0.0;
+            for (int i = 0; i < 2; ++i) {
+                im[i] = inv_mass[d[i]];
+                vo[i] = v0[d[i]];
+                ao[i] = a0[d[i]];
+                fx[i] = fext ? fext[d[i]] : 0.0;
+            }
         }
     }
     // (1) element forces from the staged nodes
@@ -612,14 +648,28 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
     }
     const double2 uo = s_u[threadIdx.x];
     const double sum[2] = {s0, s1}, uu[2] = {uo.x, uo.y};
+    double an[2], vn[2], un[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        const double an = im[i] * (fx[i] - sum[i]);
-        const double vn = vo[i] + hdt * (ao[i] + an);
-        v1[d[i]] = vn;
-        a1[d[i]] = an;
-        if (u_next) u_next[d[i]] = uu[i] + dt * vn + c * an;
-        if (fint_out) fint_out[d[i]] = sum[i];
+        an[i] = im[i] * (fx[i] - sum[i]);
+        vn[i] = vo[i] + hdt * (ao[i] + an[i]);
+        un[i] = uu[i] + dt * vn[i] + c * an[i];
+    }
+    if constexpr (IDENT) {
+        const int n = d[0] / 2;
+        reinterpret_cast<double2*>(v1)[n] = make_double2(vn[0], vn[1]);
+        reinterpret_cast<double2*>(a1)[n] = make_double2(an[0], an[1]);
+        if (u_next) reinterpret_cast<double2*>(u_next)[n] = make_double2(un[0], un[1]);
+        if (fint_out) reinterpret_cast<double2*>(fint_out)[n] = make_double2(sum[0], sum[1]);
+    }
+    else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            v1[d[i]] = vn[i];
+            a1[d[i]] = an[i];
+            if (u_next) u_next[d[i]] = un[i];
+            if (fint_out) fint_out[d[i]] = sum[i];
+        }
     }
 }
 
@@ -913,6 +963,8 @@ void build_node_blocks(gf_explicit_s* x, gf_vector v, const double* coor_d, cons
     x->bnptr.upload(bnptr);
     x->bnent.upload(bnent);
     x->nblk = static_cast<int64_t>(bkptr.size()) - 1;
+    x->dofs_ident = true;
+    for (int64_t k = 0; k < 2 * nn && x->dofs_ident; ++k) x->dofs_ident = v->dofs_h[k] == k;
     x->lds64 = lds64;
     x->lds32 = lds32;
     x->blocks = true;
@@ -1133,11 +1185,12 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
                 double* fo = (store_fint && last) ? x->fint.p : nullptr;
                 const double* un = x->u[i1].p; // this step's displacement
                 double* unext = last ? nullptr : x->u[i0].p;
-                const unsigned g = static_cast<unsigned>(x->nblk);
+                const int per_xcd = static_cast<int>((x->nblk + 7) / 8);
                 auto run = [&](auto body) {
                     constexpr int B = decltype(body)::value;
                     const size_t lds = static_cast<size_t>(body_f32(B) ? x->lds32 : x->lds64);
-                    k_explicit_blocks<B><<<g, BS, lds, s>>>(T, x->bkptr.p, x->blptr.p, x->bnloc.p, x->beptr.p,
+                    auto kern = x->dofs_ident ? k_explicit_blocks<B, true> : k_explicit_blocks<B, false>;
+                    kern<<<8 * per_xcd, BS, lds, s>>>(static_cast<int>(x->nblk), per_xcd, T, x->bkptr.p, x->blptr.p, x->bnloc.p, x->beptr.p,
                                                             x->blconn.p, x->bK.p, x->bG.p, v->dofs.p, x->coor.p,
                                                             x->bnptr.p, x->bnent.p, x->inv_mass.p, fext, dt, c, hdt,
                                                             x->vv[i0].p, x->a[i0].p, x->vv[i1].p, x->a[i1].p, un, unext,

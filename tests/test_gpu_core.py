@@ -315,7 +315,7 @@ def test_single_element_mesh(gf, et, periodic):
     assert np.array_equal(host(gM.data), diag)
 
 
-def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, distort=False, tied=False):
+def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, distort=False, tied=False, permdofs=False):
     coor, conn = oracle.quad4_regular(nx, ny)
     if distort:  # structured connectivity, perturbed coordinates: the kernel must read them
         coor = coor + 0.15 * oracle.randn(seed + 300, coor.size).reshape(coor.shape)
@@ -325,6 +325,8 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, disto
     dofs = oracle.dofs(coor.shape[0], 2)
     if tied:  # periodic DOFs: a DOF gathers the contributions of several nodes
         dofs = oracle.dofs_periodic(coor.shape[0], 2, oracle.quad4_nodes_periodic(nx, ny))
+    if permdofs:  # every DOF still belongs to one node component, numbered in another order
+        dofs = np.random.default_rng(7).permutation(dofs.size)[dofs]
     vec = oracle.Vector(conn, dofs)
     x = vec.AsElement(coor)
     q = oracle.Quadrature(QUAD4_PLANAR, x)
@@ -357,14 +359,16 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, disto
                                                             (24, 20, True, True, False, False),
                                                             (40, 24, False, False, True, False),
                                                             (70, 45, True, False, True, False),
-                                                            (24, 20, True, True, False, True)])
+                                                            (24, 20, True, True, False, True),
+                                                            (33, 29, True, True, True, "perm")])
 def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar, distort, tied):
+    # tied: False, True (periodic DOFs) or "perm" (a permuted numbering, each DOF still on one node)
     coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle, kvar=kvar, distort=distort,
-                                                                tied=tied)
+                                                                tied=tied is True, permdofs=tied == "perm")
     vec = gf.Vector(conn, dofs)
     X = gf.ExplicitQuad4(vec, coor, Ke, Ge, inv, 0.01)
     assert X.structured == (not shuffle)
-    assert X.path == ("structured" if not shuffle else "gather" if tied else "blocks")
+    assert X.path == ("structured" if not shuffle else "gather" if tied is True else "blocks")
     X.set_state(u, v, a)
     for nsteps in (1, 9):
         fint = oracle.vv_steps(nsteps, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a)
