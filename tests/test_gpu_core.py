@@ -541,6 +541,49 @@ def test_explicit_full_size_fp32_vs_fp64(gf):
     assert relerr(ffast, f64) < 1e-12
 
 
+def test_explicit_full_size_finelayer_fp32_vs_fp64(gf):
+    """The general-mesh path at full size: FineLayer(2187, 2400, nfine 1800) (3.95 M elements, element
+    sizes 1 to 81), log-normal G (sigma 0.3), K 10, bottom edge fixed, top edge driven at v_x 1e-4,
+    a random initial displacement (1e-3, every free DOF, so that every element is loaded), dt 0.01,
+    1000 steps, one fused kernel per step.  The FP64 reference-order path is bit-exact with the
+    oracle on the FineLayer meshes of test_gpu_finelayer; here the FP32 body stays within 1e-6 of it
+    (u and f_int) and the FP64 fast body within 1e-11."""
+    import torch
+
+    mesh = gf.Mesh.Quad4.FineLayer(2187, 2400, 1.0, 1800)
+    coor, conn, dofs = mesh.coor, mesh.conn, mesh.dofs
+    vec = gf.Vector(conn, dofs)
+    nodal = gf.Element.Quad4.QuadraturePlanar(vec.AsElement(coor), gf.Element.Quad4.Nodal.xi(),
+                                              gf.Element.Quad4.Nodal.w())
+    M = gf.MatrixDiagonal(vec, None)
+    M.assemble(nodal.Int_N_scalar_NT_dV(torch.ones((mesh.nelem, 4), dtype=torch.float64, device="cuda")))
+    inv = M.Solve(torch.ones(vec.ndof, dtype=torch.float64, device="cuda"))
+    del nodal
+    top = mesh.nodesTopEdge
+    fixed = np.concatenate((dofs[mesh.nodesBottomEdge].ravel(), dofs[top].ravel()))
+    inv[torch.from_numpy(fixed).cuda()] = 0.0
+    G = np.exp(0.3 * gf.randn(3, mesh.nelem)).astype(np.float32)
+    K = np.full(mesh.nelem, 10.0, dtype=np.float32)
+    v0 = np.zeros(vec.ndof)
+    v0[dofs[top, 0]] = 1e-4
+    u0 = 1e-3 * gf.randn(4, vec.ndof)
+    u0[fixed] = 0.0
+    X = gf.ExplicitQuad4(vec, coor, K, G, inv, 0.01)
+    assert X.path == "blocks"
+    out = {}
+    for prec in (gf.F64, gf.F32, gf.F64_FAST):
+        X.set_state(u0, v0, np.zeros(vec.ndof))
+        X.step(1000, prec, store_fint=True)
+        out[prec] = (host(X.state()[0]), host(X.fint()))
+    u64, f64 = out[gf.F64]
+    e = {p: (relerr(out[p][0], u64), relerr(out[p][1], f64)) for p in (gf.F32, gf.F64_FAST)}
+    print(f"\nFineLayer x 1000: FP32 u {e[gf.F32][0]:.2e} f_int {e[gf.F32][1]:.2e}; FP64 fast u "
+          f"{e[gf.F64_FAST][0]:.2e} f_int {e[gf.F64_FAST][1]:.2e}")
+    assert np.abs(u64).max() > 0.0
+    assert e[gf.F32][0] < 1e-6 and e[gf.F32][1] < 1e-6
+    assert e[gf.F64_FAST][0] < 1e-11 and e[gf.F64_FAST][1] < 1e-11
+
+
 @pytest.mark.parametrize("shuffle", [False, True])
 def test_explicit_energy(gf, shuffle, tmp_path):
     """Per-step energies of the explicit step (Elastic-VelocityVerlet/main.cpp:155-160) against
