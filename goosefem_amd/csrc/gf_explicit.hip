@@ -45,6 +45,11 @@ struct gf_explicit_s {
     bool reg_x = false;     // coordinates are exactly (i sx, j sy): generated in the kernel, not read
     double sx = 0.0, sy = 0.0;
     DBuf<double> inv_mass, fext;
+    // M^-1 as a palette when it holds at most 256 distinct values (a uniform density gives a handful:
+    // interior, edge and corner nodes, and 0 on prescribed DOFs): one byte per DOF and a 256-entry
+    // table, read instead of 8 bytes per DOF; the values, and so the results, are the same
+    DBuf<uint8_t> im_code;
+    DBuf<double> im_tab;
     bool has_fext = false;
     DBuf<double> u[2], vv[2], a[2];
     int cur = 0;
@@ -280,6 +285,7 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     int nx, int ny, int ntx, int nty, int per_xcd, ShapeTable T, const double* __restrict__ coor, double sx, double sy,
     double rI0, double rI1, double rdv,
     const float* __restrict__ Ke, float Kconst, const float* __restrict__ Ge, const double* __restrict__ inv_mass,
+    const uint8_t* __restrict__ im_code, const double* __restrict__ im_tab,
     const double* __restrict__ fext, double dt, double c, double hdt, const double* __restrict__ u0,
     const double* __restrict__ v0, const double* __restrict__ a0, double* __restrict__ u1, double* __restrict__ v1,
     double* __restrict__ a1, double* __restrict__ fint_out)
@@ -293,6 +299,7 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     __shared__ double2 s_u[HY * HX];
     __shared__ double2 s_x[RECT ? 1 : HY * HX]; // the rectangle body needs no coordinates
     __shared__ FT s_f[EY * EX][8];
+    __shared__ double s_tab[256]; // the M^-1 palette (when im_code is given)
 
     // XCD-aware tile order: hardware places block b on XCD b % 8; give each XCD a contiguous band
     const int b = blockIdx.x;
@@ -313,13 +320,17 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
     const bool owner = ogx < NX && ogy < NY;
     const int64_t on = static_cast<int64_t>(ogy) * NX + ogx;
     double2 o_u = zero, o_v = zero, o_a = zero, o_x = zero, o_im = zero, o_fe = zero;
+    uchar2 o_code = make_uchar2(0, 0);
+    if (im_code)
+        for (int t = threadIdx.x; t < 256; t += NT) s_tab[t] = im_tab[t];
     if (owner) {
         o_u = reinterpret_cast<const double2*>(u0)[on];
         o_v = reinterpret_cast<const double2*>(v0)[on];
         o_a = reinterpret_cast<const double2*>(a0)[on];
         o_x = coor ? reinterpret_cast<const double2*>(coor)[on]
                    : make_double2(static_cast<double>(ogx) * sx, static_cast<double>(ogy) * sy);
-        o_im = nt_load2(inv_mass, on);
+        if (im_code) o_code = reinterpret_cast<const uchar2*>(im_code)[on];
+        else o_im = nt_load2(inv_mass, on);
         if (fext) o_fe = nt_load2(fext, on);
     }
     int rt = -1;
@@ -433,6 +444,7 @@ __global__ void __launch_bounds__(TX * TY) k_vv_structured(
             f1 += static_cast<double>(s_f[le][1]);
         }
         const double2 un = o_un;
+        if (im_code) o_im = make_double2(s_tab[o_code.x], s_tab[o_code.y]);
         double2 an, vn;
         an.x = o_im.x * (o_fe.x - f0);
         an.y = o_im.y * (o_fe.y - f1);
@@ -525,12 +537,14 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
     const int32_t* __restrict__ beptr, const uint16_t* __restrict__ blconn, const float* __restrict__ bK,
     const float* __restrict__ bG, const int32_t* __restrict__ dofs, const double* __restrict__ coor,
     const int32_t* __restrict__ bnptr, const uint16_t* __restrict__ bnent, const double* __restrict__ inv_mass,
+    const uint8_t* __restrict__ im_code, const double* __restrict__ im_tab,
     const double* __restrict__ fext, double dt, double c, double hdt, const double* __restrict__ v0,
     const double* __restrict__ a0, double* __restrict__ v1, double* __restrict__ a1, const double* __restrict__ u,
     double* __restrict__ u_next, double* __restrict__ fint_out)
 {
     using FT = typename std::conditional<body_f32(BODY), float, double>::type;
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double s_tab[256]; // the M^-1 palette (when im_code is given)
     // XCD-aware: workgroup b runs on XCD b mod 8; each XCD takes a contiguous range of patches, which
     // are spatial neighbours (bisection order), so halo nodes hit that XCD's L2
     const int pb = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
@@ -544,6 +558,7 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
     const int64_t k = static_cast<int64_t>(kb) + threadIdx.x;
     const bool active = static_cast<int>(threadIdx.x) < own;
     // (0) local nodes; thread t's own node is local node t
+    if (im_code) s_tab[threadIdx.x] = im_tab[threadIdx.x]; // BS = 256 entries
     int2 dme = make_int2(0, 0);
     for (int base = 0; base < nl; base += 2 * BS) {
         // two local nodes per thread and pass, their loads issued together (indices clamped)
@@ -584,7 +599,13 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
         j1 = bnptr[k + 1];
         if constexpr (IDENT) {
             const int n = d[0] / 2;
-            const double2 im2 = reinterpret_cast<const double2*>(inv_mass)[n];
+            double2 im2;
+            if (im_code) {
+                const uchar2 c2 = reinterpret_cast<const uchar2*>(im_code)[n];
+                im2 = make_double2(s_tab[c2.x], s_tab[c2.y]);
+            }
+            else
+                im2 = reinterpret_cast<const double2*>(inv_mass)[n];
             const double2 vo2 = reinterpret_cast<const double2*>(v0)[n];
             const double2 ao2 = reinterpret_cast<const double2*>(a0)[n];
             im[0] = im2.x;
@@ -602,7 +623,7 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
         else {
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
-                im[i] = inv_mass[d[i]];
+                im[i] = im_code ? s_tab[im_code[d[i]]] : inv_mass[d[i]];
                 vo[i] = v0[d[i]];
                 ao[i] = a0[d[i]];
                 fx[i] = fext ? fext[d[i]] : 0.0;
@@ -798,8 +819,9 @@ void launch_body(unsigned grid, hipStream_t s, int nx, int ny, int ntx, int nty,
 {
     k_vv_structured<TX, TY, BODY><<<grid, TX * TY, 0, s>>>(
         nx, ny, ntx, nty, per_xcd, T, x->reg_x ? nullptr : x->coor.p, x->sx, x->sy, 2.0 / x->sx, 2.0 / x->sy,
-        x->sx * x->sy / 4.0, x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p, fext, dt, c, hdt,
-        x->u[i0].p, x->vv[i0].p, x->a[i0].p, x->u[i1].p, x->vv[i1].p, x->a[i1].p, fo);
+        x->sx * x->sy / 4.0, x->k_uniform ? nullptr : x->Ke.p, x->k_value, x->Ge.p, x->inv_mass.p,
+        x->im_code.n ? x->im_code.p : nullptr, x->im_tab.p, fext, dt, c, hdt, x->u[i0].p, x->vv[i0].p, x->a[i0].p,
+        x->u[i1].p, x->vv[i1].p, x->a[i1].p, fo);
 }
 
 template <int TX, int TY>
@@ -921,7 +943,7 @@ void build_node_blocks(gf_explicit_s* x, gf_vector v, const double* coor_d, cons
             });
             return emit(k0, mid) && emit(mid, k1);
         }
-        if (nl > 65535 || nl * 32 + ne * 64 > 65536) return false; // beyond LDS: the two-kernel path
+        if (nl > 65535 || nl * 32 + ne * 64 > 65536 - 2048) return false; // beyond LDS (with the 2 KiB palette)
         // own nodes in ascending number (local number k - k0), then the halo nodes, ascending
         std::sort(node.begin() + k0, node.begin() + k1);
         for (int64_t k = k0; k < k1; ++k) lidx[node[k]] = static_cast<int32_t>(k - k0);
@@ -1000,6 +1022,31 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
             GF_HIP(hipMemcpy(x->Ge.p, G_e, v->nelem * sizeof(float), hipMemcpyDeviceToDevice));
             x->inv_mass.alloc(v->ndof);
             GF_HIP(hipMemcpy(x->inv_mass.p, inv_mass, v->ndof * sizeof(double), hipMemcpyDeviceToDevice));
+            {
+                // palette of M^-1 (bit patterns, so the table returns exactly the given values)
+                std::vector<double> mh(v->ndof);
+                if (v->ndof) GF_HIP(hipMemcpy(mh.data(), inv_mass, v->ndof * sizeof(double), hipMemcpyDeviceToHost));
+                std::vector<uint64_t> keys;
+                std::vector<uint8_t> code(v->ndof);
+                bool ok = v->ndof > 0;
+                for (int64_t d = 0; ok && d < v->ndof; ++d) {
+                    uint64_t bits;
+                    std::memcpy(&bits, &mh[d], sizeof(bits));
+                    size_t j = 0;
+                    while (j < keys.size() && keys[j] != bits) ++j;
+                    if (j == keys.size()) {
+                        if (keys.size() == 256) ok = false;
+                        else keys.push_back(bits);
+                    }
+                    code[d] = static_cast<uint8_t>(j);
+                }
+                if (ok) {
+                    std::vector<double> tab(256, 0.0);
+                    for (size_t j = 0; j < keys.size(); ++j) std::memcpy(&tab[j], &keys[j], sizeof(double));
+                    x->im_code.upload(code);
+                    x->im_tab.upload(tab);
+                }
+            }
             if (fext) {
                 x->fext.alloc(v->ndof);
                 GF_HIP(hipMemcpy(x->fext.p, fext, v->ndof * sizeof(double), hipMemcpyDeviceToDevice));
@@ -1192,7 +1239,8 @@ int gf_explicit_step(gf_explicit x, int64_t nsteps, int precision, int store_fin
                     auto kern = x->dofs_ident ? k_explicit_blocks<B, true> : k_explicit_blocks<B, false>;
                     kern<<<8 * per_xcd, BS, lds, s>>>(static_cast<int>(x->nblk), per_xcd, T, x->bkptr.p, x->blptr.p, x->bnloc.p, x->beptr.p,
                                                             x->blconn.p, x->bK.p, x->bG.p, v->dofs.p, x->coor.p,
-                                                            x->bnptr.p, x->bnent.p, x->inv_mass.p, fext, dt, c, hdt,
+                                                            x->bnptr.p, x->bnent.p, x->inv_mass.p,
+                                                            x->im_code.n ? x->im_code.p : nullptr, x->im_tab.p, fext, dt, c, hdt,
                                                             x->vv[i0].p, x->a[i0].p, x->vv[i1].p, x->a[i1].p, un, unext,
                                                             fo);
                 };

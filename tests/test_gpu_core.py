@@ -378,6 +378,22 @@ def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar, distort, tied):
         assert np.array_equal(host(X.fint()), fint)
 
 
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_explicit_fp64_bit_exact_any_mass(gf, shuffle):
+    # M^-1 with thousands of distinct values (a random density): the step reads the full array
+    # instead of its palette of at most 256 values; both bit-exact with the oracle
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(40, 32, shuffle=shuffle)
+    inv = inv * (1.0 + 0.2 * oracle.randn(77, inv.size))
+    assert np.unique(inv).size > 256
+    X = gf.ExplicitQuad4(gf.Vector(conn, dofs), coor, Ke, Ge, inv, 0.01)
+    X.set_state(u, v, a)
+    fint = oracle.vv_steps(7, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a)
+    X.step(7, gf.F64, store_fint=True)
+    gu, gv_, ga = (host(t) for t in X.state())
+    assert np.array_equal(gu, u) and np.array_equal(gv_, v) and np.array_equal(ga, a)
+    assert np.array_equal(host(X.fint()), fint)
+
+
 @pytest.mark.parametrize("shuffle,kvar,distort", [(False, False, False), (False, True, False), (True, False, False),
                                                  (False, False, True)])
 def test_explicit_fp32_tolerance(gf, shuffle, kvar, distort):
