@@ -325,7 +325,8 @@ def test_warm_start_pcg_same_tolerance(gf, ws):
 
 def test_committed_weights_cut_pcg_iterations_at_512(gf, ws):
     """The committed 512^2 network on a modulus field outside its training seeds: the warm-started
-    PCG reaches the same true residual with at least 25% fewer iterations than from a zero guess."""
+    PCG reaches the same true residual with at least 35% fewer iterations than from a zero guess,
+    and at least 20% fewer than from the affine part of the start alone."""
     import os
 
     path = os.path.join(os.path.dirname(gf.__file__), "data", "warmstart_quad4_512.npz")
@@ -341,11 +342,11 @@ def test_committed_weights_cut_pcg_iterations_at_512(gf, ws):
     _, it1, r1 = pb.P.solve(b, x0=x0, tol=1e-10)
     assert r0 <= 1e-9 and r1 <= 1e-9
     assert pb.P.start_used == gf.GF_START_GIVEN  # the prediction is kept, not replaced by the fallback
-    assert it1 < 0.75 * it0, (it0, it1)
+    assert it1 < 0.65 * it0, (it0, it1)  # measured 3277 / 5871 = 0.56
     # what the network adds beyond the affine part of the start (which alone removes about 20%)
     _, it2, _ = pb.P.solve(b, x0=_affine_start(gf, W, 2, 1e-3), tol=1e-10)
     print(f"\n512^2 field 123456: plain {it0}, affine start {it2}, learned start {it1} iterations")
-    assert it1 < 0.85 * it2, (it2, it1)
+    assert it1 < 0.8 * it2, (it2, it1)  # measured 3277 / 4721 = 0.69
 
 
 def _affine_start(gf, W, dim, gamma):
