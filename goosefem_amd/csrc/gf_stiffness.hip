@@ -332,18 +332,21 @@ __global__ void k_elemmat_dot(int64_t nrows, int K, const TE* __restrict__ Ke, c
 // all of its 16-byte loads before using any, multiplies by the element vector (FP64 products),
 // stages the products in LDS and sums each row in column order (the order of k_elemmat_dot, so the
 // results are identical).  Ke and ue must be 16-byte aligned.
-template <int K, class TE>
+template <int K, class TE, int RW>
 __global__ void __launch_bounds__(256) k_elemmat_dot_rows(int64_t nrows, const TE* __restrict__ Ke,
                                                          const double* __restrict__ ue, double* __restrict__ out)
 {
+    // RW rows per wave: their RW K contiguous entries are loaded by all 64 lanes, 16 bytes at a
+    // time, the products go to LDS, and lane l < RW sums row l in column order
     constexpr int VW = 16 / static_cast<int>(sizeof(TE)); // entries per 16-byte load
     static_assert(K % VW == 0, "whole loads per row");
+    static_assert((RW * K) % (VW * 64) == 0, "whole loads per wave");
     typedef TE vec __attribute__((ext_vector_type(VW)));
     constexpr int RP = K + 1; // LDS row pitch
-    constexpr int NV = K / VW;
-    __shared__ double sh[4][64 * RP];
+    constexpr int NV = RW * K / (VW * 64);
+    __shared__ double sh[4][RW * RP];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * 64;
+    const int64_t r0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * RW;
     if (r0 >= nrows) return; // uniform per wave
     const int64_t base = r0 * K, total = nrows * K;
     const vec* KV = reinterpret_cast<const vec*>(Ke + base);
@@ -374,7 +377,7 @@ __global__ void __launch_bounds__(256) k_elemmat_dot_rows(int64_t nrows, const T
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int64_t t = r0 + l;
-    if (t < nrows) {
+    if (l < RW && t < nrows) {
         double acc = 0.0;
 #pragma unroll
         for (int c = 0; c < K; ++c) acc += sh[w][l * RP + c];
@@ -439,12 +442,13 @@ int elemmat_dot(int64_t nelem, int K, const TE* elemmat, const double* elemvec, 
         GF_REQUIRE(elemmat && elemvec && out && K > 0, "invalid argument");
         const int64_t n = nelem * K;
         if (n == 0) return;
-        const unsigned gw = static_cast<unsigned>((n + 255) / 256); // 4 waves x 64 rows per block
+        constexpr int RW = 32;                                          // rows per wave
+        const unsigned gw = static_cast<unsigned>((n + 4 * RW - 1) / (4 * RW)); // 4 waves per block
         const bool vec = aligned16(elemmat) && aligned16(elemvec);
         if (K == 24 && vec)
-            k_elemmat_dot_rows<24, TE><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
+            k_elemmat_dot_rows<24, TE, RW><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
         else if (K == 8 && vec)
-            k_elemmat_dot_rows<8, TE><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
+            k_elemmat_dot_rows<8, TE, RW><<<gw, 256, 0, S(stream)>>>(n, elemmat, elemvec, out);
         else
             k_elemmat_dot<TE><<<nblocks(n, 256), 256, 0, S(stream)>>>(n, K, elemmat, elemvec, out);
         check_launch();
