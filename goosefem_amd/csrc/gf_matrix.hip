@@ -279,7 +279,9 @@ void spmv(gf_matrix_s* A, const double* x, double* y, hipStream_t s)
     if (A->nrows == 0) return;
     if (A->maxrow <= 32)
         k_spmv<8, 32><<<nblocks(A->nrows, BS / 8), BS, 0, s>>>(A->nrows, A->rowptr.p, A->col.p, A->val.p, x, y);
-    else if (A->maxrow <= 96) // Hex8 rows of up to 81 entries: 6 per lane
+    else if (A->maxrow <= 88) // Hex8 rows of up to 81 entries: 11 per lane, 8 rows per wave
+        k_spmv<8, 88><<<nblocks(A->nrows, BS / 8), BS, 0, s>>>(A->nrows, A->rowptr.p, A->col.p, A->val.p, x, y);
+    else if (A->maxrow <= 96)
         k_spmv<16, 96><<<nblocks(A->nrows, BS / 16), BS, 0, s>>>(A->nrows, A->rowptr.p, A->col.p, A->val.p, x, y);
     else if (A->maxrow <= 128)
         k_spmv<16, 128><<<nblocks(A->nrows, BS / 16), BS, 0, s>>>(A->nrows, A->rowptr.p, A->col.p, A->val.p, x, y);
