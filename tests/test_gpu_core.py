@@ -360,7 +360,9 @@ def _explicit_setup(nx, ny, seed=1, shuffle=False, fixed=True, kvar=False, disto
                                                             (40, 24, False, False, True, False),
                                                             (70, 45, True, False, True, False),
                                                             (24, 20, True, True, False, True),
-                                                            (33, 29, True, True, True, "perm")])
+                                                            (33, 29, True, True, True, "perm"),
+                                                            (128, 20, False, True, False, False),
+                                                            (40, 32, False, False, True, False)])
 def test_explicit_fp64_bit_exact(gf, nx, ny, shuffle, kvar, distort, tied):
     # tied: False, True (periodic DOFs) or "perm" (a permuted numbering, each DOF still on one node)
     coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, shuffle=shuffle, kvar=kvar, distort=distort,
@@ -389,6 +391,23 @@ def test_explicit_fp64_bit_exact_any_mass(gf, shuffle):
     X.set_state(u, v, a)
     fint = oracle.vv_steps(7, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a)
     X.step(7, gf.F64, store_fint=True)
+    gu, gv_, ga = (host(t) for t in X.state())
+    assert np.array_equal(gu, u) and np.array_equal(gv_, v) and np.array_equal(ga, a)
+    assert np.array_equal(host(X.fint()), fint)
+
+
+@pytest.mark.parametrize("case", ["structured", "blocks", "blocks-permuted-dofs", "gather"])
+def test_explicit_fp64_bit_exact_fext(gf, case):
+    # an external force (MatrixDiagonalPartitioned::solve of f_ext - f_int, main.cpp:152-155) on each
+    # of the step's kernels, bit-exact with the oracle
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(
+        30, 26, shuffle=case != "structured", kvar=True, tied=case == "gather", permdofs=case == "blocks-permuted-dofs")
+    fext = 1e-2 * oracle.randn(55, inv.size)
+    X = gf.ExplicitQuad4(gf.Vector(conn, dofs), coor, Ke, Ge, inv, 0.01, fext=fext)
+    assert X.path == case.split("-")[0]
+    X.set_state(u, v, a)
+    fint = oracle.vv_steps(5, QUAD4_PLANAR, conn, dofs, q.dNx, q.dV(), Ke, Ge, inv, 0.01, u, v, a, fext=fext)
+    X.step(5, gf.F64, store_fint=True)
     gu, gv_, ga = (host(t) for t in X.state())
     assert np.array_equal(gu, u) and np.array_equal(gv_, v) and np.array_equal(ga, a)
     assert np.array_equal(host(X.fint()), fint)
