@@ -23,6 +23,7 @@
 // DOF-centric ordered gather + update, which also forms the next step's displacement.
 #include <cmath>
 #include <functional>
+#include <unordered_map>
 #include <cstdio>
 #include <cstdlib>
 #include <type_traits>
@@ -558,7 +559,8 @@ __global__ void __launch_bounds__(BS) k_explicit_blocks(
     const int64_t k = static_cast<int64_t>(kb) + threadIdx.x;
     const bool active = static_cast<int>(threadIdx.x) < own;
     // (0) local nodes; thread t's own node is local node t
-    if (im_code) s_tab[threadIdx.x] = im_tab[threadIdx.x]; // BS = 256 entries
+    static_assert(BS == 256, "one palette entry per thread");
+    if (im_code) s_tab[threadIdx.x] = im_tab[threadIdx.x];
     int2 dme = make_int2(0, 0);
     for (int base = 0; base < nl; base += 2 * BS) {
         // two local nodes per thread and pass, their loads issued together (indices clamped)
@@ -1027,18 +1029,22 @@ int gf_explicit_create(gf_explicit* out, gf_vector v, const double* coor, const 
                 std::vector<double> mh(v->ndof);
                 if (v->ndof) GF_HIP(hipMemcpy(mh.data(), inv_mass, v->ndof * sizeof(double), hipMemcpyDeviceToHost));
                 std::vector<uint64_t> keys;
+                std::unordered_map<uint64_t, uint8_t> index;
                 std::vector<uint8_t> code(v->ndof);
                 bool ok = v->ndof > 0;
                 for (int64_t d = 0; ok && d < v->ndof; ++d) {
                     uint64_t bits;
                     std::memcpy(&bits, &mh[d], sizeof(bits));
-                    size_t j = 0;
-                    while (j < keys.size() && keys[j] != bits) ++j;
-                    if (j == keys.size()) {
-                        if (keys.size() == 256) ok = false;
-                        else keys.push_back(bits);
+                    auto it = index.find(bits);
+                    if (it == index.end()) {
+                        if (keys.size() == 256) {
+                            ok = false;
+                            break;
+                        }
+                        it = index.emplace(bits, static_cast<uint8_t>(keys.size())).first;
+                        keys.push_back(bits);
                     }
-                    code[d] = static_cast<uint8_t>(j);
+                    code[d] = it->second;
                 }
                 if (ok) {
                     std::vector<double> tab(256, 0.0);
