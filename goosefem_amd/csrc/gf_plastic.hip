@@ -140,10 +140,28 @@ __global__ void __launch_bounds__(BS)
 // without writing Eps or C to memory.  A block owns EB elements: first one thread per quadrature
 // point forms the strain, runs the return map (updating Epsp, epsp) and leaves in LDS the point's
 // geometry, the in-plane stress and the four scalars that define its tangent; then one thread per
-// (element, node m, node n) forms the 2 x 2 block K(m., n.) and one thread per (element, node) the
-// force.  Every value follows the expression and summation order of the unfused calls
-// (Element.h:1102-1176 loops for the integrals), so the results are theirs bit for bit.
+// (element, node m) forms the force fe(m, *) and the rows K(2m .. 2m + 1, *): per quadrature point it
+// forms the 16 in-plane tangent entries C(i, j, k, l) and the products dNx(m, i) C(i, j, k, l) once
+// and uses them for all four column nodes n.  The per-point tensors written (Epsp, and Sig when
+// asked for) and the element matrices go out through LDS, so the block writes each as one contiguous,
+// coalesced stream; a thread writing its own 72- or 128-byte record directly costs up to 2.5x the
+// pass time.  Every value follows the expression and summation order of the unfused calls
+// (Element.h:1102-1176 loops for the integrals; contraction is off), so the results are theirs bit
+// for bit.
 constexpr int EB = 64, NB = 256; // elements and threads per block (EB * 4 quadrature points = NB)
+constexpr int KROW = 18;         // doubles per staged (element, m) row: 16 entries + 2 of bank padding
+// LDS carve-up (doubles): sG[NB][8], sV[NB], sS[NB][4], sT[NB][3], sNn[NB][4]
+constexpr int L_G = 0, L_V = L_G + 8 * NB, L_S = L_V + NB, L_T = L_S + 4 * NB, L_N = L_T + 3 * NB,
+              L_END = L_N + 4 * NB;
+static_assert(KROW * NB <= L_END, "the staged K rows must fit the point arrays");
+static_assert(9 * NB <= L_END - L_S, "a staged point tensor must fit the sS, sT, sNn arrays");
+
+// the block's slice of a per-point [., 3, 3] tensor, staged in R = smem + L_S, as one stream
+__device__ inline void stream_point_tensor(const double* R, double* dst, int ne, int t)
+{
+    dbl2* d = reinterpret_cast<dbl2*>(dst);
+    for (int w = t; w < 18 * ne; w += NB) __builtin_nontemporal_store(reinterpret_cast<const dbl2*>(R)[w], d + w);
+}
 
 __global__ void __launch_bounds__(NB)
     k_plastic_newton_q4(int64_t nelem, const double* __restrict__ dNx, const double* __restrict__ vol,
@@ -153,10 +171,17 @@ __global__ void __launch_bounds__(NB)
                         double* __restrict__ Epsp, double* __restrict__ epsp, double* __restrict__ fe,
                         double* __restrict__ Ke, double* __restrict__ Sig)
 {
-    __shared__ double sG[NB][8], sV[NB], sS[NB][4], sT[NB][3], sNn[NB][4];
+    __shared__ __align__(16) double smem[L_END];
+    double(*sG)[8] = reinterpret_cast<double(*)[8]>(smem + L_G);
+    double* sV = smem + L_V;
+    double(*sS)[4] = reinterpret_cast<double(*)[4]>(smem + L_S);
+    double(*sT)[3] = reinterpret_cast<double(*)[3]>(smem + L_T);
+    double(*sNn)[4] = reinterpret_cast<double(*)[4]>(smem + L_N);
+    double* R = smem + L_S; // staging for the point tensors, before sS, sT, sNn are written
     const int64_t e0 = blockIdx.x * static_cast<int64_t>(EB);
     const int ne = static_cast<int>(nelem - e0 < EB ? nelem - e0 : EB);
     const int t = threadIdx.x;
+    double sg[9], kT[3], kN[4];
     if (t < 4 * ne) {
         const int64_t p = 4 * e0 + t; // quadrature point (element-major)
         const double* g = dNx + 8 * p;
@@ -207,31 +232,47 @@ __global__ void __launch_bounds__(NB)
         }
         const double sigm = k_ * tr;
         const double a = 1.5 * dg / q;
-        double* oP = Epsp + 9 * p;
-        for (int k = 0; k < 9; ++k) oP[k] = phi > 0.0 ? ep[k] + a * s[k] : ep[k];
+        for (int k = 0; k < 9; ++k) R[9 * t + k] = phi > 0.0 ? ep[k] + a * s[k] : ep[k];
         epsp[p] = ept + dg;
-        double sg[9];
         for (int k = 0; k < 9; ++k) sg[k] = f * s[k] + (k % 4 == 0 ? sigm : 0.0);
-        if (Sig)
-            for (int k = 0; k < 9; ++k) Sig[9 * p + k] = sg[k];
+        kT[0] = k_;
+        kT[1] = 2.0 * gm * f;
+        kT[2] = c;
+        for (int k = 0; k < 4; ++k) kN[k] = n[k];
+    }
+    __syncthreads();
+    stream_point_tensor(R, Epsp + 36 * e0, ne, t);
+    if (Sig) { // uniform over the block
+        __syncthreads();
+        if (t < 4 * ne)
+            for (int k = 0; k < 9; ++k) R[9 * t + k] = sg[k];
+        __syncthreads();
+        stream_point_tensor(R, Sig + 36 * e0, ne, t);
+    }
+    __syncthreads(); // R is free: the in-plane stress and the tangent scalars take its place
+    if (t < 4 * ne) {
         sS[t][0] = sg[0];
         sS[t][1] = sg[1];
         sS[t][2] = sg[3];
         sS[t][3] = sg[4];
-        sT[t][0] = k_;
-        sT[t][1] = 2.0 * gm * f;
-        sT[t][2] = c;
-        for (int k = 0; k < 4; ++k) sNn[t][k] = n[k];
+        for (int k = 0; k < 3; ++k) sT[t][k] = kT[k];
+        for (int k = 0; k < 4; ++k) sNn[t][k] = kN[k];
     }
     __syncthreads();
-    // K(m *, n *): 16 node pairs per element, four per thread
-    for (int it = t; it < 16 * ne; it += NB) {
-        const int el = it / 16, m = (it % 16) / 4, nn = it % 4;
-        double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    const int el = t / 4, m = t % 4;
+    double acc[4][2][2]; // acc[n][j][k] = K(2m + j, 2n + k)
+    if (t < 4 * ne) {
+        for (int nn = 0; nn < 4; ++nn)
+            for (int j = 0; j < 2; ++j)
+                for (int k = 0; k < 2; ++k) acc[nn][j][k] = 0.0;
+        double f0 = 0.0, f1 = 0.0;
         for (int qp = 0; qp < 4; ++qp) {
             const int r = 4 * el + qp;
             const double* g = sG[r];
             const double v = sV[r], K0 = sT[r][0], F0 = sT[r][1], C0 = sT[r][2];
+            f0 += (g[m * 2 + 0] * sS[r][0] + g[m * 2 + 1] * sS[r][2]) * v;
+            f1 += (g[m * 2 + 0] * sS[r][1] + g[m * 2 + 1] * sS[r][3]) * v;
+            double gc[2][2][2][2]; // dNx(m, i) * C(i, j, k, l)
             for (int i = 0; i < 2; ++i)
                 for (int j = 0; j < 2; ++j)
                     for (int k = 0; k < 2; ++k)
@@ -239,27 +280,27 @@ __global__ void __launch_bounds__(NB)
                             const double dij = i == j, dkl = k == l, dik = i == k, djl = j == l, dil = i == l, djk = j == k;
                             const double C = K0 * dij * dkl + F0 * (0.5 * (dik * djl + dil * djk) - dij * dkl / 3.0) +
                                              C0 * (sNn[r][2 * i + j] * sNn[r][2 * k + l]);
-                            acc[j][k] += g[m * 2 + i] * C * g[nn * 2 + l] * v;
+                            gc[i][j][k][l] = g[m * 2 + i] * C;
                         }
-        }
-        // two 16-byte streaming stores: K(2m + j, 2n .. 2n + 1); four neighbouring threads fill a row
-        dbl2* K = reinterpret_cast<dbl2*>(Ke + 64 * (e0 + el));
-        __builtin_nontemporal_store(dbl2{acc[0][0], acc[0][1]}, K + (m * 2) * 4 + nn);
-        __builtin_nontemporal_store(dbl2{acc[1][0], acc[1][1]}, K + (m * 2 + 1) * 4 + nn);
-    }
-    // fe(m, *): one thread per (element, node)
-    if (t < 4 * ne) {
-        const int el = t / 4, m = t % 4;
-        double f0 = 0.0, f1 = 0.0;
-        for (int qp = 0; qp < 4; ++qp) {
-            const int r = 4 * el + qp;
-            const double* g = sG[r];
-            const double v = sV[r];
-            f0 += (g[m * 2 + 0] * sS[r][0] + g[m * 2 + 1] * sS[r][2]) * v;
-            f1 += (g[m * 2 + 0] * sS[r][1] + g[m * 2 + 1] * sS[r][3]) * v;
+            for (int nn = 0; nn < 4; ++nn)
+                for (int i = 0; i < 2; ++i)
+                    for (int j = 0; j < 2; ++j)
+                        for (int k = 0; k < 2; ++k)
+                            for (int l = 0; l < 2; ++l) acc[nn][j][k] += gc[i][j][k][l] * g[nn * 2 + l] * v;
         }
         __builtin_nontemporal_store(dbl2{f0, f1}, reinterpret_cast<dbl2*>(fe + 8 * (e0 + el)) + m);
     }
+    __syncthreads(); // every thread is done with the point arrays
+    if (t < 4 * ne) { // rows 2m and 2m + 1 of the element matrix: 16 contiguous doubles
+        dbl2* row = reinterpret_cast<dbl2*>(smem + KROW * t);
+        for (int j = 0; j < 2; ++j)
+            for (int nn = 0; nn < 4; ++nn) row[4 * j + nn] = dbl2{acc[nn][j][0], acc[nn][j][1]};
+    }
+    __syncthreads();
+    // the block's K is 8 * (4 ne) contiguous 16-byte pieces; piece w is piece w % 8 of row w / 8
+    dbl2* K = reinterpret_cast<dbl2*>(Ke + 64 * e0);
+    for (int w = t; w < 32 * ne; w += NB)
+        __builtin_nontemporal_store(reinterpret_cast<const dbl2*>(smem + KROW * (w / 8))[w % 8], K + w);
 }
 
 } // namespace
@@ -354,7 +395,7 @@ int gf_plastic_newton_q4(gf_plastic m, gf_quad q, const double* elemvec, double*
         GF_REQUIRE(nne == 4 && ndim == 2 && tdim == 3 && nip == 4,
                    "needs a Quad4 QuadraturePlanar with four quadrature points");
         GF_REQUIRE(m->nq == 4 * nelem, "the material must hold one point per quadrature point");
-        GF_REQUIRE(aligned16(fe) && aligned16(Ke), "fe and Ke must be 16-byte aligned");
+        GF_REQUIRE(aligned16(fe) && aligned16(Ke) && (!Sig || aligned16(Sig)), "fe, Ke and Sig must be 16-byte aligned");
         const double *dNx = nullptr, *dV = nullptr;
         gf_quad_gradN(q, &dNx);
         gf_quad_dV(q, &dV);

@@ -347,8 +347,8 @@ int gf_plastic_increment(gf_plastic m, void* stream);
  * (periodic_elastoplastic.py:87-104), fused: Eps = symGradN_vector(elemvec); refresh();
  * fe = int_gradN_dot_tensor2_dV(Sig) [nelem, 4, 2]; Ke = int_gradN_dot_tensor4_dot_gradNT_dV(C)
  * [nelem, 8, 8]; Sig [nelem, 4, 3, 3] is written only if not NULL.  Eps and C never reach memory;
- * the results equal the unfused calls' bit for bit.  No counterpart in the reference (which
- * composes the three calls). */
+ * the results equal the unfused calls' bit for bit.  fe, Ke and Sig must be 16-byte aligned
+ * (GF_EINVAL otherwise).  No counterpart in the reference (which composes the three calls). */
 int gf_plastic_newton_q4(gf_plastic m, gf_quad q, const double* elemvec, double* fe, double* Ke, double* Sig,
                          void* stream);
 /* device pointers to the state arrays; any output may be NULL */

@@ -268,6 +268,12 @@ def test_fused_newton_pass_bit_exact(gf, nx, ny):
         assert np.array_equal(host(Ke), oq.Int_gradN_dot_tensor4_dot_gradNT_dV(ref.C))
     ep = host(fused.epsp)
     assert 0.1 < np.mean(ep > 0) < 0.95
+    # the outputs are written as 16-byte streams: an unaligned Sig is a clear error, not a fault
+    buf = torch.zeros(Sig.numel() + 1, dtype=torch.float64, device="cuda")
+    shifted = buf[1:].view(Sig.shape)
+    assert shifted.data_ptr() % 16 != 0
+    with pytest.raises(gf.GooseFEMError):
+        fused.newton_q4(quad, ue, Sig=shifted)
 
 
 def test_quadrature_shapes_and_astensor(gf):
