@@ -82,6 +82,29 @@ def test_quadrature_fp64_bit_exact(gf, et, distort):
     assert np.array_equal(host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C)), q.Int_gradN_dot_tensor4_dot_gradNT_dV(C))
 
 
+@pytest.mark.parametrize("et", [QUAD4, QUAD4_PLANAR])
+@pytest.mark.parametrize("nodal", [False, True])
+def test_quad4_stiffness_fp64_blocks(gf, et, nodal):
+    # the Quad4 FP64 K_e kernel (one thread per element row, 64 elements per workgroup, rows
+    # streamed through LDS) over many full workgroups and a partial one, bit-exact with the oracle;
+    # an unaligned output takes the generic kernel and gives the same bits
+    import torch
+
+    c, conn = mesh(et, 45, 0.2, seed=8)  # 46 x 45 = 2070 elements: 32 full workgroups + 22
+    x = oracle.Vector(conn, oracle.dofs(c.shape[0], 2)).AsElement(c)
+    q = oracle.Quadrature(et, x, *oracle.quad_rule(et, nodal))
+    gq = gquad(gf, et, x, nodal)
+    rng = np.random.default_rng(12)
+    C = rng.standard_normal((q.nelem, q.nip) + (q.tdim,) * 4)
+    Ke = q.Int_gradN_dot_tensor4_dot_gradNT_dV(C)
+    assert np.array_equal(host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C)), Ke)
+    buf = torch.zeros(Ke.size + 1, dtype=torch.float64, device="cuda")
+    shifted = buf[1:].view(Ke.shape)
+    assert shifted.data_ptr() % 16 != 0
+    gq.Int_gradN_dot_tensor4_dot_gradNT_dV(torch.from_numpy(C).cuda(), out=shifted)
+    assert np.array_equal(host(shifted), Ke)
+
+
 @pytest.mark.parametrize("rule", ["centroid", "gauss3"])
 def test_hex8_gradients_custom_rules(gf, rule):
     """Hex8 gradients of rules with 1 point (the stored-geometry kernel) and 27 points (geometry
