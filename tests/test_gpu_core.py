@@ -436,6 +436,39 @@ def test_explicit_fp64_bit_exact_fext(gf, case):
     assert np.array_equal(host(X.fint()), fint)
 
 
+@pytest.mark.parametrize("prec", ["F32", "F64", "F64_FAST"])
+@pytest.mark.parametrize("nx,ny,kvar,distort,sx,fext", [(130, 17, True, False, 0.37, False),
+                                                       (64, 48, False, True, 1.0, True),
+                                                       (3, 5, False, False, 1.0, True),
+                                                       (67, 9, True, False, 1.0, False)])
+def test_explicit_multistep_equals_single_steps(gf, prec, nx, ny, kvar, distort, sx, fext):
+    # a multi-step call (state buffers swapped on the device between its steps) gives the
+    # trajectory of single-step calls bit for bit, in every precision (FP32 included), on partial
+    # tiles, with fixed DOFs, an external force, per-element K and a caller's initial acceleration
+    coor, conn, dofs, q, inv, Ke, Ge, u, v, a = _explicit_setup(nx, ny, kvar=kvar, distort=distort)
+    coor = coor * np.array([sx, 1.0])
+    fe = 1e-2 * oracle.randn(56, inv.size) if fext else None
+    p = getattr(gf, prec)
+    X1 = gf.ExplicitQuad4(gf.Vector(conn, dofs), coor, Ke, Ge, inv, 0.01, fext=fe)
+    X2 = gf.ExplicitQuad4(gf.Vector(conn, dofs), coor, Ke, Ge, inv, 0.01, fext=fe)
+    assert X1.structured
+    a = 1e-3 * oracle.randn(57, inv.size) * (inv != 0)  # a caller's acceleration, read by the first step
+    X1.set_state(u, v, a)
+    X2.set_state(u, v, a)
+    X1.step(6, p, store_fint=True)
+    for _ in range(6):
+        X2.step(1, p, store_fint=True)
+    for s1, s2 in zip(X1.state(), X2.state()):
+        assert np.array_equal(host(s1), host(s2))
+    assert np.array_equal(host(X1.fint()), host(X2.fint()))
+    # and a second call continues from the stored state
+    X1.step(3, p)
+    for _ in range(3):
+        X2.step(1, p)
+    for s1, s2 in zip(X1.state(), X2.state()):
+        assert np.array_equal(host(s1), host(s2))
+
+
 @pytest.mark.parametrize("shuffle,kvar,distort", [(False, False, False), (False, True, False), (True, False, False),
                                                  (False, False, True)])
 def test_explicit_fp32_tolerance(gf, shuffle, kvar, distort):
