@@ -12,9 +12,12 @@
 // order, its wavefronts with DPP lane moves and its waves in order; consumers reduce the block
 // partials the same way.  The iteration
 // runs with no host synchronisation: scalars live on the device and a device flag stops the
-// iteration (kernels launched after convergence return immediately).
+// iteration (kernels launched after convergence return immediately).  On the structured Quad4 cell
+// the iteration is the single-reduction (Chronopoulos-Gear) form in one launch; elsewhere the
+// standard two-reduction form.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <tuple>
 #include <utility>
 
@@ -62,6 +65,8 @@ struct gf_pcg_s {
     int nty = 0;
     DBuf<double> ke8;   // Hex8: [2][24][24] element matrices per unit K and per unit G
     DBuf<double> p2;    // Quad4: second search-direction buffer of the fused iteration
+    DBuf<double> sr[2], sw[2], ss[2]; // Quad4 single-reduction iteration: r, w, s by iteration parity
+    DBuf<double> spart;               // its block partials [2][3][ntiles]
     DBuf<double> xprev; // the last converged solution (the previous-step start of the fallback)
     bool has_prev = false;
     int last_start = 0; // start of the last solve: GF_START_GIVEN / _PREVIOUS / _ZERO
@@ -236,6 +241,220 @@ __global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, i
     }
     (void)ntiles;
     if (partial) block_reduce_store_n<NT>(acc, partial);
+}
+
+// Single-reduction (Chronopoulos-Gear) Jacobi-PCG, iteration `it` in one launch, on the structured
+// periodic Quad4 operator.  With u = r ./ diag and s = K' p carried by recurrence:
+//   beta = gam / gam_prev,  alpha = gam / (del - beta gam / alpha_prev)   (it = 0: beta = 0, alpha = gam / del)
+//   p = u + beta p;  s = w + beta s;  x += alpha p;  r -= alpha s;  u = r ./ diag;  w = K' u
+//   gam = (r, u),  del = (w, u),  rr = (r, r)   (block partials; the next launch reduces them)
+// The same iterates as the two-reduction form in exact arithmetic, with one global reduction per
+// iteration instead of two, so one launch: the tile forms the new u on its halo ring from the
+// neighbours' r, w, s (double-buffered by iteration parity, since the launch writes the tile's own)
+// and the owned nodes' p and x are read and written by their tile only.  u is never stored.
+// Every block takes the same decisions from the same ordered partials; block 0 records gam and
+// alpha (slots by parity), the iteration count and the done flag.
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_cgsr_periodic_q4(
+    int n, int ntx, int ntiles, int per_xcd, ShapeQ4 sh, const int32_t* __restrict__ sidx,
+    const double* __restrict__ Ke, const double* __restrict__ Ge, const double* __restrict__ invd,
+    const double* __restrict__ rc, const double* __restrict__ wc, const double* __restrict__ sc,
+    double* __restrict__ rn, double* __restrict__ wn, double* __restrict__ sn_, double* __restrict__ p,
+    double* __restrict__ x, const double* __restrict__ init_partial, const double* __restrict__ pin,
+    double* __restrict__ pout, double* __restrict__ scal, int32_t* __restrict__ state, int64_t it)
+{
+    if (state[0]) return; // uniform for the grid
+    constexpr int HX = TX + 2, HY = TY + 2, EX = TX + 1, EY = TY + 1, NT = TX * TY;
+    constexpr int NH = (HX * HY + NT - 1) / NT, NE = (EX * EY + NT - 1) / NT; // halo nodes, elements per thread
+    static_assert(NT == BS, "the partial sums stride by BS");
+    __shared__ double2 s_x[HY * HX];                              // the new u on the halo tile
+    __shared__ double s_f[EY * EX][8];                            // element forces
+    __shared__ double2 s_u0[NT], s_sv[NT], s_rv[NT];              // owned nodes: old u, new s, new r
+    const int b = blockIdx.x;
+    const int tile = per_xcd > 0 ? (b & 7) * per_xcd + (b >> 3) : b;
+    const int tx = tile % ntx, ty = tile / ntx;
+    const int x0 = tx * TX, y0 = ty * TY;
+    const int oly = threadIdx.x / TX, olx = threadIdx.x - (threadIdx.x / TX) * TX;
+
+    // every load that does not depend on this iteration's scalars is issued first, so that its
+    // latency overlaps the reduction of the previous launch's partials
+    int32_t hd[NH][2];
+    double hr[NH][2], hw[NH][2], hs[NH][2], hi[NH][2];
+#pragma unroll
+    for (int j = 0; j < NH; ++j) {
+        const int t = threadIdx.x + j * NT;
+        hd[j][0] = hd[j][1] = -1;
+        if (t < HX * HY) {
+            const int ly = t / HX, lx = t - (t / HX) * HX;
+            int gi = x0 - 1 + lx, gj = y0 - 1 + ly;
+            gi += gi < 0 ? n : 0;
+            gi -= gi >= n ? n : 0;
+            gj += gj < 0 ? n : 0;
+            gj -= gj >= n ? n : 0;
+            const int64_t k = (static_cast<int64_t>(gj) * n + gi) * 2;
+            hd[j][0] = sidx[k];
+            hd[j][1] = sidx[k + 1];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NH; ++j)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int32_t d = hd[j][c] >= 0 ? hd[j][c] : 0; // clamped: the values of d < 0 are unused
+            hr[j][c] = rc[d];
+            hw[j][c] = wc[d];
+            hs[j][c] = it == 0 ? 0.0 : sc[d];
+            hi[j][c] = invd[d];
+        }
+    const int64_t ko = (static_cast<int64_t>(y0 + oly) * n + x0 + olx) * 2;
+    const int32_t od[2] = {sidx[ko], sidx[ko + 1]};
+    double op[2], ox[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int32_t d = od[c] >= 0 ? od[c] : 0;
+        op[c] = it == 0 ? 0.0 : p[d];
+        ox[c] = x[d];
+    }
+    double ek[NE], eg[NE];
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+        const int t = threadIdx.x + j * NT;
+        ek[j] = eg[j] = 0.0;
+        if (t < EX * EY) {
+            const int ly = t / EX, lx = t - (t / EX) * EX;
+            int ei = x0 - 1 + lx, ej = y0 - 1 + ly;
+            ei += ei < 0 ? n : 0;
+            ej += ej < 0 ? n : 0;
+            const int64_t e = static_cast<int64_t>(ej) * n + ei;
+            ek[j] = Ke[e];
+            eg[j] = Ge[e];
+        }
+    }
+
+    // this iteration's scalars from the ordered partials (identical in every block)
+    double gam = 0.0, del = 0.0, rr = 0.0;
+    if (it == 0) { // gam, rr from the initial residual pass, del from the first operator pass
+        for (int i = threadIdx.x; i < NBR; i += BS) {
+            gam += init_partial[i];
+            rr += init_partial[NBR + i];
+        }
+        for (int i = threadIdx.x; i < ntiles; i += BS) del += pin[ntiles + i];
+    }
+    else {
+        for (int i = threadIdx.x; i < ntiles; i += BS) {
+            gam += pin[i];
+            del += pin[ntiles + i];
+            rr += pin[2 * ntiles + i];
+        }
+    }
+    block_sum3<BS>(gam, del, rr);
+    if (rr <= scal[4]) { // converged after `it` iterations
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal[3] = rr;
+            state[0] = 1;
+        }
+        return;
+    }
+    double beta = 0.0, alpha = 0.0;
+    if (it == 0) alpha = gam / del;
+    else {
+        beta = gam / scal[5 + ((it + 1) & 1)];
+        alpha = gam / (del - beta * gam / scal[7 + ((it + 1) & 1)]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[5 + (it & 1)] = gam;
+        scal[7 + (it & 1)] = alpha;
+        scal[3] = rr;
+        state[1] = static_cast<int32_t>(it + 1);
+    }
+
+    // s = w + beta s, r = r - alpha s, u = r ./ diag on the halo tile (periodic wrap, prescribed
+    // DOFs read as 0); the owned nodes also keep the old u, the new s and the new r in LDS
+#pragma unroll
+    for (int j = 0; j < NH; ++j) {
+        const int t = threadIdx.x + j * NT;
+        if (t >= HX * HY) continue;
+        const int ly = t / HX, lx = t - (t / HX) * HX;
+        double uv[2], u0[2], sv[2], rv[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            uv[c] = u0[c] = sv[c] = rv[c] = 0.0;
+            if (hd[j][c] >= 0) {
+                sv[c] = it == 0 ? hw[j][c] : hw[j][c] + beta * hs[j][c];
+                rv[c] = hr[j][c] - alpha * sv[c];
+                uv[c] = rv[c] * hi[j][c];
+                u0[c] = hr[j][c] * hi[j][c];
+            }
+        }
+        s_x[t] = make_double2(uv[0], uv[1]);
+        if (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY) {
+            const int o = (ly - 1) * TX + lx - 1;
+            s_u0[o] = make_double2(u0[0], u0[1]);
+            s_sv[o] = make_double2(sv[0], sv[1]);
+            s_rv[o] = make_double2(rv[0], rv[1]);
+        }
+    }
+    __syncthreads();
+    // w = K' u: element forces of the (TX+1) x (TY+1) elements touching the tile
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+        const int t = threadIdx.x + j * NT;
+        if (t >= EX * EY) continue;
+        const int ly = t / EX, lx = t - (t / EX) * EX;
+        const int n0 = ly * HX + lx;
+        const int nid[4] = {n0, n0 + 1, n0 + HX + 1, n0 + HX};
+        double u[4][2], f[4][2];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double2 xx = s_x[nid[m]];
+            u[m][0] = xx.x;
+            u[m][1] = xx.y;
+            f[m][0] = f[m][1] = 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) quad4_planar_qp_force(sh.g[q], sh.v[q], u, ek[j], eg[j], f);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            s_f[t][2 * m] = f[m][0];
+            s_f[t][2 * m + 1] = f[m][1];
+        }
+    }
+    __syncthreads();
+    // owned node: elements (i-1, j-1) as its node 2, (i, j-1) node 3, (i-1, j) node 1, (i, j) node 0
+    const int l0 = oly * EX + olx;
+    double f[2];
+    f[0] = s_f[l0][4];
+    f[1] = s_f[l0][5];
+    f[0] += s_f[l0 + 1][6];
+    f[1] += s_f[l0 + 1][7];
+    f[0] += s_f[l0 + EX][2];
+    f[1] += s_f[l0 + EX][3];
+    f[0] += s_f[l0 + EX + 1][0];
+    f[1] += s_f[l0 + EX + 1][1];
+    const double2 uo = s_x[(oly + 1) * HX + olx + 1], u0o = s_u0[threadIdx.x], svo = s_sv[threadIdx.x],
+                  rvo = s_rv[threadIdx.x];
+    const double un[2] = {uo.x, uo.y}, u0[2] = {u0o.x, u0o.y}, sv[2] = {svo.x, svo.y}, rv[2] = {rvo.x, rvo.y};
+    double g = 0.0, dl = 0.0, r2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int32_t d = od[c];
+        if (d < 0) continue;
+        const double pv = it == 0 ? u0[c] : u0[c] + beta * op[c];
+        x[d] = ox[c] + alpha * pv;
+        p[d] = pv;
+        sn_[d] = sv[c];
+        rn[d] = rv[c];
+        wn[d] = f[c];
+        g += rv[c] * un[c];
+        dl += f[c] * un[c];
+        r2 += rv[c] * rv[c];
+    }
+    block_sum3<NT>(g, dl, r2);
+    if (threadIdx.x == 0) {
+        pout[b] = g;
+        pout[ntiles + b] = dl;
+        pout[2 * ntiles + b] = r2;
+    }
 }
 
 // Hex8 Regular n^3 cell with periodic tyings: identical element geometry makes the element
@@ -604,6 +823,28 @@ __global__ void k_residual_final(const double* __restrict__ partial, double* __r
     if (threadIdx.x == 0) out[0] = std::sqrt(rr) / (bb > 0.0 ? std::sqrt(bb) : 1.0);
 }
 
+// the vectors and block partials of the single-reduction iteration (structured operators)
+void alloc_single_reduction(gf_pcg_s* P)
+{
+    for (int k = 0; k < 2; ++k) {
+        P->sr[k].alloc(P->nnu);
+        P->sw[k].alloc(P->nnu);
+        P->ss[k].alloc(P->nnu);
+    }
+    P->spart.alloc(6 * static_cast<size_t>(P->ntiles));
+}
+
+// GF_PCG_TWO_REDUCTIONS=1 keeps the two-reduction iteration (two launches per iteration) on the
+// structured Quad4 operator
+bool single_reduction()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("GF_PCG_TWO_REDUCTIONS");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 void upload_csr(Csr& c, const int64_t* ptr, const int64_t* idx, const double* val, int64_t nrows)
 {
     int64_t nnz = ptr[nrows];
@@ -781,6 +1022,7 @@ void detect_structured(gf_pcg P, const int64_t* cdu_ptr_h, const int64_t* cdu_co
     P->ntiles = static_cast<int>((n / TX) * (n / TY));
     P->per_xcd = P->ntiles % 8 == 0 ? P->ntiles / 8 : 0;
     P->p2.alloc(P->nnu);
+    alloc_single_reduction(P);
     P->structured = true;
 }
 
@@ -918,7 +1160,7 @@ int gf_pcg_create(gf_pcg* out, gf_vector v, gf_quad q, const double* K_e, const 
             GF_HIP(hipMemset(P->xfull.p, 0, P->ndof * sizeof(double)));
             P->fe.alloc(P->nelem * P->K);
             for (auto* b : {&P->invdiag, &P->x, &P->r, &P->z, &P->qv, &P->tmp}) b->alloc(nnu);
-            P->scal.alloc(8);
+            P->scal.alloc(12);
             P->state.alloc(4);
             detect_structured(P, cdu_ptr_h, cdu_col_h, cdu_val_h);
             detect_structured_hex8(P, cdu_ptr_h, cdu_col_h, cdu_val_h);
@@ -1043,8 +1285,6 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         int32_t cap = static_cast<int32_t>(maxiter);
         GF_HIP(hipMemcpyAsync(P->state.p + 2, &cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
         int32_t st[2] = {0, 0};
-        // Quad4: the operator forms the search direction (two launches per iteration).  Hex8 keeps
-        // the separate direction pass: its 3-D halo would read z and p_prev 2.3 times per node.
         const bool fused = P->structured && P->et == GF_QUAD4_PLANAR;
         ShapeQ4 sh;
         for (int q = 0; q < 4; ++q) {
@@ -1053,8 +1293,26 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         }
         double* pb[2] = {P->xfull.p, P->p2.p};
         const bool done0 = rr <= tol * tol * (bb > 0.0 ? bb : 1.0);
+        // Quad4: one launch per iteration (single-reduction CG), or the two-reduction form in two
+        // launches with the search direction formed in the operator.  Hex8 keeps three launches:
+        // its 3-D halo would read z and p_prev 2.3 times per node, and r, w, s, diag for the
+        // single-reduction form (measured 2.7x slower per iteration).
+        const bool single = fused && single_reduction();
+        if (single && !done0) {
+            // r0 is in r, u0 = r0 ./ diag in the u-part of xfull: w0 = K' u0 with the partials of
+            // (u0, w0) in the 'del' slot of the first launch's input
+            GF_HIP(hipMemcpyAsync(P->sr[0].p, P->r.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            apply(P, nullptr, nullptr, P->sw[0].p, true, s, P->xfull.p, P->spart.p + P->ntiles, nullptr);
+        }
         for (int64_t it = 0; it < maxiter && !done0;) {
-            for (int64_t k = 0; fused && k < check_every && it < maxiter; ++k, ++it) {
+            for (int64_t k = 0; single && k < check_every && it < maxiter; ++k, ++it) {
+                const int i0 = static_cast<int>(it & 1), i1 = 1 - i0;
+                k_cgsr_periodic_q4<32, 8><<<P->ntiles, 256, 0, s>>>(
+                    P->sn, P->ntx, P->ntiles, P->per_xcd, sh, P->sidx.p, P->Ke.p, P->Ge.p, P->invdiag.p, P->sr[i0].p,
+                    P->sw[i0].p, P->ss[i0].p, P->sr[i1].p, P->sw[i1].p, P->ss[i1].p, P->z.p, P->x.p, P->partial.p,
+                    P->spart.p + 3 * i0 * P->ntiles, P->spart.p + 3 * i1 * P->ntiles, P->scal.p, P->state.p, it);
+            }
+            for (int64_t k = 0; fused && !single && k < check_every && it < maxiter; ++k, ++it) {
                 // two launches per iteration: q = K' p with p = z + beta p_prev built in the operator,
                 // then the x / r / z update
                 k_apply_periodic_q4<32, 8, true><<<P->ntiles, 256, 0, s>>>(
@@ -1064,7 +1322,7 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
                 k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->ntiles, P->scal.p, pb[it & 1], P->qv.p,
                                                 P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p, it);
             }
-            for (int64_t k = 0; !fused && k < check_every && it < maxiter; ++k, ++it) {
+            for (int64_t k = 0; !fused && !single && k < check_every && it < maxiter; ++k, ++it) {
                 apply(P, nullptr, nullptr, P->qv.p, true, s, P->xfull.p, P->partial.p + 3 * NBR, P->state.p);
                 k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->structured ? P->ntiles : NBR, P->scal.p,
                                                 P->xfull.p, P->qv.p, P->invdiag.p, P->x.p, P->r.p, P->z.p,

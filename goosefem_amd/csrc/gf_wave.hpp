@@ -84,4 +84,35 @@ __device__ inline double block_sum(double v)
     return t;
 }
 
+// Three sums over a workgroup at once (one LDS pass, two barriers instead of six); each value is
+// added in the same order as block_sum adds it.
+template <int NTH>
+__device__ inline void block_sum3(double& a, double& b, double& c)
+{
+    static_assert(NTH % 64 == 0 && NTH <= 1024, "NTH: a multiple of 64, at most 1024");
+    constexpr int NW = NTH / 64;
+    __shared__ double wsum[3][NW];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    if (NW == 1) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        wsum[0][wave] = a;
+        wsum[1][wave] = b;
+        wsum[2][wave] = c;
+    }
+    __syncthreads();
+    double ta = wsum[0][0], tb = wsum[1][0], tc = wsum[2][0];
+    for (int k = 1; k < NW; ++k) {
+        ta += wsum[0][k];
+        tb += wsum[1][k];
+        tc += wsum[2][k];
+    }
+    __syncthreads();
+    a = ta;
+    b = tb;
+    c = tc;
+}
+
 } // namespace gf
