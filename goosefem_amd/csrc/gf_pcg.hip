@@ -17,7 +17,6 @@
 // standard two-reduction form.
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <tuple>
 #include <utility>
 
@@ -64,7 +63,6 @@ struct gf_pcg_s {
     DBuf<int32_t> sidx; // [n^d * nd] u-DOF of grid node component c, or -1
     int nty = 0;
     DBuf<double> ke8;   // Hex8: [2][24][24] element matrices per unit K and per unit G
-    DBuf<double> p2;    // Quad4: second search-direction buffer of the fused iteration
     DBuf<double> sr[2], sw[2], ss[2]; // Quad4 single-reduction iteration: r, w, s by iteration parity
     DBuf<double> spart;               // its block partials [2][3][ntiles]
     DBuf<double> xprev; // the last converged solution (the previous-step start of the fallback)
@@ -126,42 +124,19 @@ struct ShapeQ4 {
     double v[4];
 };
 
-// FUSED (CG iteration it): the operand is the new search direction p = z + beta p_prev, built on the
-// fly (beta = rz_it / rz_it-1 from the ordered partials of the previous update, identical in every
-// block); the owned nodes' p is written to `pcur` (x is then z, `pprev` the previous direction).
-// Every block takes the same convergence decision from the same partials; block 0 records rz_it
-// and raises the done flag.
-template <int TX, int TY, bool FUSED>
-__global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, int ntiles, int per_xcd, ShapeQ4 sh,
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, int per_xcd, ShapeQ4 sh,
                                                                const int32_t* __restrict__ sidx,
                                                                const double* __restrict__ Ke,
                                                                const double* __restrict__ Ge,
                                                                const double* __restrict__ x, double* __restrict__ y,
                                                                double* __restrict__ partial,
-                                                               const int32_t* __restrict__ state_c,
-                                                               const double* __restrict__ pprev, double* __restrict__ pcur,
-                                                               const double* __restrict__ rz_rr, double* __restrict__ scal,
-                                                               int32_t* __restrict__ state, int64_t it)
+                                                               const int32_t* __restrict__ state_c)
 {
     if (state_c && state_c[0]) return; // uniform for the grid
     constexpr int HX = TX + 2, HY = TY + 2, EX = TX + 1, EY = TY + 1, NT = TX * TY;
     __shared__ double2 s_x[HY * HX];
     __shared__ double s_f[EY * EX][8];
-    double beta = 0.0;
-    if constexpr (FUSED) {
-        if (it > 0) {
-            const double rz_new = reduce_partials(rz_rr), rr = reduce_partials(rz_rr + NBR);
-            if (rr <= scal[4]) { // converged after `it` iterations
-                if (blockIdx.x == 0 && threadIdx.x == 0) state[0] = 1;
-                return;
-            }
-            beta = rz_new / scal[(it - 1) & 1];
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                scal[it & 1] = rz_new;
-                scal[3] = rr;
-            }
-        }
-    }
     const int b = blockIdx.x;
     const int tile = per_xcd > 0 ? (b & 7) * per_xcd + (b >> 3) : b;
     const int tx = tile % ntx, ty = tile / ntx;
@@ -176,16 +151,7 @@ __global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, i
         gj -= gj >= n ? n : 0;
         const int64_t k = (static_cast<int64_t>(gj) * n + gi) * 2;
         const int32_t d0 = sidx[k], d1 = sidx[k + 1];
-        double v0 = 0.0, v1 = 0.0;
-        if constexpr (FUSED) {
-            if (d0 >= 0) v0 = it > 0 ? x[d0] + beta * pprev[d0] : x[d0];
-            if (d1 >= 0) v1 = it > 0 ? x[d1] + beta * pprev[d1] : x[d1];
-        }
-        else {
-            v0 = d0 >= 0 ? x[d0] : 0.0;
-            v1 = d1 >= 0 ? x[d1] : 0.0;
-        }
-        s_x[t] = make_double2(v0, v1);
+        s_x[t] = make_double2(d0 >= 0 ? x[d0] : 0.0, d1 >= 0 ? x[d1] : 0.0);
     }
     __syncthreads();
     // element forces of the (TX+1) x (TY+1) elements touching the tile
@@ -232,14 +198,11 @@ __global__ void __launch_bounds__(TX * TY) k_apply_periodic_q4(int n, int ntx, i
     if (d0 >= 0) {
         y[d0] = f0;
         acc += po.x * f0;
-        if constexpr (FUSED) pcur[d0] = po.x;
     }
     if (d1 >= 0) {
         y[d1] = f1;
         acc += po.y * f1;
-        if constexpr (FUSED) pcur[d1] = po.y;
     }
-    (void)ntiles;
     if (partial) block_reduce_store_n<NT>(acc, partial);
 }
 
@@ -834,17 +797,6 @@ void alloc_single_reduction(gf_pcg_s* P)
     P->spart.alloc(6 * static_cast<size_t>(P->ntiles));
 }
 
-// GF_PCG_TWO_REDUCTIONS=1 keeps the two-reduction iteration (two launches per iteration) on the
-// structured Quad4 operator
-bool single_reduction()
-{
-    static const bool on = [] {
-        const char* e = std::getenv("GF_PCG_TWO_REDUCTIONS");
-        return !(e && e[0] == '1');
-    }();
-    return on;
-}
-
 void upload_csr(Csr& c, const int64_t* ptr, const int64_t* idx, const double* val, int64_t nrows)
 {
     int64_t nnz = ptr[nrows];
@@ -877,10 +829,8 @@ void apply(gf_pcg_s* P, const double* xu, const double* xp, double* yu, bool xu_
                 for (int k = 0; k < 8; ++k) sh.g[q][k] = P->sh_g[q * 8 + k];
                 sh.v[q] = P->sh_v[q];
             }
-            k_apply_periodic_q4<32, 8, false><<<P->ntiles, 256, 0, s>>>(P->sn, P->ntx, P->ntiles, P->per_xcd, sh,
-                                                                        P->sidx.p, P->Ke.p, P->Ge.p, xs, yu,
-                                                                        w ? partial : nullptr, state, nullptr,
-                                                                        nullptr, nullptr, nullptr, nullptr, 0);
+            k_apply_periodic_q4<32, 8><<<P->ntiles, 256, 0, s>>>(P->sn, P->ntx, P->per_xcd, sh, P->sidx.p, P->Ke.p,
+                                                                 P->Ge.p, xs, yu, w ? partial : nullptr, state);
             check_launch();
             return;
         }
@@ -1021,7 +971,6 @@ void detect_structured(gf_pcg P, const int64_t* cdu_ptr_h, const int64_t* cdu_co
     P->ntx = static_cast<int>(n / TX);
     P->ntiles = static_cast<int>((n / TX) * (n / TY));
     P->per_xcd = P->ntiles % 8 == 0 ? P->ntiles / 8 : 0;
-    P->p2.alloc(P->nnu);
     alloc_single_reduction(P);
     P->structured = true;
 }
@@ -1285,19 +1234,16 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         int32_t cap = static_cast<int32_t>(maxiter);
         GF_HIP(hipMemcpyAsync(P->state.p + 2, &cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
         int32_t st[2] = {0, 0};
-        const bool fused = P->structured && P->et == GF_QUAD4_PLANAR;
         ShapeQ4 sh;
         for (int q = 0; q < 4; ++q) {
             for (int k = 0; k < 8; ++k) sh.g[q][k] = P->sh_g[q * 8 + k];
             sh.v[q] = P->sh_v[q];
         }
-        double* pb[2] = {P->xfull.p, P->p2.p};
         const bool done0 = rr <= tol * tol * (bb > 0.0 ? bb : 1.0);
-        // Quad4: one launch per iteration (single-reduction CG), or the two-reduction form in two
-        // launches with the search direction formed in the operator.  Hex8 keeps three launches:
-        // its 3-D halo would read z and p_prev 2.3 times per node, and r, w, s, diag for the
-        // single-reduction form (measured 2.7x slower per iteration).
-        const bool single = fused && single_reduction();
+        // Quad4: one launch per iteration (single-reduction CG).  Hex8 keeps three launches
+        // (operator, update, direction): its 3-D halo would read z and p_prev 2.3 times per node,
+        // and r, w, s and the diagonal in the single-reduction form (measured 2.7x slower).
+        const bool single = P->structured && P->et == GF_QUAD4_PLANAR;
         if (single && !done0) {
             // r0 is in r, u0 = r0 ./ diag in the u-part of xfull: w0 = K' u0 with the partials of
             // (u0, w0) in the 'del' slot of the first launch's input
@@ -1312,17 +1258,7 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
                     P->sw[i0].p, P->ss[i0].p, P->sr[i1].p, P->sw[i1].p, P->ss[i1].p, P->z.p, P->x.p, P->partial.p,
                     P->spart.p + 3 * i0 * P->ntiles, P->spart.p + 3 * i1 * P->ntiles, P->scal.p, P->state.p, it);
             }
-            for (int64_t k = 0; fused && !single && k < check_every && it < maxiter; ++k, ++it) {
-                // two launches per iteration: q = K' p with p = z + beta p_prev built in the operator,
-                // then the x / r / z update
-                k_apply_periodic_q4<32, 8, true><<<P->ntiles, 256, 0, s>>>(
-                    P->sn, P->ntx, P->ntiles, P->per_xcd, sh, P->sidx.p, P->Ke.p, P->Ge.p, P->z.p, P->qv.p,
-                    P->partial.p + 3 * NBR, P->state.p, pb[(it + 1) & 1], pb[it & 1], P->partial.p, P->scal.p,
-                    P->state.p, it);
-                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->ntiles, P->scal.p, pb[it & 1], P->qv.p,
-                                                P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p, it);
-            }
-            for (int64_t k = 0; !fused && !single && k < check_every && it < maxiter; ++k, ++it) {
+            for (int64_t k = 0; !single && k < check_every && it < maxiter; ++k, ++it) {
                 apply(P, nullptr, nullptr, P->qv.p, true, s, P->xfull.p, P->partial.p + 3 * NBR, P->state.p);
                 k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->structured ? P->ntiles : NBR, P->scal.p,
                                                 P->xfull.p, P->qv.p, P->invdiag.p, P->x.p, P->r.p, P->z.p,
