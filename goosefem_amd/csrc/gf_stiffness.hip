@@ -263,13 +263,17 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_bf16x3(int64_t nelem, 
     }
 }
 
-// FP64 in reference order, one wavefront per element, C and dNx from LDS.
+// FP64 in reference order, one wavefront per element, C and dNx from LDS.  STREAM: the element
+// matrix goes through the wave's C area in LDS and leaves as one contiguous 16-byte stream
+// (elemmat 16-byte aligned) instead of nine strided scalar stores per lane.
+template <bool STREAM>
 __global__ void __launch_bounds__(64 * WAVES) k_stiffness_f64(int64_t nelem, const double* __restrict__ C,
                                                                const double* __restrict__ dNx,
                                                                const double* __restrict__ vol,
                                                                double* __restrict__ elemmat)
 {
-    __shared__ double sC[WAVES][NC];
+    static_assert(NC >= 576, "the element matrix must fit the C area");
+    __shared__ __align__(16) double sC[WAVES][NC];
     __shared__ double sg[WAVES][NG];
     __shared__ double sv[WAVES][NQ];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -302,10 +306,27 @@ __global__ void __launch_bounds__(64 * WAVES) k_stiffness_f64(int64_t nelem, con
                             acc[j][k] += g[m * 3 + i] * Cq[((i * 3 + j) * 3 + k) * 3 + l] * g[n * 3 + l] * v;
         }
         double* Ke = elemmat + e * 576;
+        if constexpr (STREAM) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); // every lane is done reading C
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+            for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) Ke[(m * 3 + j) * 24 + n * 3 + k] = acc[j][k];
+                for (int k = 0; k < 3; ++k) sC[w][(m * 3 + j) * 24 + n * 3 + k] = acc[j][k];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            typedef double d2 __attribute__((ext_vector_type(2)));
+            for (int t = lane; t < 288; t += 64)
+                __builtin_nontemporal_store(reinterpret_cast<const d2*>(sC[w])[t], reinterpret_cast<d2*>(Ke) + t);
+        }
+        else {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) Ke[(m * 3 + j) * 24 + n * 3 + k] = acc[j][k];
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -426,7 +447,10 @@ int stiffness_f64_hex8(gf_quad q, const double* C, double* elemmat, hipStream_t 
         GF_REQUIRE(quad_etype(q) == GF_HEX8 && quad_nip(q) == NQ, "Hex8 with 8 points");
         const int64_t ne = quad_nelem(q);
         if (ne == 0) return;
-        k_stiffness_f64<<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);
+        if (aligned16(elemmat))
+            k_stiffness_f64<true><<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);
+        else
+            k_stiffness_f64<false><<<grid_for(ne), 64 * WAVES, 0, s>>>(ne, C, quad_dNx(q), quad_vol(q), elemmat);
         check_launch();
     });
 }

@@ -242,6 +242,7 @@ def test_fused_newton_pass_bit_exact(gf, nx, ny):
     sy = rng.uniform(0.005, 0.02, shape)
     H = rng.uniform(0.0, 1.0, shape)
     fused = gf.LinearHardening(K, G, sy, H)
+    nosig = gf.LinearHardening(K, G, sy, H)  # the same pass without the optional stress output
     plain = gf.LinearHardening(K, G, sy, H)
     ref = oracle.LinearHardening(K, G, sy, H)
     oq = oracle.Quadrature(QUAD4_PLANAR, host(x))
@@ -251,10 +252,14 @@ def test_fused_newton_pass_bit_exact(gf, nx, ny):
         u = rng.standard_normal(coor.shape) * 0.002 * (inc + 1)
         ue = vec.AsElement(u)
         fused.increment()
+        nosig.increment()
         plain.increment()
         ref.increment()
         Sig = torch.empty_like(fused.Sig)
         fe, Ke = fused.newton_q4(quad, ue, Sig=Sig)
+        fe3, Ke3 = nosig.newton_q4(quad, ue)
+        assert np.array_equal(host(fe3), host(fe)) and np.array_equal(host(Ke3), host(Ke))
+        assert np.array_equal(host(nosig.Epsp), host(fused.Epsp)) and np.array_equal(host(nosig.epsp), host(fused.epsp))
         quad.symGradN_vector(ue, plain.Eps)
         plain.refresh()
         fe2 = quad.Int_gradN_dot_tensor2_dV(plain.Sig)

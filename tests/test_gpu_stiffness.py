@@ -172,7 +172,8 @@ def test_fp32_elemmat_assembly_and_product(gf, periodic):
 
 def test_alignment_rules(gf):
     # the FP32-I/O stiffness needs a 16-byte aligned tensor4 (clear error otherwise); the element
-    # GEMV falls back to its per-row kernel for unaligned arrays and gives the same results
+    # GEMV and the FP64 stiffness fall back to their per-row / per-lane stores for unaligned arrays
+    # and give the same results
     import torch
 
     c, conn, x, q, C = hex8_problem(2, 0.1, seed=5)
@@ -192,3 +193,12 @@ def test_alignment_rules(gf):
     kb[1:] = torch.from_numpy(Ke.ravel()).cuda()
     assert kb[1:].data_ptr() % 16 != 0
     assert np.array_equal(host(gf.elemmat_dot(kb[1:].view(Ke.shape), ue)), ref)
+    # the FP64 Hex8 stiffness streams an aligned element matrix out of LDS and stores an unaligned
+    # one element by element: the same bits, and the oracle's
+    k64 = host(gq.Int_gradN_dot_tensor4_dot_gradNT_dV(C))
+    assert np.array_equal(k64, q.Int_gradN_dot_tensor4_dot_gradNT_dV(C))
+    ob = torch.zeros(k64.size + 1, dtype=torch.float64, device="cuda")
+    out = ob[1:].view(k64.shape)
+    assert out.data_ptr() % 16 != 0
+    gq.Int_gradN_dot_tensor4_dot_gradNT_dV(torch.from_numpy(C).cuda(), out=out)
+    assert np.array_equal(host(out), k64)
