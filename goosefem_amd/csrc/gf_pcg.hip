@@ -1241,8 +1241,9 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         }
         const bool done0 = rr <= tol * tol * (bb > 0.0 ? bb : 1.0);
         // Quad4: one launch per iteration (single-reduction CG).  Hex8 keeps three launches
-        // (operator, update, direction): its 3-D halo would read z and p_prev 2.3 times per node,
-        // and r, w, s and the diagonal in the single-reduction form (measured 2.7x slower).
+        // (operator, update, direction): on its 3-D halo of 2.34 nodes per owned node, forming the
+        // direction in the operator (z and p_prev read) measured 1.25x slower per solve, and the
+        // single-reduction form (r, w, s and the diagonal read) 2.7x slower.
         const bool single = P->structured && P->et == GF_QUAD4_PLANAR;
         if (single && !done0) {
             // r0 is in r, u0 = r0 ./ diag in the u-part of xfull: w0 = K' u0 with the partials of
