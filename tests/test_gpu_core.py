@@ -724,3 +724,29 @@ def test_explicit_rejects_bad_input(gf):
     f = gf.lib().gf_quad_int_gradN_dot_tensor2_dV
     f.restype = ctypes.c_int
     assert f(None, None, None, ctypes.c_int(0), None) == gf.GF_EINVAL
+
+
+def test_zero_element_inputs_fail_cleanly(gf):
+    # a mesh with no elements is rejected with an error (GF_EINVAL) by the calls that move element
+    # arrays to or from the device (an empty array has no data pointer): no launch with an empty
+    # grid, no fault
+    conn = np.zeros((0, 4), dtype=np.int64)
+    dofs = np.arange(8).reshape(4, 2)
+    calls = [
+        lambda: gf.Vector(conn, dofs).AsElement(np.ones(8)),
+        lambda: gf.Vector(conn, dofs).AssembleDofs(np.zeros((0, 4, 2))),
+        lambda: gf.Element.Quad4.QuadraturePlanar(np.zeros((0, 4, 2))),
+        lambda: gf.Element.Quad4.Quadrature(np.zeros((0, 4, 2))),
+        lambda: gf.Element.Hex8.Quadrature(np.zeros((0, 8, 3))),
+        lambda: (lambda A: (A.assemble(np.zeros((0, 8, 8))), A.Todense()))(gf.Matrix(conn, dofs)),
+        lambda: (lambda A: (A.assemble(np.zeros((0, 8, 8))), A.data))(gf.MatrixDiagonal(conn, dofs)),
+        lambda: gf.elemmat_dot(np.zeros((0, 8, 8)), np.zeros((0, 8))),
+        lambda: gf.LinearHardening(np.zeros((0, 4)), 1.0, 0.01, 0.1),
+    ]
+    for call in calls:
+        with pytest.raises(gf.GooseFEMError):
+            call()
+    # the library is usable afterwards
+    c, cn = oracle.quad4_regular(3, 2)
+    v = gf.Vector(cn, oracle.dofs(c.shape[0], 2))
+    assert np.array_equal(host(v.AsElement(c)), oracle.Vector(cn, oracle.dofs(c.shape[0], 2)).AsElement(c))
