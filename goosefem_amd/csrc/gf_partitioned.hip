@@ -1281,12 +1281,11 @@ int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, i
         const int64_t nu = A->nnu, ni = A->nnu + A->nnp, nd = A->ndof;
         double* w = A->work.p;
         double *cx = w, *cr = w + nu, *cz = w + 2 * nu, *cp = w + 3 * nu, *cq = w + 4 * nu, *invd = w + 5 * nu,
-               *rhs = w + 6 * nu, *xfull = w + 7 * nu, *yfull = xfull + nd;
+               *rhs = w + 6 * nu, *xfull = w + 7 * nu;
         double* partial = A->red.p;
         double* scal = partial + 3 * NPB;
         int32_t* state = A->state.p;
-        // b, x as dofval (nodevec -> dofval, overwrite in loop order); X_p lives in the work's yfull
-        // until the prescribed product has been formed
+        // b, x as dofval (nodevec -> dofval, overwrite in loop order); X_p is read from xin
         DBuf<double> bfull(nd), xin(nd);
         int rc = gf_vector_as_dofs_nodevec(A->v, b, bfull.p, stream);
         if (rc != GF_OK) throw Error(rc, gf_last_error());

@@ -39,19 +39,13 @@ constexpr int WAVES = 4;           // wavefronts per workgroup
 constexpr int NQ = 8, NN = 8;      // quadrature points, nodes
 constexpr int NC = NQ * 81;        // tensor4 doubles per element
 constexpr int NG = NQ * NN * 3;    // dNx doubles per element
-constexpr int KK = 72, KKP = 80;   // contraction length, padded to 5 x 16
+constexpr int KKP = 80;            // contraction length 72, padded to 5 x 16
 
 struct alignas(16) WaveLDS {
     float C[NC];
     float g[NG];
     float vol[NQ];
 };
-
-__device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo)
-{
-    hi = static_cast<__bf16>(x);
-    lo = static_cast<__bf16>(x - static_cast<float>(hi));
-}
 
 // hi/lo split of eight values, converted in pairs (one packed conversion per pair and part)
 __device__ inline void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo)
