@@ -50,7 +50,7 @@ struct gf_mattyings_s {
     int64_t nnu = 0, nnp = 0, nnd = 0, ndof = 0;
     DBuf<int32_t> du_rp, du_cl, dp_rp, dp_cl, ud_rp, ud_cl;
     DBuf<double> du_v, dp_v, ud_v;
-    DBuf<double> work; // x, r, z, p, q, invd, rhs [nnu]; xfull, yfull, kdiag [ndof]
+    DBuf<double> work; // x, r, z, p, q, invd, rhs [nnu]; xfull, yfull, kdiag [ndof]; p2 [nnu]
     DBuf<double> red;
     DBuf<int32_t> state;
 };
@@ -512,24 +512,50 @@ __global__ void __launch_bounds__(PB) k_cg_residual_vec(int64_t n, const double*
     block_sum_store(abb, partial + NPB + blockIdx.x);
 }
 
-// beta = rz_new / rz_old; p = z + beta p; block 0 stores rz_new, r.r, the count and the done flag
-__global__ void __launch_bounds__(PB) k_ty_direction(int64_t n, const double* __restrict__ partial,
-                                                     double* __restrict__ scal, const double* __restrict__ z,
-                                                     double* __restrict__ p, int32_t* __restrict__ state, int64_t it)
+// Iterations after the first: the direction update fused into the expansion of the operand.  beta =
+// rz_new / rz_old from the previous update's partials (identical in every block); the u part of the
+// full vector is the new direction p = z + beta p_prev (also written to pnew), the prescribed part 0,
+// and each dependent entry sum_k C_du(d, k) p(k) with every p(k) formed the same way, so the values
+// are the ones a separate direction pass would have stored.  Block 0 stores rz_new, r.r and the
+// iteration count, and raises done at convergence (r.r <= tol^2 b.b).
+__global__ void __launch_bounds__(PB) k_ty_expand_dir(int64_t nnu, int64_t nnp, int64_t nnd,
+                                                      const double* __restrict__ z, const double* __restrict__ pprev,
+                                                      double* __restrict__ pnew, const int32_t* __restrict__ du_rp,
+                                                      const int32_t* __restrict__ du_cl, const double* __restrict__ du_v,
+                                                      double* __restrict__ xfull, const double* __restrict__ partial,
+                                                      double* __restrict__ scal, int32_t* __restrict__ state, int64_t it)
 {
     if (state[0]) return;
-    const double rz_old = scal[it & 1];
     const double rz_new = block_sum_partials(partial, NPB);
-    const double beta = rz_old != 0.0 ? rz_new / rz_old : 0.0;
-    for (int64_t i = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x; i < n; i += static_cast<int64_t>(NPB) * PB)
-        p[i] = z[i] + beta * p[i];
-    if (blockIdx.x != 0) return;
     const double rr = block_sum_partials(partial + NPB, NPB);
-    if (threadIdx.x == 0) {
-        scal[(it + 1) & 1] = rz_new;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[2] = rr;
-        state[1] = static_cast<int32_t>(it + 1);
-        if (rr <= scal[4] || it + 1 >= state[2]) state[0] = 1;
+        state[1] = static_cast<int32_t>(it); // the updates done
+    }
+    if (rr <= scal[4]) { // converged after `it` iterations
+        if (blockIdx.x == 0 && threadIdx.x == 0) state[0] = 1;
+        return;
+    }
+    const double rz_old = scal[(it - 1) & 1];
+    const double beta = rz_old != 0.0 ? rz_new / rz_old : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[it & 1] = rz_new;
+    const int64_t t = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x;
+    const int64_t ni = nnu + nnp;
+    if (t < nnu) {
+        const double pv = z[t] + beta * pprev[t];
+        pnew[t] = pv;
+        xfull[t] = pv;
+    }
+    else if (t < ni)
+        xfull[t] = 0.0;
+    else if (t < ni + nnd) {
+        const int64_t d = t - ni;
+        double sd = 0.0;
+        for (int32_t k = du_rp[d]; k < du_rp[d + 1]; ++k) {
+            const int32_t c = du_cl[k];
+            sd += du_v[k] * (z[c] + beta * pprev[c]);
+        }
+        xfull[t] = sd;
     }
 }
 
@@ -1207,7 +1233,7 @@ int gf_mattyings_create(gf_mattyings* out, gf_vector v, int64_t nnu, int64_t nnp
             A->ud_rp.upload(trp);
             A->ud_cl.upload(tcl);
             A->ud_v.upload(tv);
-            A->work.alloc(7 * nnu + 3 * A->ndof);
+            A->work.alloc(8 * nnu + 3 * A->ndof);
             A->red.alloc(3 * NPB + 8);
             A->state.alloc(4);
         }
@@ -1305,13 +1331,30 @@ int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, i
             GF_HIP(hipStreamSynchronize(s));
             if (st[0] == 2) throw Error(GF_EINVAL, "non-finite right-hand side");
             constexpr int64_t check_every = 16;
+            // four launches per iteration: the operand expansion forming the search direction (by
+            // iteration parity in p, p2), the product, the fold with the p.q partials, the update
+            double* pb[2] = {cp, w + 7 * nu + 3 * nd};
+            const unsigned gexp = nblocks(nd, PB);
+            auto expand_dir = [&](int64_t it) {
+                k_ty_expand_dir<<<gexp, PB, 0, s>>>(nu, A->nnp, A->nnd, cz, pb[(it + 1) & 1], pb[it & 1], A->du_rp.p,
+                                                     A->du_cl.p, A->du_v.p, xfull, partial, scal, state, it);
+            };
             for (int64_t it = 0; !st[0] && it < maxiter;) {
                 for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
-                    ty_product(A, cp, nullptr, cq, cp, partial + 2 * NPB, s);
-                    k_cg_update<<<NPB, PB, 0, s>>>(nu, partial + 2 * NPB, NPB, scal, invd, cp, cq, cx, cr, cz, partial, state,
-                                                   it);
-                    k_ty_direction<<<NPB, PB, 0, s>>>(nu, partial, scal, cz, cp, state, it);
+                    if (it == 0)
+                        ty_product(A, cp, nullptr, cq, cp, partial + 2 * NPB, s);
+                    else {
+                        expand_dir(it);
+                        check_launch();
+                        const int rc2 = gf_matrix_dot_dofval(A->K, xfull, xfull + nd, s);
+                        if (rc2 != GF_OK) throw Error(rc2, gf_last_error());
+                        k_ty_fold<<<NPB, PB, 0, s>>>(nu, nu + A->nnp, xfull + nd, A->ud_rp.p, A->ud_cl.p, A->ud_v.p,
+                                                     pb[it & 1], cq, partial + 2 * NPB);
+                    }
+                    k_cg_update<<<NPB, PB, 0, s>>>(nu, partial + 2 * NPB, NPB, scal, invd, pb[it & 1], cq, cx, cr, cz,
+                                                   partial, state, it);
                 }
+                if (it == maxiter) expand_dir(it); // the convergence test and count of the last update
                 check_launch();
                 GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
                 GF_HIP(hipStreamSynchronize(s));
