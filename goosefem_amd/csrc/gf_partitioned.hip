@@ -451,6 +451,40 @@ __global__ void __launch_bounds__(PB) k_ty_fold(int64_t nnu, int64_t nni, const 
     if (partial) block_sum_store(acc, partial + blockIdx.x);
 }
 
+// The condensed product and its p.q partials in one launch: q_u = (K x)_u + C_ud (K x)_d, where
+// the product rows of the dependent DOFs tied to u are formed where they are folded (a periodic
+// cell ties a handful of them to each boundary DOF and none to the interior).  LPR lanes per row
+// (lane-strided products, then shuffle sums in a fixed order); the partials are per workgroup.
+__global__ void __launch_bounds__(PB) k_ty_spmv_fold(int64_t nnu, int64_t nni, const int32_t* __restrict__ krp,
+                                                     const int32_t* __restrict__ kcl, const double* __restrict__ kv,
+                                                     const double* __restrict__ x, const int32_t* __restrict__ ud_rp,
+                                                     const int32_t* __restrict__ ud_cl, const double* __restrict__ ud_v,
+                                                     const double* __restrict__ p, double* __restrict__ q,
+                                                     double* __restrict__ partial, const int32_t* __restrict__ state)
+{
+    if (state[0]) return;
+    const int64_t g = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x;
+    const int64_t i = g / LPR;
+    const int j = static_cast<int>(g % LPR);
+    auto rowdot = [&](int64_t r) { // the same value in every lane of the row's group
+        double s = 0.0;
+        for (int32_t k = krp[r] + j; k < krp[r + 1]; k += LPR) s += kv[k] * x[kcl[k]];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        return s;
+    };
+    double c = 0.0;
+    if (i < nnu) { // uniform over the row's lanes
+        double s = rowdot(i);
+        for (int32_t k = ud_rp[i]; k < ud_rp[i + 1]; ++k) s += ud_v[k] * rowdot(nni + ud_cl[k]);
+        if (j == 0) {
+            q[i] = s;
+            c = p[i] * s;
+        }
+    }
+    block_sum_store(c, partial + blockIdx.x);
+}
+
 // the diagonal of each CSR row (0 where the row has none)
 __global__ void k_csr_diag(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cl,
                            const double* __restrict__ vl, double* __restrict__ dg)
@@ -1234,7 +1268,7 @@ int gf_mattyings_create(gf_mattyings* out, gf_vector v, int64_t nnu, int64_t nnp
             A->ud_cl.upload(tcl);
             A->ud_v.upload(tv);
             A->work.alloc(8 * nnu + 3 * A->ndof);
-            A->red.alloc(3 * NPB + 8);
+            A->red.alloc(3 * NPB + 8 + nblocks(LPR * nnu, PB)); // + the fused product's p.q partials
             A->state.alloc(4);
         }
         catch (...) {
@@ -1331,10 +1365,15 @@ int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, i
             GF_HIP(hipStreamSynchronize(s));
             if (st[0] == 2) throw Error(GF_EINVAL, "non-finite right-hand side");
             constexpr int64_t check_every = 16;
-            // four launches per iteration: the operand expansion forming the search direction (by
-            // iteration parity in p, p2), the product, the fold with the p.q partials, the update
+            // three launches per iteration: the operand expansion forming the search direction (by
+            // iteration parity in p, p2), the condensed product with the p.q partials, the update
             double* pb[2] = {cp, w + 7 * nu + 3 * nd};
-            const unsigned gexp = nblocks(nd, PB);
+            const unsigned gexp = nblocks(nd, PB), gprod = nblocks(LPR * nu, PB);
+            double* pq_part = partial + 3 * NPB + 8;
+            const int32_t *krp = nullptr, *kcl = nullptr;
+            const double* kv = nullptr;
+            rc = gf_matrix_csr(A->K, &krp, &kcl, &kv);
+            if (rc != GF_OK) throw Error(rc, gf_last_error());
             auto expand_dir = [&](int64_t it) {
                 k_ty_expand_dir<<<gexp, PB, 0, s>>>(nu, A->nnp, A->nnd, cz, pb[(it + 1) & 1], pb[it & 1], A->du_rp.p,
                                                      A->du_cl.p, A->du_v.p, xfull, partial, scal, state, it);
@@ -1342,17 +1381,15 @@ int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, i
             for (int64_t it = 0; !st[0] && it < maxiter;) {
                 for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
                     if (it == 0)
-                        ty_product(A, cp, nullptr, cq, cp, partial + 2 * NPB, s);
-                    else {
+                        k_ty_expand<<<nblocks(nd, BS), BS, 0, s>>>(nu, A->nnp, A->nnd, cp, nullptr, A->du_rp.p,
+                                                                    A->du_cl.p, A->du_v.p, A->dp_rp.p, A->dp_cl.p,
+                                                                    A->dp_v.p, xfull);
+                    else
                         expand_dir(it);
-                        check_launch();
-                        const int rc2 = gf_matrix_dot_dofval(A->K, xfull, xfull + nd, s);
-                        if (rc2 != GF_OK) throw Error(rc2, gf_last_error());
-                        k_ty_fold<<<NPB, PB, 0, s>>>(nu, nu + A->nnp, xfull + nd, A->ud_rp.p, A->ud_cl.p, A->ud_v.p,
-                                                     pb[it & 1], cq, partial + 2 * NPB);
-                    }
-                    k_cg_update<<<NPB, PB, 0, s>>>(nu, partial + 2 * NPB, NPB, scal, invd, pb[it & 1], cq, cx, cr, cz,
-                                                   partial, state, it);
+                    k_ty_spmv_fold<<<gprod, PB, 0, s>>>(nu, nu + A->nnp, krp, kcl, kv, xfull, A->ud_rp.p, A->ud_cl.p,
+                                                        A->ud_v.p, pb[it & 1], cq, pq_part, state);
+                    k_cg_update<<<NPB, PB, 0, s>>>(nu, pq_part, static_cast<int>(gprod), scal, invd, pb[it & 1], cq, cx,
+                                                   cr, cz, partial, state, it);
                 }
                 if (it == maxiter) expand_dir(it); // the convergence test and count of the last update
                 check_launch();
