@@ -64,6 +64,7 @@ struct gf_pcg_s {
     int nty = 0;
     DBuf<double> ke8;   // Hex8: [2][24][24] element matrices per unit K and per unit G
     DBuf<double> sr[2], sw[2], ss[2]; // Quad4 single-reduction iteration: r, w, s by iteration parity
+    DBuf<double> gp[2];               // general operator: the search direction by iteration parity
     DBuf<double> spart;               // its block partials [2][3][ntiles]
     DBuf<double> xprev; // the last converged solution (the previous-step start of the fallback)
     bool has_prev = false;
@@ -662,6 +663,60 @@ __global__ void k_expand(int64_t nnu, int64_t nnp, int64_t nnd, const double* __
     }
 }
 
+// General operator, CG iteration it >= 1: the direction update fused into the expansion.  beta =
+// rz_new / rz_old from the previous update's partials (identical in every block); the u part of
+// x_full is the new direction p = z + beta p_old (also written to pnew), the p part 0, and each
+// dependent entry sum_k C_du(d, k) p(k) with every p(k) formed the same way (p_old and pnew are
+// distinct buffers, so no block reads an entry another block is replacing).  Block 0 records rz_new
+// and r.r and raises done at convergence (r.r <= tol^2 b.b).
+__global__ void k_expand_dir(int64_t nnu, int64_t nnp, int64_t nnd, const double* __restrict__ z,
+                             const double* __restrict__ pold, double* __restrict__ pnew,
+                             const int32_t* __restrict__ uptr, const int32_t* __restrict__ uidx,
+                             const double* __restrict__ uval, double* __restrict__ xfull,
+                             const double* __restrict__ rz_rr, double* __restrict__ scal, int32_t* __restrict__ state,
+                             int64_t it)
+{
+    if (state[0]) return;
+    double rz_new = 0.0, rr = 0.0, unused = 0.0;
+    for (int i = threadIdx.x; i < NBR; i += BS) {
+        rz_new += rz_rr[i];
+        rr += rz_rr[NBR + i];
+    }
+    block_sum3<BS>(rz_new, rr, unused);
+    if (rr <= scal[4]) { // converged after `it` iterations
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal[3] = rr;
+            state[0] = 1;
+        }
+        return;
+    }
+    const double beta = rz_new / scal[(it - 1) & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[it & 1] = rz_new;
+        scal[3] = rr;
+    }
+    const int64_t n = nnu + nnp + nnd;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(BS) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * BS) {
+        if (i < nnu) {
+            const double pv = z[i] + beta * pold[i];
+            pnew[i] = pv;
+            xfull[i] = pv;
+        }
+        else if (i < nnu + nnp)
+            xfull[i] = 0.0;
+        else {
+            const int64_t d = i - nnu - nnp;
+            double sd = 0.0;
+            for (int32_t k = uptr[d]; k < uptr[d + 1]; ++k) {
+                const int32_t c = uidx[k];
+                sd += uval[k] * (z[c] + beta * pold[c]);
+            }
+            xfull[i] = sd;
+        }
+    }
+}
+
 __global__ void k_diag_fold(int64_t nnu, int64_t nni, const int32_t* __restrict__ eptr, const int32_t* __restrict__ eidx,
                             const double* __restrict__ de, const int32_t* __restrict__ tptr,
                             const int32_t* __restrict__ tidx, const double* __restrict__ tval,
@@ -808,6 +863,23 @@ void upload_csr(Csr& c, const int64_t* ptr, const int64_t* idx, const double* va
         c.idx.alloc(1);
         c.val.alloc(1);
     }
+}
+
+// y_u = K' x on the general operator from an already expanded x_full (element forces, ordered
+// gather + C_du^T fold, with the partials of (x_u, y_u)); the second half of apply
+void operator_from_full(gf_pcg_s* P, double* yu, double* partial, const int32_t* state, hipStream_t s)
+{
+    auto v = P->v;
+    const unsigned ge = nblocks(P->nelem, 128);
+    if (P->et == GF_HEX8)
+        k_elem_force<GF_HEX8><<<ge, 128, 0, s>>>(P->nelem, v->conn.p, v->dofs.p, quad_dNx(P->q), quad_vol(P->q),
+                                                 P->Ke.p, P->Ge.p, P->xfull.p, P->fe.p, state);
+    else
+        k_elem_force<GF_QUAD4_PLANAR><<<ge, 128, 0, s>>>(P->nelem, v->conn.p, v->dofs.p, quad_dNx(P->q),
+                                                         quad_vol(P->q), P->Ke.p, P->Ge.p, P->xfull.p, P->fe.p, state);
+    k_gather_fold<<<NBR, BS, 0, s>>>(P->nnu, P->nni, v->dof_eptr.p, v->dof_eidx.p, P->fe.p, P->cduT.ptr.p,
+                                     P->cduT.idx.p, P->cduT.val.p, yu, P->xfull.p, partial, state);
+    check_launch();
 }
 
 // y_u = K' x_u + K'_up x_p (x_u or x_p may be NULL); state == NULL: always runs
@@ -1245,6 +1317,15 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
         // direction in the operator (z and p_prev read) measured 1.25x slower per solve, and the
         // single-reduction form (r, w, s and the diagonal read) 2.7x slower.
         const bool single = P->structured && P->et == GF_QUAD4_PLANAR;
+        const bool hexs = P->structured && P->et == GF_HEX8, general = !P->structured;
+        if (general && !done0) {
+            // the first direction p0 = z0 (in x_full's u part) is the p_old of iteration 1
+            if (P->gp[0].n != static_cast<size_t>(n)) {
+                P->gp[0].alloc(n);
+                P->gp[1].alloc(n);
+            }
+            GF_HIP(hipMemcpyAsync(P->gp[0].p, P->xfull.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
         if (single && !done0) {
             // r0 is in r, u0 = r0 ./ diag in the u-part of xfull: w0 = K' u0 with the partials of
             // (u0, w0) in the 'del' slot of the first launch's input
@@ -1259,12 +1340,25 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
                     P->sw[i0].p, P->ss[i0].p, P->sr[i1].p, P->sw[i1].p, P->ss[i1].p, P->z.p, P->x.p, P->partial.p,
                     P->spart.p + 3 * i0 * P->ntiles, P->spart.p + 3 * i1 * P->ntiles, P->scal.p, P->state.p, it);
             }
-            for (int64_t k = 0; !single && k < check_every && it < maxiter; ++k, ++it) {
+            for (int64_t k = 0; hexs && k < check_every && it < maxiter; ++k, ++it) {
                 apply(P, nullptr, nullptr, P->qv.p, true, s, P->xfull.p, P->partial.p + 3 * NBR, P->state.p);
-                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->structured ? P->ntiles : NBR, P->scal.p,
-                                                P->xfull.p, P->qv.p, P->invdiag.p, P->x.p, P->r.p, P->z.p,
-                                                P->partial.p, P->state.p, it);
+                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, P->ntiles, P->scal.p, P->xfull.p, P->qv.p,
+                                                P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p, it);
                 k_pcg_direction<<<NBR, BS, 0, s>>>(n, P->partial.p, P->scal.p, P->z.p, P->xfull.p, P->state.p, it);
+            }
+            for (int64_t k = 0; general && k < check_every && it < maxiter; ++k, ++it) {
+                // four launches: the expansion forming the search direction (after the first
+                // iteration), element forces, ordered gather + fold with the p.q partials, update
+                if (it == 0)
+                    apply(P, nullptr, nullptr, P->qv.p, true, s, P->xfull.p, P->partial.p + 3 * NBR, P->state.p);
+                else {
+                    k_expand_dir<<<std::min<unsigned>(nblocks(P->ndof, BS), 4096), BS, 0, s>>>(
+                        P->nnu, P->nnp, P->nnd, P->z.p, P->gp[(it + 1) & 1].p, P->gp[it & 1].p, P->cdu.ptr.p,
+                        P->cdu.idx.p, P->cdu.val.p, P->xfull.p, P->partial.p, P->scal.p, P->state.p, it);
+                    operator_from_full(P, P->qv.p, P->partial.p + 3 * NBR, P->state.p, s);
+                }
+                k_pcg_update<<<NBR, BS, 0, s>>>(n, P->partial.p + 3 * NBR, NBR, P->scal.p, P->xfull.p, P->qv.p,
+                                                P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p, it);
             }
             check_launch();
             GF_HIP(hipMemcpyAsync(st, P->state.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
