@@ -164,6 +164,39 @@ int stiffness_f64_hex8(gf_quad q, const double* qtensor4, double* elemmat, hipSt
 // Narrow an int64 host index array to int32 (device form), checking the range.
 std::vector<int32_t> narrow(const int64_t* a, size_t n, const char* what);
 
+
+// Pinned host copies of a solver's device convergence state, one per chunk parity, with the events
+// that mark their arrival: the host waits for chunk k - 1's flag while chunk k is already queued, so
+// the GPU is not left idle between chunks (a chunk launched after convergence returns at once).
+struct StatePipe {
+    int32_t* h = nullptr; // [2][2]: done flag, iteration count
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    StatePipe()
+    {
+        GF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        GF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        GF_HIP(hipHostMalloc(reinterpret_cast<void**>(&h), 4 * sizeof(int32_t), hipHostMallocDefault));
+    }
+    ~StatePipe()
+    {
+        if (h) (void)hipHostFree(h);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    StatePipe(const StatePipe&) = delete;
+    StatePipe& operator=(const StatePipe&) = delete;
+    void post(const int32_t* dstate, int k, hipStream_t s)
+    {
+        GF_HIP(hipMemcpyAsync(h + 2 * (k & 1), dstate, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GF_HIP(hipEventRecord(ev[k & 1], s));
+    }
+    const int32_t* wait(int k)
+    {
+        GF_HIP(hipEventSynchronize(ev[k & 1]));
+        return h + 2 * (k & 1);
+    }
+};
+
 } // namespace gf
 
 // Opaque handle definitions shared across translation units.

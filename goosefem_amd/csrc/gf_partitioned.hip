@@ -948,6 +948,8 @@ void matpart_cg(gf_matpart A, const double* rhs, double* xu, bool warm, double t
     double* pq_part = partial + 3 * NPB + 8; // one p.q partial per apply block
     double* pb[2] = {W.cp, W.cp2};
     int64_t it = 0;
+    StatePipe pipe;
+    int chunk = 0;
     while (!st[0] && it < maxiter) {
         for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
             if (it == 0)
@@ -962,8 +964,10 @@ void matpart_cg(gf_matpart A, const double* rhs, double* xu, bool warm, double t
             k_cg_apply_dir<<<napply, PB, 0, s>>>(n, rp, cl, vl, W.cz, pb[(it - 1) & 1], pb[it & 1], W.cq, partial, scal,
                                                  pq_part, state, it);
         check_launch();
-        GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        GF_HIP(hipStreamSynchronize(s));
+        // the host looks at the previous chunk's flag while this chunk is queued
+        pipe.post(state, chunk, s);
+        if (chunk > 0 && pipe.wait(chunk - 1)[0]) st[0] = 1;
+        ++chunk;
     }
     // the true residual ||rhs - A_uu x_u|| / ||rhs||
     k_cg_residual<<<NPB, PB, 0, s>>>(n, rp, cl, vl, rhs, W.cx, partial);
@@ -1378,6 +1382,8 @@ int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, i
                 k_ty_expand_dir<<<gexp, PB, 0, s>>>(nu, A->nnp, A->nnd, cz, pb[(it + 1) & 1], pb[it & 1], A->du_rp.p,
                                                      A->du_cl.p, A->du_v.p, xfull, partial, scal, state, it);
             };
+            StatePipe pipe;
+            int chunk = 0;
             for (int64_t it = 0; !st[0] && it < maxiter;) {
                 for (int64_t k = 0; k < check_every && it < maxiter; ++k, ++it) {
                     if (it == 0)
@@ -1393,8 +1399,10 @@ int gf_mattyings_solve(gf_mattyings A, const double* b, double* x, double tol, i
                 }
                 if (it == maxiter) expand_dir(it); // the convergence test and count of the last update
                 check_launch();
-                GF_HIP(hipMemcpyAsync(st, state, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-                GF_HIP(hipStreamSynchronize(s));
+                // the host looks at the previous chunk's flag while this chunk is queued
+                pipe.post(state, chunk, s);
+                if (chunk > 0 && pipe.wait(chunk - 1)[0]) st[0] = 1;
+                ++chunk;
             }
             // true residual of the condensed system
             ty_product(A, cx, nullptr, cq, nullptr, nullptr, s);

@@ -1332,6 +1332,8 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
             GF_HIP(hipMemcpyAsync(P->sr[0].p, P->r.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
             apply(P, nullptr, nullptr, P->sw[0].p, true, s, P->xfull.p, P->spart.p + P->ntiles, nullptr);
         }
+        StatePipe pipe;
+        int chunk = 0;
         for (int64_t it = 0; it < maxiter && !done0;) {
             for (int64_t k = 0; single && k < check_every && it < maxiter; ++k, ++it) {
                 const int i0 = static_cast<int>(it & 1), i1 = 1 - i0;
@@ -1361,9 +1363,10 @@ int gf_pcg_solve(gf_pcg P, const double* b_u, double* x_u, double tol, int64_t m
                                                 P->invdiag.p, P->x.p, P->r.p, P->z.p, P->partial.p, P->state.p, it);
             }
             check_launch();
-            GF_HIP(hipMemcpyAsync(st, P->state.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            GF_HIP(hipStreamSynchronize(s));
-            if (st[0]) break;
+            // the host looks at the previous chunk's flag while this chunk is queued
+            pipe.post(P->state.p, chunk, s);
+            if (chunk > 0 && pipe.wait(chunk - 1)[0]) break;
+            ++chunk;
         }
         // true residual ||b - K' x|| / ||b||
         GF_HIP(hipMemcpyAsync(x_u, P->x.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
