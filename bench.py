@@ -333,7 +333,8 @@ def pcg_cpu_baseline(S, b, target_s=10.0):
     ndof = ty.nni + ty.nnd
     A = sp.csr_matrix((vl, cl, rp), shape=(ndof, ndof))
     u, d = slice(0, ty.nnu), slice(ty.nni, ndof)
-    Cdu = sp.csr_matrix(ty.Cdu)
+    crp, ccl, cvl, ncu = ty.Cdu_csr  # sparse: the dense form is nnd x nnu
+    Cdu = sp.csr_matrix((cvl, ccl, crp), shape=(crp.size - 1, ncu))
     Kp = (A[u, u] + A[u, d] @ Cdu + Cdu.T @ A[d, u] + Cdu.T @ A[d, d] @ Cdu).tocsr()
     Kp.sort_indices()
     bh = b.cpu().numpy()

@@ -436,11 +436,18 @@ __global__ void __launch_bounds__(TX * TY * TZ) k_apply_periodic_h8(int n, int n
 {
     if (state && state[0]) return;
     constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2, NH = HX * HY * HZ, NT = TX * TY * TZ;
+    constexpr int NJ = (NH + NT - 1) / NT; // halo nodes per thread
     __shared__ double s_x[3][NH];
     const int tile = blockIdx.x;
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-    for (int t = threadIdx.x; t < NH; t += NT) {
+    // the halo tile: every DOF index of the thread's NJ nodes is loaded first (node indices clamped,
+    // so all addresses are in bounds), then every value (prescribed DOFs read as 0), so the two
+    // dependent loads wait once per thread rather than once per node
+    int32_t dd[NJ][3];
+#pragma unroll
+    for (int q = 0; q < NJ; ++q) {
+        const int t = min(static_cast<int>(threadIdx.x) + q * NT, NH - 1);
         const int lz = t / (HX * HY), r = t - lz * HX * HY, ly = r / HX, lx = r - (r / HX) * HX;
         int gi = x0 - 1 + lx, gj = y0 - 1 + ly, gk = z0 - 1 + lz;
         gi += gi < 0 ? n : 0;
@@ -451,10 +458,19 @@ __global__ void __launch_bounds__(TX * TY * TZ) k_apply_periodic_h8(int n, int n
         gk -= gk >= n ? n : 0;
         const int64_t k = ((static_cast<int64_t>(gk) * n + gj) * n + gi) * 3;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int32_t d = sidx[k + c];
-            s_x[c][t] = d >= 0 ? x[d] : 0.0;
-        }
+        for (int c = 0; c < 3; ++c) dd[q][c] = sidx[k + c];
+    }
+    double xv[NJ][3];
+#pragma unroll
+    for (int q = 0; q < NJ; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xv[q][c] = x[dd[q][c] >= 0 ? dd[q][c] : 0];
+#pragma unroll
+    for (int q = 0; q < NJ; ++q) {
+        const int t = static_cast<int>(threadIdx.x) + q * NT;
+        if (t < NH)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s_x[c][t] = dd[q][c] >= 0 ? xv[q][c] : 0.0;
     }
     __syncthreads();
     const int olz = threadIdx.x / (TX * TY), orr = threadIdx.x - olz * TX * TY, oly = orr / TX,
