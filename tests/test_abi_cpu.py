@@ -147,6 +147,23 @@ def test_tyings_vs_oracle_hex8(gf, nx, ny, nz):
             assert np.array_equal(x, y), name
 
 
+@pytest.mark.parametrize("dims", [(5, 4), (3, 4, 3)])
+def test_tyings_csr_matches_dense(gf, dims):
+    """The (rowptr, col, val, ncols) forms that sparse callers use expand to the dense Cdi / Cdu /
+    Cdp properties (TyingsPeriodic.h:361-392), shapes included."""
+    import scipy.sparse as sp
+
+    m = gf.Mesh.Quad4.Regular(*dims) if len(dims) == 2 else gf.Mesh.Hex8.Regular(*dims)
+    ctrl = gf.Tyings.Control(m.coor, m.dofs)
+    iip = np.concatenate((ctrl.controlDofs.ravel(), ctrl.dofs[m.nodesOrigin]))
+    ty = gf.Tyings.Periodic(ctrl.coor, ctrl.dofs, ctrl.controlDofs, m.nodesPeriodic, iip)
+    for name, ncols in (("Cdi", ty.nni), ("Cdu", ty.nnu), ("Cdp", ty.nnp)):
+        rp, cl, vl, nc = getattr(ty, name + "_csr")
+        assert nc == ncols and rp.size - 1 == ty.nnd, name
+        dense = sp.csr_matrix((vl, cl, rp), shape=(rp.size - 1, nc)).toarray()
+        assert np.array_equal(dense, getattr(ty, name)), name
+
+
 def test_tyings_hex8_affine_closed_form(gf):
     """u_d = u_i + (x_d - x_i) . ctrl (TyingsPeriodic.h:244-256) in 3-D, on the product's own Cdi:
     an affine field u = F x with the control node j holding row j of F is reproduced exactly on
