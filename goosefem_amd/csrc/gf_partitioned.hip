@@ -332,12 +332,33 @@ __global__ void __launch_bounds__(PB)
     const int64_t g = blockIdx.x * static_cast<int64_t>(PB) + threadIdx.x;
     const int64_t r = g / LPR;
     const int j = static_cast<int>(g % LPR);
+    // the lane's entries k = rp[r] + j, + LPR, ... in batches of NB: NB column indices and values
+    // (positions clamped to the row, so every address is in bounds), then the NB gathered operands,
+    // then the products added in entry order; a loop that loads one entry per pass waits twice per
+    // entry.  Same bits as that loop.
+    constexpr int NB = 5;
     double sum = 0.0;
-    if (r < n)
-        for (int32_t k = rp[r] + j; k < rp[r + 1]; k += LPR) {
-            const int32_t c = cl[k];
-            sum += vl[k] * (z[c] + beta * pprev[c]);
+    if (r < n) {
+        const int32_t e = rp[r + 1];
+        for (int32_t k = rp[r] + j; k < e; k += NB * LPR) {
+            int32_t c[NB];
+            double v[NB], zz[NB], pp[NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int32_t kk = min(k + q * LPR, e - 1);
+                c[q] = cl[kk];
+                v[q] = vl[kk];
+            }
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                zz[q] = z[c[q]];
+                pp[q] = pprev[c[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+                if (k + q * LPR < e) sum += v[q] * (zz[q] + beta * pp[q]);
         }
+    }
     sum += __shfl_xor(sum, 1);
     sum += __shfl_xor(sum, 2);
     double contrib = 0.0;
